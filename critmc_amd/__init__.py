@@ -1,0 +1,21 @@
+"""critmc_amd -- MI355X-native lattice-update engine with critmc's API.
+
+Hand-written HIP kernels (gfx950) advance Ising, Potts, Edwards-Anderson and
+O(n) chains with checkerboard sweeps; the Python layer mirrors critmc's
+``Simulation`` / ``ParallelTempering`` / runner interfaces and reaches the
+kernels through the C-ABI library ``libcritmc_hip.so`` via ctypes.
+"""
+
+from .lattice import Lattice, chain, cubic, hypercubic, make_lattice, square, triangular
+from .models import DeviceState, Ising, Model, build_model, derive
+from .rng import chain_key, make_rng, seed_sequence, spawn_rngs
+from .sim import Simulation, TimeSeries, parse_updates
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "Lattice", "chain", "cubic", "hypercubic", "make_lattice", "square", "triangular",
+    "DeviceState", "Ising", "Model", "build_model", "derive",
+    "chain_key", "make_rng", "seed_sequence", "spawn_rngs",
+    "Simulation", "TimeSeries", "parse_updates",
+]

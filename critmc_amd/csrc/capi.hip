@@ -1,0 +1,80 @@
+// Library-wide C entry points: version, error text, raw Philox draws.
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+#include "philox.hpp"
+
+namespace cm {
+
+static thread_local char g_error[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+}
+
+const char* last_error() { return g_error; }
+
+__global__ void philox_fill_kernel(uint32_t* __restrict__ out, int64_t n_blocks, uint32_t k0,
+                                   uint32_t k1, uint32_t x0, uint32_t y, uint32_t z, uint32_t w)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_blocks;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        const u32x4 r = philox4x32_10(u32x4{x0 + (uint32_t)b, y, z, w}, k0, k1);
+        out[4 * b + 0] = r.x;
+        out[4 * b + 1] = r.y;
+        out[4 * b + 2] = r.z;
+        out[4 * b + 3] = r.w;
+    }
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int32_t cm_version(void) { return 1; }
+
+const char* cm_last_error(void) { return last_error(); }
+
+int cm_philox_fill(uint32_t* out, int64_t n_blocks, uint32_t k0, uint32_t k1, uint32_t x0,
+                   uint32_t y, uint32_t z, uint32_t w, void* stream)
+{
+    CM_REQUIRE(n_blocks >= 0, "n_blocks must not be negative");
+    if (n_blocks == 0) return CM_OK;
+    const int blocks = (int)std::min<int64_t>((n_blocks + 255) / 256, 4096);
+    hipLaunchKernelGGL(philox_fill_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out,
+                       n_blocks, k0, k1, x0, y, z, w);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+void cm_philox_host(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+    const u32x4 r = philox4x32_10(u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
+int cm_device_info(int32_t* cu_count, int64_t* lds_per_block, int32_t* arch_major,
+                   int32_t* arch_minor)
+{
+    int dev = 0;
+    CM_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    CM_HIP_CHECK(hipGetDeviceProperties(&p, dev));
+    *cu_count = p.multiProcessorCount;
+    *lds_per_block = (int64_t)p.sharedMemPerBlock;
+    *arch_major = p.major;
+    *arch_minor = p.minor;
+    return CM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// Shared definitions of the critmc_amd HIP library (gfx950).
+//
+// Random stream contract (every kernel follows it; the CPU replay restates it):
+//   key = (k0, k1) per chain
+//   ctr = (block, sweep_lo, sweep_hi16 | colour << 16 | tag << 24, stream_id)
+// `sweep` is the chain's own count of local sweeps, so a run split into any
+// number of calls draws the same numbers (critmc tests/test_sim_chunks.py).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+namespace cm {
+
+enum Tag : uint32_t {
+    TAG_INIT = 1,
+    TAG_ISING = 2,
+    TAG_POTTS = 3,
+    TAG_EA = 4,
+    TAG_PT = 5,
+    TAG_ON = 6,
+};
+
+// Update codes of critmc/kernels/common.py:16-21.
+enum Code : int64_t {
+    CODE_METROPOLIS = 0,
+    CODE_METROPOLIS_RANDOM = 1,
+    CODE_HEATBATH = 2,
+    CODE_WOLFF = 3,
+    CODE_SWENDSEN_WANG = 4,
+    CODE_OVERRELAX = 5,
+};
+
+// Return codes of every C entry point.
+enum Status : int {
+    CM_OK = 0,
+    CM_EINVAL = -1,     // bad argument (shape, size, code)
+    CM_EUNSUPPORTED = -2,
+    CM_EHIP = -3,       // a HIP runtime call failed
+};
+
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define CM_HIP_CHECK(expr)                                                              \
+    do {                                                                                \
+        hipError_t err__ = (expr);                                                      \
+        if (err__ != hipSuccess) {                                                      \
+            ::cm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(err__),     \
+                            __FILE__, __LINE__);                                        \
+            return ::cm::CM_EHIP;                                                       \
+        }                                                                               \
+    } while (0)
+
+#define CM_REQUIRE(cond, ...)                                                           \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::cm::set_error(__VA_ARGS__);                                               \
+            return ::cm::CM_EINVAL;                                                     \
+        }                                                                               \
+    } while (0)
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace cm

@@ -1,0 +1,751 @@
+// Checkerboard Ising sweeps for gfx950: Metropolis and heat bath, int8 or
+// 32-spin bit-packed state.
+//
+// Reference semantics (critmc/kernels/ising.py):
+//   Metropolis  :47-56 table, :68-87 sweep   a = s_i sum_j s_j; accept iff p >= 1 or u < p
+//   heat bath   :59-65 table, :90-107 sweep  new = +1 iff u < 1/(1+exp(-2 beta (f+h)))
+//   run         :173-237 per measurement E_b, M, accepted
+// Visiting order is colour 0 then colour 1 (colour = sum of coordinates mod 2).
+//
+// Layout.  The colour-c sites of a lattice row r (x1.. fixed) are
+// x0 = 2m + ((x1 + x2 + c) & 1), m < M = L0/2, packed 32 per word:
+// group g = r * G + m / 32 (G = ceil(M/32)), bit m % 32, bit set = spin +1.
+// The packed HBM image is [replica][colour][row][G] u32, which is also the
+// LDS image of the batched kernel.  Horizontal neighbours of a word are the
+// other colour's word at the same position and the same word shifted by one
+// bit (carry from the adjacent word); vertical neighbours are the other
+// colour's words of the adjacent rows at the same position.
+//
+// Uniforms.  A group's 32 sites share 32 Philox words W_0..W_31 (8 blocks,
+// tag TAG_ISING); the site at bit b has x = sum_j bit_b(W_j) 2^(31-j), and
+// "u < p" is "x < T" with T = ceil(p 2^32) computed on the host in FP64.  The
+// comparison runs plane by plane (most significant first) on all 32 lanes of
+// a word at once and stops as soon as every lane is decided -- usually after
+// one or two Philox blocks instead of eight.  The early stop does not change
+// any decision, so the result equals the full 32-bit comparison.
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "philox.hpp"
+#include "ising_core.hpp"
+
+namespace cm {
+
+// ---------------------------------------------------------------------------
+// Plain form: one chain, one launch per half sweep, int8 spins in HBM.
+
+struct Geo {
+    int d, L0, L1, L2, M, G;
+    int64_t rows;
+};
+
+template <int Z, int NCMAX>
+__global__ __launch_bounds__(256) void ising_plain_half(int8_t* __restrict__ s, Geo g, int code,
+                                                        int colour, uint64_t sweep, uint32_t k0,
+                                                        uint32_t k1, uint32_t sid,
+                                                        const uint32_t* __restrict__ tabs,
+                                                        int32_t* __restrict__ de_out,
+                                                        int32_t* __restrict__ dm_out,
+                                                        int32_t* __restrict__ acc_out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Delta d{0, 0, 0};
+    if (t < g.rows * g.G) {
+        const int64_t r = t / g.G;
+        const int w = (int)(t - r * g.G);
+        int x1 = 0, x2 = 0;
+        if (Z >= 4) {
+            x1 = (int)(r % g.L1);
+            x2 = (int)(r / g.L1);
+        }
+        const int p = (x1 + x2 + colour) & 1;
+        const int64_t base = r * g.L0;
+        int64_t rows_nb[4] = {0, 0, 0, 0};
+        if (Z >= 4) {
+            const int64_t y0 = r - x1;
+            rows_nb[0] = (y0 + (x1 + 1 == g.L1 ? 0 : x1 + 1)) * g.L0;
+            rows_nb[1] = (y0 + (x1 == 0 ? g.L1 - 1 : x1 - 1)) * g.L0;
+        }
+        if (Z == 6) {
+            const int64_t plane = (int64_t)g.L1;
+            const int64_t z0 = r - (int64_t)x2 * plane;
+            rows_nb[2] = (z0 + (int64_t)(x2 + 1 == g.L2 ? 0 : x2 + 1) * plane) * g.L0;
+            rows_nb[3] = (z0 + (int64_t)(x2 == 0 ? g.L2 - 1 : x2 - 1) * plane) * g.L0;
+        }
+        const int nbits = min(32, g.M - 32 * w);
+        uint32_t own = 0;
+        uint32_t nb[Z];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) nb[k] = 0;
+        for (int b = 0; b < nbits; ++b) {
+            const int x0 = 2 * (32 * w + b) + p;
+            const int xp = x0 + 1 == g.L0 ? 0 : x0 + 1;
+            const int xm = x0 == 0 ? g.L0 - 1 : x0 - 1;
+            own |= (uint32_t)(s[base + x0] > 0) << b;
+            nb[0] |= (uint32_t)(s[base + xp] > 0) << b;
+            nb[1] |= (uint32_t)(s[base + xm] > 0) << b;
+#pragma unroll
+            for (int k = 2; k < Z; ++k) nb[k] |= (uint32_t)(s[rows_nb[k - 2] + x0] > 0) << b;
+        }
+        const uint32_t valid = nbits == 32 ? ~0u : ((1u << nbits) - 1u);
+        const uint32_t nw = update_word<Z, NCMAX>(code, own, nb, valid, tabs, k0, k1, sid, sweep,
+                                                  (uint32_t)colour, (uint32_t)t, d);
+        uint32_t changed = (nw ^ own) & valid;
+        while (changed) {
+            const int b = __ffs(changed) - 1;
+            changed &= changed - 1;
+            s[base + 2 * (32 * w + b) + p] = ((nw >> b) & 1u) ? (int8_t)1 : (int8_t)-1;
+        }
+    }
+    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+    if ((threadIdx.x & 63) == 0 && (de | dm | ac)) {
+        atomicAdd(de_out, de);
+        atomicAdd(dm_out, dm);
+        atomicAdd(acc_out, ac);
+    }
+}
+
+__global__ void fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Turns per-measurement deltas into the running E_b, M series (one thread per chain).
+__global__ void prefix_series(int32_t* __restrict__ e_out, int32_t* __restrict__ m_out,
+                              int64_t n_meas, int32_t* __restrict__ e_run,
+                              int32_t* __restrict__ m_run, int R)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= R) return;
+    int32_t e = e_run[rep], m = m_run[rep];
+    for (int64_t k = 0; k < n_meas; ++k) {
+        e += e_out[(int64_t)rep * n_meas + k];
+        m += m_out[(int64_t)rep * n_meas + k];
+        e_out[(int64_t)rep * n_meas + k] = e;
+        m_out[(int64_t)rep * n_meas + k] = m;
+    }
+    e_run[rep] = e;
+    m_run[rep] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: one workgroup per replica (grid-stride), the replica's
+// bit-packed state resident in LDS for the whole call, per-measurement
+// E_b / M / accepted reduced in the workgroup.  2D square lattices with
+// M <= 32 or M a multiple of 32 (L0 <= 64 or L0 a multiple of 64).
+
+__device__ __forceinline__ void unpack_pair_u32(uint32_t v, uint32_t& even, uint32_t& odd, int sh)
+{
+    // bytes b0..b3 of v are spins (+1 = 0x01, -1 = 0xFF); bit set = +1
+    const uint32_t up = ~v;
+    even |= (((up >> 7) & 1u) | ((up >> 22) & 2u)) << sh;
+    odd |= (((up >> 15) & 1u) | ((up >> 30) & 2u)) << sh;
+}
+
+__device__ __forceinline__ uint32_t pack_pair_u32(uint32_t even, uint32_t odd, int sh)
+{
+    const uint32_t e = even >> sh, o = odd >> sh;
+    const uint32_t b0 = (e & 1u) ? 0x01u : 0xFFu;
+    const uint32_t b1 = (o & 1u) ? 0x01u : 0xFFu;
+    const uint32_t b2 = (e & 2u) ? 0x01u : 0xFFu;
+    const uint32_t b3 = (o & 2u) ? 0x01u : 0xFFu;
+    return b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+}
+
+template <int LAYOUT>
+__device__ void load_replica(const void* __restrict__ spins, int rep, int L0, int L1, int G,
+                             uint32_t* __restrict__ lds)
+{
+    const int W = L1 * G;
+    if (LAYOUT == 1) {
+        const uint32_t* src = (const uint32_t*)spins + (int64_t)rep * 2 * W;
+        for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) lds[i] = src[i];
+        return;
+    }
+    const int8_t* src = (const int8_t*)spins + (int64_t)rep * L0 * L1;
+    for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+        const int y = idx / G, w = idx - (idx / G) * G;
+        const int8_t* row = src + (int64_t)y * L0 + 64 * w;
+        uint32_t even = 0, odd = 0;
+        if ((L0 & 63) == 0) {
+            const uint4* v = (const uint4*)row;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint4 q = v[k];
+                unpack_pair_u32(q.x, even, odd, 8 * k + 0);
+                unpack_pair_u32(q.y, even, odd, 8 * k + 2);
+                unpack_pair_u32(q.z, even, odd, 8 * k + 4);
+                unpack_pair_u32(q.w, even, odd, 8 * k + 6);
+            }
+        } else {
+            for (int x = 0; x < L0; ++x) {
+                const uint32_t bit = row[x] > 0 ? 1u : 0u;
+                if (x & 1) odd |= bit << (x >> 1);
+                else even |= bit << (x >> 1);
+            }
+        }
+        const int ce = y & 1;  // colour of even x in row y
+        lds[ce * W + idx] = even;
+        lds[(1 - ce) * W + idx] = odd;
+    }
+}
+
+template <int LAYOUT>
+__device__ void store_replica(void* __restrict__ spins, int rep, int L0, int L1, int G,
+                              const uint32_t* __restrict__ lds)
+{
+    const int W = L1 * G;
+    if (LAYOUT == 1) {
+        uint32_t* dst = (uint32_t*)spins + (int64_t)rep * 2 * W;
+        for (int i = threadIdx.x; i < 2 * W; i += blockDim.x) dst[i] = lds[i];
+        return;
+    }
+    int8_t* dst = (int8_t*)spins + (int64_t)rep * L0 * L1;
+    for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+        const int y = idx / G, w = idx - (idx / G) * G;
+        int8_t* row = dst + (int64_t)y * L0 + 64 * w;
+        const int ce = y & 1;
+        const uint32_t even = lds[ce * W + idx], odd = lds[(1 - ce) * W + idx];
+        if ((L0 & 63) == 0) {
+            uint4* v = (uint4*)row;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint4 q;
+                q.x = pack_pair_u32(even, odd, 8 * k + 0);
+                q.y = pack_pair_u32(even, odd, 8 * k + 2);
+                q.z = pack_pair_u32(even, odd, 8 * k + 4);
+                q.w = pack_pair_u32(even, odd, 8 * k + 6);
+                v[k] = q;
+            }
+        } else {
+            for (int x = 0; x < L0; ++x) {
+                const uint32_t bit = ((x & 1) ? (odd >> (x >> 1)) : (even >> (x >> 1))) & 1u;
+                row[x] = bit ? (int8_t)1 : (int8_t)-1;
+            }
+        }
+    }
+}
+
+template <int LAYOUT, int NCMAX>
+__global__ __launch_bounds__(256) void ising2d_batched(
+    void* __restrict__ spins, int R, int L0, int L1, const uint32_t* __restrict__ tabs,
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
+    uint64_t* __restrict__ sweeps, Sched sched, int64_t n_meas, int32_t* __restrict__ e_run,
+    int32_t* __restrict__ m_run, int32_t* __restrict__ e_out, int32_t* __restrict__ m_out,
+    int32_t* __restrict__ acc_out)
+{
+    extern __shared__ uint32_t lds[];
+    const int M = L0 >> 1;
+    const int G = (M + 31) >> 5;
+    const int W = L1 * G;
+    int* red = (int*)(lds + 2 * W);  // [waves][3]
+    const int nwaves = (blockDim.x + 63) >> 6;
+    const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
+
+    for (int rep = blockIdx.x; rep < R; rep += gridDim.x) {
+        load_replica<LAYOUT>(spins, rep, L0, L1, G, lds);
+        __syncthreads();
+        const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+        const uint32_t* tab = tabs + (int64_t)TAB_WORDS * rep;
+        uint64_t sweep = sweeps[rep];
+        int32_t e = e_run[rep], m = m_run[rep];
+
+        for (int64_t meas = 0; meas < n_meas; ++meas) {
+            Delta d{0, 0, 0};
+            for (int row = 0; row < sched.n; ++row) {
+                const int code = sched.code[row];
+                for (int rp = 0; rp < sched.count[row]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        uint32_t* Sc = lds + c * W;
+                        const uint32_t* So = lds + (1 - c) * W;
+                        for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+                            const int y = idx / G, w = idx - (idx / G) * G;
+                            const int p = (y + c) & 1;
+                            const int yp = y + 1 == L1 ? 0 : y + 1;
+                            const int ym = y == 0 ? L1 - 1 : y - 1;
+                            const uint32_t own = Sc[idx];
+                            const uint32_t same = So[idx];
+                            uint32_t sh;
+                            if (M >= 32) {
+                                if (p == 0) {
+                                    const uint32_t prev = So[y * G + (w == 0 ? G - 1 : w - 1)];
+                                    sh = (same << 1) | (prev >> 31);
+                                } else {
+                                    const uint32_t next = So[y * G + (w == G - 1 ? 0 : w + 1)];
+                                    sh = (same >> 1) | (next << 31);
+                                }
+                            } else if (p == 0) {
+                                sh = ((same << 1) | (same >> (M - 1))) & valid;
+                            } else {
+                                sh = ((same >> 1) | ((same & 1u) << (M - 1))) & valid;
+                            }
+                            const uint32_t nb[4] = {same, sh, So[yp * G + w], So[ym * G + w]};
+                            Sc[idx] = update_word<4, NCMAX>(code, own, nb, valid, tab, k0, k1,
+                                                            sid, sweep, (uint32_t)c,
+                                                            (uint32_t)idx, d);
+                        }
+                        __syncthreads();
+                    }
+                    ++sweep;
+                }
+            }
+            const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+            if ((threadIdx.x & 63) == 0) {
+                red[3 * (threadIdx.x >> 6) + 0] = de;
+                red[3 * (threadIdx.x >> 6) + 1] = dm;
+                red[3 * (threadIdx.x >> 6) + 2] = ac;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int sde = 0, sdm = 0, sac = 0;
+                for (int k = 0; k < nwaves; ++k) {
+                    sde += red[3 * k];
+                    sdm += red[3 * k + 1];
+                    sac += red[3 * k + 2];
+                }
+                e += sde;
+                m += sdm;
+                const int64_t o = (int64_t)rep * n_meas + meas;
+                e_out[o] = e;
+                m_out[o] = m;
+                acc_out[o] = sac;
+            }
+            __syncthreads();
+        }
+        store_replica<LAYOUT>(spins, rep, L0, L1, G, lds);
+        if (threadIdx.x == 0) {
+            e_run[rep] = e;
+            m_run[rep] = m;
+            sweeps[rep] = sweep;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial states, exact E_b / M, layout conversion.
+
+__global__ void ising_init_kernel(int8_t* __restrict__ s, int64_t N, int R,
+                                  const uint32_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ sids, int start)
+{
+    const int rep = blockIdx.y;
+    if (rep >= R) return;
+    const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+    for (int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; blk * 4 < N;
+         blk += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t w[4] = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+        if (start == 0) {
+            const u32x4 r = draw(k0, k1, sid, 0, 0, TAG_INIT, (uint32_t)blk);
+            w[0] = r.x; w[1] = r.y; w[2] = r.z; w[3] = r.w;
+        } else if (start == 2) {
+            w[0] = w[1] = w[2] = w[3] = 0;
+        }
+        for (int k = 0; k < 4 && blk * 4 + k < N; ++k)
+            s[(int64_t)rep * N + blk * 4 + k] = (w[k] >> 31) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
+__global__ __launch_bounds__(256) void ising_measure_kernel(const int8_t* __restrict__ s, Geo g,
+                                                            int64_t N, int32_t* __restrict__ e_run,
+                                                            int32_t* __restrict__ m_run)
+{
+    __shared__ int red[2 * 16];
+    const int rep = blockIdx.x;
+    const int8_t* sr = s + (int64_t)rep * N;
+    int e = 0, m = 0;
+    int64_t st1 = g.L0, st2 = (int64_t)g.L0 * g.L1;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const int si = sr[i];
+        m += si;
+        const int x0 = (int)(i % g.L0);
+        int nbsum = sr[i - x0 + (x0 + 1 == g.L0 ? 0 : x0 + 1)];
+        if (g.d >= 2) {
+            const int x1 = (int)((i / st1) % g.L1);
+            nbsum += sr[i + (x1 + 1 == g.L1 ? -(int64_t)(g.L1 - 1) * st1 : st1)];
+        }
+        if (g.d >= 3) {
+            const int x2 = (int)((i / st2) % g.L2);
+            nbsum += sr[i + (x2 + 1 == g.L2 ? -(int64_t)(g.L2 - 1) * st2 : st2)];
+        }
+        e -= si * nbsum;
+    }
+    e = wave_sum(e);
+    m = wave_sum(m);
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = e;
+        red[2 * (threadIdx.x >> 6) + 1] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int se = 0, sm = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            se += red[2 * k];
+            sm += red[2 * k + 1];
+        }
+        e_run[rep] = se;
+        m_run[rep] = sm;
+    }
+}
+
+// int8 [R][N] <-> packed [R][2][rows][G]; direction 0 packs, 1 unpacks.
+__global__ void ising_convert_kernel(int8_t* __restrict__ s8, uint32_t* __restrict__ sp, Geo g,
+                                     int R, int direction)
+{
+    const int64_t W = g.rows * g.G;
+    const int64_t total = (int64_t)R * 2 * W;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t rep = t / (2 * W);
+        const int64_t rem = t - rep * 2 * W;
+        const int c = (int)(rem / W);
+        const int64_t gidx = rem - (int64_t)c * W;
+        const int64_t r = gidx / g.G;
+        const int w = (int)(gidx - r * g.G);
+        int x1 = 0, x2 = 0;
+        if (g.d >= 2) {
+            x1 = (int)(r % g.L1);
+            x2 = (int)(r / g.L1);
+        }
+        const int p = (x1 + x2 + c) & 1;
+        const int nbits = min(32, g.M - 32 * w);
+        int8_t* row = s8 + rep * g.rows * g.L0 + r * g.L0;
+        if (direction == 0) {
+            uint32_t v = 0;
+            for (int b = 0; b < nbits; ++b) v |= (uint32_t)(row[2 * (32 * w + b) + p] > 0) << b;
+            sp[t] = v;
+        } else {
+            const uint32_t v = sp[t];
+            for (int b = 0; b < nbits; ++b)
+                row[2 * (32 * w + b) + p] = ((v >> b) & 1u) ? (int8_t)1 : (int8_t)-1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static uint64_t threshold(double p)
+{
+    if (!(p < 1.0)) return 1ull << 32;
+    if (!(p > 0.0)) return 0;
+    return (uint64_t)std::ceil(p * 4294967296.0);
+}
+
+// critmc/kernels/ising.py:47-65 turned into class tables (see bitserial_below).
+static void build_tables(double beta, double h, int z, uint32_t* tab, int* nc_met, int* nc_hb)
+{
+    std::fill(tab, tab + TAB_WORDS, 0u);
+    uint32_t* met = tab;
+    uint32_t* hb = tab + 32;
+    int n = 0;
+    for (int nal = 0; nal <= z; ++nal) {
+        const int a = 2 * nal - z;
+        uint64_t T[2];
+        for (int t = 0; t < 2; ++t) {
+            const int spin = 2 * t - 1;
+            const double d_e = 2.0 * a + 2.0 * h * spin;
+            const double p = d_e <= 0.0 ? 1.0 : std::exp(-beta * d_e);
+            T[t] = threshold(p);
+        }
+        if (T[0] == T[1]) {
+            if (T[0] < (1ull << 32)) {
+                met[1 + n] = (uint32_t)nal;
+                met[TAB_T + n] = (uint32_t)T[0];
+                ++n;
+            }
+        } else {
+            for (int t = 0; t < 2; ++t) {
+                if (T[t] < (1ull << 32)) {
+                    met[1 + n] = (uint32_t)nal | ((uint32_t)(t == 0 ? 1 : 2) << 8);
+                    met[TAB_T + n] = (uint32_t)T[t];
+                    ++n;
+                }
+            }
+        }
+    }
+    met[0] = (uint32_t)n;
+    *nc_met = n;
+    n = 0;
+    for (int nup = 0; nup <= z; ++nup) {
+        const int f = 2 * nup - z;
+        const uint64_t T = threshold(1.0 / (1.0 + std::exp(-2.0 * beta * (f + h))));
+        if (T < (1ull << 32)) {
+            hb[1 + n] = (uint32_t)nup;
+            hb[TAB_T + n] = (uint32_t)T;
+            ++n;
+        }
+    }
+    hb[0] = (uint32_t)n;
+    *nc_hb = n;
+}
+
+static int geo_from(int ndim, const int32_t* dims, Geo* g)
+{
+    CM_REQUIRE(ndim >= 1 && ndim <= 3, "ndim must be 1, 2 or 3 (got %d)", ndim);
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) {
+        CM_REQUIRE(dims[a] >= 2 && dims[a] % 2 == 0,
+                   "checkerboard sweeps need even extents >= 2 (axis %d has %d)", a, dims[a]);
+        N *= dims[a];
+    }
+    CM_REQUIRE(N <= (int64_t)1 << 30, "lattice too large (%lld sites)", (long long)N);
+    g->d = ndim;
+    g->L0 = dims[0];
+    g->L1 = ndim >= 2 ? dims[1] : 1;
+    g->L2 = ndim >= 3 ? dims[2] : 1;
+    g->M = g->L0 / 2;
+    g->G = (g->M + 31) / 32;
+    g->rows = N / g->L0;
+    return CM_OK;
+}
+
+static int pick_nc(int nc)
+{
+    if (nc <= 2) return 2;
+    if (nc <= 4) return 4;
+    if (nc <= 7) return 7;
+    return NC_LIMIT;
+}
+
+template <int Z, int NC>
+static void launch_plain(int8_t* s, const Geo& g, int code, int colour, uint64_t sweep,
+                         uint32_t k0, uint32_t k1, uint32_t sid, const uint32_t* tabs,
+                         int32_t* de, int32_t* dm, int32_t* ac, hipStream_t st)
+{
+    const int64_t nthr = g.rows * g.G;
+    const int blocks = (int)((nthr + 255) / 256);
+    hipLaunchKernelGGL((ising_plain_half<Z, NC>), dim3(blocks), dim3(256), 0, st, s, g, code,
+                       colour, sweep, k0, k1, sid, tabs, de, dm, ac);
+}
+
+template <int Z>
+static void launch_plain_nc(int nc, int8_t* s, const Geo& g, int code, int colour,
+                            uint64_t sweep, uint32_t k0, uint32_t k1, uint32_t sid,
+                            const uint32_t* tabs, int32_t* de, int32_t* dm, int32_t* ac,
+                            hipStream_t st)
+{
+    switch (nc) {
+        case 2: launch_plain<Z, 2>(s, g, code, colour, sweep, k0, k1, sid, tabs, de, dm, ac, st); break;
+        case 4: launch_plain<Z, 4>(s, g, code, colour, sweep, k0, k1, sid, tabs, de, dm, ac, st); break;
+        case 7: launch_plain<Z, 7>(s, g, code, colour, sweep, k0, k1, sid, tabs, de, dm, ac, st); break;
+        default: launch_plain<Z, NC_LIMIT>(s, g, code, colour, sweep, k0, k1, sid, tabs, de, dm, ac, st); break;
+    }
+}
+
+template <int LAYOUT, int NC>
+static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_t* tabs,
+                                 const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
+                                 const Sched& sc, int64_t n_meas, int32_t* e_run, int32_t* m_run,
+                                 int32_t* e_out, int32_t* m_out, int32_t* acc_out, int block,
+                                 int grid, hipStream_t st)
+{
+    const size_t lds = (size_t)2 * g.L1 * g.G * 4 + 3 * 16 * 4;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)ising2d_batched<LAYOUT, NC>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((ising2d_batched<LAYOUT, NC>), dim3(grid), dim3(block), lds, st, spins, R,
+                       g.L0, g.L1, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out,
+                       m_out, acc_out);
+    return hipGetLastError();
+}
+
+template <int LAYOUT>
+static hipError_t launch_batched_nc(int nc, void* spins, int R, const Geo& g,
+                                    const uint32_t* tabs, const uint32_t* keys,
+                                    const uint32_t* sids, uint64_t* sweeps, const Sched& sc,
+                                    int64_t n_meas, int32_t* e_run, int32_t* m_run,
+                                    int32_t* e_out, int32_t* m_out, int32_t* acc_out, int block,
+                                    int grid, hipStream_t st)
+{
+    switch (nc) {
+        case 2: return launch_batched<LAYOUT, 2>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+        case 4: return launch_batched<LAYOUT, 4>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+        case 7: return launch_batched<LAYOUT, 7>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+        default: return launch_batched<LAYOUT, NC_LIMIT>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+    }
+}
+
+static bool batched_ok(const Geo& g)
+{
+    return g.d == 2 && (g.M <= 32 || g.M % 32 == 0) && (size_t)2 * g.L1 * g.G * 4 <= 96 * 1024;
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int64_t cm_ising_workspace_bytes(int32_t n_rep) { return (int64_t)n_rep * TAB_WORDS * 4; }
+
+int cm_ising_tables(double beta, double h, int32_t z, uint32_t* tab_out)
+{
+    CM_REQUIRE(z == 2 || z == 4 || z == 6, "z must be 2, 4 or 6");
+    int a, b;
+    build_tables(beta, h, z, tab_out, &a, &b);
+    return CM_OK;
+}
+
+int cm_ising_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                  const uint32_t* keys, const uint32_t* stream_ids, int32_t start, void* stream)
+{
+    Geo g;
+    if (int rc = geo_from(ndim, dims, &g)) return rc;
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(start >= 0 && start <= 2, "start must be 0 (random), 1 (up) or 2 (down)");
+    const int64_t N = g.rows * g.L0;
+    const int64_t blocks_needed = (N / 4 + 255) / 256;
+    dim3 grid((unsigned)std::min<int64_t>(std::max<int64_t>(blocks_needed, 1), 1024), n_rep);
+    hipLaunchKernelGGL(ising_init_kernel, grid, dim3(256), 0, (hipStream_t)stream, (int8_t*)spins,
+                       N, n_rep, keys, stream_ids, start);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_ising_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                     int32_t* e_run, int32_t* m_run, void* stream)
+{
+    Geo g;
+    if (int rc = geo_from(ndim, dims, &g)) return rc;
+    CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
+    hipLaunchKernelGGL(ising_measure_kernel, dim3(n_rep), dim3(256), 0, (hipStream_t)stream,
+                       (const int8_t*)spins, g, g.rows * g.L0, e_run, m_run);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_ising_convert(void* spins_i8, void* spins_packed, int32_t ndim, const int32_t* dims,
+                     int32_t n_rep, int32_t direction, void* stream)
+{
+    Geo g;
+    if (int rc = geo_from(ndim, dims, &g)) return rc;
+    CM_REQUIRE(direction == 0 || direction == 1, "direction must be 0 (pack) or 1 (unpack)");
+    const int64_t total = (int64_t)n_rep * 2 * g.rows * g.G;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(ising_convert_kernel, dim3(std::max(blocks, 1)), dim3(256), 0,
+                       (hipStream_t)stream, (int8_t*)spins_i8, (uint32_t*)spins_packed, g, n_rep,
+                       direction);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                 const double* betas, double h, const uint32_t* keys, const uint32_t* stream_ids,
+                 uint64_t* sweeps, const uint64_t* sweeps_host, const int64_t* schedule,
+                 int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run, int32_t* e_out,
+                 int32_t* m_out, int32_t* acc_out, void* workspace, int64_t workspace_bytes,
+                 int32_t form, int32_t block, int32_t grid, void* stream)
+{
+    Geo g;
+    if (int rc = geo_from(ndim, dims, &g)) return rc;
+    CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
+    CM_REQUIRE(layout == 0 || layout == 1, "layout must be 0 (int8) or 1 (packed)");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    CM_REQUIRE(workspace_bytes >= cm_ising_workspace_bytes(n_rep), "workspace too small");
+    CM_REQUIRE(std::isfinite(h), "field h must be finite");
+    Sched sc{};
+    sc.n = n_rows;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
+                   "update code %lld has no checkerboard Ising kernel", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
+        sc.code[r] = (int32_t)code;
+        sc.count[r] = (int32_t)count;
+    }
+    if (n_meas == 0) return CM_OK;
+    const int z = 2 * g.d;
+    std::vector<uint32_t> tabs((size_t)n_rep * TAB_WORDS);
+    int nc_max = 1;
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        int a, b;
+        build_tables(betas[r], h, z, &tabs[(size_t)r * TAB_WORDS], &a, &b);
+        nc_max = std::max(nc_max, std::max(a, b));
+    }
+    const int nc = pick_nc(nc_max);
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t* dtabs = (uint32_t*)workspace;
+    CM_HIP_CHECK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice, st));
+
+    if (form == 0) form = batched_ok(g) ? 2 : 1;
+    if (form == 2) {
+        CM_REQUIRE(batched_ok(g),
+                   "batched form needs a 2D lattice with L0 <= 64 or L0 a multiple of 64");
+        if (block <= 0) block = 256;
+        CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
+        if (grid <= 0) grid = n_rep;
+        grid = std::min(grid, n_rep);
+        hipError_t err = layout == 0
+            ? launch_batched_nc<0>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
+                                   n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st)
+            : launch_batched_nc<1>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
+                                   n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+        CM_HIP_CHECK(err);
+        return CM_OK;
+    }
+    // plain form: one chain at a time, one launch per half sweep
+    CM_REQUIRE(form == 1, "form must be 0 (auto), 1 (plain) or 2 (batched)");
+    CM_REQUIRE(layout == 0, "the plain form works on int8 spins");
+    CM_REQUIRE(sweeps_host != nullptr, "the plain form needs the sweep counters on the host");
+    std::vector<uint32_t> hkeys((size_t)2 * n_rep), hsid(n_rep);
+    CM_HIP_CHECK(hipMemcpyAsync(hkeys.data(), keys, hkeys.size() * 4, hipMemcpyDeviceToHost, st));
+    CM_HIP_CHECK(hipMemcpyAsync(hsid.data(), stream_ids, hsid.size() * 4, hipMemcpyDeviceToHost, st));
+    CM_HIP_CHECK(hipStreamSynchronize(st));
+    const int64_t N = g.rows * g.L0;
+    const int64_t total = (int64_t)n_rep * n_meas;
+    const int fb = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, e_out, total, 0);
+    hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, m_out, total, 0);
+    hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, acc_out, total, 0);
+    for (int r = 0; r < n_rep; ++r) {
+        int8_t* s = (int8_t*)spins + (int64_t)r * N;
+        const uint32_t* t = dtabs + (int64_t)r * TAB_WORDS;
+        uint64_t sweep = sweeps_host[r];
+        for (int64_t meas = 0; meas < n_meas; ++meas) {
+            const int64_t o = (int64_t)r * n_meas + meas;
+            for (int row = 0; row < sc.n; ++row) {
+                for (int rp = 0; rp < sc.count[row]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        if (z == 2)
+                            launch_plain_nc<2>(nc, s, g, sc.code[row], c, sweep, hkeys[2 * r],
+                                               hkeys[2 * r + 1], hsid[r], t, e_out + o, m_out + o,
+                                               acc_out + o, st);
+                        else if (z == 4)
+                            launch_plain_nc<4>(nc, s, g, sc.code[row], c, sweep, hkeys[2 * r],
+                                               hkeys[2 * r + 1], hsid[r], t, e_out + o, m_out + o,
+                                               acc_out + o, st);
+                        else
+                            launch_plain_nc<6>(nc, s, g, sc.code[row], c, sweep, hkeys[2 * r],
+                                               hkeys[2 * r + 1], hsid[r], t, e_out + o, m_out + o,
+                                               acc_out + o, st);
+                    }
+                    ++sweep;
+                }
+            }
+        }
+    }
+    CM_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(prefix_series, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out, m_out,
+                       n_meas, e_run, m_run, n_rep);
+    // advance the device-side sweep counters like the batched form does
+    std::vector<uint64_t> nsw(n_rep);
+    int64_t per_meas = 0;
+    for (int row = 0; row < sc.n; ++row) per_meas += sc.count[row];
+    for (int r = 0; r < n_rep; ++r) nsw[r] = sweeps_host[r] + (uint64_t)(per_meas * n_meas);
+    CM_HIP_CHECK(hipMemcpyAsync(sweeps, nsw.data(), nsw.size() * 8, hipMemcpyHostToDevice, st));
+    CM_HIP_CHECK(hipStreamSynchronize(st));
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+}  // extern "C"

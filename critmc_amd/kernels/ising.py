@@ -1,0 +1,125 @@
+"""Checkerboard Ising kernels: host side of ``cm_ising_*``.
+
+Replaces critmc/kernels/ising.py's ``run`` (:173-237) for the Metropolis
+(:68-87) and heat-bath (:90-107) sweeps.  Spins stay on the device as int8
+``[R, N]`` (x fastest) or as the packed ``[R, 2, rows, G]`` u32 image.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+
+import numpy as np
+import torch
+
+from .._lib import check, lib
+from ..device import device, host_ptr, ptr, stream
+from .chains import Chains
+from .common import HEATBATH, METROPOLIS
+
+LAYOUT_INT8 = 0
+LAYOUT_PACKED = 1
+FORM_AUTO, FORM_PLAIN, FORM_BATCHED = 0, 1, 2
+START_CODES = {"random": 0, "ordered": 1, "up": 1, "down": 2}
+
+
+def _dims(shape: Sequence[int]) -> np.ndarray:
+    return np.ascontiguousarray(shape, dtype=np.int32)
+
+
+def packed_words(shape: Sequence[int]) -> int:
+    """u32 words per replica in the packed layout."""
+    L0 = int(shape[0])
+    rows = int(np.prod(shape)) // L0
+    return 2 * rows * ((L0 // 2 + 31) // 32)
+
+
+def thresholds_table(beta: float, h: float, z: int) -> np.ndarray:
+    """The library's class table for one chain (64 u32), for inspection and tests."""
+    out = np.zeros(64, dtype=np.uint32)
+    check(lib().cm_ising_tables(float(beta), float(h), int(z), host_ptr(out, ctypes.c_uint32)),
+          "cm_ising_tables")
+    return out
+
+
+def init_spins(spins: torch.Tensor, shape, chains: Chains, start: str = "random") -> None:
+    """Fill int8 spins ``[R, N]`` on the device (random draws use the chain's Philox key)."""
+    if start not in START_CODES:
+        raise ValueError(f"unknown start {start!r}")
+    d = _dims(shape)
+    rc = lib().cm_ising_init(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), chains.R,
+                             ptr(chains.keys_d), ptr(chains.sids_d), START_CODES[start], stream())
+    check(rc, "cm_ising_init")
+
+
+def measure(spins: torch.Tensor, shape, R: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """Exact E_b and M per replica (int32 device tensors), critmc/kernels/ising.py:31-44."""
+    d = _dims(shape)
+    e = torch.empty(R, dtype=torch.int32, device=spins.device)
+    m = torch.empty(R, dtype=torch.int32, device=spins.device)
+    rc = lib().cm_ising_measure(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, ptr(e),
+                                ptr(m), stream())
+    check(rc, "cm_ising_measure")
+    return e, m
+
+
+def pack(spins: torch.Tensor, shape, R: int) -> torch.Tensor:
+    d = _dims(shape)
+    out = torch.empty(R * packed_words(shape), dtype=torch.int32, device=spins.device)
+    rc = lib().cm_ising_convert(ptr(spins), ptr(out), len(d), host_ptr(d, ctypes.c_int32), R, 0,
+                                stream())
+    check(rc, "cm_ising_convert")
+    return out
+
+
+def unpack(packed: torch.Tensor, shape, R: int) -> torch.Tensor:
+    d = _dims(shape)
+    out = torch.empty((R, int(np.prod(shape))), dtype=torch.int8, device=packed.device)
+    rc = lib().cm_ising_convert(ptr(out), ptr(packed), len(d), host_ptr(d, ctypes.c_int32), R, 1,
+                                stream())
+    check(rc, "cm_ising_convert")
+    return out
+
+
+class Workspace:
+    """Device scratch owned by the caller (class tables, one 256-byte row per chain)."""
+
+    def __init__(self, R: int, dev: torch.device | None = None):
+        nbytes = int(lib().cm_ising_workspace_bytes(R))
+        self.nbytes = nbytes
+        self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev or device())
+
+
+def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: np.ndarray,
+        n_meas: int, e_run: torch.Tensor, m_run: torch.Tensor, *, layout: int = LAYOUT_INT8,
+        form: int = FORM_AUTO, block: int = 0, grid: int = 0, ws: Workspace | None = None,
+        out: tuple[torch.Tensor, torch.Tensor, torch.Tensor] | None = None):
+    """Apply ``schedule`` ``n_meas`` times to every replica.
+
+    Returns device int32 ``(E_b, M, accepted)`` series of shape ``[R, n_meas]``;
+    ``e_run``/``m_run`` and the chains' sweep counters are advanced in place.
+    """
+    R = chains.R
+    betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, dtype=np.float64), (R,)))
+    sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+    for code in sched[:, 0]:
+        if code not in (METROPOLIS, HEATBATH):
+            raise ValueError(f"update code {int(code)} has no checkerboard Ising kernel")
+    d = _dims(shape)
+    dev = spins.device
+    if out is None:
+        out = tuple(torch.empty((R, n_meas), dtype=torch.int32, device=dev) for _ in range(3))
+    e_out, m_out, acc_out = out
+    ws = ws or Workspace(R, dev)
+    rc = lib().cm_ising_run(
+        ptr(spins), layout, len(d), host_ptr(d, ctypes.c_int32), R,
+        host_ptr(betas, ctypes.c_double), float(h), ptr(chains.keys_d), ptr(chains.sids_d),
+        ptr(chains.sweeps_d), host_ptr(chains.sweeps_h, ctypes.c_uint64),
+        host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(e_run), ptr(m_run),
+        ptr(e_out), ptr(m_out), ptr(acc_out), ptr(ws.buf), ws.nbytes, form, block, grid,
+        stream(),
+    )
+    check(rc, "cm_ising_run")
+    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    return e_out, m_out, acc_out

@@ -1,0 +1,218 @@
+"""Spin models bound to the HIP kernels.
+
+Mirrors critmc/models.py:32-351: a model creates states, validates update
+schedules, advances a chain through its kernel (``advance``, the plug-in seam
+of models.py:58-68) and turns raw measurement streams into per-site
+observables (``derive``).  States are created on the host exactly as the
+reference does (same generator calls, so the same seed gives the same start)
+and then live on the GPU; ``advance`` takes the device state and the chain's
+Philox record where the reference takes the numpy array and ``Generator``.
+
+Conventions (J = k_B = 1): Ising ``H = -sum s_i s_j - h sum s_i``; Potts
+``H = -sum delta(s_i, s_j)``; O(n) ``H = -sum S_i . S_j``.
+"""
+
+from __future__ import annotations
+
+import math
+from abc import ABC, abstractmethod
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+import torch
+
+from .device import device
+from .kernels import ising as k_ising
+from .kernels.chains import Chains
+from .kernels.common import GPU_UPDATES, N_AUX, UPDATE_CODES
+from .lattice import Lattice, make_lattice, require_checkerboard
+
+__all__ = ["Model", "Ising", "build_model", "derive", "DeviceState"]
+
+
+@dataclass
+class DeviceState:
+    """A chain's device-resident state: spins plus running integer bookkeeping."""
+
+    spins: torch.Tensor
+    extra: dict[str, torch.Tensor] = field(default_factory=dict)
+
+
+class Model(ABC):
+    kind: str = ""
+    updates: tuple[str, ...] = ()
+
+    def __init__(self, lattice: Lattice):
+        self.lattice = lattice
+
+    @property
+    def n_sites(self) -> int:
+        return self.lattice.n_sites
+
+    @abstractmethod
+    def initial_state(self, start: str, rng: np.random.Generator) -> np.ndarray: ...
+
+    @abstractmethod
+    def energy(self, state: np.ndarray) -> float: ...
+
+    @abstractmethod
+    def begin(self, state: np.ndarray) -> dict[str, Any]: ...
+
+    @abstractmethod
+    def to_device(self, state: np.ndarray, running: dict[str, Any]) -> DeviceState: ...
+
+    @abstractmethod
+    def from_device(self, ds: DeviceState) -> np.ndarray: ...
+
+    @abstractmethod
+    def advance(self, ds: DeviceState, running: dict[str, Any], beta: float, chains: Chains,
+                schedule: np.ndarray, n_meas: int, ws: Any) -> dict[str, np.ndarray]:
+        """Run ``n_meas`` schedule applications on the GPU and return the raw streams."""
+
+    def workspace(self) -> Any:
+        return None
+
+    def check_updates(self, names: list[str]) -> None:
+        """Same errors as critmc/models.py:70-77, plus the GPU path's own limits."""
+        bad = [n for n in names if n not in self.updates]
+        if bad:
+            gpu = GPU_UPDATES.get(self.kind, ())
+            unsupported = [n for n in bad if n in UPDATE_CODES and n not in gpu]
+            if unsupported and len(unsupported) == len(bad):
+                raise ValueError(
+                    f"{type(self).__name__} updates {unsupported} have no checkerboard GPU kernel; "
+                    f"available: {list(self.updates)}"
+                )
+            raise ValueError(
+                f"{type(self).__name__} does not support updates {bad}; "
+                f"available: {list(self.updates)}"
+            )
+        require_checkerboard(self.lattice)
+
+    def spec(self) -> dict[str, Any]:
+        return {"model": self.kind, "lattice": self.lattice.name, "L": self.lattice.L}
+
+    def meta(self) -> dict[str, Any]:
+        return {**self.spec(), "N": self.n_sites, "dim": self.lattice.dim,
+                "n_bonds": self.lattice.n_bonds}
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}({self.lattice!r})"
+
+
+def _ising_bond_energy(s: np.ndarray, bonds: np.ndarray) -> int:
+    return -int(np.sum(s[bonds[:, 0]].astype(np.int64) * s[bonds[:, 1]]))
+
+
+class Ising(Model):
+    """Ising model with optional uniform field ``h`` (critmc/models.py:95-162)."""
+
+    kind = "ising"
+    updates = GPU_UPDATES["ising"]
+
+    def __init__(self, lattice: Lattice, h: float = 0.0):
+        super().__init__(lattice)
+        self.h = float(h)
+        if not math.isfinite(self.h):
+            raise ValueError(f"field h must be finite, got {h}")
+
+    def check_updates(self, names):
+        cluster = {"wolff", "sw", "swendsen-wang"}
+        if self.h != 0.0 and cluster.intersection(names):
+            raise ValueError("cluster updates require zero field h")
+        super().check_updates(names)
+
+    def initial_state(self, start, rng):
+        n = self.n_sites
+        if start == "random":
+            return (2 * rng.integers(0, 2, size=n) - 1).astype(np.int8)
+        if start in ("ordered", "up"):
+            return np.ones(n, dtype=np.int8)
+        if start == "down":
+            return -np.ones(n, dtype=np.int8)
+        raise ValueError(f"unknown start {start!r}")
+
+    def energy(self, state):
+        s = np.asarray(state)
+        return float(_ising_bond_energy(s, self.lattice.bonds)) - self.h * float(s.sum())
+
+    def begin(self, state):
+        s = np.asarray(state)
+        return {"e_b": _ising_bond_energy(s, self.lattice.bonds), "m": int(s.astype(np.int64).sum())}
+
+    def to_device(self, state, running):
+        dev = device()
+        spins = torch.from_numpy(np.ascontiguousarray(state, dtype=np.int8)).to(dev).reshape(1, -1)
+        e = torch.tensor([running["e_b"]], dtype=torch.int32, device=dev)
+        m = torch.tensor([running["m"]], dtype=torch.int32, device=dev)
+        return DeviceState(spins, {"e": e, "m": m})
+
+    def from_device(self, ds):
+        return ds.spins.reshape(-1).cpu().numpy().copy()
+
+    def workspace(self):
+        return k_ising.Workspace(1)
+
+    def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
+        e_out, m_out, acc = k_ising.run(ds.spins, self.lattice.shape, [float(beta)], self.h,
+                                        chains, schedule, n_meas, ds.extra["e"], ds.extra["m"],
+                                        ws=ws, form=_FORM[0])
+        e_b = e_out[0].cpu().numpy().astype(np.int64)
+        m = m_out[0].cpu().numpy().astype(np.int64)
+        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
+        aux[:, 0] = acc[0].cpu().numpy()
+        running["e_b"] = int(e_b[-1])
+        running["m"] = int(m[-1])
+        energy = e_b - self.h * m if self.h != 0.0 else e_b
+        return {"E": energy, "Eb": e_b, "M": m, "aux": aux}
+
+    def spec(self):
+        return {**super().spec(), "h": self.h}
+
+
+# kernel form used by single-chain models: 0 auto, 1 plain (one launch per half sweep), 2 batched
+_FORM = [0]
+
+
+def set_kernel_form(form: int) -> None:
+    if form not in (0, 1, 2):
+        raise ValueError("form must be 0 (auto), 1 (plain) or 2 (batched)")
+    _FORM[0] = int(form)
+
+
+_MODEL_KINDS: dict[str, Any] = {"ising": Ising}
+
+
+def build_model(spec: dict[str, Any]) -> Model:
+    """Inverse of :meth:`Model.spec` (critmc/models.py:312-322)."""
+    lattice = make_lattice(spec["lattice"], int(spec["L"]))
+    kind = spec["model"]
+    if kind == "ising":
+        return Ising(lattice, h=float(spec.get("h", 0.0)))
+    raise ValueError(f"unknown model kind {kind!r}")
+
+
+def derive(raw: dict[str, np.ndarray], meta: dict[str, Any]) -> dict[str, np.ndarray]:
+    """Per-site observables (critmc/models.py:325-351)."""
+    n = float(meta["N"])
+    kind = meta["model"]
+    out: dict[str, np.ndarray] = {"e": np.asarray(raw["E"], dtype=np.float64) / n}
+    if kind == "ising":
+        m = raw["M"] / n
+        out["m"] = m
+        out["m2"] = m * m
+    elif kind == "potts":
+        q = int(meta["q"])
+        rho = raw["counts"] / n
+        out["m2"] = np.clip((q * np.sum(rho * rho, axis=1) - 1.0) / (q - 1.0), 0.0, None)
+        out["mmax"] = (q * rho.max(axis=1) - 1.0) / (q - 1.0)
+    elif kind == "on":
+        mv = raw["M"] / n
+        out["m2"] = np.sum(mv * mv, axis=1)
+    else:
+        raise ValueError(f"unknown model kind {kind!r}")
+    out["absm"] = np.sqrt(out["m2"])
+    out["m4"] = out["m2"] ** 2
+    return out
+

@@ -1,0 +1,108 @@
+/*
+ * critmc_hip.h -- C-ABI of libcritmc_hip.so, the MI355X (gfx950) lattice-update
+ * engine behind critmc's kernel seam.
+ *
+ * The reference has no FFI: its seam is a Python call into numba,
+ * Model.advance -> kernels/<model>.run (critmc/models.py:58-68).  Each entry
+ * point below replaces one of those compiled calls and keeps its conventions:
+ * the caller owns every buffer (state, outputs, scratch), state and running
+ * totals are updated in place, and argument checks report errors instead of
+ * raising (return 0 = ok, -1 = invalid argument, -2 = unsupported, -3 = HIP
+ * error; cm_last_error() gives the text).
+ *
+ * Pointers named *_dev are device memory; everything else is host memory.
+ * `stream` is a hipStream_t (NULL = default stream); kernels are launched on
+ * it asynchronously unless stated otherwise.
+ *
+ * Random numbers: Philox4x32-10, key (k0, k1) per chain, counter
+ * (block, sweep_lo, sweep_hi16 | colour << 16 | tag << 24, stream_id), where
+ * `sweep` is the chain's own count of local sweeps -- the replacement of the
+ * numpy Generator argument (critmc/rng.py:49-66), reproducible under any
+ * chunking (critmc tests/test_sim_chunks.py).
+ */
+#ifndef CRITMC_HIP_H
+#define CRITMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------ */
+
+/* ABI version of this header. */
+int32_t cm_version(void);
+
+/* Text of the last error raised on the calling thread. */
+const char *cm_last_error(void);
+
+/* Device properties used by the host-side tuner. */
+int cm_device_info(int32_t *cu_count, int64_t *lds_per_block, int32_t *arch_major,
+                   int32_t *arch_minor);
+
+/* Raw Philox4x32-10 blocks with counters (x0 + b, y, z, w), b < n_blocks,
+ * written as 4 u32 per block; replaces the numpy bit generator draws
+ * (critmc/rng.py:66) for testing the stream contract. */
+int cm_philox_fill(uint32_t *out_dev, int64_t n_blocks, uint32_t k0, uint32_t k1, uint32_t x0,
+                   uint32_t y, uint32_t z, uint32_t w, void *stream);
+
+/* The same function on the host (ctr[4], key[2] -> out[4]). */
+void cm_philox_host(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+/* ---- Ising (critmc/kernels/ising.py) ------------------------------------- */
+
+/* Bytes of caller-owned device scratch cm_ising_run needs for n_rep chains. */
+int64_t cm_ising_workspace_bytes(int32_t n_rep);
+
+/* The acceptance-class table cm_ising_run builds for one chain (64 u32):
+ * critmc/kernels/ising.py:47-65 (metropolis_table, heatbath_table) as
+ * integer thresholds T = ceil(p 2^32).  For tests and inspection. */
+int cm_ising_tables(double beta, double h, int32_t z, uint32_t *tab_out);
+
+/* Initial int8 spins [n_rep][N]: start 0 = random (Philox tag INIT),
+ * 1 = all +1, 2 = all -1.  Replaces Ising.initial_state (critmc/models.py:113-121)
+ * for device-side batches. */
+int cm_ising_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
+                  const uint32_t *keys_dev, const uint32_t *stream_ids_dev, int32_t start,
+                  void *stream);
+
+/* Exact E_b = -sum s_i s_j and M = sum s_i per replica (int32):
+ * critmc/kernels/ising.py:31-44 (bond_energy, magnetization). */
+int cm_ising_measure(const void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
+                     int32_t *e_run_dev, int32_t *m_run_dev, void *stream);
+
+/* int8 [n_rep][N] <-> bit-packed [n_rep][2][rows][ceil(L0/64)] u32;
+ * direction 0 packs, 1 unpacks.  No reference counterpart (layout conversion). */
+int cm_ising_convert(void *spins_i8_dev, void *spins_packed_dev, int32_t ndim,
+                     const int32_t *dims, int32_t n_rep, int32_t direction, void *stream);
+
+/* Apply `schedule` (n_rows rows of (update code, count); codes 0 = metropolis,
+ * 2 = heatbath) n_meas times to n_rep chains of a periodic hypercubic lattice
+ * with even extents, recording E_b, M and the accepted count after every
+ * application into e_out/m_out/acc_out [n_rep][n_meas] (int32).
+ * Replaces critmc/kernels/ising.py:173-237 (run) for checkerboard sweeps.
+ *   layout  0 = int8 spins, 1 = packed (see cm_ising_convert)
+ *   betas   host [n_rep]; h is the uniform field
+ *   keys_dev [n_rep][2], stream_ids_dev [n_rep], sweeps_dev [n_rep] (in/out);
+ *   sweeps_host [n_rep] mirrors sweeps_dev (read by the plain form)
+ *   e_run_dev / m_run_dev [n_rep]: running E_b, M (in/out)
+ *   workspace_dev: >= cm_ising_workspace_bytes(n_rep) bytes
+ *   form    0 = auto, 1 = plain (one chain, one launch per half sweep),
+ *           2 = batched (one launch, LDS-resident bit-packed replicas; 2D with
+ *           L0 <= 64 or L0 % 64 == 0)
+ *   block, grid: launch shape of the batched form (0 = default)
+ * The plain form synchronises `stream` before returning. */
+int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *dims,
+                 int32_t n_rep, const double *betas, double h, const uint32_t *keys_dev,
+                 const uint32_t *stream_ids_dev, uint64_t *sweeps_dev, const uint64_t *sweeps_host,
+                 const int64_t *schedule, int32_t n_rows, int64_t n_meas, int32_t *e_run_dev,
+                 int32_t *m_run_dev, int32_t *e_out_dev, int32_t *m_out_dev, int32_t *acc_out_dev,
+                 void *workspace_dev, int64_t workspace_bytes, int32_t form, int32_t block,
+                 int32_t grid, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRITMC_HIP_H */

@@ -1,0 +1,207 @@
+"""ORACLE -- test infrastructure, never part of the product path.
+
+CPU restatements of the reference's algorithms used to check the HIP path:
+
+* ``liboracle.so`` (C, built from the ``*.c`` files here by ``Makefile``):
+  Philox4x32-10 and bit-exact replays of the checkerboard schedules.
+* ``exact.py``: exact results (Kaufman torus, enumeration, O(n) rings).
+* ``stats.py``: the Gamma-method error analysis used to judge pulls.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package, and only as the checker.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def build(force: bool = False) -> Path:
+    """Compile ``liboracle.so`` with the Makefile next to this file."""
+    if force and LIB_PATH.exists():
+        LIB_PATH.unlink()
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.orc_philox.argtypes = [_u32p, _u32p, _u32p]
+    L.orc_philox.restype = None
+    L.orc_ising_thresholds.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _u64p, _u64p]
+    L.orc_ising_thresholds.restype = None
+    L.orc_ising_site_uniform.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_uint64, ctypes.c_int,
+                                                                  ctypes.c_uint32, ctypes.c_int]
+    L.orc_ising_site_uniform.restype = ctypes.c_uint32
+    L.orc_ising_energy.argtypes = [_i8p, ctypes.c_int, _i32p, _i64p, _i64p]
+    L.orc_ising_energy.restype = ctypes.c_int
+    L.orc_ising_run.argtypes = [
+        _i8p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_double,
+        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p,
+        _i64p, ctypes.c_int, ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p,
+    ]
+    L.orc_ising_run.restype = ctypes.c_int
+    L.orc_init_spins.argtypes = [_i8p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32]
+    L.orc_init_spins.restype = None
+    for name, decl in _EXTRA_DECLS.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes, fn.restype = decl
+
+
+_EXTRA_DECLS: dict = {}
+
+
+def _ptr(a: np.ndarray, typ):
+    return a.ctypes.data_as(typ)
+
+
+# ---------------------------------------------------------------------------
+def philox(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox(_ptr(c, _u32p), _ptr(k, _u32p), _ptr(out, _u32p))
+    return out
+
+
+def ising_thresholds(beta: float, h: float, z: int) -> tuple[np.ndarray, np.ndarray]:
+    met = np.zeros((2 * z + 1) * 2, dtype=np.uint64)
+    hb = np.zeros(2 * z + 1, dtype=np.uint64)
+    lib().orc_ising_thresholds(float(beta), float(h), int(z), _ptr(met, _u64p), _ptr(hb, _u64p))
+    return met.reshape(2 * z + 1, 2), hb
+
+
+def ising_energy(s: np.ndarray, dims) -> tuple[int, int]:
+    s = np.ascontiguousarray(s, dtype=np.int8)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    e = np.zeros(1, dtype=np.int64)
+    m = np.zeros(1, dtype=np.int64)
+    rc = lib().orc_ising_energy(_ptr(s, _i8p), len(d), _ptr(d, _i32p), _ptr(e, _i64p), _ptr(m, _i64p))
+    if rc:
+        raise ValueError(f"orc_ising_energy failed ({rc})")
+    return int(e[0]), int(m[0])
+
+
+def ising_init(n: int, key, stream: int = 0) -> np.ndarray:
+    s = np.zeros(n, dtype=np.int8)
+    lib().orc_init_spins(_ptr(s, _i8p), int(n), int(key[0]), int(key[1]), int(stream))
+    return s
+
+
+class IsingReplay:
+    """One checkerboard Ising chain replayed on the CPU (mutates ``s`` in place)."""
+
+    def __init__(self, s: np.ndarray, dims, beta: float, h: float, key, stream: int = 0,
+                 sweep: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.int8).copy()
+        self.dims = np.ascontiguousarray(dims, dtype=np.int32)
+        self.beta = float(beta)
+        self.h = float(h)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.sweep = np.array([sweep], dtype=np.uint64)
+        e, m = ising_energy(self.s, self.dims)
+        self.e_b = np.array([e], dtype=np.int64)
+        self.m = np.array([m], dtype=np.int64)
+
+    def run(self, schedule, n_meas: int) -> dict[str, np.ndarray]:
+        sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+        e_out = np.zeros(n_meas, dtype=np.int64)
+        m_out = np.zeros(n_meas, dtype=np.int64)
+        acc = np.zeros(n_meas, dtype=np.int64)
+        rc = lib().orc_ising_run(
+            _ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.beta, self.h,
+            self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p),
+            _ptr(sched, _i64p), sched.shape[0], int(n_meas),
+            _ptr(self.e_b, _i64p), _ptr(self.m, _i64p),
+            _ptr(e_out, _i64p), _ptr(m_out, _i64p), _ptr(acc, _i64p),
+        )
+        if rc:
+            raise ValueError(f"orc_ising_run failed ({rc})")
+        return {"Eb": e_out, "M": m_out, "accepted": acc}
+
+
+os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))
+
+
+_EXTRA_DECLS.update({
+    "orc_ising_run_batch": ([_i8p, ctypes.c_int, ctypes.c_int, _i32p, _f64p, ctypes.c_double,
+                             _u32p, _u32p, _u64p, _i64p, ctypes.c_int, ctypes.c_int64, _i64p,
+                             _i64p, _i64p, _i64p, _i64p, ctypes.c_int], ctypes.c_int),
+    "orc_ising_typewriter_batch": ([_i8p, ctypes.c_int, ctypes.c_int, _i32p, _f64p,
+                                    ctypes.c_double, _u32p, ctypes.c_int64, _i64p, _i64p,
+                                    ctypes.c_int], ctypes.c_int),
+})
+
+
+def ising_energy_batch(s: np.ndarray, dims) -> tuple[np.ndarray, np.ndarray]:
+    e = np.zeros(s.shape[0], np.int64)
+    m = np.zeros(s.shape[0], np.int64)
+    for r in range(s.shape[0]):
+        e[r], m[r] = ising_energy(s[r], dims)
+    return e, m
+
+
+def ising_batch_checkerboard(s: np.ndarray, dims, betas, h, keys, sids, sweeps, schedule,
+                             n_meas: int, threads: int) -> dict[str, np.ndarray]:
+    """Checkerboard replay of R chains with OpenMP (mutates ``s``, ``sweeps``)."""
+    R = s.shape[0]
+    d = np.ascontiguousarray(dims, np.int32)
+    e, m = ising_energy_batch(s, dims)
+    sched = np.ascontiguousarray(schedule, np.int64).reshape(-1, 2)
+    eo = np.zeros((R, n_meas), np.int64)
+    mo = np.zeros((R, n_meas), np.int64)
+    ao = np.zeros((R, n_meas), np.int64)
+    kk = np.ascontiguousarray(keys, np.uint32)
+    ss = np.ascontiguousarray(sids, np.uint32)
+    rc = lib().orc_ising_run_batch(
+        _ptr(s, _i8p), R, len(d), _ptr(d, _i32p), _ptr(np.ascontiguousarray(betas, np.float64), _f64p),
+        float(h), _ptr(kk, _u32p), _ptr(ss, _u32p), _ptr(sweeps, _u64p), _ptr(sched, _i64p),
+        sched.shape[0], int(n_meas), _ptr(e, _i64p), _ptr(m, _i64p), _ptr(eo, _i64p),
+        _ptr(mo, _i64p), _ptr(ao, _i64p), int(threads))
+    if rc:
+        raise ValueError(f"orc_ising_run_batch failed ({rc})")
+    return {"Eb": eo, "M": mo, "accepted": ao}
+
+
+def ising_batch_typewriter(s: np.ndarray, dims, betas, h, keys, n_sweeps: int,
+                           threads: int) -> tuple[np.ndarray, np.ndarray]:
+    R = s.shape[0]
+    d = np.ascontiguousarray(dims, np.int32)
+    e, m = ising_energy_batch(s, dims)
+    kk = np.ascontiguousarray(keys, np.uint32)
+    rc = lib().orc_ising_typewriter_batch(
+        _ptr(s, _i8p), R, len(d), _ptr(d, _i32p), _ptr(np.ascontiguousarray(betas, np.float64), _f64p),
+        float(h), _ptr(kk, _u32p), int(n_sweeps), _ptr(e, _i64p), _ptr(m, _i64p), int(threads))
+    if rc:
+        raise ValueError(f"orc_ising_typewriter_batch failed ({rc})")
+    return e, m
