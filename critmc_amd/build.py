@@ -32,6 +32,10 @@ CFLAGS = [
     "-Wno-unused-command-line-argument",
     # packed FP32 forms are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # gfx950's three-input v_bitop3_b32, as formed by this toolchain (ROCm 7.2
+    # clang 22), gave wrong bit-sliced comparison results against the same code
+    # compiled for the host; without it the device matches the host bit for bit
+    "-Xclang", "-target-feature", "-Xclang", "-bitop3-insts",
 ]
 
 

@@ -19,6 +19,50 @@ void set_error(const char* fmt, ...)
 
 const char* last_error() { return g_error; }
 
+namespace {
+struct Staging {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+constexpr int kMaxDevices = 64;
+thread_local Staging g_staging[kMaxDevices];
+}  // namespace
+
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    Staging& st = g_staging[dev];
+    if (st.done == nullptr) {
+        e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    if (st.pending) {
+        e = hipEventSynchronize(st.done);
+        if (e != hipSuccess) return e;
+        st.pending = false;
+    }
+    if (st.cap < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.cap = 0;
+        e = hipHostMalloc(&st.host, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        st.cap = bytes;
+    }
+    std::memcpy(st.host, src, bytes);
+    e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(st.done, stream);
+    if (e != hipSuccess) return e;
+    st.pending = true;
+    return hipSuccess;
+}
+
 __global__ void philox_fill_kernel(uint32_t* __restrict__ out, int64_t n_blocks, uint32_t k0,
                                    uint32_t k1, uint32_t x0, uint32_t y, uint32_t z, uint32_t w)
 {

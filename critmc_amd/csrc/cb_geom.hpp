@@ -1,0 +1,65 @@
+// Checkerboard geometry of periodic hypercubic lattices (d <= 3, even extents).
+//
+// Site numbering is x-fastest, i = x0 + L0 (x1 + L1 x2) (critmc/lattice.py:118-128).
+// The colour-c sites are enumerated row by row: k = r * M + m with row
+// r = x1 + L1 x2, M = L0/2 and x0 = 2m + ((x1 + x2 + c) & 1).  Neighbours are
+// returned in the reference's column order +x, -x, +y, -y, +z, -z
+// (critmc/lattice.py:129-134).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cm {
+
+struct CbGeo {
+    int d, L0, L1, L2, M;
+    int64_t N, half;  // sites, sites per colour
+};
+
+template <int Z>
+__host__ __device__ __forceinline__ int64_t cb_site(const CbGeo& g, int64_t k, int c,
+                                                    int64_t (&nb)[Z])
+{
+    const int64_t r = k / g.M;
+    const int m = (int)(k - r * g.M);
+    int x1 = 0, x2 = 0;
+    if (Z >= 4) {
+        x1 = (int)(r % g.L1);
+        x2 = (int)(r / g.L1);
+    }
+    const int x0 = 2 * m + ((x1 + x2 + c) & 1);
+    const int64_t base = r * g.L0;
+    const int64_t i = base + x0;
+    nb[0] = base + (x0 + 1 == g.L0 ? 0 : x0 + 1);
+    nb[1] = base + (x0 == 0 ? g.L0 - 1 : x0 - 1);
+    if (Z >= 4) {
+        const int64_t row0 = (int64_t)x2 * g.L1;
+        nb[2] = (row0 + (x1 + 1 == g.L1 ? 0 : x1 + 1)) * g.L0 + x0;
+        nb[3] = (row0 + (x1 == 0 ? g.L1 - 1 : x1 - 1)) * g.L0 + x0;
+    }
+    if (Z == 6) {
+        nb[4] = ((int64_t)(x2 + 1 == g.L2 ? 0 : x2 + 1) * g.L1 + x1) * g.L0 + x0;
+        nb[5] = ((int64_t)(x2 == 0 ? g.L2 - 1 : x2 - 1) * g.L1 + x1) * g.L0 + x0;
+    }
+    return i;
+}
+
+inline int cb_geo_from(int ndim, const int32_t* dims, CbGeo* g)
+{
+    if (ndim < 1 || ndim > 3) return -1;
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) {
+        if (dims[a] < 2 || dims[a] % 2) return -2;
+        N *= dims[a];
+    }
+    g->d = ndim;
+    g->L0 = dims[0];
+    g->L1 = ndim >= 2 ? dims[1] : 1;
+    g->L2 = ndim >= 3 ? dims[2] : 1;
+    g->M = g->L0 / 2;
+    g->N = N;
+    g->half = N / 2;
+    return 0;
+}
+
+}  // namespace cm

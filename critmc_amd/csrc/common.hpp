@@ -44,6 +44,12 @@ enum Status : int {
 void set_error(const char* fmt, ...);
 const char* last_error();
 
+// Stream-ordered host-to-device copy of a small host buffer.  The bytes go
+// through a pinned staging buffer owned by the calling thread, so `src` may be
+// released as soon as this returns; the staging buffer is reused only after
+// the previous copy from it has completed.
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 #define CM_HIP_CHECK(expr)                                                              \
     do {                                                                                \
         hipError_t err__ = (expr);                                                      \

@@ -675,7 +675,7 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     const int nc = pick_nc(nc_max);
     hipStream_t st = (hipStream_t)stream;
     uint32_t* dtabs = (uint32_t*)workspace;
-    CM_HIP_CHECK(hipMemcpyAsync(dtabs, tabs.data(), tabs.size() * 4, hipMemcpyHostToDevice, st));
+    CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * 4, st));
 
     if (form == 0) form = batched_ok(g) ? 2 : 1;
     if (form == 2) {

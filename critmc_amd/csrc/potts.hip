@@ -1,0 +1,374 @@
+// Checkerboard q-state Potts sweeps (int8 states 0..q-1), Metropolis and heat bath.
+//
+// Reference semantics (critmc/kernels/potts.py):
+//   Metropolis :44-71  new = randbelow(q-1), +1 if >= old; delta = n_old - n_new;
+//                      accept iff delta <= 0 or u < exp(-beta delta) (table :194-197)
+//   heat bath  :74-103 w[n] = exp(beta n) (:198); r = u * sum_k w[n_k]; new = first k
+//                      with r < cumulative sum (k ascending), default q-1
+//   run        :170-235 per measurement E = -#satisfied bonds, counts[q], accepted
+//
+// One workgroup owns one replica at a time (grid-stride over replicas) and runs
+// the whole call's sweeps; the replica's spins sit in LDS when they fit,
+// otherwise the workgroup updates them in place in global memory.
+//
+// Uniforms: the colour-c sites are enumerated k = 0 .. N/2-1 (row-major, see
+// cb_geom.hpp); site k uses words 2(k&1) and 2(k&1)+1 of the Philox block
+// k >> 1 (tag TAG_POTTS): the first is the proposal (randbelow(n) =
+// floor(x n / 2^32)) or the heat-bath u, the second the Metropolis u.  The
+// heat-bath arithmetic is FP64 in the reference's order, compiled without
+// contraction, so it is bit-reproducible on the host.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cb_geom.hpp"
+#include "common.hpp"
+#include "ising_core.hpp"
+#include "philox.hpp"
+
+namespace cm {
+
+struct PottsTab {
+    uint64_t T[8];  // Metropolis: accept iff x < T[delta] (delta >= 1); T = ceil(exp(-beta d) 2^32)
+    double w[8];    // heat bath: exp(beta n)
+};
+
+constexpr int POTTS_QMAX = 127;
+constexpr int64_t POTTS_LDS_STATE = 96 * 1024;
+
+template <int Z>
+__device__ __forceinline__ void potts_site(int8_t* __restrict__ S, int64_t i,
+                                           const int64_t (&nbi)[Z], int q, int code, uint32_t xa,
+                                           uint32_t xb, const PottsTab* __restrict__ tb, int& de, int& acc,
+                                           int* __restrict__ counts)
+{
+    const int old = S[i];
+    int nb[Z];
+#pragma unroll
+    for (int j = 0; j < Z; ++j) nb[j] = S[nbi[j]];
+    if (code == CODE_METROPOLIS) {
+        int nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
+        if (nw >= old) ++nw;
+        int n_old = 0, n_new = 0;
+#pragma unroll
+        for (int j = 0; j < Z; ++j) {
+            n_old += nb[j] == old;
+            n_new += nb[j] == nw;
+        }
+        const int delta = n_old - n_new;
+        if (delta <= 0 || (uint64_t)xb < tb->T[delta]) {
+            S[i] = (int8_t)nw;
+            de += delta;
+            ++acc;
+            atomicAdd(&counts[old], -1);
+            atomicAdd(&counts[nw], 1);
+        }
+    } else {
+        double total = 0.0;
+        for (int k = 0; k < q; ++k) {
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < Z; ++j) c += nb[j] == k;
+            total = __dadd_rn(total, tb->w[c]);
+        }
+        const double r = __dmul_rn((double)xa * 0x1p-32, total);
+        int nw = q - 1;
+        double cum = 0.0;
+        for (int k = 0; k < q; ++k) {
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < Z; ++j) c += nb[j] == k;
+            cum = __dadd_rn(cum, tb->w[c]);
+            if (r < cum) {
+                nw = k;
+                break;
+            }
+        }
+        if (nw != old) {
+            int c_old = 0, c_new = 0;
+#pragma unroll
+            for (int j = 0; j < Z; ++j) {
+                c_old += nb[j] == old;
+                c_new += nb[j] == nw;
+            }
+            de += c_old - c_new;
+            ++acc;
+            S[i] = (int8_t)nw;
+            atomicAdd(&counts[old], -1);
+            atomicAdd(&counts[nw], 1);
+        }
+    }
+}
+
+template <int Z>
+__global__ __launch_bounds__(256) void potts_batched(
+    int8_t* __restrict__ spins, CbGeo g, int q, int R, const PottsTab* __restrict__ tabs,
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
+    uint64_t* __restrict__ sweeps, Sched sc, int64_t n_meas, int32_t* __restrict__ e_run,
+    int32_t* __restrict__ counts_run, int32_t* __restrict__ e_out, int32_t* __restrict__ acc_out,
+    int32_t* __restrict__ counts_out, int use_lds)
+{
+    extern __shared__ int4 lds4[];
+    int* counts = (int*)lds4;          // [128]
+    int* red = counts + 128;           // [16][2]
+    int8_t* lstate = (int8_t*)(red + 32);
+    const int nwaves = (blockDim.x + 63) >> 6;
+    const int64_t npairs = (g.half + 1) >> 1;
+
+    for (int rep = blockIdx.x; rep < R; rep += gridDim.x) {
+        int8_t* gs = spins + (int64_t)rep * g.N;
+        int8_t* S = use_lds ? lstate : gs;
+        if (use_lds) {
+            if ((g.N & 15) == 0) {
+                const int4* src = (const int4*)gs;
+                int4* dst = (int4*)lstate;
+                for (int64_t t = threadIdx.x; t < g.N / 16; t += blockDim.x) dst[t] = src[t];
+            } else {
+                for (int64_t t = threadIdx.x; t < g.N; t += blockDim.x) lstate[t] = gs[t];
+            }
+        }
+        for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = counts_run[(int64_t)rep * q + k];
+        __syncthreads();
+        const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+        const PottsTab* tb = tabs + rep;
+        uint64_t sweep = sweeps[rep];
+        int32_t e = e_run[rep];
+        for (int64_t meas = 0; meas < n_meas; ++meas) {
+            int de = 0, acc = 0;
+            for (int row = 0; row < sc.n; ++row) {
+                const int code = sc.code[row];
+                for (int rp = 0; rp < sc.count[row]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        const uint32_t zc = ctr_z(sweep, (uint32_t)c, TAG_POTTS);
+                        for (int64_t pi = threadIdx.x; pi < npairs; pi += blockDim.x) {
+                            const u32x4 rn = philox4x32_10(
+                                u32x4{(uint32_t)pi, (uint32_t)sweep, zc, sid}, k0, k1);
+                            int64_t nbi[Z];
+                            int64_t i = cb_site<Z>(g, 2 * pi, c, nbi);
+                            potts_site<Z>(S, i, nbi, q, code, rn.x, rn.y, tb, de, acc, counts);
+                            if (2 * pi + 1 < g.half) {
+                                i = cb_site<Z>(g, 2 * pi + 1, c, nbi);
+                                potts_site<Z>(S, i, nbi, q, code, rn.z, rn.w, tb, de, acc, counts);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    ++sweep;
+                }
+            }
+            de = wave_sum(de);
+            acc = wave_sum(acc);
+            if ((threadIdx.x & 63) == 0) {
+                red[2 * (threadIdx.x >> 6)] = de;
+                red[2 * (threadIdx.x >> 6) + 1] = acc;
+            }
+            __syncthreads();
+            int tde = 0, tac = 0;
+            for (int w = 0; w < nwaves; ++w) {
+                tde += red[2 * w];
+                tac += red[2 * w + 1];
+            }
+            e += tde;
+            const int64_t o = (int64_t)rep * n_meas + meas;
+            if (threadIdx.x == 0) {
+                e_out[o] = e;
+                acc_out[o] = tac;
+            }
+            for (int k = threadIdx.x; k < q; k += blockDim.x) counts_out[o * q + k] = counts[k];
+            __syncthreads();
+        }
+        if (use_lds) {
+            if ((g.N & 15) == 0) {
+                const int4* src = (const int4*)lstate;
+                int4* dst = (int4*)gs;
+                for (int64_t t = threadIdx.x; t < g.N / 16; t += blockDim.x) dst[t] = src[t];
+            } else {
+                for (int64_t t = threadIdx.x; t < g.N; t += blockDim.x) gs[t] = lstate[t];
+            }
+        }
+        for (int k = threadIdx.x; k < q; k += blockDim.x) counts_run[(int64_t)rep * q + k] = counts[k];
+        if (threadIdx.x == 0) {
+            e_run[rep] = e;
+            sweeps[rep] = sweep;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void potts_init_kernel(int8_t* __restrict__ s, int64_t N, int R, int q,
+                                  const uint32_t* __restrict__ keys,
+                                  const uint32_t* __restrict__ sids, int start)
+{
+    const int rep = blockIdx.y;
+    const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+    for (int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; blk * 4 < N;
+         blk += (int64_t)gridDim.x * blockDim.x) {
+        u32x4 r{0, 0, 0, 0};
+        if (start == 0) r = draw(k0, k1, sid, 0, 0, TAG_INIT, (uint32_t)blk);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+        for (int k = 0; k < 4 && blk * 4 + k < N; ++k)
+            s[(int64_t)rep * N + blk * 4 + k] = (int8_t)(((uint64_t)w[k] * (uint32_t)q) >> 32);
+    }
+}
+
+// E = -#satisfied bonds and counts[q] per replica (critmc/kernels/potts.py:27-41).
+template <int Z>
+__global__ __launch_bounds__(256) void potts_measure_kernel(const int8_t* __restrict__ s, CbGeo g,
+                                                            int q, int32_t* __restrict__ e_run,
+                                                            int32_t* __restrict__ counts_run)
+{
+    __shared__ int counts[128];
+    __shared__ int red[16];
+    const int rep = blockIdx.x;
+    const int8_t* S = s + (int64_t)rep * g.N;
+    for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = 0;
+    __syncthreads();
+    int e = 0;
+    for (int c = 0; c < 2; ++c)
+        for (int64_t k = threadIdx.x; k < g.half; k += blockDim.x) {
+            int64_t nbi[Z];
+            const int64_t i = cb_site<Z>(g, k, c, nbi);
+            const int v = S[i];
+            atomicAdd(&counts[v], 1);
+            // forward bonds only: columns 0, 2, 4
+#pragma unroll
+            for (int j = 0; j < Z; j += 2) e -= S[nbi[j]] == v;
+        }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+        e_run[rep] = t;
+    }
+    for (int k = threadIdx.x; k < q; k += blockDim.x) counts_run[(int64_t)rep * q + k] = counts[k];
+}
+
+static uint64_t potts_threshold(double p)
+{
+    if (!(p < 1.0)) return 1ull << 32;
+    if (!(p > 0.0)) return 0;
+    return (uint64_t)std::ceil(p * 4294967296.0);
+}
+
+template <int Z>
+static hipError_t launch_potts(int8_t* spins, const CbGeo& g, int q, int R, const PottsTab* tabs,
+                               const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
+                               const Sched& sc, int64_t n_meas, int32_t* e_run,
+                               int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
+                               int32_t* counts_out, int block, int grid, hipStream_t st)
+{
+    const int use_lds = g.N <= POTTS_LDS_STATE ? 1 : 0;
+    const size_t lds = (128 + 32) * 4 + (use_lds ? (size_t)((g.N + 15) / 16) * 16 : 0);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)potts_batched<Z>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((potts_batched<Z>), dim3(grid), dim3(block), lds, st, spins, g, q, R, tabs,
+                       keys, sids, sweeps, sc, n_meas, e_run, counts_run, e_out, acc_out,
+                       counts_out, use_lds);
+    return hipGetLastError();
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int64_t cm_potts_workspace_bytes(int32_t n_rep) { return (int64_t)n_rep * sizeof(PottsTab); }
+
+int cm_potts_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                  const uint32_t* keys, const uint32_t* stream_ids, int32_t start, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(start == 0 || start == 1, "start must be 0 (random) or 1 (ordered)");
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((g.N / 4 + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(potts_init_kernel, dim3((unsigned)blocks, n_rep), dim3(256), 0,
+                       (hipStream_t)stream, (int8_t*)spins, g.N, n_rep, q, keys, stream_ids,
+                       start);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_potts_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                     int32_t q, int32_t* e_run, int32_t* counts_run, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
+    hipStream_t st = (hipStream_t)stream;
+    const int8_t* s = (const int8_t*)spins;
+    if (g.d == 1)
+        hipLaunchKernelGGL(potts_measure_kernel<2>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    else if (g.d == 2)
+        hipLaunchKernelGGL(potts_measure_kernel<4>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    else
+        hipLaunchKernelGGL(potts_measure_kernel<6>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                 const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
+                 uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
+                 int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
+                 int32_t* counts_out, void* workspace, int64_t workspace_bytes, int32_t block,
+                 int32_t grid, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
+    CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    CM_REQUIRE(workspace_bytes >= cm_potts_workspace_bytes(n_rep), "workspace too small");
+    CM_REQUIRE(g.N <= (int64_t)1 << 30, "lattice too large");
+    Sched sc{};
+    sc.n = n_rows;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
+                   "update code %lld has no checkerboard Potts kernel", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
+        sc.code[r] = (int32_t)code;
+        sc.count[r] = (int32_t)count;
+    }
+    if (n_meas == 0) return CM_OK;
+    const int z = 2 * g.d;
+    std::vector<PottsTab> tabs(n_rep);
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        for (int d = 0; d < 8; ++d) {
+            tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
+            tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    PottsTab* dtabs = (PottsTab*)workspace;
+    CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * sizeof(PottsTab), st));
+    if (block <= 0) block = 256;
+    CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
+    if (grid <= 0) grid = n_rep;
+    grid = std::min(grid, n_rep);
+    hipError_t err;
+    int8_t* s = (int8_t*)spins;
+    if (z == 2)
+        err = launch_potts<2>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+    else if (z == 4)
+        err = launch_potts<4>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+    else
+        err = launch_potts<6>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+    CM_HIP_CHECK(err);
+    return CM_OK;
+}
+
+}  // extern "C"

@@ -24,11 +24,12 @@ import torch
 
 from .device import device
 from .kernels import ising as k_ising
+from .kernels import potts as k_potts
 from .kernels.chains import Chains
 from .kernels.common import GPU_UPDATES, N_AUX, UPDATE_CODES
 from .lattice import Lattice, make_lattice, require_checkerboard
 
-__all__ = ["Model", "Ising", "build_model", "derive", "DeviceState"]
+__all__ = ["Model", "Ising", "Potts", "build_model", "derive", "DeviceState"]
 
 
 @dataclass
@@ -171,6 +172,67 @@ class Ising(Model):
         return {**super().spec(), "h": self.h}
 
 
+def _potts_bond_energy(s: np.ndarray, bonds: np.ndarray) -> int:
+    return -int(np.sum(s[bonds[:, 0]] == s[bonds[:, 1]]))
+
+
+class Potts(Model):
+    """q-state Potts model (critmc/models.py:165-222)."""
+
+    kind = "potts"
+    updates = GPU_UPDATES["potts"]
+
+    def __init__(self, lattice: Lattice, q: int):
+        super().__init__(lattice)
+        if q < 2 or q > 127:
+            raise ValueError(f"q must be in [2, 127], got {q}")
+        self.q = int(q)
+
+    def initial_state(self, start, rng):
+        n = self.n_sites
+        if start == "random":
+            return rng.integers(0, self.q, size=n).astype(np.int8)
+        if start in ("ordered", "up"):
+            return np.zeros(n, dtype=np.int8)
+        raise ValueError(f"unknown start {start!r}")
+
+    def energy(self, state):
+        return float(_potts_bond_energy(np.asarray(state), self.lattice.bonds))
+
+    def begin(self, state):
+        s = np.asarray(state)
+        return {"energy": _potts_bond_energy(s, self.lattice.bonds),
+                "counts": np.bincount(s, minlength=self.q).astype(np.int64)}
+
+    def to_device(self, state, running):
+        dev = device()
+        spins = torch.from_numpy(np.ascontiguousarray(state, dtype=np.int8)).to(dev).reshape(1, -1)
+        e = torch.tensor([running["energy"]], dtype=torch.int32, device=dev)
+        c = torch.from_numpy(np.asarray(running["counts"], dtype=np.int32)).to(dev).reshape(1, -1)
+        return DeviceState(spins, {"e": e, "counts": c})
+
+    def from_device(self, ds):
+        return ds.spins.reshape(-1).cpu().numpy().copy()
+
+    def workspace(self):
+        return k_potts.Workspace(1)
+
+    def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
+        e_out, acc, counts = k_potts.run(ds.spins, self.lattice.shape, self.q, [float(beta)],
+                                         chains, schedule, n_meas, ds.extra["e"],
+                                         ds.extra["counts"], ws=ws)
+        e = e_out[0].cpu().numpy().astype(np.int64)
+        c = counts[0].cpu().numpy().astype(np.int64)
+        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
+        aux[:, 0] = acc[0].cpu().numpy()
+        running["energy"] = int(e[-1])
+        running["counts"] = c[-1].copy()
+        return {"E": e, "counts": c, "aux": aux}
+
+    def spec(self):
+        return {**super().spec(), "q": self.q}
+
+
 # kernel form used by single-chain models: 0 auto, 1 plain (one launch per half sweep), 2 batched
 _FORM = [0]
 
@@ -181,15 +243,14 @@ def set_kernel_form(form: int) -> None:
     _FORM[0] = int(form)
 
 
-_MODEL_KINDS: dict[str, Any] = {"ising": Ising}
-
-
 def build_model(spec: dict[str, Any]) -> Model:
     """Inverse of :meth:`Model.spec` (critmc/models.py:312-322)."""
     lattice = make_lattice(spec["lattice"], int(spec["L"]))
     kind = spec["model"]
     if kind == "ising":
         return Ising(lattice, h=float(spec.get("h", 0.0)))
+    if kind == "potts":
+        return Potts(lattice, int(spec["q"]))
     raise ValueError(f"unknown model kind {kind!r}")
 
 

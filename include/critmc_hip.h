@@ -101,6 +101,35 @@ int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *d
                  void *workspace_dev, int64_t workspace_bytes, int32_t form, int32_t block,
                  int32_t grid, void *stream);
 
+/* ---- Potts (critmc/kernels/potts.py) -------------------------------------- */
+
+/* Bytes of caller-owned device scratch cm_potts_run needs for n_rep chains. */
+int64_t cm_potts_workspace_bytes(int32_t n_rep);
+
+/* Initial int8 states [n_rep][N] in 0..q-1: start 0 = random (Philox tag
+ * INIT, floor(x q / 2^32)), 1 = all 0.  Replaces Potts.initial_state
+ * (critmc/models.py:177-183) for device-side batches. */
+int cm_potts_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
+                  const uint32_t *keys_dev, const uint32_t *stream_ids_dev, int32_t start,
+                  void *stream);
+
+/* E = -#satisfied bonds and counts[q] per replica: critmc/kernels/potts.py:27-41. */
+int cm_potts_measure(const void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
+                     int32_t q, int32_t *e_run_dev, int32_t *counts_run_dev, void *stream);
+
+/* Apply `schedule` (codes 0 = metropolis, 2 = heatbath) n_meas times to n_rep
+ * q-state Potts chains (2 <= q <= 127), recording E, accepted [n_rep][n_meas]
+ * and counts [n_rep][n_meas][q] after every application.
+ * Replaces critmc/kernels/potts.py:170-235 (run) for checkerboard sweeps.
+ * e_run_dev [n_rep] and counts_run_dev [n_rep][q] are the running totals (in/out);
+ * other arguments as cm_ising_run. */
+int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
+                 const double *betas, const uint32_t *keys_dev, const uint32_t *stream_ids_dev,
+                 uint64_t *sweeps_dev, const int64_t *schedule, int32_t n_rows, int64_t n_meas,
+                 int32_t *e_run_dev, int32_t *counts_run_dev, int32_t *e_out_dev,
+                 int32_t *acc_out_dev, int32_t *counts_out_dev, void *workspace_dev,
+                 int64_t workspace_bytes, int32_t block, int32_t grid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -205,3 +205,61 @@ def ising_batch_typewriter(s: np.ndarray, dims, betas, h, keys, n_sweeps: int,
     if rc:
         raise ValueError(f"orc_ising_typewriter_batch failed ({rc})")
     return e, m
+
+
+_EXTRA_DECLS.update({
+    "orc_potts_energy": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, _i64p, _i64p], ctypes.c_int),
+    "orc_potts_run": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32,
+                       ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
+                       ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p], ctypes.c_int),
+    "orc_potts_init": ([_i8p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                        ctypes.c_uint32], None),
+})
+
+
+def potts_energy(s: np.ndarray, dims, q: int) -> tuple[int, np.ndarray]:
+    s = np.ascontiguousarray(s, dtype=np.int8)
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    e = np.zeros(1, np.int64)
+    c = np.zeros(q, np.int64)
+    rc = lib().orc_potts_energy(_ptr(s, _i8p), len(d), _ptr(d, _i32p), int(q), _ptr(e, _i64p),
+                                _ptr(c, _i64p))
+    if rc:
+        raise ValueError(f"orc_potts_energy failed ({rc})")
+    return int(e[0]), c
+
+
+def potts_init(n: int, q: int, key, stream: int = 0) -> np.ndarray:
+    s = np.zeros(n, dtype=np.int8)
+    lib().orc_potts_init(_ptr(s, _i8p), int(n), int(q), int(key[0]), int(key[1]), int(stream))
+    return s
+
+
+class PottsReplay:
+    """One checkerboard Potts chain replayed on the CPU."""
+
+    def __init__(self, s, dims, q: int, beta: float, key, stream: int = 0, sweep: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.int8).copy()
+        self.dims = np.ascontiguousarray(dims, dtype=np.int32)
+        self.q = int(q)
+        self.beta = float(beta)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.sweep = np.array([sweep], dtype=np.uint64)
+        e, c = potts_energy(self.s, self.dims, self.q)
+        self.energy = np.array([e], np.int64)
+        self.counts = c
+
+    def run(self, schedule, n_meas: int) -> dict[str, np.ndarray]:
+        sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+        e_out = np.zeros(n_meas, np.int64)
+        c_out = np.zeros((n_meas, self.q), np.int64)
+        acc = np.zeros(n_meas, np.int64)
+        rc = lib().orc_potts_run(
+            _ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.q, self.beta,
+            self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p), _ptr(sched, _i64p),
+            sched.shape[0], int(n_meas), _ptr(self.energy, _i64p), _ptr(self.counts, _i64p),
+            _ptr(e_out, _i64p), _ptr(c_out, _i64p), _ptr(acc, _i64p))
+        if rc:
+            raise ValueError(f"orc_potts_run failed ({rc})")
+        return {"E": e_out, "counts": c_out, "accepted": acc}

@@ -1,0 +1,180 @@
+/*
+ * ORACLE (test infrastructure only) -- CPU replay of the checkerboard Potts
+ * schedule, bit-exact target of the GPU kernels.
+ *
+ * Rules restated from the reference (critmc/kernels/potts.py):
+ *   Metropolis :44-71 (boltz[d] = exp(-beta d), :194-197)
+ *   heat bath  :74-103 (weight[n] = exp(beta n), :198), FP64 sums in k order
+ *   energy / counts :27-41, run :170-235
+ * Visiting order: colour 0 then colour 1; within a colour the sites are
+ * k = r*M + m (row r = x1 + L1 x2, x0 = 2m + ((x1+x2+c)&1)).  Site k uses words
+ * 2(k&1), 2(k&1)+1 of Philox block k>>1 (tag POTTS): proposal / heat-bath u,
+ * then the Metropolis u.  randbelow(n) = floor(x n / 2^32).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "philox.h"
+
+typedef struct {
+    int d, L[3], M;
+    int64_t N, half;
+} orc_cb;
+
+static int orc_cb_init(orc_cb *g, int ndim, const int32_t *dims)
+{
+    if (ndim < 1 || ndim > 3) return -1;
+    g->d = ndim;
+    g->N = 1;
+    for (int a = 0; a < 3; ++a) g->L[a] = 1;
+    for (int a = 0; a < ndim; ++a) {
+        if (dims[a] < 2 || (dims[a] & 1)) return -2;
+        g->L[a] = dims[a];
+        g->N *= dims[a];
+    }
+    g->M = g->L[0] / 2;
+    g->half = g->N / 2;
+    return 0;
+}
+
+/* site of colour-c index k and its neighbours in column order +x,-x,+y,-y,+z,-z */
+int64_t orc_cb_site(const orc_cb *g, int64_t k, int c, int64_t *nb)
+{
+    int64_t r = k / g->M;
+    int m = (int)(k - r * g->M);
+    int x1 = (int)(r % g->L[1]), x2 = (int)(r / g->L[1]);
+    int x0 = 2 * m + ((x1 + x2 + c) & 1);
+    int64_t base = r * g->L[0];
+    nb[0] = base + (x0 + 1 == g->L[0] ? 0 : x0 + 1);
+    nb[1] = base + (x0 == 0 ? g->L[0] - 1 : x0 - 1);
+    if (g->d >= 2) {
+        int64_t row0 = (int64_t)x2 * g->L[1];
+        nb[2] = (row0 + (x1 + 1 == g->L[1] ? 0 : x1 + 1)) * g->L[0] + x0;
+        nb[3] = (row0 + (x1 == 0 ? g->L[1] - 1 : x1 - 1)) * g->L[0] + x0;
+    }
+    if (g->d == 3) {
+        nb[4] = ((int64_t)(x2 + 1 == g->L[2] ? 0 : x2 + 1) * g->L[1] + x1) * g->L[0] + x0;
+        nb[5] = ((int64_t)(x2 == 0 ? g->L[2] - 1 : x2 - 1) * g->L[1] + x1) * g->L[0] + x0;
+    }
+    return base + x0;
+}
+
+static uint64_t orc_thr(double p)
+{
+    if (!(p < 1.0)) return 1ull << 32;
+    if (!(p > 0.0)) return 0;
+    return (uint64_t)ceil(p * 4294967296.0);
+}
+
+int orc_potts_energy(const int8_t *s, int ndim, const int32_t *dims, int q, int64_t *energy,
+                     int64_t *counts)
+{
+    orc_cb g;
+    int rc = orc_cb_init(&g, ndim, dims);
+    if (rc) return rc;
+    int64_t e = 0, nb[6];
+    for (int k = 0; k < q; ++k) counts[k] = 0;
+    for (int c = 0; c < 2; ++c)
+        for (int64_t k = 0; k < g.half; ++k) {
+            int64_t i = orc_cb_site(&g, k, c, nb);
+            counts[s[i]] += 1;
+            for (int j = 0; j < 2 * g.d; j += 2) e -= s[nb[j]] == s[i];
+        }
+    *energy = e;
+    return 0;
+}
+
+int orc_potts_run(int8_t *s, int ndim, const int32_t *dims, int q, double beta, uint32_t k0,
+                  uint32_t k1, uint32_t stream, uint64_t *sweep, const int64_t *sched, int nrows,
+                  int64_t n_meas, int64_t *energy, int64_t *counts, int64_t *e_out,
+                  int64_t *counts_out, int64_t *acc_out)
+{
+    orc_cb g;
+    int rc = orc_cb_init(&g, ndim, dims);
+    if (rc) return rc;
+    const int z = 2 * g.d;
+    uint64_t T[8];
+    double w[8];
+    for (int d = 0; d <= z; ++d) {
+        T[d] = orc_thr(exp(-beta * d));
+        w[d] = exp(beta * d);
+    }
+    int64_t nbi[6];
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        int64_t acc = 0;
+        for (int row = 0; row < nrows; ++row) {
+            int code = (int)sched[2 * row];
+            if (code != 0 && code != 2) return -3;
+            for (int64_t rp = 0; rp < sched[2 * row + 1]; ++rp) {
+                for (int c = 0; c < 2; ++c) {
+                    uint32_t blk[4];
+                    for (int64_t k = 0; k < g.half; ++k) {
+                        if ((k & 1) == 0)
+                            orc_draw_block(k0, k1, stream, *sweep, c, ORC_TAG_POTTS,
+                                           (uint32_t)(k >> 1), blk);
+                        uint32_t xa = blk[2 * (k & 1)], xb = blk[2 * (k & 1) + 1];
+                        int64_t i = orc_cb_site(&g, k, c, nbi);
+                        int old = s[i];
+                        int nb[6];
+                        for (int j = 0; j < z; ++j) nb[j] = s[nbi[j]];
+                        if (code == 0) {
+                            int nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
+                            if (nw >= old) ++nw;
+                            int n_old = 0, n_new = 0;
+                            for (int j = 0; j < z; ++j) {
+                                n_old += nb[j] == old;
+                                n_new += nb[j] == nw;
+                            }
+                            int delta = n_old - n_new;
+                            if (delta <= 0 || (uint64_t)xb < T[delta]) {
+                                s[i] = (int8_t)nw;
+                                counts[old] -= 1;
+                                counts[nw] += 1;
+                                *energy += delta;
+                                ++acc;
+                            }
+                        } else {
+                            int nk[128];
+                            for (int kk = 0; kk < q; ++kk) nk[kk] = 0;
+                            for (int j = 0; j < z; ++j) nk[nb[j]] += 1;
+                            double total = 0.0;
+                            for (int kk = 0; kk < q; ++kk) total += w[nk[kk]];
+                            double r = ((double)xa * 0x1p-32) * total;
+                            int nw = q - 1;
+                            double cum = 0.0;
+                            for (int kk = 0; kk < q; ++kk) {
+                                cum += w[nk[kk]];
+                                if (r < cum) {
+                                    nw = kk;
+                                    break;
+                                }
+                            }
+                            if (nw != old) {
+                                *energy += nk[old] - nk[nw];
+                                counts[old] -= 1;
+                                counts[nw] += 1;
+                                s[i] = (int8_t)nw;
+                                ++acc;
+                            }
+                        }
+                    }
+                }
+                *sweep += 1;
+            }
+        }
+        e_out[meas] = *energy;
+        for (int kk = 0; kk < q; ++kk) counts_out[meas * q + kk] = counts[kk];
+        acc_out[meas] = acc;
+    }
+    return 0;
+}
+
+void orc_potts_init(int8_t *s, int64_t n, int q, uint32_t k0, uint32_t k1, uint32_t stream)
+{
+    uint32_t w[4];
+    for (int64_t i = 0; i < n; ++i) {
+        if ((i & 3) == 0) orc_draw_block(k0, k1, stream, 0, 0, ORC_TAG_INIT, (uint32_t)(i >> 2), w);
+        s[i] = (int8_t)(((uint64_t)w[i & 3] * (uint32_t)q) >> 32);
+    }
+}
