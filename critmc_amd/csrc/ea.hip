@@ -1,0 +1,490 @@
+// Edwards-Anderson +-J spin glass with parallel tempering, batched over disorder
+// samples of uneven size.
+//
+// Reference (critmc/kernels/glass.py, critmc/glass.py):
+//   sweep     glass.py:41-54   h = sum_k J_ik s_k, dE = 2 s h; Metropolis accept iff
+//                              dE <= 0 or u < exp(-beta dE); the heat bath takes
+//                              P(+1) = 1/(1 + exp(-2 beta h)) (ising.py:59-65 with field h)
+//   swaps     glass.py:129-137 k = 0..m-2 in order, x = (b_k - b_{k+1})(E_c1 - E_c2),
+//                              accept iff x >= 0 or u < exp(x); only perm changes
+//   labels    glass.py:139-152 hot end -> 1, cold end -> 2, one-way trips, label_hist
+//   measure   glass.py:56-77, 153-163  q, q_l, |q(k_min)|^2, log-binned accumulators
+//
+// Layout: sample s owns int8 spins [2 sets][m configurations][N_s] at spin_off[s]
+// and int8 forward-bond couplings [d][N_s] at j_off[s] (bond a*N + i joins i and
+// its +e_a neighbour, critmc/lattice.py:136-138).  perm/kofc/labels/energies/
+// sweep counters are [S][2][m] (configuration-indexed except perm, which maps
+// temperature -> configuration).
+//
+// One PT step is three launches: ea_sweep (one workgroup per configuration,
+// spins staged in LDS), ea_swap (one thread per (sample, set)), ea_measure
+// (one workgroup per (sample, temperature)).
+//
+// Uniforms (key = the sample's): a configuration's site k of colour c uses
+// word k&3 of block k>>2 at (sweep, c, TAG_EA, stream r*m + c); the swap
+// between temperatures k and k+1 of set r uses word k&3 of block k>>2 at
+// (step, 0, TAG_PT, stream r).  The swap test compares u = x 2^-32 with
+// det_exp(x), a fixed sequence of IEEE operations (Cody-Waite reduction,
+// degree-13 Taylor polynomial in fma form) evaluated identically on the host.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cb_geom.hpp"
+#include "common.hpp"
+#include "ising_core.hpp"
+#include "philox.hpp"
+
+namespace cm {
+
+constexpr int EA_NOBS = 7;
+constexpr int EA_TAB = 16;
+constexpr int64_t EA_LDS_MAX = 32768;
+
+__host__ __device__ inline double det_exp(double x)
+{
+    if (x < -745.0) return 0.0;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double inv_ln2 = 1.44269504088896338700e+00;
+    const double n = rint(x * inv_ln2);
+    double r = fma(-n, ln2_hi, x);
+    r = fma(-n, ln2_lo, r);
+    double p = 1.0 / 6227020800.0;
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+struct EaArgs {
+    int8_t* spins;
+    const int8_t* J;
+    const int32_t* L;
+    const int64_t* spin_off;
+    const int64_t* j_off;
+    const uint32_t* keys;   // [S][2]
+    const double* betas;    // [S][m], hot -> cold
+    const uint64_t* thr;    // [S][m][EA_TAB]
+    uint64_t* sweeps;       // [S][2][m]
+    int32_t* energies;      // [S][2][m]
+    int32_t* perm;          // [S][2][m]
+    int32_t* kofc;          // [S][2][m]
+    int32_t* labels;        // [S][2][m]
+    int32_t* trips;         // [S][2]
+    int32_t* label_hist;    // [S][2][m][2]
+    int32_t* swap_acc;      // [S][2][m-1]
+    int32_t* swap_try;      // [S][2][m-1]
+    double* acc;            // [S][n_bins][m][EA_NOBS]
+    int32_t* e_series;      // [S][2][m][n_series] (temperature-indexed) or null
+    int32_t* q_series;      // [S][m][n_series] or null
+    int S, m, d, code, sweeps_per_step, n_bins;
+    int64_t n_series;
+};
+
+__device__ __forceinline__ CbGeo ea_geo(int d, int L)
+{
+    CbGeo g;
+    g.d = d;
+    g.L0 = L;
+    g.L1 = d >= 2 ? L : 1;
+    g.L2 = d >= 3 ? L : 1;
+    g.M = L / 2;
+    g.N = (int64_t)L * g.L1 * g.L2;
+    g.half = g.N / 2;
+    return g;
+}
+
+template <int Z>
+__global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
+{
+    extern __shared__ int8_t lds8[];
+    __shared__ int red[16];
+    const int m = a.m;
+    const int s = blockIdx.x / (2 * m);
+    const int rc = blockIdx.x - s * 2 * m;  // r * m + c
+    const CbGeo g = ea_geo(Z / 2, a.L[s]);
+    const int64_t N = g.N;
+    int8_t* gs = a.spins + a.spin_off[s] + (int64_t)rc * N;
+    const int8_t* J = a.J + a.j_off[s];
+    const bool use_lds = N <= EA_LDS_MAX;
+    int8_t* S = use_lds ? lds8 : gs;
+    if (use_lds)
+        for (int64_t t = threadIdx.x; t < N; t += blockDim.x) lds8[t] = gs[t];
+    const int64_t cfg = (int64_t)s * 2 * m + rc;
+    const int k = a.kofc[cfg];
+    const uint64_t* T = a.thr + ((int64_t)s * m + k) * EA_TAB;
+    uint64_t sweep = a.sweeps[cfg];
+    const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1], sid = (uint32_t)rc;
+    __syncthreads();
+    int de = 0;
+    const int64_t nquad = (g.half + 3) >> 2;
+    for (int sp = 0; sp < a.sweeps_per_step; ++sp) {
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t zc = ctr_z(sweep, (uint32_t)c, TAG_EA);
+            for (int64_t qd = threadIdx.x; qd < nquad; qd += blockDim.x) {
+                const u32x4 rn = philox4x32_10(u32x4{(uint32_t)qd, (uint32_t)sweep, zc, sid}, k0, k1);
+                const uint32_t xs[4] = {rn.x, rn.y, rn.z, rn.w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int64_t kk = 4 * qd + t;
+                    if (kk >= g.half) break;
+                    int64_t nb[Z];
+                    const int64_t i = cb_site<Z>(g, kk, c, nb);
+                    int h = 0;
+#pragma unroll
+                    for (int ax = 0; ax < Z / 2; ++ax) {
+                        h += J[ax * N + i] * S[nb[2 * ax]];
+                        h += J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
+                    }
+                    const int si = S[i];
+                    if (a.code == CODE_METROPOLIS) {
+                        const int aa = si * h;
+                        if (aa <= 0 || (uint64_t)xs[t] < T[aa + Z]) {
+                            S[i] = (int8_t)(-si);
+                            de += 2 * aa;
+                        }
+                    } else {
+                        const int nw = (uint64_t)xs[t] < T[h + Z] ? 1 : -1;
+                        if (nw != si) {
+                            de += 2 * si * h;
+                            S[i] = (int8_t)nw;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        ++sweep;
+    }
+    de = wave_sum(de);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = de;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
+        a.energies[cfg] += t;
+        a.sweeps[cfg] = sweep;
+    }
+    if (use_lds)
+        for (int64_t t = threadIdx.x; t < N; t += blockDim.x) gs[t] = lds8[t];
+}
+
+__global__ void ea_swap(EaArgs a, uint64_t step)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * a.S) return;
+    const int s = t >> 1, r = t & 1;
+    const int m = a.m;
+    const int64_t base = (int64_t)t * m;
+    int32_t* perm = a.perm + base;
+    int32_t* kofc = a.kofc + base;
+    int32_t* lab = a.labels + base;
+    const int32_t* E = a.energies + base;
+    const double* beta = a.betas + (int64_t)s * m;
+    int32_t* sacc = a.swap_acc + (int64_t)t * (m - 1);
+    int32_t* stry = a.swap_try + (int64_t)t * (m - 1);
+    const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
+    const uint32_t zc = ctr_z(step, 0, TAG_PT);
+    uint32_t xs[4] = {0, 0, 0, 0};
+    for (int k = 0; k < m - 1; ++k) {
+        if ((k & 3) == 0) {
+            const u32x4 rn = philox4x32_10(u32x4{(uint32_t)(k >> 2), (uint32_t)step, zc, (uint32_t)r}, k0, k1);
+            xs[0] = rn.x; xs[1] = rn.y; xs[2] = rn.z; xs[3] = rn.w;
+        }
+        const int c1 = perm[k], c2 = perm[k + 1];
+        const double x = (beta[k] - beta[k + 1]) * (double)(E[c1] - E[c2]);
+        stry[k] += 1;
+        if (x >= 0.0 || (double)xs[k & 3] * 0x1p-32 < det_exp(x)) {
+            perm[k] = c2;
+            perm[k + 1] = c1;
+            kofc[c2] = k;
+            kofc[c1] = k + 1;
+            sacc[k] += 1;
+        }
+    }
+    const int hot = perm[0], cold = perm[m - 1];
+    if (lab[hot] == 2) a.trips[t] += 1;
+    lab[hot] = 1;
+    if (lab[cold] == 1) a.trips[t] += 1;
+    lab[cold] = 2;
+    int32_t* hist = a.label_hist + (int64_t)t * m * 2;
+    for (int k = 0; k < m; ++k) {
+        const int l = lab[perm[k]];
+        if (l == 1) hist[2 * k] += 1;
+        else if (l == 2) hist[2 * k + 1] += 1;
+    }
+}
+
+template <int Z>
+__global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
+{
+    __shared__ int ired[2 * 16];
+    __shared__ double dred[6 * 16];
+    const int m = a.m;
+    const int s = blockIdx.x / m, k = blockIdx.x - (blockIdx.x / m) * m;
+    const int L = a.L[s];
+    const CbGeo g = ea_geo(Z / 2, L);
+    const int64_t N = g.N;
+    const int ca = a.perm[((int64_t)s * 2 + 0) * m + k];
+    const int cb = a.perm[((int64_t)s * 2 + 1) * m + k];
+    const int8_t* sa = a.spins + a.spin_off[s] + ((int64_t)0 * m + ca) * N;
+    const int8_t* sb = a.spins + a.spin_off[s] + ((int64_t)1 * m + cb) * N;
+    int q = 0, ql = 0;
+    double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const int qi = sa[i] * sb[i];
+        q += qi;
+        int64_t rest = i;
+        for (int ax = 0; ax < Z / 2; ++ax) {
+            const int x = (int)(rest % L);
+            rest /= L;
+            const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
+            const int64_t j = i + (x + 1 == L ? -(int64_t)(L - 1) * stride : stride);
+            ql += qi * sa[j] * sb[j];
+            double sn, cs;
+            sincospi(2.0 * x / L, &sn, &cs);
+            re[ax] += qi * cs;
+            im[ax] += qi * sn;
+        }
+    }
+    q = wave_sum(q);
+    ql = wave_sum(ql);
+    for (int ax = 0; ax < 3; ++ax) {
+        re[ax] = wave_sum_f64(re[ax]);
+        im[ax] = wave_sum_f64(im[ax]);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ired[2 * w] = q;
+        ired[2 * w + 1] = ql;
+        for (int ax = 0; ax < 3; ++ax) {
+            dred[6 * w + ax] = re[ax];
+            dred[6 * w + 3 + ax] = im[ax];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int nw = (blockDim.x + 63) >> 6;
+    int Q = 0, QL = 0;
+    double RE[3] = {0, 0, 0}, IM[3] = {0, 0, 0};
+    for (int v = 0; v < nw; ++v) {
+        Q += ired[2 * v];
+        QL += ired[2 * v + 1];
+        for (int ax = 0; ax < 3; ++ax) {
+            RE[ax] += dred[6 * v + ax];
+            IM[ax] += dred[6 * v + 3 + ax];
+        }
+    }
+    const int d = Z / 2;
+    double qk2 = 0.0;
+    for (int ax = 0; ax < d; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
+    qk2 /= d;
+    const int32_t Ea = a.energies[((int64_t)s * 2 + 0) * m + ca];
+    const int32_t Eb = a.energies[((int64_t)s * 2 + 1) * m + cb];
+    const int bin = 63 - __clzll((long long)(slot + 1));  // steps of this call, from 0
+    if (bin < a.n_bins) {
+        double* o = a.acc + (((int64_t)s * a.n_bins + bin) * m + k) * EA_NOBS;
+        const double qq = (double)Q / (double)N;
+        o[0] += 0.5 * ((double)Ea + (double)Eb) / (double)N;
+        o[1] += (double)QL / (double)(d * N);
+        o[2] += fabs(qq);
+        o[3] += qq * qq;
+        o[4] += qq * qq * qq * qq;
+        o[5] += qk2;
+        o[6] += 1.0;
+    }
+    if (a.e_series) {
+        a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = Ea;
+        a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = Eb;
+    }
+    if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
+}
+
+// E = -sum_b J_b s_i s_j for every configuration (critmc/kernels/glass.py:33-38).
+template <int Z>
+__global__ __launch_bounds__(256) void ea_energy(EaArgs a)
+{
+    __shared__ int red[16];
+    const int m = a.m;
+    const int s = blockIdx.x / (2 * m);
+    const int rc = blockIdx.x - s * 2 * m;
+    const int L = a.L[s];
+    const CbGeo g = ea_geo(Z / 2, L);
+    const int64_t N = g.N;
+    const int8_t* sp = a.spins + a.spin_off[s] + (int64_t)rc * N;
+    const int8_t* J = a.J + a.j_off[s];
+    int e = 0;
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        int64_t rest = i;
+        for (int ax = 0; ax < Z / 2; ++ax) {
+            const int x = (int)(rest % L);
+            rest /= L;
+            const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
+            const int64_t j = i + (x + 1 == L ? -(int64_t)(L - 1) * stride : stride);
+            e -= J[ax * N + i] * sp[i] * sp[j];
+        }
+    }
+    e = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
+        a.energies[(int64_t)s * 2 * m + rc] = t;
+    }
+}
+
+// Random start: site i of configuration (r, c) of sample s takes bit 31 of
+// word i&3 of block i>>2 at (sweep 0, colour 0, TAG_INIT, stream r*m + c).
+__global__ void ea_init_kernel(EaArgs a)
+{
+    const int cfg = blockIdx.y;
+    const int s = cfg / (2 * a.m), rc = cfg - (cfg / (2 * a.m)) * 2 * a.m;
+    const int L = a.L[s];
+    const int64_t N = (int64_t)L * (a.d >= 2 ? L : 1) * (a.d >= 3 ? L : 1);
+    int8_t* sp = a.spins + a.spin_off[s] + (int64_t)rc * N;
+    const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
+    for (int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; blk * 4 < N;
+         blk += (int64_t)gridDim.x * blockDim.x) {
+        const u32x4 r = draw(k0, k1, (uint32_t)rc, 0, 0, TAG_INIT, (uint32_t)blk);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+        for (int k = 0; k < 4 && blk * 4 + k < N; ++k) sp[blk * 4 + k] = (w[k] >> 31) ? 1 : -1;
+    }
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m)
+{
+    return (int64_t)n_samples * m * EA_TAB * 8;
+}
+
+double cm_det_exp(double x) { return det_exp(x); }
+
+int cm_ea_init(void* spins, const int32_t* L_dev, const int64_t* spin_off_dev, int32_t n_samples,
+               int32_t m, int32_t ndim, const uint32_t* keys_dev, void* stream)
+{
+    CM_REQUIRE(ndim == 2 || ndim == 3, "EA tempering supports square and cubic lattices");
+    CM_REQUIRE(n_samples >= 1 && m >= 2 && (int64_t)n_samples * 2 * m <= 65535,
+               "need 1..65535 configurations in all");
+    EaArgs a{};
+    a.spins = (int8_t*)spins;
+    a.L = L_dev;
+    a.spin_off = spin_off_dev;
+    a.keys = keys_dev;
+    a.S = n_samples;
+    a.m = m;
+    a.d = ndim;
+    hipLaunchKernelGGL(ea_init_kernel, dim3(8, n_samples * 2 * m), dim3(256), 0,
+                       (hipStream_t)stream, a);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_ea_energy(void* spins, const void* J, const int32_t* L_dev, const int64_t* spin_off_dev,
+                 const int64_t* j_off_dev, int32_t n_samples, int32_t m, int32_t ndim,
+                 int32_t* energies_dev, void* stream)
+{
+    CM_REQUIRE(ndim == 2 || ndim == 3, "EA tempering supports square and cubic lattices");
+    CM_REQUIRE(n_samples >= 1 && m >= 2, "need at least one sample and two temperatures");
+    EaArgs a{};
+    a.spins = (int8_t*)spins;
+    a.J = (const int8_t*)J;
+    a.L = L_dev;
+    a.spin_off = spin_off_dev;
+    a.j_off = j_off_dev;
+    a.energies = energies_dev;
+    a.S = n_samples;
+    a.m = m;
+    a.d = ndim;
+    const int blocks = n_samples * 2 * m;
+    if (ndim == 2)
+        hipLaunchKernelGGL(ea_energy<4>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(ea_energy<6>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_ea_pt_run(void* spins, const void* J, const int32_t* L_dev, const int64_t* spin_off_dev,
+                 const int64_t* j_off_dev, int32_t n_samples, int32_t m, int32_t ndim,
+                 const int32_t* L_host, const double* betas_host, const double* betas_dev,
+                 const uint32_t* keys_dev, uint64_t* sweeps_dev, int32_t* energies_dev,
+                 int32_t* perm_dev, int32_t* kofc_dev, int32_t* labels_dev, int32_t* trips_dev,
+                 int32_t* label_hist_dev, int32_t* swap_acc_dev, int32_t* swap_try_dev,
+                 double* acc_dev, int32_t n_bins, int32_t* e_series_dev, int32_t* q_series_dev,
+                 int32_t code, int32_t sweeps_per_step, uint64_t step0, int64_t n_steps,
+                 void* workspace, int64_t workspace_bytes, int32_t block, void* stream)
+{
+    CM_REQUIRE(ndim == 2 || ndim == 3, "EA tempering supports square and cubic lattices");
+    CM_REQUIRE(n_samples >= 1 && m >= 2, "need at least one sample and two temperatures");
+    CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
+               "update code %d has no checkerboard EA kernel", code);
+    CM_REQUIRE(sweeps_per_step >= 1, "sweeps_per_step must be positive");
+    CM_REQUIRE(n_steps >= 0 && n_bins >= 1, "bad step or bin count");
+    int max_L = 2;
+    for (int s = 0; s < n_samples; ++s) {
+        CM_REQUIRE(L_host[s] >= 2 && L_host[s] % 2 == 0, "extents must be even and >= 2 (sample %d has %d)", s, L_host[s]);
+        max_L = std::max(max_L, (int)L_host[s]);
+    }
+    CM_REQUIRE(workspace_bytes >= cm_ea_workspace_bytes(n_samples, m), "workspace too small");
+    if (n_steps == 0) return CM_OK;
+    const int z = 2 * ndim;
+    std::vector<uint64_t> thr((size_t)n_samples * m * EA_TAB, 0);
+    for (int s = 0; s < n_samples; ++s)
+        for (int k = 0; k < m; ++k) {
+            const double b = betas_host[(size_t)s * m + k];
+            CM_REQUIRE(std::isfinite(b), "beta must be finite");
+            uint64_t* T = &thr[((size_t)s * m + k) * EA_TAB];
+            for (int v = -z; v <= z; ++v) {
+                double p;
+                if (code == CODE_METROPOLIS) p = v <= 0 ? 1.0 : std::exp(-b * 2.0 * v);
+                else p = 1.0 / (1.0 + std::exp(-2.0 * b * v));
+                T[v + z] = !(p < 1.0) ? (1ull << 32) : (!(p > 0.0) ? 0 : (uint64_t)std::ceil(p * 4294967296.0));
+            }
+        }
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t* dthr = (uint64_t*)workspace;
+    CM_HIP_CHECK(upload_async(dthr, thr.data(), thr.size() * 8, st));
+    EaArgs a{(int8_t*)spins, (const int8_t*)J, L_dev, spin_off_dev, j_off_dev, keys_dev,
+             betas_dev, dthr, sweeps_dev, energies_dev, perm_dev, kofc_dev, labels_dev, trips_dev,
+             label_hist_dev, swap_acc_dev, swap_try_dev, acc_dev, e_series_dev, q_series_dev,
+             n_samples, m, ndim, code, sweeps_per_step, n_bins, n_steps};
+    if (block <= 0) block = 64;
+    CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
+    const int64_t maxN = (int64_t)max_L * max_L * (ndim == 3 ? max_L : 1);
+    const size_t lds = (size_t)std::min<int64_t>(maxN, EA_LDS_MAX);
+    const int sweep_blocks = n_samples * 2 * m;
+    const int swap_threads = 2 * n_samples;
+    for (int64_t t = 0; t < n_steps; ++t) {
+        const uint64_t step = step0 + (uint64_t)t;
+        if (ndim == 2) {
+            hipLaunchKernelGGL(ea_sweep<4>, dim3(sweep_blocks), dim3(block), lds, st, a);
+            hipLaunchKernelGGL(ea_swap, dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
+            hipLaunchKernelGGL(ea_measure<4>, dim3(n_samples * m), dim3(256), 0, st, a, t);
+        } else {
+            hipLaunchKernelGGL(ea_sweep<6>, dim3(sweep_blocks), dim3(block), lds, st, a);
+            hipLaunchKernelGGL(ea_swap, dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
+            hipLaunchKernelGGL(ea_measure<6>, dim3(n_samples * m), dim3(256), 0, st, a, t);
+        }
+    }
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+}  // extern "C"

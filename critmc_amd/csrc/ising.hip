@@ -665,12 +665,19 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     if (n_meas == 0) return CM_OK;
     const int z = 2 * g.d;
     std::vector<uint32_t> tabs((size_t)n_rep * TAB_WORDS);
+    bool uses_met = false, uses_hb = false;
+    for (int r = 0; r < n_rows; ++r) {
+        uses_met |= sc.code[r] == CODE_METROPOLIS;
+        uses_hb |= sc.code[r] == CODE_HEATBATH;
+    }
+    // the class loop is sized for the tables this schedule actually reads
     int nc_max = 1;
     for (int r = 0; r < n_rep; ++r) {
         CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
         int a, b;
         build_tables(betas[r], h, z, &tabs[(size_t)r * TAB_WORDS], &a, &b);
-        nc_max = std::max(nc_max, std::max(a, b));
+        if (uses_met) nc_max = std::max(nc_max, a);
+        if (uses_hb) nc_max = std::max(nc_max, b);
     }
     const int nc = pick_nc(nc_max);
     hipStream_t st = (hipStream_t)stream;

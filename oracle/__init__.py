@@ -263,3 +263,75 @@ class PottsReplay:
         if rc:
             raise ValueError(f"orc_potts_run failed ({rc})")
         return {"E": e_out, "counts": c_out, "accepted": acc}
+
+
+_EXTRA_DECLS.update({
+    "orc_det_exp": ([ctypes.c_double], ctypes.c_double),
+    "orc_ea_energy": ([_i8p, _i8p, ctypes.c_int, ctypes.c_int], ctypes.c_int32),
+    "orc_ea_init": ([_i8p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32], None),
+    "orc_ea_pt_run": ([_i8p, _i8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int,
+                       ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _u64p, _i32p, _i32p, _i32p,
+                       _i32p, _i32p, _i32p, _i32p, _i32p, _f64p, ctypes.c_int, _i32p, _i32p,
+                       ctypes.c_uint64, ctypes.c_int64], ctypes.c_int),
+})
+
+
+def det_exp(x: float) -> float:
+    return float(lib().orc_det_exp(float(x)))
+
+
+class EaReplay:
+    """One +-J EA sample under parallel tempering, replayed on the CPU.
+
+    ``spins``: [2, m, N] int8 start; ``J``: [d*N] +-1 forward-bond couplings;
+    ``betas``: hot -> cold.  State arrays follow the device layout.
+    """
+
+    def __init__(self, spins, J, ndim: int, L: int, betas, key, code: int = 2,
+                 sweeps_per_step: int = 1):
+        self.spins = np.ascontiguousarray(spins, dtype=np.int8).copy()
+        self.J = np.ascontiguousarray(J, dtype=np.int8)
+        self.d, self.L = int(ndim), int(L)
+        self.betas = np.ascontiguousarray(betas, dtype=np.float64)
+        self.m = len(self.betas)
+        self.key = (int(key[0]), int(key[1]))
+        self.code, self.sps = int(code), int(sweeps_per_step)
+        m = self.m
+        self.N = self.L ** self.d
+        self.sweeps = np.zeros(2 * m, np.uint64)
+        self.energies = np.array([lib().orc_ea_energy(_ptr(self.spins[r, c], _i8p), _ptr(self.J, _i8p),
+                                                      self.d, self.L) for r in range(2) for c in range(m)],
+                                 dtype=np.int32)
+        self.perm = np.tile(np.arange(m, dtype=np.int32), 2)
+        self.kofc = self.perm.copy()
+        self.labels = np.zeros(2 * m, np.int32)
+        self.step = 0
+
+    def run(self, n_steps: int, n_bins: int, record: bool = True) -> dict[str, np.ndarray]:
+        m = self.m
+        trips = np.zeros(2, np.int32)
+        hist = np.zeros(2 * m * 2, np.int32)
+        sacc = np.zeros(2 * (m - 1), np.int32)
+        stry = np.zeros(2 * (m - 1), np.int32)
+        acc = np.zeros(n_bins * m * 7, np.float64)
+        es = np.zeros(2 * m * n_steps, np.int32)
+        qs = np.zeros(m * n_steps, np.int32)
+        rc = lib().orc_ea_pt_run(
+            _ptr(self.spins, _i8p), _ptr(self.J, _i8p), self.d, self.L, m, _ptr(self.betas, _f64p),
+            self.code, self.sps, self.key[0], self.key[1], _ptr(self.sweeps, _u64p),
+            _ptr(self.energies, _i32p), _ptr(self.perm, _i32p), _ptr(self.kofc, _i32p),
+            _ptr(self.labels, _i32p), _ptr(trips, _i32p), _ptr(hist, _i32p), _ptr(sacc, _i32p),
+            _ptr(stry, _i32p), _ptr(acc, _f64p), int(n_bins), _ptr(es, _i32p), _ptr(qs, _i32p),
+            int(self.step), int(n_steps))
+        if rc:
+            raise ValueError(f"orc_ea_pt_run failed ({rc})")
+        self.step += n_steps
+        return {"acc": acc.reshape(n_bins, m, 7), "trips": trips, "label_hist": hist.reshape(2, m, 2),
+                "swap_acc": sacc.reshape(2, m - 1), "swap_try": stry.reshape(2, m - 1),
+                "E": es.reshape(2, m, n_steps), "q": qs.reshape(m, n_steps)}
+
+
+def ea_init(m: int, N: int, key) -> np.ndarray:
+    s = np.zeros(2 * m * N, np.int8)
+    lib().orc_ea_init(_ptr(s, _i8p), int(m), int(N), int(key[0]), int(key[1]))
+    return s.reshape(2, m, N)
