@@ -7,7 +7,7 @@ kernels through the C-ABI library ``libcritmc_hip.so`` via ctypes.
 """
 
 from .lattice import Lattice, chain, cubic, hypercubic, make_lattice, square, triangular
-from .models import DeviceState, Ising, Model, Potts, build_model, derive
+from .models import ON, DeviceState, Heisenberg, Ising, Model, Potts, XY, build_model, derive
 from .rng import chain_key, make_rng, seed_sequence, spawn_rngs
 from .sim import Simulation, TimeSeries, parse_updates
 
@@ -15,7 +15,7 @@ __version__ = "0.1.0"
 
 __all__ = [
     "Lattice", "chain", "cubic", "hypercubic", "make_lattice", "square", "triangular",
-    "DeviceState", "Ising", "Model", "Potts", "build_model", "derive",
+    "DeviceState", "Ising", "Model", "Potts", "ON", "XY", "Heisenberg", "build_model", "derive",
     "chain_key", "make_rng", "seed_sequence", "spawn_rngs",
     "Simulation", "TimeSeries", "parse_updates",
 ]

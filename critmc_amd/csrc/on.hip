@@ -1,0 +1,423 @@
+// Classical O(n) vector model (XY n = 2, Heisenberg n = 3): FP32 spins,
+// checkerboard Metropolis and over-relaxation sweeps, FP64 renormalisation
+// and measurements.
+//
+// Reference (critmc/kernels/on.py):
+//   Metropolis   :112-137  S' = (S + step g)/|S + step g|, g ~ N(0,1)^n;
+//                          dE = -(S' - S).h; accept iff dE <= 0 or u < exp(-beta dE)
+//   over-relax   :140-183  S' = 2 (S.h)/|h|^2 h - S, unchanged when |h|^2 = 0
+//   measurement  :35-87, 404-413  normalise, then E = -sum S_i.S_j, M, and per
+//                axis c_a = sum (S_i^0 S_j^0 + S_i^1 S_j^1), s_a = sum (S_i^0 S_j^1 - S_i^1 S_j^0)
+//
+// Layout: spins [R][N][n] FP32, x-fastest sites.  A half sweep is one launch
+// over all replicas (grid = colour-site chunks x replicas); measurements are a
+// normalise launch, a partial-sum launch and an in-order finalise launch, so
+// every FP64 result is reproducible run to run.
+//
+// Uniforms (Metropolis, tag TAG_ON): colour site k of a replica uses Philox
+// block k (n = 2: words 0,1 -> Box-Muller pair, word 2 -> acceptance) or blocks
+// 2k, 2k+1 (n = 3: words 0..3 -> two Box-Muller pairs, third Gaussian from the
+// first of the second pair; block 2k+1 word 0 -> acceptance).  u1 = (x+1)/2^32
+// in (0, 1], u2 = x/2^32 in [0, 1).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cb_geom.hpp"
+#include "common.hpp"
+#include "ising_core.hpp"
+#include "philox.hpp"
+
+namespace cm {
+
+constexpr int ON_CHUNK = 4096;  // sites per measurement partial
+
+template <int NC>
+struct Vec {
+    float v[NC];
+};
+
+template <int NC>
+__device__ __forceinline__ Vec<NC> load_vec(const float* __restrict__ p, int64_t i)
+{
+    Vec<NC> r;
+    if constexpr (NC == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p + 2 * i);
+        r.v[0] = t.x;
+        r.v[1] = t.y;
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) r.v[c] = p[NC * i + c];
+    }
+    return r;
+}
+
+template <int NC>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, int64_t i, const Vec<NC>& r)
+{
+    if constexpr (NC == 2) {
+        *reinterpret_cast<float2*>(p + 2 * i) = make_float2(r.v[0], r.v[1]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) p[NC * i + c] = r.v[c];
+    }
+}
+
+__device__ __forceinline__ void box_muller(uint32_t xa, uint32_t xb, float& g0, float& g1)
+{
+    const float u1 = ((float)xa + 1.0f) * 0x1p-32f;  // (0, 1]
+    const float u2 = (float)xb * 0x1p-32f;           // [0, 1)
+    const float r = sqrtf(-2.0f * logf(u1));
+    float sn, cs;
+    sincospif(2.0f * u2, &sn, &cs);
+    g0 = r * cs;
+    g1 = r * sn;
+}
+
+struct OnArgs {
+    float* spins;
+    const float* betas;     // [R]
+    const float* steps;     // [R]
+    const uint32_t* keys;   // [R][2]
+    const uint32_t* sids;   // [R]
+    const uint64_t* sweeps; // [R]
+    int32_t* acc;           // [R][n_meas] (accumulated)
+    CbGeo g;
+    int R;
+    int64_t n_meas, meas;
+    uint64_t sweep_add;
+    int colour;
+};
+
+template <int NC, int Z, int CODE>
+__global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
+{
+    const int rep = blockIdx.y;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int accepted = 0;
+    if (k < a.g.half) {
+        float* S = a.spins + (int64_t)rep * a.g.N * NC;
+        int64_t nb[Z];
+        const int64_t i = cb_site<Z>(a.g, k, a.colour, nb);
+        float h[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) h[c] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < Z; ++j) {
+            const Vec<NC> t = load_vec<NC>(S, nb[j]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) h[c] += t.v[c];
+        }
+        Vec<NC> s = load_vec<NC>(S, i);
+        if (CODE == CODE_OVERRELAX) {
+            float hh = 0.0f, sh = 0.0f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                hh += h[c] * h[c];
+                sh += s.v[c] * h[c];
+            }
+            if (hh != 0.0f) {
+                const float f = 2.0f * sh / hh;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s.v[c] = f * h[c] - s.v[c];
+                store_vec<NC>(S, i, s);
+            }
+        } else {
+            const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+            const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+            const uint32_t zc = ctr_z(sweep, (uint32_t)a.colour, TAG_ON);
+            float g[4];
+            uint32_t xacc;
+            if constexpr (NC == 2) {
+                const u32x4 r = philox4x32_10(u32x4{(uint32_t)k, (uint32_t)sweep, zc, sid}, k0, k1);
+                box_muller(r.x, r.y, g[0], g[1]);
+                xacc = r.z;
+            } else {
+                const u32x4 r = philox4x32_10(u32x4{(uint32_t)(2 * k), (uint32_t)sweep, zc, sid}, k0, k1);
+                const u32x4 r2 = philox4x32_10(u32x4{(uint32_t)(2 * k + 1), (uint32_t)sweep, zc, sid}, k0, k1);
+                box_muller(r.x, r.y, g[0], g[1]);
+                box_muller(r.z, r.w, g[2], g[3]);
+                xacc = r2.x;
+            }
+            const float step = a.steps[rep];
+            float tr[NC];
+            float nrm = 0.0f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                tr[c] = s.v[c] + step * g[c];
+                nrm += tr[c] * tr[c];
+            }
+            nrm = sqrtf(nrm);
+            float de = 0.0f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                tr[c] /= nrm;
+                de -= (tr[c] - s.v[c]) * h[c];
+            }
+            if (de <= 0.0f || (float)xacc * 0x1p-32f < expf(-a.betas[rep] * de)) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s.v[c] = tr[c];
+                store_vec<NC>(S, i, s);
+                accepted = 1;
+            }
+        }
+    }
+    if (CODE != CODE_OVERRELAX) {
+        const int t = wave_sum(accepted);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], t);
+    }
+}
+
+// Renormalise every spin in FP64 (critmc/kernels/on.py:78-87).
+template <int NC>
+__global__ void on_normalize(float* __restrict__ spins, int64_t total_sites)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total_sites;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        Vec<NC> s = load_vec<NC>(spins, t);
+        double n2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
+        const double nrm = sqrt(n2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+        store_vec<NC>(spins, t, s);
+    }
+}
+
+// Partial FP64 sums of E, M[n], c_a, s_a over chunks of ON_CHUNK sites.
+template <int NC, int D>
+__global__ __launch_bounds__(256) void on_measure_partial(const float* __restrict__ spins, CbGeo g,
+                                                          double* __restrict__ part, int nchunk)
+{
+    constexpr int K = 1 + NC + 2 * D;
+    __shared__ double red[K * 4];
+    const int rep = blockIdx.y, chunk = blockIdx.x;
+    const float* S = spins + (int64_t)rep * g.N * NC;
+    double v[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = 0.0;
+    const int64_t i0 = (int64_t)chunk * ON_CHUNK;
+    const int64_t i1 = min(g.N, i0 + ON_CHUNK);
+    const int64_t st[3] = {1, g.L0, (int64_t)g.L0 * g.L1};
+    const int ext[3] = {g.L0, g.L1, g.L2};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const Vec<NC> s = load_vec<NC>(S, i);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c];
+        int64_t rest = i;
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+            const int x = (int)(rest % ext[ax]);
+            rest /= ext[ax];
+            const int64_t j = i + (x + 1 == ext[ax] ? -(int64_t)(ext[ax] - 1) * st[ax] : st[ax]);
+            const Vec<NC> t = load_vec<NC>(S, j);
+            double dot = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dot += (double)s.v[c] * (double)t.v[c];
+            v[0] -= dot;
+            v[1 + NC + ax] += (double)s.v[0] * t.v[0] + (double)s.v[1] * t.v[1];
+            v[1 + NC + D + ax] += (double)s.v[0] * t.v[1] - (double)s.v[1] * t.v[0];
+        }
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const double t = wave_sum_f64(v[q]);
+        if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double t = 0.0;
+        for (int u = 0; u < (int)(blockDim.x >> 6); ++u) t += red[threadIdx.x * 4 + u];
+        part[((int64_t)rep * nchunk + chunk) * K + threadIdx.x] = t;
+    }
+}
+
+template <int NC, int D>
+__global__ void on_measure_final(const double* __restrict__ part, int nchunk, int R,
+                                 int64_t n_meas, int64_t meas, double* __restrict__ e_out,
+                                 double* __restrict__ m_out, double* __restrict__ sc_out,
+                                 double* __restrict__ ss_out)
+{
+    constexpr int K = 1 + NC + 2 * D;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= R * K) return;
+    const int rep = t / K, q = t - (t / K) * K;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)rep * nchunk + c) * K + q];
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    if (q == 0) e_out[o] = s;
+    else if (q < 1 + NC) m_out[o * NC + (q - 1)] = s;
+    else if (q < 1 + NC + D) sc_out[o * D + (q - 1 - NC)] = s;
+    else ss_out[o * D + (q - 1 - NC - D)] = s;
+}
+
+__global__ void fill_i32_on(int32_t* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ void on_advance_sweeps(uint64_t* __restrict__ sweeps, int R, uint64_t n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < R) sweeps[t] += n;
+}
+
+__global__ void on_init_kernel(float* __restrict__ spins, int64_t N, int NC,
+                               const uint32_t* __restrict__ keys,
+                               const uint32_t* __restrict__ sids)
+{
+    const int rep = blockIdx.y;
+    const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const u32x4 r = draw(k0, k1, sid, 0, 0, TAG_INIT, (uint32_t)i);
+        float* p = spins + ((int64_t)rep * N + i) * NC;
+        if (NC == 2) {
+            // angle uniform in [0, 2 pi): theta = 2 pi x / 2^32
+            float sn, cs;
+            sincospif(2.0f * ((float)r.x * 0x1p-32f), &sn, &cs);
+            p[0] = cs;
+            p[1] = sn;
+        } else {
+            // isotropic: z uniform in [-1, 1), azimuth uniform
+            const float z = 2.0f * ((float)r.x * 0x1p-32f) - 1.0f;
+            const float rho = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            float sn, cs;
+            sincospif(2.0f * ((float)r.y * 0x1p-32f), &sn, &cs);
+            p[0] = rho * cs;
+            p[1] = rho * sn;
+            p[2] = z;
+        }
+    }
+}
+
+template <int NC, int Z>
+static void launch_half(int code, const OnArgs& a, hipStream_t st)
+{
+    const dim3 grid((unsigned)((a.g.half + 255) / 256), a.R);
+    if (code == CODE_OVERRELAX)
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS>), grid, dim3(256), 0, st, a);
+}
+
+template <int NC>
+static void launch_half_z(int code, const OnArgs& a, hipStream_t st)
+{
+    if (a.g.d == 1) launch_half<NC, 2>(code, a, st);
+    else if (a.g.d == 2) launch_half<NC, 4>(code, a, st);
+    else launch_half<NC, 6>(code, a, st);
+}
+
+template <int NC>
+static void launch_measure(const float* spins, const CbGeo& g, int R, double* part, int nchunk,
+                           int64_t n_meas, int64_t meas, double* e_out, double* m_out,
+                           double* sc_out, double* ss_out, hipStream_t st)
+{
+    const dim3 grid(nchunk, R);
+    if (g.d == 1) {
+        hipLaunchKernelGGL((on_measure_partial<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nchunk);
+        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+    } else if (g.d == 2) {
+        hipLaunchKernelGGL((on_measure_partial<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nchunk);
+        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+    } else {
+        hipLaunchKernelGGL((on_measure_partial<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nchunk);
+        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+    }
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t n)
+{
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= dims[a];
+    const int64_t nchunk = (N + ON_CHUNK - 1) / ON_CHUNK;
+    return (int64_t)n_rep * nchunk * (1 + n + 2 * ndim) * 8 + (int64_t)n_rep * 8;
+}
+
+int cm_on_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t n,
+               const uint32_t* keys, const uint32_t* stream_ids, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(n == 2 || n == 3, "the GPU O(n) kernels support n = 2 (XY) and n = 3 (Heisenberg)");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    const int blocks = (int)std::min<int64_t>((g.N + 255) / 256, 1024);
+    hipLaunchKernelGGL(on_init_kernel, dim3(blocks, n_rep), dim3(256), 0, (hipStream_t)stream,
+                       (float*)spins, g.N, n, keys, stream_ids);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t n,
+              const float* betas_dev, const float* steps_dev, const uint32_t* keys,
+              const uint32_t* stream_ids, uint64_t* sweeps, const int64_t* schedule,
+              int32_t n_rows, int64_t n_meas, int32_t* acc_out, double* e_out, double* m_out,
+              double* sc_out, double* ss_out, void* workspace, int64_t workspace_bytes,
+              void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(n == 2 || n == 3, "the GPU O(n) kernels support n = 2 (XY) and n = 3 (Heisenberg)");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    CM_REQUIRE(g.half <= 0xFFFFFFFFll / 2, "lattice too large");
+    CM_REQUIRE(workspace_bytes >= cm_on_workspace_bytes(ndim, dims, n_rep, n), "workspace too small");
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_OVERRELAX,
+                   "update code %lld has no checkerboard O(n) kernel", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
+    }
+    if (n_meas == 0) return CM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int nchunk = (int)((g.N + ON_CHUNK - 1) / ON_CHUNK);
+    double* part = (double*)workspace;
+    const int64_t total = (int64_t)n_rep * n_meas;
+    hipLaunchKernelGGL(fill_i32_on, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)),
+                       dim3(256), 0, st, acc_out, total);
+    OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, g, n_rep,
+             n_meas, 0, 0, 0};
+    uint64_t sw = 0;
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        a.meas = meas;
+        for (int r = 0; r < n_rows; ++r) {
+            const int code = (int)schedule[2 * r];
+            for (int64_t rp = 0; rp < schedule[2 * r + 1]; ++rp) {
+                a.sweep_add = sw;
+                for (int c = 0; c < 2; ++c) {
+                    a.colour = c;
+                    if (n == 2) launch_half_z<2>(code, a, st);
+                    else launch_half_z<3>(code, a, st);
+                }
+                ++sw;
+            }
+        }
+        const int64_t sites = (int64_t)n_rep * g.N;
+        const int nb = (int)std::min<int64_t>((sites + 255) / 256, 8192);
+        if (n == 2) {
+            hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, (float*)spins, sites);
+            launch_measure<2>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, st);
+        } else {
+            hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, (float*)spins, sites);
+            launch_measure<3>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, st);
+        }
+    }
+    hipLaunchKernelGGL(on_advance_sweeps, dim3((n_rep + 255) / 256), dim3(256), 0, st, sweeps,
+                       n_rep, sw);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+}  // extern "C"

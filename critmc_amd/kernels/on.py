@@ -1,0 +1,64 @@
+"""Checkerboard O(n) kernels (n = 2 XY, n = 3 Heisenberg): host side of ``cm_on_*``.
+
+Replaces critmc/kernels/on.py's ``run`` (:348-418) for Metropolis (:112-137)
+and over-relaxation (:140-183) sweeps; spins are FP32 ``[R, N, n]`` on the
+device, measurements (after an FP64 renormalisation) are FP64.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .._lib import check, lib
+from ..device import device, host_ptr, ptr, stream
+from .chains import Chains
+from .common import METROPOLIS, OVERRELAX
+
+
+def _dims(shape) -> np.ndarray:
+    return np.ascontiguousarray(shape, dtype=np.int32)
+
+
+class Workspace:
+    def __init__(self, shape, R: int, n: int, dev: torch.device | None = None):
+        d = _dims(shape)
+        self.nbytes = int(lib().cm_on_workspace_bytes(len(d), host_ptr(d, ctypes.c_int32), R, n))
+        self.buf = torch.empty(max(self.nbytes, 256), dtype=torch.uint8, device=dev or device())
+
+
+def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
+    """Random unit vectors: XY angle uniform in [0, 2 pi), Heisenberg isotropic."""
+    d = _dims(shape)
+    check(lib().cm_on_init(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), chains.R, int(n),
+                           ptr(chains.keys_d), ptr(chains.sids_d), stream()), "cm_on_init")
+
+
+def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedule, n_meas: int,
+        *, ws: Workspace | None = None):
+    """Returns device tensors (accepted int32 [R, n_meas], E f64 [R, n_meas],
+    M f64 [R, n_meas, n], stiff_c / stiff_s f64 [R, n_meas, d])."""
+    R = chains.R
+    sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+    for code in sched[:, 0]:
+        if code not in (METROPOLIS, OVERRELAX):
+            raise ValueError(f"update code {int(code)} has no checkerboard O(n) kernel")
+    dv = spins.device
+    d = _dims(shape)
+    b = torch.as_tensor(np.broadcast_to(np.asarray(betas, np.float32), (R,)).copy(), device=dv)
+    st = torch.as_tensor(np.broadcast_to(np.asarray(steps, np.float32), (R,)).copy(), device=dv)
+    acc = torch.empty((R, n_meas), dtype=torch.int32, device=dv)
+    e = torch.empty((R, n_meas), dtype=torch.float64, device=dv)
+    m = torch.empty((R, n_meas, n), dtype=torch.float64, device=dv)
+    sc = torch.empty((R, n_meas, len(d)), dtype=torch.float64, device=dv)
+    ss = torch.empty((R, n_meas, len(d)), dtype=torch.float64, device=dv)
+    ws = ws or Workspace(shape, R, n, dv)
+    rc = lib().cm_on_run(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, int(n), ptr(b),
+                         ptr(st), ptr(chains.keys_d), ptr(chains.sids_d), ptr(chains.sweeps_d),
+                         host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(acc),
+                         ptr(e), ptr(m), ptr(sc), ptr(ss), ptr(ws.buf), ws.nbytes, stream())
+    check(rc, "cm_on_run")
+    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    return acc, e, m, sc, ss

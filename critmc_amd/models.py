@@ -24,12 +24,14 @@ import torch
 
 from .device import device
 from .kernels import ising as k_ising
+from .kernels import on as k_on
 from .kernels import potts as k_potts
 from .kernels.chains import Chains
 from .kernels.common import GPU_UPDATES, N_AUX, UPDATE_CODES
 from .lattice import Lattice, make_lattice, require_checkerboard
 
-__all__ = ["Model", "Ising", "Potts", "build_model", "derive", "DeviceState"]
+__all__ = ["Model", "Ising", "Potts", "ON", "XY", "Heisenberg", "build_model", "derive",
+           "DeviceState"]
 
 
 @dataclass
@@ -233,6 +235,71 @@ class Potts(Model):
         return {**super().spec(), "q": self.q}
 
 
+class ON(Model):
+    """Classical O(n) model with unit spins (critmc/models.py:225-299); GPU path n = 2, 3."""
+
+    kind = "on"
+    updates = GPU_UPDATES["on"]
+
+    def __init__(self, lattice: Lattice, n: int):
+        super().__init__(lattice)
+        if n < 2:
+            raise ValueError(f"n must be at least 2, got {n} (n = 1 is the Ising model)")
+        if n > 3:
+            raise ValueError(f"the GPU O(n) kernels support n = 2 and 3, got {n}")
+        self.n = int(n)
+
+    def initial_state(self, start, rng):
+        if start == "random":
+            s = rng.standard_normal((self.n_sites, self.n))
+            s /= np.linalg.norm(s, axis=1, keepdims=True)
+            return s
+        if start in ("ordered", "up"):
+            s = np.zeros((self.n_sites, self.n))
+            s[:, 0] = 1.0
+            return s
+        raise ValueError(f"unknown start {start!r}")
+
+    def energy(self, state):
+        b = self.lattice.bonds
+        s = np.asarray(state, dtype=np.float64)
+        return float(-np.sum(s[b[:, 0]] * s[b[:, 1]]))
+
+    def begin(self, state):
+        return {"step": 1.0}
+
+    def to_device(self, state, running):
+        s = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float32)).to(device())
+        return DeviceState(s.reshape(1, self.n_sites, self.n))
+
+    def from_device(self, ds):
+        return ds.spins.reshape(self.n_sites, self.n).cpu().numpy().astype(np.float64)
+
+    def workspace(self):
+        return k_on.Workspace(self.lattice.shape, 1, self.n)
+
+    def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
+        acc, e, m, sc, ss = k_on.run(ds.spins, self.lattice.shape, self.n, [float(beta)],
+                                     [float(running["step"])], chains, schedule, n_meas, ws=ws)
+        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
+        aux[:, 0] = acc[0].cpu().numpy()
+        return {"E": e[0].cpu().numpy(), "M": m[0].cpu().numpy(), "stiff_c": sc[0].cpu().numpy(),
+                "stiff_s": ss[0].cpu().numpy(), "aux": aux}
+
+    def spec(self):
+        return {**super().spec(), "n": self.n}
+
+
+def XY(lattice: Lattice) -> ON:
+    """Plane rotator, O(2)."""
+    return ON(lattice, 2)
+
+
+def Heisenberg(lattice: Lattice) -> ON:
+    """Classical Heisenberg model, O(3)."""
+    return ON(lattice, 3)
+
+
 # kernel form used by single-chain models: 0 auto, 1 plain (one launch per half sweep), 2 batched
 _FORM = [0]
 
@@ -251,6 +318,8 @@ def build_model(spec: dict[str, Any]) -> Model:
         return Ising(lattice, h=float(spec.get("h", 0.0)))
     if kind == "potts":
         return Potts(lattice, int(spec["q"]))
+    if kind == "on":
+        return ON(lattice, int(spec["n"]))
     raise ValueError(f"unknown model kind {kind!r}")
 
 

@@ -181,6 +181,30 @@ int cm_ea_pt_run(void *spins_dev, const void *J_dev, const int32_t *L_dev,
                  int64_t n_steps, void *workspace_dev, int64_t workspace_bytes, int32_t block,
                  void *stream);
 
+/* ---- O(n) vector model, n = 2 (XY) and 3 (critmc/kernels/on.py) ---------- */
+
+/* Bytes of caller-owned device scratch cm_on_run needs (FP64 partial sums). */
+int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n);
+
+/* Random FP32 unit vectors [n_rep][N][n] (XY: angle uniform in [0, 2 pi);
+ * n = 3: isotropic), Philox tag INIT.  Replaces ON.initial_state
+ * (critmc/models.py:248-257) for device-side batches. */
+int cm_on_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
+               const uint32_t *keys_dev, const uint32_t *stream_ids_dev, void *stream);
+
+/* Apply `schedule` (codes 0 = metropolis, 5 = overrelax) n_meas times to
+ * n_rep FP32 O(n) chains; after every application renormalise in FP64 and
+ * record FP64 E [n_rep][n_meas], M [n_rep][n_meas][n], stiffness sums
+ * c_a / s_a [n_rep][n_meas][ndim] and int32 accepted Metropolis moves.
+ * betas_dev / steps_dev: FP32 [n_rep] (inverse temperature, Metropolis step).
+ * Replaces critmc/kernels/on.py:348-418 (run) for checkerboard sweeps. */
+int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
+              const float *betas_dev, const float *steps_dev, const uint32_t *keys_dev,
+              const uint32_t *stream_ids_dev, uint64_t *sweeps_dev, const int64_t *schedule,
+              int32_t n_rows, int64_t n_meas, int32_t *acc_out_dev, double *e_out_dev,
+              double *m_out_dev, double *sc_out_dev, double *ss_out_dev, void *workspace_dev,
+              int64_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

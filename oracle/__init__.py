@@ -335,3 +335,51 @@ def ea_init(m: int, N: int, key) -> np.ndarray:
     s = np.zeros(2 * m * N, np.int8)
     lib().orc_ea_init(_ptr(s, _i8p), int(m), int(N), int(key[0]), int(key[1]))
     return s.reshape(2, m, N)
+
+
+_EXTRA_DECLS.update({
+    "orc_on_half_sweep": ([_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32,
+                           ctypes.c_uint32, ctypes.c_uint64], ctypes.c_int64),
+    "orc_on_measure": ([_f64p, ctypes.c_int, _i32p, ctypes.c_int, _f64p, _f64p, _f64p, _f64p],
+                       ctypes.c_int),
+    "orc_on_run": ([_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
+                    ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _i64p], ctypes.c_int),
+})
+
+
+class OnReplay:
+    """FP64 replay of one O(n) chain (``s``: [N, n])."""
+
+    def __init__(self, s, dims, beta: float, step: float, key, stream: int = 0, sweep: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.float64).copy()
+        self.n = self.s.shape[1]
+        self.dims = np.ascontiguousarray(dims, dtype=np.int32)
+        self.beta, self.step = float(beta), float(step)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.sweep = np.array([sweep], np.uint64)
+
+    def half_sweep(self, code: int, colour: int) -> int:
+        return int(lib().orc_on_half_sweep(
+            _ptr(self.s, _f64p), len(self.dims), _ptr(self.dims, _i32p), self.n, int(code),
+            int(colour), self.beta, self.step, self.key[0], self.key[1], self.stream,
+            int(self.sweep[0])))
+
+    def run(self, schedule, n_meas: int) -> dict[str, np.ndarray]:
+        sched = np.ascontiguousarray(schedule, np.int64).reshape(-1, 2)
+        d = len(self.dims)
+        e = np.zeros(n_meas)
+        m = np.zeros((n_meas, self.n))
+        sc = np.zeros((n_meas, d))
+        ss = np.zeros((n_meas, d))
+        acc = np.zeros(n_meas, np.int64)
+        rc = lib().orc_on_run(_ptr(self.s, _f64p), d, _ptr(self.dims, _i32p), self.n, self.beta,
+                              self.step, self.key[0], self.key[1], self.stream,
+                              _ptr(self.sweep, _u64p), _ptr(sched, _i64p), sched.shape[0],
+                              int(n_meas), _ptr(e, _f64p), _ptr(m, _f64p), _ptr(sc, _f64p),
+                              _ptr(ss, _f64p), _ptr(acc, _i64p))
+        if rc:
+            raise ValueError(f"orc_on_run failed ({rc})")
+        return {"E": e, "M": m, "stiff_c": sc, "stiff_s": ss, "accepted": acc}

@@ -1,0 +1,145 @@
+/*
+ * ORACLE (test infrastructure only) -- FP64 replay of the checkerboard O(n)
+ * schedule (n = 2, 3) with the GPU path's uniforms.  The GPU runs in FP32, so
+ * agreement is to a stated tolerance, not bit-exact.
+ *
+ * Restated from the reference (critmc/kernels/on.py):
+ *   Metropolis :112-137, over-relaxation :140-183, normalize :78-87,
+ *   energy / magnetization / stiffness_sums :35-76, run loop :348-418.
+ * Uniforms: colour site k uses Philox block k (n = 2) or blocks 2k, 2k+1
+ * (n = 3) at (sweep, colour, TAG_ON, stream); Box-Muller with
+ * u1 = (x+1)/2^32, u2 = x/2^32: g0 = sqrt(-2 ln u1) cos(2 pi u2),
+ * g1 = ... sin(2 pi u2); the acceptance uniform is word 2 of block k (n = 2) or
+ * word 0 of block 2k+1 (n = 3).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "cb.h"
+#include "philox.h"
+
+static void orc_bm(uint32_t xa, uint32_t xb, double *g0, double *g1)
+{
+    double u1 = ((double)xa + 1.0) * 0x1p-32, u2 = (double)xb * 0x1p-32;
+    double r = sqrt(-2.0 * log(u1));
+    *g0 = r * cos(2.0 * M_PI * u2);
+    *g1 = r * sin(2.0 * M_PI * u2);
+}
+
+/* One half sweep; returns accepted moves (Metropolis) or 0. */
+int64_t orc_on_half_sweep(double *s, int ndim, const int32_t *dims, int n, int code, int colour,
+                          double beta, double step, uint32_t k0, uint32_t k1, uint32_t sid,
+                          uint64_t sweep)
+{
+    orc_cb g;
+    if (orc_cb_init(&g, ndim, dims)) return -1;
+    const int z = 2 * ndim;
+    int64_t nb[6], acc = 0;
+    for (int64_t k = 0; k < g.half; ++k) {
+        int64_t i = orc_cb_site(&g, k, colour, nb);
+        double h[3] = {0, 0, 0};
+        for (int j = 0; j < z; ++j)
+            for (int c = 0; c < n; ++c) h[c] += s[nb[j] * n + c];
+        double *si = s + i * n;
+        if (code == 5) {
+            double hh = 0, sh = 0;
+            for (int c = 0; c < n; ++c) {
+                hh += h[c] * h[c];
+                sh += si[c] * h[c];
+            }
+            if (hh == 0.0) continue;
+            double f = 2.0 * sh / hh;
+            for (int c = 0; c < n; ++c) si[c] = f * h[c] - si[c];
+        } else {
+            uint32_t b1[4], b2[4];
+            double gg[4];
+            uint32_t xacc;
+            if (n == 2) {
+                orc_draw_block(k0, k1, sid, sweep, colour, ORC_TAG_ON, (uint32_t)k, b1);
+                orc_bm(b1[0], b1[1], &gg[0], &gg[1]);
+                xacc = b1[2];
+            } else {
+                orc_draw_block(k0, k1, sid, sweep, colour, ORC_TAG_ON, (uint32_t)(2 * k), b1);
+                orc_draw_block(k0, k1, sid, sweep, colour, ORC_TAG_ON, (uint32_t)(2 * k + 1), b2);
+                orc_bm(b1[0], b1[1], &gg[0], &gg[1]);
+                orc_bm(b1[2], b1[3], &gg[2], &gg[3]);
+                xacc = b2[0];
+            }
+            double tr[3], nrm = 0.0, de = 0.0;
+            for (int c = 0; c < n; ++c) {
+                tr[c] = si[c] + step * gg[c];
+                nrm += tr[c] * tr[c];
+            }
+            nrm = sqrt(nrm);
+            for (int c = 0; c < n; ++c) {
+                tr[c] /= nrm;
+                de -= (tr[c] - si[c]) * h[c];
+            }
+            if (de <= 0.0 || (double)xacc * 0x1p-32 < exp(-beta * de)) {
+                for (int c = 0; c < n; ++c) si[c] = tr[c];
+                ++acc;
+            }
+        }
+    }
+    return acc;
+}
+
+/* Normalise, then E, M[n], c_a, s_a (on.py:35-87) over forward bonds. */
+int orc_on_measure(double *s, int ndim, const int32_t *dims, int n, double *e, double *m,
+                   double *sc, double *ss)
+{
+    orc_cb g;
+    if (orc_cb_init(&g, ndim, dims)) return -1;
+    for (int64_t i = 0; i < g.N; ++i) {
+        double n2 = 0;
+        for (int c = 0; c < n; ++c) n2 += s[i * n + c] * s[i * n + c];
+        double nrm = sqrt(n2);
+        for (int c = 0; c < n; ++c) s[i * n + c] /= nrm;
+    }
+    *e = 0;
+    for (int c = 0; c < n; ++c) m[c] = 0;
+    for (int a = 0; a < ndim; ++a) sc[a] = ss[a] = 0;
+    int64_t stride[3] = {1, g.L[0], (int64_t)g.L[0] * g.L[1]};
+    for (int64_t i = 0; i < g.N; ++i) {
+        for (int c = 0; c < n; ++c) m[c] += s[i * n + c];
+        int64_t rest = i;
+        for (int a = 0; a < ndim; ++a) {
+            int x = (int)(rest % g.L[a]);
+            rest /= g.L[a];
+            int64_t j = i + (x + 1 == g.L[a] ? -(int64_t)(g.L[a] - 1) * stride[a] : stride[a]);
+            double dot = 0;
+            for (int c = 0; c < n; ++c) dot += s[i * n + c] * s[j * n + c];
+            *e -= dot;
+            sc[a] += s[i * n] * s[j * n] + s[i * n + 1] * s[j * n + 1];
+            ss[a] += s[i * n] * s[j * n + 1] - s[i * n + 1] * s[j * n];
+        }
+    }
+    return 0;
+}
+
+int orc_on_run(double *s, int ndim, const int32_t *dims, int n, double beta, double step,
+               uint32_t k0, uint32_t k1, uint32_t sid, uint64_t *sweep, const int64_t *sched,
+               int nrows, int64_t n_meas, double *e_out, double *m_out, double *sc_out,
+               double *ss_out, int64_t *acc_out)
+{
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        int64_t acc = 0;
+        for (int r = 0; r < nrows; ++r) {
+            int code = (int)sched[2 * r];
+            if (code != 0 && code != 5) return -3;
+            for (int64_t rp = 0; rp < sched[2 * r + 1]; ++rp) {
+                for (int c = 0; c < 2; ++c) {
+                    int64_t a = orc_on_half_sweep(s, ndim, dims, n, code, c, beta, step, k0, k1, sid, *sweep);
+                    if (a < 0) return -2;
+                    acc += a;
+                }
+                *sweep += 1;
+            }
+        }
+        orc_on_measure(s, ndim, dims, n, &e_out[meas], &m_out[meas * n], &sc_out[meas * ndim],
+                       &ss_out[meas * ndim]);
+        acc_out[meas] = acc;
+    }
+    return 0;
+}

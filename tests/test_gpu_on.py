@@ -1,0 +1,85 @@
+"""GPU O(n) kernels (FP32) against the FP64 replay.
+
+One sweep from an identical state agrees to 1e-5 rad per spin; Metropolis
+decisions may flip only where the acceptance uniform sits within the FP32
+error of exp(-beta dE), and the test bounds their number.  Measurements of a
+given state agree with the FP64 sums to 1e-9 relative.  Statistics: rings of
+8 spins against the transfer operator through the reference's Simulation API.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from critmc_amd.kernels import on as K
+from critmc_amd.kernels.chains import Chains
+from oracle import exact
+from oracle.stats import batch_mean_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _angle_err(a, b):
+    """Angle between unit vectors, per site."""
+    dot = np.clip(np.sum(a * b, axis=1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1)), -1, 1)
+    return np.arccos(dot)
+
+
+@pytest.mark.parametrize("n,shape", [(2, (512, 512)), (2, (64, 32)), (3, (16, 16, 16)), (2, (10,))])
+@pytest.mark.parametrize("code", [0, 5])
+def test_one_sweep_within_1e5_rad(n, shape, code):
+    dev = torch.device("cuda", 0)
+    R = 2
+    rng = np.random.default_rng(n + len(shape))
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    chains = Chains(keys, [3, 4], None, dev)
+    N = int(np.prod(shape))
+    s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
+    K.init_spins(s, shape, n, chains)
+    s0 = s.cpu().numpy().astype(np.float64)
+    betas = np.array([1.12, 0.9])
+    steps = np.array([1.0, 2.0])
+    acc, e, m, sc, ss = K.run(s, shape, n, betas, steps, chains, np.array([[code, 1]]), 1)
+    got = s.cpu().numpy().astype(np.float64)
+    for r in range(R):
+        rep = oracle.OnReplay(s0[r], shape, float(np.float32(betas[r])), float(np.float32(steps[r])),
+                              keys[r], [3, 4][r])
+        ref = rep.run([[code, 1]], 1)
+        err = _angle_err(got[r], rep.s)
+        bad = err > 1e-5
+        # a differing Metropolis decision moves a spin far; allow a handful, none for over-relaxation
+        limit = 0 if code == 5 else max(3, int(2e-5 * N))
+        assert bad.sum() <= limit, (bad.sum(), err.max())
+        if code == 0:
+            assert abs(int(acc[r, 0]) - int(ref["accepted"][0])) <= limit
+        # measurements of the final state: FP64 sums of the GPU's own FP32 state
+        fin = got[r] / np.linalg.norm(got[r], axis=1, keepdims=True)
+        chk = oracle.OnReplay(got[r], shape, 1.0, 1.0, (0, 0))
+        d = len(shape)
+        ev, mv, scv, ssv = (np.zeros(1), np.zeros(n), np.zeros(d), np.zeros(d))
+        oracle.lib().orc_on_measure(chk.s.ctypes.data_as(oracle._f64p), d,
+                                    chk.dims.ctypes.data_as(oracle._i32p), n,
+                                    ev.ctypes.data_as(oracle._f64p), mv.ctypes.data_as(oracle._f64p),
+                                    scv.ctypes.data_as(oracle._f64p), ssv.ctypes.data_as(oracle._f64p))
+        assert float(e[r, 0]) == pytest.approx(ev[0], rel=1e-9, abs=1e-6)
+        np.testing.assert_allclose(m[r, 0].cpu().numpy(), mv, rtol=1e-9, atol=1e-6)
+        np.testing.assert_allclose(sc[r, 0].cpu().numpy(), scv, rtol=1e-9, atol=1e-6)
+        np.testing.assert_allclose(ss[r, 0].cpu().numpy(), ssv, rtol=1e-9, atol=1e-6)
+        assert np.allclose(np.linalg.norm(fin, axis=1), 1.0)
+
+
+@pytest.mark.parametrize("n_comp", [2, 3])
+@pytest.mark.parametrize("updates", ["metropolis", "metropolis+3*overrelax"])
+def test_ring_matches_transfer_operator(n_comp, updates):
+    from critmc_amd import ON, Simulation, chain
+
+    beta, n_sites = 1.1, 8
+    sim = Simulation(ON(chain(n_sites), n_comp), beta, updates, seed=5)
+    info = sim.thermalize(2000)
+    assert 1e-3 <= info["step"] <= 10
+    ts = sim.run(100_000)
+    mean, err = batch_mean_error(ts["e"])
+    z = (mean - exact.on_ring_energy(beta, n_sites, n_comp)) / err
+    assert abs(z) < 4.5, z
+    assert np.allclose(np.linalg.norm(sim.state, axis=1), 1.0, atol=1e-6)

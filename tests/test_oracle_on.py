@@ -1,0 +1,65 @@
+"""FP64 replay of checkerboard O(n) sweeps (critmc/kernels/on.py:112-183).
+
+Pinned by: O(n) rings against the transfer operator (critmc tests/
+test_kernels.py:95-109, here with checkerboard order), over-relaxation
+invariants (energy conserved, twice = identity, zero field = no change,
+critmc tests/test_overrelax_xy.py), and a planted wrong beta.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import exact
+from oracle.stats import batch_mean_error
+
+
+def _random_state(N, n, seed):
+    s = np.random.default_rng(seed).standard_normal((N, n))
+    return s / np.linalg.norm(s, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_overrelax_invariants(oracle_lib, n):
+    from critmc_amd import square
+
+    lat = square(8)
+    s0 = _random_state(64, n, 1)
+    rep = oracle_lib.OnReplay(s0, (8, 8), 1.0, 1.0, (1, 1))
+    e0 = -np.sum(s0[lat.bonds[:, 0]] * s0[lat.bonds[:, 1]])
+    rep.half_sweep(5, 0)
+    e1 = -np.sum(rep.s[lat.bonds[:, 0]] * rep.s[lat.bonds[:, 1]])
+    assert e1 == pytest.approx(e0, abs=1e-12)
+    rep.half_sweep(5, 0)  # the reflection is an involution when the field is fixed
+    assert np.allclose(rep.s, s0, atol=1e-12)
+    z = np.zeros((4, n))
+    z[:, 0] = 1.0
+    z[1, 0] = z[3, 0] = -1.0  # chain of 4: both neighbours of sites 0, 2 cancel
+    rep = oracle_lib.OnReplay(z, (4,), 1.0, 1.0, (1, 1))
+    rep.half_sweep(5, 0)
+    assert np.array_equal(rep.s, z)
+
+
+def _ring(oracle_lib, n_comp, sched, beta, n_meas, seed, beta_true=None):
+    rep = oracle_lib.OnReplay(_random_state(8, n_comp, seed), (8,), beta, 1.0, (seed, 3))
+    for _ in range(10):  # step tuning as critmc/sim.py:226-229
+        out = rep.run(sched, 500)
+        nm = sum(c for code, c in sched if code == 0)
+        if nm:
+            rate = out["accepted"].sum() / (nm * 8 * 500)
+            rep.step = float(np.clip(rep.step * np.clip(rate / 0.5, 0.5, 2.0), 1e-3, 10.0))
+    out = rep.run(sched, n_meas)
+    bt = beta if beta_true is None else beta_true
+    mean, err = batch_mean_error(out["E"] / 8)
+    return (mean - exact.on_ring_energy(bt, 8, n_comp)) / err
+
+
+@pytest.mark.parametrize("n_comp", [2, 3])
+@pytest.mark.parametrize("sched", [[[0, 1]], [[0, 1], [5, 3]]])
+def test_ring_matches_transfer_operator(oracle_lib, n_comp, sched):
+    z = _ring(oracle_lib, n_comp, sched, 1.1, 100_000, seed=5 + n_comp)
+    assert abs(z) < 4.5, z
+
+
+def test_ring_planted_wrong_beta_is_caught(oracle_lib):
+    z = _ring(oracle_lib, 2, [[0, 1]], 1.1 * 0.85, 100_000, seed=3, beta_true=1.1)
+    assert abs(z) > 4.5, z
