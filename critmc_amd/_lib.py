@@ -44,7 +44,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_ising_convert": ([_vp, _vp, _c_i32, _i32p, _c_i32, _c_i32, _vp], _c_i32),
     "cm_ising_run": (
         [_vp, _c_i32, _c_i32, _i32p, _c_i32, _f64p, _c_f64, _vp, _vp, _vp, _u64p, _i64p, _c_i32,
-         _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _c_i32, _vp],
+         _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _c_i32, _vp, _vp],
         _c_i32,
     ),
     "cm_potts_workspace_bytes": ([_c_i32], _c_i64),
@@ -52,7 +52,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_potts_measure": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
     "cm_potts_run": (
         [_vp, _c_i32, _i32p, _c_i32, _c_i32, _f64p, _vp, _vp, _vp, _i64p, _c_i32, _c_i64, _vp,
-         _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _vp],
+         _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _vp, _vp],
         _c_i32,
     ),
     "cm_ea_workspace_bytes": ([_c_i32, _c_i32], _c_i64),
@@ -69,7 +69,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_on_init": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
     "cm_on_run": (
         [_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp, _vp, _vp, _i64p, _c_i32, _c_i64,
-         _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp],
+         _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp],
         _c_i32,
     ),
 }

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
     uint64_t* __restrict__ sweeps, Sched sched, int64_t n_meas, int32_t* __restrict__ e_run,
     int32_t* __restrict__ m_run, int32_t* __restrict__ e_out, int32_t* __restrict__ m_out,
-    int32_t* __restrict__ acc_out)
+    int32_t* __restrict__ acc_out, const int64_t* __restrict__ n_meas_rep)
 {
     extern __shared__ uint32_t lds[];
     const int M = L0 >> 1;
@@ -254,8 +254,9 @@ __global__ __launch_bounds__(256) void ising2d_batched(
         const uint32_t* tab = tabs + (int64_t)TAB_WORDS * rep;
         uint64_t sweep = sweeps[rep];
         int32_t e = e_run[rep], m = m_run[rep];
+        const int64_t n_rep_meas = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
 
-        for (int64_t meas = 0; meas < n_meas; ++meas) {
+        for (int64_t meas = 0; meas < n_rep_meas; ++meas) {
             Delta d{0, 0, 0};
             for (int row = 0; row < sched.n; ++row) {
                 const int code = sched.code[row];
@@ -542,8 +543,8 @@ template <int LAYOUT, int NC>
 static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_t* tabs,
                                  const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
                                  const Sched& sc, int64_t n_meas, int32_t* e_run, int32_t* m_run,
-                                 int32_t* e_out, int32_t* m_out, int32_t* acc_out, int block,
-                                 int grid, hipStream_t st)
+                                 int32_t* e_out, int32_t* m_out, int32_t* acc_out,
+                                 const int64_t* n_meas_rep, int block, int grid, hipStream_t st)
 {
     const size_t lds = (size_t)2 * g.L1 * g.G * 4 + 3 * 16 * 4;
     if (lds > 64 * 1024) {
@@ -553,7 +554,7 @@ static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_
     }
     hipLaunchKernelGGL((ising2d_batched<LAYOUT, NC>), dim3(grid), dim3(block), lds, st, spins, R,
                        g.L0, g.L1, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out,
-                       m_out, acc_out);
+                       m_out, acc_out, n_meas_rep);
     return hipGetLastError();
 }
 
@@ -562,14 +563,14 @@ static hipError_t launch_batched_nc(int nc, void* spins, int R, const Geo& g,
                                     const uint32_t* tabs, const uint32_t* keys,
                                     const uint32_t* sids, uint64_t* sweeps, const Sched& sc,
                                     int64_t n_meas, int32_t* e_run, int32_t* m_run,
-                                    int32_t* e_out, int32_t* m_out, int32_t* acc_out, int block,
-                                    int grid, hipStream_t st)
+                                    int32_t* e_out, int32_t* m_out, int32_t* acc_out,
+                                    const int64_t* n_meas_rep, int block, int grid, hipStream_t st)
 {
     switch (nc) {
-        case 2: return launch_batched<LAYOUT, 2>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
-        case 4: return launch_batched<LAYOUT, 4>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
-        case 7: return launch_batched<LAYOUT, 7>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
-        default: return launch_batched<LAYOUT, NC_LIMIT>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+        case 2: return launch_batched<LAYOUT, 2>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
+        case 4: return launch_batched<LAYOUT, 4>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
+        case 7: return launch_batched<LAYOUT, 7>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
+        default: return launch_batched<LAYOUT, NC_LIMIT>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
     }
 }
 
@@ -642,7 +643,8 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
                  uint64_t* sweeps, const uint64_t* sweeps_host, const int64_t* schedule,
                  int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run, int32_t* e_out,
                  int32_t* m_out, int32_t* acc_out, void* workspace, int64_t workspace_bytes,
-                 int32_t form, int32_t block, int32_t grid, void* stream)
+                 int32_t form, int32_t block, int32_t grid, const int64_t* n_meas_rep,
+                 void* stream)
 {
     Geo g;
     if (int rc = geo_from(ndim, dims, &g)) return rc;
@@ -694,14 +696,16 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         grid = std::min(grid, n_rep);
         hipError_t err = layout == 0
             ? launch_batched_nc<0>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
-                                   n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st)
+                                   n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block,
+                                   grid, st)
             : launch_batched_nc<1>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
-                                   n_meas, e_run, m_run, e_out, m_out, acc_out, block, grid, st);
+                                   n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
         CM_HIP_CHECK(err);
         return CM_OK;
     }
     // plain form: one chain at a time, one launch per half sweep
     CM_REQUIRE(form == 1, "form must be 0 (auto), 1 (plain) or 2 (batched)");
+    CM_REQUIRE(n_meas_rep == nullptr, "per-chain measurement counts need the batched form");
     CM_REQUIRE(layout == 0, "the plain form works on int8 spins");
     CM_REQUIRE(sweeps_host != nullptr, "the plain form needs the sweep counters on the host");
     std::vector<uint32_t> hkeys((size_t)2 * n_rep), hsid(n_rep);

@@ -2,6 +2,9 @@
 // checkerboard Metropolis and over-relaxation sweeps, FP64 renormalisation
 // and measurements.
 //
+// Precision: spins and the proposal / reflection arithmetic are FP32; the
+// local field is accumulated in FP64 and rounded once (see on_half_sweep).
+//
 // Reference (critmc/kernels/on.py):
 //   Metropolis   :112-137  S' = (S + step g)/|S + step g|, g ~ N(0,1)^n;
 //                          dE = -(S' - S).h; accept iff dE <= 0 or u < exp(-beta dE)
@@ -82,6 +85,7 @@ struct OnArgs {
     const uint32_t* sids;   // [R]
     const uint64_t* sweeps; // [R]
     int32_t* acc;           // [R][n_meas] (accumulated)
+    const int64_t* n_meas_rep;  // [R] measurements per replica, or null
     CbGeo g;
     int R;
     int64_t n_meas, meas;
@@ -93,21 +97,28 @@ template <int NC, int Z, int CODE>
 __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 {
     const int rep = blockIdx.y;
+    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int accepted = 0;
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
         int64_t nb[Z];
         const int64_t i = cb_site<Z>(a.g, k, a.colour, nb);
-        float h[NC];
+        // The local field is summed in FP64: its Z FP32 terms are exact there, so
+        // a nearly cancelling field keeps full relative precision (in FP32 the
+        // sum's absolute error eps * Z would dominate when |h| -> 0).
+        double hd[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) h[c] = 0.0f;
+        for (int c = 0; c < NC; ++c) hd[c] = 0.0;
 #pragma unroll
         for (int j = 0; j < Z; ++j) {
             const Vec<NC> t = load_vec<NC>(S, nb[j]);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) h[c] += t.v[c];
+            for (int c = 0; c < NC; ++c) hd[c] += (double)t.v[c];
         }
+        float h[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) h[c] = (float)hd[c];
         Vec<NC> s = load_vec<NC>(S, i);
         if (CODE == CODE_OVERRELAX) {
             float hh = 0.0f, sh = 0.0f;
@@ -170,10 +181,12 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 
 // Renormalise every spin in FP64 (critmc/kernels/on.py:78-87).
 template <int NC>
-__global__ void on_normalize(float* __restrict__ spins, int64_t total_sites)
+__global__ void on_normalize(float* __restrict__ spins, int64_t total_sites, int64_t N,
+                             const int64_t* __restrict__ n_meas_rep, int64_t meas)
 {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total_sites;
          t += (int64_t)gridDim.x * blockDim.x) {
+        if (n_meas_rep && meas >= n_meas_rep[t / N]) continue;
         Vec<NC> s = load_vec<NC>(spins, t);
         double n2 = 0.0;
 #pragma unroll
@@ -188,11 +201,14 @@ __global__ void on_normalize(float* __restrict__ spins, int64_t total_sites)
 // Partial FP64 sums of E, M[n], c_a, s_a over chunks of ON_CHUNK sites.
 template <int NC, int D>
 __global__ __launch_bounds__(256) void on_measure_partial(const float* __restrict__ spins, CbGeo g,
-                                                          double* __restrict__ part, int nchunk)
+                                                          double* __restrict__ part, int nchunk,
+                                                          const int64_t* __restrict__ n_meas_rep,
+                                                          int64_t meas)
 {
     constexpr int K = 1 + NC + 2 * D;
     __shared__ double red[K * 4];
     const int rep = blockIdx.y, chunk = blockIdx.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;
     const float* S = spins + (int64_t)rep * g.N * NC;
     double v[K];
 #pragma unroll
@@ -238,12 +254,13 @@ template <int NC, int D>
 __global__ void on_measure_final(const double* __restrict__ part, int nchunk, int R,
                                  int64_t n_meas, int64_t meas, double* __restrict__ e_out,
                                  double* __restrict__ m_out, double* __restrict__ sc_out,
-                                 double* __restrict__ ss_out)
+                                 double* __restrict__ ss_out, const int64_t* __restrict__ n_meas_rep)
 {
     constexpr int K = 1 + NC + 2 * D;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= R * K) return;
     const int rep = t / K, q = t - (t / K) * K;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += part[((int64_t)rep * nchunk + c) * K + q];
     const int64_t o = (int64_t)rep * n_meas + meas;
@@ -260,10 +277,14 @@ __global__ void fill_i32_on(int32_t* __restrict__ p, int64_t n)
         p[i] = 0;
 }
 
-__global__ void on_advance_sweeps(uint64_t* __restrict__ sweeps, int R, uint64_t n)
+__global__ void on_advance_sweeps(uint64_t* __restrict__ sweeps, int R, uint64_t per_meas,
+                                  int64_t n_meas, const int64_t* __restrict__ n_meas_rep)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < R) sweeps[t] += n;
+    if (t < R) {
+        const int64_t n = n_meas_rep ? min(n_meas, n_meas_rep[t]) : n_meas;
+        sweeps[t] += per_meas * (uint64_t)n;
+    }
 }
 
 __global__ void on_init_kernel(float* __restrict__ spins, int64_t N, int NC,
@@ -316,18 +337,18 @@ static void launch_half_z(int code, const OnArgs& a, hipStream_t st)
 template <int NC>
 static void launch_measure(const float* spins, const CbGeo& g, int R, double* part, int nchunk,
                            int64_t n_meas, int64_t meas, double* e_out, double* m_out,
-                           double* sc_out, double* ss_out, hipStream_t st)
+                           double* sc_out, double* ss_out, const int64_t* nr, hipStream_t st)
 {
     const dim3 grid(nchunk, R);
     if (g.d == 1) {
-        hipLaunchKernelGGL((on_measure_partial<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nchunk);
-        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+        hipLaunchKernelGGL((on_measure_partial<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else if (g.d == 2) {
-        hipLaunchKernelGGL((on_measure_partial<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nchunk);
-        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+        hipLaunchKernelGGL((on_measure_partial<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else {
-        hipLaunchKernelGGL((on_measure_partial<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nchunk);
-        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out);
+        hipLaunchKernelGGL((on_measure_partial<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     }
 }
 
@@ -364,7 +385,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
               const uint32_t* stream_ids, uint64_t* sweeps, const int64_t* schedule,
               int32_t n_rows, int64_t n_meas, int32_t* acc_out, double* e_out, double* m_out,
               double* sc_out, double* ss_out, void* workspace, int64_t workspace_bytes,
-              void* stream)
+              const int64_t* n_meas_rep, void* stream)
 {
     CbGeo g;
     CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
@@ -387,8 +408,8 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     const int64_t total = (int64_t)n_rep * n_meas;
     hipLaunchKernelGGL(fill_i32_on, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)),
                        dim3(256), 0, st, acc_out, total);
-    OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, g, n_rep,
-             n_meas, 0, 0, 0};
+    OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, n_meas_rep,
+             g, n_rep, n_meas, 0, 0, 0};
     uint64_t sw = 0;
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
@@ -407,15 +428,17 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         const int64_t sites = (int64_t)n_rep * g.N;
         const int nb = (int)std::min<int64_t>((sites + 255) / 256, 8192);
         if (n == 2) {
-            hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, (float*)spins, sites);
-            launch_measure<2>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, st);
+            hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
+            launch_measure<2>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         } else {
-            hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, (float*)spins, sites);
-            launch_measure<3>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, st);
+            hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
+            launch_measure<3>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         }
     }
+    int64_t per_meas = 0;
+    for (int r = 0; r < n_rows; ++r) per_meas += schedule[2 * r + 1];
     hipLaunchKernelGGL(on_advance_sweeps, dim3((n_rep + 255) / 256), dim3(256), 0, st, sweeps,
-                       n_rep, sw);
+                       n_rep, (uint64_t)per_meas, n_meas, n_meas_rep);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }

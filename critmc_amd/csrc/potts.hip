@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void potts_batched(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
     uint64_t* __restrict__ sweeps, Sched sc, int64_t n_meas, int32_t* __restrict__ e_run,
     int32_t* __restrict__ counts_run, int32_t* __restrict__ e_out, int32_t* __restrict__ acc_out,
-    int32_t* __restrict__ counts_out, int use_lds)
+    int32_t* __restrict__ counts_out, int use_lds, const int64_t* __restrict__ n_meas_rep)
 {
     extern __shared__ int4 lds4[];
     int* counts = (int*)lds4;          // [128]
@@ -133,7 +133,8 @@ __global__ __launch_bounds__(256) void potts_batched(
         const PottsTab* tb = tabs + rep;
         uint64_t sweep = sweeps[rep];
         int32_t e = e_run[rep];
-        for (int64_t meas = 0; meas < n_meas; ++meas) {
+        const int64_t n_rep_meas = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
+        for (int64_t meas = 0; meas < n_rep_meas; ++meas) {
             int de = 0, acc = 0;
             for (int row = 0; row < sc.n; ++row) {
                 const int code = sc.code[row];
@@ -257,7 +258,8 @@ static hipError_t launch_potts(int8_t* spins, const CbGeo& g, int q, int R, cons
                                const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
                                const Sched& sc, int64_t n_meas, int32_t* e_run,
                                int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
-                               int32_t* counts_out, int block, int grid, hipStream_t st)
+                               int32_t* counts_out, const int64_t* n_meas_rep, int block,
+                               int grid, hipStream_t st)
 {
     const int use_lds = g.N <= POTTS_LDS_STATE ? 1 : 0;
     const size_t lds = (128 + 32) * 4 + (use_lds ? (size_t)((g.N + 15) / 16) * 16 : 0);
@@ -268,7 +270,7 @@ static hipError_t launch_potts(int8_t* spins, const CbGeo& g, int q, int R, cons
     }
     hipLaunchKernelGGL((potts_batched<Z>), dim3(grid), dim3(block), lds, st, spins, g, q, R, tabs,
                        keys, sids, sweeps, sc, n_meas, e_run, counts_run, e_out, acc_out,
-                       counts_out, use_lds);
+                       counts_out, use_lds, n_meas_rep);
     return hipGetLastError();
 }
 
@@ -319,7 +321,7 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
                  uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
                  int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
                  int32_t* counts_out, void* workspace, int64_t workspace_bytes, int32_t block,
-                 int32_t grid, void* stream)
+                 int32_t grid, const int64_t* n_meas_rep, void* stream)
 {
     CbGeo g;
     CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
@@ -360,13 +362,13 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     int8_t* s = (int8_t*)spins;
     if (z == 2)
         err = launch_potts<2>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
-                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+                              counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     else if (z == 4)
         err = launch_potts<4>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
-                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+                              counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     else
         err = launch_potts<6>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
-                              counts_run, e_out, acc_out, counts_out, block, grid, st);
+                              counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     CM_HIP_CHECK(err);
     return CM_OK;
 }

@@ -94,11 +94,14 @@ class Workspace:
 def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: np.ndarray,
         n_meas: int, e_run: torch.Tensor, m_run: torch.Tensor, *, layout: int = LAYOUT_INT8,
         form: int = FORM_AUTO, block: int = 0, grid: int = 0, ws: Workspace | None = None,
-        out: tuple[torch.Tensor, torch.Tensor, torch.Tensor] | None = None):
+        out: tuple[torch.Tensor, torch.Tensor, torch.Tensor] | None = None,
+        n_meas_rep: torch.Tensor | None = None):
     """Apply ``schedule`` ``n_meas`` times to every replica.
 
     Returns device int32 ``(E_b, M, accepted)`` series of shape ``[R, n_meas]``;
     ``e_run``/``m_run`` and the chains' sweep counters are advanced in place.
+    ``n_meas_rep`` (int64 device ``[R]``, batched form) caps each replica's
+    count: replica r performs ``min(n_meas, n_meas_rep[r])`` measurements.
     """
     R = chains.R
     betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, dtype=np.float64), (R,)))
@@ -118,8 +121,9 @@ def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: n
         ptr(chains.sweeps_d), host_ptr(chains.sweeps_h, ctypes.c_uint64),
         host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(e_run), ptr(m_run),
         ptr(e_out), ptr(m_out), ptr(acc_out), ptr(ws.buf), ws.nbytes, form, block, grid,
-        stream(),
+        ptr(n_meas_rep), stream(),
     )
     check(rc, "cm_ising_run")
-    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    if n_meas_rep is None:
+        chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return e_out, m_out, acc_out

@@ -37,7 +37,7 @@ def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
 
 
 def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedule, n_meas: int,
-        *, ws: Workspace | None = None):
+        *, ws: Workspace | None = None, n_meas_rep: torch.Tensor | None = None):
     """Returns device tensors (accepted int32 [R, n_meas], E f64 [R, n_meas],
     M f64 [R, n_meas, n], stiff_c / stiff_s f64 [R, n_meas, d])."""
     R = chains.R
@@ -58,7 +58,9 @@ def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedu
     rc = lib().cm_on_run(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, int(n), ptr(b),
                          ptr(st), ptr(chains.keys_d), ptr(chains.sids_d), ptr(chains.sweeps_d),
                          host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(acc),
-                         ptr(e), ptr(m), ptr(sc), ptr(ss), ptr(ws.buf), ws.nbytes, stream())
+                         ptr(e), ptr(m), ptr(sc), ptr(ss), ptr(ws.buf), ws.nbytes,
+                         ptr(n_meas_rep), stream())
     check(rc, "cm_on_run")
-    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    if n_meas_rep is None:
+        chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return acc, e, m, sc, ss

@@ -50,7 +50,7 @@ def measure(spins: torch.Tensor, shape, q: int, R: int) -> tuple[torch.Tensor, t
 
 def run(spins: torch.Tensor, shape, q: int, betas, chains: Chains, schedule, n_meas: int,
         e_run: torch.Tensor, counts_run: torch.Tensor, *, block: int = 0, grid: int = 0,
-        ws: Workspace | None = None):
+        ws: Workspace | None = None, n_meas_rep: torch.Tensor | None = None):
     """Returns device int32 ``(E [R, n_meas], accepted [R, n_meas], counts [R, n_meas, q])``."""
     R = chains.R
     betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, dtype=np.float64), (R,)))
@@ -69,7 +69,8 @@ def run(spins: torch.Tensor, shape, q: int, betas, chains: Chains, schedule, n_m
         host_ptr(betas, ctypes.c_double), ptr(chains.keys_d), ptr(chains.sids_d),
         ptr(chains.sweeps_d), host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas),
         ptr(e_run), ptr(counts_run), ptr(e_out), ptr(acc), ptr(counts), ptr(ws.buf), ws.nbytes,
-        block, grid, stream())
+        block, grid, ptr(n_meas_rep), stream())
     check(rc, "cm_potts_run")
-    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    if n_meas_rep is None:
+        chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return e_out, acc, counts

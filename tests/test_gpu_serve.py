@@ -1,0 +1,68 @@
+"""Continuous batching: each job's result is the same alone and in the mix.
+
+Jobs arriving at different times, of different models and sizes, share
+launches; their series must equal, bit for bit, the series of the same job
+served alone.  Admission never exceeds the byte budget.
+"""
+
+import numpy as np
+import pytest
+
+from critmc_amd.serve import Job, Server, poisson_jobs
+
+pytestmark = pytest.mark.gpu
+
+PROBES = [Job("ising", 32, 0.44, 1300, seed=11), Job("potts", 16, 1.42, 700, seed=12),
+          Job("xy", 16, 1.1, 900, seed=13), Job("ising", 16, 0.40, 333, seed=14)]
+
+
+def _clone(j, arrival=0.0):
+    return Job(j.model, j.L, j.beta, j.sweeps, j.seed, q=j.q, step=j.step, arrival=arrival)
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    alone = []
+    for j in PROBES:
+        srv = Server(budget_bytes=1e9, slice_sweeps=256)
+        job = srv.submit(_clone(j))
+        srv.run(realtime=False)
+        alone.append(job.results())
+    mix = poisson_jobs(24, rate=2000.0, seed=5, L_choices=(16, 32, 64), sweeps_range=(200, 2000))
+    srv = Server(budget_bytes=1e9, slice_sweeps=300)
+    probes = [srv.submit(_clone(j, arrival=0.003 * (k + 1))) for k, j in enumerate(PROBES)]
+    for j in mix:
+        srv.submit(j)
+    done = srv.run(realtime=True)
+    assert len(done) == len(mix) + len(PROBES)
+    for res, job in zip(alone, probes):
+        got = job.results()
+        assert got.keys() == res.keys()
+        for k in res:
+            assert np.array_equal(got[k], res[k]), (job.model, k)
+    assert srv.ticks > 1
+
+
+def test_admission_respects_the_budget():
+    jobs = [Job("ising", 64, 0.44, 3000, seed=s) for s in range(6)]
+    budget = 2.5 * jobs[0].footprint()
+    srv = Server(budget_bytes=budget, slice_sweeps=1000)
+    for j in jobs:
+        srv.submit(j)
+    done = srv.run(realtime=False)
+    assert len(done) == 6
+    assert srv.peak_used <= budget
+    assert all(j.done == j.sweeps for j in done)
+    # never more than two at a time: the third started only after one finished
+    starts = sorted(j.admitted_at for j in done)
+    assert starts[2] > 0
+
+
+def test_latency_report():
+    srv = Server(budget_bytes=2e9, slice_sweeps=500)
+    for j in poisson_jobs(40, rate=500.0, seed=9, L_choices=(16, 32, 64, 128), sweeps_range=(300, 3000)):
+        srv.submit(j)
+    done = srv.run(realtime=True)
+    lat = np.array([j.latency for j in done])
+    assert np.all(lat > 0)
+    assert np.percentile(lat, 99) >= np.percentile(lat, 50)
+    assert srv.site_updates == sum(j.sweeps * j.sites for j in done)
