@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "ising2d_traffic.json"))
+    ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "serve", "copy"), default="ising2d",
+                    help="ising2d: the headline batch; ea / xy / serve: the other configurations")
+    ap.add_argument("--samples", type=int, default=512, help="ea: disorder samples")
+    ap.add_argument("--jobs", type=int, default=10_000, help="serve: number of jobs")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 70%% of capacity)")
     return ap.parse_args()
 
 
@@ -72,12 +77,156 @@ def cpu_baseline(seconds: float) -> dict:
                       f"one replica per thread ({dt:.1f} s)"}
 
 
+def bench_ea(args) -> dict:
+    """512 +-J samples, L in {8, 12, 16} from seed 7, 2 sets x 32 temperatures in [0.8, 2.0],
+    heat bath + device swaps every sweep, int32 E and q recorded per sweep."""
+    import torch
+
+    from critmc_amd import hypercubic
+    from critmc_amd.glass import EdwardsAnderson, TemperingBatch, geometric_temperatures
+    from critmc_amd.rng import make_rng
+
+    rng = make_rng(7)
+    Ls = rng.choice([8, 12, 16], size=args.samples)
+    models = [EdwardsAnderson(hypercubic((int(L),) * 3), disorder="bimodal", seed=make_rng(7, s))
+              for s, L in enumerate(Ls)]
+    temps = geometric_temperatures(0.8, 2.0, 32)
+    batch = TemperingBatch(models, temps, seeds=[(7 << 20) + s for s in range(args.samples)],
+                           update="heatbath")
+    eng = batch.engine
+    steps = args.steps
+    n_bins = int(np.floor(np.log2(steps))) + 1
+    eng.run(max(args.warmup, 1), 1, update="heatbath")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = eng.run(steps, n_bins, update="heatbath", record=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    updates = int(64 * (Ls.astype(np.int64) ** 3).sum()) * steps
+    q = out["q"]
+    return {"metric": "site updates/ns, 3D EA +-J parallel tempering (512 samples x 2 x 32 T)",
+            "value": round(updates / dt / 1e9, 3), "unit": "site updates/ns", "n_gpus": 1,
+            "steps": steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+            "data": "synthetic: +-J bonds and L from seed 7", "config": {
+                "workload": "ea3d_pt_heatbath", "samples": args.samples,
+                "L_counts": {int(L): int((Ls == L).sum()) for L in (8, 12, 16)},
+                "temperatures": 32, "sets": 2, "record": "int32 E [S][2][32][steps], q [S][32][steps]"},
+            "series_bytes": int(out["E"].numel() * 4 + q.numel() * 4)}
+
+
+def bench_xy(args) -> dict:
+    """256 replicas of 512^2 XY, T = 0.89 +- 0.02, metropolis+2*overrelax, FP64 E and
+    helicity sums per measurement; Metropolis step tuned towards 50 % acceptance."""
+    import torch
+
+    from critmc_amd.kernels import on as K
+    from critmc_amd.kernels.chains import Chains
+    from critmc_amd.rng import seed_sequence
+
+    L, R = 512, args.replicas if args.replicas != R_DEFAULT else 256
+    dev = torch.device("cuda", 0)
+    key = seed_sequence(99).generate_state(2, np.uint32).astype(np.uint64)
+    ch = Chains(np.tile(key, (R, 1)), np.arange(R), None, dev)
+    T = 0.89 + np.random.default_rng(99).uniform(-0.02, 0.02, size=R)
+    betas = 1.0 / T
+    s = torch.empty((R, L * L, 2), dtype=torch.float32, device=dev)
+    K.init_spins(s, (L, L), 2, ch)
+    ws = K.Workspace((L, L), R, 2, dev)
+    sched = np.array([[0, 1], [5, 2]])
+    steps = np.full(R, 1.0)
+    for _ in range(5):  # tuning chunks, critmc/sim.py:226-229
+        acc, *_ = K.run(s, (L, L), 2, betas, steps, ch, sched, max(args.warmup // 5, 2), ws=ws)
+        rate = acc.sum(dim=1).cpu().numpy() / (L * L * acc.shape[1])
+        steps = np.clip(steps * np.clip(rate / 0.5, 0.5, 2.0), 1e-3, 10.0)
+    torch.cuda.synchronize()
+    n = args.steps
+    t0 = time.perf_counter()
+    acc, e, m, sc, ss = K.run(s, (L, L), 2, betas, steps, ch, sched, n, ws=ws)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    updates = R * L * L * 3 * n
+    N = L * L
+    ups = ((sc.mean(dim=2) - torch.as_tensor(betas, device=dev)[:, None] * (ss ** 2).mean(dim=2)) / N)
+    return {"metric": "site updates/ns, XY 256 x 512^2 FP32 metropolis+2*overrelax",
+            "value": round(updates / dt / 1e9, 3), "unit": "site updates/ns", "n_gpus": 1,
+            "steps": n, "warmup": args.warmup, "ms_per_step": dt * 1e3 / n,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 sums)",
+            "data": "synthetic: angles uniform from seed 99", "config": {
+                "workload": "xy2d_metropolis_2overrelax", "L": L, "replicas": R,
+                "T": "0.89 +- 0.02", "measurements": n, "sweeps_per_measurement": 3},
+            "acceptance_mean": float(acc.sum().item() / (R * N * n)),
+            "helicity_mean": float(ups.mean().item()), "energy_per_site_mean": float(e.mean().item() / N)}
+
+
+def bench_serve(args) -> dict:
+    """Poisson stream of Ising / Potts q=10 / XY jobs, L log-uniform in 16..512, beta within
+    10 % of the critical point, sweeps log-uniform in [1e3, 1e5], 200 GB admission budget."""
+    import torch
+
+    from critmc_amd.serve import Server, poisson_jobs
+
+    jobs = poisson_jobs(args.jobs, rate=1.0, seed=2024)
+    work = sum(j.sweeps * j.sites for j in jobs)
+    rate = args.rate
+    if rate <= 0:  # arrivals at ~70 % of a capacity estimate of ~1e12 site updates/s
+        rate = 0.7 * 1.0e12 / (work / len(jobs))
+    jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
+    srv = Server(budget_bytes=200e9, slice_sweeps=1000)
+    for j in jobs:
+        srv.submit(j)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = srv.run(realtime=True, t0=t0)
+    wall = time.perf_counter() - t0
+    lat = np.array([j.latency for j in done])
+    return {"metric": "p50/p99 job latency and site updates/ns, Poisson job stream",
+            "value": round(srv.site_updates / wall / 1e9, 3), "unit": "site updates/ns",
+            "n_gpus": 1, "steps": len(done), "warmup": 0, "ms_per_step": wall * 1e3 / len(done),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int8 / f32", "data": "synthetic Poisson job stream, seed 2024",
+            "config": {"workload": "serve_continuous_batching", "jobs": len(done),
+                       "arrival_rate_per_s": rate, "budget_GB": 200, "slice_sweeps": 1000},
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
+            "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks}
+
+
+def bench_copy(args) -> dict:
+    """Calibration for the HBM traffic counters: an elementwise copy of a known size
+    (1 GiB read + 1 GiB written per launch), profiled with the same counters."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.add(src, 0, out=dst)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(args.steps, 1)
+    for _ in range(reps):
+        torch.add(src, 0, out=dst)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"metric": "copy bandwidth (calibration)", "value": 2 * 4 * n / dt / 1e9, "unit": "GB/s",
+            "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n, "launches": reps + 3}
+
+
+R_DEFAULT = R
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; replicas shard across GPUs with no "
                                    "traffic, so --gpus N > 1 is not measured here"}))
         sys.exit(2)
+
+    if args.workload != "ising2d":
+        fn = {"ea": bench_ea, "xy": bench_xy, "serve": bench_serve, "copy": bench_copy}[args.workload]
+        print(json.dumps(fn(args)))
+        return
 
     import torch
 
@@ -146,7 +295,7 @@ def main():
     tpath = Path(args.traffic)
     if tpath.exists():
         t = json.loads(tpath.read_text())
-        traffic = t.get(f"{args.layout}_spl{spl}_bytes_per_launch")
+        traffic = t.get(f"{args.layout}_L{L}_R{Rn}_spl{spl}_bytes_per_launch")
 
     # plain one-chain-per-launch form of the same kernel on one replica
     plain_rate = None
