@@ -177,7 +177,7 @@ def bench_serve(args) -> dict:
         srv.submit(j)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    done = srv.run(realtime=True, t0=t0)
+    done = srv.run(realtime=True, t0=t0, progress=10.0)
     wall = time.perf_counter() - t0
     lat = np.array([j.latency for j in done])
     return {"metric": "p50/p99 job latency and site updates/ns, Poisson job stream",

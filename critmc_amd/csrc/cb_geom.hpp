@@ -11,24 +11,58 @@
 
 namespace cm {
 
+// Division of x < 2^31 by a runtime constant d >= 1 with one high multiply:
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1, q = (mulhi(x, m) + x) >> s
+// (Granlund & Montgomery, PLDI 1994, the round-up variant).  x < 2^31 keeps the
+// sum below 2^32.
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d)
+{
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    const uint64_t m = (((1ull << s) - d) << 32) / d + 1;
+    return FastDiv{d, (uint32_t)m, s};
+}
+
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv& f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t hi = __umulhi(x, f.m);
+#else
+    const uint32_t hi = (uint32_t)(((uint64_t)x * f.m) >> 32);
+#endif
+    return (hi + x) >> f.s;
+}
+
 struct CbGeo {
     int d, L0, L1, L2, M;
     int64_t N, half;  // sites, sites per colour
+    FastDiv divM, divL1;
 };
+
+__host__ __device__ inline void cb_finish(CbGeo& g)
+{
+    g.divM = make_fastdiv((uint32_t)g.M);
+    g.divL1 = make_fastdiv((uint32_t)g.L1);
+}
 
 template <int Z>
 __host__ __device__ __forceinline__ int64_t cb_site(const CbGeo& g, int64_t k, int c,
                                                     int64_t (&nb)[Z])
 {
-    const int64_t r = k / g.M;
-    const int m = (int)(k - r * g.M);
+    const uint32_t r = fdiv((uint32_t)k, g.divM);
+    const int m = (int)((uint32_t)k - r * (uint32_t)g.M);
     int x1 = 0, x2 = 0;
     if (Z >= 4) {
-        x1 = (int)(r % g.L1);
-        x2 = (int)(r / g.L1);
+        const uint32_t q = fdiv(r, g.divL1);
+        x1 = (int)(r - q * (uint32_t)g.L1);
+        x2 = (int)q;
     }
     const int x0 = 2 * m + ((x1 + x2 + c) & 1);
-    const int64_t base = r * g.L0;
+    const int64_t base = (int64_t)r * g.L0;
     const int64_t i = base + x0;
     nb[0] = base + (x0 + 1 == g.L0 ? 0 : x0 + 1);
     nb[1] = base + (x0 == 0 ? g.L0 - 1 : x0 - 1);
@@ -59,6 +93,7 @@ inline int cb_geo_from(int ndim, const int32_t* dims, CbGeo* g)
     g->M = g->L0 / 2;
     g->N = N;
     g->half = N / 2;
+    cb_finish(*g);
     return 0;
 }
 

@@ -40,6 +40,7 @@ namespace cm {
 constexpr int EA_NOBS = 7;
 constexpr int EA_TAB = 16;
 constexpr int64_t EA_LDS_MAX = 32768;
+constexpr int EA_LMAX = 256;
 
 __host__ __device__ inline double det_exp(double x)
 {
@@ -102,7 +103,21 @@ __device__ __forceinline__ CbGeo ea_geo(int d, int L)
     g.M = L / 2;
     g.N = (int64_t)L * g.L1 * g.L2;
     g.half = g.N / 2;
+    cb_finish(g);
     return g;
+}
+
+// Coordinates of site i (x fastest) on an L^d lattice, 32-bit arithmetic.
+template <int D>
+__device__ __forceinline__ void ea_coords(uint32_t i, uint32_t L, const FastDiv& dl, int (&x)[3])
+{
+    uint32_t rest = i;
+#pragma unroll
+    for (int ax = 0; ax < D; ++ax) {
+        const uint32_t q = fdiv(rest, dl);
+        x[ax] = (int)(rest - q * L);
+        rest = q;
+    }
 }
 
 template <int Z>
@@ -231,6 +246,7 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
 {
     __shared__ int ired[2 * 16];
     __shared__ double dred[6 * 16];
+    __shared__ double ctab[EA_LMAX], stab[EA_LMAX];
     const int m = a.m;
     const int s = blockIdx.x / m, k = blockIdx.x - (blockIdx.x / m) * m;
     const int L = a.L[s];
@@ -240,22 +256,29 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
     const int cb = a.perm[((int64_t)s * 2 + 1) * m + k];
     const int8_t* sa = a.spins + a.spin_off[s] + ((int64_t)0 * m + ca) * N;
     const int8_t* sb = a.spins + a.spin_off[s] + ((int64_t)1 * m + cb) * N;
+    // cos / sin of 2 pi x / L for every coordinate value (L <= EA_LMAX)
+    for (int x = threadIdx.x; x < L; x += blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * x / L, &sn, &cs);
+        ctab[x] = cs;
+        stab[x] = sn;
+    }
+    __syncthreads();
+    const FastDiv dl = make_fastdiv((uint32_t)L);
     int q = 0, ql = 0;
     double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
     for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
         const int qi = sa[i] * sb[i];
         q += qi;
-        int64_t rest = i;
+        int x[3];
+        ea_coords<Z / 2>((uint32_t)i, (uint32_t)L, dl, x);
+#pragma unroll
         for (int ax = 0; ax < Z / 2; ++ax) {
-            const int x = (int)(rest % L);
-            rest /= L;
             const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
-            const int64_t j = i + (x + 1 == L ? -(int64_t)(L - 1) * stride : stride);
+            const int64_t j = i + (x[ax] + 1 == L ? -(int64_t)(L - 1) * stride : stride);
             ql += qi * sa[j] * sb[j];
-            double sn, cs;
-            sincospi(2.0 * x / L, &sn, &cs);
-            re[ax] += qi * cs;
-            im[ax] += qi * sn;
+            re[ax] += qi * ctab[x[ax]];
+            im[ax] += qi * stab[x[ax]];
         }
     }
     q = wave_sum(q);
@@ -324,14 +347,15 @@ __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
     const int64_t N = g.N;
     const int8_t* sp = a.spins + a.spin_off[s] + (int64_t)rc * N;
     const int8_t* J = a.J + a.j_off[s];
+    const FastDiv dl = make_fastdiv((uint32_t)L);
     int e = 0;
     for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-        int64_t rest = i;
+        int x[3];
+        ea_coords<Z / 2>((uint32_t)i, (uint32_t)L, dl, x);
+#pragma unroll
         for (int ax = 0; ax < Z / 2; ++ax) {
-            const int x = (int)(rest % L);
-            rest /= L;
             const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
-            const int64_t j = i + (x + 1 == L ? -(int64_t)(L - 1) * stride : stride);
+            const int64_t j = i + (x[ax] + 1 == L ? -(int64_t)(L - 1) * stride : stride);
             e -= J[ax * N + i] * sp[i] * sp[j];
         }
     }
@@ -439,7 +463,8 @@ int cm_ea_pt_run(void* spins, const void* J, const int32_t* L_dev, const int64_t
     CM_REQUIRE(n_steps >= 0 && n_bins >= 1, "bad step or bin count");
     int max_L = 2;
     for (int s = 0; s < n_samples; ++s) {
-        CM_REQUIRE(L_host[s] >= 2 && L_host[s] % 2 == 0, "extents must be even and >= 2 (sample %d has %d)", s, L_host[s]);
+        CM_REQUIRE(L_host[s] >= 2 && L_host[s] % 2 == 0 && L_host[s] <= EA_LMAX,
+                   "extents must be even, >= 2 and <= 256 (sample %d has %d)", s, L_host[s]);
         max_L = std::max(max_L, (int)L_host[s]);
     }
     CM_REQUIRE(workspace_bytes >= cm_ea_workspace_bytes(n_samples, m), "workspace too small");

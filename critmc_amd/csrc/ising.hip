@@ -690,7 +690,10 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     if (form == 2) {
         CM_REQUIRE(batched_ok(g),
                    "batched form needs a 2D lattice with L0 <= 64 or L0 a multiple of 64");
-        if (block <= 0) block = 256;
+        if (block <= 0) {  // one lane per word of a colour, in whole waves, at most 256
+            const int64_t words = (int64_t)g.L1 * g.G;
+            block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (words + 63) / 64 * 64));
+        }
         CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
         if (grid <= 0) grid = n_rep;
         grid = std::min(grid, n_rep);

@@ -217,15 +217,22 @@ __global__ __launch_bounds__(256) void on_measure_partial(const float* __restric
     const int64_t i1 = min(g.N, i0 + ON_CHUNK);
     const int64_t st[3] = {1, g.L0, (int64_t)g.L0 * g.L1};
     const int ext[3] = {g.L0, g.L1, g.L2};
+    const FastDiv dv[2] = {make_fastdiv((uint32_t)g.L0), make_fastdiv((uint32_t)g.L1)};
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const Vec<NC> s = load_vec<NC>(S, i);
 #pragma unroll
         for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c];
-        int64_t rest = i;
+        uint32_t rest = (uint32_t)i;
 #pragma unroll
         for (int ax = 0; ax < D; ++ax) {
-            const int x = (int)(rest % ext[ax]);
-            rest /= ext[ax];
+            int x;
+            if (ax < 2) {
+                const uint32_t q = fdiv(rest, dv[ax]);
+                x = (int)(rest - q * (uint32_t)ext[ax]);
+                rest = q;
+            } else {
+                x = (int)rest;
+            }
             const int64_t j = i + (x + 1 == ext[ax] ? -(int64_t)(ext[ax] - 1) * st[ax] : st[ax]);
             const Vec<NC> t = load_vec<NC>(S, j);
             double dot = 0.0;

@@ -354,7 +354,10 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     hipStream_t st = (hipStream_t)stream;
     PottsTab* dtabs = (PottsTab*)workspace;
     CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * sizeof(PottsTab), st));
-    if (block <= 0) block = 256;
+    if (block <= 0) {  // one lane per site pair of a colour, in whole waves, at most 256
+        const int64_t pairs = (g.half + 1) / 2;
+        block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (pairs + 63) / 64 * 64));
+    }
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
     if (grid <= 0) grid = n_rep;
     grid = std::min(grid, n_rep);

@@ -279,12 +279,17 @@ class Server:
         if admitted:  # initial states are written on the current stream
             torch.cuda.current_stream().synchronize()
 
-    def run(self, *, realtime: bool = True, t0: float | None = None) -> list[Job]:
+    def run(self, *, realtime: bool = True, t0: float | None = None,
+            progress: float = 0.0) -> list[Job]:
         """Serve until every submitted job has finished; returns them in completion order.
 
         With ``realtime`` the arrival times are seconds after ``t0`` (default:
-        now); otherwise every job is available at once.
+        now); otherwise every job is available at once.  ``progress`` > 0 prints
+        a status line to stderr every that many seconds.
         """
+        import sys
+
+        last = time.perf_counter()
         t0 = time.perf_counter() if t0 is None else t0
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: math.inf)
         torch.cuda.synchronize()
@@ -297,6 +302,11 @@ class Server:
                 continue
             launched = [(p, p.launch(self.slice)) for p in self.pools.values() if p.active]
             self.ticks += 1
+            if progress > 0 and time.perf_counter() - last > progress:
+                last = time.perf_counter()
+                print(f"serve: t={last - t0:.1f}s finished={len(self.finished)} active={self.n_active} "
+                      f"queued={len(self.queue) + len(self.waiting)} ticks={self.ticks}",
+                      file=sys.stderr, flush=True)
             for s in self.streams:
                 s.synchronize()
             end = clock() if realtime else time.perf_counter() - t0
