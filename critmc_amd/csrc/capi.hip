@@ -20,6 +20,10 @@ void set_error(const char* fmt, ...)
 const char* last_error() { return g_error; }
 
 namespace {
+// Pinned staging for small host->device uploads.  A ring of slots per device,
+// each guarded by the event recorded after its copy, so an upload on one
+// stream waits only for the copy that last used the same slot, not for work
+// queued on other streams.
 struct Staging {
     void* host = nullptr;
     size_t cap = 0;
@@ -27,7 +31,9 @@ struct Staging {
     bool pending = false;
 };
 constexpr int kMaxDevices = 64;
-thread_local Staging g_staging[kMaxDevices];
+constexpr int kSlots = 16;
+thread_local Staging g_staging[kMaxDevices][kSlots];
+thread_local unsigned g_next[kMaxDevices];
 }  // namespace
 
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream)
@@ -36,7 +42,7 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t st
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
-    Staging& st = g_staging[dev];
+    Staging& st = g_staging[dev][g_next[dev]++ % kSlots];
     if (st.done == nullptr) {
         e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
         if (e != hipSuccess) return e;

@@ -21,6 +21,7 @@ enum Tag : uint32_t {
     TAG_EA = 4,
     TAG_PT = 5,
     TAG_ON = 6,
+    TAG_ONHB = 8,  // O(n) heat bath (tag 7 belongs to the CPU typewriter baseline)
 };
 
 // Update codes of critmc/kernels/common.py:16-21.
@@ -67,6 +68,19 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t st
             return ::cm::CM_EINVAL;                                                     \
         }                                                                               \
     } while (0)
+
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// L2.  xcd_block maps the dispatch index w of a grid of `total` workgroups to a
+// logical block so that each XCD receives one contiguous range of logical
+// blocks; lattice rows processed by neighbouring blocks then share an L2.  The
+// map is a bijection for any dispatch order, so it never changes results.
+__host__ __device__ __forceinline__ uint32_t xcd_block(uint32_t w, uint32_t total)
+{
+    constexpr uint32_t kXcd = 8;
+    const uint32_t q = total / kXcd, r = total % kXcd;
+    const uint32_t x = w % kXcd, j = w / kXcd;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
 
 __device__ __forceinline__ int wave_sum(int v)
 {

@@ -17,6 +17,17 @@
 // normalise launch, a partial-sum launch and an in-order finalise launch, so
 // every FP64 result is reproducible run to run.
 //
+// Heat bath (:186-261), FP64 from the exact FP64 field:
+//   XY         phi = atan2(h) + von Mises(kappa = beta |h|) by Best & Fisher's
+//              rejection; attempt a of colour site k uses Philox block k with
+//              the attempt in bits 1..7 of the colour field (tag TAG_ONHB):
+//              word 0 -> z = cos(pi U), word 1 -> u2 in (0, 1], word 2's top bit
+//              -> sign.  After 128 rejections the last proposal is kept
+//              (probability < 1e-50).  kappa < 1e-8: phi uniform from word 0.
+//   Heisenberg u = 1 + ln(r + (1 - r) e^{-2 kappa}) / kappa along h, azimuth
+//              2 pi U in the reference's (h, e1, e2) frame; block k words 0, 1;
+//              |h| = 0: z = 2U - 1 and azimuth from words 2, 3.
+//
 // Uniforms (Metropolis, tag TAG_ON): colour site k of a replica uses Philox
 // block k (n = 2: words 0,1 -> Box-Muller pair, word 2 -> acceptance) or blocks
 // 2k, 2k+1 (n = 3: words 0..3 -> two Box-Muller pairs, third Gaussian from the
@@ -77,6 +88,37 @@ __device__ __forceinline__ void box_muller(uint32_t xa, uint32_t xb, float& g0, 
     g1 = r * sn;
 }
 
+__device__ __forceinline__ double unit_open0(uint32_t x) { return ((double)x + 1.0) * 0x1p-32; }
+__device__ __forceinline__ double unit_open1(uint32_t x) { return (double)x * 0x1p-32; }
+
+// Best & Fisher (1979) von Mises deviate on (-pi, pi] (critmc/kernels/on.py:186-204).
+__device__ double von_mises(double kappa, uint32_t blk, uint64_t sweep, uint32_t colour,
+                            uint32_t sid, uint32_t k0, uint32_t k1)
+{
+    if (kappa < 1e-8) {
+        const u32x4 b = philox4x32_10(u32x4{blk, (uint32_t)sweep, ctr_z(sweep, colour, TAG_ONHB), sid}, k0, k1);
+        return M_PI * (2.0 * unit_open1(b.x) - 1.0);
+    }
+    const double root = sqrt(1.0 + 4.0 * kappa * kappa);
+    const double tau = 1.0 + root;
+    const double rho = 2.0 * kappa * tau / ((root + 1.0) * (tau + sqrt(2.0 * tau)));
+    const double r = (1.0 + rho * rho) / (2.0 * rho);
+    double f = 0.0;
+    uint32_t sign = 0;
+    for (uint32_t a = 0; a < 128; ++a) {
+        const u32x4 b = philox4x32_10(
+            u32x4{blk, (uint32_t)sweep, ctr_z(sweep, colour | (a << 1), TAG_ONHB), sid}, k0, k1);
+        const double z = cos(M_PI * unit_open1(b.x));
+        f = (1.0 + r * z) / (r + z);
+        const double c = kappa * (r - f);
+        const double u2 = unit_open0(b.y);
+        sign = b.z >> 31;
+        if (c * (2.0 - c) - u2 > 0.0 || log(c / u2) + 1.0 - c >= 0.0) break;
+    }
+    const double theta = acos(fmin(1.0, fmax(-1.0, f)));
+    return sign == 0 ? theta : -theta;
+}
+
 struct OnArgs {
     float* spins;
     const float* betas;     // [R]
@@ -96,9 +138,10 @@ struct OnArgs {
 template <int NC, int Z, int CODE>
 __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 {
-    const int rep = blockIdx.y;
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x);
     if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)(lb - (uint32_t)rep * gridDim.x) * blockDim.x + threadIdx.x;
     int accepted = 0;
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
@@ -120,7 +163,54 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 #pragma unroll
         for (int c = 0; c < NC; ++c) h[c] = (float)hd[c];
         Vec<NC> s = load_vec<NC>(S, i);
-        if (CODE == CODE_OVERRELAX) {
+        if (CODE == CODE_HEATBATH) {
+            const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+            const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+            double hn2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) hn2 += hd[c] * hd[c];
+            const double hn = sqrt(hn2);
+            const double kappa = (double)a.betas[rep] * hn;
+            if constexpr (NC == 2) {
+                const double phi0 = hn > 0.0 ? atan2(hd[1], hd[0]) : 0.0;
+                const double phi = phi0 + von_mises(kappa, (uint32_t)k, sweep, (uint32_t)a.colour, sid, k0, k1);
+                s.v[0] = (float)cos(phi);
+                s.v[1] = (float)sin(phi);
+            } else {
+                const u32x4 b = philox4x32_10(
+                    u32x4{(uint32_t)k, (uint32_t)sweep, ctr_z(sweep, (uint32_t)a.colour, TAG_ONHB), sid}, k0, k1);
+                if (hn == 0.0) {
+                    const double z = 2.0 * unit_open1(b.z) - 1.0;
+                    const double az = 2.0 * M_PI * unit_open1(b.w);
+                    const double w = sqrt(fmax(0.0, 1.0 - z * z));
+                    s.v[0] = (float)(w * cos(az));
+                    s.v[1] = (float)(w * sin(az));
+                    s.v[2] = (float)z;
+                } else {
+                    const double r = unit_open0(b.x);
+                    double u = kappa < 1e-8 ? 2.0 * r - 1.0
+                                            : 1.0 + log(r + (1.0 - r) * exp(-2.0 * kappa)) / kappa;
+                    u = fmin(1.0, fmax(-1.0, u));
+                    const double hx = hd[0] / hn, hy = hd[1] / hn, hz = hd[2] / hn;
+                    const double ax = fabs(hx) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
+                    const double proj = ax * hx + ay * hy;
+                    double e1x = ax - proj * hx, e1y = ay - proj * hy, e1z = -proj * hz;
+                    const double nrm = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+                    e1x /= nrm;
+                    e1y /= nrm;
+                    e1z /= nrm;
+                    const double e2x = hy * e1z - hz * e1y, e2y = hz * e1x - hx * e1z,
+                                 e2z = hx * e1y - hy * e1x;
+                    const double phi = 2.0 * M_PI * unit_open1(b.y);
+                    const double w = sqrt(fmax(0.0, 1.0 - u * u));
+                    const double cp = cos(phi), sp = sin(phi);
+                    s.v[0] = (float)(u * hx + w * (cp * e1x + sp * e2x));
+                    s.v[1] = (float)(u * hy + w * (cp * e1y + sp * e2y));
+                    s.v[2] = (float)(u * hz + w * (cp * e1z + sp * e2z));
+                }
+            }
+            store_vec<NC>(S, i, s);
+        } else if (CODE == CODE_OVERRELAX) {
             float hh = 0.0f, sh = 0.0f;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -173,9 +263,15 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
             }
         }
     }
-    if (CODE != CODE_OVERRELAX) {
+    if (CODE == CODE_METROPOLIS) {  // one atomic per workgroup
+        __shared__ int part[4];
         const int t = wave_sum(accepted);
-        if ((threadIdx.x & 63) == 0 && t) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], t);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = part[0] + part[1] + part[2] + part[3];
+            if (tot) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], tot);
+        }
     }
 }
 
@@ -329,6 +425,8 @@ static void launch_half(int code, const OnArgs& a, hipStream_t st)
     const dim3 grid((unsigned)((a.g.half + 255) / 256), a.R);
     if (code == CODE_OVERRELAX)
         hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX>), grid, dim3(256), 0, st, a);
+    else if (code == CODE_HEATBATH)
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_HEATBATH>), grid, dim3(256), 0, st, a);
     else
         hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS>), grid, dim3(256), 0, st, a);
 }
@@ -401,10 +499,11 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
     CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
     CM_REQUIRE(g.half <= 0xFFFFFFFFll / 2, "lattice too large");
+    CM_REQUIRE((g.half + 255) / 256 * n_rep < (1ll << 32), "too many sites x replicas per launch");
     CM_REQUIRE(workspace_bytes >= cm_on_workspace_bytes(ndim, dims, n_rep, n), "workspace too small");
     for (int r = 0; r < n_rows; ++r) {
         const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
-        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_OVERRELAX,
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_OVERRELAX,
                    "update code %lld has no checkerboard O(n) kernel", (long long)code);
         CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
     }

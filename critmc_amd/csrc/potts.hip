@@ -7,9 +7,12 @@
 //                      with r < cumulative sum (k ascending), default q-1
 //   run        :170-235 per measurement E = -#satisfied bonds, counts[q], accepted
 //
-// One workgroup owns one replica at a time (grid-stride over replicas) and runs
-// the whole call's sweeps; the replica's spins sit in LDS when they fit,
-// otherwise the workgroup updates them in place in global memory.
+// Up to 96 KB of states per replica (L <= 256 in 2D): one workgroup owns one
+// replica at a time (grid-stride over replicas) and runs the whole call's
+// sweeps with the states in LDS.  Larger lattices, and small batches of
+// lattices of 16384 sites or more, take the wide form: one launch per half
+// sweep over all replicas, states in global memory.  Both forms draw the same
+// uniforms for the same site and sweep, so they give identical chains.
 //
 // Uniforms: the colour-c sites are enumerated k = 0 .. N/2-1 (row-major, see
 // cb_geom.hpp); site k uses words 2(k&1) and 2(k&1)+1 of the Philox block
@@ -35,6 +38,8 @@ struct PottsTab {
 
 constexpr int POTTS_QMAX = 127;
 constexpr int64_t POTTS_LDS_STATE = 96 * 1024;
+constexpr int POTTS_WIDE_REPS = 128;        // below this many replicas ...
+constexpr int64_t POTTS_WIDE_SITES = 16384;  // ... lattices this large go wide
 
 template <int Z>
 __device__ __forceinline__ void potts_site(int8_t* __restrict__ S, int64_t i,
@@ -194,6 +199,96 @@ __global__ __launch_bounds__(256) void potts_batched(
         }
         __syncthreads();
     }
+}
+
+// Wide form for lattices too large for LDS: one launch per half sweep over
+// (site-pair chunks x replicas), states in global memory.  Energy, accepted and
+// occupation deltas are reduced per workgroup and added with integer atomics,
+// so every sum is exact; potts_prefix turns them into running values.
+template <int Z>
+__global__ __launch_bounds__(256) void potts_half_wide(
+    int8_t* __restrict__ spins, CbGeo g, int q, int code, int colour, uint64_t sweep_add,
+    const PottsTab* __restrict__ tabs, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
+    int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
+    int32_t* __restrict__ acc_out, int32_t* __restrict__ dcounts_out)
+{
+    __shared__ int counts[128];
+    __shared__ int red[2 * 4];
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x);
+    const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;  // whole block
+    for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = 0;
+    __syncthreads();
+    int8_t* S = spins + (int64_t)rep * g.N;
+    const PottsTab* tb = tabs + rep;
+    const uint64_t sweep = sweeps[rep] + sweep_add;
+    const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
+    const int64_t npairs = (g.half + 1) >> 1;
+    const int64_t pi = (int64_t)bx * blockDim.x + threadIdx.x;
+    int de = 0, acc = 0;
+    if (pi < npairs) {
+        const u32x4 rn = philox4x32_10(
+            u32x4{(uint32_t)pi, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_POTTS), sid}, k0, k1);
+        int64_t nbi[Z];
+        int64_t i = cb_site<Z>(g, 2 * pi, colour, nbi);
+        potts_site<Z>(S, i, nbi, q, code, rn.x, rn.y, tb, de, acc, counts);
+        if (2 * pi + 1 < g.half) {
+            i = cb_site<Z>(g, 2 * pi + 1, colour, nbi);
+            potts_site<Z>(S, i, nbi, q, code, rn.z, rn.w, tb, de, acc, counts);
+        }
+    }
+    de = wave_sum(de);
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = de;
+        red[2 * (threadIdx.x >> 6) + 1] = acc;
+    }
+    __syncthreads();
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    if (threadIdx.x == 0) {
+        int tde = 0, tac = 0;
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) {
+            tde += red[2 * w];
+            tac += red[2 * w + 1];
+        }
+        if (tde) atomicAdd(&de_out[o], tde);
+        if (tac) atomicAdd(&acc_out[o], tac);
+    }
+    for (int k = threadIdx.x; k < q; k += blockDim.x)
+        if (counts[k]) atomicAdd(&dcounts_out[o * q + k], counts[k]);
+}
+
+__global__ void potts_zero(int32_t* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Deltas -> running E and counts per replica (one thread per replica).
+__global__ void potts_prefix(int32_t* __restrict__ e_out, int32_t* __restrict__ counts_out,
+                             int64_t n_meas, int32_t* __restrict__ e_run,
+                             int32_t* __restrict__ counts_run, int q, int R,
+                             const int64_t* __restrict__ n_meas_rep, uint64_t* __restrict__ sweeps,
+                             uint64_t per_meas)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= R) return;
+    const int64_t n = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
+    int32_t e = e_run[rep];
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t o = (int64_t)rep * n_meas + k;
+        e += e_out[o];
+        e_out[o] = e;
+        for (int c = 0; c < q; ++c) {
+            counts_run[(int64_t)rep * q + c] += counts_out[o * q + c];
+            counts_out[o * q + c] = counts_run[(int64_t)rep * q + c];
+        }
+    }
+    e_run[rep] = e;
+    sweeps[rep] += per_meas * (uint64_t)n;
 }
 
 __global__ void potts_init_kernel(int8_t* __restrict__ s, int64_t N, int R, int q,
@@ -359,6 +454,44 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
         block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (pairs + 63) / 64 * 64));
     }
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
+    // grid < 0 asks for the wide form, grid > 0 for one workgroup per replica;
+    // 0 picks the wide form when the states do not fit LDS or when there are
+    // too few replicas of a large lattice to give every CU a workgroup.
+    const bool wide = grid < 0 || (grid == 0 && (g.N > POTTS_LDS_STATE ||
+                                                 (n_rep < POTTS_WIDE_REPS && g.N >= POTTS_WIDE_SITES)));
+    if (wide) {
+        // wide form: the whole GPU on each half sweep of every replica
+        CM_REQUIRE(n_rep <= 65535, "n_rep out of range");
+        CM_REQUIRE((g.half + 511) / 512 * n_rep < (1ll << 32), "too many sites x replicas per launch");
+        const int64_t total = (int64_t)n_rep * n_meas;
+        const int zb = (int)std::min<int64_t>((total * (q + 2) + 255) / 256, 8192);
+        hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, e_out, total);
+        hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, acc_out, total);
+        hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, counts_out, total * q);
+        const int64_t npairs = (g.half + 1) / 2;
+        const dim3 wgrid((unsigned)((npairs + 255) / 256), n_rep);
+        uint64_t sw = 0;
+        int64_t per_meas = 0;
+        for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];
+        for (int64_t meas = 0; meas < n_meas; ++meas)
+            for (int r = 0; r < sc.n; ++r)
+                for (int rp = 0; rp < sc.count[r]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        if (z == 2)
+                            hipLaunchKernelGGL(potts_half_wide<2>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                        else if (z == 4)
+                            hipLaunchKernelGGL(potts_half_wide<4>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                        else
+                            hipLaunchKernelGGL(potts_half_wide<6>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                    }
+                    ++sw;
+                }
+        hipLaunchKernelGGL(potts_prefix, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out,
+                           counts_out, n_meas, e_run, counts_run, q, n_rep, n_meas_rep, sweeps,
+                           (uint64_t)per_meas);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    }
     if (grid <= 0) grid = n_rep;
     grid = std::min(grid, n_rep);
     hipError_t err;

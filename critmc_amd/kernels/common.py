@@ -35,5 +35,5 @@ N_AUX = 4
 GPU_UPDATES = {
     "ising": ("metropolis", "heatbath"),
     "potts": ("metropolis", "heatbath"),
-    "on": ("metropolis", "overrelax"),
+    "on": ("metropolis", "heatbath", "overrelax"),
 }

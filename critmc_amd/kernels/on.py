@@ -15,7 +15,7 @@ import torch
 from .._lib import check, lib
 from ..device import device, host_ptr, ptr, stream
 from .chains import Chains
-from .common import METROPOLIS, OVERRELAX
+from .common import HEATBATH, METROPOLIS, OVERRELAX
 
 
 def _dims(shape) -> np.ndarray:
@@ -43,7 +43,7 @@ def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedu
     R = chains.R
     sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
     for code in sched[:, 0]:
-        if code not in (METROPOLIS, OVERRELAX):
+        if code not in (METROPOLIS, HEATBATH, OVERRELAX):
             raise ValueError(f"update code {int(code)} has no checkerboard O(n) kernel")
     dv = spins.device
     d = _dims(shape)

@@ -85,7 +85,11 @@ class _Pool:
         self.cap = 0
         self.jobs: list[Job | None] = []
         self.free: list[int] = []
-        self._grow(8)
+        self.pending: list[Job] = []      # admitted, joining at the next slice boundary
+        self.inflight = None              # caps of the slice on the device
+        self.done_event = torch.cuda.Event()
+        with torch.cuda.stream(self.stream):
+            self._grow(8)
 
     # -- storage --------------------------------------------------------------
     def _alloc(self, cap: int):
@@ -122,6 +126,10 @@ class _Pool:
         self.chains = None
 
     def add(self, job: Job):
+        with torch.cuda.stream(self.stream):
+            self._add(job)
+
+    def _add(self, job: Job):
         if not self.free:
             self._grow(2 * self.cap)
         slot = min(self.free)
@@ -204,6 +212,8 @@ class _Pool:
                 if j is not None and caps[s]:
                     for k, v in rec.items():
                         j.series[k][j.done: j.done + caps[s]].copy_(v[s, : caps[s]])
+            self.done_event.record(self.stream)
+        self.inflight = caps
         return caps
 
 
@@ -264,66 +274,75 @@ class Server:
         while self.queue and self.queue[0][0] <= now:
             self.waiting.append(heapq.heappop(self.queue)[2])
         still = []
-        admitted = False
         for job in self.waiting:
             if self.n_active < self.max_active and self.used + job.footprint() <= self.budget:
                 self.used += job.footprint()
                 self.peak_used = max(self.peak_used, self.used)
-                self._pool(job).add(job)
+                self._pool(job).pending.append(job)
                 job.admitted_at = now
                 self.n_active += 1
-                admitted = True
             else:
                 still.append(job)
         self.waiting = still
-        if admitted:  # initial states are written on the current stream
-            torch.cuda.current_stream().synchronize()
+
+    def _retire(self, pool: _Pool, now: float):
+        caps, pool.inflight = pool.inflight, None
+        for slot, job in enumerate(pool.jobs):
+            if job is None or not caps[slot]:
+                continue
+            job.done += int(caps[slot])
+            self.site_updates += int(caps[slot]) * job.sites
+            if job.done >= job.sweeps:
+                job.finished_at = now
+                pool.remove(slot)
+                self.used -= job.footprint()
+                self.n_active -= 1
+                self.finished.append(job)
 
     def run(self, *, realtime: bool = True, t0: float | None = None,
             progress: float = 0.0) -> list[Job]:
         """Serve until every submitted job has finished; returns them in completion order.
 
-        With ``realtime`` the arrival times are seconds after ``t0`` (default:
-        now); otherwise every job is available at once.  ``progress`` > 0 prints
-        a status line to stderr every that many seconds.
+        Pools run independently: whenever a pool's slice has completed on the
+        device, its finished jobs are retired, its newly admitted jobs join,
+        and its next slice is launched, while other pools keep running.  With
+        ``realtime`` the arrival times are seconds after ``t0`` (default: now);
+        otherwise every job is available at once.  ``progress`` > 0 prints a
+        status line to stderr every that many seconds.
         """
         import sys
 
-        last = time.perf_counter()
         t0 = time.perf_counter() if t0 is None else t0
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: math.inf)
+        stamp = lambda: time.perf_counter() - t0  # noqa: E731
+        last = time.perf_counter()
         torch.cuda.synchronize()
         while self.queue or self.waiting or self.n_active:
-            now = clock()
-            self._admit(now)
-            if not self.n_active:
-                if self.queue:
-                    time.sleep(max(0.0, min(self.queue[0][0] - clock(), 0.01)))
-                continue
-            launched = [(p, p.launch(self.slice)) for p in self.pools.values() if p.active]
-            self.ticks += 1
+            self._admit(clock())
+            busy = False
+            for pool in self.pools.values():
+                if pool.inflight is not None:
+                    if not pool.done_event.query():
+                        busy = True
+                        continue
+                    self._retire(pool, stamp())
+                for job in pool.pending:
+                    pool.add(job)
+                pool.pending = []
+                if pool.active and pool.launch(self.slice) is not None:
+                    self.ticks += 1
+                    busy = True
             if progress > 0 and time.perf_counter() - last > progress:
                 last = time.perf_counter()
                 print(f"serve: t={last - t0:.1f}s finished={len(self.finished)} active={self.n_active} "
-                      f"queued={len(self.queue) + len(self.waiting)} ticks={self.ticks}",
+                      f"queued={len(self.queue) + len(self.waiting)} launches={self.ticks}",
                       file=sys.stderr, flush=True)
-            for s in self.streams:
-                s.synchronize()
-            end = clock() if realtime else time.perf_counter() - t0
-            for pool, caps in launched:
-                if caps is None:
-                    continue
-                for slot, job in enumerate(pool.jobs):
-                    if job is None or not caps[slot]:
-                        continue
-                    job.done += int(caps[slot])
-                    self.site_updates += int(caps[slot]) * job.sites
-                    if job.done >= job.sweeps:
-                        job.finished_at = end
-                        pool.remove(slot)
-                        self.used -= job.footprint()
-                        self.n_active -= 1
-                        self.finished.append(job)
+            if not busy:
+                if self.queue and not self.waiting and realtime:
+                    time.sleep(max(0.0, min(self.queue[0][0] - clock(), 0.002)))
+            else:
+                time.sleep(20e-6)
+        torch.cuda.synchronize()
         return self.finished
 
 

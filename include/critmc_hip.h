@@ -125,7 +125,10 @@ int cm_potts_measure(const void *spins_dev, int32_t ndim, const int32_t *dims, i
  * and counts [n_rep][n_meas][q] after every application.
  * Replaces critmc/kernels/potts.py:170-235 (run) for checkerboard sweeps.
  * e_run_dev [n_rep] and counts_run_dev [n_rep][q] are the running totals (in/out);
- * other arguments as cm_ising_run (batched form). */
+ * grid 0 picks the form, grid > 0 runs one workgroup per replica (states in
+ * LDS up to 96 KB), grid < 0 the wide form (one launch per half sweep over all
+ * replicas); every form gives the same chains.
+ * Other arguments as cm_ising_run (batched form). */
 int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
                  const double *betas, const uint32_t *keys_dev, const uint32_t *stream_ids_dev,
                  uint64_t *sweeps_dev, const int64_t *schedule, int32_t n_rows, int64_t n_meas,

@@ -314,7 +314,7 @@ int orc_ising_typewriter_batch(int8_t *s, int R, int ndim, const int32_t *dims,
                 int accept = T >> 32 ? 1 : 0;
                 if (!accept) {
                     if (used == 4) {
-                        orc_draw_block(keys[2 * r], keys[2 * r + 1], (uint32_t)r, 0, 0, 7, blk++,
+                        orc_draw_block(keys[2 * r], keys[2 * r + 1], (uint32_t)r, 0, 0, ORC_TAG_TYPEWRITER, blk++,
                                        buf);
                         used = 0;
                     }

@@ -6,6 +6,12 @@
  * Restated from the reference (critmc/kernels/on.py):
  *   Metropolis :112-137, over-relaxation :140-183, normalize :78-87,
  *   energy / magnetization / stiffness_sums :35-76, run loop :348-418.
+ * Heat bath (:186-261) in FP64, uniforms at tag ORC_TAG_ONHB: XY attempt a of
+ * colour site k draws block k with colour field colour | a << 1 (word 0 ->
+ * z = cos(pi U), word 1 -> u2 = (x+1)/2^32, top bit of word 2 -> sign), at most
+ * 128 attempts, the last proposal kept if all reject; kappa < 1e-8 gives
+ * phi = pi (2U - 1) from word 0 of attempt 0.  Heisenberg: block k, r = (x0+1)/2^32,
+ * azimuth 2 pi x1/2^32; |h| = 0: z = 2 x2/2^32 - 1, azimuth from x3.
  * Uniforms: colour site k uses Philox block k (n = 2) or blocks 2k, 2k+1
  * (n = 3) at (sweep, colour, TAG_ON, stream); Box-Muller with
  * u1 = (x+1)/2^32, u2 = x/2^32: g0 = sqrt(-2 ln u1) cos(2 pi u2),
@@ -27,6 +33,80 @@ static void orc_bm(uint32_t xa, uint32_t xb, double *g0, double *g1)
     *g1 = r * sin(2.0 * M_PI * u2);
 }
 
+static double orc_u0(uint32_t x) { return ((double)x + 1.0) * 0x1p-32; }
+static double orc_u1(uint32_t x) { return (double)x * 0x1p-32; }
+
+/* Best & Fisher von Mises deviate (critmc/kernels/on.py:186-204). */
+static double orc_von_mises(double kappa, uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep,
+                            int colour, uint32_t blk)
+{
+    uint32_t b[4];
+    if (kappa < 1e-8) {
+        orc_draw_block(k0, k1, sid, sweep, colour, ORC_TAG_ONHB, blk, b);
+        return M_PI * (2.0 * orc_u1(b[0]) - 1.0);
+    }
+    double root = sqrt(1.0 + 4.0 * kappa * kappa);
+    double tau = 1.0 + root;
+    double rho = 2.0 * kappa * tau / ((root + 1.0) * (tau + sqrt(2.0 * tau)));
+    double r = (1.0 + rho * rho) / (2.0 * rho);
+    double f = 0.0;
+    uint32_t sign = 0;
+    for (int a = 0; a < 128; ++a) {
+        orc_draw_block(k0, k1, sid, sweep, colour | (a << 1), ORC_TAG_ONHB, blk, b);
+        double z = cos(M_PI * orc_u1(b[0]));
+        f = (1.0 + r * z) / (r + z);
+        double c = kappa * (r - f);
+        double u2 = orc_u0(b[1]);
+        sign = b[2] >> 31;
+        if (c * (2.0 - c) - u2 > 0.0 || log(c / u2) + 1.0 - c >= 0.0) break;
+    }
+    double theta = acos(fmin(1.0, fmax(-1.0, f)));
+    return sign == 0 ? theta : -theta;
+}
+
+/* Conditional sample of spin si given field h (critmc/kernels/on.py:207-261). */
+static void orc_on_heatbath_site(double *si, const double *h, int n, double beta, uint32_t k0,
+                                 uint32_t k1, uint32_t sid, uint64_t sweep, int colour, uint32_t k)
+{
+    double hn2 = 0.0;
+    for (int c = 0; c < n; ++c) hn2 += h[c] * h[c];
+    double hn = sqrt(hn2);
+    double kappa = beta * hn;
+    if (n == 2) {
+        double phi0 = hn > 0.0 ? atan2(h[1], h[0]) : 0.0;
+        double phi = phi0 + orc_von_mises(kappa, k0, k1, sid, sweep, colour, k);
+        si[0] = cos(phi);
+        si[1] = sin(phi);
+        return;
+    }
+    uint32_t b[4];
+    orc_draw_block(k0, k1, sid, sweep, colour, ORC_TAG_ONHB, k, b);
+    if (hn == 0.0) {
+        double z = 2.0 * orc_u1(b[2]) - 1.0, az = 2.0 * M_PI * orc_u1(b[3]);
+        double w = sqrt(fmax(0.0, 1.0 - z * z));
+        si[0] = w * cos(az);
+        si[1] = w * sin(az);
+        si[2] = z;
+        return;
+    }
+    double r = orc_u0(b[0]);
+    double u = kappa < 1e-8 ? 2.0 * r - 1.0 : 1.0 + log(r + (1.0 - r) * exp(-2.0 * kappa)) / kappa;
+    u = fmin(1.0, fmax(-1.0, u));
+    double hx = h[0] / hn, hy = h[1] / hn, hz = h[2] / hn;
+    double ax = fabs(hx) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
+    double proj = ax * hx + ay * hy;
+    double e1[3] = {ax - proj * hx, ay - proj * hy, -proj * hz};
+    double nrm = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    for (int c = 0; c < 3; ++c) e1[c] /= nrm;
+    double e2[3] = {hy * e1[2] - hz * e1[1], hz * e1[0] - hx * e1[2], hx * e1[1] - hy * e1[0]};
+    double phi = 2.0 * M_PI * orc_u1(b[1]);
+    double w = sqrt(fmax(0.0, 1.0 - u * u));
+    double cp = cos(phi), sp = sin(phi);
+    si[0] = u * hx + w * (cp * e1[0] + sp * e2[0]);
+    si[1] = u * hy + w * (cp * e1[1] + sp * e2[1]);
+    si[2] = u * hz + w * (cp * e1[2] + sp * e2[2]);
+}
+
 /* One half sweep; returns accepted moves (Metropolis) or 0. */
 int64_t orc_on_half_sweep(double *s, int ndim, const int32_t *dims, int n, int code, int colour,
                           double beta, double step, uint32_t k0, uint32_t k1, uint32_t sid,
@@ -42,7 +122,9 @@ int64_t orc_on_half_sweep(double *s, int ndim, const int32_t *dims, int n, int c
         for (int j = 0; j < z; ++j)
             for (int c = 0; c < n; ++c) h[c] += s[nb[j] * n + c];
         double *si = s + i * n;
-        if (code == 5) {
+        if (code == 2) {
+            orc_on_heatbath_site(si, h, n, beta, k0, k1, sid, sweep, colour, (uint32_t)k);
+        } else if (code == 5) {
             double hh = 0, sh = 0;
             for (int c = 0; c < n; ++c) {
                 hh += h[c] * h[c];
@@ -127,7 +209,7 @@ int orc_on_run(double *s, int ndim, const int32_t *dims, int n, double beta, dou
         int64_t acc = 0;
         for (int r = 0; r < nrows; ++r) {
             int code = (int)sched[2 * r];
-            if (code != 0 && code != 5) return -3;
+            if (code != 0 && code != 2 && code != 5) return -3;
             for (int64_t rp = 0; rp < sched[2 * r + 1]; ++rp) {
                 for (int c = 0; c < 2; ++c) {
                     int64_t a = orc_on_half_sweep(s, ndim, dims, n, code, c, beta, step, k0, k1, sid, *sweep);

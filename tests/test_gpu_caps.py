@@ -1,0 +1,96 @@
+"""Per-replica measurement caps (continuous batching) against the CPU replay.
+
+A launch of n_meas measurements with caps c[r] must leave replica r exactly
+where c[r] measurements alone would: same state, same first c[r] series
+entries, sweep counter advanced by c[r] schedules, running totals updated.
+Replicas with cap 0 are untouched.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from critmc_amd.kernels import ising as KI
+from critmc_amd.kernels import potts as KP
+from critmc_amd.kernels.chains import Chains
+
+pytestmark = pytest.mark.gpu
+
+
+def _chains(R, seed, dev):
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    sids = rng.integers(0, 1 << 20, size=R, dtype=np.uint64)
+    sweeps = rng.integers(0, 1 << 36, size=R, dtype=np.uint64)
+    return Chains(keys, sids, sweeps, dev), keys, sids, sweeps
+
+
+CAPS = [3, 0, 6, 1, 5]
+
+
+@pytest.mark.parametrize("layout", ["int8", "packed"])
+def test_ising_caps_match_replay(layout):
+    dev = torch.device("cuda", 0)
+    shape, R, n_meas = (64, 64), len(CAPS), max(CAPS)
+    chains, keys, sids, sw0 = _chains(R, 11, dev)
+    n = shape[0] * shape[1]
+    s = torch.empty((R, n), dtype=torch.int8, device=dev)
+    KI.init_spins(s, shape, chains, "random")
+    s0 = s.cpu().numpy()
+    e, m = KI.measure(s, shape, R)
+    betas = np.linspace(0.3, 0.6, R)
+    sched = np.array([[0, 1], [2, 2]])
+    caps = torch.tensor(CAPS, dtype=torch.int64, device=dev)
+    lay = KI.LAYOUT_PACKED if layout == "packed" else KI.LAYOUT_INT8
+    st = KI.pack(s, shape, R) if lay == KI.LAYOUT_PACKED else s
+    e_out, m_out, acc = KI.run(st, shape, betas, 0.0, chains, sched, n_meas, e, m, layout=lay,
+                               form=KI.FORM_BATCHED, n_meas_rep=caps)
+    if lay == KI.LAYOUT_PACKED:
+        s = KI.unpack(st, shape, R)
+    got = s.cpu().numpy()
+    e_out, m_out, acc = (t.cpu().numpy() for t in (e_out, m_out, acc))
+    sweeps = chains.sweeps_d.cpu().numpy().view(np.uint64)
+    for r, c in enumerate(CAPS):
+        rep = oracle.IsingReplay(s0[r], shape, betas[r], 0.0, keys[r], int(sids[r]), int(sw0[r]))
+        ref = rep.run(sched, c) if c else None
+        assert np.array_equal(got[r], rep.s), f"replica {r}"
+        if c:
+            assert np.array_equal(e_out[r, :c], ref["Eb"])
+            assert np.array_equal(m_out[r, :c], ref["M"])
+            assert np.array_equal(acc[r, :c], ref["accepted"])
+            assert int(e[r]) == int(ref["Eb"][-1]) and int(m[r]) == int(ref["M"][-1])
+        assert int(sweeps[r]) == int(sw0[r]) + 3 * c
+
+
+@pytest.mark.parametrize("shape,grid", [((32, 32), 2), ((128, 128), -1), ((4, 4, 6), -1)],
+                         ids=["per-replica", "wide-2d", "wide-3d"])
+def test_potts_caps_match_replay(shape, grid):
+    dev = torch.device("cuda", 0)
+    q, R, n_meas = 5, len(CAPS), max(CAPS)
+    chains, keys, sids, sw0 = _chains(R, 12, dev)
+    n = int(np.prod(shape))
+    s = torch.empty((R, n), dtype=torch.int8, device=dev)
+    KP.init_states(s, shape, q, chains, "random")
+    s0 = s.cpu().numpy()
+    e, cnt = KP.measure(s, shape, q, R)
+    betas = np.linspace(0.8, 1.4, R)
+    sched = np.array([[0, 1], [2, 1]])
+    caps = torch.tensor(CAPS, dtype=torch.int64, device=dev)
+    e_out, acc, counts = KP.run(s, shape, q, betas, chains, sched, n_meas, e, cnt, grid=grid,
+                                n_meas_rep=caps)
+    got = s.cpu().numpy()
+    e_out, acc, counts = (t.cpu().numpy() for t in (e_out, acc, counts))
+    sweeps = chains.sweeps_d.cpu().numpy().view(np.uint64)
+    e_h, cnt_h = e.cpu().numpy(), cnt.cpu().numpy()
+    for r, c in enumerate(CAPS):
+        rep = oracle.PottsReplay(s0[r], shape, q, betas[r], keys[r], int(sids[r]), int(sw0[r]))
+        ref = rep.run(sched, c) if c else None
+        assert np.array_equal(got[r], rep.s), f"replica {r}"
+        if c:
+            assert np.array_equal(e_out[r, :c], ref["E"])
+            assert np.array_equal(acc[r, :c], ref["accepted"])
+            assert np.array_equal(counts[r, :c], ref["counts"])
+            assert int(e_h[r]) == int(ref["E"][-1])
+            assert np.array_equal(cnt_h[r], ref["counts"][-1])
+        assert int(sweeps[r]) == int(sw0[r]) + 2 * c

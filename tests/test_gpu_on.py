@@ -6,6 +6,10 @@ conditioning-bound exceptions:
   FP32 error of exp(-beta dE); a proposal S + step g of tiny length (|.| <
   ~0.03, ~1e-4 of the proposals) turns the ~1e-7 rounding of the FP32
   Gaussian into an angle above 1e-5.  Their number is bounded.
+* Heat bath (FP64 sampling from the FP64 field, stored FP32): the same
+  conditioning as over-relaxation, angle_error * |h| < 1e-5, except where the
+  ~1e-7 field difference flips a rejection decision; those are bounded like
+  Metropolis flips.
 * Over-relaxation: the second colour reflects about fields built from
   first-colour spins stored in FP32 (6e-8 rounding); the reflection axis is
   then uncertain by ~2 delta_h / |h| with delta_h ~ 1e-7.  Every spin beyond
@@ -35,7 +39,7 @@ def _angle_err(a, b):
 
 
 @pytest.mark.parametrize("n,shape", [(2, (512, 512)), (2, (64, 32)), (3, (16, 16, 16)), (2, (10,))])
-@pytest.mark.parametrize("code", [0, 5])
+@pytest.mark.parametrize("code", [0, 2, 5])
 def test_one_sweep_within_1e5_rad(n, shape, code):
     dev = torch.device("cuda", 0)
     R = 2
@@ -56,13 +60,17 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
         ref = rep.run([[code, 1]], 1)
         err = _angle_err(got[r], rep.s)
         bad = err > 1e-5
-        assert err.max() < 1e-3 or code == 0, err.max()
-        if code == 5:
+        assert err.max() < 1e-3 or code != 5, err.max()
+        if code in (2, 5):
             from critmc_amd import hypercubic
 
             lat = hypercubic(shape)
             hmag = np.linalg.norm(rep.s[lat.nbr].sum(axis=1), axis=1)
-            assert np.all(err[bad] * hmag[bad] < 1e-5), (bad.sum(), err.max(), (err * hmag)[bad].max())
+            cond = err * hmag < 1e-5
+            if code == 5:
+                assert np.all(cond[bad]), (bad.sum(), err.max(), (err * hmag)[bad].max())
+            else:  # rejection decisions flipped by the FP32-stored neighbours
+                assert (bad & ~cond).sum() <= int(5e-4 * N) + 5, ((bad & ~cond).sum(), err.max())
             limit = int(bad.sum())
         else:
             # decision flips plus ill-conditioned proposals
@@ -87,14 +95,15 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
 
 
 @pytest.mark.parametrize("n_comp", [2, 3])
-@pytest.mark.parametrize("updates", ["metropolis", "metropolis+3*overrelax"])
+@pytest.mark.parametrize("updates", ["metropolis", "metropolis+3*overrelax", "heatbath",
+                                     "heatbath+2*overrelax"])
 def test_ring_matches_transfer_operator(n_comp, updates):
     from critmc_amd import ON, Simulation, chain
 
     beta, n_sites = 1.1, 8
     sim = Simulation(ON(chain(n_sites), n_comp), beta, updates, seed=5)
     info = sim.thermalize(2000)
-    assert 1e-3 <= info["step"] <= 10
+    assert 1e-3 <= info.get("step", 1.0) <= 10
     ts = sim.run(100_000)
     mean, err = batch_mean_error(ts["e"])
     z = (mean - exact.on_ring_energy(beta, n_sites, n_comp)) / err

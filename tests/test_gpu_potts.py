@@ -22,7 +22,8 @@ def _chains(R, seed, dev):
 @pytest.mark.parametrize("shape,q", [((4, 4), 3), ((32, 32), 10), ((16, 6), 2), ((10,), 4),
                                      ((4, 4, 6), 5), ((512, 512), 10), ((6, 4), 127)])
 @pytest.mark.parametrize("sched", [[[0, 1]], [[2, 1]], [[0, 1], [2, 2]]])
-def test_potts_run_is_bit_exact_against_replay(shape, q, sched):
+@pytest.mark.parametrize("grid", [0, 2, -1], ids=["auto", "per-replica", "wide"])
+def test_potts_run_is_bit_exact_against_replay(shape, q, sched, grid):
     dev = torch.device("cuda", 0)
     R = 1 if shape == (512, 512) else 3
     chains, keys, sids, sw0 = _chains(R, q + sum(shape), dev)
@@ -35,7 +36,7 @@ def test_potts_run_is_bit_exact_against_replay(shape, q, sched):
     e, c = P.measure(s, shape, q, R)
     betas = np.random.default_rng(q).uniform(0.5, 1.6, size=R)
     n_meas = 2 if shape == (512, 512) else 6
-    e_out, acc, counts = P.run(s, shape, q, betas, chains, np.array(sched), n_meas, e, c)
+    e_out, acc, counts = P.run(s, shape, q, betas, chains, np.array(sched), n_meas, e, c, grid=grid)
     got = s.cpu().numpy()
     e_out, acc, counts = (t.cpu().numpy() for t in (e_out, acc, counts))
     for r in range(R):

@@ -3,7 +3,11 @@
 Pinned by: O(n) rings against the transfer operator (critmc tests/
 test_kernels.py:95-109, here with checkerboard order), over-relaxation
 invariants (energy conserved, twice = identity, zero field = no change,
-critmc tests/test_overrelax_xy.py), and a planted wrong beta.
+critmc tests/test_overrelax_xy.py), and a planted wrong beta.  The heat bath
+(critmc/kernels/on.py:186-261) is pinned by its conditional law: on a 2-site
+ring each update draws the relative angle afresh, so the samples are i.i.d.
+von Mises (XY) or exponential in cos (Heisenberg) with kappa = 2 beta, checked
+by Kolmogorov-Smirnov tests that also reject a kappa 15 % off.
 """
 
 import numpy as np
@@ -54,7 +58,7 @@ def _ring(oracle_lib, n_comp, sched, beta, n_meas, seed, beta_true=None):
 
 
 @pytest.mark.parametrize("n_comp", [2, 3])
-@pytest.mark.parametrize("sched", [[[0, 1]], [[0, 1], [5, 3]]])
+@pytest.mark.parametrize("sched", [[[0, 1]], [[0, 1], [5, 3]], [[2, 1]], [[2, 1], [5, 2]]])
 def test_ring_matches_transfer_operator(oracle_lib, n_comp, sched):
     z = _ring(oracle_lib, n_comp, sched, 1.1, 100_000, seed=5 + n_comp)
     assert abs(z) < 4.5, z
@@ -63,3 +67,43 @@ def test_ring_matches_transfer_operator(oracle_lib, n_comp, sched):
 def test_ring_planted_wrong_beta_is_caught(oracle_lib):
     z = _ring(oracle_lib, 2, [[0, 1]], 1.1 * 0.85, 100_000, seed=3, beta_true=1.1)
     assert abs(z) > 4.5, z
+
+
+def _pair_samples(oracle_lib, n_comp, beta, n, seed):
+    """Relative angle (XY) or cosine (Heisenberg) of a 2-site ring after each heat-bath sweep."""
+    rep = oracle_lib.OnReplay(_random_state(2, n_comp, seed), (2,), beta, 1.0, (seed, 11))
+    out = np.empty(n)
+    for t in range(n):
+        rep.run([[2, 1]], 1)
+        a, b = rep.s
+        if n_comp == 2:
+            out[t] = np.arctan2(a[0] * b[1] - a[1] * b[0], a @ b)
+        else:
+            out[t] = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+    return out
+
+
+@pytest.mark.parametrize("beta", [0.7, 1.6, 1e-10])
+def test_xy_heatbath_draws_von_mises(oracle_lib, beta):
+    from scipy import stats
+
+    kappa = 2.0 * beta
+    x = _pair_samples(oracle_lib, 2, beta, 20_000, seed=int(beta * 100) + 1)
+    assert stats.kstest(x, stats.vonmises(kappa).cdf).pvalue > 1e-3
+    if beta > 0.1:  # the test has the power to see a wrong kappa
+        assert stats.kstest(x, stats.vonmises(0.85 * kappa).cdf).pvalue < 1e-6
+    else:
+        assert stats.kstest(x, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 1e-3
+
+
+@pytest.mark.parametrize("beta", [0.5, 1.3])
+def test_heisenberg_heatbath_draws_exponential_cosine(oracle_lib, beta):
+    from scipy import stats
+
+    def cdf(k):
+        return lambda u: np.expm1(k * (u + 1)) / np.expm1(2 * k)
+
+    kappa = 2.0 * beta
+    u = _pair_samples(oracle_lib, 3, beta, 20_000, seed=int(beta * 10) + 2)
+    assert stats.kstest(u, cdf(kappa)).pvalue > 1e-3
+    assert stats.kstest(u, cdf(0.85 * kappa)).pvalue < 1e-6
