@@ -58,9 +58,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_ea_workspace_bytes": ([_c_i32, _c_i32], _c_i64),
     "cm_det_exp": ([_c_f64], _c_f64),
     "cm_ea_init": ([_vp, _vp, _vp, _c_i32, _c_i32, _c_i32, _vp, _vp], _c_i32),
-    "cm_ea_energy": ([_vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_i32, _vp, _vp], _c_i32),
+    "cm_ea_energy": ([_vp, _vp, _c_i32, _vp, _vp, _vp, _c_i32, _c_i32, _c_i32, _vp, _vp], _c_i32),
     "cm_ea_pt_run": (
-        [_vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_i32, _i32p, _f64p, _vp, _vp, _vp, _vp, _vp,
+        [_vp, _vp, _c_i32, _vp, _vp, _vp, _c_i32, _c_i32, _c_i32, _i32p, _f64p, _vp, _vp, _vp, _vp, _vp,
          _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _vp, _vp, _c_i32, _c_i32, _c_u64, _c_i64,
          _vp, _c_i64, _c_i32, _vp],
         _c_i32,

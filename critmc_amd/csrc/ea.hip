@@ -1,5 +1,6 @@
-// Edwards-Anderson +-J spin glass with parallel tempering, batched over disorder
-// samples of uneven size.
+// Edwards-Anderson spin glass with parallel tempering, batched over disorder
+// samples of uneven size; +-J couplings (int8, exact int32 energies) or
+// Gaussian couplings (FP64 couplings, fields and energies).
 //
 // Reference (critmc/kernels/glass.py, critmc/glass.py):
 //   sweep     glass.py:41-54   h = sum_k J_ik s_k, dE = 2 s h; Metropolis accept iff
@@ -11,8 +12,8 @@
 //   measure   glass.py:56-77, 153-163  q, q_l, |q(k_min)|^2, log-binned accumulators
 //
 // Layout: sample s owns int8 spins [2 sets][m configurations][N_s] at spin_off[s]
-// and int8 forward-bond couplings [d][N_s] at j_off[s] (bond a*N + i joins i and
-// its +e_a neighbour, critmc/lattice.py:136-138).  perm/kofc/labels/energies/
+// and forward-bond couplings [d][N_s] (int8 +-1 or FP64) at j_off[s] (bond
+// a*N + i joins i and its +e_a neighbour, critmc/lattice.py:136-138).  perm/kofc/labels/energies/
 // sweep counters are [S][2][m] (configuration-indexed except perm, which maps
 // temperature -> configuration).
 //
@@ -26,6 +27,14 @@
 // (step, 0, TAG_PT, stream r).  The swap test compares u = x 2^-32 with
 // det_exp(x), a fixed sequence of IEEE operations (Cody-Waite reduction,
 // degree-13 Taylor polynomial in fma form) evaluated identically on the host.
+//
+// Gaussian couplings: h = sum_k J_ik s_k is added in the reference's column
+// order (+x, -x, +y, ...), one FP64 addition per neighbour; Metropolis accepts
+// iff dE <= 0 or u < det_exp(-beta dE), the heat bath sets +1 iff
+// u < 1 / (1 + det_exp(-2 beta h)), u = x 2^-32.  Spins, swaps and labels are
+// therefore reproducible bit for bit on the host; the running FP64 energies
+// are sums in the workgroup's reduction order, recomputed from the spins at
+// the start of every bin after the first (critmc/kernels/glass.py:116-122).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -70,7 +79,8 @@ __host__ __device__ inline double det_exp(double x)
 
 struct EaArgs {
     int8_t* spins;
-    const int8_t* J;
+    const int8_t* J;        // +-1 couplings, or
+    const double* Jf;       // Gaussian couplings (one of the two is set)
     const int32_t* L;
     const int64_t* spin_off;
     const int64_t* j_off;
@@ -78,7 +88,8 @@ struct EaArgs {
     const double* betas;    // [S][m], hot -> cold
     const uint64_t* thr;    // [S][m][EA_TAB]
     uint64_t* sweeps;       // [S][2][m]
-    int32_t* energies;      // [S][2][m]
+    int32_t* energies;      // [S][2][m] (+-J)
+    double* energies_f;     // [S][2][m] (Gaussian)
     int32_t* perm;          // [S][2][m]
     int32_t* kofc;          // [S][2][m]
     int32_t* labels;        // [S][2][m]
@@ -88,6 +99,7 @@ struct EaArgs {
     int32_t* swap_try;      // [S][2][m-1]
     double* acc;            // [S][n_bins][m][EA_NOBS]
     int32_t* e_series;      // [S][2][m][n_series] (temperature-indexed) or null
+    double* e_series_f;     // the same for Gaussian couplings
     int32_t* q_series;      // [S][m][n_series] or null
     int S, m, d, code, sweeps_per_step, n_bins;
     int64_t n_series;
@@ -120,18 +132,20 @@ __device__ __forceinline__ void ea_coords(uint32_t i, uint32_t L, const FastDiv&
     }
 }
 
-template <int Z>
+template <int Z, bool G>
 __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
 {
     extern __shared__ int8_t lds8[];
     __shared__ int red[16];
+    __shared__ double redf[4];
     const int m = a.m;
     const int s = blockIdx.x / (2 * m);
     const int rc = blockIdx.x - s * 2 * m;  // r * m + c
     const CbGeo g = ea_geo(Z / 2, a.L[s]);
     const int64_t N = g.N;
     int8_t* gs = a.spins + a.spin_off[s] + (int64_t)rc * N;
-    const int8_t* J = a.J + a.j_off[s];
+    const int8_t* J = G ? nullptr : a.J + a.j_off[s];
+    const double* Jf = G ? a.Jf + a.j_off[s] : nullptr;
     const bool use_lds = N <= EA_LDS_MAX;
     int8_t* S = use_lds ? lds8 : gs;
     if (use_lds)
@@ -141,8 +155,10 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
     const uint64_t* T = a.thr + ((int64_t)s * m + k) * EA_TAB;
     uint64_t sweep = a.sweeps[cfg];
     const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1], sid = (uint32_t)rc;
+    const double beta = a.betas[(int64_t)s * m + k];
     __syncthreads();
     int de = 0;
+    double def = 0.0;
     const int64_t nquad = (g.half + 3) >> 2;
     for (int sp = 0; sp < a.sweeps_per_step; ++sp) {
         for (int c = 0; c < 2; ++c) {
@@ -156,24 +172,47 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
                     if (kk >= g.half) break;
                     int64_t nb[Z];
                     const int64_t i = cb_site<Z>(g, kk, c, nb);
-                    int h = 0;
-#pragma unroll
-                    for (int ax = 0; ax < Z / 2; ++ax) {
-                        h += J[ax * N + i] * S[nb[2 * ax]];
-                        h += J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
-                    }
                     const int si = S[i];
-                    if (a.code == CODE_METROPOLIS) {
-                        const int aa = si * h;
-                        if (aa <= 0 || (uint64_t)xs[t] < T[aa + Z]) {
-                            S[i] = (int8_t)(-si);
-                            de += 2 * aa;
+                    if constexpr (G) {
+                        double h = 0.0;
+#pragma unroll
+                        for (int ax = 0; ax < Z / 2; ++ax) {
+                            h += Jf[ax * N + i] * (double)S[nb[2 * ax]];
+                            h += Jf[ax * N + nb[2 * ax + 1]] * (double)S[nb[2 * ax + 1]];
+                        }
+                        const double u = (double)xs[t] * 0x1p-32;
+                        if (a.code == CODE_METROPOLIS) {
+                            const double dE = 2.0 * (double)si * h;
+                            if (dE <= 0.0 || u < det_exp(-beta * dE)) {
+                                S[i] = (int8_t)(-si);
+                                def += dE;
+                            }
+                        } else {
+                            const int nw = u < 1.0 / (1.0 + det_exp(-2.0 * beta * h)) ? 1 : -1;
+                            if (nw != si) {
+                                def += 2.0 * (double)si * h;
+                                S[i] = (int8_t)nw;
+                            }
                         }
                     } else {
-                        const int nw = (uint64_t)xs[t] < T[h + Z] ? 1 : -1;
-                        if (nw != si) {
-                            de += 2 * si * h;
-                            S[i] = (int8_t)nw;
+                        int h = 0;
+#pragma unroll
+                        for (int ax = 0; ax < Z / 2; ++ax) {
+                            h += J[ax * N + i] * S[nb[2 * ax]];
+                            h += J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
+                        }
+                        if (a.code == CODE_METROPOLIS) {
+                            const int aa = si * h;
+                            if (aa <= 0 || (uint64_t)xs[t] < T[aa + Z]) {
+                                S[i] = (int8_t)(-si);
+                                de += 2 * aa;
+                            }
+                        } else {
+                            const int nw = (uint64_t)xs[t] < T[h + Z] ? 1 : -1;
+                            if (nw != si) {
+                                de += 2 * si * h;
+                                S[i] = (int8_t)nw;
+                            }
                         }
                     }
                 }
@@ -182,19 +221,32 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
         }
         ++sweep;
     }
-    de = wave_sum(de);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = de;
+    if constexpr (G) {
+        def = wave_sum_f64(def);
+        if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = def;
+    } else {
+        de = wave_sum(de);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = de;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
-        a.energies[cfg] += t;
+        const int nw = (int)((blockDim.x + 63) >> 6);
+        if constexpr (G) {
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += redf[w];
+            a.energies_f[cfg] += t;
+        } else {
+            int t = 0;
+            for (int w = 0; w < nw; ++w) t += red[w];
+            a.energies[cfg] += t;
+        }
         a.sweeps[cfg] = sweep;
     }
     if (use_lds)
         for (int64_t t = threadIdx.x; t < N; t += blockDim.x) gs[t] = lds8[t];
 }
 
+template <bool G>
 __global__ void ea_swap(EaArgs a, uint64_t step)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,7 +257,8 @@ __global__ void ea_swap(EaArgs a, uint64_t step)
     int32_t* perm = a.perm + base;
     int32_t* kofc = a.kofc + base;
     int32_t* lab = a.labels + base;
-    const int32_t* E = a.energies + base;
+    const int32_t* E = G ? nullptr : a.energies + base;
+    const double* Ef = G ? a.energies_f + base : nullptr;
     const double* beta = a.betas + (int64_t)s * m;
     int32_t* sacc = a.swap_acc + (int64_t)t * (m - 1);
     int32_t* stry = a.swap_try + (int64_t)t * (m - 1);
@@ -218,7 +271,7 @@ __global__ void ea_swap(EaArgs a, uint64_t step)
             xs[0] = rn.x; xs[1] = rn.y; xs[2] = rn.z; xs[3] = rn.w;
         }
         const int c1 = perm[k], c2 = perm[k + 1];
-        const double x = (beta[k] - beta[k + 1]) * (double)(E[c1] - E[c2]);
+        const double x = (beta[k] - beta[k + 1]) * (G ? Ef[c1] - Ef[c2] : (double)(E[c1] - E[c2]));
         stry[k] += 1;
         if (x >= 0.0 || (double)xs[k & 3] * 0x1p-32 < det_exp(x)) {
             perm[k] = c2;
@@ -241,7 +294,7 @@ __global__ void ea_swap(EaArgs a, uint64_t step)
     }
 }
 
-template <int Z>
+template <int Z, bool G>
 __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
 {
     __shared__ int ired[2 * 16];
@@ -313,13 +366,15 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
     double qk2 = 0.0;
     for (int ax = 0; ax < d; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
     qk2 /= d;
-    const int32_t Ea = a.energies[((int64_t)s * 2 + 0) * m + ca];
-    const int32_t Eb = a.energies[((int64_t)s * 2 + 1) * m + cb];
+    const double Ea = G ? a.energies_f[((int64_t)s * 2 + 0) * m + ca]
+                        : (double)a.energies[((int64_t)s * 2 + 0) * m + ca];
+    const double Eb = G ? a.energies_f[((int64_t)s * 2 + 1) * m + cb]
+                        : (double)a.energies[((int64_t)s * 2 + 1) * m + cb];
     const int bin = 63 - __clzll((long long)(slot + 1));  // steps of this call, from 0
     if (bin < a.n_bins) {
         double* o = a.acc + (((int64_t)s * a.n_bins + bin) * m + k) * EA_NOBS;
         const double qq = (double)Q / (double)N;
-        o[0] += 0.5 * ((double)Ea + (double)Eb) / (double)N;
+        o[0] += 0.5 * (Ea + Eb) / (double)N;
         o[1] += (double)QL / (double)(d * N);
         o[2] += fabs(qq);
         o[3] += qq * qq;
@@ -327,18 +382,22 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
         o[5] += qk2;
         o[6] += 1.0;
     }
-    if (a.e_series) {
-        a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = Ea;
-        a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = Eb;
+    if (G && a.e_series_f) {
+        a.e_series_f[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = Ea;
+        a.e_series_f[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = Eb;
+    } else if (!G && a.e_series) {
+        a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = (int32_t)Ea;
+        a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = (int32_t)Eb;
     }
     if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
 }
 
 // E = -sum_b J_b s_i s_j for every configuration (critmc/kernels/glass.py:33-38).
-template <int Z>
+template <int Z, bool G>
 __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
 {
     __shared__ int red[16];
+    __shared__ double redf[16];
     const int m = a.m;
     const int s = blockIdx.x / (2 * m);
     const int rc = blockIdx.x - s * 2 * m;
@@ -346,9 +405,11 @@ __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
     const CbGeo g = ea_geo(Z / 2, L);
     const int64_t N = g.N;
     const int8_t* sp = a.spins + a.spin_off[s] + (int64_t)rc * N;
-    const int8_t* J = a.J + a.j_off[s];
+    const int8_t* J = G ? nullptr : a.J + a.j_off[s];
+    const double* Jf = G ? a.Jf + a.j_off[s] : nullptr;
     const FastDiv dl = make_fastdiv((uint32_t)L);
     int e = 0;
+    double ef = 0.0;
     for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
         int x[3];
         ea_coords<Z / 2>((uint32_t)i, (uint32_t)L, dl, x);
@@ -356,16 +417,29 @@ __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
         for (int ax = 0; ax < Z / 2; ++ax) {
             const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
             const int64_t j = i + (x[ax] + 1 == L ? -(int64_t)(L - 1) * stride : stride);
-            e -= J[ax * N + i] * sp[i] * sp[j];
+            if constexpr (G) ef -= Jf[ax * N + i] * (double)(sp[i] * sp[j]);
+            else e -= J[ax * N + i] * sp[i] * sp[j];
         }
     }
-    e = wave_sum(e);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    if constexpr (G) {
+        ef = wave_sum_f64(ef);
+        if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = ef;
+    } else {
+        e = wave_sum(e);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += red[w];
-        a.energies[(int64_t)s * 2 * m + rc] = t;
+        const int nw = (int)((blockDim.x + 63) >> 6);
+        if constexpr (G) {
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += redf[w];
+            a.energies_f[(int64_t)s * 2 * m + rc] = t;
+        } else {
+            int t = 0;
+            for (int w = 0; w < nw; ++w) t += red[w];
+            a.energies[(int64_t)s * 2 * m + rc] = t;
+        }
     }
 }
 
@@ -384,6 +458,33 @@ __global__ void ea_init_kernel(EaArgs a)
         const u32x4 r = draw(k0, k1, (uint32_t)rc, 0, 0, TAG_INIT, (uint32_t)blk);
         const uint32_t w[4] = {r.x, r.y, r.z, r.w};
         for (int k = 0; k < 4 && blk * 4 + k < N; ++k) sp[blk * 4 + k] = (w[k] >> 31) ? 1 : -1;
+    }
+}
+
+template <bool G>
+static void launch_step(const EaArgs& a, int ndim, int sweep_blocks, int block, size_t lds,
+                        int swap_threads, uint64_t step, int64_t t, hipStream_t st)
+{
+    if (ndim == 2) {
+        hipLaunchKernelGGL((ea_sweep<4, G>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
+        hipLaunchKernelGGL((ea_measure<4, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
+    } else {
+        hipLaunchKernelGGL((ea_sweep<6, G>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
+        hipLaunchKernelGGL((ea_measure<6, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
+    }
+}
+
+static void launch_energy(const EaArgs& a, int gauss, hipStream_t st)
+{
+    const int blocks = a.S * 2 * a.m;
+    if (a.d == 2) {
+        if (gauss) hipLaunchKernelGGL((ea_energy<4, true>), dim3(blocks), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((ea_energy<4, false>), dim3(blocks), dim3(256), 0, st, a);
+    } else {
+        if (gauss) hipLaunchKernelGGL((ea_energy<6, true>), dim3(blocks), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((ea_energy<6, false>), dim3(blocks), dim3(256), 0, st, a);
     }
 }
 
@@ -420,38 +521,38 @@ int cm_ea_init(void* spins, const int32_t* L_dev, const int64_t* spin_off_dev, i
     return CM_OK;
 }
 
-int cm_ea_energy(void* spins, const void* J, const int32_t* L_dev, const int64_t* spin_off_dev,
-                 const int64_t* j_off_dev, int32_t n_samples, int32_t m, int32_t ndim,
-                 int32_t* energies_dev, void* stream)
+int cm_ea_energy(void* spins, const void* J, int32_t jtype, const int32_t* L_dev,
+                 const int64_t* spin_off_dev, const int64_t* j_off_dev, int32_t n_samples,
+                 int32_t m, int32_t ndim, void* energies_dev, void* stream)
 {
     CM_REQUIRE(ndim == 2 || ndim == 3, "EA tempering supports square and cubic lattices");
     CM_REQUIRE(n_samples >= 1 && m >= 2, "need at least one sample and two temperatures");
+    CM_REQUIRE(jtype == 0 || jtype == 1, "jtype must be 0 (int8 +-1) or 1 (FP64 couplings)");
     EaArgs a{};
     a.spins = (int8_t*)spins;
-    a.J = (const int8_t*)J;
+    a.J = jtype ? nullptr : (const int8_t*)J;
+    a.Jf = jtype ? (const double*)J : nullptr;
     a.L = L_dev;
     a.spin_off = spin_off_dev;
     a.j_off = j_off_dev;
-    a.energies = energies_dev;
+    a.energies = jtype ? nullptr : (int32_t*)energies_dev;
+    a.energies_f = jtype ? (double*)energies_dev : nullptr;
     a.S = n_samples;
     a.m = m;
     a.d = ndim;
-    const int blocks = n_samples * 2 * m;
-    if (ndim == 2)
-        hipLaunchKernelGGL(ea_energy<4>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
-    else
-        hipLaunchKernelGGL(ea_energy<6>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, a);
+    launch_energy(a, jtype, (hipStream_t)stream);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }
 
-int cm_ea_pt_run(void* spins, const void* J, const int32_t* L_dev, const int64_t* spin_off_dev,
-                 const int64_t* j_off_dev, int32_t n_samples, int32_t m, int32_t ndim,
-                 const int32_t* L_host, const double* betas_host, const double* betas_dev,
-                 const uint32_t* keys_dev, uint64_t* sweeps_dev, int32_t* energies_dev,
+int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev,
+                 const int64_t* spin_off_dev, const int64_t* j_off_dev, int32_t n_samples,
+                 int32_t m, int32_t ndim, const int32_t* L_host, const double* betas_host,
+                 const double* betas_dev, const uint32_t* keys_dev, uint64_t* sweeps_dev,
+                 void* energies_dev,
                  int32_t* perm_dev, int32_t* kofc_dev, int32_t* labels_dev, int32_t* trips_dev,
                  int32_t* label_hist_dev, int32_t* swap_acc_dev, int32_t* swap_try_dev,
-                 double* acc_dev, int32_t n_bins, int32_t* e_series_dev, int32_t* q_series_dev,
+                 double* acc_dev, int32_t n_bins, void* e_series_dev, int32_t* q_series_dev,
                  int32_t code, int32_t sweeps_per_step, uint64_t step0, int64_t n_steps,
                  void* workspace, int64_t workspace_bytes, int32_t block, void* stream)
 {
@@ -459,6 +560,7 @@ int cm_ea_pt_run(void* spins, const void* J, const int32_t* L_dev, const int64_t
     CM_REQUIRE(n_samples >= 1 && m >= 2, "need at least one sample and two temperatures");
     CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
                "update code %d has no checkerboard EA kernel", code);
+    CM_REQUIRE(jtype == 0 || jtype == 1, "jtype must be 0 (int8 +-1) or 1 (FP64 couplings)");
     CM_REQUIRE(sweeps_per_step >= 1, "sweeps_per_step must be positive");
     CM_REQUIRE(n_steps >= 0 && n_bins >= 1, "bad step or bin count");
     int max_L = 2;
@@ -486,27 +588,54 @@ int cm_ea_pt_run(void* spins, const void* J, const int32_t* L_dev, const int64_t
     hipStream_t st = (hipStream_t)stream;
     uint64_t* dthr = (uint64_t*)workspace;
     CM_HIP_CHECK(upload_async(dthr, thr.data(), thr.size() * 8, st));
-    EaArgs a{(int8_t*)spins, (const int8_t*)J, L_dev, spin_off_dev, j_off_dev, keys_dev,
-             betas_dev, dthr, sweeps_dev, energies_dev, perm_dev, kofc_dev, labels_dev, trips_dev,
-             label_hist_dev, swap_acc_dev, swap_try_dev, acc_dev, e_series_dev, q_series_dev,
-             n_samples, m, ndim, code, sweeps_per_step, n_bins, n_steps};
+    EaArgs a{};
+    a.spins = (int8_t*)spins;
+    a.J = jtype ? nullptr : (const int8_t*)J;
+    a.Jf = jtype ? (const double*)J : nullptr;
+    a.L = L_dev;
+    a.spin_off = spin_off_dev;
+    a.j_off = j_off_dev;
+    a.keys = keys_dev;
+    a.betas = betas_dev;
+    a.thr = dthr;
+    a.sweeps = sweeps_dev;
+    a.energies = jtype ? nullptr : (int32_t*)energies_dev;
+    a.energies_f = jtype ? (double*)energies_dev : nullptr;
+    a.perm = perm_dev;
+    a.kofc = kofc_dev;
+    a.labels = labels_dev;
+    a.trips = trips_dev;
+    a.label_hist = label_hist_dev;
+    a.swap_acc = swap_acc_dev;
+    a.swap_try = swap_try_dev;
+    a.acc = acc_dev;
+    a.e_series = jtype ? nullptr : (int32_t*)e_series_dev;
+    a.e_series_f = jtype ? (double*)e_series_dev : nullptr;
+    a.q_series = q_series_dev;
+    a.S = n_samples;
+    a.m = m;
+    a.d = ndim;
+    a.code = code;
+    a.sweeps_per_step = sweeps_per_step;
+    a.n_bins = n_bins;
+    a.n_series = n_steps;
     if (block <= 0) block = 64;
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
     const int64_t maxN = (int64_t)max_L * max_L * (ndim == 3 ? max_L : 1);
     const size_t lds = (size_t)std::min<int64_t>(maxN, EA_LDS_MAX);
     const int sweep_blocks = n_samples * 2 * m;
     const int swap_threads = 2 * n_samples;
+    int64_t next_edge = 1;
+    int bin = 0;
     for (int64_t t = 0; t < n_steps; ++t) {
         const uint64_t step = step0 + (uint64_t)t;
-        if (ndim == 2) {
-            hipLaunchKernelGGL(ea_sweep<4>, dim3(sweep_blocks), dim3(block), lds, st, a);
-            hipLaunchKernelGGL(ea_swap, dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
-            hipLaunchKernelGGL(ea_measure<4>, dim3(n_samples * m), dim3(256), 0, st, a, t);
-        } else {
-            hipLaunchKernelGGL(ea_sweep<6>, dim3(sweep_blocks), dim3(block), lds, st, a);
-            hipLaunchKernelGGL(ea_swap, dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
-            hipLaunchKernelGGL(ea_measure<6>, dim3(n_samples * m), dim3(256), 0, st, a, t);
+        if (t >= next_edge) {  // bin edge: FP64 energies are recomputed (integer ones are exact)
+            ++bin;
+            next_edge = ((int64_t)1 << (bin + 1)) - 1;
+            if (jtype) launch_energy(a, 1, st);
         }
+        if (jtype) launch_step<true>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st);
+        else launch_step<false>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st);
     }
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;

@@ -5,11 +5,12 @@ independent replica sets run at every temperature and give the overlap
 ``q``, the link overlap ``q_l`` and ``|q(k_min)|^2``, accumulated in
 logarithmic bins of PT steps (bin b covers steps [2^b - 1, 2^(b+1) - 1)).
 
-The device path takes +-1 (``disorder="bimodal"``) couplings, which keep
-energies and the replay exact integers; Gaussian disorder is refused with a
-clear error.  :class:`TemperingBatch` advances many samples of uneven size in
-the same launches; :class:`ParallelTempering` is the reference's one-sample
-driver on top of it.
++-1 couplings (``disorder="bimodal"``) run on int8 couplings with exact
+integer energies; any other couplings (``disorder="gaussian"``) run with FP64
+couplings, fields and energies, recomputed at every bin edge as the reference
+does.  :class:`TemperingBatch` advances many samples of uneven size in the same
+launches (samples of one batch share the coupling kind); :class:`ParallelTempering`
+is the reference's one-sample driver on top of it.
 """
 
 from __future__ import annotations
@@ -68,12 +69,11 @@ class EdwardsAnderson:
         return float(-np.sum(self.couplings * s[b[:, 0]] * s[b[:, 1]]))
 
     def device_couplings(self) -> np.ndarray:
-        if not np.all(np.abs(self.couplings) == 1.0):
-            raise ValueError("GPU tempering supports +-1 couplings (disorder='bimodal')")
+        """Forward-bond couplings in the device order (axis-major, one per site and axis)."""
         require_checkerboard(self.lattice)
         if self.lattice.dim not in (2, 3) or len(set(self.lattice.shape)) != 1:
             raise ValueError("GPU tempering supports L x L and L x L x L lattices")
-        return self.couplings.astype(np.int8)
+        return self.couplings.copy()
 
 
 def geometric_temperatures(t_min: float, t_max: float, n: int) -> np.ndarray:

@@ -1,8 +1,10 @@
-"""Batched +-J Edwards-Anderson parallel tempering on the device (``cm_ea_*``).
+"""Batched Edwards-Anderson parallel tempering on the device (``cm_ea_*``).
 
 Replaces critmc/kernels/glass.py's ``pt_run`` (:80-163) for many disorder
 samples at once, of possibly different linear sizes.  All samples of a batch
 share the dimension (square or cubic) and the number of temperatures ``m``.
+A batch whose couplings are all +-1 runs on int8 couplings with exact int32
+energies; any other couplings (Gaussian disorder) run in FP64.
 """
 
 from __future__ import annotations
@@ -23,8 +25,9 @@ OBS_E, OBS_QL, OBS_Q, OBS_Q2, OBS_Q4, OBS_QK2, OBS_COUNT = range(N_OBS)
 class EaBatch:
     """Device state of S samples x 2 replica sets x m temperatures.
 
-    ``couplings[s]``: +-1 values of sample s's forward bonds, axis-major
-    (``bonds[a*N + i] = (i, i + e_a)``, critmc/lattice.py:136-138).
+    ``couplings[s]``: sample s's forward-bond couplings, axis-major
+    (``bonds[a*N + i] = (i, i + e_a)``, critmc/lattice.py:136-138); +-1 values
+    select the integer path (``gaussian`` False), others the FP64 one.
     ``betas``: ``[S, m]`` or ``[m]``, hot -> cold.  ``keys``: ``[S, 2]`` Philox keys.
     ``spins``: optional host ``[2, m, N_s]`` int8 start per sample (default:
     device draw from the sample's key).
@@ -50,14 +53,19 @@ class EaBatch:
             raise ValueError("need at least two temperatures")
         self.spin_off = np.concatenate([[0], np.cumsum(2 * self.m * self.N)[:-1]]).astype(np.int64)
         self.j_off = np.concatenate([[0], np.cumsum(self.d * self.N)[:-1]]).astype(np.int64)
-        J = np.empty(int((self.d * self.N).sum()), dtype=np.int8)
+        couplings = [np.asarray(c, dtype=np.float64) for c in couplings]
+        if len(couplings) != self.S:
+            raise ValueError("need the couplings of every sample")
         for s, c in enumerate(couplings):
-            c = np.asarray(c)
             if c.shape != (self.d * self.N[s],):
                 raise ValueError("need one coupling per bond")
-            if not np.all(np.abs(c) == 1):
-                raise ValueError("GPU tempering needs +-1 (bimodal) couplings")
-            J[self.j_off[s]:self.j_off[s] + c.size] = c.astype(np.int8)
+            if not np.all(np.isfinite(c)):
+                raise ValueError("couplings must be finite")
+        self.gaussian = not all(np.all(np.abs(c) == 1.0) for c in couplings)
+        self.jtype = 1 if self.gaussian else 0
+        J = np.empty(int((self.d * self.N).sum()), dtype=np.float64 if self.gaussian else np.int8)
+        for s, c in enumerate(couplings):
+            J[self.j_off[s]:self.j_off[s] + c.size] = c
         dv = self.dev
         self.J = torch.from_numpy(J).to(dv)
         self.L_d = torch.from_numpy(self.L).to(dv)
@@ -79,7 +87,8 @@ class EaBatch:
         S2m = self.S * 2 * self.m
         i32 = dict(dtype=torch.int32, device=dv)
         self.sweeps = torch.zeros(S2m, dtype=torch.int64, device=dv)
-        self.energies = torch.empty(S2m, **i32)
+        self.energies = torch.empty(S2m, dtype=torch.float64 if self.gaussian else torch.int32,
+                                    device=dv)
         ar = torch.arange(self.m, **i32).repeat(self.S * 2)
         self.perm = ar.clone()
         self.kofc = ar.clone()
@@ -95,9 +104,9 @@ class EaBatch:
         self.recompute_energies()
 
     def recompute_energies(self) -> None:
-        check(lib().cm_ea_energy(ptr(self.spins), ptr(self.J), ptr(self.L_d), ptr(self.spin_off_d),
-                                 ptr(self.j_off_d), self.S, self.m, self.d, ptr(self.energies),
-                                 stream()), "cm_ea_energy")
+        check(lib().cm_ea_energy(ptr(self.spins), ptr(self.J), self.jtype, ptr(self.L_d),
+                                 ptr(self.spin_off_d), ptr(self.j_off_d), self.S, self.m, self.d,
+                                 ptr(self.energies), stream()), "cm_ea_energy")
 
     def configurations(self, s: int) -> np.ndarray:
         """Host copy of sample s's spins ``[2, m, N]``."""
@@ -116,11 +125,12 @@ class EaBatch:
             t.zero_()
         e_series = q_series = None
         if record:
-            e_series = torch.empty(self.S * 2 * self.m * n_steps, dtype=torch.int32, device=dv)
+            e_series = torch.empty(self.S * 2 * self.m * n_steps, device=dv,
+                                   dtype=torch.float64 if self.gaussian else torch.int32)
             q_series = torch.empty(self.S * self.m * n_steps, dtype=torch.int32, device=dv)
         rc = lib().cm_ea_pt_run(
-            ptr(self.spins), ptr(self.J), ptr(self.L_d), ptr(self.spin_off_d), ptr(self.j_off_d),
-            self.S, self.m, self.d, host_ptr(self.L, ctypes.c_int32),
+            ptr(self.spins), ptr(self.J), self.jtype, ptr(self.L_d), ptr(self.spin_off_d),
+            ptr(self.j_off_d), self.S, self.m, self.d, host_ptr(self.L, ctypes.c_int32),
             host_ptr(self.betas, ctypes.c_double), ptr(self.betas_d), ptr(self.keys_d),
             ptr(self.sweeps), ptr(self.energies), ptr(self.perm), ptr(self.kofc), ptr(self.labels),
             ptr(self.trips), ptr(self.label_hist), ptr(self.swap_acc), ptr(self.swap_try),

@@ -141,9 +141,11 @@ int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_r
 
 /* A batch holds n_samples disorder samples of the same dimension (2 or 3) and
  * possibly different even extents L[s]; sample s owns int8 spins
- * [2 sets][m configurations][L^d] at spin_off[s] and int8 +-1 forward-bond
- * couplings [d][L^d] at j_off[s] (bond a*N + i joins i and its +e_a
- * neighbour, critmc/lattice.py:136-138). */
+ * [2 sets][m configurations][L^d] at spin_off[s] and forward-bond couplings
+ * [d][L^d] at j_off[s] (bond a*N + i joins i and its +e_a neighbour,
+ * critmc/lattice.py:136-138).  jtype 0: int8 +-1 couplings (disorder
+ * "bimodal"), int32 energies and energy series; jtype 1: FP64 couplings (e.g.
+ * disorder "gaussian", critmc/glass.py:60-64), FP64 energies and series. */
 
 /* Bytes of caller-owned device scratch cm_ea_pt_run needs. */
 int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m);
@@ -160,9 +162,9 @@ int cm_ea_init(void *spins_dev, const int32_t *L_dev, const int64_t *spin_off_de
 
 /* E = -sum_b J_b s_i s_j of every configuration into energies_dev [S][2][m]:
  * critmc/kernels/glass.py:33-38 (energy). */
-int cm_ea_energy(void *spins_dev, const void *J_dev, const int32_t *L_dev,
+int cm_ea_energy(void *spins_dev, const void *J_dev, int32_t jtype, const int32_t *L_dev,
                  const int64_t *spin_off_dev, const int64_t *j_off_dev, int32_t n_samples,
-                 int32_t m, int32_t ndim, int32_t *energies_dev, void *stream);
+                 int32_t m, int32_t ndim, void *energies_dev, void *stream);
 
 /* n_steps parallel-tempering steps of every sample: sweeps_per_step sweeps
  * (code 0 = Metropolis as the reference, 2 = heat bath) of every
@@ -170,20 +172,22 @@ int cm_ea_energy(void *spins_dev, const void *J_dev, const int32_t *L_dev,
  * round-trip bookkeeping, then per temperature q, q_l, |q(k_min)|^2 into the
  * log-binned accumulators acc [S][n_bins][m][7] (bin of the call's step t:
  * floor(log2(t+1)); columns e, q_l, |q|, q^2, q^4, |q(k)|^2, count) and, when
- * the pointers are non-null, the int32 series e_series [S][2][m][n_steps]
- * (energy at temperature k) and q_series [S][m][n_steps] (N q).
+ * the pointers are non-null, the series e_series [S][2][m][n_steps] (energy
+ * at temperature k; int32 or FP64 per jtype) and q_series [S][m][n_steps] (N q).
+ * With FP64 couplings the energies are recomputed from the spins at the start
+ * of every bin after the first (critmc/kernels/glass.py:116-122).
  * Replaces critmc/kernels/glass.py:80-163 (pt_run); state arrays are
  * [S][2][m] (perm: temperature -> configuration; kofc its inverse), trips
  * [S][2], label_hist [S][2][m][2], swap_acc / swap_try [S][2][m-1].
  * betas_host / betas_dev: [S][m], hot -> cold.  step0: the global PT step of
  * the first step (keys the swap draws).  L_host mirrors L_dev. */
-int cm_ea_pt_run(void *spins_dev, const void *J_dev, const int32_t *L_dev,
+int cm_ea_pt_run(void *spins_dev, const void *J_dev, int32_t jtype, const int32_t *L_dev,
                  const int64_t *spin_off_dev, const int64_t *j_off_dev, int32_t n_samples,
                  int32_t m, int32_t ndim, const int32_t *L_host, const double *betas_host,
                  const double *betas_dev, const uint32_t *keys_dev, uint64_t *sweeps_dev,
-                 int32_t *energies_dev, int32_t *perm_dev, int32_t *kofc_dev, int32_t *labels_dev,
+                 void *energies_dev, int32_t *perm_dev, int32_t *kofc_dev, int32_t *labels_dev,
                  int32_t *trips_dev, int32_t *label_hist_dev, int32_t *swap_acc_dev,
-                 int32_t *swap_try_dev, double *acc_dev, int32_t n_bins, int32_t *e_series_dev,
+                 int32_t *swap_try_dev, double *acc_dev, int32_t n_bins, void *e_series_dev,
                  int32_t *q_series_dev, int32_t code, int32_t sweeps_per_step, uint64_t step0,
                  int64_t n_steps, void *workspace_dev, int64_t workspace_bytes, int32_t block,
                  void *stream);

@@ -268,11 +268,13 @@ class PottsReplay:
 _EXTRA_DECLS.update({
     "orc_det_exp": ([ctypes.c_double], ctypes.c_double),
     "orc_ea_energy": ([_i8p, _i8p, ctypes.c_int, ctypes.c_int], ctypes.c_int32),
+    "orc_ea_energy_f64": ([_i8p, _f64p, ctypes.c_int, ctypes.c_int], ctypes.c_double),
     "orc_ea_init": ([_i8p, ctypes.c_int, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32], None),
-    "orc_ea_pt_run": ([_i8p, _i8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, ctypes.c_int,
-                       ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _u64p, _i32p, _i32p, _i32p,
-                       _i32p, _i32p, _i32p, _i32p, _i32p, _f64p, ctypes.c_int, _i32p, _i32p,
-                       ctypes.c_uint64, ctypes.c_int64], ctypes.c_int),
+    "orc_ea_pt_run": ([_i8p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                       _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _u64p,
+                       ctypes.c_void_p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f64p,
+                       ctypes.c_int, ctypes.c_void_p, _i32p, ctypes.c_uint64, ctypes.c_int64],
+                      ctypes.c_int),
 })
 
 
@@ -281,16 +283,19 @@ def det_exp(x: float) -> float:
 
 
 class EaReplay:
-    """One +-J EA sample under parallel tempering, replayed on the CPU.
+    """One EA sample under parallel tempering, replayed on the CPU.
 
-    ``spins``: [2, m, N] int8 start; ``J``: [d*N] +-1 forward-bond couplings;
-    ``betas``: hot -> cold.  State arrays follow the device layout.
+    ``spins``: [2, m, N] int8 start; ``J``: [d*N] forward-bond couplings (all
+    +-1: the integer path, else FP64 as for Gaussian disorder); ``betas``:
+    hot -> cold.  State arrays follow the device layout.
     """
 
     def __init__(self, spins, J, ndim: int, L: int, betas, key, code: int = 2,
                  sweeps_per_step: int = 1):
         self.spins = np.ascontiguousarray(spins, dtype=np.int8).copy()
-        self.J = np.ascontiguousarray(J, dtype=np.int8)
+        J = np.asarray(J, dtype=np.float64)
+        self.jtype = 0 if np.all(np.abs(J) == 1.0) else 1
+        self.J = np.ascontiguousarray(J if self.jtype else J.astype(np.int8))
         self.d, self.L = int(ndim), int(L)
         self.betas = np.ascontiguousarray(betas, dtype=np.float64)
         self.m = len(self.betas)
@@ -299,9 +304,14 @@ class EaReplay:
         m = self.m
         self.N = self.L ** self.d
         self.sweeps = np.zeros(2 * m, np.uint64)
-        self.energies = np.array([lib().orc_ea_energy(_ptr(self.spins[r, c], _i8p), _ptr(self.J, _i8p),
-                                                      self.d, self.L) for r in range(2) for c in range(m)],
-                                 dtype=np.int32)
+        if self.jtype:
+            self.energies = np.array([lib().orc_ea_energy_f64(_ptr(self.spins[r, c], _i8p),
+                                                              _ptr(self.J, _f64p), self.d, self.L)
+                                      for r in range(2) for c in range(m)], dtype=np.float64)
+        else:
+            self.energies = np.array([lib().orc_ea_energy(_ptr(self.spins[r, c], _i8p),
+                                                          _ptr(self.J, _i8p), self.d, self.L)
+                                      for r in range(2) for c in range(m)], dtype=np.int32)
         self.perm = np.tile(np.arange(m, dtype=np.int32), 2)
         self.kofc = self.perm.copy()
         self.labels = np.zeros(2 * m, np.int32)
@@ -314,15 +324,15 @@ class EaReplay:
         sacc = np.zeros(2 * (m - 1), np.int32)
         stry = np.zeros(2 * (m - 1), np.int32)
         acc = np.zeros(n_bins * m * 7, np.float64)
-        es = np.zeros(2 * m * n_steps, np.int32)
+        es = np.zeros(2 * m * n_steps, np.float64 if self.jtype else np.int32)
         qs = np.zeros(m * n_steps, np.int32)
         rc = lib().orc_ea_pt_run(
-            _ptr(self.spins, _i8p), _ptr(self.J, _i8p), self.d, self.L, m, _ptr(self.betas, _f64p),
-            self.code, self.sps, self.key[0], self.key[1], _ptr(self.sweeps, _u64p),
-            _ptr(self.energies, _i32p), _ptr(self.perm, _i32p), _ptr(self.kofc, _i32p),
-            _ptr(self.labels, _i32p), _ptr(trips, _i32p), _ptr(hist, _i32p), _ptr(sacc, _i32p),
-            _ptr(stry, _i32p), _ptr(acc, _f64p), int(n_bins), _ptr(es, _i32p), _ptr(qs, _i32p),
-            int(self.step), int(n_steps))
+            _ptr(self.spins, _i8p), self.J.ctypes.data, self.jtype, self.d, self.L, m,
+            _ptr(self.betas, _f64p), self.code, self.sps, self.key[0], self.key[1],
+            _ptr(self.sweeps, _u64p), self.energies.ctypes.data, _ptr(self.perm, _i32p),
+            _ptr(self.kofc, _i32p), _ptr(self.labels, _i32p), _ptr(trips, _i32p), _ptr(hist, _i32p),
+            _ptr(sacc, _i32p), _ptr(stry, _i32p), _ptr(acc, _f64p), int(n_bins), es.ctypes.data,
+            _ptr(qs, _i32p), int(self.step), int(n_steps))
         if rc:
             raise ValueError(f"orc_ea_pt_run failed ({rc})")
         self.step += n_steps

@@ -1,8 +1,15 @@
 /*
- * ORACLE (test infrastructure only) -- CPU replay of batched +-J
+ * ORACLE (test infrastructure only) -- CPU replay of batched
  * Edwards-Anderson parallel tempering, bit-exact target of the GPU kernels
- * for spins, energies, permutations, labels, trips, swap counts and the
- * integer q / E series (the FP64 accumulators agree to rounding).
+ * for spins, permutations, labels, trips, swap counts and the integer q
+ * series; with +-J couplings (jtype 0) also for the int32 energies and E
+ * series.  With FP64 couplings (jtype 1, Gaussian disorder) the field is
+ * h = sum_k J_ik s_k added in column order (+x, -x, +y, ...), Metropolis
+ * accepts iff dE <= 0 or u < det_exp(-beta dE), the heat bath sets +1 iff
+ * u < 1/(1 + det_exp(-2 beta h)) (u = x 2^-32), and energies are recomputed at
+ * every bin edge after the first (critmc/kernels/glass.py:116-122); the FP64
+ * energies then agree with the device to rounding (its sums run in another
+ * order), and the FP64 accumulators always do.
  *
  * Restated from the reference (critmc/kernels/glass.py):
  *   energy :33-38; Metropolis sweep :41-54; heat bath = Ising heat-bath rule
@@ -50,6 +57,16 @@ static int64_t ea_nbr_fwd(int64_t i, int ax, int L)
     return i + (x + 1 == L ? -(int64_t)(L - 1) * stride : stride);
 }
 
+double orc_ea_energy_f64(const int8_t *s, const double *J, int ndim, int L)
+{
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= L;
+    double e = 0.0;
+    for (int64_t i = 0; i < N; ++i)
+        for (int ax = 0; ax < ndim; ++ax) e -= J[ax * N + i] * (double)(s[i] * s[ea_nbr_fwd(i, ax, L)]);
+    return e;
+}
+
 int32_t orc_ea_energy(const int8_t *s, const int8_t *J, int ndim, int L)
 {
     int64_t N = 1;
@@ -60,13 +77,17 @@ int32_t orc_ea_energy(const int8_t *s, const int8_t *J, int ndim, int L)
     return e;
 }
 
-int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const double *betas,
-                  int code, int sweeps_per_step, uint32_t k0, uint32_t k1, uint64_t *sweeps,
-                  int32_t *energies, int32_t *perm, int32_t *kofc, int32_t *labels,
-                  int32_t *trips, int32_t *label_hist, int32_t *swap_acc, int32_t *swap_try,
-                  double *acc, int n_bins, int32_t *e_series, int32_t *q_series, uint64_t step0,
-                  int64_t n_steps)
+int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int m,
+                  const double *betas, int code, int sweeps_per_step, uint32_t k0, uint32_t k1,
+                  uint64_t *sweeps, void *energies_v, int32_t *perm, int32_t *kofc,
+                  int32_t *labels, int32_t *trips, int32_t *label_hist, int32_t *swap_acc,
+                  int32_t *swap_try, double *acc, int n_bins, void *e_series_v, int32_t *q_series,
+                  uint64_t step0, int64_t n_steps)
 {
+    const int8_t *J = jtype ? NULL : (const int8_t *)Jv;
+    const double *Jf = jtype ? (const double *)Jv : NULL;
+    int32_t *ei = jtype ? NULL : (int32_t *)energies_v;
+    double *ef = jtype ? (double *)energies_v : NULL;
     int32_t dims[3] = {L, L, L};
     orc_cb g;
     if (orc_cb_init(&g, ndim, dims)) return -2;
@@ -81,8 +102,19 @@ int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const 
             thr[k * 16 + v + z] = orc_thr_p(p);
         }
     int64_t nb[6];
+    int64_t next_edge = 1;
+    int ebin = 0;
+    /* working energies: exact for integers, the running FP64 sums otherwise */
+    double *E = (double *)malloc(sizeof(double) * 2 * m);
+    for (int c = 0; c < 2 * m; ++c) E[c] = jtype ? ef[c] : (double)ei[c];
     for (int64_t t = 0; t < n_steps; ++t) {
         const uint64_t step = step0 + (uint64_t)t;
+        if (t >= next_edge) {
+            ++ebin;
+            next_edge = ((int64_t)1 << (ebin + 1)) - 1;
+            if (jtype)
+                for (int c = 0; c < 2 * m; ++c) E[c] = orc_ea_energy_f64(spins + (int64_t)c * N, Jf, ndim, L);
+        }
         for (int r = 0; r < 2; ++r)
             for (int c = 0; c < m; ++c) {
                 int8_t *S = spins + ((int64_t)r * m + c) * N;
@@ -96,22 +128,45 @@ int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const 
                                 orc_draw_block(k0, k1, (uint32_t)(r * m + c), *sw, col, ORC_TAG_EA,
                                                (uint32_t)(kk >> 2), blk);
                             int64_t i = orc_cb_site(&g, kk, col, nb);
+                            int si = S[i];
+                            uint32_t x = blk[kk & 3];
+                            if (jtype) {
+                                const double beta = betas[kofc[r * m + c]];
+                                double h = 0.0;
+                                for (int ax = 0; ax < ndim; ++ax) {
+                                    h += Jf[ax * N + i] * (double)S[nb[2 * ax]];
+                                    h += Jf[ax * N + nb[2 * ax + 1]] * (double)S[nb[2 * ax + 1]];
+                                }
+                                double u = (double)x * 0x1p-32;
+                                if (code == 0) {
+                                    double dE = 2.0 * (double)si * h;
+                                    if (dE <= 0.0 || u < orc_det_exp(-beta * dE)) {
+                                        S[i] = (int8_t)(-si);
+                                        E[r * m + c] += dE;
+                                    }
+                                } else {
+                                    int nw = u < 1.0 / (1.0 + orc_det_exp(-2.0 * beta * h)) ? 1 : -1;
+                                    if (nw != si) {
+                                        E[r * m + c] += 2.0 * (double)si * h;
+                                        S[i] = (int8_t)nw;
+                                    }
+                                }
+                                continue;
+                            }
                             int h = 0;
                             for (int ax = 0; ax < ndim; ++ax)
                                 h += J[ax * N + i] * S[nb[2 * ax]] +
                                      J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
-                            int si = S[i];
-                            uint32_t x = blk[kk & 3];
                             if (code == 0) {
                                 int aa = si * h;
                                 if (aa <= 0 || (uint64_t)x < T[aa + z]) {
                                     S[i] = (int8_t)(-si);
-                                    energies[r * m + c] += 2 * aa;
+                                    E[r * m + c] += 2 * aa;
                                 }
                             } else {
                                 int nw = (uint64_t)x < T[h + z] ? 1 : -1;
                                 if (nw != si) {
-                                    energies[r * m + c] += 2 * si * h;
+                                    E[r * m + c] += 2 * si * h;
                                     S[i] = (int8_t)nw;
                                 }
                             }
@@ -122,13 +177,13 @@ int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const 
             }
         for (int r = 0; r < 2; ++r) {
             int32_t *p = perm + r * m, *ko = kofc + r * m, *lab = labels + r * m;
-            const int32_t *E = energies + r * m;
+            const double *Er = E + r * m;
             uint32_t blk[4] = {0, 0, 0, 0};
             for (int k = 0; k < m - 1; ++k) {
                 if ((k & 3) == 0)
                     orc_draw_block(k0, k1, (uint32_t)r, step, 0, ORC_TAG_PT, (uint32_t)(k >> 2), blk);
                 int c1 = p[k], c2 = p[k + 1];
-                double x = (betas[k] - betas[k + 1]) * (double)(E[c1] - E[c2]);
+                double x = (betas[k] - betas[k + 1]) * (Er[c1] - Er[c2]);
                 swap_try[r * (m - 1) + k] += 1;
                 if (x >= 0.0 || (double)blk[k & 3] * 0x1p-32 < orc_det_exp(x)) {
                     p[k] = c2;
@@ -173,11 +228,11 @@ int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const 
             for (int ax = 0; ax < ndim; ++ax)
                 qk2 += (re[ax] * re[ax] + im[ax] * im[ax]) / ((double)N * (double)N);
             qk2 /= ndim;
-            int32_t Ea = energies[ca], Eb = energies[m + cb];
+            double Ea = E[ca], Eb = E[m + cb];
             if (bin < n_bins) {
                 double *o = acc + ((int64_t)bin * m + k) * 7;
                 double qq = (double)q / (double)N;
-                o[0] += 0.5 * ((double)Ea + (double)Eb) / (double)N;
+                o[0] += 0.5 * (Ea + Eb) / (double)N;
                 o[1] += (double)ql / (double)(ndim * N);
                 o[2] += fabs(qq);
                 o[3] += qq * qq;
@@ -185,13 +240,21 @@ int orc_ea_pt_run(int8_t *spins, const int8_t *J, int ndim, int L, int m, const 
                 o[5] += qk2;
                 o[6] += 1.0;
             }
-            if (e_series) {
-                e_series[((int64_t)0 * m + k) * n_steps + t] = Ea;
-                e_series[((int64_t)1 * m + k) * n_steps + t] = Eb;
+            if (e_series_v && jtype) {
+                ((double *)e_series_v)[((int64_t)0 * m + k) * n_steps + t] = Ea;
+                ((double *)e_series_v)[((int64_t)1 * m + k) * n_steps + t] = Eb;
+            } else if (e_series_v) {
+                ((int32_t *)e_series_v)[((int64_t)0 * m + k) * n_steps + t] = (int32_t)Ea;
+                ((int32_t *)e_series_v)[((int64_t)1 * m + k) * n_steps + t] = (int32_t)Eb;
             }
             if (q_series) q_series[(int64_t)k * n_steps + t] = (int32_t)q;
         }
     }
+    for (int c = 0; c < 2 * m; ++c) {
+        if (jtype) ef[c] = E[c];
+        else ei[c] = (int32_t)E[c];
+    }
+    free(E);
     free(thr);
     return 0;
 }
