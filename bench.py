@@ -112,7 +112,16 @@ def bench_ea(args) -> dict:
                 "workload": "ea3d_pt_heatbath", "samples": args.samples,
                 "L_counts": {int(L): int((Ls == L).sum()) for L in (8, 12, 16)},
                 "temperatures": 32, "sets": 2, "record": "int32 E [S][2][32][steps], q [S][32][steps]"},
-            "series_bytes": int(out["E"].numel() * 4 + q.numel() * 4)}
+            "series_bytes": int(out["E"].numel() * 4 + q.numel() * 4),
+            "roofline": _step_roofline(2.0 * updates, dt, "whole PT step (sweep + swap + measure "
+                                       "launches); 2 B per site update")}
+
+
+def _step_roofline(nbytes: float, seconds: float, basis: str) -> dict:
+    """HBM roofline of a whole timed region: algorithmic bytes over wall time."""
+    achieved = nbytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None, "basis": basis}
 
 
 def bench_xy(args) -> dict:
@@ -155,6 +164,8 @@ def bench_xy(args) -> dict:
             "data": "synthetic: angles uniform from seed 99", "config": {
                 "workload": "xy2d_metropolis_2overrelax", "L": L, "replicas": R,
                 "T": "0.89 +- 0.02", "measurements": n, "sweeps_per_measurement": 3},
+            "roofline": _step_roofline(16.0 * updates, dt, "whole measurement (6 half sweeps + "
+                                       "renormalise + measure launches); 16 B per site update"),
             "acceptance_mean": float(acc.sum().item() / (R * N * n)),
             "helicity_mean": float(ups.mean().item()), "energy_per_site_mean": float(e.mean().item() / N)}
 

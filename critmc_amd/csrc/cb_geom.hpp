@@ -78,6 +78,36 @@ __host__ __device__ __forceinline__ int64_t cb_site(const CbGeo& g, int64_t k, i
     return i;
 }
 
+// cb_site with 32-bit site indices, for lattices of fewer than 2^31 sites.
+template <int Z>
+__host__ __device__ __forceinline__ uint32_t cb_site32(const CbGeo& g, uint32_t k, int c,
+                                                       uint32_t (&nb)[Z])
+{
+    const uint32_t r = fdiv(k, g.divM);
+    const int m = (int)(k - r * (uint32_t)g.M);
+    int x1 = 0, x2 = 0;
+    if (Z >= 4) {
+        const uint32_t q = fdiv(r, g.divL1);
+        x1 = (int)(r - q * (uint32_t)g.L1);
+        x2 = (int)q;
+    }
+    const int x0 = 2 * m + ((x1 + x2 + c) & 1);
+    const uint32_t L0 = (uint32_t)g.L0, L1 = (uint32_t)g.L1;
+    const uint32_t base = r * L0;
+    nb[0] = base + (uint32_t)(x0 + 1 == g.L0 ? 0 : x0 + 1);
+    nb[1] = base + (uint32_t)(x0 == 0 ? g.L0 - 1 : x0 - 1);
+    if (Z >= 4) {
+        const uint32_t row0 = (uint32_t)x2 * L1;
+        nb[2] = (row0 + (uint32_t)(x1 + 1 == g.L1 ? 0 : x1 + 1)) * L0 + (uint32_t)x0;
+        nb[3] = (row0 + (uint32_t)(x1 == 0 ? g.L1 - 1 : x1 - 1)) * L0 + (uint32_t)x0;
+    }
+    if (Z == 6) {
+        nb[4] = ((uint32_t)(x2 + 1 == g.L2 ? 0 : x2 + 1) * L1 + (uint32_t)x1) * L0 + (uint32_t)x0;
+        nb[5] = ((uint32_t)(x2 == 0 ? g.L2 - 1 : x2 - 1) * L1 + (uint32_t)x1) * L0 + (uint32_t)x0;
+    }
+    return base + (uint32_t)x0;
+}
+
 inline int cb_geo_from(int ndim, const int32_t* dims, CbGeo* g)
 {
     if (ndim < 1 || ndim > 3) return -1;

@@ -1,0 +1,528 @@
+// Cluster updates of the Ising model: Swendsen-Wang and Wolff
+// (critmc/kernels/ising.py:110-170), batched over replicas of one lattice.
+//
+// Random numbers (tag TAG_CLUSTER, key and stream of the chain, one sweep
+// counter value per cluster update):
+//   bond b = a N + i joins site i and its +e_a neighbour; it is active iff the
+//   two spins agree and word b&3 of block b>>2 (colour field 0) is below
+//   ceil(p 2^32), p = 1 - exp(-2 beta) (the reference's p_add);
+//   Swendsen-Wang: the cluster whose smallest site is r flips iff word r&3 of
+//   block r>>2 (colour field 1) has its top bit clear, i.e. u < 1/2;
+//   Wolff: the seed is floor(x N / 2^32), x = word 0 of block 0 (colour
+//   field 2); its cluster is the seed's component among sites of the seed's
+//   spin joined by active bonds.
+// Every bond has its own uniform, so a cluster does not depend on the order in
+// which it is explored: the device and a sequential host replay build the
+// same clusters.
+//
+// Swendsen-Wang runs as four launches over all replicas: parent init, union of
+// the active bonds (lock-free: a root is only ever hooked below a smaller
+// root, with atomicCAS, so each component's root ends as its smallest site),
+// flatten + cluster sizes, flip + (sum |C|^2, cluster count).  Wolff runs one
+// workgroup per replica: a breadth-first search whose frontier lives in a
+// per-replica queue, members flipped as they join (the reference's order of
+// effects), `count` clusters per launch.
+//
+// A schedule mixing local and cluster rows runs row by row: local rows through
+// cm_ising_run, cluster rows here, then the bond energy and magnetisation are
+// recounted (cm_ising_measure), as the reference recounts after
+// Swendsen-Wang.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cb_geom.hpp"
+#include "common.hpp"
+#include "philox.hpp"
+
+extern "C" {
+int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                 const double* betas, double h, const uint32_t* keys, const uint32_t* stream_ids,
+                 uint64_t* sweeps, const uint64_t* sweeps_host, const int64_t* schedule,
+                 int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run, int32_t* e_out,
+                 int32_t* m_out, int32_t* acc_out, void* workspace, int64_t workspace_bytes,
+                 int32_t form, int32_t block, int32_t grid, const int64_t* n_meas_rep,
+                 void* stream);
+int cm_ising_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                     int32_t* e_run, int32_t* m_run, void* stream);
+int64_t cm_ising_workspace_bytes(int32_t n_rep);
+}
+
+namespace cm {
+
+constexpr uint32_t TAG_CLUSTER = 9;
+
+struct ClGeo {
+    int d;
+    uint32_t L[3];
+    uint32_t N;
+    FastDiv div0, div1;
+};
+
+// Coordinates of site i (x fastest).
+__device__ __forceinline__ void cl_coords(const ClGeo& g, uint32_t i, uint32_t (&x)[3])
+{
+    const uint32_t q0 = fdiv(i, g.div0);
+    x[0] = i - q0 * g.L[0];
+    x[1] = 0;
+    x[2] = 0;
+    if (g.d >= 2) {
+        const uint32_t q1 = fdiv(q0, g.div1);
+        x[1] = q0 - q1 * g.L[1];
+        x[2] = q1;
+    }
+}
+
+__device__ __forceinline__ uint32_t cl_stride(const ClGeo& g, int a)
+{
+    return a == 0 ? 1u : (a == 1 ? g.L[0] : g.L[0] * g.L[1]);
+}
+
+// Neighbour of site i with coordinates x along axis a, direction +1 / -1.
+__device__ __forceinline__ uint32_t cl_nbr(const ClGeo& g, uint32_t i, const uint32_t (&x)[3],
+                                           int a, bool fwd)
+{
+    const uint32_t st = cl_stride(g, a), L = g.L[a];
+    if (fwd) return x[a] + 1 == L ? i - (L - 1) * st : i + st;
+    return x[a] == 0 ? i + (L - 1) * st : i - st;
+}
+
+__device__ __forceinline__ uint32_t ld_relaxed(const int32_t* p)
+{
+    return (uint32_t)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed(int32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, (int32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving; parents only ever point to smaller sites.
+__device__ __forceinline__ uint32_t cl_find(int32_t* parent, uint32_t x)
+{
+    while (true) {
+        const uint32_t p = ld_relaxed(parent + x);
+        if (p == x) return x;
+        const uint32_t gp = ld_relaxed(parent + p);
+        if (gp != p) st_relaxed(parent + x, gp);
+        x = gp;
+    }
+}
+
+// Root of x without writes: while parents are being flattened each site's
+// owner is the only writer of its entry, so every write stores a root.
+__device__ __forceinline__ uint32_t cl_find_ro(const int32_t* parent, uint32_t x)
+{
+    while (true) {
+        const uint32_t p = ld_relaxed(parent + x);
+        if (p == x) return x;
+        x = p;
+    }
+}
+
+__device__ __forceinline__ void cl_unite(int32_t* parent, uint32_t i, uint32_t j)
+{
+    while (true) {
+        i = cl_find(parent, i);
+        j = cl_find(parent, j);
+        if (i == j) return;
+        if (i > j) {
+            const uint32_t t = i;
+            i = j;
+            j = t;
+        }
+        // hook the larger root below the smaller; fails if j stopped being a root
+        if (atomicCAS(parent + j, (int32_t)j, (int32_t)i) == (int32_t)j) return;
+    }
+}
+
+__device__ __forceinline__ uint32_t cl_word(uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep,
+                                            uint32_t field, uint32_t index)
+{
+    const u32x4 r = philox4x32_10(
+        u32x4{index >> 2, (uint32_t)sweep, ctr_z(sweep, field, TAG_CLUSTER), sid}, k0, k1);
+    const uint32_t w = index & 3;
+    return w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
+}
+
+struct ClArgs {
+    int8_t* spins;          // [R][N] int8
+    int32_t* parent;        // [R][N]
+    int32_t* size;          // [R][N]
+    int32_t* queue;         // [R][N]
+    int32_t* mark;          // [R][N]
+    int64_t* stats;         // [R][2]: sum |C|^2, clusters of the last Swendsen-Wang update
+    int64_t* wsize;         // [R]: Wolff cluster sizes of the current measurement
+    const uint64_t* t_add;  // [R]
+    const uint32_t* keys;   // [R][2]
+    const uint32_t* sids;   // [R]
+    const uint64_t* sweeps; // [R]
+    uint64_t sweep_add;
+    ClGeo g;
+    int R;
+};
+
+// parent[i] = i, size[i] = 0, stats = 0
+__global__ __launch_bounds__(256) void sw_init(ClArgs a)
+{
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    int32_t* Z = a.size + (int64_t)rep * N;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        P[i] = (int32_t)i;
+        Z[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) a.stats[2 * rep + threadIdx.x] = 0;
+}
+
+// One lane per Philox block of bonds (4 bonds).
+__global__ __launch_bounds__(256) void sw_union(ClArgs a)
+{
+    const int rep = blockIdx.y;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N, nb = (uint32_t)g.d * N;
+    const int8_t* S = a.spins + (int64_t)rep * N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+    for (uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x; 4 * blk < nb;
+         blk += gridDim.x * blockDim.x) {
+        const u32x4 r = philox4x32_10(u32x4{blk, (uint32_t)sweep, zc, sid}, k0, k1);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t b = 4 * blk + t;
+            if (b >= nb) break;
+            if ((uint64_t)w[t] >= T) continue;
+            const int ax = (int)(b >= N) + (int)(b >= 2 * N);
+            const uint32_t i = b - (uint32_t)ax * N;
+            uint32_t x[3];
+            cl_coords(g, i, x);
+            const uint32_t j = cl_nbr(g, i, x, ax, true);
+            if (S[i] != S[j]) continue;
+            cl_unite(P, i, j);
+        }
+    }
+}
+
+// parent[i] = root(i); size[root] += 1.  Only lane i writes parent[i], and it
+// writes a root, so the flip pass reads every site's root.
+__global__ __launch_bounds__(256) void sw_flatten(ClArgs a)
+{
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    int32_t* Z = a.size + (int64_t)rep * N;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t r = cl_find_ro(P, i);
+        st_relaxed(P + i, r);
+        atomicAdd(Z + r, 1);
+    }
+}
+
+// Flip every cluster whose flip bit is set; sum |C|^2 and count the clusters.
+__global__ __launch_bounds__(256) void sw_flip(ClArgs a)
+{
+    __shared__ long long red[2][4];
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    int8_t* S = a.spins + (int64_t)rep * N;
+    const int32_t* P = a.parent + (int64_t)rep * N;
+    const int32_t* Z = a.size + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    long long c2 = 0, ncl = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)P[i];
+        if (r == i) {
+            const long long z = Z[i];
+            c2 += z * z;
+            ncl += 1;
+        }
+        if ((cl_word(k0, k1, sid, sweep, 1, r) >> 31) == 0) S[i] = (int8_t)(-S[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c2 += __shfl_xor(c2, off, 64);
+        ncl += __shfl_xor(ncl, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = c2;
+        red[1][threadIdx.x >> 6] = ncl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t0 = 0, t1 = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        if (t0) atomicAdd((unsigned long long*)&a.stats[2 * rep], (unsigned long long)t0);
+        if (t1) atomicAdd((unsigned long long*)&a.stats[2 * rep + 1], (unsigned long long)t1);
+    }
+}
+
+__global__ void cl_fill(int32_t* p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// `count` Wolff clusters per replica; one workgroup per replica.
+__global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
+{
+    __shared__ uint32_t s_head, s_tail;
+    __shared__ int s_s0;
+    const int rep = blockIdx.x;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N;
+    int8_t* S = a.spins + (int64_t)rep * N;
+    int32_t* Q = a.queue + (int64_t)rep * N;
+    int32_t* Mk = a.mark + (int64_t)rep * N;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const int z = 2 * g.d;
+    long long total = 0;
+    for (int f = 0; f < count; ++f) {
+        const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
+        if (threadIdx.x == 0) {
+            const uint32_t x = cl_word(k0, k1, sid, sweep, 2, 0);
+            const uint32_t seed = (uint32_t)(((uint64_t)x * N) >> 32);
+            const int s0 = S[seed];
+            S[seed] = (int8_t)(-s0);
+            Mk[seed] = f;
+            Q[0] = (int32_t)seed;
+            s_s0 = s0;
+            s_head = 0;
+            s_tail = 1;
+        }
+        __syncthreads();
+        const int s0 = s_s0;
+        const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+        while (true) {
+            const uint32_t lo = s_head, hi = s_tail;
+            if (lo == hi) break;
+            for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+                const uint32_t k = (uint32_t)Q[q];
+                uint32_t x[3];
+                cl_coords(g, k, x);
+                for (int c = 0; c < z; ++c) {
+                    const int ax = c >> 1;
+                    const uint32_t j = cl_nbr(g, k, x, ax, (c & 1) == 0);
+                    if (S[j] != s0) continue;
+                    const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
+                    const u32x4 r = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+                    const uint32_t w = b & 3;
+                    const uint32_t u = w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
+                    if ((uint64_t)u >= T) continue;
+                    if (atomicExch(Mk + j, f) == f) continue;  // another lane added it
+                    S[j] = (int8_t)(-s0);
+                    Q[atomicAdd(&s_tail, 1u)] = (int32_t)j;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_head = hi;
+            __syncthreads();
+        }
+        total += s_tail;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.wsize[rep] += total;
+}
+
+__global__ void cl_advance(uint64_t* sweeps, int R, uint64_t n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < R) sweeps[t] += n;
+}
+
+// aux[rep][meas][4] = (accepted, wolff size, sum |C|^2, clusters); e/m record.
+__global__ void cl_record(int R, int64_t n_meas, int64_t meas, const int32_t* e_run,
+                          const int32_t* m_run, int32_t* e_out, int32_t* m_out,
+                          const int64_t* acc, const int64_t* wsize, const int64_t* stats,
+                          int have_sw, int64_t* aux)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= R) return;
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    e_out[o] = e_run[rep];
+    m_out[o] = m_run[rep];
+    aux[4 * o + 0] = acc[rep];
+    aux[4 * o + 1] = wsize[rep];
+    aux[4 * o + 2] = have_sw ? stats[2 * rep] : 0;
+    aux[4 * o + 3] = have_sw ? stats[2 * rep + 1] : 0;
+}
+
+__global__ void cl_add_acc(int R, const int32_t* row_acc, int64_t* acc)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep < R) acc[rep] += row_acc[rep];
+}
+
+__global__ void cl_zero64(int64_t* p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+inline uint64_t cl_threshold(double p)
+{
+    if (!(p < 1.0)) return 1ull << 32;
+    if (!(p > 0.0)) return 0;
+    return (uint64_t)std::ceil(p * 4294967296.0);
+}
+
+struct ClLayout {
+    int64_t tab, parent, size, queue, mark, stats, wsize, acc, tadd, rowacc, rowe, rowm, total;
+};
+
+inline ClLayout cl_layout(int64_t N, int32_t n_rep)
+{
+    auto up = [](int64_t x) { return (x + 255) / 256 * 256; };
+    ClLayout l{};
+    int64_t o = 0;
+    l.tab = o;    o += up(cm_ising_workspace_bytes(n_rep));
+    l.parent = o; o += up(4 * N * n_rep);
+    l.size = o;   o += up(4 * N * n_rep);
+    l.queue = o;  o += up(4 * N * n_rep);
+    l.mark = o;   o += up(4 * N * n_rep);
+    l.stats = o;  o += up(16 * (int64_t)n_rep);
+    l.wsize = o;  o += up(8 * (int64_t)n_rep);
+    l.acc = o;    o += up(8 * (int64_t)n_rep);
+    l.tadd = o;   o += up(8 * (int64_t)n_rep);
+    l.rowacc = o; o += up(4 * (int64_t)n_rep);
+    l.rowe = o;   o += up(4 * (int64_t)n_rep);
+    l.rowm = o;   o += up(4 * (int64_t)n_rep);
+    l.total = o;
+    return l;
+}
+
+}  // namespace cm
+
+using namespace cm;
+
+extern "C" {
+
+int64_t cm_ising_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep)
+{
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= dims[a];
+    return cl_layout(N, n_rep).total;
+}
+
+int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                         const double* betas, double h, const uint32_t* keys,
+                         const uint32_t* stream_ids, uint64_t* sweeps, uint64_t* sweeps_host,
+                         const int64_t* schedule, int32_t n_rows, int64_t n_meas, int32_t* e_run,
+                         int32_t* m_run, int32_t* e_out, int32_t* m_out, int64_t* aux_out,
+                         void* workspace, int64_t workspace_bytes, void* stream)
+{
+    CM_REQUIRE(ndim >= 1 && ndim <= 3, "lattices of 1..3 dimensions");
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) {
+        CM_REQUIRE(dims[a] >= 2 && dims[a] % 2 == 0, "extents must be even and >= 2");
+        N *= dims[a];
+    }
+    CM_REQUIRE(N * ndim < ((int64_t)1 << 31), "lattice too large for cluster updates");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    CM_REQUIRE(sweeps_host != nullptr, "the host sweep counters are required");
+    const ClLayout lay = cl_layout(N, n_rep);
+    CM_REQUIRE(workspace_bytes >= lay.total, "workspace too small");
+    bool any_cluster = false, any_sw = false;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_WOLFF ||
+                       code == CODE_SWENDSEN_WANG,
+                   "update code %lld has no Ising GPU kernel", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 24), "repetition count out of range");
+        any_cluster |= code == CODE_WOLFF || code == CODE_SWENDSEN_WANG;
+        any_sw |= code == CODE_SWENDSEN_WANG;
+    }
+    CM_REQUIRE(!any_cluster || h == 0.0, "cluster updates require zero field h");
+    if (n_meas == 0) return CM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    std::vector<uint64_t> tadd(n_rep);
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        tadd[r] = cl_threshold(1.0 - std::exp(-2.0 * betas[r]));
+    }
+    CM_HIP_CHECK(upload_async(ws + lay.tadd, tadd.data(), tadd.size() * 8, st));
+    ClArgs a{};
+    a.spins = (int8_t*)spins;
+    a.parent = (int32_t*)(ws + lay.parent);
+    a.size = (int32_t*)(ws + lay.size);
+    a.queue = (int32_t*)(ws + lay.queue);
+    a.mark = (int32_t*)(ws + lay.mark);
+    a.stats = (int64_t*)(ws + lay.stats);
+    a.wsize = (int64_t*)(ws + lay.wsize);
+    a.t_add = (const uint64_t*)(ws + lay.tadd);
+    a.keys = keys;
+    a.sids = stream_ids;
+    a.sweeps = sweeps;
+    a.g.d = ndim;
+    for (int k = 0; k < 3; ++k) a.g.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
+    a.g.N = (uint32_t)N;
+    a.g.div0 = make_fastdiv(a.g.L[0]);
+    a.g.div1 = make_fastdiv(a.g.L[1]);
+    a.R = n_rep;
+    int64_t* acc = (int64_t*)(ws + lay.acc);
+    int32_t* row_acc = (int32_t*)(ws + lay.rowacc);
+    int32_t* row_e = (int32_t*)(ws + lay.rowe);
+    int32_t* row_m = (int32_t*)(ws + lay.rowm);
+    const int rb = (n_rep + 255) / 256;
+    const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
+    const unsigned bond_chunks = (unsigned)std::min<int64_t>((ndim * N / 4 + 256) / 256, 1024);
+    hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, a.wsize, n_rep);
+        for (int row = 0; row < n_rows; ++row) {
+            const int64_t code = schedule[2 * row], count = schedule[2 * row + 1];
+            if (code == CODE_METROPOLIS || code == CODE_HEATBATH) {
+                const int64_t one[2] = {code, count};
+                int rc = cm_ising_run(spins, 0, ndim, dims, n_rep, betas, h, keys, stream_ids,
+                                      sweeps, sweeps_host, one, 1, 1, e_run, m_run, row_e, row_m,
+                                      row_acc, ws + lay.tab, lay.parent - lay.tab, 0, 0, 0,
+                                      nullptr, stream);
+                if (rc != CM_OK) return rc;
+                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
+            } else if (code == CODE_SWENDSEN_WANG) {
+                for (int64_t k = 0; k < count; ++k) {
+                    a.sweep_add = (uint64_t)k;
+                    hipLaunchKernelGGL(sw_init, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
+                    hipLaunchKernelGGL(sw_union, dim3(bond_chunks, n_rep), dim3(256), 0, st, a);
+                    hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
+                    hipLaunchKernelGGL(sw_flip, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
+                }
+                hipLaunchKernelGGL(cl_advance, dim3(rb), dim3(256), 0, st, sweeps, n_rep, (uint64_t)count);
+            } else {
+                a.sweep_add = 0;
+                const int64_t nm = N * n_rep;
+                hipLaunchKernelGGL(cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
+                                   dim3(256), 0, st, a.mark, nm, -1);
+                hipLaunchKernelGGL(wolff_run, dim3(n_rep), dim3(256), 0, st, a, (int)count);
+                hipLaunchKernelGGL(cl_advance, dim3(rb), dim3(256), 0, st, sweeps, n_rep, (uint64_t)count);
+            }
+            if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
+                for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
+                int rc = cm_ising_measure(spins, ndim, dims, n_rep, e_run, m_run, stream);
+                if (rc != CM_OK) return rc;
+            } else {
+                for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
+            }
+        }
+        hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas, e_run,
+                           m_run, e_out, m_out, acc, a.wsize, a.stats, any_sw ? 1 : 0, aux_out);
+    }
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+}  // extern "C"

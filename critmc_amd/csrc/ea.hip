@@ -132,7 +132,10 @@ __device__ __forceinline__ void ea_coords(uint32_t i, uint32_t L, const FastDiv&
     }
 }
 
-template <int Z, bool G>
+// LDS: every sample's configuration fits EA_LDS_MAX bytes and is staged in
+// LDS for the sweep (so spin reads compile to LDS loads); otherwise the sweep
+// works on global memory.
+template <int Z, bool G, bool LDS>
 __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
 {
     extern __shared__ int8_t lds8[];
@@ -146,9 +149,8 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
     int8_t* gs = a.spins + a.spin_off[s] + (int64_t)rc * N;
     const int8_t* J = G ? nullptr : a.J + a.j_off[s];
     const double* Jf = G ? a.Jf + a.j_off[s] : nullptr;
-    const bool use_lds = N <= EA_LDS_MAX;
-    int8_t* S = use_lds ? lds8 : gs;
-    if (use_lds)
+    int8_t* S = LDS ? lds8 : gs;
+    if (LDS)
         for (int64_t t = threadIdx.x; t < N; t += blockDim.x) lds8[t] = gs[t];
     const int64_t cfg = (int64_t)s * 2 * m + rc;
     const int k = a.kofc[cfg];
@@ -170,15 +172,16 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
                 for (int t = 0; t < 4; ++t) {
                     const int64_t kk = 4 * qd + t;
                     if (kk >= g.half) break;
-                    int64_t nb[Z];
-                    const int64_t i = cb_site<Z>(g, kk, c, nb);
+                    uint32_t nb[Z];
+                    const uint32_t i = cb_site32<Z>(g, (uint32_t)kk, c, nb);
+                    const uint32_t N32 = (uint32_t)N;
                     const int si = S[i];
                     if constexpr (G) {
                         double h = 0.0;
 #pragma unroll
                         for (int ax = 0; ax < Z / 2; ++ax) {
-                            h += Jf[ax * N + i] * (double)S[nb[2 * ax]];
-                            h += Jf[ax * N + nb[2 * ax + 1]] * (double)S[nb[2 * ax + 1]];
+                            h += Jf[ax * N32 + i] * (double)S[nb[2 * ax]];
+                            h += Jf[ax * N32 + nb[2 * ax + 1]] * (double)S[nb[2 * ax + 1]];
                         }
                         const double u = (double)xs[t] * 0x1p-32;
                         if (a.code == CODE_METROPOLIS) {
@@ -198,8 +201,8 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
                         int h = 0;
 #pragma unroll
                         for (int ax = 0; ax < Z / 2; ++ax) {
-                            h += J[ax * N + i] * S[nb[2 * ax]];
-                            h += J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
+                            h += J[ax * N32 + i] * S[nb[2 * ax]];
+                            h += J[ax * N32 + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
                         }
                         if (a.code == CODE_METROPOLIS) {
                             const int aa = si * h;
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
         }
         a.sweeps[cfg] = sweep;
     }
-    if (use_lds)
+    if (LDS)
         for (int64_t t = threadIdx.x; t < N; t += blockDim.x) gs[t] = lds8[t];
 }
 
@@ -463,14 +466,16 @@ __global__ void ea_init_kernel(EaArgs a)
 
 template <bool G>
 static void launch_step(const EaArgs& a, int ndim, int sweep_blocks, int block, size_t lds,
-                        int swap_threads, uint64_t step, int64_t t, hipStream_t st)
+                        int swap_threads, uint64_t step, int64_t t, hipStream_t st, bool all_lds)
 {
     if (ndim == 2) {
-        hipLaunchKernelGGL((ea_sweep<4, G>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        if (all_lds) hipLaunchKernelGGL((ea_sweep<4, G, true>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        else hipLaunchKernelGGL((ea_sweep<4, G, false>), dim3(sweep_blocks), dim3(block), 0, st, a);
         hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
         hipLaunchKernelGGL((ea_measure<4, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
     } else {
-        hipLaunchKernelGGL((ea_sweep<6, G>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        if (all_lds) hipLaunchKernelGGL((ea_sweep<6, G, true>), dim3(sweep_blocks), dim3(block), lds, st, a);
+        else hipLaunchKernelGGL((ea_sweep<6, G, false>), dim3(sweep_blocks), dim3(block), 0, st, a);
         hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
         hipLaunchKernelGGL((ea_measure<6, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
     }
@@ -622,7 +627,8 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
     if (block <= 0) block = 64;
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
     const int64_t maxN = (int64_t)max_L * max_L * (ndim == 3 ? max_L : 1);
-    const size_t lds = (size_t)std::min<int64_t>(maxN, EA_LDS_MAX);
+    const bool all_lds = maxN <= EA_LDS_MAX;
+    const size_t lds = all_lds ? (size_t)maxN : 0;
     const int sweep_blocks = n_samples * 2 * m;
     const int swap_threads = 2 * n_samples;
     int64_t next_edge = 1;
@@ -634,8 +640,8 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
             next_edge = ((int64_t)1 << (bin + 1)) - 1;
             if (jtype) launch_energy(a, 1, st);
         }
-        if (jtype) launch_step<true>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st);
-        else launch_step<false>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st);
+        if (jtype) launch_step<true>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st, all_lds);
+        else launch_step<false>(a, ndim, sweep_blocks, block, lds, swap_threads, step, t, st, all_lds);
     }
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;

@@ -33,7 +33,7 @@ N_AUX = 4
 
 # codes every checkerboard GPU kernel implements, per model kind
 GPU_UPDATES = {
-    "ising": ("metropolis", "heatbath"),
+    "ising": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
     "potts": ("metropolis", "heatbath"),
     "on": ("metropolis", "heatbath", "overrelax"),
 }

@@ -1,8 +1,10 @@
 """Checkerboard Ising kernels: host side of ``cm_ising_*``.
 
 Replaces critmc/kernels/ising.py's ``run`` (:173-237) for the Metropolis
-(:68-87) and heat-bath (:90-107) sweeps.  Spins stay on the device as int8
-``[R, N]`` (x fastest) or as the packed ``[R, 2, rows, G]`` u32 image.
+(:68-87) and heat-bath (:90-107) sweeps, and with :func:`run_cluster` for
+schedules with Wolff / Swendsen-Wang rows (:110-170).  Spins stay on the
+device as int8 ``[R, N]`` (x fastest) or as the packed ``[R, 2, rows, G]`` u32
+image.
 """
 
 from __future__ import annotations
@@ -16,7 +18,7 @@ import torch
 from .._lib import check, lib
 from ..device import device, host_ptr, ptr, stream
 from .chains import Chains
-from .common import HEATBATH, METROPOLIS
+from .common import HEATBATH, METROPOLIS, SWENDSEN_WANG, WOLFF
 
 LAYOUT_INT8 = 0
 LAYOUT_PACKED = 1
@@ -127,3 +129,45 @@ def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: n
     if n_meas_rep is None:
         chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return e_out, m_out, acc_out
+
+
+class ClusterWorkspace:
+    """Device scratch of :func:`run_cluster` (tables, parent/size/queue/mark arrays)."""
+
+    def __init__(self, shape, R: int, dev: torch.device | None = None):
+        d = _dims(shape)
+        self.nbytes = int(lib().cm_ising_cluster_workspace_bytes(len(d), host_ptr(d, ctypes.c_int32), R))
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=dev or device())
+
+
+def run_cluster(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule,
+                n_meas: int, e_run: torch.Tensor, m_run: torch.Tensor, *,
+                ws: ClusterWorkspace | None = None):
+    """Apply a schedule that may contain Wolff (3) and Swendsen-Wang (4) rows.
+
+    Returns device ``(E_b, M)`` int32 ``[R, n_meas]`` and ``aux`` int64
+    ``[R, n_meas, 4]`` (accepted, Wolff sizes, sum |C|^2, clusters).  Every
+    repetition of every row advances the chains' sweep counters by one.
+    """
+    R = chains.R
+    betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, dtype=np.float64), (R,)))
+    sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+    for code in sched[:, 0]:
+        if code not in (METROPOLIS, HEATBATH, WOLFF, SWENDSEN_WANG):
+            raise ValueError(f"update code {int(code)} has no Ising GPU kernel")
+    if h != 0.0 and np.isin(sched[:, 0], (WOLFF, SWENDSEN_WANG)).any():
+        raise ValueError("cluster updates require zero field h")
+    d = _dims(shape)
+    dev = spins.device
+    e_out = torch.empty((R, n_meas), dtype=torch.int32, device=dev)
+    m_out = torch.empty((R, n_meas), dtype=torch.int32, device=dev)
+    aux = torch.empty((R, n_meas, 4), dtype=torch.int64, device=dev)
+    ws = ws or ClusterWorkspace(shape, R, dev)
+    rc = lib().cm_ising_cluster_run(
+        ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, host_ptr(betas, ctypes.c_double),
+        float(h), ptr(chains.keys_d), ptr(chains.sids_d), ptr(chains.sweeps_d),
+        host_ptr(chains.sweeps_h, ctypes.c_uint64), host_ptr(sched, ctypes.c_int64),
+        sched.shape[0], int(n_meas), ptr(e_run), ptr(m_run), ptr(e_out), ptr(m_out), ptr(aux),
+        ptr(ws.buf), ws.nbytes, stream())
+    check(rc, "cm_ising_cluster_run")
+    return e_out, m_out, aux

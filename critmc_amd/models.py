@@ -27,7 +27,7 @@ from .kernels import ising as k_ising
 from .kernels import on as k_on
 from .kernels import potts as k_potts
 from .kernels.chains import Chains
-from .kernels.common import GPU_UPDATES, N_AUX, UPDATE_CODES
+from .kernels.common import CLUSTER_CODES, GPU_UPDATES, N_AUX, UPDATE_CODES
 from .lattice import Lattice, make_lattice, require_checkerboard
 
 __all__ = ["Model", "Ising", "Potts", "ON", "XY", "Heisenberg", "build_model", "derive",
@@ -158,13 +158,21 @@ class Ising(Model):
         return k_ising.Workspace(1)
 
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
-        e_out, m_out, acc = k_ising.run(ds.spins, self.lattice.shape, [float(beta)], self.h,
-                                        chains, schedule, n_meas, ds.extra["e"], ds.extra["m"],
-                                        ws=ws, form=_FORM[0])
+        if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
+            if "cluster_ws" not in ds.extra:
+                ds.extra["cluster_ws"] = k_ising.ClusterWorkspace(self.lattice.shape, 1)
+            e_out, m_out, aux_d = k_ising.run_cluster(
+                ds.spins, self.lattice.shape, [float(beta)], self.h, chains, schedule, n_meas,
+                ds.extra["e"], ds.extra["m"], ws=ds.extra["cluster_ws"])
+            aux = aux_d[0].cpu().numpy()
+        else:
+            e_out, m_out, acc = k_ising.run(ds.spins, self.lattice.shape, [float(beta)], self.h,
+                                            chains, schedule, n_meas, ds.extra["e"],
+                                            ds.extra["m"], ws=ws, form=_FORM[0])
+            aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
+            aux[:, 0] = acc[0].cpu().numpy()
         e_b = e_out[0].cpu().numpy().astype(np.int64)
         m = m_out[0].cpu().numpy().astype(np.int64)
-        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
-        aux[:, 0] = acc[0].cpu().numpy()
         running["e_b"] = int(e_b[-1])
         running["m"] = int(m[-1])
         energy = e_b - self.h * m if self.h != 0.0 else e_b

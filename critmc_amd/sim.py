@@ -143,6 +143,10 @@ class Simulation:
         self.schedule = np.array(
             [[UPDATE_CODES[name], 1 if k is None else k] for name, k in rows], dtype=np.int64
         )
+        # a "wolff" row without a count is tuned towards N / <|C|> clusters (critmc/sim.py:173)
+        self._auto_wolff = [r for r, (name, k) in enumerate(rows) if name == "wolff" and k is None]
+        self._mean_cluster = float("nan")
+        self._wolff_history: list[int] = []
         host = model.initial_state(start, self.rng) if state is None else np.array(state, copy=True)
         self.start = "given" if state is not None else start
         self.chains = Chains([chain_key(self.rng)])
@@ -172,6 +176,7 @@ class Simulation:
         n_chunks = max(1, min(n_chunks, n_meas))
         sizes = np.diff(np.linspace(0, n_meas, n_chunks + 1).astype(int))
         n_metro = self._count(METROPOLIS, METROPOLIS_RANDOM)
+        n_wolff = self._count(WOLFF)
         for chunk in sizes:
             if chunk == 0:
                 continue
@@ -179,6 +184,19 @@ class Simulation:
             if not tune:
                 continue
             aux = raw["aux"]
+            if n_wolff:
+                # critmc/sim.py:208-225: the later half of the chunk is the better equilibrated
+                late = aux[len(aux) // 2:, AUX_WOLFF_SIZE]
+                mean_c = late.sum() / (n_wolff * len(late))
+                self._mean_cluster = float(mean_c)
+                if self._auto_wolff and mean_c > 0:
+                    target = max(1, int(round(self.model.n_sites / mean_c)))
+                    for row in self._auto_wolff:
+                        current = int(self.schedule[row, 1])
+                        # at most a factor 4 per chunk: a hot start has tiny clusters
+                        self.schedule[row, 1] = int(np.clip(target, max(1, current // 4), 4 * current))
+                    self._wolff_history.append(int(self.schedule[self._auto_wolff[0], 1]))
+                    n_wolff = self._count(WOLFF)
             if "step" in self.running and n_metro:
                 rate = aux[:, AUX_ACCEPTED].sum() / (n_metro * self.model.n_sites * len(aux))
                 factor = float(np.clip(rate / 0.5, 0.5, 2.0))
@@ -188,6 +206,11 @@ class Simulation:
 
     def tuning(self) -> dict[str, Any]:
         info: dict[str, Any] = {"schedule": self.schedule.tolist()}
+        if self._count(WOLFF):
+            info["wolff_per_meas"] = self._count(WOLFF)
+            info["mean_cluster"] = self._mean_cluster
+            if self._wolff_history:
+                info["wolff_history"] = list(self._wolff_history)
         if "step" in self.running:
             info["step"] = self.running["step"]
         return info
@@ -217,7 +240,12 @@ class Simulation:
         self.n_meas += n_meas
         aux = raw["aux"]
         n = self.model.n_sites
-        sweeps = self._count(METROPOLIS, METROPOLIS_RANDOM, HEATBATH, OVERRELAX, SWENDSEN_WANG)
+        n_wolff = self._count(WOLFF)
+        mean_cluster = (float(aux[:, AUX_WOLFF_SIZE].sum() / (n_wolff * n_meas)) if n_wolff
+                        else float("nan"))
+        # critmc/sim.py:271-277: Wolff clusters count as their share of a sweep
+        sweeps = self._count(METROPOLIS, METROPOLIS_RANDOM, HEATBATH, OVERRELAX, SWENDSEN_WANG) + (
+            n_wolff * mean_cluster / n if n_wolff else 0.0)
         n_single = self._count(METROPOLIS, METROPOLIS_RANDOM)
         meta = {
             **self.model.meta(),
@@ -229,7 +257,7 @@ class Simulation:
             "n_therm": self.n_therm,
             "n_meas": n_meas,
             "sweeps_per_meas": sweeps,
-            "mean_cluster": float("nan"),
+            "mean_cluster": mean_cluster,
             "acceptance": (
                 float(aux[:, AUX_ACCEPTED].sum() / (n_single * n * n_meas)) if n_single else None
             ),
@@ -242,4 +270,4 @@ class Simulation:
         return TimeSeries(raw, meta)
 
 
-_ = (AUX_WOLFF_SIZE, WOLFF)
+

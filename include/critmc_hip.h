@@ -104,6 +104,28 @@ int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *d
                  void *workspace_dev, int64_t workspace_bytes, int32_t form, int32_t block,
                  int32_t grid, const int64_t *n_meas_rep_dev, void *stream);
 
+/* Bytes of caller-owned device scratch cm_ising_cluster_run needs (tables plus
+ * parent, size, queue and mark arrays of n_rep x N int32). */
+int64_t cm_ising_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep);
+
+/* Any Ising schedule, including cluster rows: code 3 = Wolff (one cluster per
+ * repetition), 4 = Swendsen-Wang (one update per repetition), mixed with
+ * Metropolis / heat-bath rows; replaces critmc/kernels/ising.py:173-237 (run)
+ * together with wolff_update / sw_update (:110-170).  Int8 spins [n_rep][N].
+ * Records E_b, M (int32 [n_rep][n_meas]) and the reference's aux columns
+ * aux_out int64 [n_rep][n_meas][4] = (accepted, Wolff cluster sizes summed over
+ * the measurement, sum |C|^2 and cluster count of the measurement's last
+ * Swendsen-Wang update).  Every repetition of any row advances the chain's
+ * sweep counter by one; sweeps_host mirrors sweeps_dev and is advanced here.
+ * Cluster rows need h == 0.  Local rows use the same kernels as cm_ising_run. */
+int cm_ising_cluster_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
+                         const double *betas, double h, const uint32_t *keys_dev,
+                         const uint32_t *stream_ids_dev, uint64_t *sweeps_dev,
+                         uint64_t *sweeps_host, const int64_t *schedule, int32_t n_rows,
+                         int64_t n_meas, int32_t *e_run_dev, int32_t *m_run_dev,
+                         int32_t *e_out_dev, int32_t *m_out_dev, int64_t *aux_out_dev,
+                         void *workspace_dev, int64_t workspace_bytes, void *stream);
+
 /* ---- Potts (critmc/kernels/potts.py) -------------------------------------- */
 
 /* Bytes of caller-owned device scratch cm_potts_run needs for n_rep chains. */

@@ -68,6 +68,8 @@ def _declare(L: ctypes.CDLL) -> None:
         _i64p, ctypes.c_int, ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p,
     ]
     L.orc_ising_run.restype = ctypes.c_int
+    L.orc_ising_run_aux.argtypes = L.orc_ising_run.argtypes + [_i64p]
+    L.orc_ising_run_aux.restype = ctypes.c_int
     L.orc_init_spins.argtypes = [_i8p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_uint32]
     L.orc_init_spins.restype = None
@@ -138,16 +140,17 @@ class IsingReplay:
         e_out = np.zeros(n_meas, dtype=np.int64)
         m_out = np.zeros(n_meas, dtype=np.int64)
         acc = np.zeros(n_meas, dtype=np.int64)
-        rc = lib().orc_ising_run(
+        aux = np.zeros((n_meas, 4), dtype=np.int64)
+        rc = lib().orc_ising_run_aux(
             _ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.beta, self.h,
             self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p),
             _ptr(sched, _i64p), sched.shape[0], int(n_meas),
             _ptr(self.e_b, _i64p), _ptr(self.m, _i64p),
-            _ptr(e_out, _i64p), _ptr(m_out, _i64p), _ptr(acc, _i64p),
+            _ptr(e_out, _i64p), _ptr(m_out, _i64p), _ptr(acc, _i64p), _ptr(aux, _i64p),
         )
         if rc:
             raise ValueError(f"orc_ising_run failed ({rc})")
-        return {"Eb": e_out, "M": m_out, "accepted": acc}
+        return {"Eb": e_out, "M": m_out, "accepted": acc, "aux": aux}
 
 
 os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))

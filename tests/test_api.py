@@ -31,9 +31,9 @@ def test_lattice_contract():
 
 def test_model_validation():
     m = Ising(square(4))
-    m.check_updates(["metropolis", "heatbath"])
+    m.check_updates(["metropolis", "heatbath", "wolff", "sw"])
     with pytest.raises(ValueError, match="checkerboard GPU kernel"):
-        m.check_updates(["wolff"])
+        m.check_updates(["metropolis-random"])
     with pytest.raises(ValueError):
         m.check_updates(["overrelax"])
     with pytest.raises(ValueError, match="bipartite"):

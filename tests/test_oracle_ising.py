@@ -123,6 +123,10 @@ def _pull(oracle_lib, shape, beta, h, sched, n_meas, seed, beta_true=None):
     ((4, 4), 0.3, 0.4, [[2, 1]]),
     ((8,), 0.8, 0.0, [[0, 1], [2, 1]]),
     ((2, 2, 4), 0.3, 0.0, [[0, 1]]),
+    ((4, 4), 0.44, 0.0, [[4, 1]]),
+    ((4, 4), 0.44, 0.0, [[3, 2]]),
+    ((2, 6), 0.6, 0.0, [[0, 1], [4, 1]]),
+    ((2, 2, 4), 0.25, 0.0, [[3, 1], [2, 1]]),
 ])
 def test_replay_matches_enumeration(oracle_lib, shape, beta, h, sched):
     zs = _pull(oracle_lib, shape, beta, h, sched, 200_000, seed=sum(shape))
@@ -132,3 +136,34 @@ def test_replay_matches_enumeration(oracle_lib, shape, beta, h, sched):
 def test_planted_wrong_beta_is_caught(oracle_lib):
     zs = _pull(oracle_lib, (4, 4), 0.35 * 0.9, 0.0, [[0, 1]], 200_000, seed=8, beta_true=0.35)
     assert max(abs(z) for z in zs) > 4.5, zs
+
+
+@pytest.mark.parametrize("sched", [[[4, 1]], [[3, 3]]])
+def test_planted_wrong_beta_is_caught_by_cluster_updates(oracle_lib, sched):
+    zs = _pull(oracle_lib, (4, 4), 0.44 * 0.9, 0.0, sched, 200_000, seed=8, beta_true=0.44)
+    assert max(abs(z) for z in zs) > 4.5, zs
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (6, 4, 2), (16,)])
+def test_cluster_diagnostics_and_bookkeeping(oracle_lib, shape):
+    n = int(np.prod(shape))
+    r = oracle_lib.IsingReplay(oracle_lib.ising_init(n, (2, 3), 0), shape, 0.4, 0.0, (7, 8))
+    out = r.run([[0, 1], [3, 2], [4, 1]], 40)
+    aux = out["aux"]
+    assert np.array_equal(aux[:, 0], out["accepted"])
+    assert np.all(aux[:, 1] >= 2) and np.all(aux[:, 1] <= 2 * n)  # two Wolff clusters
+    assert np.all(aux[:, 3] >= 1) and np.all(aux[:, 3] <= n)      # SW cluster count
+    assert np.all(aux[:, 2] >= n) and np.all(aux[:, 2] <= n * n)  # sum |C|^2 over a partition
+    assert oracle_lib.ising_energy(r.s, shape) == (int(out["Eb"][-1]), int(out["M"][-1]))
+    assert int(r.sweep[0]) == 40 * 4
+
+
+def test_sw_at_infinite_temperature_flips_isolated_sites(oracle_lib):
+    """beta = 0 activates no bond: n singleton clusters, each flipped with
+    probability 1/2 by its own Philox bit."""
+    shape, n = (8, 8), 64
+    r = oracle_lib.IsingReplay(np.ones(n, np.int8), shape, 0.0, 0.0, (4, 4))
+    out = r.run([[4, 1]], 400)
+    assert np.all(out["aux"][:, 3] == n) and np.all(out["aux"][:, 2] == n)
+    frac_down = (n - out["M"]) / (2 * n)
+    assert abs(frac_down.mean() - 0.5) < 0.01
