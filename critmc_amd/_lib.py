@@ -54,6 +54,12 @@ SIGNATURES: dict[str, tuple[list, object]] = {
         _c_i32,
     ),
     "cm_potts_workspace_bytes": ([_c_i32], _c_i64),
+    "cm_potts_cluster_workspace_bytes": ([_c_i32, _i32p, _c_i32, _c_i32], _c_i64),
+    "cm_potts_cluster_run": (
+        [_vp, _c_i32, _i32p, _c_i32, _c_i32, _f64p, _vp, _vp, _vp, _i64p, _c_i32, _c_i64, _vp,
+         _vp, _vp, _vp, _vp, _vp, _c_i64, _vp],
+        _c_i32,
+    ),
     "cm_potts_init": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _c_i32, _vp], _c_i32),
     "cm_potts_measure": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
     "cm_potts_run": (

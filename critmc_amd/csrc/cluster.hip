@@ -1,16 +1,21 @@
-// Cluster updates of the Ising model: Swendsen-Wang and Wolff
-// (critmc/kernels/ising.py:110-170), batched over replicas of one lattice.
+// Cluster updates of the Ising and q-state Potts models: Swendsen-Wang and
+// Wolff (critmc/kernels/ising.py:110-170, critmc/kernels/potts.py:106-167),
+// batched over replicas of one lattice.  Ising spins are int8 +-1 (q = 0
+// below), Potts states int8 0..q-1.
 //
 // Random numbers (tag TAG_CLUSTER, key and stream of the chain, one sweep
 // counter value per cluster update):
 //   bond b = a N + i joins site i and its +e_a neighbour; it is active iff the
-//   two spins agree and word b&3 of block b>>2 (colour field 0) is below
-//   ceil(p 2^32), p = 1 - exp(-2 beta) (the reference's p_add);
-//   Swendsen-Wang: the cluster whose smallest site is r flips iff word r&3 of
-//   block r>>2 (colour field 1) has its top bit clear, i.e. u < 1/2;
+//   two states agree and word b&3 of block b>>2 (colour field 0) is below
+//   ceil(p 2^32), p = 1 - exp(-2 beta) (Ising) or 1 - exp(-beta) (Potts), the
+//   reference's p_add;
+//   Swendsen-Wang: with x = word r&3 of block r>>2 (colour field 1) for the
+//   cluster whose smallest site is r, an Ising cluster flips iff x's top bit
+//   is clear (u < 1/2) and a Potts cluster takes state floor(x q / 2^32);
 //   Wolff: the seed is floor(x N / 2^32), x = word 0 of block 0 (colour
-//   field 2); its cluster is the seed's component among sites of the seed's
-//   spin joined by active bonds.
+//   field 2); the new state is -a (Ising) or randbelow(q - 1) from word 1,
+//   +1 if >= a (Potts); the cluster is the seed's component among sites of
+//   the seed's state a joined by active bonds.
 // Every bond has its own uniform, so a cluster does not depend on the order in
 // which it is explored: the device and a sequential host replay build the
 // same clusters.
@@ -24,9 +29,9 @@
 // effects), `count` clusters per launch.
 //
 // A schedule mixing local and cluster rows runs row by row: local rows through
-// cm_ising_run, cluster rows here, then the bond energy and magnetisation are
-// recounted (cm_ising_measure), as the reference recounts after
-// Swendsen-Wang.
+// cm_ising_run / cm_potts_run, cluster rows here, then the energy and
+// magnetisation or state counts are recounted (cm_ising_measure /
+// cm_potts_measure), as the reference recounts after Swendsen-Wang.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -46,6 +51,15 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
 int cm_ising_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
                      int32_t* e_run, int32_t* m_run, void* stream);
 int64_t cm_ising_workspace_bytes(int32_t n_rep);
+int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                 const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
+                 uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
+                 int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
+                 int32_t* counts_out, void* workspace, int64_t workspace_bytes, int32_t block,
+                 int32_t grid, const int64_t* n_meas_rep, void* stream);
+int cm_potts_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                     int32_t q, int32_t* e_run, int32_t* counts_run, void* stream);
+int64_t cm_potts_workspace_bytes(int32_t n_rep);
 }
 
 namespace cm {
@@ -160,6 +174,7 @@ struct ClArgs {
     uint64_t sweep_add;
     ClGeo g;
     int R;
+    int q;                  // 0: Ising +-1, else Potts states 0..q-1
 };
 
 // parent[i] = i, size[i] = 0, stats = 0
@@ -242,7 +257,12 @@ __global__ __launch_bounds__(256) void sw_flip(ClArgs a)
             c2 += z * z;
             ncl += 1;
         }
-        if ((cl_word(k0, k1, sid, sweep, 1, r) >> 31) == 0) S[i] = (int8_t)(-S[i]);
+        const uint32_t x = cl_word(k0, k1, sid, sweep, 1, r);
+        if (a.q == 0) {
+            if ((x >> 31) == 0) S[i] = (int8_t)(-S[i]);
+        } else {
+            S[i] = (int8_t)(((uint64_t)x * (uint32_t)a.q) >> 32);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -276,7 +296,7 @@ __global__ void cl_fill(int32_t* p, int64_t n, int32_t v)
 __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
 {
     __shared__ uint32_t s_head, s_tail;
-    __shared__ int s_s0;
+    __shared__ int s_a, s_b;
     const int rep = blockIdx.x;
     const ClGeo& g = a.g;
     const uint32_t N = g.N;
@@ -290,18 +310,27 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
     for (int f = 0; f < count; ++f) {
         const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
         if (threadIdx.x == 0) {
-            const uint32_t x = cl_word(k0, k1, sid, sweep, 2, 0);
-            const uint32_t seed = (uint32_t)(((uint64_t)x * N) >> 32);
-            const int s0 = S[seed];
-            S[seed] = (int8_t)(-s0);
+            const u32x4 r = philox4x32_10(
+                u32x4{0u, (uint32_t)sweep, ctr_z(sweep, 2, TAG_CLUSTER), sid}, k0, k1);
+            const uint32_t seed = (uint32_t)(((uint64_t)r.x * N) >> 32);
+            const int av = S[seed];
+            int bv;
+            if (a.q == 0) {
+                bv = -av;
+            } else {
+                bv = (int)(((uint64_t)r.y * (uint32_t)(a.q - 1)) >> 32);
+                if (bv >= av) ++bv;
+            }
+            S[seed] = (int8_t)bv;
             Mk[seed] = f;
             Q[0] = (int32_t)seed;
-            s_s0 = s0;
+            s_a = av;
+            s_b = bv;
             s_head = 0;
             s_tail = 1;
         }
         __syncthreads();
-        const int s0 = s_s0;
+        const int s0 = s_a, s1 = s_b;
         const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
         while (true) {
             const uint32_t lo = s_head, hi = s_tail;
@@ -320,7 +349,7 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
                     const uint32_t u = w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
                     if ((uint64_t)u >= T) continue;
                     if (atomicExch(Mk + j, f) == f) continue;  // another lane added it
-                    S[j] = (int8_t)(-s0);
+                    S[j] = (int8_t)s1;
                     Q[atomicAdd(&s_tail, 1u)] = (int32_t)j;
                 }
             }
@@ -376,16 +405,34 @@ inline uint64_t cl_threshold(double p)
     return (uint64_t)std::ceil(p * 4294967296.0);
 }
 
+__global__ void cl_record_potts(int R, int q, int64_t n_meas, int64_t meas, const int32_t* e_run,
+                                const int32_t* counts_run, int32_t* e_out, int32_t* counts_out,
+                                const int64_t* acc, const int64_t* wsize, const int64_t* stats,
+                                int have_sw, int64_t* aux)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= R) return;
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    e_out[o] = e_run[rep];
+    for (int k = 0; k < q; ++k) counts_out[o * q + k] = counts_run[(int64_t)rep * q + k];
+    aux[4 * o + 0] = acc[rep];
+    aux[4 * o + 1] = wsize[rep];
+    aux[4 * o + 2] = have_sw ? stats[2 * rep] : 0;
+    aux[4 * o + 3] = have_sw ? stats[2 * rep + 1] : 0;
+}
+
 struct ClLayout {
     int64_t tab, parent, size, queue, mark, stats, wsize, acc, tadd, rowacc, rowe, rowm, total;
 };
 
-inline ClLayout cl_layout(int64_t N, int32_t n_rep)
+// Workspace: the local kernels' tables (tab_bytes), then the cluster arrays;
+// rowm holds row_m_words int32 per replica (M for Ising, q counts for Potts).
+inline ClLayout cl_layout(int64_t N, int32_t n_rep, int64_t tab_bytes, int row_m_words)
 {
     auto up = [](int64_t x) { return (x + 255) / 256 * 256; };
     ClLayout l{};
     int64_t o = 0;
-    l.tab = o;    o += up(cm_ising_workspace_bytes(n_rep));
+    l.tab = o;    o += up(tab_bytes);
     l.parent = o; o += up(4 * N * n_rep);
     l.size = o;   o += up(4 * N * n_rep);
     l.queue = o;  o += up(4 * N * n_rep);
@@ -396,9 +443,89 @@ inline ClLayout cl_layout(int64_t N, int32_t n_rep)
     l.tadd = o;   o += up(8 * (int64_t)n_rep);
     l.rowacc = o; o += up(4 * (int64_t)n_rep);
     l.rowe = o;   o += up(4 * (int64_t)n_rep);
-    l.rowm = o;   o += up(4 * (int64_t)n_rep);
+    l.rowm = o;   o += up(4 * (int64_t)n_rep * row_m_words);
     l.total = o;
     return l;
+}
+
+// Validate the geometry and schedule shared by both entry points.
+inline int cl_check(int32_t ndim, const int32_t* dims, int32_t n_rep, int64_t n_meas,
+                    const int64_t* schedule, int32_t n_rows, int64_t* N_out, bool* any_cluster,
+                    bool* any_sw)
+{
+    CM_REQUIRE(ndim >= 1 && ndim <= 3, "lattices of 1..3 dimensions");
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) {
+        CM_REQUIRE(dims[a] >= 2 && dims[a] % 2 == 0, "extents must be even and >= 2");
+        N *= dims[a];
+    }
+    CM_REQUIRE(N * ndim < ((int64_t)1 << 31), "lattice too large for cluster updates");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    *any_cluster = *any_sw = false;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_WOLFF ||
+                       code == CODE_SWENDSEN_WANG,
+                   "update code %lld has no GPU kernel for this model", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 24), "repetition count out of range");
+        *any_cluster |= code == CODE_WOLFF || code == CODE_SWENDSEN_WANG;
+        *any_sw |= code == CODE_SWENDSEN_WANG;
+    }
+    *N_out = N;
+    return CM_OK;
+}
+
+inline void cl_args(ClArgs& a, char* ws, const ClLayout& lay, void* spins, int32_t ndim,
+                    const int32_t* dims, int64_t N, int32_t n_rep, int q, const uint32_t* keys,
+                    const uint32_t* sids, uint64_t* sweeps)
+{
+    a.spins = (int8_t*)spins;
+    a.parent = (int32_t*)(ws + lay.parent);
+    a.size = (int32_t*)(ws + lay.size);
+    a.queue = (int32_t*)(ws + lay.queue);
+    a.mark = (int32_t*)(ws + lay.mark);
+    a.stats = (int64_t*)(ws + lay.stats);
+    a.wsize = (int64_t*)(ws + lay.wsize);
+    a.t_add = (const uint64_t*)(ws + lay.tadd);
+    a.keys = keys;
+    a.sids = sids;
+    a.sweeps = sweeps;
+    a.g.d = ndim;
+    for (int k = 0; k < 3; ++k) a.g.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
+    a.g.N = (uint32_t)N;
+    a.g.div0 = make_fastdiv(a.g.L[0]);
+    a.g.div1 = make_fastdiv(a.g.L[1]);
+    a.R = n_rep;
+    a.q = q;
+}
+
+// `count` Swendsen-Wang updates or Wolff clusters of every replica, then the
+// device sweep counters advance by `count`.
+inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* sweeps,
+                           hipStream_t st)
+{
+    const int64_t N = a.g.N;
+    const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
+    const unsigned bond_chunks = (unsigned)std::min<int64_t>((a.g.d * N / 4 + 256) / 256, 1024);
+    if (code == CODE_SWENDSEN_WANG) {
+        for (int64_t k = 0; k < count; ++k) {
+            a.sweep_add = (uint64_t)k;
+            hipLaunchKernelGGL(sw_init, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(sw_flip, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+        }
+    } else {
+        a.sweep_add = 0;
+        const int64_t nm = N * a.R;
+        hipLaunchKernelGGL(cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
+                           dim3(256), 0, st, a.mark, nm, -1);
+        hipLaunchKernelGGL(wolff_run, dim3(a.R), dim3(256), 0, st, a, (int)count);
+    }
+    hipLaunchKernelGGL(cl_advance, dim3((a.R + 255) / 256), dim3(256), 0, st, sweeps, a.R,
+                       (uint64_t)count);
 }
 
 }  // namespace cm
@@ -411,7 +538,7 @@ int64_t cm_ising_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int3
 {
     int64_t N = 1;
     for (int a = 0; a < ndim; ++a) N *= dims[a];
-    return cl_layout(N, n_rep).total;
+    return cl_layout(N, n_rep, cm_ising_workspace_bytes(n_rep), 1).total;
 }
 
 int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
@@ -421,29 +548,13 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                          int32_t* m_run, int32_t* e_out, int32_t* m_out, int64_t* aux_out,
                          void* workspace, int64_t workspace_bytes, void* stream)
 {
-    CM_REQUIRE(ndim >= 1 && ndim <= 3, "lattices of 1..3 dimensions");
-    int64_t N = 1;
-    for (int a = 0; a < ndim; ++a) {
-        CM_REQUIRE(dims[a] >= 2 && dims[a] % 2 == 0, "extents must be even and >= 2");
-        N *= dims[a];
-    }
-    CM_REQUIRE(N * ndim < ((int64_t)1 << 31), "lattice too large for cluster updates");
-    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
-    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
-    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    int64_t N;
+    bool any_cluster, any_sw;
+    if (int rc = cl_check(ndim, dims, n_rep, n_meas, schedule, n_rows, &N, &any_cluster, &any_sw))
+        return rc;
     CM_REQUIRE(sweeps_host != nullptr, "the host sweep counters are required");
-    const ClLayout lay = cl_layout(N, n_rep);
+    const ClLayout lay = cl_layout(N, n_rep, cm_ising_workspace_bytes(n_rep), 1);
     CM_REQUIRE(workspace_bytes >= lay.total, "workspace too small");
-    bool any_cluster = false, any_sw = false;
-    for (int r = 0; r < n_rows; ++r) {
-        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
-        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_WOLFF ||
-                       code == CODE_SWENDSEN_WANG,
-                   "update code %lld has no Ising GPU kernel", (long long)code);
-        CM_REQUIRE(count >= 1 && count <= (1 << 24), "repetition count out of range");
-        any_cluster |= code == CODE_WOLFF || code == CODE_SWENDSEN_WANG;
-        any_sw |= code == CODE_SWENDSEN_WANG;
-    }
     CM_REQUIRE(!any_cluster || h == 0.0, "cluster updates require zero field h");
     if (n_meas == 0) return CM_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -455,30 +566,12 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
     }
     CM_HIP_CHECK(upload_async(ws + lay.tadd, tadd.data(), tadd.size() * 8, st));
     ClArgs a{};
-    a.spins = (int8_t*)spins;
-    a.parent = (int32_t*)(ws + lay.parent);
-    a.size = (int32_t*)(ws + lay.size);
-    a.queue = (int32_t*)(ws + lay.queue);
-    a.mark = (int32_t*)(ws + lay.mark);
-    a.stats = (int64_t*)(ws + lay.stats);
-    a.wsize = (int64_t*)(ws + lay.wsize);
-    a.t_add = (const uint64_t*)(ws + lay.tadd);
-    a.keys = keys;
-    a.sids = stream_ids;
-    a.sweeps = sweeps;
-    a.g.d = ndim;
-    for (int k = 0; k < 3; ++k) a.g.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
-    a.g.N = (uint32_t)N;
-    a.g.div0 = make_fastdiv(a.g.L[0]);
-    a.g.div1 = make_fastdiv(a.g.L[1]);
-    a.R = n_rep;
+    cl_args(a, ws, lay, spins, ndim, dims, N, n_rep, 0, keys, stream_ids, sweeps);
     int64_t* acc = (int64_t*)(ws + lay.acc);
     int32_t* row_acc = (int32_t*)(ws + lay.rowacc);
     int32_t* row_e = (int32_t*)(ws + lay.rowe);
     int32_t* row_m = (int32_t*)(ws + lay.rowm);
     const int rb = (n_rep + 255) / 256;
-    const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
-    const unsigned bond_chunks = (unsigned)std::min<int64_t>((ndim * N / 4 + 256) / 256, 1024);
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
@@ -493,33 +586,79 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                                       nullptr, stream);
                 if (rc != CM_OK) return rc;
                 hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
-            } else if (code == CODE_SWENDSEN_WANG) {
-                for (int64_t k = 0; k < count; ++k) {
-                    a.sweep_add = (uint64_t)k;
-                    hipLaunchKernelGGL(sw_init, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
-                    hipLaunchKernelGGL(sw_union, dim3(bond_chunks, n_rep), dim3(256), 0, st, a);
-                    hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
-                    hipLaunchKernelGGL(sw_flip, dim3(site_chunks, n_rep), dim3(256), 0, st, a);
-                }
-                hipLaunchKernelGGL(cl_advance, dim3(rb), dim3(256), 0, st, sweeps, n_rep, (uint64_t)count);
             } else {
-                a.sweep_add = 0;
-                const int64_t nm = N * n_rep;
-                hipLaunchKernelGGL(cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
-                                   dim3(256), 0, st, a.mark, nm, -1);
-                hipLaunchKernelGGL(wolff_run, dim3(n_rep), dim3(256), 0, st, a, (int)count);
-                hipLaunchKernelGGL(cl_advance, dim3(rb), dim3(256), 0, st, sweeps, n_rep, (uint64_t)count);
-            }
-            if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
-                for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
+                cl_cluster_row(a, code, count, sweeps, st);
                 int rc = cm_ising_measure(spins, ndim, dims, n_rep, e_run, m_run, stream);
                 if (rc != CM_OK) return rc;
-            } else {
-                for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
             }
+            for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
         }
         hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas, e_run,
                            m_run, e_out, m_out, acc, a.wsize, a.stats, any_sw ? 1 : 0, aux_out);
+    }
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int64_t cm_potts_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep,
+                                         int32_t q)
+{
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= dims[a];
+    return cl_layout(N, n_rep, cm_potts_workspace_bytes(n_rep), q).total;
+}
+
+int cm_potts_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                         const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
+                         uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
+                         int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* counts_out,
+                         int64_t* aux_out, void* workspace, int64_t workspace_bytes, void* stream)
+{
+    int64_t N;
+    bool any_cluster, any_sw;
+    if (int rc = cl_check(ndim, dims, n_rep, n_meas, schedule, n_rows, &N, &any_cluster, &any_sw))
+        return rc;
+    CM_REQUIRE(q >= 2 && q <= 127, "q must be in [2, 127], got %d", q);
+    const ClLayout lay = cl_layout(N, n_rep, cm_potts_workspace_bytes(n_rep), q);
+    CM_REQUIRE(workspace_bytes >= lay.total, "workspace too small");
+    if (n_meas == 0) return CM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = (char*)workspace;
+    std::vector<uint64_t> tadd(n_rep);
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        tadd[r] = cl_threshold(1.0 - std::exp(-betas[r]));
+    }
+    CM_HIP_CHECK(upload_async(ws + lay.tadd, tadd.data(), tadd.size() * 8, st));
+    ClArgs a{};
+    cl_args(a, ws, lay, spins, ndim, dims, N, n_rep, q, keys, stream_ids, sweeps);
+    int64_t* acc = (int64_t*)(ws + lay.acc);
+    int32_t* row_acc = (int32_t*)(ws + lay.rowacc);
+    int32_t* row_e = (int32_t*)(ws + lay.rowe);
+    int32_t* row_c = (int32_t*)(ws + lay.rowm);
+    const int rb = (n_rep + 255) / 256;
+    hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, a.wsize, n_rep);
+        for (int row = 0; row < n_rows; ++row) {
+            const int64_t code = schedule[2 * row], count = schedule[2 * row + 1];
+            if (code == CODE_METROPOLIS || code == CODE_HEATBATH) {
+                const int64_t one[2] = {code, count};
+                int rc = cm_potts_run(spins, ndim, dims, n_rep, q, betas, keys, stream_ids, sweeps,
+                                      one, 1, 1, e_run, counts_run, row_e, row_acc, row_c,
+                                      ws + lay.tab, lay.parent - lay.tab, 0, 0, nullptr, stream);
+                if (rc != CM_OK) return rc;
+                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
+            } else {
+                cl_cluster_row(a, code, count, sweeps, st);
+                int rc = cm_potts_measure(spins, ndim, dims, n_rep, q, e_run, counts_run, stream);
+                if (rc != CM_OK) return rc;
+            }
+        }
+        hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, st, n_rep, q, n_meas, meas,
+                           e_run, counts_run, e_out, counts_out, acc, a.wsize, a.stats,
+                           any_sw ? 1 : 0, aux_out);
     }
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;

@@ -34,6 +34,6 @@ N_AUX = 4
 # codes every checkerboard GPU kernel implements, per model kind
 GPU_UPDATES = {
     "ising": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
-    "potts": ("metropolis", "heatbath"),
+    "potts": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
     "on": ("metropolis", "heatbath", "overrelax"),
 }

@@ -14,7 +14,7 @@ import torch
 from .._lib import check, lib
 from ..device import device, host_ptr, ptr, stream
 from .chains import Chains
-from .common import HEATBATH, METROPOLIS
+from .common import HEATBATH, METROPOLIS, SWENDSEN_WANG, WOLFF
 
 START_CODES = {"random": 0, "ordered": 1, "up": 1}
 
@@ -74,3 +74,41 @@ def run(spins: torch.Tensor, shape, q: int, betas, chains: Chains, schedule, n_m
     if n_meas_rep is None:
         chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return e_out, acc, counts
+
+
+class ClusterWorkspace:
+    """Device scratch of :func:`run_cluster`."""
+
+    def __init__(self, shape, R: int, q: int, dev: torch.device | None = None):
+        d = _dims(shape)
+        self.nbytes = int(lib().cm_potts_cluster_workspace_bytes(len(d), host_ptr(d, ctypes.c_int32),
+                                                                 R, int(q)))
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=dev or device())
+
+
+def run_cluster(spins: torch.Tensor, shape, q: int, betas, chains: Chains, schedule, n_meas: int,
+                e_run: torch.Tensor, counts_run: torch.Tensor, *,
+                ws: ClusterWorkspace | None = None):
+    """Apply a schedule that may contain Wolff (3) and Swendsen-Wang (4) rows
+    (critmc/kernels/potts.py:106-167).  Returns device ``(E [R, n_meas],
+    counts [R, n_meas, q])`` int32 and ``aux`` int64 ``[R, n_meas, 4]``."""
+    R = chains.R
+    betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, dtype=np.float64), (R,)))
+    sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+    for code in sched[:, 0]:
+        if code not in (METROPOLIS, HEATBATH, WOLFF, SWENDSEN_WANG):
+            raise ValueError(f"update code {int(code)} has no Potts GPU kernel")
+    d = _dims(shape)
+    dev = spins.device
+    e_out = torch.empty((R, n_meas), dtype=torch.int32, device=dev)
+    counts = torch.empty((R, n_meas, q), dtype=torch.int32, device=dev)
+    aux = torch.empty((R, n_meas, 4), dtype=torch.int64, device=dev)
+    ws = ws or ClusterWorkspace(shape, R, q, dev)
+    rc = lib().cm_potts_cluster_run(
+        ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, int(q), host_ptr(betas, ctypes.c_double),
+        ptr(chains.keys_d), ptr(chains.sids_d), ptr(chains.sweeps_d), host_ptr(sched, ctypes.c_int64),
+        sched.shape[0], int(n_meas), ptr(e_run), ptr(counts_run), ptr(e_out), ptr(counts), ptr(aux),
+        ptr(ws.buf), ws.nbytes, stream())
+    check(rc, "cm_potts_cluster_run")
+    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    return e_out, counts, aux

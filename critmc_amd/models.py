@@ -228,13 +228,21 @@ class Potts(Model):
         return k_potts.Workspace(1)
 
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
-        e_out, acc, counts = k_potts.run(ds.spins, self.lattice.shape, self.q, [float(beta)],
-                                         chains, schedule, n_meas, ds.extra["e"],
-                                         ds.extra["counts"], ws=ws)
+        if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
+            if "cluster_ws" not in ds.extra:
+                ds.extra["cluster_ws"] = k_potts.ClusterWorkspace(self.lattice.shape, 1, self.q)
+            e_out, counts, aux_d = k_potts.run_cluster(
+                ds.spins, self.lattice.shape, self.q, [float(beta)], chains, schedule, n_meas,
+                ds.extra["e"], ds.extra["counts"], ws=ds.extra["cluster_ws"])
+            aux = aux_d[0].cpu().numpy()
+        else:
+            e_out, acc, counts = k_potts.run(ds.spins, self.lattice.shape, self.q, [float(beta)],
+                                             chains, schedule, n_meas, ds.extra["e"],
+                                             ds.extra["counts"], ws=ws)
+            aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
+            aux[:, 0] = acc[0].cpu().numpy()
         e = e_out[0].cpu().numpy().astype(np.int64)
         c = counts[0].cpu().numpy().astype(np.int64)
-        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
-        aux[:, 0] = acc[0].cpu().numpy()
         running["energy"] = int(e[-1])
         running["counts"] = c[-1].copy()
         return {"E": e, "counts": c, "aux": aux}

@@ -159,6 +159,24 @@ int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_r
                  int64_t workspace_bytes, int32_t block, int32_t grid,
                  const int64_t *n_meas_rep_dev, void *stream);
 
+/* Bytes of device scratch cm_potts_cluster_run needs. */
+int64_t cm_potts_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep,
+                                         int32_t q);
+
+/* Any Potts schedule, including Wolff (3) and Swendsen-Wang (4) rows:
+ * critmc/kernels/potts.py:170-235 with wolff_update / sw_update (:106-167).
+ * Records E and counts like cm_potts_run (int32) and the reference's aux
+ * columns aux_out int64 [n_rep][n_meas][4] (accepted, Wolff sizes, sum |C|^2
+ * and cluster count of the last Swendsen-Wang update).  Every repetition of
+ * any row advances the chain's sweep counter by one. */
+int cm_potts_cluster_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
+                         int32_t q, const double *betas, const uint32_t *keys_dev,
+                         const uint32_t *stream_ids_dev, uint64_t *sweeps_dev,
+                         const int64_t *schedule, int32_t n_rows, int64_t n_meas,
+                         int32_t *e_run_dev, int32_t *counts_run_dev, int32_t *e_out_dev,
+                         int32_t *counts_out_dev, int64_t *aux_out_dev, void *workspace_dev,
+                         int64_t workspace_bytes, void *stream);
+
 /* ---- Edwards-Anderson parallel tempering (critmc/kernels/glass.py) -------- */
 
 /* A batch holds n_samples disorder samples of the same dimension (2 or 3) and

@@ -215,6 +215,9 @@ _EXTRA_DECLS.update({
     "orc_potts_run": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32,
                        ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
                        ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p], ctypes.c_int),
+    "orc_potts_run_aux": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32,
+                           ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
+                           ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p], ctypes.c_int),
     "orc_potts_init": ([_i8p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                         ctypes.c_uint32], None),
 })
@@ -258,14 +261,15 @@ class PottsReplay:
         e_out = np.zeros(n_meas, np.int64)
         c_out = np.zeros((n_meas, self.q), np.int64)
         acc = np.zeros(n_meas, np.int64)
-        rc = lib().orc_potts_run(
+        aux = np.zeros((n_meas, 4), np.int64)
+        rc = lib().orc_potts_run_aux(
             _ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.q, self.beta,
             self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p), _ptr(sched, _i64p),
             sched.shape[0], int(n_meas), _ptr(self.energy, _i64p), _ptr(self.counts, _i64p),
-            _ptr(e_out, _i64p), _ptr(c_out, _i64p), _ptr(acc, _i64p))
+            _ptr(e_out, _i64p), _ptr(c_out, _i64p), _ptr(acc, _i64p), _ptr(aux, _i64p))
         if rc:
             raise ValueError(f"orc_potts_run failed ({rc})")
-        return {"E": e_out, "counts": c_out, "accepted": acc}
+        return {"E": e_out, "counts": c_out, "accepted": acc, "aux": aux}
 
 
 _EXTRA_DECLS.update({

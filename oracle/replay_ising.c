@@ -213,85 +213,13 @@ int orc_ising_energy(const int8_t *s, int ndim, const int32_t *dims, int64_t *e_
  * the chain's local-sweep counter, advanced by one per Metropolis or
  * heat-bath sweep.  Returns 0, or <0 for an unsupported code or geometry.
  */
-/*
- * Cluster updates (critmc/kernels/ising.py:110-170), one sweep-counter value
- * per update, uniforms at tag ORC_TAG_CLUSTER:
- *   bond b = a N + i (i and its +e_a neighbour) is active iff its spins agree
- *   and word b&3 of block b>>2 (colour field 0) is below ceil(p 2^32),
- *   p = 1 - exp(-2 beta);
- *   Swendsen-Wang: clusters are the components of the active bonds; the
- *   cluster whose smallest site is r flips iff word r&3 of block r>>2 (colour
- *   field 1) has its top bit clear (u < 1/2); returns sum |C|^2 and the count;
- *   Wolff: seed = floor(x N / 2^32) with x = word 0 of block 0 (colour field
- *   2); the cluster is the seed's component among sites of the seed's spin
- *   (a breadth-first search, flipping members as they join); returns its size.
- */
-static int64_t orc_ising_wolff(const orc_geom *g, int8_t *s, uint64_t t_add, uint32_t k0,
-                               uint32_t k1, uint32_t stream, uint64_t sweep, int32_t *queue)
-{
-    const int z = 2 * g->d;
-    uint32_t w[4];
-    orc_draw_block(k0, k1, stream, sweep, 2, ORC_TAG_CLUSTER, 0, w);
-    const int64_t seed = (int64_t)(((uint64_t)w[0] * (uint64_t)g->N) >> 32);
-    const int s0 = s[seed];
-    s[seed] = (int8_t)(-s0);
-    queue[0] = (int32_t)seed;
-    int64_t head = 0, tail = 1;
-    while (head < tail) {
-        const int64_t k = queue[head++];
-        for (int c = 0; c < z; ++c) {
-            const int64_t j = g->nbr[k * z + c];
-            if (s[j] != s0) continue;
-            const int64_t b = (int64_t)(c >> 1) * g->N + ((c & 1) ? j : k);
-            orc_draw_block(k0, k1, stream, sweep, 0, ORC_TAG_CLUSTER, (uint32_t)(b >> 2), w);
-            if ((uint64_t)w[b & 3] < t_add) {
-                s[j] = (int8_t)(-s0);
-                queue[tail++] = (int32_t)j;
-            }
-        }
-    }
-    return tail;
-}
-
-static int32_t orc_find(int32_t *parent, int32_t i)
-{
-    while (parent[i] != i) i = parent[i] = parent[parent[i]];
-    return i;
-}
-
-static void orc_ising_sw(const orc_geom *g, int8_t *s, uint64_t t_add, uint32_t k0, uint32_t k1,
-                         uint32_t stream, uint64_t sweep, int32_t *parent, int32_t *size,
-                         int64_t *c2, int64_t *ncl)
-{
-    const int z = 2 * g->d;
-    for (int64_t i = 0; i < g->N; ++i) parent[i] = (int32_t)i;
-    uint32_t w[4];
-    for (int a = 0; a < g->d; ++a)
-        for (int64_t i = 0; i < g->N; ++i) {
-            const int64_t b = (int64_t)a * g->N + i;
-            if ((b & 3) == 0 || i == 0)
-                orc_draw_block(k0, k1, stream, sweep, 0, ORC_TAG_CLUSTER, (uint32_t)(b >> 2), w);
-            const int64_t j = g->nbr[i * z + 2 * a];
-            if (s[i] != s[j] || (uint64_t)w[b & 3] >= t_add) continue;
-            int32_t ri = orc_find(parent, (int32_t)i), rj = orc_find(parent, (int32_t)j);
-            if (ri == rj) continue;
-            if (ri < rj) parent[rj] = ri; /* the smaller site is the root */
-            else parent[ri] = rj;
-        }
-    for (int64_t i = 0; i < g->N; ++i) size[i] = 0;
-    for (int64_t i = 0; i < g->N; ++i) size[orc_find(parent, (int32_t)i)] += 1;
-    *c2 = 0;
-    *ncl = 0;
-    for (int64_t i = 0; i < g->N; ++i) {
-        const int32_t r = orc_find(parent, (int32_t)i);
-        if (r == i) {
-            *c2 += (int64_t)size[i] * size[i];
-            *ncl += 1;
-        }
-        orc_draw_block(k0, k1, stream, sweep, 1, ORC_TAG_CLUSTER, (uint32_t)(r >> 2), w);
-        if ((w[r & 3] >> 31) == 0) s[i] = (int8_t)(-s[i]);
-    }
-}
+/* Wolff / Swendsen-Wang (replay_cluster.c, q = 0 selects the Ising rules). */
+int64_t orc_cluster_wolff(int8_t *s, int ndim, const int32_t *dims, int q, uint64_t t_add,
+                          uint32_t k0, uint32_t k1, uint32_t stream, uint64_t sweep,
+                          int32_t *queue);
+void orc_cluster_sw(int8_t *s, int ndim, const int32_t *dims, int q, uint64_t t_add, uint32_t k0,
+                    uint32_t k1, uint32_t stream, uint64_t sweep, int32_t *parent, int32_t *size,
+                    int64_t *c2, int64_t *ncl);
 
 int orc_ising_run_aux(int8_t *s, int ndim, const int32_t *dims, double beta, double h,
                       uint32_t k0, uint32_t k1, uint32_t stream, uint64_t *sweep,
@@ -326,9 +254,10 @@ int orc_ising_run_aux(int8_t *s, int ndim, const int32_t *dims, double beta, dou
             int64_t code = sched[2 * row], count = sched[2 * row + 1];
             for (int64_t k = 0; k < count; ++k) {
                 if (code == ORC_WOLFF) {
-                    wsize += orc_ising_wolff(&g, s, t_add, k0, k1, stream, *sweep, work);
+                    wsize += orc_cluster_wolff(s, ndim, dims, 0, t_add, k0, k1, stream, *sweep, work);
                 } else if (code == ORC_SW) {
-                    orc_ising_sw(&g, s, t_add, k0, k1, stream, *sweep, work, work + g.N, &c2, &ncl);
+                    orc_cluster_sw(s, ndim, dims, 0, t_add, k0, k1, stream, *sweep, work,
+                                   work + g.N, &c2, &ncl);
                 } else {
                     orc_half_sweep(&g, s, (int)code, 0, met, hb, k0, k1, stream, *sweep, e_b, m, &acc);
                     orc_half_sweep(&g, s, (int)code, 1, met, hb, k0, k1, stream, *sweep, e_b, m, &acc);

@@ -37,10 +37,20 @@ int orc_potts_energy(const int8_t *s, int ndim, const int32_t *dims, int q, int6
     return 0;
 }
 
-int orc_potts_run(int8_t *s, int ndim, const int32_t *dims, int q, double beta, uint32_t k0,
-                  uint32_t k1, uint32_t stream, uint64_t *sweep, const int64_t *sched, int nrows,
-                  int64_t n_meas, int64_t *energy, int64_t *counts, int64_t *e_out,
-                  int64_t *counts_out, int64_t *acc_out)
+int64_t orc_cluster_wolff(int8_t *s, int ndim, const int32_t *dims, int q, uint64_t t_add,
+                          uint32_t k0, uint32_t k1, uint32_t stream, uint64_t sweep,
+                          int32_t *queue);
+void orc_cluster_sw(int8_t *s, int ndim, const int32_t *dims, int q, uint64_t t_add, uint32_t k0,
+                    uint32_t k1, uint32_t stream, uint64_t sweep, int32_t *parent, int32_t *size,
+                    int64_t *c2, int64_t *ncl);
+
+/* Schedule interpreter with Wolff (3) and Swendsen-Wang (4) rows
+ * (critmc/kernels/potts.py:170-235); aux_out [n_meas][4] = (accepted, Wolff
+ * sizes, sum |C|^2 and cluster count of the measurement's last SW update). */
+int orc_potts_run_aux(int8_t *s, int ndim, const int32_t *dims, int q, double beta, uint32_t k0,
+                      uint32_t k1, uint32_t stream, uint64_t *sweep, const int64_t *sched,
+                      int nrows, int64_t n_meas, int64_t *energy, int64_t *counts, int64_t *e_out,
+                      int64_t *counts_out, int64_t *acc_out, int64_t *aux_out)
 {
     orc_cb g;
     int rc = orc_cb_init(&g, ndim, dims);
@@ -53,11 +63,30 @@ int orc_potts_run(int8_t *s, int ndim, const int32_t *dims, int q, double beta, 
         w[d] = exp(beta * d);
     }
     int64_t nbi[6];
+    const uint64_t t_add = orc_thr_p(1.0 - exp(-beta));
+    int32_t *work = NULL;
+    for (int row = 0; row < nrows; ++row) {
+        int code = (int)sched[2 * row];
+        if (code != 0 && code != 2 && code != 3 && code != 4) return -3;
+        if ((code == 3 || code == 4) && !aux_out) return -5;
+        if ((code == 3 || code == 4) && !work) work = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)g.N);
+    }
     for (int64_t meas = 0; meas < n_meas; ++meas) {
-        int64_t acc = 0;
+        int64_t acc = 0, wsize = 0, c2 = 0, ncl = 0;
         for (int row = 0; row < nrows; ++row) {
             int code = (int)sched[2 * row];
-            if (code != 0 && code != 2) return -3;
+            if (code == 3 || code == 4) {
+                for (int64_t rp = 0; rp < sched[2 * row + 1]; ++rp) {
+                    if (code == 3)
+                        wsize += orc_cluster_wolff(s, ndim, dims, q, t_add, k0, k1, stream, *sweep, work);
+                    else
+                        orc_cluster_sw(s, ndim, dims, q, t_add, k0, k1, stream, *sweep, work,
+                                       work + g.N, &c2, &ncl);
+                    *sweep += 1;
+                }
+                orc_potts_energy(s, ndim, dims, q, energy, counts);
+                continue;
+            }
             for (int64_t rp = 0; rp < sched[2 * row + 1]; ++rp) {
                 for (int c = 0; c < 2; ++c) {
                     uint32_t blk[4];
@@ -118,8 +147,24 @@ int orc_potts_run(int8_t *s, int ndim, const int32_t *dims, int q, double beta, 
         e_out[meas] = *energy;
         for (int kk = 0; kk < q; ++kk) counts_out[meas * q + kk] = counts[kk];
         acc_out[meas] = acc;
+        if (aux_out) {
+            aux_out[4 * meas + 0] = acc;
+            aux_out[4 * meas + 1] = wsize;
+            aux_out[4 * meas + 2] = c2;
+            aux_out[4 * meas + 3] = ncl;
+        }
     }
+    free(work);
     return 0;
+}
+
+int orc_potts_run(int8_t *s, int ndim, const int32_t *dims, int q, double beta, uint32_t k0,
+                  uint32_t k1, uint32_t stream, uint64_t *sweep, const int64_t *sched, int nrows,
+                  int64_t n_meas, int64_t *energy, int64_t *counts, int64_t *e_out,
+                  int64_t *counts_out, int64_t *acc_out)
+{
+    return orc_potts_run_aux(s, ndim, dims, q, beta, k0, k1, stream, sweep, sched, nrows, n_meas,
+                             energy, counts, e_out, counts_out, acc_out, NULL);
 }
 
 void orc_potts_init(int8_t *s, int64_t n, int q, uint32_t k0, uint32_t k1, uint32_t stream)

@@ -1,4 +1,4 @@
-"""GPU Wolff and Swendsen-Wang updates of the Ising model against the CPU replay.
+"""GPU Wolff and Swendsen-Wang updates (Ising and Potts) against the CPU replay.
 
 Bit-exact: spins, E_b, M and the aux columns (accepted, Wolff sizes,
 sum |C|^2, cluster count) of every replica, for cluster-only and mixed
@@ -14,6 +14,7 @@ import torch
 
 import oracle
 from critmc_amd.kernels import ising as K
+from critmc_amd.kernels import potts as P
 from critmc_amd.kernels.chains import Chains
 from oracle import exact
 from oracle.stats import batch_mean_error
@@ -88,3 +89,43 @@ def test_wolff_count_is_tuned_towards_n_over_cluster_size():
     assert abs(n_wolff - 1024 / mean_c) <= max(2.0, 0.5 * 1024 / mean_c)
     ts = sim.run(200)
     assert ts.meta["sweeps_per_meas"] == pytest.approx(n_wolff * ts.meta["mean_cluster"] / 1024)
+
+
+@pytest.mark.parametrize("shape,q,R", [((8, 8), 3, 3), ((6, 4, 2), 10, 2), ((128, 128), 10, 2),
+                                       ((12,), 2, 2)])
+@pytest.mark.parametrize("sched", [[[4, 1]], [[3, 3]], [[0, 1], [3, 2], [4, 1]], [[2, 1], [4, 1]]])
+def test_potts_cluster_schedule_is_bit_exact_against_replay(shape, q, R, sched):
+    dev = torch.device("cuda", 0)
+    chains, keys, sids, sw0 = _chains(R, q + sum(shape), dev)
+    n = int(np.prod(shape))
+    s = torch.empty((R, n), dtype=torch.int8, device=dev)
+    P.init_states(s, shape, q, chains, "random")
+    s0 = s.cpu().numpy()
+    e, c = P.measure(s, shape, q, R)
+    betas = np.linspace(0.9, 1.5, R)
+    n_meas = 3 if n > 10_000 else 6
+    e_out, counts, aux = P.run_cluster(s, shape, q, betas, chains, np.array(sched), n_meas, e, c)
+    got = s.cpu().numpy()
+    e_out, counts, aux = (t.cpu().numpy() for t in (e_out, counts, aux))
+    per = sum(k for _, k in sched)
+    assert np.array_equal(chains.sweeps_d.cpu().numpy().view(np.uint64), sw0 + np.uint64(per * n_meas))
+    for r in range(R):
+        rep = oracle.PottsReplay(s0[r], shape, q, betas[r], keys[r], int(sids[r]), int(sw0[r]))
+        ref = rep.run(sched, n_meas)
+        assert np.array_equal(got[r], rep.s), f"states differ (replica {r})"
+        assert np.array_equal(e_out[r], ref["E"]) and np.array_equal(counts[r], ref["counts"])
+        assert np.array_equal(aux[r], ref["aux"]), (aux[r], ref["aux"])
+
+
+@pytest.mark.parametrize("updates", ["sw", "wolff", "heatbath+sw"])
+def test_potts_cluster_simulation_matches_enumeration(updates):
+    from critmc_amd import Potts, Simulation, hypercubic
+
+    lat = hypercubic((2, 4))
+    q, beta = 3, 1.0
+    sim = Simulation(Potts(lat, q), beta, updates, seed=23)
+    sim.thermalize(2000)
+    ts = sim.run(100_000)
+    mean, err = batch_mean_error(ts["E"].astype(float))
+    z = (mean - exact.potts_average(lat.bonds, 8, q, beta, lambda e, cnt: e)) / err
+    assert abs(z) < 4.5, (updates, z)

@@ -61,6 +61,10 @@ def _pull(oracle_lib, shape, q, beta, sched, n_meas, seed, beta_true=None):
     ((2, 4), 3, 0.8, [[0, 1]]),
     ((2, 4), 3, 0.8, [[2, 1]]),
     ((2, 6), 3, 1.1, [[0, 1], [2, 1]]),
+    ((2, 4), 3, 1.0, [[4, 1]]),
+    ((2, 4), 3, 1.0, [[3, 2]]),
+    ((2, 6), 3, 1.2, [[2, 1], [4, 1], [3, 1]]),
+    ((2, 2, 2), 4, 0.9, [[4, 1]]),
     ((2, 2), 10, 1.4, [[0, 1]]),
     ((2, 2), 10, 1.4, [[2, 1]]),
     ((6,), 4, 1.0, [[0, 1]]),
@@ -73,3 +77,23 @@ def test_potts_replay_matches_enumeration(oracle_lib, shape, q, beta, sched):
 def test_potts_planted_wrong_beta_is_caught(oracle_lib):
     zs = _pull(oracle_lib, (2, 4), 3, 0.8 * 0.85, [[0, 1]], 150_000, seed=5, beta_true=0.8)
     assert max(abs(z) for z in zs) > 4.5, zs
+
+
+@pytest.mark.parametrize("sched", [[[4, 1]], [[3, 3]]])
+def test_potts_planted_wrong_beta_is_caught_by_cluster_updates(oracle_lib, sched):
+    zs = _pull(oracle_lib, (2, 4), 3, 1.0 * 0.85, sched, 150_000, seed=6, beta_true=1.0)
+    assert max(abs(z) for z in zs) > 4.5, zs
+
+
+@pytest.mark.parametrize("shape,q", [((8, 8), 3), ((6, 4, 2), 10), ((12,), 2)])
+def test_potts_cluster_bookkeeping(oracle_lib, shape, q):
+    n = int(np.prod(shape))
+    r = oracle_lib.PottsReplay(oracle_lib.potts_init(n, q, (8, 1), 0), shape, q, 1.1, (3, 9), 5)
+    out = r.run([[0, 1], [3, 2], [4, 1]], 30)
+    aux = out["aux"]
+    assert np.array_equal(aux[:, 0], out["accepted"])
+    assert np.all(aux[:, 1] >= 2) and np.all(aux[:, 1] <= 2 * n)
+    assert np.all(aux[:, 3] >= 1) and np.all(aux[:, 2] >= n) and np.all(aux[:, 2] <= n * n)
+    e, c = oracle_lib.potts_energy(r.s, shape, q)
+    assert e == out["E"][-1] and np.array_equal(c, out["counts"][-1])
+    assert int(r.sweep[0]) == 30 * 4
