@@ -41,6 +41,7 @@
 
 #include "cb_geom.hpp"
 #include "common.hpp"
+#include "det_exp.hpp"
 #include "ising_core.hpp"
 #include "philox.hpp"
 
@@ -50,32 +51,6 @@ constexpr int EA_NOBS = 7;
 constexpr int EA_TAB = 16;
 constexpr int64_t EA_LDS_MAX = 32768;
 constexpr int EA_LMAX = 256;
-
-__host__ __device__ inline double det_exp(double x)
-{
-    if (x < -745.0) return 0.0;
-    const double ln2_hi = 6.93147180369123816490e-01;
-    const double ln2_lo = 1.90821492927058770002e-10;
-    const double inv_ln2 = 1.44269504088896338700e+00;
-    const double n = rint(x * inv_ln2);
-    double r = fma(-n, ln2_hi, x);
-    r = fma(-n, ln2_lo, r);
-    double p = 1.0 / 6227020800.0;
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
-    return ldexp(p, (int)n);
-}
 
 struct EaArgs {
     int8_t* spins;

@@ -28,6 +28,22 @@
 //              2 pi U in the reference's (h, e1, e2) frame; block k words 0, 1;
 //              |h| = 0: z = 2U - 1 and azimuth from words 2, 3.
 //
+// Cluster updates (:264-345), the embedded Ising model along a random unit
+// vector r, uniforms at tag TAG_CLUSTER (one sweep-counter value per update):
+//   r: block 0 of colour field 3 -- XY angle 2 pi U (word 0); n = 3:
+//      z = 2U - 1 (word 0), azimuth 2 pi U (word 1);
+//   p_i = S_i . r in FP64; bond b = a N + i (i and its +e_a neighbour j) is
+//   active iff x = p_i p_j > 0 and u < 1 - det_exp(-2 beta x), u = word b&3 of
+//   block b>>2 (colour field 0); det_exp is the fixed-operation e^x, so the
+//   host replay takes the same decisions;
+//   Swendsen-Wang: the cluster whose smallest site is rt reflects,
+//   S -> S - 2 p r, iff word rt&3 of block rt>>2 (colour field 1) has its top
+//   bit clear; Wolff: seed floor(x N / 2^32) (word 0 of block 0, colour field
+//   2), a neighbour j of member k joins iff its bond is active with p_k the
+//   member's projection before its reflection (the reference's order of
+//   effects; every bond has its own uniform, so the cluster does not depend on
+//   the order of exploration).  Reflections are FP64, stored FP32.
+//
 // Uniforms (Metropolis, tag TAG_ON): colour site k of a replica uses Philox
 // block k (n = 2: words 0,1 -> Box-Muller pair, word 2 -> acceptance) or blocks
 // 2k, 2k+1 (n = 3: words 0..3 -> two Box-Muller pairs, third Gaussian from the
@@ -38,7 +54,9 @@
 #include <vector>
 
 #include "cb_geom.hpp"
+#include "cluster_core.hpp"
 #include "common.hpp"
+#include "det_exp.hpp"
 #include "ising_core.hpp"
 #include "philox.hpp"
 
@@ -390,6 +408,310 @@ __global__ void on_advance_sweeps(uint64_t* __restrict__ sweeps, int R, uint64_t
     }
 }
 
+struct OnClArgs {
+    float* spins;
+    const float* betas;
+    const uint32_t* keys;
+    const uint32_t* sids;
+    const uint64_t* sweeps;
+    uint64_t sweep_add;
+    int32_t* parent;  // [R][N]
+    int32_t* size;    // [R][N]
+    int32_t* queue;   // [R][N]
+    int32_t* mark;    // [R][N]
+    double* proj;     // [R][N]
+    int64_t* stats;   // [R][2]: sum |C|^2, clusters of the last Swendsen-Wang update
+    int64_t* wsize;   // [R]: Wolff sizes of the current measurement
+    ClGeo g;
+    int R;
+};
+
+template <int NC>
+__device__ __forceinline__ void on_axis(uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep,
+                                        double (&r)[NC])
+{
+    const u32x4 b = philox4x32_10(u32x4{0u, (uint32_t)sweep, ctr_z(sweep, 3, TAG_CLUSTER), sid}, k0, k1);
+    if constexpr (NC == 2) {
+        const double phi = 2.0 * M_PI * ((double)b.x * 0x1p-32);
+        r[0] = cos(phi);
+        r[1] = sin(phi);
+    } else {
+        const double z = 2.0 * ((double)b.x * 0x1p-32) - 1.0;
+        const double az = 2.0 * M_PI * ((double)b.y * 0x1p-32);
+        const double w = sqrt(fmax(0.0, 1.0 - z * z));
+        r[0] = w * cos(az);
+        r[1] = w * sin(az);
+        r[2] = z;
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ double on_proj(const float* S, uint32_t i, const double (&r)[NC])
+{
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) p += (double)S[(int64_t)NC * i + c] * r[c];
+    return p;
+}
+
+template <int NC>
+__device__ __forceinline__ void on_reflect(float* S, uint32_t i, double p, const double (&r)[NC])
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        float* q = S + (int64_t)NC * i + c;
+        *q = (float)((double)*q - 2.0 * p * r[c]);
+    }
+}
+
+__device__ __forceinline__ bool on_bond_active(double x, double beta, uint32_t w)
+{
+    return x > 0.0 && (double)w * 0x1p-32 < 1.0 - det_exp(-2.0 * beta * x);
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void on_sw_prep(OnClArgs a)
+{
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    const float* S = a.spins + (int64_t)rep * N * NC;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    int32_t* Z = a.size + (int64_t)rep * N;
+    double* Pj = a.proj + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    double r[NC];
+    on_axis<NC>(a.keys[2 * rep], a.keys[2 * rep + 1], a.sids[rep], sweep, r);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        P[i] = (int32_t)i;
+        Z[i] = 0;
+        Pj[i] = on_proj<NC>(S, i, r);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) a.stats[2 * rep + threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void on_sw_union(OnClArgs a)
+{
+    const int rep = blockIdx.y;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N, nb = (uint32_t)g.d * N;
+    const double* Pj = a.proj + (int64_t)rep * N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const double beta = (double)a.betas[rep];
+    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+    for (uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x; 4 * blk < nb;
+         blk += gridDim.x * blockDim.x) {
+        const u32x4 r = philox4x32_10(u32x4{blk, (uint32_t)sweep, zc, sid}, k0, k1);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t b = 4 * blk + t;
+            if (b >= nb) break;
+            const int ax = (int)(b >= N) + (int)(b >= 2 * N);
+            const uint32_t i = b - (uint32_t)ax * N;
+            uint32_t x[3];
+            cl_coords(g, i, x);
+            const uint32_t j = cl_nbr(g, i, x, ax, true);
+            if (on_bond_active(Pj[i] * Pj[j], beta, w[t])) cl_unite(P, i, j);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void on_sw_flatten(OnClArgs a)
+{
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    int32_t* Z = a.size + (int64_t)rep * N;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t r = cl_find_ro(P, i);
+        __hip_atomic_store(P + i, (int32_t)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(Z + r, 1);
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void on_sw_flip(OnClArgs a)
+{
+    __shared__ long long red[2][4];
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    float* S = a.spins + (int64_t)rep * N * NC;
+    const int32_t* P = a.parent + (int64_t)rep * N;
+    const int32_t* Z = a.size + (int64_t)rep * N;
+    const double* Pj = a.proj + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    double r[NC];
+    on_axis<NC>(k0, k1, sid, sweep, r);
+    long long c2 = 0, ncl = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t rt = (uint32_t)P[i];
+        if (rt == i) {
+            const long long z = Z[i];
+            c2 += z * z;
+            ncl += 1;
+        }
+        if ((cl_word(k0, k1, sid, sweep, 1, rt) >> 31) == 0) on_reflect<NC>(S, i, Pj[i], r);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c2 += __shfl_xor(c2, off, 64);
+        ncl += __shfl_xor(ncl, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = c2;
+        red[1][threadIdx.x >> 6] = ncl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t0 = 0, t1 = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            t0 += red[0][w];
+            t1 += red[1][w];
+        }
+        if (t0) atomicAdd((unsigned long long*)&a.stats[2 * rep], (unsigned long long)t0);
+        if (t1) atomicAdd((unsigned long long*)&a.stats[2 * rep + 1], (unsigned long long)t1);
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void on_wolff(OnClArgs a, int count)
+{
+    __shared__ uint32_t s_head, s_tail;
+    const int rep = blockIdx.x;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N;
+    float* S = a.spins + (int64_t)rep * N * NC;
+    int32_t* Q = a.queue + (int64_t)rep * N;
+    int32_t* Mk = a.mark + (int64_t)rep * N;
+    double* Pj = a.proj + (int64_t)rep * N;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const double beta = (double)a.betas[rep];
+    const int z = 2 * g.d;
+    long long total = 0;
+    for (int f = 0; f < count; ++f) {
+        const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
+        double r[NC];
+        on_axis<NC>(k0, k1, sid, sweep, r);
+        if (threadIdx.x == 0) {
+            const uint32_t x = cl_word(k0, k1, sid, sweep, 2, 0);
+            const uint32_t seed = (uint32_t)(((uint64_t)x * N) >> 32);
+            const double p0 = on_proj<NC>(S, seed, r);
+            Pj[seed] = p0;
+            on_reflect<NC>(S, seed, p0, r);
+            Mk[seed] = f;
+            Q[0] = (int32_t)seed;
+            s_head = 0;
+            s_tail = 1;
+        }
+        __syncthreads();
+        const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+        while (true) {
+            const uint32_t lo = s_head, hi = s_tail;
+            if (lo == hi) break;
+            for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+                const uint32_t k = (uint32_t)Q[q];
+                const double pk = Pj[k];
+                uint32_t x[3];
+                cl_coords(g, k, x);
+                for (int c = 0; c < z; ++c) {
+                    const int ax = c >> 1;
+                    const uint32_t j = cl_nbr(g, k, x, ax, (c & 1) == 0);
+                    if (__hip_atomic_load(Mk + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == f)
+                        continue;
+                    const double pj = on_proj<NC>(S, j, r);
+                    const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
+                    const u32x4 rn = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+                    const uint32_t w = b & 3;
+                    const uint32_t u = w == 0 ? rn.x : (w == 1 ? rn.y : (w == 2 ? rn.z : rn.w));
+                    if (!on_bond_active(pk * pj, beta, u)) continue;
+                    if (atomicExch(Mk + j, f) == f) continue;  // another lane added it
+                    Pj[j] = pj;
+                    on_reflect<NC>(S, j, pj, r);
+                    Q[atomicAdd(&s_tail, 1u)] = (int32_t)j;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_head = hi;
+            __syncthreads();
+        }
+        total += s_tail;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.wsize[rep] += total;
+}
+
+__global__ void on_cl_fill(int32_t* p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void on_cl_zero64(int64_t* p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+__global__ void on_cl_record(int R, int64_t n_meas, int64_t meas, const int32_t* acc,
+                             const int64_t* wsize, const int64_t* stats, int have_sw,
+                             int64_t* aux)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= R) return;
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    aux[4 * o + 0] = acc[o];
+    aux[4 * o + 1] = wsize ? wsize[rep] : 0;  // null without cluster rows
+    aux[4 * o + 2] = have_sw && stats ? stats[2 * rep] : 0;
+    aux[4 * o + 3] = have_sw && stats ? stats[2 * rep + 1] : 0;
+}
+
+struct OnClLayout {
+    int64_t parent, size, queue, mark, proj, stats, wsize, total;
+};
+
+inline OnClLayout on_cl_layout(int64_t N, int32_t n_rep)
+{
+    auto up = [](int64_t x) { return (x + 255) / 256 * 256; };
+    OnClLayout l{};
+    int64_t o = 0;
+    l.parent = o; o += up(4 * N * n_rep);
+    l.size = o;   o += up(4 * N * n_rep);
+    l.queue = o;  o += up(4 * N * n_rep);
+    l.mark = o;   o += up(4 * N * n_rep);
+    l.proj = o;   o += up(8 * N * n_rep);
+    l.stats = o;  o += up(16 * (int64_t)n_rep);
+    l.wsize = o;  o += up(8 * (int64_t)n_rep);
+    l.total = o;
+    return l;
+}
+
+template <int NC>
+static void on_cluster_row(OnClArgs a, int64_t code, int64_t count, hipStream_t st)
+{
+    const int64_t N = a.g.N;
+    const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
+    const unsigned bond_chunks = (unsigned)std::min<int64_t>((a.g.d * N / 4 + 256) / 256, 1024);
+    if (code == CODE_SWENDSEN_WANG) {
+        const uint64_t base = a.sweep_add;
+        for (int64_t k = 0; k < count; ++k) {
+            a.sweep_add = base + (uint64_t)k;
+            hipLaunchKernelGGL((on_sw_prep<NC>), dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(on_sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(on_sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            hipLaunchKernelGGL((on_sw_flip<NC>), dim3(site_chunks, a.R), dim3(256), 0, st, a);
+        }
+    } else {
+        const int64_t nm = N * a.R;
+        hipLaunchKernelGGL(on_cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
+                           dim3(256), 0, st, a.mark, nm, -1);
+        hipLaunchKernelGGL((on_wolff<NC>), dim3(a.R), dim3(256), 0, st, a, (int)count);
+    }
+}
+
 __global__ void on_init_kernel(float* __restrict__ spins, int64_t N, int NC,
                                const uint32_t* __restrict__ keys,
                                const uint32_t* __restrict__ sids)
@@ -471,6 +793,13 @@ int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep, 
     return (int64_t)n_rep * nchunk * (1 + n + 2 * ndim) * 8 + (int64_t)n_rep * 8;
 }
 
+int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep)
+{
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= dims[a];
+    return on_cl_layout(N, n_rep).total;
+}
+
 int cm_on_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t n,
                const uint32_t* keys, const uint32_t* stream_ids, void* stream)
 {
@@ -490,7 +819,8 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
               const uint32_t* stream_ids, uint64_t* sweeps, const int64_t* schedule,
               int32_t n_rows, int64_t n_meas, int32_t* acc_out, double* e_out, double* m_out,
               double* sc_out, double* ss_out, void* workspace, int64_t workspace_bytes,
-              const int64_t* n_meas_rep, void* stream)
+              const int64_t* n_meas_rep, int64_t* aux_out, void* cluster_ws,
+              int64_t cluster_ws_bytes, void* stream)
 {
     CbGeo g;
     CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
@@ -501,11 +831,42 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     CM_REQUIRE(g.half <= 0xFFFFFFFFll / 2, "lattice too large");
     CM_REQUIRE((g.half + 255) / 256 * n_rep < (1ll << 32), "too many sites x replicas per launch");
     CM_REQUIRE(workspace_bytes >= cm_on_workspace_bytes(ndim, dims, n_rep, n), "workspace too small");
+    bool any_cluster = false, any_sw = false;
     for (int r = 0; r < n_rows; ++r) {
         const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
-        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_OVERRELAX,
-                   "update code %lld has no checkerboard O(n) kernel", (long long)code);
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH || code == CODE_OVERRELAX ||
+                       code == CODE_WOLFF || code == CODE_SWENDSEN_WANG,
+                   "update code %lld has no O(n) GPU kernel", (long long)code);
         CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
+        any_cluster |= code == CODE_WOLFF || code == CODE_SWENDSEN_WANG;
+        any_sw |= code == CODE_SWENDSEN_WANG;
+    }
+    OnClArgs ca{};
+    if (any_cluster) {
+        CM_REQUIRE(n_meas_rep == nullptr, "cluster updates take no per-chain measurement caps");
+        CM_REQUIRE(g.N * ndim < ((int64_t)1 << 31), "lattice too large for cluster updates");
+        const OnClLayout lay = on_cl_layout(g.N, n_rep);
+        CM_REQUIRE(cluster_ws != nullptr && cluster_ws_bytes >= lay.total,
+                   "cluster updates need a cluster workspace of cm_on_cluster_workspace_bytes bytes");
+        char* cw = (char*)cluster_ws;
+        ca.spins = (float*)spins;
+        ca.betas = betas_dev;
+        ca.keys = keys;
+        ca.sids = stream_ids;
+        ca.sweeps = sweeps;
+        ca.parent = (int32_t*)(cw + lay.parent);
+        ca.size = (int32_t*)(cw + lay.size);
+        ca.queue = (int32_t*)(cw + lay.queue);
+        ca.mark = (int32_t*)(cw + lay.mark);
+        ca.proj = (double*)(cw + lay.proj);
+        ca.stats = (int64_t*)(cw + lay.stats);
+        ca.wsize = (int64_t*)(cw + lay.wsize);
+        ca.g.d = ndim;
+        for (int k = 0; k < 3; ++k) ca.g.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
+        ca.g.N = (uint32_t)g.N;
+        ca.g.div0 = make_fastdiv(ca.g.L[0]);
+        ca.g.div1 = make_fastdiv(ca.g.L[1]);
+        ca.R = n_rep;
     }
     if (n_meas == 0) return CM_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -517,10 +878,21 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, n_meas_rep,
              g, n_rep, n_meas, 0, 0, 0};
     uint64_t sw = 0;
+    const int rb = (n_rep + 255) / 256;
+    if (any_cluster)
+        hipLaunchKernelGGL(on_cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, ca.stats, 2 * n_rep);
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
+        if (any_cluster) hipLaunchKernelGGL(on_cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
         for (int r = 0; r < n_rows; ++r) {
             const int code = (int)schedule[2 * r];
+            if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
+                ca.sweep_add = sw;
+                if (n == 2) on_cluster_row<2>(ca, code, schedule[2 * r + 1], st);
+                else on_cluster_row<3>(ca, code, schedule[2 * r + 1], st);
+                sw += (uint64_t)schedule[2 * r + 1];
+                continue;
+            }
             for (int64_t rp = 0; rp < schedule[2 * r + 1]; ++rp) {
                 a.sweep_add = sw;
                 for (int c = 0; c < 2; ++c) {
@@ -540,6 +912,9 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
             hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
             launch_measure<3>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         }
+        if (aux_out)
+            hipLaunchKernelGGL(on_cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas,
+                               acc_out, ca.wsize, ca.stats, any_sw ? 1 : 0, aux_out);
     }
     int64_t per_meas = 0;
     for (int r = 0; r < n_rows; ++r) per_meas += schedule[2 * r + 1];

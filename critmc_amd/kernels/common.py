@@ -35,5 +35,5 @@ N_AUX = 4
 GPU_UPDATES = {
     "ising": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
     "potts": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
-    "on": ("metropolis", "heatbath", "overrelax"),
+    "on": ("metropolis", "heatbath", "overrelax", "wolff", "sw", "swendsen-wang"),
 }

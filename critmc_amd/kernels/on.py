@@ -15,7 +15,7 @@ import torch
 from .._lib import check, lib
 from ..device import device, host_ptr, ptr, stream
 from .chains import Chains
-from .common import HEATBATH, METROPOLIS, OVERRELAX
+from .common import CLUSTER_CODES, HEATBATH, METROPOLIS, OVERRELAX, SWENDSEN_WANG, WOLFF
 
 
 def _dims(shape) -> np.ndarray:
@@ -29,6 +29,15 @@ class Workspace:
         self.buf = torch.empty(max(self.nbytes, 256), dtype=torch.uint8, device=dev or device())
 
 
+class ClusterWorkspace:
+    """Device scratch for Wolff / Swendsen-Wang rows (parent, size, queue, mark, projections)."""
+
+    def __init__(self, shape, R: int, dev: torch.device | None = None):
+        d = _dims(shape)
+        self.nbytes = int(lib().cm_on_cluster_workspace_bytes(len(d), host_ptr(d, ctypes.c_int32), R))
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=dev or device())
+
+
 def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
     """Random unit vectors: XY angle uniform in [0, 2 pi), Heisenberg isotropic."""
     d = _dims(shape)
@@ -37,14 +46,19 @@ def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
 
 
 def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedule, n_meas: int,
-        *, ws: Workspace | None = None, n_meas_rep: torch.Tensor | None = None):
+        *, ws: Workspace | None = None, n_meas_rep: torch.Tensor | None = None,
+        cluster_ws: ClusterWorkspace | None = None, with_aux: bool = False):
     """Returns device tensors (accepted int32 [R, n_meas], E f64 [R, n_meas],
-    M f64 [R, n_meas, n], stiff_c / stiff_s f64 [R, n_meas, d])."""
+    M f64 [R, n_meas, n], stiff_c / stiff_s f64 [R, n_meas, d]), plus ``aux``
+    int64 [R, n_meas, 4] (accepted, Wolff sizes, sum |C|^2, clusters) when
+    ``with_aux``."""
     R = chains.R
     sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
     for code in sched[:, 0]:
-        if code not in (METROPOLIS, HEATBATH, OVERRELAX):
-            raise ValueError(f"update code {int(code)} has no checkerboard O(n) kernel")
+        if code not in (METROPOLIS, HEATBATH, OVERRELAX, WOLFF, SWENDSEN_WANG):
+            raise ValueError(f"update code {int(code)} has no O(n) GPU kernel")
+    if np.isin(sched[:, 0], CLUSTER_CODES).any() and cluster_ws is None:
+        cluster_ws = ClusterWorkspace(shape, R, spins.device)
     dv = spins.device
     d = _dims(shape)
     b = torch.as_tensor(np.broadcast_to(np.asarray(betas, np.float32), (R,)).copy(), device=dv)
@@ -55,12 +69,16 @@ def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedu
     sc = torch.empty((R, n_meas, len(d)), dtype=torch.float64, device=dv)
     ss = torch.empty((R, n_meas, len(d)), dtype=torch.float64, device=dv)
     ws = ws or Workspace(shape, R, n, dv)
+    aux = torch.empty((R, n_meas, 4), dtype=torch.int64, device=dv) if with_aux else None
     rc = lib().cm_on_run(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, int(n), ptr(b),
                          ptr(st), ptr(chains.keys_d), ptr(chains.sids_d), ptr(chains.sweeps_d),
                          host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(acc),
                          ptr(e), ptr(m), ptr(sc), ptr(ss), ptr(ws.buf), ws.nbytes,
-                         ptr(n_meas_rep), stream())
+                         ptr(n_meas_rep), ptr(aux), ptr(cluster_ws.buf) if cluster_ws else None,
+                         cluster_ws.nbytes if cluster_ws else 0, stream())
     check(rc, "cm_on_run")
     if n_meas_rep is None:
         chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    if with_aux:
+        return acc, e, m, sc, ss, aux
     return acc, e, m, sc, ss

@@ -295,10 +295,15 @@ class ON(Model):
         return k_on.Workspace(self.lattice.shape, 1, self.n)
 
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
-        acc, e, m, sc, ss = k_on.run(ds.spins, self.lattice.shape, self.n, [float(beta)],
-                                     [float(running["step"])], chains, schedule, n_meas, ws=ws)
-        aux = np.zeros((n_meas, N_AUX), dtype=np.int64)
-        aux[:, 0] = acc[0].cpu().numpy()
+        cws = None
+        if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
+            if "cluster_ws" not in ds.extra:
+                ds.extra["cluster_ws"] = k_on.ClusterWorkspace(self.lattice.shape, 1)
+            cws = ds.extra["cluster_ws"]
+        acc, e, m, sc, ss, aux_d = k_on.run(ds.spins, self.lattice.shape, self.n, [float(beta)],
+                                            [float(running["step"])], chains, schedule, n_meas,
+                                            ws=ws, cluster_ws=cws, with_aux=True)
+        aux = aux_d[0].cpu().numpy()
         return {"E": e[0].cpu().numpy(), "M": m[0].cpu().numpy(), "stiff_c": sc[0].cpu().numpy(),
                 "stiff_s": ss[0].cpu().numpy(), "aux": aux}
 

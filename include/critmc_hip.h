@@ -243,19 +243,29 @@ int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep, 
 int cm_on_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
                const uint32_t *keys_dev, const uint32_t *stream_ids_dev, void *stream);
 
-/* Apply `schedule` (codes 0 = metropolis, 5 = overrelax) n_meas times to
- * n_rep FP32 O(n) chains; after every application renormalise in FP64 and
- * record FP64 E [n_rep][n_meas], M [n_rep][n_meas][n], stiffness sums
- * c_a / s_a [n_rep][n_meas][ndim] and int32 accepted Metropolis moves.
+/* Bytes of the cluster workspace cm_on_run needs for Wolff / Swendsen-Wang rows. */
+int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep);
+
+/* Apply `schedule` (codes 0 = metropolis, 2 = heatbath, 3 = wolff,
+ * 4 = swendsen-wang, 5 = overrelax) n_meas times to n_rep FP32 O(n) chains;
+ * after every application renormalise in FP64 and record FP64 E
+ * [n_rep][n_meas], M [n_rep][n_meas][n], stiffness sums c_a / s_a
+ * [n_rep][n_meas][ndim] and int32 accepted Metropolis moves.
  * betas_dev / steps_dev: FP32 [n_rep] (inverse temperature, Metropolis step).
- * n_meas_rep_dev: null or int64 [n_rep] per-chain caps as in cm_ising_run.
- * Replaces critmc/kernels/on.py:348-418 (run) for checkerboard sweeps. */
+ * n_meas_rep_dev: null or int64 [n_rep] per-chain caps as in cm_ising_run
+ * (local updates only).  aux_out_dev: null or int64 [n_rep][n_meas][4], the
+ * reference's aux columns (accepted, Wolff sizes, sum |C|^2 and cluster count
+ * of the last Swendsen-Wang update).  cluster_ws_dev: null, or at least
+ * cm_on_cluster_workspace_bytes for schedules with cluster rows (the embedded
+ * Ising updates of critmc/kernels/on.py:264-345).
+ * Replaces critmc/kernels/on.py:348-418 (run). */
 int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
               const float *betas_dev, const float *steps_dev, const uint32_t *keys_dev,
               const uint32_t *stream_ids_dev, uint64_t *sweeps_dev, const int64_t *schedule,
               int32_t n_rows, int64_t n_meas, int32_t *acc_out_dev, double *e_out_dev,
               double *m_out_dev, double *sc_out_dev, double *ss_out_dev, void *workspace_dev,
-              int64_t workspace_bytes, const int64_t *n_meas_rep_dev, void *stream);
+              int64_t workspace_bytes, const int64_t *n_meas_rep_dev, int64_t *aux_out_dev,
+              void *cluster_ws_dev, int64_t cluster_ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

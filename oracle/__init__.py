@@ -363,6 +363,9 @@ _EXTRA_DECLS.update({
     "orc_on_run": ([_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
                     ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _i64p], ctypes.c_int),
+    "orc_on_run_aux": ([_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _i64p, ctypes.c_int,
+                        ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _i64p, _i64p], ctypes.c_int),
 })
 
 
@@ -392,11 +395,12 @@ class OnReplay:
         sc = np.zeros((n_meas, d))
         ss = np.zeros((n_meas, d))
         acc = np.zeros(n_meas, np.int64)
-        rc = lib().orc_on_run(_ptr(self.s, _f64p), d, _ptr(self.dims, _i32p), self.n, self.beta,
-                              self.step, self.key[0], self.key[1], self.stream,
-                              _ptr(self.sweep, _u64p), _ptr(sched, _i64p), sched.shape[0],
-                              int(n_meas), _ptr(e, _f64p), _ptr(m, _f64p), _ptr(sc, _f64p),
-                              _ptr(ss, _f64p), _ptr(acc, _i64p))
+        aux = np.zeros((n_meas, 4), np.int64)
+        rc = lib().orc_on_run_aux(_ptr(self.s, _f64p), d, _ptr(self.dims, _i32p), self.n, self.beta,
+                                  self.step, self.key[0], self.key[1], self.stream,
+                                  _ptr(self.sweep, _u64p), _ptr(sched, _i64p), sched.shape[0],
+                                  int(n_meas), _ptr(e, _f64p), _ptr(m, _f64p), _ptr(sc, _f64p),
+                                  _ptr(ss, _f64p), _ptr(acc, _i64p), _ptr(aux, _i64p))
         if rc:
             raise ValueError(f"orc_on_run failed ({rc})")
-        return {"E": e, "M": m, "stiff_c": sc, "stiff_s": ss, "accepted": acc}
+        return {"E": e, "M": m, "stiff_c": sc, "stiff_s": ss, "accepted": acc, "aux": aux}

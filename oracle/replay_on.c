@@ -12,6 +12,16 @@
  * 128 attempts, the last proposal kept if all reject; kappa < 1e-8 gives
  * phi = pi (2U - 1) from word 0 of attempt 0.  Heisenberg: block k, r = (x0+1)/2^32,
  * azimuth 2 pi x1/2^32; |h| = 0: z = 2 x2/2^32 - 1, azimuth from x3.
+ * Cluster updates (:264-345): the embedded Ising model along a random unit
+ * vector r (block 0 of colour field 3, tag ORC_TAG_CLUSTER: XY angle 2 pi U from
+ * word 0; n = 3: z = 2U - 1 from word 0, azimuth from word 1); p_i = S_i . r;
+ * bond b = a N + i (i, +e_a neighbour j) active iff x = p_i p_j > 0 and
+ * u < 1 - det_exp(-2 beta x), u = word b&3 of block b>>2 (colour field 0);
+ * Swendsen-Wang reflects S -> S - 2 p r the clusters whose smallest site rt
+ * has word rt&3 of block rt>>2 (colour field 1) with top bit clear; Wolff
+ * grows from seed floor(x N / 2^32) (word 0 of block 0, colour field 2) with
+ * p_k the member's projection before its reflection.  One sweep-counter value
+ * per update.
  * Uniforms: colour site k uses Philox block k (n = 2) or blocks 2k, 2k+1
  * (n = 3) at (sweep, colour, TAG_ON, stream); Box-Muller with
  * u1 = (x+1)/2^32, u2 = x/2^32: g0 = sqrt(-2 ln u1) cos(2 pi u2),
@@ -167,6 +177,128 @@ int64_t orc_on_half_sweep(double *s, int ndim, const int32_t *dims, int n, int c
     return acc;
 }
 
+double orc_det_exp(double x);
+
+static int64_t orc_on_nbr(const int32_t *dims, int ndim, int64_t i, int a, int fwd)
+{
+    (void)ndim;
+    int64_t stride = 1;
+    for (int b = 0; b < a; ++b) stride *= dims[b];
+    const int64_t L = dims[a], x = (i / stride) % L;
+    if (fwd) return x + 1 == L ? i - (L - 1) * stride : i + stride;
+    return x == 0 ? i + (L - 1) * stride : i - stride;
+}
+
+static uint32_t orc_on_word(uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep, int field,
+                            int64_t index)
+{
+    uint32_t w[4];
+    orc_draw_block(k0, k1, sid, sweep, field, ORC_TAG_CLUSTER, (uint32_t)(index >> 2), w);
+    return w[index & 3];
+}
+
+static void orc_on_axis(int n, uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep, double *r)
+{
+    uint32_t w[4];
+    orc_draw_block(k0, k1, sid, sweep, 3, ORC_TAG_CLUSTER, 0, w);
+    if (n == 2) {
+        double phi = 2.0 * M_PI * ((double)w[0] * 0x1p-32);
+        r[0] = cos(phi);
+        r[1] = sin(phi);
+    } else {
+        double z = 2.0 * ((double)w[0] * 0x1p-32) - 1.0, az = 2.0 * M_PI * ((double)w[1] * 0x1p-32);
+        double ww = sqrt(fmax(0.0, 1.0 - z * z));
+        r[0] = ww * cos(az);
+        r[1] = ww * sin(az);
+        r[2] = z;
+    }
+}
+
+static double orc_on_proj(const double *s, int n, int64_t i, const double *r)
+{
+    double p = 0.0;
+    for (int c = 0; c < n; ++c) p += s[i * n + c] * r[c];
+    return p;
+}
+
+static int orc_on_active(double x, double beta, uint32_t w)
+{
+    return x > 0.0 && (double)w * 0x1p-32 < 1.0 - orc_det_exp(-2.0 * beta * x);
+}
+
+static int32_t orc_on_find(int32_t *parent, int32_t i)
+{
+    while (parent[i] != i) i = parent[i] = parent[parent[i]];
+    return i;
+}
+
+static void orc_on_sw(double *s, int ndim, const int32_t *dims, int n, double beta, uint32_t k0,
+                      uint32_t k1, uint32_t sid, uint64_t sweep, int64_t N, int32_t *parent,
+                      int32_t *size, double *proj, int64_t *c2, int64_t *ncl)
+{
+    double r[3];
+    orc_on_axis(n, k0, k1, sid, sweep, r);
+    for (int64_t i = 0; i < N; ++i) {
+        parent[i] = (int32_t)i;
+        proj[i] = orc_on_proj(s, n, i, r);
+    }
+    for (int a = 0; a < ndim; ++a)
+        for (int64_t i = 0; i < N; ++i) {
+            int64_t j = orc_on_nbr(dims, ndim, i, a, 1);
+            if (!orc_on_active(proj[i] * proj[j], beta, orc_on_word(k0, k1, sid, sweep, 0, (int64_t)a * N + i)))
+                continue;
+            int32_t ri = orc_on_find(parent, (int32_t)i), rj = orc_on_find(parent, (int32_t)j);
+            if (ri == rj) continue;
+            if (ri < rj) parent[rj] = ri;
+            else parent[ri] = rj;
+        }
+    for (int64_t i = 0; i < N; ++i) size[i] = 0;
+    for (int64_t i = 0; i < N; ++i) size[orc_on_find(parent, (int32_t)i)] += 1;
+    *c2 = 0;
+    *ncl = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        int32_t rt = orc_on_find(parent, (int32_t)i);
+        if (rt == i) {
+            *c2 += (int64_t)size[i] * size[i];
+            *ncl += 1;
+        }
+        if ((orc_on_word(k0, k1, sid, sweep, 1, rt) >> 31) == 0)
+            for (int c = 0; c < n; ++c) s[i * n + c] -= 2.0 * proj[i] * r[c];
+    }
+}
+
+static int64_t orc_on_wolff(double *s, int ndim, const int32_t *dims, int n, double beta,
+                            uint32_t k0, uint32_t k1, uint32_t sid, uint64_t sweep, int64_t N,
+                            int32_t *queue, int32_t *mark, int tag, double *proj)
+{
+    double r[3];
+    orc_on_axis(n, k0, k1, sid, sweep, r);
+    int64_t seed = (int64_t)(((uint64_t)orc_on_word(k0, k1, sid, sweep, 2, 0) * (uint64_t)N) >> 32);
+    double p0 = orc_on_proj(s, n, seed, r);
+    proj[seed] = p0;
+    for (int c = 0; c < n; ++c) s[seed * n + c] -= 2.0 * p0 * r[c];
+    mark[seed] = tag;
+    queue[0] = (int32_t)seed;
+    int64_t head = 0, tail = 1;
+    while (head < tail) {
+        int64_t k = queue[head++];
+        double pk = proj[k];
+        for (int cc = 0; cc < 2 * ndim; ++cc) {
+            int ax = cc >> 1;
+            int64_t j = orc_on_nbr(dims, ndim, k, ax, (cc & 1) == 0);
+            if (mark[j] == tag) continue;
+            double pj = orc_on_proj(s, n, j, r);
+            int64_t b = (int64_t)ax * N + ((cc & 1) ? j : k);
+            if (!orc_on_active(pk * pj, beta, orc_on_word(k0, k1, sid, sweep, 0, b))) continue;
+            proj[j] = pj;
+            for (int c = 0; c < n; ++c) s[j * n + c] -= 2.0 * pj * r[c];
+            mark[j] = tag;
+            queue[tail++] = (int32_t)j;
+        }
+    }
+    return tail;
+}
+
 /* Normalise, then E, M[n], c_a, s_a (on.py:35-87) over forward bonds. */
 int orc_on_measure(double *s, int ndim, const int32_t *dims, int n, double *e, double *m,
                    double *sc, double *ss)
@@ -200,16 +332,44 @@ int orc_on_measure(double *s, int ndim, const int32_t *dims, int n, double *e, d
     return 0;
 }
 
-int orc_on_run(double *s, int ndim, const int32_t *dims, int n, double beta, double step,
-               uint32_t k0, uint32_t k1, uint32_t sid, uint64_t *sweep, const int64_t *sched,
-               int nrows, int64_t n_meas, double *e_out, double *m_out, double *sc_out,
-               double *ss_out, int64_t *acc_out)
+int orc_on_run_aux(double *s, int ndim, const int32_t *dims, int n, double beta, double step,
+                   uint32_t k0, uint32_t k1, uint32_t sid, uint64_t *sweep, const int64_t *sched,
+                   int nrows, int64_t n_meas, double *e_out, double *m_out, double *sc_out,
+                   double *ss_out, int64_t *acc_out, int64_t *aux_out)
 {
+    int64_t N = 1;
+    for (int a = 0; a < ndim; ++a) N *= dims[a];
+    int32_t *work = NULL;
+    double *proj = NULL;
+    for (int r = 0; r < nrows; ++r) {
+        int code = (int)sched[2 * r];
+        if (code != 0 && code != 2 && code != 3 && code != 4 && code != 5) return -3;
+        if ((code == 3 || code == 4) && !work) {
+            work = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)N);
+            proj = (double *)malloc(sizeof(double) * (size_t)N);
+        }
+    }
     for (int64_t meas = 0; meas < n_meas; ++meas) {
-        int64_t acc = 0;
+        int64_t acc = 0, wsize = 0, c2 = 0, ncl = 0;
         for (int r = 0; r < nrows; ++r) {
             int code = (int)sched[2 * r];
-            if (code != 0 && code != 2 && code != 5) return -3;
+            if (code == 3) {
+                for (int64_t i = 0; i < N; ++i) work[2 * N + i] = -1;
+                for (int64_t rp = 0; rp < sched[2 * r + 1]; ++rp) {
+                    wsize += orc_on_wolff(s, ndim, dims, n, beta, k0, k1, sid, *sweep, N, work,
+                                          work + 2 * N, (int)rp, proj);
+                    *sweep += 1;
+                }
+                continue;
+            }
+            if (code == 4) {
+                for (int64_t rp = 0; rp < sched[2 * r + 1]; ++rp) {
+                    orc_on_sw(s, ndim, dims, n, beta, k0, k1, sid, *sweep, N, work, work + N,
+                              proj, &c2, &ncl);
+                    *sweep += 1;
+                }
+                continue;
+            }
             for (int64_t rp = 0; rp < sched[2 * r + 1]; ++rp) {
                 for (int c = 0; c < 2; ++c) {
                     int64_t a = orc_on_half_sweep(s, ndim, dims, n, code, c, beta, step, k0, k1, sid, *sweep);
@@ -222,6 +382,23 @@ int orc_on_run(double *s, int ndim, const int32_t *dims, int n, double beta, dou
         orc_on_measure(s, ndim, dims, n, &e_out[meas], &m_out[meas * n], &sc_out[meas * ndim],
                        &ss_out[meas * ndim]);
         acc_out[meas] = acc;
+        if (aux_out) {
+            aux_out[4 * meas + 0] = acc;
+            aux_out[4 * meas + 1] = wsize;
+            aux_out[4 * meas + 2] = c2;
+            aux_out[4 * meas + 3] = ncl;
+        }
     }
+    free(work);
+    free(proj);
     return 0;
+}
+
+int orc_on_run(double *s, int ndim, const int32_t *dims, int n, double beta, double step,
+               uint32_t k0, uint32_t k1, uint32_t sid, uint64_t *sweep, const int64_t *sched,
+               int nrows, int64_t n_meas, double *e_out, double *m_out, double *sc_out,
+               double *ss_out, int64_t *acc_out)
+{
+    return orc_on_run_aux(s, ndim, dims, n, beta, step, k0, k1, sid, sweep, sched, nrows, n_meas,
+                          e_out, m_out, sc_out, ss_out, acc_out, NULL);
 }

@@ -96,7 +96,7 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
 
 @pytest.mark.parametrize("n_comp", [2, 3])
 @pytest.mark.parametrize("updates", ["metropolis", "metropolis+3*overrelax", "heatbath",
-                                     "heatbath+2*overrelax"])
+                                     "heatbath+2*overrelax", "sw", "wolff", "overrelax+sw"])
 def test_ring_matches_transfer_operator(n_comp, updates):
     from critmc_amd import ON, Simulation, chain
 
@@ -109,3 +109,32 @@ def test_ring_matches_transfer_operator(n_comp, updates):
     z = (mean - exact.on_ring_energy(beta, n_sites, n_comp)) / err
     assert abs(z) < 4.5, z
     assert np.allclose(np.linalg.norm(sim.state, axis=1), 1.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,shape", [(2, (16, 16)), (3, (8, 8, 4)), (2, (64, 64)), (3, (12,))])
+@pytest.mark.parametrize("sched", [[[4, 1]], [[3, 2]], [[5, 1], [4, 1], [3, 1]]])
+def test_cluster_update_matches_replay(n, shape, sched):
+    """Cluster decisions use FP64 projections of the FP32 spins and the fixed-operation
+    exp, so from an identical state the clusters (aux: Wolff sizes, sum |C|^2, cluster
+    count) agree exactly and the reflected spins to FP32 rounding."""
+    dev = torch.device("cuda", 0)
+    R = 2
+    rng = np.random.default_rng(n + len(shape) + len(sched))
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    chains = Chains(keys, [5, 6], None, dev)
+    N = int(np.prod(shape))
+    s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
+    K.init_spins(s, shape, n, chains)
+    s0 = s.cpu().numpy().astype(np.float64)
+    betas = np.array([0.8, 1.3], dtype=np.float32)
+    acc, e, m, sc, ss, aux = K.run(s, shape, n, betas, [1.0, 1.0], chains, np.array(sched), 1,
+                                   with_aux=True)
+    got = s.cpu().numpy().astype(np.float64)
+    aux = aux.cpu().numpy()
+    for r in range(R):
+        rep = oracle.OnReplay(s0[r], shape, float(betas[r]), 1.0, keys[r], [5, 6][r])
+        ref = rep.run(sched, 1)
+        assert np.array_equal(aux[r], ref["aux"]), (aux[r], ref["aux"])
+        err = _angle_err(got[r], rep.s)
+        assert err.max() < 1e-5, err.max()
+        assert float(e[r, 0]) == pytest.approx(ref["E"][0], rel=1e-6, abs=1e-5)

@@ -58,7 +58,8 @@ def _ring(oracle_lib, n_comp, sched, beta, n_meas, seed, beta_true=None):
 
 
 @pytest.mark.parametrize("n_comp", [2, 3])
-@pytest.mark.parametrize("sched", [[[0, 1]], [[0, 1], [5, 3]], [[2, 1]], [[2, 1], [5, 2]]])
+@pytest.mark.parametrize("sched", [[[0, 1]], [[0, 1], [5, 3]], [[2, 1]], [[2, 1], [5, 2]],
+                                   [[4, 1]], [[3, 2]], [[5, 2], [4, 1]]])
 def test_ring_matches_transfer_operator(oracle_lib, n_comp, sched):
     z = _ring(oracle_lib, n_comp, sched, 1.1, 100_000, seed=5 + n_comp)
     assert abs(z) < 4.5, z
@@ -107,3 +108,31 @@ def test_heisenberg_heatbath_draws_exponential_cosine(oracle_lib, beta):
     u = _pair_samples(oracle_lib, 3, beta, 20_000, seed=int(beta * 10) + 2)
     assert stats.kstest(u, cdf(kappa)).pvalue > 1e-3
     assert stats.kstest(u, cdf(0.85 * kappa)).pvalue < 1e-6
+
+
+@pytest.mark.parametrize("sched", [[[4, 1]], [[3, 2]]])
+def test_ring_cluster_planted_wrong_beta_is_caught(oracle_lib, sched):
+    z = _ring(oracle_lib, 2, sched, 1.1 * 0.85, 100_000, seed=4, beta_true=1.1)
+    assert abs(z) > 4.5, z
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_cluster_reflections_keep_unit_length_and_count(oracle_lib, n):
+    rep = oracle_lib.OnReplay(_random_state(64, n, 9), (8, 8), 0.9, 1.0, (3, 5))
+    out = rep.run([[4, 1], [3, 3]], 20)
+    aux = out["aux"]
+    assert np.all(aux[:, 1] >= 3) and np.all(aux[:, 1] <= 3 * 64)
+    assert np.all(aux[:, 3] >= 1) and np.all(aux[:, 2] >= 64) and np.all(aux[:, 2] <= 64 * 64)
+    assert np.allclose(np.linalg.norm(rep.s, axis=1), 1.0, atol=1e-12)
+    assert int(rep.sweep[0]) == 20 * 4
+
+
+def test_sw_at_infinite_temperature_reflects_half_the_sites(oracle_lib):
+    """beta = 0 activates no bond: every site is its own cluster and reflects
+    with probability 1/2, which flips the sign of its projection on r."""
+    s0 = _random_state(256, 2, 2)
+    rep = oracle_lib.OnReplay(s0, (16, 16), 0.0, 1.0, (8, 8))
+    out = rep.run([[4, 1]], 1)
+    assert out["aux"][0, 3] == 256 and out["aux"][0, 2] == 256
+    moved = np.linalg.norm(rep.s - s0, axis=1) > 1e-12
+    assert 0.4 < moved.mean() < 0.6
