@@ -267,6 +267,28 @@ int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
               int64_t workspace_bytes, const int64_t *n_meas_rep_dev, int64_t *aux_out_dev,
               void *cluster_ws_dev, int64_t cluster_ws_bytes, void *stream);
 
+/* ---- Multi-histogram reweighting (critmc/reweight.py:159-345) ------------ */
+
+/* Bytes of device scratch for K runs and n_targets target betas. */
+int64_t cm_mbar_workspace_bytes(int32_t K, int32_t n_targets);
+
+/* MBAR objective of MultiHistogram._objective (critmc/reweight.py:241-259) over
+ * U distinct shifted energies u_energy_dev [U] with log-multiplicities
+ * u_logmult_dev [U]: out_dev [1 + K + K*K] = (F, gradient [K], Hessian [K][K])
+ * at free energies f_dev [K]; betas_dev, log_n_dev (= ln N_k / g_k) [K];
+ * 1 <= K <= 64.  ld_out_dev: null, or [U] receives ln D(E_u).  Partial sums
+ * are combined in a fixed order, so results are reproducible. */
+int cm_mbar_objective(const double *u_energy_dev, const double *u_logmult_dev, int64_t U,
+                      const double *betas_dev, const double *log_n_dev, const double *f_dev,
+                      int32_t K, double *out_dev, double *ld_out_dev, void *workspace_dev,
+                      int64_t workspace_bytes, void *stream);
+
+/* ln sum_u exp(m_u - beta E_u - ld_u) for each of n_targets betas (MultiHistogram.log_z,
+ * critmc/reweight.py:318-326, before the energy-shift correction). */
+int cm_mbar_log_z(const double *u_energy_dev, const double *u_logmult_dev, const double *ld_dev,
+                  int64_t U, const double *targets_dev, int32_t n_targets, double *out_dev,
+                  void *workspace_dev, int64_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
