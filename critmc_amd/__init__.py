@@ -8,6 +8,8 @@ interfaces and reaches the kernels through the C-ABI library
 ``libcritmc_hip.so`` via ctypes.
 """
 
+from .flathist import (EnergyLevels, Multicanonical, MulticanonicalBatch, WangLandau,
+                       WangLandauBatch)
 from .lattice import Lattice, chain, cubic, hypercubic, make_lattice, square, triangular
 from .models import ON, DeviceState, Heisenberg, Ising, Model, Potts, XY, build_model, derive
 from .reweight import MultiHistogram, Reweighter, canonical_observables, kish_ess
@@ -22,4 +24,5 @@ __all__ = [
     "chain_key", "make_rng", "seed_sequence", "spawn_rngs",
     "Simulation", "TimeSeries", "parse_updates",
     "MultiHistogram", "Reweighter", "canonical_observables", "kish_ess",
+    "EnergyLevels", "WangLandau", "WangLandauBatch", "Multicanonical", "MulticanonicalBatch",
 ]

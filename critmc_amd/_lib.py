@@ -83,6 +83,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_mbar_objective": ([_vp, _vp, _c_i64, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp, _c_i64, _vp],
                           _c_i32),
     "cm_mbar_log_z": ([_vp, _vp, _vp, _c_i64, _vp, _c_i32, _vp, _vp, _c_i64, _vp], _c_i32),
+    "cm_wl_run": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp, _vp,
+                   _vp, _vp, _vp, _vp, _vp, _c_i64, _vp], _c_i32),
+    "cm_muca_run": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp,
+                     _vp, _vp, _c_i64, _c_i64, _vp, _vp, _vp], _c_i32),
     "cm_on_cluster_workspace_bytes": ([_c_i32, _i32p, _c_i32], _c_i64),
     "cm_on_run": (
         [_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp, _vp, _vp, _i64p, _c_i32, _c_i64,

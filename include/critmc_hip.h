@@ -267,6 +267,36 @@ int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
               int64_t workspace_bytes, const int64_t *n_meas_rep_dev, int64_t *aux_out_dev,
               void *cluster_ws_dev, int64_t cluster_ws_bytes, void *stream);
 
+/* ---- Flat-histogram walkers (critmc/kernels/flathist.py) ------------------ */
+
+/* n_moves Wang-Landau moves of each of n_rep independent walkers
+ * (critmc/kernels/flathist.py:26-74, wl_ising / wl_potts), one lane per walker.
+ * q = 0: Ising, levels (E_b + B)/4; q >= 2: Potts, levels E + B; n_levels must
+ * be B/2 + 1 or B + 1.  Per walker: int8 spins [N], idx (level, in/out),
+ * ln_g and int64 hist [n_levels] (in/out), window [lo, hi], Philox key and
+ * stream, moves (the walker's move count, in/out), ln_f, mode_1t (nonzero:
+ * ln f = n_levels_1t / (t + 1) per move t, Belardinelli-Pereyra); active
+ * (null = all) skips walkers whose entry is 0.  Acceptance uses the
+ * fixed-operation e^x (cm_det_exp). */
+int cm_wl_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
+              int32_t *idx_dev, double *ln_g_dev, int64_t *hist_dev, int32_t n_levels,
+              const int32_t *lo_dev, const int32_t *hi_dev, const uint32_t *keys_dev,
+              const uint32_t *stream_ids_dev, uint64_t *moves_dev, const double *ln_f_dev,
+              const int32_t *mode_1t_dev, const double *n_levels_1t_dev, const int32_t *active_dev,
+              int64_t n_moves, void *stream);
+
+/* Multicanonical production with fixed weights ln_w [n_rep][n_levels]
+ * (critmc/kernels/flathist.py:77-132, muca_ising / muca_potts): n_meas
+ * measurements of moves_per_meas moves per walker, recording the level
+ * (int32 idx_out [n_rep][n_meas]) and M (Ising, order = M [n_rep]) or
+ * sum_k n_k^2 (Potts, order = counts [n_rep][q]) into int64 ord_out. */
+int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
+                int32_t *idx_dev, int32_t *order_dev, const double *ln_w_dev, int64_t *hist_dev,
+                int32_t n_levels, const int32_t *lo_dev, const int32_t *hi_dev,
+                const uint32_t *keys_dev, const uint32_t *stream_ids_dev, uint64_t *moves_dev,
+                int64_t n_meas, int64_t moves_per_meas, int32_t *idx_out_dev, int64_t *ord_out_dev,
+                void *stream);
+
 /* ---- Multi-histogram reweighting (critmc/reweight.py:159-345) ------------ */
 
 /* Bytes of device scratch for K runs and n_targets target betas. */

@@ -404,3 +404,48 @@ class OnReplay:
         if rc:
             raise ValueError(f"orc_on_run failed ({rc})")
         return {"E": e, "M": m, "stiff_c": sc, "stiff_s": ss, "accepted": acc, "aux": aux}
+
+
+_EXTRA_DECLS.update({
+    "orc_wl_run": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _f64p, _i64p, ctypes.c_int,
+                    ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p,
+                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int64], ctypes.c_int),
+    "orc_muca_run": ([_i8p, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p, _f64p, _i64p,
+                      ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                      _u64p, ctypes.c_int64, ctypes.c_int64, _i32p, _i64p], ctypes.c_int),
+})
+
+
+class FlatWalker:
+    """One flat-histogram walker replayed on the CPU (q = 0: Ising, else Potts)."""
+
+    def __init__(self, s, dims, q: int, idx: int, lo: int, hi: int, key, stream: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.int8).copy()
+        self.dims = np.ascontiguousarray(dims, dtype=np.int32)
+        self.q = int(q)
+        self.idx = np.array([idx], np.int32)
+        self.lo, self.hi = int(lo), int(hi)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.moves = np.zeros(1, np.uint64)
+        if self.q == 0:
+            self.order = np.array([int(self.s.astype(np.int64).sum())], np.int32)
+        else:
+            self.order = np.bincount(self.s, minlength=self.q).astype(np.int32)
+
+    def wl(self, ln_g: np.ndarray, hist: np.ndarray, ln_f: float, mode_1t: bool, n_levels: float,
+           n_moves: int) -> None:
+        lib().orc_wl_run(_ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.q,
+                         _ptr(self.idx, _i32p), _ptr(ln_g, _f64p), _ptr(hist, _i64p), self.lo,
+                         self.hi, self.key[0], self.key[1], self.stream, _ptr(self.moves, _u64p),
+                         float(ln_f), int(mode_1t), float(n_levels), int(n_moves))
+
+    def muca(self, ln_w: np.ndarray, hist: np.ndarray, n_meas: int, moves_per_meas: int):
+        idx_out = np.zeros(n_meas, np.int32)
+        ord_out = np.zeros(n_meas, np.int64)
+        lib().orc_muca_run(_ptr(self.s, _i8p), len(self.dims), _ptr(self.dims, _i32p), self.q,
+                           _ptr(self.idx, _i32p), _ptr(self.order, _i32p), _ptr(ln_w, _f64p),
+                           _ptr(hist, _i64p), self.lo, self.hi, self.key[0], self.key[1],
+                           self.stream, _ptr(self.moves, _u64p), int(n_meas), int(moves_per_meas),
+                           _ptr(idx_out, _i32p), _ptr(ord_out, _i64p))
+        return idx_out, ord_out

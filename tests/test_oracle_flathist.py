@@ -1,0 +1,126 @@
+"""Flat-histogram replay (critmc/kernels/flathist.py:20-132) against exact results.
+
+* Wang-Landau (halving, then 1/t) on the 4x4 Ising model and on q=3 Potts on
+  2x4 recovers the exact density of states from enumeration; a windowed walk
+  stays in its window and recovers ln g there up to a constant.
+* Multicanonical production with the exact ln g as weights visits every level
+  uniformly and reweights to the exact canonical energy.
+* Planted mistake: weighting the walk with the density of states of another
+  temperature scale (ln g * 1.1) makes the visit histogram visibly non-flat.
+"""
+
+import numpy as np
+import pytest
+from scipy.special import logsumexp
+
+import oracle
+from oracle import exact
+
+
+def _ising_dos():
+    from critmc_amd import square
+
+    lat = square(4)
+    e_b, _ = exact.ising_states(lat.bonds, 16)
+    idx = ((e_b + lat.n_bonds) // 4).astype(int)
+    g = np.bincount(idx, minlength=lat.n_bonds // 2 + 1).astype(float)
+    return lat, g
+
+
+def _potts_dos():
+    from critmc_amd import hypercubic
+
+    lat = hypercubic((2, 4))
+    q = 3
+    import itertools
+
+    st = np.array(list(itertools.product(range(q), repeat=8)))
+    e = -np.sum(st[:, lat.bonds[:, 0]] == st[:, lat.bonds[:, 1]], axis=1)
+    g = np.bincount(e + lat.n_bonds, minlength=lat.n_bonds + 1).astype(float)
+    return lat, q, g
+
+
+def _wl(walker, S, n_sites, ln_f_final=1e-5, flat=0.8, block_sweeps=100):
+    ln_g = np.zeros(S)
+    hist = np.zeros(S, np.int64)
+    seen = np.zeros(S, bool)
+    ln_f, mode, nl, t = 1.0, False, 0.0, 0
+    block = block_sweeps * n_sites
+    while ln_f >= ln_f_final:
+        walker.wl(ln_g, hist, ln_f, mode, nl, block)
+        t += block
+        seen |= hist > 0
+        if mode:
+            ln_f = nl / t
+            continue
+        h = hist[seen]
+        if h.min() >= flat * h.mean():
+            ln_f /= 2.0
+            hist[:] = 0
+            if ln_f < seen.sum() / t:
+                mode, nl = True, float(seen.sum())
+                ln_f = nl / t
+    return ln_g, seen
+
+
+def test_wang_landau_ising_dos():
+    lat, g = _ising_dos()
+    s0 = np.ones(16, np.int8)
+    idx0 = int((-lat.n_bonds * 1 + lat.n_bonds) // 4)  # all up: E_b = -B
+    w = oracle.FlatWalker(s0, (4, 4), 0, idx0, 0, len(g) - 1, (1, 2))
+    ln_g, seen = _wl(w, len(g), 16)
+    assert np.array_equal(seen, g > 0)
+    v = g > 0
+    est = ln_g[v] - logsumexp(ln_g[v]) + np.log(2.0 ** 16)
+    assert np.max(np.abs(est - np.log(g[v]))) < 0.06, est - np.log(g[v])
+
+
+def test_wang_landau_potts_dos_and_window():
+    lat, q, g = _potts_dos()
+    s0 = np.zeros(8, np.int8)
+    w = oracle.FlatWalker(s0, (2, 4), q, 0, 0, len(g) - 1, (3, 4))
+    ln_g, seen = _wl(w, len(g), 8)
+    v = g > 0
+    est = ln_g[v] - logsumexp(ln_g[v]) + 8 * np.log(q)
+    assert np.max(np.abs(est - np.log(g[v]))) < 0.06
+    # a window over the upper half of the levels
+    lo = int(np.flatnonzero(v)[len(np.flatnonzero(v)) // 2])
+    st = np.array([0, 1, 2, 0, 1, 2, 0, 1], np.int8)  # a high-energy start
+    e0 = -int(np.sum(st[lat.bonds[:, 0]] == st[lat.bonds[:, 1]]))
+    w2 = oracle.FlatWalker(st, (2, 4), q, e0 + lat.n_bonds, lo, len(g) - 1, (5, 6))
+    assert lo <= int(w2.idx[0])
+    ln_g2, seen2 = _wl(w2, len(g), 8)
+    assert not seen2[:lo].any()
+    vv = seen2 & v
+    d = ln_g2[vv] - np.log(g[vv])
+    assert np.max(np.abs(d - d.mean())) < 0.06
+
+
+def test_multicanonical_with_exact_weights_is_flat_and_reweights():
+    lat, g = _ising_dos()
+    v = g > 0
+    ln_w = np.where(v, -np.log(np.where(v, g, 1.0)), -np.inf)
+    w = oracle.FlatWalker(np.ones(16, np.int8), (4, 4), 0, 0, 0, len(g) - 1, (7, 8))
+    hist = np.zeros(len(g), np.int64)
+    idx, m = w.muca(ln_w, hist, 200_000, 16)
+    frac = hist[v] / hist.sum()
+    assert np.max(np.abs(frac / frac.mean() - 1.0)) < 0.1
+    e = 4.0 * idx - lat.n_bonds
+    for beta in (0.3, 0.44):
+        lw = -beta * e - ln_w[idx]
+        exact_u = np.sum(g[v] * np.exp(-beta * (4.0 * np.flatnonzero(v) - 32)) * (4.0 * np.flatnonzero(v) - 32))
+        exact_u /= np.sum(g[v] * np.exp(-beta * (4.0 * np.flatnonzero(v) - 32)))
+        got = np.sum(np.exp(lw - lw.max()) * e) / np.sum(np.exp(lw - lw.max()))
+        assert got == pytest.approx(exact_u, abs=0.3)
+    assert np.all(np.abs(m) <= 16)
+
+
+def test_planted_wrong_weights_are_not_flat():
+    lat, g = _ising_dos()
+    v = g > 0
+    ln_w = np.where(v, -1.1 * np.log(np.where(v, g, 1.0)), -np.inf)
+    w = oracle.FlatWalker(np.ones(16, np.int8), (4, 4), 0, 0, 0, len(g) - 1, (7, 8))
+    hist = np.zeros(len(g), np.int64)
+    w.muca(ln_w, hist, 100_000, 16)
+    frac = hist[v] / hist.sum()
+    assert np.max(np.abs(frac / frac.mean() - 1.0)) > 0.5
