@@ -224,55 +224,86 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
         for (int64_t t = threadIdx.x; t < N; t += blockDim.x) gs[t] = lds8[t];
 }
 
+// One 64-lane workgroup per (sample, set): the lanes stage the set's
+// permutation, labels, energies, betas and swap uniforms in LDS, lane 0 runs
+// the sequential swap chain k = 0..m-2 out of LDS, and the lanes write back.
+constexpr int EA_MMAX = 256;
+
 template <bool G>
-__global__ void ea_swap(EaArgs a, uint64_t step)
+__global__ __launch_bounds__(64) void ea_swap(EaArgs a, uint64_t step)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * a.S) return;
+    __shared__ int32_t sp[EA_MMAX], sk[EA_MMAX], sl[EA_MMAX], sacc_f[EA_MMAX];
+    __shared__ double sE[EA_MMAX], sb[EA_MMAX];
+    __shared__ uint32_t sx[EA_MMAX];
+    __shared__ int s_trip;
+    const int t = blockIdx.x;  // s * 2 + r
     const int s = t >> 1, r = t & 1;
     const int m = a.m;
     const int64_t base = (int64_t)t * m;
     int32_t* perm = a.perm + base;
     int32_t* kofc = a.kofc + base;
     int32_t* lab = a.labels + base;
-    const int32_t* E = G ? nullptr : a.energies + base;
-    const double* Ef = G ? a.energies_f + base : nullptr;
-    const double* beta = a.betas + (int64_t)s * m;
-    int32_t* sacc = a.swap_acc + (int64_t)t * (m - 1);
-    int32_t* stry = a.swap_try + (int64_t)t * (m - 1);
     const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
     const uint32_t zc = ctr_z(step, 0, TAG_PT);
-    uint32_t xs[4] = {0, 0, 0, 0};
-    for (int k = 0; k < m - 1; ++k) {
-        if ((k & 3) == 0) {
-            const u32x4 rn = philox4x32_10(u32x4{(uint32_t)(k >> 2), (uint32_t)step, zc, (uint32_t)r}, k0, k1);
-            xs[0] = rn.x; xs[1] = rn.y; xs[2] = rn.z; xs[3] = rn.w;
-        }
-        const int c1 = perm[k], c2 = perm[k + 1];
-        const double x = (beta[k] - beta[k + 1]) * (G ? Ef[c1] - Ef[c2] : (double)(E[c1] - E[c2]));
-        stry[k] += 1;
-        if (x >= 0.0 || (double)xs[k & 3] * 0x1p-32 < det_exp(x)) {
-            perm[k] = c2;
-            perm[k + 1] = c1;
-            kofc[c2] = k;
-            kofc[c1] = k + 1;
-            sacc[k] += 1;
-        }
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        sp[k] = perm[k];
+        sk[k] = kofc[k];
+        sl[k] = lab[k];
+        sE[k] = G ? a.energies_f[base + k] : (double)a.energies[base + k];
+        sb[k] = a.betas[(int64_t)s * m + k];
+        sacc_f[k] = 0;
     }
-    const int hot = perm[0], cold = perm[m - 1];
-    if (lab[hot] == 2) a.trips[t] += 1;
-    lab[hot] = 1;
-    if (lab[cold] == 1) a.trips[t] += 1;
-    lab[cold] = 2;
+    for (int b = threadIdx.x; 4 * b < m - 1; b += blockDim.x) {
+        const u32x4 rn = philox4x32_10(u32x4{(uint32_t)b, (uint32_t)step, zc, (uint32_t)r}, k0, k1);
+        sx[4 * b] = rn.x;
+        if (4 * b + 1 < EA_MMAX) sx[4 * b + 1] = rn.y;
+        if (4 * b + 2 < EA_MMAX) sx[4 * b + 2] = rn.z;
+        if (4 * b + 3 < EA_MMAX) sx[4 * b + 3] = rn.w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < m - 1; ++k) {
+            const int c1 = sp[k], c2 = sp[k + 1];
+            const double x = (sb[k] - sb[k + 1]) * (sE[c1] - sE[c2]);
+            if (x >= 0.0 || (double)sx[k] * 0x1p-32 < det_exp(x)) {
+                sp[k] = c2;
+                sp[k + 1] = c1;
+                sk[c2] = k;
+                sk[c1] = k + 1;
+                sacc_f[k] = 1;
+            }
+        }
+        const int hot = sp[0], cold = sp[m - 1];
+        int trips = 0;
+        if (sl[hot] == 2) ++trips;
+        sl[hot] = 1;
+        if (sl[cold] == 1) ++trips;
+        sl[cold] = 2;
+        s_trip = trips;
+    }
+    __syncthreads();
+    int32_t* sacc = a.swap_acc + (int64_t)t * (m - 1);
+    int32_t* stry = a.swap_try + (int64_t)t * (m - 1);
     int32_t* hist = a.label_hist + (int64_t)t * m * 2;
-    for (int k = 0; k < m; ++k) {
-        const int l = lab[perm[k]];
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        perm[k] = sp[k];
+        kofc[k] = sk[k];
+        lab[k] = sl[k];
+        if (k < m - 1) {
+            stry[k] += 1;
+            sacc[k] += sacc_f[k];
+        }
+        const int l = sl[sp[k]];
         if (l == 1) hist[2 * k] += 1;
         else if (l == 2) hist[2 * k + 1] += 1;
     }
+    if (threadIdx.x == 0 && s_trip) a.trips[t] += s_trip;
 }
 
-template <int Z, bool G>
+// LDS: the site products q_i = s_a[i] s_b[i] of the two sets' configurations
+// at this temperature are staged in LDS (16-byte loads), and q_l uses
+// s_a[j] s_b[j] = q_j, so the site loop reads only that array.
+template <int Z, bool G, bool LDS>
 __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
 {
     __shared__ int ired[2 * 16];
@@ -294,20 +325,34 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
         ctab[x] = cs;
         stab[x] = sn;
     }
+    extern __shared__ int8_t qlds[];
+    if constexpr (LDS) {
+        // N = L^d with L even and d >= 2 is a multiple of 4, and every
+        // configuration starts at a multiple of N: aligned 4-byte chunks
+        const uint32_t* va = (const uint32_t*)sa;
+        const uint32_t* vb = (const uint32_t*)sb;
+        uint32_t* vq = (uint32_t*)qlds;
+        for (uint32_t t = threadIdx.x; t < (uint32_t)(N >> 2); t += blockDim.x) {
+            // +-1 bytes: the product is +1 (0x01) where the bytes agree and -1
+            // (0xFF) where they differ; x ^ y is 0x00 or 0xFE per byte
+            vq[t] = 0x01010101u | (va[t] ^ vb[t]);
+        }
+    }
     __syncthreads();
     const FastDiv dl = make_fastdiv((uint32_t)L);
     int q = 0, ql = 0;
     double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-        const int qi = sa[i] * sb[i];
+    const uint32_t L32 = (uint32_t)L;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)N; i += blockDim.x) {
+        const int qi = LDS ? qlds[i] : sa[i] * sb[i];
         q += qi;
         int x[3];
-        ea_coords<Z / 2>((uint32_t)i, (uint32_t)L, dl, x);
+        ea_coords<Z / 2>(i, L32, dl, x);
 #pragma unroll
         for (int ax = 0; ax < Z / 2; ++ax) {
-            const int64_t stride = ax == 0 ? 1 : (ax == 1 ? L : (int64_t)L * L);
-            const int64_t j = i + (x[ax] + 1 == L ? -(int64_t)(L - 1) * stride : stride);
-            ql += qi * sa[j] * sb[j];
+            const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L32 : L32 * L32);
+            const uint32_t j = (uint32_t)x[ax] + 1 == L32 ? i - (L32 - 1) * stride : i + stride;
+            ql += qi * (LDS ? qlds[j] : sa[j] * sb[j]);
             re[ax] += qi * ctab[x[ax]];
             im[ax] += qi * stab[x[ax]];
         }
@@ -446,13 +491,15 @@ static void launch_step(const EaArgs& a, int ndim, int sweep_blocks, int block, 
     if (ndim == 2) {
         if (all_lds) hipLaunchKernelGGL((ea_sweep<4, G, true>), dim3(sweep_blocks), dim3(block), lds, st, a);
         else hipLaunchKernelGGL((ea_sweep<4, G, false>), dim3(sweep_blocks), dim3(block), 0, st, a);
-        hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
-        hipLaunchKernelGGL((ea_measure<4, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
+        hipLaunchKernelGGL((ea_swap<G>), dim3(swap_threads), dim3(64), 0, st, a, step);
+        if (all_lds) hipLaunchKernelGGL((ea_measure<4, G, true>), dim3(a.S * a.m), dim3(256), lds, st, a, t);
+        else hipLaunchKernelGGL((ea_measure<4, G, false>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
     } else {
         if (all_lds) hipLaunchKernelGGL((ea_sweep<6, G, true>), dim3(sweep_blocks), dim3(block), lds, st, a);
         else hipLaunchKernelGGL((ea_sweep<6, G, false>), dim3(sweep_blocks), dim3(block), 0, st, a);
-        hipLaunchKernelGGL((ea_swap<G>), dim3((swap_threads + 63) / 64), dim3(64), 0, st, a, step);
-        hipLaunchKernelGGL((ea_measure<6, G>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
+        hipLaunchKernelGGL((ea_swap<G>), dim3(swap_threads), dim3(64), 0, st, a, step);
+        if (all_lds) hipLaunchKernelGGL((ea_measure<6, G, true>), dim3(a.S * a.m), dim3(256), lds, st, a, t);
+        else hipLaunchKernelGGL((ea_measure<6, G, false>), dim3(a.S * a.m), dim3(256), 0, st, a, t);
     }
 }
 
