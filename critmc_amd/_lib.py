@@ -79,6 +79,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     ),
     "cm_on_workspace_bytes": ([_c_i32, _i32p, _c_i32, _c_i32], _c_i64),
     "cm_on_init": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
+    "cm_on_convert": ([_vp, _vp, _c_i32, _i32p, _c_i32, _c_i32, _c_i32, _vp], _c_i32),
     "cm_mbar_workspace_bytes": ([_c_i32, _c_i32], _c_i64),
     "cm_mbar_objective": ([_vp, _vp, _c_i64, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp, _c_i64, _vp],
                           _c_i32),

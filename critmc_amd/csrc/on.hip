@@ -12,7 +12,11 @@
 //   measurement  :35-87, 404-413  normalise, then E = -sum S_i.S_j, M, and per
 //                axis c_a = sum (S_i^0 S_j^0 + S_i^1 S_j^1), s_a = sum (S_i^0 S_j^1 - S_i^1 S_j^0)
 //
-// Layout: spins [R][N][n] FP32, x-fastest sites.  A half sweep is one launch
+// Layout: spins [R][2 colours][N/2][n] FP32: site i (x fastest) of colour
+// c = (x0 + x1 + x2) & 1 sits in plane c at i >> 1 (cm_on_convert translates
+// from and to site order).  A half sweep then reads and writes its own plane
+// contiguously, and its neighbours are all in the other plane at nb >> 1:
+// full cache lines instead of every other 8 bytes.  A half sweep is one launch
 // over all replicas (grid = colour-site chunks x replicas); measurements are a
 // normalise launch, a partial-sum launch and an in-order finalise launch, so
 // every FP64 result is reproducible run to run.
@@ -151,6 +155,7 @@ struct OnArgs {
     int64_t n_meas, meas;
     uint64_t sweep_add;
     int colour;
+    int norm_out;  // write every site of this half sweep renormalised (the measurement's last)
 };
 
 template <int NC, int Z, int CODE>
@@ -164,7 +169,9 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
         int64_t nb[Z];
-        const int64_t i = cb_site<Z>(a.g, k, a.colour, nb);
+        (void)cb_site<Z>(a.g, k, a.colour, nb);
+        const int64_t i = (int64_t)a.colour * a.g.half + k;            // own plane
+        const int64_t other = (int64_t)(1 - a.colour) * a.g.half;     // neighbours' plane
         // The local field is summed in FP64: its Z FP32 terms are exact there, so
         // a nearly cancelling field keeps full relative precision (in FP32 the
         // sum's absolute error eps * Z would dominate when |h| -> 0).
@@ -173,7 +180,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
         for (int c = 0; c < NC; ++c) hd[c] = 0.0;
 #pragma unroll
         for (int j = 0; j < Z; ++j) {
-            const Vec<NC> t = load_vec<NC>(S, nb[j]);
+            const Vec<NC> t = load_vec<NC>(S, other + (nb[j] >> 1));
 #pragma unroll
             for (int c = 0; c < NC; ++c) hd[c] += (double)t.v[c];
         }
@@ -181,6 +188,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 #pragma unroll
         for (int c = 0; c < NC; ++c) h[c] = (float)hd[c];
         Vec<NC> s = load_vec<NC>(S, i);
+        bool changed = false;
         if (CODE == CODE_HEATBATH) {
             const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
             const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
                     s.v[2] = (float)(u * hz + w * (cp * e1z + sp * e2z));
                 }
             }
-            store_vec<NC>(S, i, s);
+            changed = true;
         } else if (CODE == CODE_OVERRELAX) {
             float hh = 0.0f, sh = 0.0f;
 #pragma unroll
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
                 const float f = 2.0f * sh / hh;
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s.v[c] = f * h[c] - s.v[c];
-                store_vec<NC>(S, i, s);
+                changed = true;
             }
         } else {
             const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
@@ -276,9 +284,21 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
             if (de <= 0.0f || (float)xacc * 0x1p-32f < expf(-a.betas[rep] * de)) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s.v[c] = tr[c];
-                store_vec<NC>(S, i, s);
+                changed = true;
                 accepted = 1;
             }
+        }
+        if (a.norm_out) {
+            // the measurement's renormalisation of this site, fused into its last update
+            double n2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
+            const double nrm = sqrt(n2);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+            store_vec<NC>(S, i, s);
+        } else if (changed) {
+            store_vec<NC>(S, i, s);
         }
     }
     if (CODE == CODE_METROPOLIS) {  // one atomic per workgroup
@@ -333,28 +353,102 @@ __global__ __launch_bounds__(256) void on_measure_partial(const float* __restric
     const int ext[3] = {g.L0, g.L1, g.L2};
     const FastDiv dv[2] = {make_fastdiv((uint32_t)g.L0), make_fastdiv((uint32_t)g.L1)};
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const Vec<NC> s = load_vec<NC>(S, i);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c];
+        int x[3] = {0, 0, 0};
         uint32_t rest = (uint32_t)i;
 #pragma unroll
         for (int ax = 0; ax < D; ++ax) {
-            int x;
             if (ax < 2) {
                 const uint32_t q = fdiv(rest, dv[ax]);
-                x = (int)(rest - q * (uint32_t)ext[ax]);
+                x[ax] = (int)(rest - q * (uint32_t)ext[ax]);
                 rest = q;
             } else {
-                x = (int)rest;
+                x[ax] = (int)rest;
             }
-            const int64_t j = i + (x + 1 == ext[ax] ? -(int64_t)(ext[ax] - 1) * st[ax] : st[ax]);
-            const Vec<NC> t = load_vec<NC>(S, j);
+        }
+        const int par = (x[0] + x[1] + x[2]) & 1;
+        const Vec<NC> s = load_vec<NC>(S, (int64_t)par * g.half + (i >> 1));
+        const int64_t other = (int64_t)(1 - par) * g.half;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c];
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+            const int64_t j = i + (x[ax] + 1 == ext[ax] ? -(int64_t)(ext[ax] - 1) * st[ax] : st[ax]);
+            const Vec<NC> t = load_vec<NC>(S, other + (j >> 1));
             double dot = 0.0;
 #pragma unroll
             for (int c = 0; c < NC; ++c) dot += (double)s.v[c] * (double)t.v[c];
             v[0] -= dot;
             v[1 + NC + ax] += (double)s.v[0] * t.v[0] + (double)s.v[1] * t.v[1];
             v[1 + NC + D + ax] += (double)s.v[0] * t.v[1] - (double)s.v[1] * t.v[0];
+        }
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const double t = wave_sum_f64(v[q]);
+        if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double t = 0.0;
+        for (int u = 0; u < (int)(blockDim.x >> 6); ++u) t += red[threadIdx.x * 4 + u];
+        part[((int64_t)rep * nchunk + chunk) * K + threadIdx.x] = t;
+    }
+}
+
+// Renormalise-and-measure over the colour-0 sites, for measurements whose
+// last half sweep (colour 1) already wrote its sites renormalised.  Every
+// bond joins a colour-0 site i to a colour-1 site, so i's lane takes all 2d
+// bonds of i (forward (i, i+e_a) and backward (i-e_a, i), oriented as the
+// reference's forward bonds), renormalises i in place (no other lane reads
+// it) and adds M of i and of i+e_0 (a bijection onto colour 1).
+template <int NC, int D>
+__global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, CbGeo g,
+                                                     double* __restrict__ part, int nchunk,
+                                                     const int64_t* __restrict__ n_meas_rep,
+                                                     int64_t meas)
+{
+    constexpr int K = 1 + NC + 2 * D;
+    constexpr int Z = 2 * D;
+    __shared__ double red[K * 4];
+    const int rep = blockIdx.y, chunk = blockIdx.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;
+    float* S = spins + (int64_t)rep * g.N * NC;
+    const int64_t P1 = g.half;  // colour-1 plane
+    double v[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = 0.0;
+    const int64_t k0 = (int64_t)chunk * ON_CHUNK;
+    const int64_t k1 = min(g.half, k0 + ON_CHUNK);
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        int64_t nb[Z];
+        (void)cb_site<Z>(g, k, 0, nb);
+        Vec<NC> s = load_vec<NC>(S, k);
+        double n2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
+        const double nrm = sqrt(n2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+        store_vec<NC>(S, k, s);
+        const Vec<NC> xr = load_vec<NC>(S, P1 + (nb[0] >> 1));  // i + e_0
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c] + (double)xr.v[c];
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+#pragma unroll
+            for (int dir = 0; dir < 2; ++dir) {
+                const Vec<NC> t = load_vec<NC>(S, P1 + (nb[2 * ax + dir] >> 1));
+                // bond (a, b) = (i, i+e_ax) forward, (i-e_ax, i) backward
+                const Vec<NC>& A = dir == 0 ? s : t;
+                const Vec<NC>& B = dir == 0 ? t : s;
+                double dot = 0.0;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) dot += (double)A.v[c] * (double)B.v[c];
+                v[0] -= dot;
+                v[1 + NC + ax] += (double)A.v[0] * B.v[0] + (double)A.v[1] * B.v[1];
+                v[1 + NC + D + ax] += (double)A.v[0] * B.v[1] - (double)A.v[1] * B.v[0];
+            }
         }
     }
     const int w = threadIdx.x >> 6;
@@ -445,21 +539,29 @@ __device__ __forceinline__ void on_axis(uint32_t k0, uint32_t k1, uint32_t sid, 
     }
 }
 
+// Position of site i in the colour-split layout.
+__device__ __forceinline__ uint32_t on_split(const ClGeo& g, uint32_t i)
+{
+    uint32_t x[3];
+    cl_coords(g, i, x);
+    return ((x[0] + x[1] + x[2]) & 1u) * (g.N >> 1) + (i >> 1);
+}
+
 template <int NC>
-__device__ __forceinline__ double on_proj(const float* S, uint32_t i, const double (&r)[NC])
+__device__ __forceinline__ double on_proj(const float* S, uint32_t p_i, const double (&r)[NC])
 {
     double p = 0.0;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) p += (double)S[(int64_t)NC * i + c] * r[c];
+    for (int c = 0; c < NC; ++c) p += (double)S[(int64_t)NC * p_i + c] * r[c];
     return p;
 }
 
 template <int NC>
-__device__ __forceinline__ void on_reflect(float* S, uint32_t i, double p, const double (&r)[NC])
+__device__ __forceinline__ void on_reflect(float* S, uint32_t p_i, double p, const double (&r)[NC])
 {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        float* q = S + (int64_t)NC * i + c;
+        float* q = S + (int64_t)NC * p_i + c;
         *q = (float)((double)*q - 2.0 * p * r[c]);
     }
 }
@@ -484,7 +586,7 @@ __global__ __launch_bounds__(256) void on_sw_prep(OnClArgs a)
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         P[i] = (int32_t)i;
         Z[i] = 0;
-        Pj[i] = on_proj<NC>(S, i, r);
+        Pj[i] = on_proj<NC>(S, on_split(a.g, i), r);
     }
     if (blockIdx.x == 0 && threadIdx.x < 2) a.stats[2 * rep + threadIdx.x] = 0;
 }
@@ -553,7 +655,7 @@ __global__ __launch_bounds__(256) void on_sw_flip(OnClArgs a)
             c2 += z * z;
             ncl += 1;
         }
-        if ((cl_word(k0, k1, sid, sweep, 1, rt) >> 31) == 0) on_reflect<NC>(S, i, Pj[i], r);
+        if ((cl_word(k0, k1, sid, sweep, 1, rt) >> 31) == 0) on_reflect<NC>(S, on_split(a.g, i), Pj[i], r);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -598,9 +700,10 @@ __global__ __launch_bounds__(256) void on_wolff(OnClArgs a, int count)
         if (threadIdx.x == 0) {
             const uint32_t x = cl_word(k0, k1, sid, sweep, 2, 0);
             const uint32_t seed = (uint32_t)(((uint64_t)x * N) >> 32);
-            const double p0 = on_proj<NC>(S, seed, r);
+            const uint32_t ps = on_split(g, seed);
+            const double p0 = on_proj<NC>(S, ps, r);
             Pj[seed] = p0;
-            on_reflect<NC>(S, seed, p0, r);
+            on_reflect<NC>(S, ps, p0, r);
             Mk[seed] = f;
             Q[0] = (int32_t)seed;
             s_head = 0;
@@ -621,7 +724,8 @@ __global__ __launch_bounds__(256) void on_wolff(OnClArgs a, int count)
                     const uint32_t j = cl_nbr(g, k, x, ax, (c & 1) == 0);
                     if (__hip_atomic_load(Mk + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == f)
                         continue;
-                    const double pj = on_proj<NC>(S, j, r);
+                    const uint32_t sj = on_split(g, j);
+                    const double pj = on_proj<NC>(S, sj, r);
                     const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
                     const u32x4 rn = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
                     const uint32_t w = b & 3;
@@ -629,7 +733,7 @@ __global__ __launch_bounds__(256) void on_wolff(OnClArgs a, int count)
                     if (!on_bond_active(pk * pj, beta, u)) continue;
                     if (atomicExch(Mk + j, f) == f) continue;  // another lane added it
                     Pj[j] = pj;
-                    on_reflect<NC>(S, j, pj, r);
+                    on_reflect<NC>(S, sj, pj, r);
                     Q[atomicAdd(&s_tail, 1u)] = (int32_t)j;
                 }
             }
@@ -667,6 +771,37 @@ __global__ void on_cl_record(int R, int64_t n_meas, int64_t meas, const int32_t*
     aux[4 * o + 1] = wsize ? wsize[rep] : 0;  // null without cluster rows
     aux[4 * o + 2] = have_sw && stats ? stats[2 * rep] : 0;
     aux[4 * o + 3] = have_sw && stats ? stats[2 * rep + 1] : 0;
+}
+
+inline ClGeo on_clgeo(int32_t ndim, const int32_t* dims)
+{
+    ClGeo c{};
+    c.d = ndim;
+    int64_t N = 1;
+    for (int k = 0; k < 3; ++k) {
+        c.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
+        N *= c.L[k];
+    }
+    c.N = (uint32_t)N;
+    c.div0 = make_fastdiv(c.L[0]);
+    c.div1 = make_fastdiv(c.L[1]);
+    return c;
+}
+
+// Site order [R][N][n] <-> colour-split [R][2][N/2][n]; direction 0: to split.
+__global__ void on_convert_kernel(const float* __restrict__ src, float* __restrict__ dst, ClGeo g,
+                                  int NC, int direction)
+{
+    const int rep = blockIdx.y;
+    const int64_t N = g.N;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.N; i += gridDim.x * blockDim.x) {
+        const int64_t a = ((int64_t)rep * N + i) * NC;
+        const int64_t b = ((int64_t)rep * N + on_split(g, i)) * NC;
+        for (int c = 0; c < NC; ++c) {
+            if (direction == 0) dst[b + c] = src[a + c];
+            else dst[a + c] = src[b + c];
+        }
+    }
 }
 
 struct OnClLayout {
@@ -712,16 +847,17 @@ static void on_cluster_row(OnClArgs a, int64_t code, int64_t count, hipStream_t 
     }
 }
 
-__global__ void on_init_kernel(float* __restrict__ spins, int64_t N, int NC,
+__global__ void on_init_kernel(float* __restrict__ spins, ClGeo g, int NC,
                                const uint32_t* __restrict__ keys,
                                const uint32_t* __restrict__ sids)
 {
+    const int64_t N = g.N;
     const int rep = blockIdx.y;
     const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x) {
         const u32x4 r = draw(k0, k1, sid, 0, 0, TAG_INIT, (uint32_t)i);
-        float* p = spins + ((int64_t)rep * N + i) * NC;
+        float* p = spins + ((int64_t)rep * N + on_split(g, (uint32_t)i)) * NC;
         if (NC == 2) {
             // angle uniform in [0, 2 pi): theta = 2 pi x / 2^32
             float sn, cs;
@@ -779,6 +915,25 @@ static void launch_measure(const float* spins, const CbGeo& g, int R, double* pa
     }
 }
 
+template <int NC>
+static void launch_measure_c0(float* spins, const CbGeo& g, int R, double* part, int64_t n_meas,
+                              int64_t meas, double* e_out, double* m_out, double* sc_out,
+                              double* ss_out, const int64_t* nr, hipStream_t st)
+{
+    const int nc0 = (int)((g.half + ON_CHUNK - 1) / ON_CHUNK);
+    const dim3 grid(nc0, R);
+    if (g.d == 1) {
+        hipLaunchKernelGGL((on_measure_c0<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+    } else if (g.d == 2) {
+        hipLaunchKernelGGL((on_measure_c0<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+    } else {
+        hipLaunchKernelGGL((on_measure_c0<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
+        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+    }
+}
+
 }  // namespace cm
 
 using namespace cm;
@@ -809,7 +964,23 @@ int cm_on_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, in
     CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
     const int blocks = (int)std::min<int64_t>((g.N + 255) / 256, 1024);
     hipLaunchKernelGGL(on_init_kernel, dim3(blocks, n_rep), dim3(256), 0, (hipStream_t)stream,
-                       (float*)spins, g.N, n, keys, stream_ids);
+                       (float*)spins, on_clgeo(ndim, dims), n, keys, stream_ids);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_on_convert(const void* src, void* dst, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                  int32_t n, int32_t direction, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(n == 2 || n == 3, "n must be 2 or 3");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(direction == 0 || direction == 1, "direction must be 0 (to split) or 1 (to sites)");
+    CM_REQUIRE(src != dst, "conversion is out of place");
+    const int blocks = (int)std::min<int64_t>((g.N + 255) / 256, 1024);
+    hipLaunchKernelGGL(on_convert_kernel, dim3(blocks, n_rep), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)src, (float*)dst, on_clgeo(ndim, dims), n, direction);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }
@@ -861,11 +1032,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         ca.proj = (double*)(cw + lay.proj);
         ca.stats = (int64_t*)(cw + lay.stats);
         ca.wsize = (int64_t*)(cw + lay.wsize);
-        ca.g.d = ndim;
-        for (int k = 0; k < 3; ++k) ca.g.L[k] = k < ndim ? (uint32_t)dims[k] : 1u;
-        ca.g.N = (uint32_t)g.N;
-        ca.g.div0 = make_fastdiv(ca.g.L[0]);
-        ca.g.div1 = make_fastdiv(ca.g.L[1]);
+        ca.g = on_clgeo(ndim, dims);
         ca.R = n_rep;
     }
     if (n_meas == 0) return CM_OK;
@@ -876,7 +1043,12 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     hipLaunchKernelGGL(fill_i32_on, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)),
                        dim3(256), 0, st, acc_out, total);
     OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, n_meas_rep,
-             g, n_rep, n_meas, 0, 0, 0};
+             g, n_rep, n_meas, 0, 0, 0, 0};
+    // When the schedule ends with a local row, its last half sweep (colour 1)
+    // writes renormalised spins and the measurement renormalises colour 0.
+    const int64_t last_code = schedule[2 * (n_rows - 1)];
+    const bool fused = last_code == CODE_METROPOLIS || last_code == CODE_HEATBATH ||
+                       last_code == CODE_OVERRELAX;
     uint64_t sw = 0;
     const int rb = (n_rep + 255) / 256;
     if (any_cluster)
@@ -897,6 +1069,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                 a.sweep_add = sw;
                 for (int c = 0; c < 2; ++c) {
                     a.colour = c;
+                    a.norm_out = fused && r == n_rows - 1 && rp == schedule[2 * r + 1] - 1 && c == 1;
                     if (n == 2) launch_half_z<2>(code, a, st);
                     else launch_half_z<3>(code, a, st);
                 }
@@ -905,7 +1078,10 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         }
         const int64_t sites = (int64_t)n_rep * g.N;
         const int nb = (int)std::min<int64_t>((sites + 255) / 256, 8192);
-        if (n == 2) {
+        if (fused) {
+            if (n == 2) launch_measure_c0<2>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+            else launch_measure_c0<3>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+        } else if (n == 2) {
             hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
             launch_measure<2>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         } else {

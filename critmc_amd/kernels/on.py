@@ -38,8 +38,27 @@ class ClusterWorkspace:
         self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=dev or device())
 
 
+def to_split(sites: torch.Tensor, shape, n: int) -> torch.Tensor:
+    """Site-ordered ``[R, N, n]`` spins -> the device's colour-split layout (same shape)."""
+    d = _dims(shape)
+    out = torch.empty_like(sites)
+    check(lib().cm_on_convert(ptr(sites.contiguous()), ptr(out), len(d), host_ptr(d, ctypes.c_int32),
+                              sites.shape[0], int(n), 0, stream()), "cm_on_convert")
+    return out
+
+
+def to_sites(split: torch.Tensor, shape, n: int) -> torch.Tensor:
+    """Colour-split device spins -> site order ``[R, N, n]`` (x fastest)."""
+    d = _dims(shape)
+    out = torch.empty_like(split)
+    check(lib().cm_on_convert(ptr(split.contiguous()), ptr(out), len(d), host_ptr(d, ctypes.c_int32),
+                              split.shape[0], int(n), 1, stream()), "cm_on_convert")
+    return out
+
+
 def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
-    """Random unit vectors: XY angle uniform in [0, 2 pi), Heisenberg isotropic."""
+    """Random unit vectors (XY angle uniform in [0, 2 pi), Heisenberg isotropic), written
+    in the colour-split layout."""
     d = _dims(shape)
     check(lib().cm_on_init(ptr(spins), len(d), host_ptr(d, ctypes.c_int32), chains.R, int(n),
                            ptr(chains.keys_d), ptr(chains.sids_d), stream()), "cm_on_init")

@@ -286,10 +286,11 @@ class ON(Model):
 
     def to_device(self, state, running):
         s = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float32)).to(device())
-        return DeviceState(s.reshape(1, self.n_sites, self.n))
+        return DeviceState(k_on.to_split(s.reshape(1, self.n_sites, self.n), self.lattice.shape, self.n))
 
     def from_device(self, ds):
-        return ds.spins.reshape(self.n_sites, self.n).cpu().numpy().astype(np.float64)
+        sites = k_on.to_sites(ds.spins, self.lattice.shape, self.n)
+        return sites.reshape(self.n_sites, self.n).cpu().numpy().astype(np.float64)
 
     def workspace(self):
         return k_on.Workspace(self.lattice.shape, 1, self.n)

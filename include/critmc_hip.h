@@ -237,9 +237,19 @@ int cm_ea_pt_run(void *spins_dev, const void *J_dev, int32_t jtype, const int32_
 /* Bytes of caller-owned device scratch cm_on_run needs (FP64 partial sums). */
 int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n);
 
-/* Random FP32 unit vectors [n_rep][N][n] (XY: angle uniform in [0, 2 pi);
- * n = 3: isotropic), Philox tag INIT.  Replaces ON.initial_state
- * (critmc/models.py:248-257) for device-side batches. */
+/* O(n) spins live on the device in the colour-split layout
+ * [n_rep][2 colours][N/2][n] FP32: site i (x fastest) of colour
+ * c = (x0 + x1 + x2) mod 2 is entry i >> 1 of plane c. */
+
+/* Site order [n_rep][N][n] <-> colour-split layout, out of place;
+ * direction 0: src in site order, dst split; 1: the reverse. */
+int cm_on_convert(const void *src_dev, void *dst_dev, int32_t ndim, const int32_t *dims,
+                  int32_t n_rep, int32_t n, int32_t direction, void *stream);
+
+/* Random FP32 unit vectors (XY: angle uniform in [0, 2 pi); n = 3:
+ * isotropic), Philox tag INIT, drawn per site index and stored in the
+ * colour-split layout.  Replaces ON.initial_state (critmc/models.py:248-257)
+ * for device-side batches. */
 int cm_on_init(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
                const uint32_t *keys_dev, const uint32_t *stream_ids_dev, void *stream);
 

@@ -49,11 +49,11 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
     N = int(np.prod(shape))
     s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
     K.init_spins(s, shape, n, chains)
-    s0 = s.cpu().numpy().astype(np.float64)
+    s0 = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
     betas = np.array([1.12, 0.9])
     steps = np.array([1.0, 2.0])
     acc, e, m, sc, ss = K.run(s, shape, n, betas, steps, chains, np.array([[code, 1]]), 1)
-    got = s.cpu().numpy().astype(np.float64)
+    got = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
     for r in range(R):
         rep = oracle.OnReplay(s0[r], shape, float(np.float32(betas[r])), float(np.float32(steps[r])),
                               keys[r], [3, 4][r])
@@ -125,11 +125,11 @@ def test_cluster_update_matches_replay(n, shape, sched):
     N = int(np.prod(shape))
     s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
     K.init_spins(s, shape, n, chains)
-    s0 = s.cpu().numpy().astype(np.float64)
+    s0 = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
     betas = np.array([0.8, 1.3], dtype=np.float32)
     acc, e, m, sc, ss, aux = K.run(s, shape, n, betas, [1.0, 1.0], chains, np.array(sched), 1,
                                    with_aux=True)
-    got = s.cpu().numpy().astype(np.float64)
+    got = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
     aux = aux.cpu().numpy()
     for r in range(R):
         rep = oracle.OnReplay(s0[r], shape, float(betas[r]), 1.0, keys[r], [5, 6][r])
@@ -138,3 +138,18 @@ def test_cluster_update_matches_replay(n, shape, sched):
         err = _angle_err(got[r], rep.s)
         assert err.max() < 1e-5, err.max()
         assert float(e[r, 0]) == pytest.approx(ref["E"][0], rel=1e-6, abs=1e-5)
+
+
+@pytest.mark.parametrize("n,shape", [(2, (8, 6)), (3, (4, 4, 6)), (2, (10,))])
+def test_split_layout_round_trip(n, shape):
+    """Site i of colour c = (sum of coordinates) mod 2 is entry i >> 1 of plane c."""
+    dev = torch.device("cuda", 0)
+    N = int(np.prod(shape))
+    x = torch.randn((2, N, n), dtype=torch.float32, device=dev)
+    sp = K.to_split(x, shape, n)
+    coords = np.stack(np.unravel_index(np.arange(N), shape[::-1])[::-1], axis=1)
+    par = coords.sum(axis=1) % 2
+    want = np.empty((2, N, n), np.float32)
+    want[:, par * (N // 2) + np.arange(N) // 2] = x.cpu().numpy()
+    assert np.array_equal(sp.cpu().numpy(), want)
+    assert torch.equal(K.to_sites(sp, shape, n), x)
