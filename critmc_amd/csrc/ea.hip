@@ -18,7 +18,7 @@
 // temperature -> configuration).
 //
 // One PT step is three launches: ea_sweep (one workgroup per configuration,
-// spins staged in LDS), ea_swap (one thread per (sample, set)), ea_measure
+// spins staged in LDS), ea_swap (one 64-lane workgroup per (sample, set), chain in LDS), ea_measure
 // (one workgroup per (sample, temperature)).
 //
 // Uniforms (key = the sample's): a configuration's site k of colour c uses

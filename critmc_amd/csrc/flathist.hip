@@ -1,8 +1,8 @@
 // Flat-histogram random walks in energy space, batched as many walkers:
 // Wang-Landau (with the 1/t refinement) and multicanonical production
-// (critmc/kernels/flathist.py:20-132, critmc/flathist.py:174-545).
+// (critmc/kernels/flathist.py:22-132, critmc/flathist.py:174-545).
 //
-// Energy levels (flathist.py:1-12): Ising idx = (E_b + B) / 4 (every flip
+// Energy levels (kernels/flathist.py:3-8): Ising idx = (E_b + B) / 4 (every flip
 // changes E_b by a multiple of 4), Potts idx = E + B, B = number of bonds.
 // A move that would leave the walker's window [lo, hi] is rejected and the
 // current level is counted as for a rejected move (Schulz et al. 2003).

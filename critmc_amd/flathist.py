@@ -1,6 +1,6 @@
 """Flat-histogram Monte Carlo on the device: Wang-Landau and multicanonical walkers.
 
-Interface of critmc/flathist.py:33-545.  Wang-Landau (Wang & Landau 2001) walks
+Interface of critmc/flathist.py:36-545.  Wang-Landau (Wang & Landau 2001) walks
 in energy space with acceptance ``min(1, g(E)/g(E'))`` and ``ln g(E) += ln f``
 at every move; ``ln f`` is halved whenever the histogram is flat, and the 1/t
 refinement (Belardinelli & Pereyra 2007) takes over once ``ln f`` falls below
@@ -38,7 +38,7 @@ __all__ = ["EnergyLevels", "WangLandau", "WangLandauBatch", "WLResult", "Multica
 
 @dataclass(frozen=True)
 class EnergyLevels:
-    """Level index <-> (bond) energy of a discrete model (critmc/flathist.py:36-99):
+    """Level index <-> (bond) energy of a discrete model (critmc/flathist.py:36-93):
     Ising ``idx = (E_b + B) / 4``, Potts ``idx = E + B``."""
 
     kind: str
@@ -93,7 +93,7 @@ class EnergyLevels:
 
 def _walk_into_window(model: Model, state: np.ndarray, levels: EnergyLevels, lo: int, hi: int,
                       rng: np.random.Generator, max_steps: int = 10 ** 7) -> int:
-    """Greedy single-site walk until the level is inside ``[lo, hi]`` (critmc/flathist.py:106-132);
+    """Greedy single-site walk until the level is inside ``[lo, hi]`` (critmc/flathist.py:100-129);
     host-side and drawn from the walker's numpy generator, before its Philox key."""
     nbr = model.lattice.nbr
     idx = levels.index(model.energy(state))
@@ -118,7 +118,7 @@ def _walk_into_window(model: Model, state: np.ndarray, levels: EnergyLevels, lo:
 
 @dataclass
 class WLResult:
-    """Outcome of one Wang-Landau walker (critmc/flathist.py:135-176)."""
+    """Outcome of one Wang-Landau walker (critmc/flathist.py:132-171)."""
 
     levels: EnergyLevels
     ln_g: np.ndarray
@@ -217,7 +217,7 @@ class WangLandauBatch:
     ``seeds``: one seed per walker (each walker's stream follows from its seed
     as in :class:`WangLandau`); ``e_windows``: ``None``, one ``(e_min, e_max)``
     for all walkers, or one per walker.  ``flatness`` and ``criterion`` as in
-    critmc/flathist.py:179-198.
+    critmc/flathist.py:174-210.
     """
 
     def __init__(self, model: Model, seeds, *, e_windows=None, flatness: float = 0.8,
@@ -239,7 +239,7 @@ class WangLandauBatch:
 
     def run(self, ln_f_final: float = 1e-6, *, one_over_t: bool = True, check_sweeps: int = 100,
             max_trials: float = 1e12, checkpoints=None) -> list[WLResult]:
-        """Iterate every walker until ``ln f < ln_f_final`` (critmc/flathist.py:237-306)."""
+        """Iterate every walker until ``ln f < ln_f_final`` (critmc/flathist.py:247-318)."""
         W = self.walkers
         R, S = W.R, self.levels.size
         dv = W.dev
@@ -300,7 +300,7 @@ class WangLandauBatch:
 
 
 class WangLandau:
-    """The reference's one-walker Wang-Landau sampler (critmc/flathist.py:179-306)."""
+    """The reference's one-walker Wang-Landau sampler (critmc/flathist.py:174-318)."""
 
     def __init__(self, model: Model, *, e_window=None, flatness: float = 0.8,
                  criterion: str = "flat", seed=None):
@@ -316,7 +316,7 @@ class WangLandau:
 
 @dataclass
 class MucaResult:
-    """Multicanonical series plus the weights that produced them (critmc/flathist.py:309-450)."""
+    """Multicanonical series plus the weights that produced them (critmc/flathist.py:321-452)."""
 
     levels: EnergyLevels
     ln_g: np.ndarray
@@ -421,7 +421,7 @@ class MucaResult:
 
 
 def _muca_weights(levels: EnergyLevels, ln_g, window) -> tuple[np.ndarray, np.ndarray]:
-    """ln g with never-visited levels forbidden, and the weights -ln g (critmc/flathist.py:487-504)."""
+    """ln g with never-visited levels forbidden, and the weights -ln g (critmc/flathist.py:477-490)."""
     if isinstance(ln_g, WLResult):
         ln_g = np.where(ln_g.visited, ln_g.ln_g, -np.inf)
     ln_g = np.asarray(ln_g, dtype=np.float64).copy()
@@ -471,7 +471,7 @@ class MulticanonicalBatch:
 
 
 class Multicanonical:
-    """The reference's one-walker multicanonical driver (critmc/flathist.py:453-545)."""
+    """The reference's one-walker multicanonical driver (critmc/flathist.py:455-545)."""
 
     def __init__(self, model: Model, ln_g, *, e_window=None, seed=None):
         self._batch = MulticanonicalBatch(model, ln_g, [seed], e_windows=[e_window])

@@ -280,7 +280,7 @@ int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep,
 /* ---- Flat-histogram walkers (critmc/kernels/flathist.py) ------------------ */
 
 /* n_moves Wang-Landau moves of each of n_rep independent walkers
- * (critmc/kernels/flathist.py:26-74, wl_ising / wl_potts), one lane per walker.
+ * (critmc/kernels/flathist.py:22-74, wl_ising / wl_potts), one lane per walker.
  * q = 0: Ising, levels (E_b + B)/4; q >= 2: Potts, levels E + B; n_levels must
  * be B/2 + 1 or B + 1.  Per walker: int8 spins [N], idx (level, in/out),
  * ln_g and int64 hist [n_levels] (in/out), window [lo, hi], Philox key and
@@ -312,7 +312,7 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
 /* Bytes of device scratch for K runs and n_targets target betas. */
 int64_t cm_mbar_workspace_bytes(int32_t K, int32_t n_targets);
 
-/* MBAR objective of MultiHistogram._objective (critmc/reweight.py:241-259) over
+/* MBAR objective of MultiHistogram._objective (critmc/reweight.py:231-247) over
  * U distinct shifted energies u_energy_dev [U] with log-multiplicities
  * u_logmult_dev [U]: out_dev [1 + K + K*K] = (F, gradient [K], Hessian [K][K])
  * at free energies f_dev [K]; betas_dev, log_n_dev (= ln N_k / g_k) [K];
@@ -324,7 +324,7 @@ int cm_mbar_objective(const double *u_energy_dev, const double *u_logmult_dev, i
                       int64_t workspace_bytes, void *stream);
 
 /* ln sum_u exp(m_u - beta E_u - ld_u) for each of n_targets betas (MultiHistogram.log_z,
- * critmc/reweight.py:318-326, before the energy-shift correction). */
+ * critmc/reweight.py:313-320, before the energy-shift correction). */
 int cm_mbar_log_z(const double *u_energy_dev, const double *u_logmult_dev, const double *ld_dev,
                   int64_t U, const double *targets_dev, int32_t n_targets, double *out_dev,
                   void *workspace_dev, int64_t workspace_bytes, void *stream);

@@ -1,10 +1,10 @@
 """ORACLE (test infrastructure only): numpy restatement of the reference's
 multi-histogram (MBAR) estimator, critmc/reweight.py:159-345.
 
-Distinct (energy, run weight) pairs with multiplicities (:220-233), the convex
-objective with gradient and Hessian (:241-259), the chained single-histogram
-initial guess (:261-269), Newton with backtracking in the gauge f_0 = 0
-(:271-298), and log Z at a target beta (:318-326).  Pinned in
+Distinct (energy, run weight) pairs with multiplicities (:210-223), the convex
+objective with gradient and Hessian (:231-247), the chained single-histogram
+initial guess (:249-257), Newton with backtracking in the gauge f_0 = 0
+(:259-286), and log Z at a target beta (:313-320).  Pinned in
 tests/test_oracle_mbar.py by closed forms (one run = single-histogram
 reweighting; two runs at one beta; exact samples of a 4x4 Ising model drawn
 from its enumerated energy distribution recover ln Z ratios).

@@ -3,7 +3,7 @@
  * (Wang-Landau with the 1/t refinement, multicanonical production), the
  * bit-exact target of the batched GPU walkers.
  *
- * Restated from the reference: critmc/kernels/flathist.py:20-132 (moves,
+ * Restated from the reference: critmc/kernels/flathist.py:22-132 (moves,
  * window rejection, ln g / histogram updates, recorded series).  Levels:
  * Ising idx = (E_b + B) / 4, Potts idx = E + B.  Move t of a walker draws
  * Philox block 0 at counter t (tag ORC_TAG_FLAT, the walker's stream): word 0

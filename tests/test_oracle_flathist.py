@@ -1,4 +1,4 @@
-"""Flat-histogram replay (critmc/kernels/flathist.py:20-132) against exact results.
+"""Flat-histogram replay (critmc/kernels/flathist.py:22-132) against exact results.
 
 * Wang-Landau (halving, then 1/t) on the 4x4 Ising model and on q=3 Potts on
   2x4 recovers the exact density of states from enumeration; a windowed walk

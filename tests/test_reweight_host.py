@@ -1,4 +1,4 @@
-"""Single-histogram reweighting (critmc/reweight.py:39-157) on the host.
+"""Single-histogram reweighting (critmc/reweight.py:40-156) on the host.
 
 Exact samples of the 4x4 Ising energy drawn at beta0 and reweighted to a
 nearby beta reproduce the exact <E> there; at beta0 the weights are uniform;
