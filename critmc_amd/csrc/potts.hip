@@ -41,71 +41,131 @@ constexpr int64_t POTTS_LDS_STATE = 96 * 1024;
 constexpr int POTTS_WIDE_REPS = 128;        // below this many replicas ...
 constexpr int64_t POTTS_WIDE_SITES = 16384;  // ... lattices this large go wide
 
-template <int Z>
-__device__ __forceinline__ void potts_site(int8_t* __restrict__ S, int64_t i,
-                                           const int64_t (&nbi)[Z], int q, int code, uint32_t xa,
-                                           uint32_t xb, const PottsTab* __restrict__ tb, int& de, int& acc,
-                                           int* __restrict__ counts)
+// One site update; returns the new state (== old when the site keeps it) and
+// adds the energy change and the acceptance to de / acc.  tl is the replica's
+// table staged in LDS.
+//
+// QB > 0 (q <= QB <= 16): the neighbour states are folded into a histogram of
+// 4-bit fields, h = sum_j 16^{s_j}, so n_k = (h >> 4k) & 15 is one bit-field
+// extract.  The heat bath adds w[n_k] for k = 0 .. q-1 in that order, as the
+// reference does, keeping every prefix sum; since the prefix sums never
+// decrease, the first k with r < cum_k equals the number of k <= q-2 with
+// !(r < cum_k), and the default q-1 falls out of the same count.
+// QB == 0: any q <= 127, counts recomputed per state.
+template <int Z, int QB>
+__device__ __forceinline__ int potts_update(int8_t* __restrict__ S, uint32_t i,
+                                            const uint32_t (&nbi)[Z], int q, int code, uint32_t xa,
+                                            uint32_t xb, const PottsTab* __restrict__ tl, int& de,
+                                            int& acc, int& old_out)
 {
     const int old = S[i];
+    old_out = old;
     int nb[Z];
 #pragma unroll
     for (int j = 0; j < Z; ++j) nb[j] = S[nbi[j]];
-    if (code == CODE_METROPOLIS) {
-        int nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
-        if (nw >= old) ++nw;
-        int n_old = 0, n_new = 0;
+    int nw, c_old, c_new;
+    if constexpr (QB > 0) {
+        uint64_t h = 0;
 #pragma unroll
-        for (int j = 0; j < Z; ++j) {
-            n_old += nb[j] == old;
-            n_new += nb[j] == nw;
-        }
-        const int delta = n_old - n_new;
-        if (delta <= 0 || (uint64_t)xb < tb->T[delta]) {
-            S[i] = (int8_t)nw;
-            de += delta;
-            ++acc;
-            atomicAdd(&counts[old], -1);
-            atomicAdd(&counts[nw], 1);
+        for (int j = 0; j < Z; ++j) h += 1ull << (4 * nb[j]);
+        if (code == CODE_METROPOLIS) {
+            nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
+            if (nw >= old) ++nw;
+            c_old = (int)(h >> (4 * old)) & 15;
+            c_new = (int)(h >> (4 * nw)) & 15;
+            const int delta = c_old - c_new;
+            if (!(delta <= 0 || (uint64_t)xb < tl->T[delta])) return old;
+        } else {
+            double cum[QB];
+            double total = 0.0;
+#pragma unroll
+            for (int k = 0; k < QB; ++k) {
+                if (k < q) total = __dadd_rn(total, tl->w[(h >> (4 * k)) & 15]);
+                cum[k] = total;
+            }
+            const double r = __dmul_rn((double)xa * 0x1p-32, total);
+            nw = 0;
+#pragma unroll
+            for (int k = 0; k < QB - 1; ++k)
+                if (k < q - 1) nw += !(r < cum[k]);
+            if (nw == old) return old;
+            c_old = (int)(h >> (4 * old)) & 15;
+            c_new = (int)(h >> (4 * nw)) & 15;
         }
     } else {
-        double total = 0.0;
-        for (int k = 0; k < q; ++k) {
-            int c = 0;
-#pragma unroll
-            for (int j = 0; j < Z; ++j) c += nb[j] == k;
-            total = __dadd_rn(total, tb->w[c]);
-        }
-        const double r = __dmul_rn((double)xa * 0x1p-32, total);
-        int nw = q - 1;
-        double cum = 0.0;
-        for (int k = 0; k < q; ++k) {
-            int c = 0;
-#pragma unroll
-            for (int j = 0; j < Z; ++j) c += nb[j] == k;
-            cum = __dadd_rn(cum, tb->w[c]);
-            if (r < cum) {
-                nw = k;
-                break;
-            }
-        }
-        if (nw != old) {
-            int c_old = 0, c_new = 0;
+        if (code == CODE_METROPOLIS) {
+            nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
+            if (nw >= old) ++nw;
+            c_old = 0;
+            c_new = 0;
 #pragma unroll
             for (int j = 0; j < Z; ++j) {
                 c_old += nb[j] == old;
                 c_new += nb[j] == nw;
             }
-            de += c_old - c_new;
-            ++acc;
-            S[i] = (int8_t)nw;
-            atomicAdd(&counts[old], -1);
-            atomicAdd(&counts[nw], 1);
+            const int delta = c_old - c_new;
+            if (!(delta <= 0 || (uint64_t)xb < tl->T[delta])) return old;
+        } else {
+            double total = 0.0;
+            for (int k = 0; k < q; ++k) {
+                int c = 0;
+#pragma unroll
+                for (int j = 0; j < Z; ++j) c += nb[j] == k;
+                total = __dadd_rn(total, tl->w[c]);
+            }
+            const double r = __dmul_rn((double)xa * 0x1p-32, total);
+            nw = q - 1;
+            double cum = 0.0;
+            for (int k = 0; k < q; ++k) {
+                int c = 0;
+#pragma unroll
+                for (int j = 0; j < Z; ++j) c += nb[j] == k;
+                cum = __dadd_rn(cum, tl->w[c]);
+                if (r < cum) {
+                    nw = k;
+                    break;
+                }
+            }
+            if (nw == old) return old;
+            c_old = 0;
+            c_new = 0;
+#pragma unroll
+            for (int j = 0; j < Z; ++j) {
+                c_old += nb[j] == old;
+                c_new += nb[j] == nw;
+            }
+        }
+    }
+    de += c_old - c_new;
+    ++acc;
+    S[i] = (int8_t)nw;
+    return nw;
+}
+
+// Occupation changes of one wave: for q <= QB every lane reports up to two
+// (old, new) pairs (-1 when unchanged) and the wave counts each state with
+// ballots into the wave-uniform wd[]; must be reached by the whole wave.
+template <int QB>
+__device__ __forceinline__ void wave_count_changes(int (&wd)[QB], int q, int o0, int n0, int o1,
+                                                   int n1)
+{
+#pragma unroll
+    for (int k = 0; k < QB; ++k) {
+        if (k < q) {
+            wd[k] += __popcll(__ballot(n0 == k)) + __popcll(__ballot(n1 == k)) -
+                     __popcll(__ballot(o0 == k)) - __popcll(__ballot(o1 == k));
         }
     }
 }
 
-template <int Z>
+// Copy one replica's 128-byte table into LDS (lanes 0..31).
+__device__ __forceinline__ void stage_tab(PottsTab* __restrict__ dst, const PottsTab* __restrict__ src)
+{
+    if (threadIdx.x < sizeof(PottsTab) / 4)
+        ((uint32_t*)dst)[threadIdx.x] = ((const uint32_t*)src)[threadIdx.x];
+}
+
+template <int Z, int QB>
 __global__ __launch_bounds__(256) void potts_batched(
     int8_t* __restrict__ spins, CbGeo g, int q, int R, const PottsTab* __restrict__ tabs,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
@@ -114,11 +174,14 @@ __global__ __launch_bounds__(256) void potts_batched(
     int32_t* __restrict__ counts_out, int use_lds, const int64_t* __restrict__ n_meas_rep)
 {
     extern __shared__ int4 lds4[];
-    int* counts = (int*)lds4;          // [128]
+    PottsTab* tl = (PottsTab*)lds4;
+    int* counts = (int*)(tl + 1);      // [128]
     int* red = counts + 128;           // [16][2]
     int8_t* lstate = (int8_t*)(red + 32);
     const int nwaves = (blockDim.x + 63) >> 6;
-    const int64_t npairs = (g.half + 1) >> 1;
+    const int lane = threadIdx.x & 63;
+    const uint32_t npairs = (uint32_t)((g.half + 1) >> 1);
+    const uint32_t half = (uint32_t)g.half;
 
     for (int rep = blockIdx.x; rep < R; rep += gridDim.x) {
         int8_t* gs = spins + (int64_t)rep * g.N;
@@ -132,29 +195,56 @@ __global__ __launch_bounds__(256) void potts_batched(
                 for (int64_t t = threadIdx.x; t < g.N; t += blockDim.x) lstate[t] = gs[t];
             }
         }
+        stage_tab(tl, tabs + rep);
         for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = counts_run[(int64_t)rep * q + k];
         __syncthreads();
         const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
-        const PottsTab* tb = tabs + rep;
         uint64_t sweep = sweeps[rep];
         int32_t e = e_run[rep];
         const int64_t n_rep_meas = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
         for (int64_t meas = 0; meas < n_rep_meas; ++meas) {
             int de = 0, acc = 0;
+            int wd[QB > 0 ? QB : 1] = {};
             for (int row = 0; row < sc.n; ++row) {
                 const int code = sc.code[row];
                 for (int rp = 0; rp < sc.count[row]; ++rp) {
                     for (int c = 0; c < 2; ++c) {
                         const uint32_t zc = ctr_z(sweep, (uint32_t)c, TAG_POTTS);
-                        for (int64_t pi = threadIdx.x; pi < npairs; pi += blockDim.x) {
-                            const u32x4 rn = philox4x32_10(
-                                u32x4{(uint32_t)pi, (uint32_t)sweep, zc, sid}, k0, k1);
-                            int64_t nbi[Z];
-                            int64_t i = cb_site<Z>(g, 2 * pi, c, nbi);
-                            potts_site<Z>(S, i, nbi, q, code, rn.x, rn.y, tb, de, acc, counts);
-                            if (2 * pi + 1 < g.half) {
-                                i = cb_site<Z>(g, 2 * pi + 1, c, nbi);
-                                potts_site<Z>(S, i, nbi, q, code, rn.z, rn.w, tb, de, acc, counts);
+                        // wave-uniform trip count: ballots below see the whole wave
+                        for (uint32_t p0 = threadIdx.x - lane; p0 < npairs; p0 += blockDim.x) {
+                            const uint32_t pi = p0 + lane;
+                            int o0 = -1, n0 = -1, o1 = -1, n1 = -1;
+                            if (pi < npairs) {
+                                const u32x4 rn = philox4x32_10(
+                                    u32x4{pi, (uint32_t)sweep, zc, sid}, k0, k1);
+                                uint32_t nbi[Z];
+                                int old;
+                                uint32_t i = cb_site32<Z>(g, 2 * pi, c, nbi);
+                                int nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.x, rn.y, tl, de, acc, old);
+                                if (nw != old) {
+                                    o0 = old;
+                                    n0 = nw;
+                                }
+                                if (2 * pi + 1 < half) {
+                                    i = cb_site32<Z>(g, 2 * pi + 1, c, nbi);
+                                    nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.z, rn.w, tl, de, acc, old);
+                                    if (nw != old) {
+                                        o1 = old;
+                                        n1 = nw;
+                                    }
+                                }
+                            }
+                            if constexpr (QB > 0) {
+                                wave_count_changes<QB>(wd, q, o0, n0, o1, n1);
+                            } else {
+                                if (o0 >= 0) {
+                                    atomicAdd(&counts[o0], -1);
+                                    atomicAdd(&counts[n0], 1);
+                                }
+                                if (o1 >= 0) {
+                                    atomicAdd(&counts[o1], -1);
+                                    atomicAdd(&counts[n1], 1);
+                                }
                             }
                         }
                         __syncthreads();
@@ -164,9 +254,14 @@ __global__ __launch_bounds__(256) void potts_batched(
             }
             de = wave_sum(de);
             acc = wave_sum(acc);
-            if ((threadIdx.x & 63) == 0) {
+            if (lane == 0) {
                 red[2 * (threadIdx.x >> 6)] = de;
                 red[2 * (threadIdx.x >> 6) + 1] = acc;
+                if constexpr (QB > 0) {
+#pragma unroll
+                    for (int k = 0; k < QB; ++k)
+                        if (k < q && wd[k]) atomicAdd(&counts[k], wd[k]);
+                }
             }
             __syncthreads();
             int tde = 0, tac = 0;
@@ -205,7 +300,7 @@ __global__ __launch_bounds__(256) void potts_batched(
 // (site-pair chunks x replicas), states in global memory.  Energy, accepted and
 // occupation deltas are reduced per workgroup and added with integer atomics,
 // so every sum is exact; potts_prefix turns them into running values.
-template <int Z>
+template <int Z, int QB>
 __global__ __launch_bounds__(256) void potts_half_wide(
     int8_t* __restrict__ spins, CbGeo g, int q, int code, int colour, uint64_t sweep_add,
     const PottsTab* __restrict__ tabs, const uint32_t* __restrict__ keys,
@@ -213,6 +308,7 @@ __global__ __launch_bounds__(256) void potts_half_wide(
     int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
     int32_t* __restrict__ acc_out, int32_t* __restrict__ dcounts_out)
 {
+    __shared__ PottsTab tl;
     __shared__ int counts[128];
     __shared__ int red[2 * 4];
     const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
@@ -220,23 +316,51 @@ __global__ __launch_bounds__(256) void potts_half_wide(
     const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;  // whole block
     for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = 0;
+    stage_tab(&tl, tabs + rep);
     __syncthreads();
     int8_t* S = spins + (int64_t)rep * g.N;
-    const PottsTab* tb = tabs + rep;
     const uint64_t sweep = sweeps[rep] + sweep_add;
     const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
-    const int64_t npairs = (g.half + 1) >> 1;
-    const int64_t pi = (int64_t)bx * blockDim.x + threadIdx.x;
+    const uint32_t npairs = (uint32_t)((g.half + 1) >> 1);
+    const uint32_t pi = bx * blockDim.x + threadIdx.x;
     int de = 0, acc = 0;
+    int o0 = -1, n0 = -1, o1 = -1, n1 = -1;
     if (pi < npairs) {
         const u32x4 rn = philox4x32_10(
-            u32x4{(uint32_t)pi, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_POTTS), sid}, k0, k1);
-        int64_t nbi[Z];
-        int64_t i = cb_site<Z>(g, 2 * pi, colour, nbi);
-        potts_site<Z>(S, i, nbi, q, code, rn.x, rn.y, tb, de, acc, counts);
-        if (2 * pi + 1 < g.half) {
-            i = cb_site<Z>(g, 2 * pi + 1, colour, nbi);
-            potts_site<Z>(S, i, nbi, q, code, rn.z, rn.w, tb, de, acc, counts);
+            u32x4{pi, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_POTTS), sid}, k0, k1);
+        uint32_t nbi[Z];
+        int old;
+        uint32_t i = cb_site32<Z>(g, 2 * pi, colour, nbi);
+        int nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.x, rn.y, &tl, de, acc, old);
+        if (nw != old) {
+            o0 = old;
+            n0 = nw;
+        }
+        if (2 * pi + 1 < (uint32_t)g.half) {
+            i = cb_site32<Z>(g, 2 * pi + 1, colour, nbi);
+            nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.z, rn.w, &tl, de, acc, old);
+            if (nw != old) {
+                o1 = old;
+                n1 = nw;
+            }
+        }
+    }
+    if constexpr (QB > 0) {
+        int wd[QB] = {};
+        wave_count_changes<QB>(wd, q, o0, n0, o1, n1);
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < QB; ++k)
+                if (k < q && wd[k]) atomicAdd(&counts[k], wd[k]);
+        }
+    } else {
+        if (o0 >= 0) {
+            atomicAdd(&counts[o0], -1);
+            atomicAdd(&counts[n0], 1);
+        }
+        if (o1 >= 0) {
+            atomicAdd(&counts[o1], -1);
+            atomicAdd(&counts[n1], 1);
         }
     }
     de = wave_sum(de);
@@ -348,7 +472,7 @@ static uint64_t potts_threshold(double p)
     return (uint64_t)std::ceil(p * 4294967296.0);
 }
 
-template <int Z>
+template <int Z, int QB>
 static hipError_t launch_potts(int8_t* spins, const CbGeo& g, int q, int R, const PottsTab* tabs,
                                const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
                                const Sched& sc, int64_t n_meas, int32_t* e_run,
@@ -357,16 +481,56 @@ static hipError_t launch_potts(int8_t* spins, const CbGeo& g, int q, int R, cons
                                int grid, hipStream_t st)
 {
     const int use_lds = g.N <= POTTS_LDS_STATE ? 1 : 0;
-    const size_t lds = (128 + 32) * 4 + (use_lds ? (size_t)((g.N + 15) / 16) * 16 : 0);
+    const size_t lds = sizeof(PottsTab) + (128 + 32) * 4 +
+                       (use_lds ? (size_t)((g.N + 15) / 16) * 16 : 0);
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)potts_batched<Z>,
+        hipError_t e = hipFuncSetAttribute((const void*)potts_batched<Z, QB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((potts_batched<Z>), dim3(grid), dim3(block), lds, st, spins, g, q, R, tabs,
-                       keys, sids, sweeps, sc, n_meas, e_run, counts_run, e_out, acc_out,
+    hipLaunchKernelGGL((potts_batched<Z, QB>), dim3(grid), dim3(block), lds, st, spins, g, q, R,
+                       tabs, keys, sids, sweeps, sc, n_meas, e_run, counts_run, e_out, acc_out,
                        counts_out, use_lds, n_meas_rep);
     return hipGetLastError();
+}
+
+template <int Z>
+static hipError_t launch_potts_q(int8_t* spins, const CbGeo& g, int q, int R, const PottsTab* tabs,
+                                 const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
+                                 const Sched& sc, int64_t n_meas, int32_t* e_run,
+                                 int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
+                                 int32_t* counts_out, const int64_t* n_meas_rep, int block,
+                                 int grid, hipStream_t st)
+{
+    if (q <= 8)
+        return launch_potts<Z, 8>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
+                                  counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
+    if (q <= 16)
+        return launch_potts<Z, 16>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
+                                   counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
+    return launch_potts<Z, 0>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
+                              counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
+}
+
+template <int Z>
+static void launch_wide(const dim3& wgrid, hipStream_t st, int8_t* spins, const CbGeo& g, int q,
+                        int code, int c, uint64_t sw, const PottsTab* tabs, const uint32_t* keys,
+                        const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
+                        const int64_t* n_meas_rep, int32_t* e_out, int32_t* acc_out,
+                        int32_t* counts_out)
+{
+    if (q <= 8)
+        hipLaunchKernelGGL((potts_half_wide<Z, 8>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
+                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
+                           counts_out);
+    else if (q <= 16)
+        hipLaunchKernelGGL((potts_half_wide<Z, 16>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
+                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
+                           counts_out);
+    else
+        hipLaunchKernelGGL((potts_half_wide<Z, 0>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
+                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
+                           counts_out);
 }
 
 }  // namespace cm
@@ -478,11 +642,11 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
                 for (int rp = 0; rp < sc.count[r]; ++rp) {
                     for (int c = 0; c < 2; ++c) {
                         if (z == 2)
-                            hipLaunchKernelGGL(potts_half_wide<2>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<2>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
                         else if (z == 4)
-                            hipLaunchKernelGGL(potts_half_wide<4>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<4>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
                         else
-                            hipLaunchKernelGGL(potts_half_wide<6>, wgrid, dim3(256), 0, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<6>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
                     }
                     ++sw;
                 }
@@ -497,13 +661,13 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     hipError_t err;
     int8_t* s = (int8_t*)spins;
     if (z == 2)
-        err = launch_potts<2>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+        err = launch_potts_q<2>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
                               counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     else if (z == 4)
-        err = launch_potts<4>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+        err = launch_potts_q<4>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
                               counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     else
-        err = launch_potts<6>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
+        err = launch_potts_q<6>(s, g, q, n_rep, dtabs, keys, stream_ids, sweeps, sc, n_meas, e_run,
                               counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     CM_HIP_CHECK(err);
     return CM_OK;
