@@ -200,7 +200,8 @@ def bench_serve(args) -> dict:
                        "arrival_rate_per_s": rate, "budget_GB": 200, "slice_sweeps": 1000},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
-            "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks}
+            "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,
+            "pools": srv.pool_stats()}
 
 
 def bench_copy(args) -> dict:

@@ -115,15 +115,20 @@ __global__ void fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v)
         p[i] = v;
 }
 
-// Turns per-measurement deltas into the running E_b, M series (one thread per chain).
+// Turns per-measurement deltas into the running E_b, M series (one thread per
+// chain).  n_meas_rep: null or per-chain caps; sweeps: null, or advanced by
+// per_meas sweeps per recorded measurement.
 __global__ void prefix_series(int32_t* __restrict__ e_out, int32_t* __restrict__ m_out,
                               int64_t n_meas, int32_t* __restrict__ e_run,
-                              int32_t* __restrict__ m_run, int R)
+                              int32_t* __restrict__ m_run, int R,
+                              const int64_t* __restrict__ n_meas_rep,
+                              uint64_t* __restrict__ sweeps, uint64_t per_meas)
 {
     const int rep = blockIdx.x * blockDim.x + threadIdx.x;
     if (rep >= R) return;
+    const int64_t n = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
     int32_t e = e_run[rep], m = m_run[rep];
-    for (int64_t k = 0; k < n_meas; ++k) {
+    for (int64_t k = 0; k < n; ++k) {
         e += e_out[(int64_t)rep * n_meas + k];
         m += m_out[(int64_t)rep * n_meas + k];
         e_out[(int64_t)rep * n_meas + k] = e;
@@ -131,6 +136,7 @@ __global__ void prefix_series(int32_t* __restrict__ e_out, int32_t* __restrict__
     }
     e_run[rep] = e;
     m_run[rep] = m;
+    if (sweeps) sweeps[rep] += per_meas * (uint64_t)n;
 }
 
 // ---------------------------------------------------------------------------
@@ -325,6 +331,121 @@ __global__ __launch_bounds__(256) void ising2d_batched(
             sweeps[rep] = sweep;
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wide form: one launch per half sweep over every replica, one lane per word
+// of 32 same-colour sites, state in HBM (int8 or packed).  For a few large
+// lattices, where one workgroup per replica would leave most CUs idle.  2D,
+// L0 a multiple of 64, so a word covers one 64-byte segment of a row.  Lane
+// idx = y * G + w draws the same uniforms as word idx of the batched form, so
+// the two forms give identical chains.  A lane rewrites its whole 64-byte
+// segment when a spin changed: the other colour's bytes in it are rewritten
+// with the values every reader of this launch sees, since that colour is not
+// updated in this half sweep.
+
+// Parity-p word (bit m = site x = 2m + p) of the 64 int8 spins at row + 64 w.
+__device__ __forceinline__ void load_segment(const int8_t* __restrict__ seg, uint32_t& even,
+                                             uint32_t& odd)
+{
+    const uint4* v = (const uint4*)seg;
+    even = 0;
+    odd = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 q = v[k];
+        unpack_pair_u32(q.x, even, odd, 8 * k + 0);
+        unpack_pair_u32(q.y, even, odd, 8 * k + 2);
+        unpack_pair_u32(q.z, even, odd, 8 * k + 4);
+        unpack_pair_u32(q.w, even, odd, 8 * k + 6);
+    }
+}
+
+template <int LAYOUT, int NC>
+__global__ __launch_bounds__(64) void ising2d_half_wide(
+    void* __restrict__ spins, int L0, int L1, int code, int c, uint64_t sweep_add,
+    const uint32_t* __restrict__ tabs, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
+    int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
+    int32_t* __restrict__ dm_out, int32_t* __restrict__ acc_out)
+{
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x);
+    const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;  // whole workgroup
+    const int G = L0 >> 6;
+    const int W = L1 * G;
+    const int idx = (int)(bx * blockDim.x + threadIdx.x);
+    Delta d{0, 0, 0};
+    if (idx < W) {
+        const int y = idx / G, w = idx - (idx / G) * G;
+        const int p = (y + c) & 1;
+        const int yp = y + 1 == L1 ? 0 : y + 1;
+        const int ym = y == 0 ? L1 - 1 : y - 1;
+        uint32_t own, same, sh, up, dn, other;
+        int8_t* row8 = nullptr;
+        uint32_t* own_word = nullptr;
+        if (LAYOUT == 1) {
+            uint32_t* S = (uint32_t*)spins + (int64_t)rep * 2 * W;
+            const uint32_t* So = S + (1 - c) * W;
+            own_word = S + c * W + idx;
+            own = *own_word;
+            same = So[idx];
+            if (p == 0) sh = (same << 1) | (So[y * G + (w == 0 ? G - 1 : w - 1)] >> 31);
+            else sh = (same >> 1) | (So[y * G + (w == G - 1 ? 0 : w + 1)] << 31);
+            up = So[yp * G + w];
+            dn = So[ym * G + w];
+            other = 0;
+        } else {
+            int8_t* S = (int8_t*)spins + (int64_t)rep * L0 * L1;
+            row8 = S + (int64_t)y * L0 + 64 * w;
+            uint32_t ev, od;
+            load_segment(row8, ev, od);
+            own = p == 0 ? ev : od;
+            same = p == 0 ? od : ev;
+            other = same;
+            if (p == 0) {
+                const int8_t b = S[(int64_t)y * L0 + (w == 0 ? L0 - 1 : 64 * w - 1)];
+                sh = (same << 1) | (b > 0 ? 1u : 0u);
+            } else {
+                const int8_t b = S[(int64_t)y * L0 + (w == G - 1 ? 0 : 64 * w + 64)];
+                sh = (same >> 1) | (b > 0 ? 0x80000000u : 0u);
+            }
+            load_segment(S + (int64_t)yp * L0 + 64 * w, ev, od);
+            up = p == 0 ? ev : od;
+            load_segment(S + (int64_t)ym * L0 + 64 * w, ev, od);
+            dn = p == 0 ? ev : od;
+        }
+        const uint32_t nb[4] = {same, sh, up, dn};
+        const uint64_t sweep = sweeps[rep] + sweep_add;
+        const uint32_t nw = update_word<4, NC>(code, own, nb, ~0u, tabs + (int64_t)TAB_WORDS * rep,
+                                               keys[2 * rep], keys[2 * rep + 1], sids[rep], sweep,
+                                               (uint32_t)c, (uint32_t)idx, d);
+        if (nw != own) {
+            if (LAYOUT == 1) {
+                *own_word = nw;
+            } else {
+                const uint32_t ev = p == 0 ? nw : other, od = p == 0 ? other : nw;
+                uint4* v = (uint4*)row8;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint4 q;
+                    q.x = pack_pair_u32(ev, od, 8 * k + 0);
+                    q.y = pack_pair_u32(ev, od, 8 * k + 2);
+                    q.z = pack_pair_u32(ev, od, 8 * k + 4);
+                    q.w = pack_pair_u32(ev, od, 8 * k + 6);
+                    v[k] = q;
+                }
+            }
+        }
+    }
+    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+    if (threadIdx.x == 0 && (de | dm | ac)) {
+        const int64_t o = (int64_t)rep * n_meas + meas;
+        atomicAdd(&de_out[o], de);
+        atomicAdd(&dm_out[o], dm);
+        atomicAdd(&acc_out[o], ac);
     }
 }
 
@@ -579,6 +700,47 @@ static bool batched_ok(const Geo& g)
     return g.d == 2 && (g.M <= 32 || g.M % 32 == 0) && (size_t)2 * g.L1 * g.G * 4 <= 96 * 1024;
 }
 
+static bool wide_ok(const Geo& g) { return g.d == 2 && g.L0 % 64 == 0; }
+
+// Auto choice between the batched and wide forms, from measured crossovers on
+// MI355X: the batched form runs one workgroup per replica, so below a few
+// hundred replicas of a large lattice most CUs would idle; the wide form
+// re-reads neighbour rows from HBM every half sweep, which costs more in the
+// int8 layout (8 B per site) than packed (0.6 B per site).
+static bool prefer_wide(const Geo& g, int layout, int n_rep)
+{
+    const int64_t N = g.rows * g.L0;
+    if (!(g.d == 2 && g.L0 % 64 == 0) || N < 65536) return false;
+    if (layout == 1) return n_rep < 512;
+    return n_rep < 64 || (N >= 262144 && n_rep < 256);
+}
+
+template <int LAYOUT, int NC>
+static void launch_wide(const dim3& grid, hipStream_t st, void* spins, const Geo& g, int code,
+                        int c, uint64_t sw, const uint32_t* tabs, const uint32_t* keys,
+                        const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
+                        const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out, int32_t* acc_out)
+{
+    hipLaunchKernelGGL((ising2d_half_wide<LAYOUT, NC>), grid, dim3(64), 0, st, spins, g.L0, g.L1,
+                       code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                       m_out, acc_out);
+}
+
+template <int LAYOUT>
+static void launch_wide_nc(int nc, const dim3& grid, hipStream_t st, void* spins, const Geo& g,
+                           int code, int c, uint64_t sw, const uint32_t* tabs, const uint32_t* keys,
+                           const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas,
+                           int64_t meas, const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out,
+                           int32_t* acc_out)
+{
+    switch (nc) {
+        case 2: launch_wide<LAYOUT, 2>(grid, st, spins, g, code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 4: launch_wide<LAYOUT, 4>(grid, st, spins, g, code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 7: launch_wide<LAYOUT, 7>(grid, st, spins, g, code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        default: launch_wide<LAYOUT, NC_LIMIT>(grid, st, spins, g, code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+    }
+}
+
 }  // namespace cm
 
 using namespace cm;
@@ -686,7 +848,44 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     uint32_t* dtabs = (uint32_t*)workspace;
     CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * 4, st));
 
-    if (form == 0) form = batched_ok(g) ? 2 : 1;
+    if (form == 0) {
+        if (prefer_wide(g, layout, n_rep)) form = 3;
+        else form = batched_ok(g) ? 2 : 1;
+    }
+    if (form == 3) {
+        CM_REQUIRE(wide_ok(g), "wide form needs a 2D lattice with L0 a multiple of 64");
+        CM_REQUIRE(n_rep <= 65535, "n_rep out of range");
+        const int64_t total = (int64_t)n_rep * n_meas;
+        const int fb = (int)std::min<int64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, e_out, total, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, m_out, total, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, acc_out, total, 0);
+        const int64_t words = (int64_t)g.L1 * (g.L0 / 64);
+        const dim3 wgrid((unsigned)((words + 63) / 64), (unsigned)n_rep);
+        uint64_t sw = 0;
+        int64_t per_meas = 0;
+        for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];
+        for (int64_t meas = 0; meas < n_meas; ++meas)
+            for (int r = 0; r < sc.n; ++r)
+                for (int rp = 0; rp < sc.count[r]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        if (layout == 0)
+                            launch_wide_nc<0>(nc, wgrid, st, spins, g, sc.code[r], c, sw, dtabs,
+                                              keys, stream_ids, sweeps, n_meas, meas, n_meas_rep,
+                                              e_out, m_out, acc_out);
+                        else
+                            launch_wide_nc<1>(nc, wgrid, st, spins, g, sc.code[r], c, sw, dtabs,
+                                              keys, stream_ids, sweeps, n_meas, meas, n_meas_rep,
+                                              e_out, m_out, acc_out);
+                    }
+                    ++sw;
+                }
+        hipLaunchKernelGGL(prefix_series, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out,
+                           m_out, n_meas, e_run, m_run, n_rep, n_meas_rep, sweeps,
+                           (uint64_t)per_meas);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    }
     if (form == 2) {
         CM_REQUIRE(batched_ok(g),
                    "batched form needs a 2D lattice with L0 <= 64 or L0 a multiple of 64");
@@ -707,7 +906,7 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         return CM_OK;
     }
     // plain form: one chain at a time, one launch per half sweep
-    CM_REQUIRE(form == 1, "form must be 0 (auto), 1 (plain) or 2 (batched)");
+    CM_REQUIRE(form == 1, "form must be 0 (auto), 1 (plain), 2 (batched) or 3 (wide)");
     CM_REQUIRE(n_meas_rep == nullptr, "per-chain measurement counts need the batched form");
     CM_REQUIRE(layout == 0, "the plain form works on int8 spins");
     CM_REQUIRE(sweeps_host != nullptr, "the plain form needs the sweep counters on the host");
@@ -750,7 +949,8 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     }
     CM_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(prefix_series, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out, m_out,
-                       n_meas, e_run, m_run, n_rep);
+                       n_meas, e_run, m_run, n_rep, (const int64_t*)nullptr, (uint64_t*)nullptr,
+                       (uint64_t)0);
     // advance the device-side sweep counters like the batched form does
     std::vector<uint64_t> nsw(n_rep);
     int64_t per_meas = 0;

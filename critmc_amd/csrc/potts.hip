@@ -9,9 +9,9 @@
 //
 // Up to 96 KB of states per replica (L <= 256 in 2D): one workgroup owns one
 // replica at a time (grid-stride over replicas) and runs the whole call's
-// sweeps with the states in LDS.  Larger lattices, and small batches of
-// lattices of 16384 sites or more, take the wide form: one launch per half
-// sweep over all replicas, states in global memory.  Both forms draw the same
+// sweeps with the states in LDS.  Larger lattices, and batches too small to
+// fill the CUs that way, take the wide form: one launch per half sweep over
+// all replicas, states in global memory.  Both forms draw the same
 // uniforms for the same site and sweep, so they give identical chains.
 //
 // Uniforms: the colour-c sites are enumerated k = 0 .. N/2-1 (row-major, see
@@ -38,8 +38,14 @@ struct PottsTab {
 
 constexpr int POTTS_QMAX = 127;
 constexpr int64_t POTTS_LDS_STATE = 96 * 1024;
-constexpr int POTTS_WIDE_REPS = 128;        // below this many replicas ...
-constexpr int64_t POTTS_WIDE_SITES = 16384;  // ... lattices this large go wide
+// Auto choice of the wide form, from measured crossovers on MI355X (q = 10):
+// one workgroup per replica needs enough replicas to fill the CUs, and at 64
+// KB of LDS per replica (L = 256) only two fit on a CU.
+static bool potts_prefer_wide(int64_t N, int n_rep)
+{
+    return N > POTTS_LDS_STATE || N >= 65536 || (N >= 16384 && n_rep < 1024) ||
+           (N >= 4096 && n_rep < 256);
+}
 
 // One site update; returns the new state (== old when the site keeps it) and
 // adds the energy change and the acceptance to de / acc.  tl is the replica's
@@ -619,10 +625,8 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     }
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
     // grid < 0 asks for the wide form, grid > 0 for one workgroup per replica;
-    // 0 picks the wide form when the states do not fit LDS or when there are
-    // too few replicas of a large lattice to give every CU a workgroup.
-    const bool wide = grid < 0 || (grid == 0 && (g.N > POTTS_LDS_STATE ||
-                                                 (n_rep < POTTS_WIDE_REPS && g.N >= POTTS_WIDE_SITES)));
+    // 0 picks by potts_prefer_wide.
+    const bool wide = grid < 0 || (grid == 0 && potts_prefer_wide(g.N, n_rep));
     if (wide) {
         // wide form: the whole GPU on each half sweep of every replica
         CM_REQUIRE(n_rep <= 65535, "n_rep out of range");

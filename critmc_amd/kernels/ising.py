@@ -22,7 +22,7 @@ from .common import HEATBATH, METROPOLIS, SWENDSEN_WANG, WOLFF
 
 LAYOUT_INT8 = 0
 LAYOUT_PACKED = 1
-FORM_AUTO, FORM_PLAIN, FORM_BATCHED = 0, 1, 2
+FORM_AUTO, FORM_PLAIN, FORM_BATCHED, FORM_WIDE = 0, 1, 2, 3
 START_CODES = {"random": 0, "ordered": 1, "up": 1, "down": 2}
 
 
@@ -102,7 +102,7 @@ def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: n
 
     Returns device int32 ``(E_b, M, accepted)`` series of shape ``[R, n_meas]``;
     ``e_run``/``m_run`` and the chains' sweep counters are advanced in place.
-    ``n_meas_rep`` (int64 device ``[R]``, batched form) caps each replica's
+    ``n_meas_rep`` (int64 device ``[R]``, batched and wide forms) caps each replica's
     count: replica r performs ``min(n_meas, n_meas_rep[r])`` measurements.
     """
     R = chains.R

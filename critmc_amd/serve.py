@@ -87,7 +87,10 @@ class _Pool:
         self.free: list[int] = []
         self.pending: list[Job] = []      # admitted, joining at the next slice boundary
         self.inflight = None              # caps of the slice on the device
-        self.done_event = torch.cuda.Event()
+        self.start_event = torch.cuda.Event(enable_timing=True)
+        self.done_event = torch.cuda.Event(enable_timing=True)
+        self.busy_s = 0.0                 # device time of this pool's slices
+        self.site_updates = 0
         with torch.cuda.stream(self.stream):
             self._grow(8)
 
@@ -193,12 +196,13 @@ class _Pool:
         st = self.state
         with torch.cuda.stream(self.stream):
             caps_d = torch.from_numpy(caps).to(self.dev, non_blocking=False)
+            self.start_event.record(self.stream)
             ch = _DeviceChains(st["keys"], st["sids"], st["sweeps"], self.cap)
             sched = np.array(SCHEDULE[self.model])
             shape = (self.L, self.L)
             if self.model == "ising":
                 out = k_ising.run(st["spins"], shape, self.betas, 0.0, ch, sched, n, st["e"], st["m"],
-                                  form=k_ising.FORM_BATCHED, n_meas_rep=caps_d)
+                                  n_meas_rep=caps_d)
                 rec = {"E": out[0], "M": out[1]}
             elif self.model == "potts":
                 e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
@@ -287,11 +291,13 @@ class Server:
 
     def _retire(self, pool: _Pool, now: float):
         caps, pool.inflight = pool.inflight, None
+        pool.busy_s += pool.start_event.elapsed_time(pool.done_event) / 1e3
         for slot, job in enumerate(pool.jobs):
             if job is None or not caps[slot]:
                 continue
             job.done += int(caps[slot])
             self.site_updates += int(caps[slot]) * job.sites
+            pool.site_updates += int(caps[slot]) * job.sites
             if job.done >= job.sweeps:
                 job.finished_at = now
                 pool.remove(slot)
@@ -344,6 +350,14 @@ class Server:
                 time.sleep(20e-6)
         torch.cuda.synchronize()
         return self.finished
+
+    def pool_stats(self) -> dict[str, dict]:
+        """Per (model, L) pool: device seconds of its slices (pools overlap on the
+        device, so these add up to more than the wall time) and site updates."""
+        return {f"{m}-L{L}" + (f"-q{q}" if m == "potts" else ""):
+                {"busy_s": round(p.busy_s, 3), "site_updates": p.site_updates,
+                 "site_updates_per_ns": round(p.site_updates / max(p.busy_s, 1e-9) / 1e9, 2)}
+                for (m, L, q), p in sorted(self.pools.items())}
 
 
 def poisson_jobs(n: int, rate: float, seed: int = 2024, L_choices=(16, 32, 64, 128, 256, 512),

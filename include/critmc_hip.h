@@ -90,9 +90,12 @@ int cm_ising_convert(void *spins_i8_dev, void *spins_packed_dev, int32_t ndim,
  *   workspace_dev: >= cm_ising_workspace_bytes(n_rep) bytes
  *   form    0 = auto, 1 = plain (one chain, one launch per half sweep),
  *           2 = batched (one launch, LDS-resident bit-packed replicas; 2D with
- *           L0 <= 64 or L0 % 64 == 0)
+ *           L0 <= 64 or L0 % 64 == 0), 3 = wide (one launch per half sweep
+ *           over all replicas, one lane per 32-site word; 2D with L0 % 64 == 0;
+ *           auto picks it for few replicas of >= 65536 sites, where one
+ *           workgroup per replica would leave CUs idle)
  *   block, grid: launch shape of the batched form (0 = default)
- *   n_meas_rep_dev: null, or int64 [n_rep] caps (batched form): chain r performs
+ *   n_meas_rep_dev: null, or int64 [n_rep] caps (batched and wide forms): chain r performs
  *           min(n_meas, n_meas_rep[r]) measurements (continuous batching of
  *           jobs with different remaining work)
  * The plain form synchronises `stream` before returning. */

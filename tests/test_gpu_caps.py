@@ -29,8 +29,9 @@ def _chains(R, seed, dev):
 CAPS = [3, 0, 6, 1, 5]
 
 
-@pytest.mark.parametrize("layout", ["int8", "packed"])
-def test_ising_caps_match_replay(layout):
+@pytest.mark.parametrize("layout,form", [("int8", KI.FORM_BATCHED), ("packed", KI.FORM_BATCHED),
+                                         ("int8", KI.FORM_WIDE), ("packed", KI.FORM_WIDE)])
+def test_ising_caps_match_replay(layout, form):
     dev = torch.device("cuda", 0)
     shape, R, n_meas = (64, 64), len(CAPS), max(CAPS)
     chains, keys, sids, sw0 = _chains(R, 11, dev)
@@ -45,7 +46,7 @@ def test_ising_caps_match_replay(layout):
     lay = KI.LAYOUT_PACKED if layout == "packed" else KI.LAYOUT_INT8
     st = KI.pack(s, shape, R) if lay == KI.LAYOUT_PACKED else s
     e_out, m_out, acc = KI.run(st, shape, betas, 0.0, chains, sched, n_meas, e, m, layout=lay,
-                               form=KI.FORM_BATCHED, n_meas_rep=caps)
+                               form=form, n_meas_rep=caps)
     if lay == KI.LAYOUT_PACKED:
         s = KI.unpack(st, shape, R)
     got = s.cpu().numpy()

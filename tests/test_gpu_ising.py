@@ -84,6 +84,11 @@ CASES = [
     ((8,), K.FORM_PLAIN, 0),
     ((4, 6, 8), K.FORM_PLAIN, 0),
     ((34, 2, 2), K.FORM_PLAIN, 0),
+    ((64, 64), K.FORM_WIDE, 0),
+    ((128, 32), K.FORM_WIDE, 0),
+    ((64, 64), K.FORM_WIDE, 1),
+    ((256, 16), K.FORM_WIDE, 1),
+    ((192, 6), K.FORM_WIDE, 0),
 ]
 
 
@@ -91,7 +96,7 @@ CASES = [
 @pytest.mark.parametrize("sched,h", [([[0, 1]], 0.0), ([[2, 1]], 0.0), ([[0, 2], [2, 1]], 0.25)])
 def test_run_is_bit_exact_against_replay(shape, form, layout, sched, h):
     dev = _dev()
-    R = 4 if form == K.FORM_BATCHED else 2
+    R = 2 if form == K.FORM_PLAIN else 4
     seed = sum(shape) + len(sched)
     chains, keys, sids, sweeps0 = _chains(R, seed, dev)
     n = int(np.prod(shape))
@@ -119,20 +124,21 @@ def test_run_is_bit_exact_against_replay(shape, form, layout, sched, h):
     assert np.array_equal(chains.sweeps_h, sweeps0 + np.uint64(per))
 
 
-def test_plain_and_batched_forms_agree():
+def test_plain_batched_and_wide_forms_agree():
     dev = _dev()
     shape, R = (64, 32), 2
     out = []
-    for form in (K.FORM_PLAIN, K.FORM_BATCHED):
+    for form in (K.FORM_PLAIN, K.FORM_BATCHED, K.FORM_WIDE):
         chains, *_ = _chains(R, 21, dev)
         s = torch.empty((R, 64 * 32), dtype=torch.int8, device=dev)
         K.init_spins(s, shape, chains, "random")
         e, m = K.measure(s, shape, R)
         series = K.run(s, shape, [0.44, 0.47], 0.0, chains, np.array([[0, 1]]), 20, e, m, form=form)
         out.append((s.cpu().numpy(), [t.cpu().numpy() for t in series]))
-    assert np.array_equal(out[0][0], out[1][0])
-    for a, b in zip(out[0][1], out[1][1]):
-        assert np.array_equal(a, b)
+    for other in out[1:]:
+        assert np.array_equal(out[0][0], other[0])
+        for a, b in zip(out[0][1], other[1]):
+            assert np.array_equal(a, b)
 
 
 def test_full_size_batch_properties():
