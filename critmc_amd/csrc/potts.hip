@@ -15,9 +15,10 @@
 // uniforms for the same site and sweep, so they give identical chains.
 //
 // Uniforms: the colour-c sites are enumerated k = 0 .. N/2-1 (row-major, see
-// cb_geom.hpp); site k uses words 2(k&1) and 2(k&1)+1 of the Philox block
-// k >> 1 (tag TAG_POTTS): the first is the proposal (randbelow(n) =
-// floor(x n / 2^32)) or the heat-bath u, the second the Metropolis u.  The
+// cb_geom.hpp), tag TAG_POTTS.  Metropolis: site k uses words 2(k&1) and
+// 2(k&1)+1 of the Philox block k >> 1, the proposal (randbelow(n) =
+// floor(x n / 2^32)) and the acceptance u.  Heat bath: site k uses word k&3 of
+// block k >> 2.  A lane updates a quad of colour sites 4u .. 4u+3.  The
 // heat-bath arithmetic is FP64 in the reference's order, compiled without
 // contraction, so it is bit-reproducible on the host.
 #include <algorithm>
@@ -33,7 +34,7 @@ namespace cm {
 
 struct PottsTab {
     uint64_t T[8];  // Metropolis: accept iff x < T[delta] (delta >= 1); T = ceil(exp(-beta d) 2^32)
-    double w[8];    // heat bath: exp(beta n)
+    double w[16];   // heat bath: exp(beta n) for n <= z; 0 elsewhere (w[15]: padding states)
 };
 
 constexpr int POTTS_QMAX = 127;
@@ -52,17 +53,21 @@ static bool potts_prefer_wide(int64_t N, int n_rep)
 // table staged in LDS.
 //
 // QB > 0 (q <= QB <= 16): the neighbour states are folded into a histogram of
-// 4-bit fields, h = sum_j 16^{s_j}, so n_k = (h >> 4k) & 15 is one bit-field
-// extract.  The heat bath adds w[n_k] for k = 0 .. q-1 in that order, as the
-// reference does, keeping every prefix sum; since the prefix sums never
-// decrease, the first k with r < cum_k equals the number of k <= q-2 with
-// !(r < cum_k), and the default q-1 falls out of the same count.
+// 4-bit fields, h = pad + sum_j 16^{s_j}, so n_k = (h >> 4k) & 15 is one
+// bit-field extract; pad sets the fields of the states q .. QB-1 to 15, whose
+// weight w[15] is 0, so all QB terms can be added without a branch.  The heat
+// bath adds w[n_k] for k = 0 .. q-1 in that order, as the reference does,
+// keeping every prefix sum; since the prefix sums never decrease, the first k
+// with r < cum_k equals the number of k <= q-2 with !(r < cum_k), and the
+// default q-1 falls out of the same count.  The padded terms do not change
+// the count: their prefix sums equal the total, and r = u * total < total.
 // QB == 0: any q <= 127, counts recomputed per state.
 template <int Z, int QB>
 __device__ __forceinline__ int potts_update(int8_t* __restrict__ S, uint32_t i,
-                                            const uint32_t (&nbi)[Z], int q, int code, uint32_t xa,
-                                            uint32_t xb, const PottsTab* __restrict__ tl, int& de,
-                                            int& acc, int& old_out)
+                                            const uint32_t (&nbi)[Z], int q, uint64_t pad,
+                                            int code, uint32_t xa, uint32_t xb,
+                                            const PottsTab* __restrict__ tl, int& de, int& acc,
+                                            int& old_out)
 {
     const int old = S[i];
     old_out = old;
@@ -71,13 +76,13 @@ __device__ __forceinline__ int potts_update(int8_t* __restrict__ S, uint32_t i,
     for (int j = 0; j < Z; ++j) nb[j] = S[nbi[j]];
     int nw, c_old, c_new;
     if constexpr (QB > 0) {
-        uint64_t h = 0;
+        uint64_t h = pad;
 #pragma unroll
         for (int j = 0; j < Z; ++j) h += 1ull << (4 * nb[j]);
+        c_old = (int)(h >> (4 * old)) & 15;
         if (code == CODE_METROPOLIS) {
             nw = (int)(((uint64_t)xa * (uint32_t)(q - 1)) >> 32);
             if (nw >= old) ++nw;
-            c_old = (int)(h >> (4 * old)) & 15;
             c_new = (int)(h >> (4 * nw)) & 15;
             const int delta = c_old - c_new;
             if (!(delta <= 0 || (uint64_t)xb < tl->T[delta])) return old;
@@ -86,16 +91,14 @@ __device__ __forceinline__ int potts_update(int8_t* __restrict__ S, uint32_t i,
             double total = 0.0;
 #pragma unroll
             for (int k = 0; k < QB; ++k) {
-                if (k < q) total = __dadd_rn(total, tl->w[(h >> (4 * k)) & 15]);
+                total = __dadd_rn(total, tl->w[(uint32_t)(h >> (4 * k)) & 15u]);
                 cum[k] = total;
             }
             const double r = __dmul_rn((double)xa * 0x1p-32, total);
             nw = 0;
 #pragma unroll
-            for (int k = 0; k < QB - 1; ++k)
-                if (k < q - 1) nw += !(r < cum[k]);
+            for (int k = 0; k < QB - 1; ++k) nw += !(r < cum[k]);
             if (nw == old) return old;
-            c_old = (int)(h >> (4 * old)) & 15;
             c_new = (int)(h >> (4 * nw)) & 15;
         }
     } else {
@@ -148,27 +151,58 @@ __device__ __forceinline__ int potts_update(int8_t* __restrict__ S, uint32_t i,
     return nw;
 }
 
-// Occupation changes of one wave: for q <= QB every lane reports up to two
-// (old, new) pairs (-1 when unchanged) and the wave counts each state with
-// ballots into the wave-uniform wd[]; must be reached by the whole wave.
+// Fields q .. QB-1 of the padded histogram set to 15.
 template <int QB>
-__device__ __forceinline__ void wave_count_changes(int (&wd)[QB], int q, int o0, int n0, int o1,
-                                                   int n1)
+__device__ __forceinline__ uint64_t potts_pad(int q)
 {
-#pragma unroll
-    for (int k = 0; k < QB; ++k) {
-        if (k < q) {
-            wd[k] += __popcll(__ballot(n0 == k)) + __popcll(__ballot(n1 == k)) -
-                     __popcll(__ballot(o0 == k)) - __popcll(__ballot(o1 == k));
-        }
-    }
+    uint64_t pad = 0;
+    for (int k = q; k < QB; ++k) pad |= 15ull << (4 * k);
+    return pad;
 }
 
-// Copy one replica's 128-byte table into LDS (lanes 0..31).
+// Copy one replica's table into LDS.
 __device__ __forceinline__ void stage_tab(PottsTab* __restrict__ dst, const PottsTab* __restrict__ src)
 {
     if (threadIdx.x < sizeof(PottsTab) / 4)
         ((uint32_t*)dst)[threadIdx.x] = ((const uint32_t*)src)[threadIdx.x];
+}
+
+// The quad of colour-c sites 4u .. 4u+3 (those below `half`): Metropolis draws
+// blocks 2u and 2u+1 (two words per site), the heat bath block u (one word
+// per site).  Occupation changes go to the LDS counts with atomics.
+template <int Z, int QB>
+__device__ __forceinline__ void potts_quad(int8_t* __restrict__ S, const CbGeo& g, uint32_t u,
+                                           int c, int q, uint64_t pad, int code, uint64_t sweep,
+                                           uint32_t zc, uint32_t k0, uint32_t k1, uint32_t sid,
+                                           const PottsTab* __restrict__ tl, int& de, int& acc,
+                                           int* __restrict__ counts)
+{
+    uint32_t xa[4], xb[4];
+    if (code == CODE_METROPOLIS) {
+        const u32x4 r0 = philox4x32_10(u32x4{2 * u, (uint32_t)sweep, zc, sid}, k0, k1);
+        const u32x4 r1 = philox4x32_10(u32x4{2 * u + 1, (uint32_t)sweep, zc, sid}, k0, k1);
+        xa[0] = r0.x; xb[0] = r0.y; xa[1] = r0.z; xb[1] = r0.w;
+        xa[2] = r1.x; xb[2] = r1.y; xa[3] = r1.z; xb[3] = r1.w;
+    } else {
+        const u32x4 r0 = philox4x32_10(u32x4{u, (uint32_t)sweep, zc, sid}, k0, k1);
+        xa[0] = r0.x; xa[1] = r0.y; xa[2] = r0.z; xa[3] = r0.w;
+        xb[0] = xb[1] = xb[2] = xb[3] = 0;
+    }
+    const uint32_t half = (uint32_t)g.half;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t k = 4 * u + j;
+        if (k < half) {
+            uint32_t nbi[Z];
+            const uint32_t i = cb_site32<Z>(g, k, c, nbi);
+            int old;
+            const int nw = potts_update<Z, QB>(S, i, nbi, q, pad, code, xa[j], xb[j], tl, de, acc, old);
+            if (nw != old) {
+                atomicAdd(&counts[old], -1);
+                atomicAdd(&counts[nw], 1);
+            }
+        }
+    }
 }
 
 template <int Z, int QB>
@@ -185,9 +219,8 @@ __global__ __launch_bounds__(256) void potts_batched(
     int* red = counts + 128;           // [16][2]
     int8_t* lstate = (int8_t*)(red + 32);
     const int nwaves = (blockDim.x + 63) >> 6;
-    const int lane = threadIdx.x & 63;
-    const uint32_t npairs = (uint32_t)((g.half + 1) >> 1);
-    const uint32_t half = (uint32_t)g.half;
+    const uint32_t nquads = (uint32_t)((g.half + 3) >> 2);
+    const uint64_t pad = potts_pad<QB>(q);
 
     for (int rep = blockIdx.x; rep < R; rep += gridDim.x) {
         int8_t* gs = spins + (int64_t)rep * g.N;
@@ -210,49 +243,14 @@ __global__ __launch_bounds__(256) void potts_batched(
         const int64_t n_rep_meas = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
         for (int64_t meas = 0; meas < n_rep_meas; ++meas) {
             int de = 0, acc = 0;
-            int wd[QB > 0 ? QB : 1] = {};
             for (int row = 0; row < sc.n; ++row) {
                 const int code = sc.code[row];
                 for (int rp = 0; rp < sc.count[row]; ++rp) {
                     for (int c = 0; c < 2; ++c) {
                         const uint32_t zc = ctr_z(sweep, (uint32_t)c, TAG_POTTS);
-                        // wave-uniform trip count: ballots below see the whole wave
-                        for (uint32_t p0 = threadIdx.x - lane; p0 < npairs; p0 += blockDim.x) {
-                            const uint32_t pi = p0 + lane;
-                            int o0 = -1, n0 = -1, o1 = -1, n1 = -1;
-                            if (pi < npairs) {
-                                const u32x4 rn = philox4x32_10(
-                                    u32x4{pi, (uint32_t)sweep, zc, sid}, k0, k1);
-                                uint32_t nbi[Z];
-                                int old;
-                                uint32_t i = cb_site32<Z>(g, 2 * pi, c, nbi);
-                                int nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.x, rn.y, tl, de, acc, old);
-                                if (nw != old) {
-                                    o0 = old;
-                                    n0 = nw;
-                                }
-                                if (2 * pi + 1 < half) {
-                                    i = cb_site32<Z>(g, 2 * pi + 1, c, nbi);
-                                    nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.z, rn.w, tl, de, acc, old);
-                                    if (nw != old) {
-                                        o1 = old;
-                                        n1 = nw;
-                                    }
-                                }
-                            }
-                            if constexpr (QB > 0) {
-                                wave_count_changes<QB>(wd, q, o0, n0, o1, n1);
-                            } else {
-                                if (o0 >= 0) {
-                                    atomicAdd(&counts[o0], -1);
-                                    atomicAdd(&counts[n0], 1);
-                                }
-                                if (o1 >= 0) {
-                                    atomicAdd(&counts[o1], -1);
-                                    atomicAdd(&counts[n1], 1);
-                                }
-                            }
-                        }
+                        for (uint32_t u = threadIdx.x; u < nquads; u += blockDim.x)
+                            potts_quad<Z, QB>(S, g, u, c, q, pad, code, sweep, zc, k0, k1, sid, tl,
+                                              de, acc, counts);
                         __syncthreads();
                     }
                     ++sweep;
@@ -260,14 +258,9 @@ __global__ __launch_bounds__(256) void potts_batched(
             }
             de = wave_sum(de);
             acc = wave_sum(acc);
-            if (lane == 0) {
+            if ((threadIdx.x & 63) == 0) {
                 red[2 * (threadIdx.x >> 6)] = de;
                 red[2 * (threadIdx.x >> 6) + 1] = acc;
-                if constexpr (QB > 0) {
-#pragma unroll
-                    for (int k = 0; k < QB; ++k)
-                        if (k < q && wd[k]) atomicAdd(&counts[k], wd[k]);
-                }
             }
             __syncthreads();
             int tde = 0, tac = 0;
@@ -303,7 +296,7 @@ __global__ __launch_bounds__(256) void potts_batched(
 }
 
 // Wide form for lattices too large for LDS: one launch per half sweep over
-// (site-pair chunks x replicas), states in global memory.  Energy, accepted and
+// (quad chunks x replicas), states in global memory.  Energy, accepted and
 // occupation deltas are reduced per workgroup and added with integer atomics,
 // so every sum is exact; potts_prefix turns them into running values.
 template <int Z, int QB>
@@ -327,48 +320,13 @@ __global__ __launch_bounds__(256) void potts_half_wide(
     int8_t* S = spins + (int64_t)rep * g.N;
     const uint64_t sweep = sweeps[rep] + sweep_add;
     const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
-    const uint32_t npairs = (uint32_t)((g.half + 1) >> 1);
-    const uint32_t pi = bx * blockDim.x + threadIdx.x;
+    const uint32_t nquads = (uint32_t)((g.half + 3) >> 2);
+    const uint32_t u = bx * blockDim.x + threadIdx.x;
     int de = 0, acc = 0;
-    int o0 = -1, n0 = -1, o1 = -1, n1 = -1;
-    if (pi < npairs) {
-        const u32x4 rn = philox4x32_10(
-            u32x4{pi, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_POTTS), sid}, k0, k1);
-        uint32_t nbi[Z];
-        int old;
-        uint32_t i = cb_site32<Z>(g, 2 * pi, colour, nbi);
-        int nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.x, rn.y, &tl, de, acc, old);
-        if (nw != old) {
-            o0 = old;
-            n0 = nw;
-        }
-        if (2 * pi + 1 < (uint32_t)g.half) {
-            i = cb_site32<Z>(g, 2 * pi + 1, colour, nbi);
-            nw = potts_update<Z, QB>(S, i, nbi, q, code, rn.z, rn.w, &tl, de, acc, old);
-            if (nw != old) {
-                o1 = old;
-                n1 = nw;
-            }
-        }
-    }
-    if constexpr (QB > 0) {
-        int wd[QB] = {};
-        wave_count_changes<QB>(wd, q, o0, n0, o1, n1);
-        if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-            for (int k = 0; k < QB; ++k)
-                if (k < q && wd[k]) atomicAdd(&counts[k], wd[k]);
-        }
-    } else {
-        if (o0 >= 0) {
-            atomicAdd(&counts[o0], -1);
-            atomicAdd(&counts[n0], 1);
-        }
-        if (o1 >= 0) {
-            atomicAdd(&counts[o1], -1);
-            atomicAdd(&counts[n1], 1);
-        }
-    }
+    if (u < nquads)
+        potts_quad<Z, QB>(S, g, u, colour, q, potts_pad<QB>(q), code, sweep,
+                          ctr_z(sweep, (uint32_t)colour, TAG_POTTS), k0, k1, sid, &tl, de, acc,
+                          counts);
     de = wave_sum(de);
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) {
@@ -508,14 +466,15 @@ static hipError_t launch_potts_q(int8_t* spins, const CbGeo& g, int q, int R, co
                                  int32_t* counts_out, const int64_t* n_meas_rep, int block,
                                  int grid, hipStream_t st)
 {
-    if (q <= 8)
-        return launch_potts<Z, 8>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
-                                  counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
-    if (q <= 16)
-        return launch_potts<Z, 16>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
-                                   counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
-    return launch_potts<Z, 0>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run,
-                              counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
+#define CM_POTTS_B(QB) \
+    launch_potts<Z, QB>(spins, g, q, R, tabs, keys, sids, sweeps, sc, n_meas, e_run, counts_run, \
+                        e_out, acc_out, counts_out, n_meas_rep, block, grid, st)
+    if (q <= 4) return CM_POTTS_B(4);
+    if (q <= 8) return CM_POTTS_B(8);
+    if (q <= 12) return CM_POTTS_B(12);
+    if (q <= 16) return CM_POTTS_B(16);
+    return CM_POTTS_B(0);
+#undef CM_POTTS_B
 }
 
 template <int Z>
@@ -525,18 +484,16 @@ static void launch_wide(const dim3& wgrid, hipStream_t st, int8_t* spins, const 
                         const int64_t* n_meas_rep, int32_t* e_out, int32_t* acc_out,
                         int32_t* counts_out)
 {
-    if (q <= 8)
-        hipLaunchKernelGGL((potts_half_wide<Z, 8>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
-                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
-                           counts_out);
-    else if (q <= 16)
-        hipLaunchKernelGGL((potts_half_wide<Z, 16>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
-                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
-                           counts_out);
-    else
-        hipLaunchKernelGGL((potts_half_wide<Z, 0>), wgrid, dim3(256), 0, st, spins, g, q, code, c,
-                           sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out,
-                           counts_out);
+#define CM_POTTS_W(QB) \
+    hipLaunchKernelGGL((potts_half_wide<Z, QB>), wgrid, dim3(256), 0, st, spins, g, q, code, c, \
+                       sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, \
+                       counts_out)
+    if (q <= 4) CM_POTTS_W(4);
+    else if (q <= 8) CM_POTTS_W(8);
+    else if (q <= 12) CM_POTTS_W(12);
+    else if (q <= 16) CM_POTTS_W(16);
+    else CM_POTTS_W(0);
+#undef CM_POTTS_W
 }
 
 }  // namespace cm
@@ -611,17 +568,15 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     std::vector<PottsTab> tabs(n_rep);
     for (int r = 0; r < n_rep; ++r) {
         CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
-        for (int d = 0; d < 8; ++d) {
-            tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
-            tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
-        }
+        for (int d = 0; d < 8; ++d) tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
+        for (int d = 0; d < 16; ++d) tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
     }
     hipStream_t st = (hipStream_t)stream;
     PottsTab* dtabs = (PottsTab*)workspace;
     CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * sizeof(PottsTab), st));
-    if (block <= 0) {  // one lane per site pair of a colour, in whole waves, at most 256
-        const int64_t pairs = (g.half + 1) / 2;
-        block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (pairs + 63) / 64 * 64));
+    if (block <= 0) {  // one lane per site quad of a colour, in whole waves, at most 256
+        const int64_t quads = (g.half + 3) / 4;
+        block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (quads + 63) / 64 * 64));
     }
     CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
     // grid < 0 asks for the wide form, grid > 0 for one workgroup per replica;
@@ -630,14 +585,14 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     if (wide) {
         // wide form: the whole GPU on each half sweep of every replica
         CM_REQUIRE(n_rep <= 65535, "n_rep out of range");
-        CM_REQUIRE((g.half + 511) / 512 * n_rep < (1ll << 32), "too many sites x replicas per launch");
+        CM_REQUIRE((g.half + 1023) / 1024 * n_rep < (1ll << 32), "too many sites x replicas per launch");
         const int64_t total = (int64_t)n_rep * n_meas;
         const int zb = (int)std::min<int64_t>((total * (q + 2) + 255) / 256, 8192);
         hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, e_out, total);
         hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, acc_out, total);
         hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, counts_out, total * q);
-        const int64_t npairs = (g.half + 1) / 2;
-        const dim3 wgrid((unsigned)((npairs + 255) / 256), n_rep);
+        const int64_t nquads = (g.half + 3) / 4;
+        const dim3 wgrid((unsigned)((nquads + 255) / 256), n_rep);
         uint64_t sw = 0;
         int64_t per_meas = 0;
         for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];

@@ -7,9 +7,10 @@
  *   heat bath  :74-103 (weight[n] = exp(beta n), :198), FP64 sums in k order
  *   energy / counts :27-41, run :170-235
  * Visiting order: colour 0 then colour 1; within a colour the sites are
- * k = r*M + m (row r = x1 + L1 x2, x0 = 2m + ((x1+x2+c)&1)).  Site k uses words
- * 2(k&1), 2(k&1)+1 of Philox block k>>1 (tag POTTS): proposal / heat-bath u,
- * then the Metropolis u.  randbelow(n) = floor(x n / 2^32).
+ * k = r*M + m (row r = x1 + L1 x2, x0 = 2m + ((x1+x2+c)&1)).  Philox tag
+ * POTTS.  Metropolis: site k uses words 2(k&1), 2(k&1)+1 of block k>>1, the
+ * proposal and then the acceptance u.  Heat bath: site k uses word k&3 of block
+ * k>>2.  randbelow(n) = floor(x n / 2^32).
  */
 #include <math.h>
 #include <stdint.h>
@@ -91,10 +92,19 @@ int orc_potts_run_aux(int8_t *s, int ndim, const int32_t *dims, int q, double be
                 for (int c = 0; c < 2; ++c) {
                     uint32_t blk[4];
                     for (int64_t k = 0; k < g.half; ++k) {
-                        if ((k & 1) == 0)
-                            orc_draw_block(k0, k1, stream, *sweep, c, ORC_TAG_POTTS,
-                                           (uint32_t)(k >> 1), blk);
-                        uint32_t xa = blk[2 * (k & 1)], xb = blk[2 * (k & 1) + 1];
+                        uint32_t xa, xb = 0;
+                        if (code == 0) {
+                            if ((k & 1) == 0)
+                                orc_draw_block(k0, k1, stream, *sweep, c, ORC_TAG_POTTS,
+                                               (uint32_t)(k >> 1), blk);
+                            xa = blk[2 * (k & 1)];
+                            xb = blk[2 * (k & 1) + 1];
+                        } else {
+                            if ((k & 3) == 0)
+                                orc_draw_block(k0, k1, stream, *sweep, c, ORC_TAG_POTTS,
+                                               (uint32_t)(k >> 2), blk);
+                            xa = blk[k & 3];
+                        }
                         int64_t i = orc_cb_site(&g, k, c, nbi);
                         int old = s[i];
                         int nb[6];
