@@ -237,6 +237,52 @@ __device__ void store_replica(void* __restrict__ spins, int rep, int L0, int L1,
     }
 }
 
+// Words of one colour of an LDS-resident replica: own word idx and its four
+// neighbour words (same position in the other colour, shifted by one bit
+// with the carry from the adjacent word, rows above and below).
+__device__ __forceinline__ void batched_word(const uint32_t* __restrict__ Sc,
+                                             const uint32_t* __restrict__ So, int idx, int G, int M,
+                                             int L1, int c, uint32_t valid, uint32_t& own,
+                                             uint32_t (&nb)[4])
+{
+    const int y = idx / G, w = idx - (idx / G) * G;
+    const int p = (y + c) & 1;
+    const int yp = y + 1 == L1 ? 0 : y + 1;
+    const int ym = y == 0 ? L1 - 1 : y - 1;
+    own = Sc[idx];
+    const uint32_t same = So[idx];
+    uint32_t sh;
+    if (M >= 32) {
+        if (p == 0) {
+            const uint32_t prev = So[y * G + (w == 0 ? G - 1 : w - 1)];
+            sh = (same << 1) | (prev >> 31);
+        } else {
+            const uint32_t next = So[y * G + (w == G - 1 ? 0 : w + 1)];
+            sh = (same >> 1) | (next << 31);
+        }
+    } else if (p == 0) {
+        sh = ((same << 1) | (same >> (M - 1))) & valid;
+    } else {
+        sh = ((same >> 1) | ((same & 1u) << (M - 1))) & valid;
+    }
+    nb[0] = same;
+    nb[1] = sh;
+    nb[2] = So[yp * G + w];
+    nb[3] = So[ym * G + w];
+}
+
+// Philox blocks a word may draw before it leaves the wave's lockstep.  A wave
+// keeps drawing until its last lane is decided, about 3.3 blocks for 64 words
+// of near-critical Metropolis, while a word needs 1.9 on average.  Words
+// still undecided after this budget go to an LDS queue that the whole
+// workgroup then drains densely, continuing the same comparison, so every
+// decision is unchanged.
+#ifndef CM_ISING_DRAW_BUDGET
+#define CM_ISING_DRAW_BUDGET 2
+#endif
+constexpr uint32_t ISING_DRAW_BUDGET = CM_ISING_DRAW_BUDGET;
+constexpr int ISING_QCAP = 512;
+
 template <int LAYOUT, int NCMAX>
 __global__ __launch_bounds__(256) void ising2d_batched(
     void* __restrict__ spins, int R, int L0, int L1, const uint32_t* __restrict__ tabs,
@@ -249,9 +295,13 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     const int M = L0 >> 1;
     const int G = (M + 31) >> 5;
     const int W = L1 * G;
-    int* red = (int*)(lds + 2 * W);  // [waves][3]
+    int* red = (int*)(lds + 2 * W);         // [16][3]
+    uint32_t* queue = lds + 2 * W + 48;     // [ISING_QCAP][3]: word, pending, below
+    int* qcount = (int*)(queue + 3 * ISING_QCAP);  // [2], alternating half sweeps
     const int nwaves = (blockDim.x + 63) >> 6;
     const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
+    if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
+    uint32_t hs = 0;
 
     for (int rep = blockIdx.x; rep < R; rep += gridDim.x) {
         load_replica<LAYOUT>(spins, rep, L0, L1, G, lds);
@@ -267,34 +317,44 @@ __global__ __launch_bounds__(256) void ising2d_batched(
             for (int row = 0; row < sched.n; ++row) {
                 const int code = sched.code[row];
                 for (int rp = 0; rp < sched.count[row]; ++rp) {
-                    for (int c = 0; c < 2; ++c) {
+                    for (int c = 0; c < 2; ++c, ++hs) {
                         uint32_t* Sc = lds + c * W;
                         const uint32_t* So = lds + (1 - c) * W;
+                        int* qc = qcount + (hs & 1);
                         for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
-                            const int y = idx / G, w = idx - (idx / G) * G;
-                            const int p = (y + c) & 1;
-                            const int yp = y + 1 == L1 ? 0 : y + 1;
-                            const int ym = y == 0 ? L1 - 1 : y - 1;
-                            const uint32_t own = Sc[idx];
-                            const uint32_t same = So[idx];
-                            uint32_t sh;
-                            if (M >= 32) {
-                                if (p == 0) {
-                                    const uint32_t prev = So[y * G + (w == 0 ? G - 1 : w - 1)];
-                                    sh = (same << 1) | (prev >> 31);
-                                } else {
-                                    const uint32_t next = So[y * G + (w == G - 1 ? 0 : w + 1)];
-                                    sh = (same >> 1) | (next << 31);
+                            uint32_t own, nb[4];
+                            batched_word(Sc, So, idx, G, M, L1, c, valid, own, nb);
+                            WordDecision<NCMAX> wd;
+                            word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
+                            word_draw<NCMAX>(wd, 0, ISING_DRAW_BUDGET, k0, k1, sid, sweep,
+                                             (uint32_t)c, (uint32_t)idx);
+                            if (wd.pending) {
+                                const int slot = atomicAdd(qc, 1);
+                                if (slot < ISING_QCAP) {
+                                    queue[3 * slot] = (uint32_t)idx;
+                                    queue[3 * slot + 1] = wd.pending;
+                                    queue[3 * slot + 2] = wd.below;
+                                    continue;
                                 }
-                            } else if (p == 0) {
-                                sh = ((same << 1) | (same >> (M - 1))) & valid;
-                            } else {
-                                sh = ((same >> 1) | ((same & 1u) << (M - 1))) & valid;
+                                word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
+                                                 (uint32_t)c, (uint32_t)idx);
                             }
-                            const uint32_t nb[4] = {same, sh, So[yp * G + w], So[ym * G + w]};
-                            Sc[idx] = update_word<4, NCMAX>(code, own, nb, valid, tab, k0, k1,
-                                                            sid, sweep, (uint32_t)c,
-                                                            (uint32_t)idx, d);
+                            Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
+                        }
+                        __syncthreads();
+                        const int nq = min(*qc, ISING_QCAP);
+                        if (threadIdx.x == 0) qcount[(hs + 1) & 1] = 0;
+                        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+                            const int idx = (int)queue[3 * t];
+                            uint32_t own, nb[4];
+                            batched_word(Sc, So, idx, G, M, L1, c, valid, own, nb);
+                            WordDecision<NCMAX> wd;
+                            word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
+                            wd.pending = queue[3 * t + 1];
+                            wd.below = queue[3 * t + 2];
+                            word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
+                                             (uint32_t)c, (uint32_t)idx);
+                            Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
                         }
                         __syncthreads();
                     }
@@ -667,7 +727,7 @@ static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_
                                  int32_t* e_out, int32_t* m_out, int32_t* acc_out,
                                  const int64_t* n_meas_rep, int block, int grid, hipStream_t st)
 {
-    const size_t lds = (size_t)2 * g.L1 * g.G * 4 + 3 * 16 * 4;
+    const size_t lds = (size_t)2 * g.L1 * g.G * 4 + 3 * 16 * 4 + (3 * ISING_QCAP + 2) * 4;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)ising2d_batched<LAYOUT, NC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -63,20 +63,38 @@ __host__ __device__ __forceinline__ int popc(uint32_t v)
 #endif
 }
 
-// Lanes whose uniform is below their class threshold.  `tab` is one 32-word
-// table: [0] = class count, [1 + k] = count value | spin selector << 8
-// (0 any, 1 spin -1, 2 spin +1), [TAB_T + k] = threshold.  Lanes in no class
-// accept unconditionally (p >= 1).
+// Bit-serial decision state of one word: lane class masks and thresholds, the
+// lanes still undecided (pending) and those already known to be below their
+// threshold (below).  Lanes in no class accept unconditionally (p >= 1).
 template <int NCMAX>
-__host__ __device__ __forceinline__ uint32_t bitserial_below(uint32_t c0, uint32_t c1, uint32_t c2,
-                                                    uint32_t own, uint32_t valid,
-                                                    const uint32_t* __restrict__ tab, uint32_t k0,
-                                                    uint32_t k1, uint32_t sid, uint64_t sweep,
-                                                    uint32_t colour, uint32_t group)
-{
-    const uint32_t ncls = tab[0];
+struct WordDecision {
+    uint32_t c0, c1, c2;     // bit-sliced neighbour count
     uint32_t mask[NCMAX];
     uint32_t thr[NCMAX];
+    uint32_t pending, below;
+};
+
+// Count neighbours (aligned for Metropolis, up for the heat bath) and sort the
+// lanes into the table's classes.  `tabs` is the replica's 64-word table:
+// [0, 32) Metropolis, [32, 64) heat bath; one 32-word table holds [0] = class
+// count, [1 + k] = count value | spin selector << 8 (0 any, 1 spin -1,
+// 2 spin +1), [TAB_T + k] = threshold.
+template <int Z, int NCMAX>
+__host__ __device__ __forceinline__ void word_classify(int code, uint32_t own, const uint32_t (&nb)[Z],
+                                                       uint32_t valid, const uint32_t* __restrict__ tabs,
+                                                       WordDecision<NCMAX>& w)
+{
+    const uint32_t* tab = tabs;
+    if (code == CODE_METROPOLIS) {
+        uint32_t in[Z];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) in[k] = ~(own ^ nb[k]);  // aligned neighbours
+        count_bits<Z>(in, w.c0, w.c1, w.c2);
+    } else {
+        count_bits<Z>(nb, w.c0, w.c1, w.c2);  // neighbours up
+        tab = tabs + 32;
+    }
+    const uint32_t ncls = tab[0];
     uint32_t pending = 0;
 #pragma unroll
     for (int k = 0; k < NCMAX; ++k) {
@@ -84,19 +102,31 @@ __host__ __device__ __forceinline__ uint32_t bitserial_below(uint32_t c0, uint32
         if ((uint32_t)k < ncls) {
             const uint32_t d = tab[1 + k];
             const uint32_t v = d & 0xffu, sel = d >> 8;
-            mk = (c0 ^ ((v & 1u) ? 0u : ~0u)) & (c1 ^ ((v & 2u) ? 0u : ~0u)) &
-                 (c2 ^ ((v & 4u) ? 0u : ~0u));
+            mk = (w.c0 ^ ((v & 1u) ? 0u : ~0u)) & (w.c1 ^ ((v & 2u) ? 0u : ~0u)) &
+                 (w.c2 ^ ((v & 4u) ? 0u : ~0u));
             if (sel == 1u) mk &= ~own;
             else if (sel == 2u) mk &= own;
             mk &= valid;
         }
-        mask[k] = mk;
-        thr[k] = tab[TAB_T + k];
+        w.mask[k] = mk;
+        w.thr[k] = tab[TAB_T + k];
         pending |= mk;
     }
-    uint32_t below = valid & ~pending;
+    w.pending = pending;
+    w.below = valid & ~pending;
+}
+
+// Continue the comparison with Philox blocks q_begin .. q_end-1 of the word's
+// group (block 8 group + q gives planes 4q .. 4q+3, most significant first);
+// stops early once no lane is pending, which cannot change a decision.
+template <int NCMAX>
+__host__ __device__ __forceinline__ void word_draw(WordDecision<NCMAX>& w, uint32_t q_begin,
+                                                   uint32_t q_end, uint32_t k0, uint32_t k1,
+                                                   uint32_t sid, uint64_t sweep, uint32_t colour,
+                                                   uint32_t group)
+{
     const uint32_t zc = ctr_z(sweep, colour, TAG_ISING);
-    for (uint32_t q = 0; q < 8u && pending != 0u; ++q) {
+    for (uint32_t q = q_begin; q < q_end && w.pending != 0u; ++q) {
         const u32x4 r = philox4x32_10(u32x4{8u * group + q, (uint32_t)sweep, zc, sid}, k0, k1);
         const uint32_t planes[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
@@ -104,16 +134,41 @@ __host__ __device__ __forceinline__ uint32_t bitserial_below(uint32_t c0, uint32
             const uint32_t shift = 31u - (4u * q + (uint32_t)t);
             uint32_t tb = 0;
 #pragma unroll
-            for (int k = 0; k < NCMAX; ++k) tb |= mask[k] & (0u - ((thr[k] >> shift) & 1u));
-            const uint32_t diff = pending & (planes[t] ^ tb);
-            below |= diff & tb;
-            pending &= ~diff;
+            for (int k = 0; k < NCMAX; ++k) tb |= w.mask[k] & (0u - ((w.thr[k] >> shift) & 1u));
+            const uint32_t diff = w.pending & (planes[t] ^ tb);
+            w.below |= diff & tb;
+            w.pending &= ~diff;
         }
     }
-    return below;
 }
 
-// One word of 32 same-colour sites; returns the new spins and accumulates deltas.
+// Apply a finished decision: returns the new word and accumulates deltas.
+template <int Z, int NCMAX>
+__host__ __device__ __forceinline__ uint32_t word_apply(int code, uint32_t own, uint32_t valid,
+                                                        const WordDecision<NCMAX>& w, Delta& d)
+{
+    if (code == CODE_METROPOLIS) {
+        const uint32_t F = w.below & valid;
+        const int nf = popc(F);
+        const int nal = popc(F & w.c0) + 2 * popc(F & w.c1) + 4 * popc(F & w.c2);
+        d.de += 4 * nal - 2 * Z * nf;          // sum of 2a, a = 2 n_aligned - z
+        d.dm -= 2 * (2 * popc(F & own) - nf);  // sum of -2 s_old
+        d.acc += nf;
+        return own ^ F;
+    }
+    const uint32_t X = (w.below ^ own) & valid;
+    const uint32_t Xu = X & own, Xd = X & ~own;
+    const int fu = 2 * (popc(Xu & w.c0) + 2 * popc(Xu & w.c1) + 4 * popc(Xu & w.c2)) - Z * popc(Xu);
+    const int fd = 2 * (popc(Xd & w.c0) + 2 * popc(Xd & w.c1) + 4 * popc(Xd & w.c2)) - Z * popc(Xd);
+    d.de += 2 * (fu - fd);                 // sum of 2 s_old f
+    d.dm += 2 * (popc(Xd) - popc(Xu));     // sum of 2 s_new
+    d.acc += popc(X);
+    return own ^ X;
+}
+
+// One word of 32 same-colour sites; returns the new spins and accumulates
+// deltas.  "u < p" for the site at bit b is "x < T" with x the 32-bit uniform
+// whose bit 31-j is bit b of plane j; the comparison runs plane by plane.
 template <int Z, int NCMAX>
 __host__ __device__ __forceinline__ uint32_t update_word(int code, uint32_t own, const uint32_t (&nb)[Z],
                                                 uint32_t valid, const uint32_t* __restrict__ tabs,
@@ -121,33 +176,10 @@ __host__ __device__ __forceinline__ uint32_t update_word(int code, uint32_t own,
                                                 uint64_t sweep, uint32_t colour, uint32_t group,
                                                 Delta& d)
 {
-    uint32_t in[Z];
-    uint32_t c0, c1, c2;
-    if (code == CODE_METROPOLIS) {
-#pragma unroll
-        for (int k = 0; k < Z; ++k) in[k] = ~(own ^ nb[k]);  // aligned neighbours
-        count_bits<Z>(in, c0, c1, c2);
-        const uint32_t F = bitserial_below<NCMAX>(c0, c1, c2, own, valid, tabs, k0, k1, sid,
-                                                  sweep, colour, group) & valid;
-        const int nf = popc(F);
-        const int nal = popc(F & c0) + 2 * popc(F & c1) + 4 * popc(F & c2);
-        d.de += 4 * nal - 2 * Z * nf;          // sum of 2a, a = 2 n_aligned - z
-        d.dm -= 2 * (2 * popc(F & own) - nf);  // sum of -2 s_old
-        d.acc += nf;
-        return own ^ F;
-    } else {
-        count_bits<Z>(nb, c0, c1, c2);  // neighbours up
-        const uint32_t up = bitserial_below<NCMAX>(c0, c1, c2, own, valid, tabs + 32, k0, k1,
-                                                   sid, sweep, colour, group);
-        const uint32_t X = (up ^ own) & valid;
-        const uint32_t Xu = X & own, Xd = X & ~own;
-        const int fu = 2 * (popc(Xu & c0) + 2 * popc(Xu & c1) + 4 * popc(Xu & c2)) - Z * popc(Xu);
-        const int fd = 2 * (popc(Xd & c0) + 2 * popc(Xd & c1) + 4 * popc(Xd & c2)) - Z * popc(Xd);
-        d.de += 2 * (fu - fd);                 // sum of 2 s_old f
-        d.dm += 2 * (popc(Xd) - popc(Xu));     // sum of 2 s_new
-        d.acc += popc(X);
-        return own ^ X;
-    }
+    WordDecision<NCMAX> w;
+    word_classify<Z, NCMAX>(code, own, nb, valid, tabs, w);
+    word_draw<NCMAX>(w, 0, 8, k0, k1, sid, sweep, colour, group);
+    return word_apply<Z, NCMAX>(code, own, valid, w, d);
 }
 
 }  // namespace cm
