@@ -76,6 +76,10 @@ struct EaArgs {
     int32_t* e_series;      // [S][2][m][n_series] (temperature-indexed) or null
     double* e_series_f;     // the same for Gaussian couplings
     int32_t* q_series;      // [S][m][n_series] or null
+    // multi-spin +-J path (m <= 32): one bit per configuration
+    uint32_t* msw;          // [sum_s 2 N_s] words: sample s, set r, site i at ms_off[s] + r N_s + i
+    uint8_t* jneg;          // [sum_s N_s]: bit j set where the coupling to neighbour j is -1
+    const int64_t* ms_off;  // [S] word offsets (the jneg offset is ms_off[s] / 2)
     int S, m, d, code, sweeps_per_step, n_bins;
     int64_t n_series;
 };
@@ -222,6 +226,206 @@ __global__ __launch_bounds__(256) void ea_sweep(EaArgs a)
     }
     if (LDS)
         for (int64_t t = threadIdx.x; t < N; t += blockDim.x) gs[t] = lds8[t];
+}
+
+// ---------------------------------------------------------------------------
+// Multi-spin +-J path (m <= 32 temperatures).  Word i of set r of sample s
+// holds bit c = spin of configuration c (1 = +1) at site i, for every
+// configuration of the set, so one 32-bit operation updates a site of all of
+// them.  Configurations never change lanes: a swap changes only which
+// temperature a configuration sits at, and each lane reads its own
+// temperature's thresholds.  Colour site k draws the set's 32 bit planes from
+// words 0..3 of Philox blocks 8k .. 8k+7 at (the set's sweep counter, colour,
+// TAG_EA, stream r); configuration c's uniform is x = sum_t bit_c(plane_t)
+// 2^(31-t), compared plane by plane with its threshold as in the Ising
+// kernels.
+
+// Pack int8 configurations into words; record the negative couplings of each
+// site's 2d neighbours (column order +x, -x, +y, -y, +z, -z).
+template <int D>
+__global__ void ea_ms_pack(EaArgs a)
+{
+    const int s = blockIdx.y;
+    const int L = a.L[s];
+    const uint32_t N = (uint32_t)L * (D >= 2 ? L : 1) * (D >= 3 ? L : 1);
+    const int m = a.m;
+    const int8_t* sp = a.spins + a.spin_off[s];
+    uint32_t* w = a.msw + a.ms_off[s];
+    uint8_t* jn = a.jneg + a.ms_off[s] / 2;
+    const int8_t* J = a.J + a.j_off[s];
+    const FastDiv dl = make_fastdiv((uint32_t)L);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        for (int r = 0; r < 2; ++r) {
+            uint32_t v = 0;
+            for (int c = 0; c < m; ++c) v |= (sp[((int64_t)r * m + c) * N + i] > 0 ? 1u : 0u) << c;
+            w[(int64_t)r * N + i] = v;
+        }
+        int x[3];
+        ea_coords<D>(i, (uint32_t)L, dl, x);
+        uint32_t bits = 0;
+#pragma unroll
+        for (int ax = 0; ax < D; ++ax) {
+            const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? (uint32_t)L : (uint32_t)L * L);
+            const uint32_t back = x[ax] == 0 ? i + (uint32_t)(L - 1) * stride : i - stride;
+            bits |= (J[ax * N + i] < 0 ? 1u : 0u) << (2 * ax);
+            bits |= (J[ax * N + back] < 0 ? 1u : 0u) << (2 * ax + 1);
+        }
+        jn[i] = (uint8_t)bits;
+    }
+}
+
+__global__ void ea_ms_unpack(EaArgs a)
+{
+    const int s = blockIdx.y;
+    const int L = a.L[s];
+    const uint32_t N = (uint32_t)L * (a.d >= 2 ? L : 1) * (a.d >= 3 ? L : 1);
+    const int m = a.m;
+    int8_t* sp = a.spins + a.spin_off[s];
+    const uint32_t* w = a.msw + a.ms_off[s];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t v = w[(int64_t)r * N + i];
+            for (int c = 0; c < m; ++c) sp[((int64_t)r * m + c) * N + i] = ((v >> c) & 1u) ? 1 : -1;
+        }
+}
+
+// out = V[c] per lane, c = c0 + 2 c1 + 4 c2 the lane's bit-sliced count.
+__device__ __forceinline__ uint32_t ms_select(const uint32_t (&V)[8], uint32_t c0, uint32_t c1,
+                                              uint32_t c2)
+{
+    const uint32_t a0 = (V[1] & c0) | (V[0] & ~c0), a1 = (V[3] & c0) | (V[2] & ~c0);
+    const uint32_t a2 = (V[5] & c0) | (V[4] & ~c0), a3 = (V[7] & c0) | (V[6] & ~c0);
+    const uint32_t b0 = (a1 & c1) | (a0 & ~c1), b1 = (a3 & c1) | (a2 & ~c1);
+    return (b1 & c2) | (b0 & ~c2);
+}
+
+constexpr int EA_MS_CNT = 12;  // bit-sliced counter planes for the bond count of one lane
+
+// One workgroup per (sample, set): every sweep of the step for all of the set's
+// configurations, then each configuration's energy recounted from its bonds.
+// LDS: the set's words and neighbour-coupling bytes when they fit, and the
+// threshold planes P[t][class] (bit c = bit 31-t of configuration c's
+// threshold for that neighbour class) with the always-accept masks.
+template <int Z, bool LDS>
+__global__ __launch_bounds__(512) void ea_ms_sweep(EaArgs a)
+{
+    extern __shared__ uint32_t lds32[];
+    __shared__ __align__(16) uint32_t P[32][8];
+    __shared__ uint32_t ALW[8];
+    __shared__ int ucnt[32];
+    const int m = a.m;
+    const int s = blockIdx.x >> 1, r = blockIdx.x & 1;
+    const CbGeo g = ea_geo(Z / 2, a.L[s]);
+    const uint32_t N = (uint32_t)g.N;
+    uint32_t* gw = a.msw + a.ms_off[s] + (int64_t)r * N;
+    const uint8_t* gj = a.jneg + a.ms_off[s] / 2;
+    uint32_t* W = LDS ? lds32 : gw;
+    const uint8_t* JN = LDS ? (const uint8_t*)(lds32 + N) : gj;
+    if (LDS) {
+        for (uint32_t t = threadIdx.x; t < N; t += blockDim.x) lds32[t] = gw[t];
+        uint8_t* jl = (uint8_t*)(lds32 + N);
+        for (uint32_t t = threadIdx.x; t < N; t += blockDim.x) jl[t] = gj[t];
+    }
+    const int64_t cfg0 = ((int64_t)s * 2 + r) * m;
+    if (threadIdx.x < 32) ucnt[threadIdx.x] = 0;
+    if (threadIdx.x < 64) {
+        // lane c < m: configuration c's thresholds T[2 cl] for neighbour class cl
+        const int c = threadIdx.x;
+        const uint64_t* T = c < m ? a.thr + ((int64_t)s * m + a.kofc[cfg0 + c]) * EA_TAB : nullptr;
+        for (int cl = 0; cl < 8; ++cl) {
+            const uint64_t tv = (c < m && cl <= Z) ? T[2 * cl] : 0;
+            const uint64_t alw = __ballot(tv >= (1ull << 32));
+            if (threadIdx.x == 0) ALW[cl] = (uint32_t)alw;
+            for (int t = 0; t < 32; ++t) {
+                const uint64_t b = __ballot(tv < (1ull << 32) && ((tv >> (31 - t)) & 1ull));
+                if (threadIdx.x == 0) P[t][cl] = (uint32_t)b;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t lanes = m >= 32 ? ~0u : ((1u << m) - 1u);
+    uint32_t alw[8];
+#pragma unroll
+    for (int cl = 0; cl < 8; ++cl) alw[cl] = ALW[cl];
+    uint64_t sweep = a.sweeps[cfg0];
+    const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
+    const uint32_t half = (uint32_t)g.half;
+    for (int sp = 0; sp < a.sweeps_per_step; ++sp) {
+        for (int col = 0; col < 2; ++col) {
+            const uint32_t zc = ctr_z(sweep, (uint32_t)col, TAG_EA);
+            for (uint32_t kk = threadIdx.x; kk < half; kk += blockDim.x) {
+                uint32_t nb[Z];
+                const uint32_t i = cb_site32<Z>(g, kk, col, nb);
+                const uint32_t own = W[i];
+                const uint32_t jn = JN[i];
+                uint32_t in[Z];
+#pragma unroll
+                for (int j = 0; j < Z; ++j) {
+                    const uint32_t up = W[nb[j]] ^ (0u - ((jn >> j) & 1u));  // J_ij s_j = +1
+                    in[j] = a.code == CODE_METROPOLIS ? ~(own ^ up) : up;
+                }
+                uint32_t c0, c1, c2;
+                count_bits<Z>(in, c0, c1, c2);
+                uint32_t below = ms_select(alw, c0, c1, c2) & lanes;
+                uint32_t pending = lanes & ~below;
+                for (uint32_t q = 0; q < 8u && pending != 0u; ++q) {
+                    const u32x4 rn = philox4x32_10(u32x4{8u * kk + q, (uint32_t)sweep, zc, (uint32_t)r}, k0, k1);
+                    const uint32_t planes[4] = {rn.x, rn.y, rn.z, rn.w};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const uint4* pr = (const uint4*)P[4 * q + t];
+                        const uint4 lo = pr[0], hi = pr[1];
+                        const uint32_t V[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                        const uint32_t tb = ms_select(V, c0, c1, c2);
+                        const uint32_t diff = pending & (planes[t] ^ tb);
+                        below |= diff & tb;
+                        pending &= ~diff;
+                    }
+                }
+                W[i] = a.code == CODE_METROPOLIS ? own ^ below : below;
+            }
+            __syncthreads();
+        }
+        ++sweep;
+    }
+    // energies: E_c = 2 U_c - d N with U_c the configuration's unsatisfied bonds,
+    // counted per lane in bit-sliced counters and reduced with ballots
+    uint32_t cnt[EA_MS_CNT];
+#pragma unroll
+    for (int p = 0; p < EA_MS_CNT; ++p) cnt[p] = 0;
+    const uint32_t L32 = (uint32_t)g.L0;
+    const FastDiv dl = make_fastdiv(L32);
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
+        int x[3];
+        ea_coords<Z / 2>(i, L32, dl, x);
+        const uint32_t si = W[i], jn = JN[i];
+#pragma unroll
+        for (int ax = 0; ax < Z / 2; ++ax) {
+            const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L32 : L32 * L32);
+            const uint32_t j = (uint32_t)x[ax] + 1 == L32 ? i - (L32 - 1) * stride : i + stride;
+            uint32_t carry = si ^ W[j] ^ (0u - ((jn >> (2 * ax)) & 1u));  // J s_i s_j = -1
+            for (int p = 0; p < EA_MS_CNT && carry; ++p) {
+                const uint32_t t = cnt[p] & carry;
+                cnt[p] ^= carry;
+                carry = t;
+            }
+        }
+    }
+    // planes that can be non-zero: a lane counts at most d ceil(N / blockDim) bonds
+    const uint32_t most = (uint32_t)(Z / 2) * ((N + blockDim.x - 1) / blockDim.x);
+    const int planes = min(EA_MS_CNT, 32 - __clz(most));
+    for (int c = 0; c < m; ++c) {
+        int u = 0;
+        for (int p = 0; p < planes; ++p) u += __popcll(__ballot((cnt[p] >> c) & 1u)) << p;
+        if ((threadIdx.x & 63) == 0 && u) atomicAdd(&ucnt[c], u);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)m) {
+        a.energies[cfg0 + threadIdx.x] = 2 * ucnt[threadIdx.x] - (Z / 2) * (int)N;
+        a.sweeps[cfg0 + threadIdx.x] = sweep;
+    }
+    if (LDS)
+        for (uint32_t t = threadIdx.x; t < N; t += blockDim.x) gw[t] = lds32[t];
 }
 
 // One 64-lane workgroup per (sample, set): the lanes stage the set's
@@ -415,6 +619,122 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
     if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
 }
 
+// Multi-spin measurement, one workgroup per (sample, temperature k): the
+// site products q_i of configurations perm_0[k] and perm_1[k] (bits of the
+// two sets' words) are staged in LDS, then each lane walks whole lattice rows
+// (x0 = 0 .. L-1 at fixed x1, x2).  Along a row it sums q, the link products
+// and the x-axis Fourier terms; the row sum goes to per-plane integer sums
+// M1[x1], M2[x2], from which the y and z Fourier terms follow at the end.
+template <int Z>
+__global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
+{
+    __shared__ int ired[2 * 16];
+    __shared__ double dred[2 * 16];
+    __shared__ double ctab[EA_LMAX], stab[EA_LMAX];
+    __shared__ int M1[EA_LMAX], M2[EA_LMAX];
+    extern __shared__ int8_t qlds[];
+    constexpr int D = Z / 2;
+    const int m = a.m;
+    const int s = blockIdx.x / m, k = blockIdx.x - (blockIdx.x / m) * m;
+    const int L = a.L[s];
+    const uint32_t L32 = (uint32_t)L;
+    const uint32_t N = L32 * (D >= 2 ? L32 : 1u) * (D >= 3 ? L32 : 1u);
+    const int ca = a.perm[((int64_t)s * 2 + 0) * m + k];
+    const int cb = a.perm[((int64_t)s * 2 + 1) * m + k];
+    const uint32_t* wa = a.msw + a.ms_off[s];
+    const uint32_t* wb = wa + N;
+    for (int x = threadIdx.x; x < L; x += blockDim.x) {
+        double sn, cs;
+        sincospi(2.0 * x / L, &sn, &cs);
+        ctab[x] = cs;
+        stab[x] = sn;
+        M1[x] = 0;
+        M2[x] = 0;
+    }
+    for (uint32_t t = threadIdx.x; t < N; t += blockDim.x)
+        qlds[t] = (((wa[t] >> ca) ^ (wb[t] >> cb)) & 1u) ? (int8_t)-1 : (int8_t)1;
+    __syncthreads();
+    const uint32_t rows = N / L32;
+    int q = 0, ql = 0;
+    double re0 = 0.0, im0 = 0.0;
+    for (uint32_t row = threadIdx.x; row < rows; row += blockDim.x) {
+        const uint32_t x1 = D >= 2 ? row % L32 : 0u, x2 = D >= 3 ? row / L32 : 0u;
+        const uint32_t base = row * L32;
+        const uint32_t by = D >= 2 ? ((x1 + 1 == L32 ? 0u : x1 + 1) + L32 * x2) * L32 : 0u;
+        const uint32_t bz = D >= 3 ? (x1 + L32 * (x2 + 1 == L32 ? 0u : x2 + 1)) * L32 : 0u;
+        const int first = qlds[base];
+        int prev = first, rsum = 0;
+        for (uint32_t x0 = 0; x0 < L32; ++x0) {
+            const int qi = x0 == 0 ? first : prev;
+            const int qn = x0 + 1 == L32 ? first : qlds[base + x0 + 1];
+            rsum += qi;
+            ql += qi * qn;
+            if (D >= 2) ql += qi * qlds[by + x0];
+            if (D >= 3) ql += qi * qlds[bz + x0];
+            re0 += qi * ctab[x0];
+            im0 += qi * stab[x0];
+            prev = qn;
+        }
+        q += rsum;
+        if (D >= 2) atomicAdd(&M1[x1], rsum);
+        if (D >= 3) atomicAdd(&M2[x2], rsum);
+    }
+    q = wave_sum(q);
+    ql = wave_sum(ql);
+    re0 = wave_sum_f64(re0);
+    im0 = wave_sum_f64(im0);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ired[2 * w] = q;
+        ired[2 * w + 1] = ql;
+        dred[2 * w] = re0;
+        dred[2 * w + 1] = im0;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int nw = (blockDim.x + 63) >> 6;
+    int Q = 0, QL = 0;
+    double RE[3] = {0, 0, 0}, IM[3] = {0, 0, 0};
+    for (int v = 0; v < nw; ++v) {
+        Q += ired[2 * v];
+        QL += ired[2 * v + 1];
+        RE[0] += dred[2 * v];
+        IM[0] += dred[2 * v + 1];
+    }
+    for (int v = 0; v < L; ++v) {
+        if (D >= 2) {
+            RE[1] += M1[v] * ctab[v];
+            IM[1] += M1[v] * stab[v];
+        }
+        if (D >= 3) {
+            RE[2] += M2[v] * ctab[v];
+            IM[2] += M2[v] * stab[v];
+        }
+    }
+    double qk2 = 0.0;
+    for (int ax = 0; ax < D; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
+    qk2 /= D;
+    const double Ea = (double)a.energies[((int64_t)s * 2 + 0) * m + ca];
+    const double Eb = (double)a.energies[((int64_t)s * 2 + 1) * m + cb];
+    const int bin = 63 - __clzll((long long)(slot + 1));  // steps of this call, from 0
+    if (bin < a.n_bins) {
+        double* o = a.acc + (((int64_t)s * a.n_bins + bin) * m + k) * EA_NOBS;
+        const double qq = (double)Q / (double)N;
+        o[0] += 0.5 * (Ea + Eb) / (double)N;
+        o[1] += (double)QL / (double)(D * N);
+        o[2] += fabs(qq);
+        o[3] += qq * qq;
+        o[4] += qq * qq * qq * qq;
+        o[5] += qk2;
+        o[6] += 1.0;
+    }
+    if (a.e_series) {
+        a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = (int32_t)Ea;
+        a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = (int32_t)Eb;
+    }
+    if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
+}
+
 // E = -sum_b J_b s_i s_j for every configuration (critmc/kernels/glass.py:33-38).
 template <int Z, bool G>
 __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
@@ -521,9 +841,23 @@ using namespace cm;
 
 extern "C" {
 
-int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m)
+// Thresholds, then (multi-spin +-J path) per-sample word offsets, the packed
+// words and the neighbour-coupling bytes.
+static int64_t ea_ws_layout(int32_t n_samples, int32_t m, int64_t total_sites, int64_t* off_ms,
+                            int64_t* off_w, int64_t* off_j)
 {
-    return (int64_t)n_samples * m * EA_TAB * 8;
+    const int64_t thr = ((int64_t)n_samples * m * EA_TAB * 8 + 255) / 256 * 256;
+    const int64_t offs = ((int64_t)n_samples * 8 + 255) / 256 * 256;
+    const int64_t words = (2 * total_sites * 4 + 255) / 256 * 256;
+    if (off_ms) *off_ms = thr;
+    if (off_w) *off_w = thr + offs;
+    if (off_j) *off_j = thr + offs + words;
+    return thr + offs + words + (total_sites + 255) / 256 * 256;
+}
+
+int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m, int64_t total_sites)
+{
+    return ea_ws_layout(n_samples, m, total_sites, nullptr, nullptr, nullptr);
 }
 
 double cm_det_exp(double x) { return det_exp(x); }
@@ -596,7 +930,14 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
                    "extents must be even, >= 2 and <= 256 (sample %d has %d)", s, L_host[s]);
         max_L = std::max(max_L, (int)L_host[s]);
     }
-    CM_REQUIRE(workspace_bytes >= cm_ea_workspace_bytes(n_samples, m), "workspace too small");
+    int64_t total_sites = 0;
+    std::vector<int64_t> ms_off(n_samples);
+    for (int s = 0; s < n_samples; ++s) {
+        const int64_t Ls = L_host[s];
+        ms_off[s] = 2 * total_sites;
+        total_sites += Ls * Ls * (ndim == 3 ? Ls : 1);
+    }
+    CM_REQUIRE(workspace_bytes >= cm_ea_workspace_bytes(n_samples, m, total_sites), "workspace too small");
     if (n_steps == 0) return CM_OK;
     const int z = 2 * ndim;
     std::vector<uint64_t> thr((size_t)n_samples * m * EA_TAB, 0);
@@ -653,6 +994,39 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
     const size_t lds = all_lds ? (size_t)maxN : 0;
     const int sweep_blocks = n_samples * 2 * m;
     const int swap_threads = 2 * n_samples;
+    if (!jtype && m <= 32 && all_lds) {
+        // multi-spin path: pack, step, unpack
+        int64_t o_ms, o_w, o_j;
+        ea_ws_layout(n_samples, m, total_sites, &o_ms, &o_w, &o_j);
+        int64_t* d_off = (int64_t*)((char*)workspace + o_ms);
+        CM_HIP_CHECK(upload_async(d_off, ms_off.data(), ms_off.size() * 8, st));
+        a.ms_off = d_off;
+        a.msw = (uint32_t*)((char*)workspace + o_w);
+        a.jneg = (uint8_t*)((char*)workspace + o_j);
+        const dim3 pgrid((unsigned)std::min<int64_t>((maxN + 255) / 256, 256), (unsigned)n_samples);
+        if (ndim == 2) hipLaunchKernelGGL(ea_ms_pack<2>, pgrid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL(ea_ms_pack<3>, pgrid, dim3(256), 0, st, a);
+        const bool ms_lds = maxN * 5 <= 60 * 1024;
+        const size_t mlds = ms_lds ? (size_t)maxN * 5 : 0;
+        const size_t qlds = (size_t)maxN;
+        for (int64_t t = 0; t < n_steps; ++t) {
+            const uint64_t step = step0 + (uint64_t)t;
+            if (ndim == 2) {
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(512), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(512), 0, st, a);
+                hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
+                hipLaunchKernelGGL((ea_measure_ms<4>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
+            } else {
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(512), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(512), 0, st, a);
+                hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
+                hipLaunchKernelGGL((ea_measure_ms<6>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
+            }
+        }
+        hipLaunchKernelGGL(ea_ms_unpack, pgrid, dim3(256), 0, st, a);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    }
     int64_t next_edge = 1;
     int bin = 0;
     for (int64_t t = 0; t < n_steps; ++t) {

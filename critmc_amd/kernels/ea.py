@@ -97,7 +97,7 @@ class EaBatch:
         self.label_hist = torch.zeros(S2m * 2, **i32)
         self.swap_acc = torch.zeros(self.S * 2 * (self.m - 1), **i32)
         self.swap_try = torch.zeros(self.S * 2 * (self.m - 1), **i32)
-        nbytes = int(lib().cm_ea_workspace_bytes(self.S, self.m))
+        nbytes = int(lib().cm_ea_workspace_bytes(self.S, self.m, int(self.N.sum())))
         self.ws_bytes = nbytes
         self.ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dv)
         self.step = 0

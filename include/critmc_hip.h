@@ -190,8 +190,9 @@ int cm_potts_cluster_run(void *spins_dev, int32_t ndim, const int32_t *dims, int
  * "bimodal"), int32 energies and energy series; jtype 1: FP64 couplings (e.g.
  * disorder "gaussian", critmc/glass.py:60-64), FP64 energies and series. */
 
-/* Bytes of caller-owned device scratch cm_ea_pt_run needs. */
-int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m);
+/* Bytes of caller-owned device scratch cm_ea_pt_run needs; total_sites is
+ * sum_s L[s]^d. */
+int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m, int64_t total_sites);
 
 /* e^x as the tempering swap test evaluates it (fixed FP64 operation sequence). */
 double cm_det_exp(double x);
@@ -223,7 +224,13 @@ int cm_ea_energy(void *spins_dev, const void *J_dev, int32_t jtype, const int32_
  * [S][2][m] (perm: temperature -> configuration; kofc its inverse), trips
  * [S][2], label_hist [S][2][m][2], swap_acc / swap_try [S][2][m-1].
  * betas_host / betas_dev: [S][m], hot -> cold.  step0: the global PT step of
- * the first step (keys the swap draws).  L_host mirrors L_dev. */
+ * the first step (keys the swap draws).  L_host mirrors L_dev.
+ * Uniforms: with +-1 couplings, m <= 32 and L^d <= 32768 the sweep is
+ * multi-spin coded (one bit per configuration): colour site k of set r draws
+ * its 32 bit planes from Philox blocks 8k .. 8k+7 (tag EA, stream r, the
+ * set's sweep counter = that of its configuration 0), and configuration c
+ * takes bit c of every plane, most significant first; otherwise configuration
+ * c of set r uses word k&3 of block k>>2 (stream r*m + c, its own counter). */
 int cm_ea_pt_run(void *spins_dev, const void *J_dev, int32_t jtype, const int32_t *L_dev,
                  const int64_t *spin_off_dev, const int64_t *j_off_dev, int32_t n_samples,
                  int32_t m, int32_t ndim, const int32_t *L_host, const double *betas_host,

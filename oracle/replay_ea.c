@@ -21,6 +21,15 @@
  * temperature is measured.  Bin of the call's step t (from 0): floor(log2(t + 1));
  * random numbers are keyed by the global step step0 + t.
  *
+ * Uniforms.  Gaussian couplings, or more than 32 temperatures: configuration
+ * c of set r, colour site k uses word k&3 of Philox block k>>2 at (its own
+ * sweep counter, colour, tag EA, stream r*m + c).  +-J couplings with m <= 32
+ * (the multi-spin path, one bit per configuration): the set's 32 bit planes
+ * for colour site k are words 0..3 of Philox blocks 8k .. 8k+7 at (the set's
+ * sweep counter = that of its configuration 0, colour, tag EA, stream r), and
+ * configuration c's uniform is x = sum_t bit_c(plane_t) 2^(31-t); every
+ * configuration of the set then carries the set's counter.
+ *
  * det_exp(x): e^x by Cody-Waite reduction x = n ln2 + r and the degree-13
  * Taylor polynomial of e^r in Horner/fma form -- a fixed sequence of IEEE
  * operations, so any correct implementation of it gives the same bits.
@@ -115,7 +124,47 @@ int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int
             if (jtype)
                 for (int c = 0; c < 2 * m; ++c) E[c] = orc_ea_energy_f64(spins + (int64_t)c * N, Jf, ndim, L);
         }
-        for (int r = 0; r < 2; ++r)
+        if (!jtype && m <= 32) {
+            /* multi-spin mapping: the set's planes are shared by its configurations */
+            uint32_t planes[32];
+            for (int r = 0; r < 2; ++r) {
+                uint64_t sw = sweeps[r * m];
+                for (int sp = 0; sp < sweeps_per_step; ++sp, ++sw)
+                    for (int col = 0; col < 2; ++col)
+                        for (int64_t kk = 0; kk < g.half; ++kk) {
+                            for (int q = 0; q < 8; ++q)
+                                orc_draw_block(k0, k1, (uint32_t)r, sw, col, ORC_TAG_EA,
+                                               (uint32_t)(8 * kk + q), planes + 4 * q);
+                            int64_t i = orc_cb_site(&g, kk, col, nb);
+                            for (int c = 0; c < m; ++c) {
+                                int8_t *S = spins + ((int64_t)r * m + c) * N;
+                                const uint64_t *T = thr + kofc[r * m + c] * 16;
+                                uint32_t x = 0;
+                                for (int t = 0; t < 32; ++t) x |= ((planes[t] >> c) & 1u) << (31 - t);
+                                int si = S[i];
+                                int h = 0;
+                                for (int ax = 0; ax < ndim; ++ax)
+                                    h += J[ax * N + i] * S[nb[2 * ax]] +
+                                         J[ax * N + nb[2 * ax + 1]] * S[nb[2 * ax + 1]];
+                                if (code == 0) {
+                                    int aa = si * h;
+                                    if (aa <= 0 || (uint64_t)x < T[aa + z]) {
+                                        S[i] = (int8_t)(-si);
+                                        E[r * m + c] += 2 * aa;
+                                    }
+                                } else {
+                                    int nw = (uint64_t)x < T[h + z] ? 1 : -1;
+                                    if (nw != si) {
+                                        E[r * m + c] += 2 * si * h;
+                                        S[i] = (int8_t)nw;
+                                    }
+                                }
+                            }
+                        }
+                for (int c = 0; c < m; ++c) sweeps[r * m + c] = sw;
+            }
+        }
+        for (int r = 0; r < 2 && (jtype || m > 32); ++r)
             for (int c = 0; c < m; ++c) {
                 int8_t *S = spins + ((int64_t)r * m + c) * N;
                 const uint64_t *T = thr + kofc[r * m + c] * 16;
