@@ -80,6 +80,7 @@ struct EaArgs {
     uint32_t* msw;          // [sum_s 2 N_s] words: sample s, set r, site i at ms_off[s] + r N_s + i
     uint8_t* jneg;          // [sum_s N_s]: bit j set where the coupling to neighbour j is -1
     const int64_t* ms_off;  // [S] word offsets (the jneg offset is ms_off[s] / 2)
+    const int32_t* ms_order;  // [2 S] (sample, set) = (v >> 1, v & 1) per workgroup, largest first
     int S, m, d, code, sweeps_per_step, n_bins;
     int64_t n_series;
 };
@@ -300,6 +301,54 @@ __device__ __forceinline__ uint32_t ms_select(const uint32_t (&V)[8], uint32_t c
 }
 
 constexpr int EA_MS_CNT = 12;  // bit-sliced counter planes for the bond count of one lane
+// Philox blocks a word may draw before leaving its wave's lockstep; words
+// still undecided go to an LDS queue drained densely after the half sweep
+// (the comparison continues unchanged, so every decision is the same).
+constexpr uint32_t EA_MS_BUDGET = 2;
+constexpr int EA_MS_QCAP = 1024;
+
+// Colour site kk: its index, own word and the bit-sliced count of neighbours
+// with J_ij s_j = +1 (heat bath) or aligned with s_i (Metropolis).
+template <int Z>
+__device__ __forceinline__ void ms_classify(const CbGeo& g, const uint32_t* __restrict__ W,
+                                            const uint8_t* __restrict__ JN, uint32_t kk, int col,
+                                            int code, uint32_t& i, uint32_t& own, uint32_t& c0,
+                                            uint32_t& c1, uint32_t& c2)
+{
+    uint32_t nb[Z];
+    i = cb_site32<Z>(g, kk, col, nb);
+    own = W[i];
+    const uint32_t jn = JN[i];
+    uint32_t in[Z];
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        const uint32_t up = W[nb[j]] ^ (0u - ((jn >> j) & 1u));  // J_ij s_j = +1
+        in[j] = code == CODE_METROPOLIS ? ~(own ^ up) : up;
+    }
+    count_bits<Z>(in, c0, c1, c2);
+}
+
+// Bit planes 4 q_begin .. 4 q_end - 1 of the comparison of every pending lane.
+__device__ __forceinline__ void ms_draw(const uint32_t (*__restrict__ P)[8], uint32_t c0, uint32_t c1,
+                                        uint32_t c2, uint32_t& pending, uint32_t& below,
+                                        uint32_t q_begin, uint32_t q_end, uint32_t kk,
+                                        uint64_t sweep, uint32_t zc, int r, uint32_t k0, uint32_t k1)
+{
+    for (uint32_t q = q_begin; q < q_end && pending != 0u; ++q) {
+        const u32x4 rn = philox4x32_10(u32x4{8u * kk + q, (uint32_t)sweep, zc, (uint32_t)r}, k0, k1);
+        const uint32_t planes[4] = {rn.x, rn.y, rn.z, rn.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint4* pr = (const uint4*)P[4 * q + t];
+            const uint4 lo = pr[0], hi = pr[1];
+            const uint32_t V[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            const uint32_t tb = ms_select(V, c0, c1, c2);
+            const uint32_t diff = pending & (planes[t] ^ tb);
+            below |= diff & tb;
+            pending &= ~diff;
+        }
+    }
+}
 
 // One workgroup per (sample, set): every sweep of the step for all of the set's
 // configurations, then each configuration's energy recounted from its bonds.
@@ -307,14 +356,17 @@ constexpr int EA_MS_CNT = 12;  // bit-sliced counter planes for the bond count o
 // threshold planes P[t][class] (bit c = bit 31-t of configuration c's
 // threshold for that neighbour class) with the always-accept masks.
 template <int Z, bool LDS>
-__global__ __launch_bounds__(512) void ea_ms_sweep(EaArgs a)
+__global__ __launch_bounds__(256) void ea_ms_sweep(EaArgs a)
 {
     extern __shared__ uint32_t lds32[];
     __shared__ __align__(16) uint32_t P[32][8];
     __shared__ uint32_t ALW[8];
     __shared__ int ucnt[32];
+    __shared__ uint32_t queue[3 * EA_MS_QCAP];  // colour site, pending, below
+    __shared__ int qcount[2];                   // alternating half sweeps
     const int m = a.m;
-    const int s = blockIdx.x >> 1, r = blockIdx.x & 1;
+    const int v = a.ms_order[blockIdx.x];
+    const int s = v >> 1, r = v & 1;
     const CbGeo g = ea_geo(Z / 2, a.L[s]);
     const uint32_t N = (uint32_t)g.N;
     uint32_t* gw = a.msw + a.ms_off[s] + (int64_t)r * N;
@@ -328,6 +380,7 @@ __global__ __launch_bounds__(512) void ea_ms_sweep(EaArgs a)
     }
     const int64_t cfg0 = ((int64_t)s * 2 + r) * m;
     if (threadIdx.x < 32) ucnt[threadIdx.x] = 0;
+    if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
     if (threadIdx.x < 64) {
         // lane c < m: configuration c's thresholds T[2 cl] for neighbour class cl
         const int c = threadIdx.x;
@@ -350,38 +403,38 @@ __global__ __launch_bounds__(512) void ea_ms_sweep(EaArgs a)
     uint64_t sweep = a.sweeps[cfg0];
     const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
     const uint32_t half = (uint32_t)g.half;
+    uint32_t hs = 0;
     for (int sp = 0; sp < a.sweeps_per_step; ++sp) {
-        for (int col = 0; col < 2; ++col) {
+        for (int col = 0; col < 2; ++col, ++hs) {
             const uint32_t zc = ctr_z(sweep, (uint32_t)col, TAG_EA);
+            int* qc = qcount + (hs & 1);
             for (uint32_t kk = threadIdx.x; kk < half; kk += blockDim.x) {
-                uint32_t nb[Z];
-                const uint32_t i = cb_site32<Z>(g, kk, col, nb);
-                const uint32_t own = W[i];
-                const uint32_t jn = JN[i];
-                uint32_t in[Z];
-#pragma unroll
-                for (int j = 0; j < Z; ++j) {
-                    const uint32_t up = W[nb[j]] ^ (0u - ((jn >> j) & 1u));  // J_ij s_j = +1
-                    in[j] = a.code == CODE_METROPOLIS ? ~(own ^ up) : up;
-                }
-                uint32_t c0, c1, c2;
-                count_bits<Z>(in, c0, c1, c2);
+                uint32_t i, own, c0, c1, c2;
+                ms_classify<Z>(g, W, JN, kk, col, a.code, i, own, c0, c1, c2);
                 uint32_t below = ms_select(alw, c0, c1, c2) & lanes;
                 uint32_t pending = lanes & ~below;
-                for (uint32_t q = 0; q < 8u && pending != 0u; ++q) {
-                    const u32x4 rn = philox4x32_10(u32x4{8u * kk + q, (uint32_t)sweep, zc, (uint32_t)r}, k0, k1);
-                    const uint32_t planes[4] = {rn.x, rn.y, rn.z, rn.w};
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const uint4* pr = (const uint4*)P[4 * q + t];
-                        const uint4 lo = pr[0], hi = pr[1];
-                        const uint32_t V[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                        const uint32_t tb = ms_select(V, c0, c1, c2);
-                        const uint32_t diff = pending & (planes[t] ^ tb);
-                        below |= diff & tb;
-                        pending &= ~diff;
+                ms_draw(P, c0, c1, c2, pending, below, 0, EA_MS_BUDGET, kk, sweep, zc, r, k0, k1);
+                if (pending) {
+                    const int slot = atomicAdd(qc, 1);
+                    if (slot < EA_MS_QCAP) {
+                        queue[3 * slot] = kk;
+                        queue[3 * slot + 1] = pending;
+                        queue[3 * slot + 2] = below;
+                        continue;
                     }
+                    ms_draw(P, c0, c1, c2, pending, below, EA_MS_BUDGET, 8, kk, sweep, zc, r, k0, k1);
                 }
+                W[i] = a.code == CODE_METROPOLIS ? own ^ below : below;
+            }
+            __syncthreads();
+            const int nq = min(*qc, EA_MS_QCAP);
+            if (threadIdx.x == 0) qcount[(hs + 1) & 1] = 0;
+            for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+                const uint32_t kk = queue[3 * t];
+                uint32_t i, own, c0, c1, c2;
+                ms_classify<Z>(g, W, JN, kk, col, a.code, i, own, c0, c1, c2);
+                uint32_t pending = queue[3 * t + 1], below = queue[3 * t + 2];
+                ms_draw(P, c0, c1, c2, pending, below, EA_MS_BUDGET, 8, kk, sweep, zc, r, k0, k1);
                 W[i] = a.code == CODE_METROPOLIS ? own ^ below : below;
             }
             __syncthreads();
@@ -847,7 +900,7 @@ static int64_t ea_ws_layout(int32_t n_samples, int32_t m, int64_t total_sites, i
                             int64_t* off_w, int64_t* off_j)
 {
     const int64_t thr = ((int64_t)n_samples * m * EA_TAB * 8 + 255) / 256 * 256;
-    const int64_t offs = ((int64_t)n_samples * 8 + 255) / 256 * 256;
+    const int64_t offs = ((int64_t)n_samples * 16 + 255) / 256 * 256;
     const int64_t words = (2 * total_sites * 4 + 255) / 256 * 256;
     if (off_ms) *off_ms = thr;
     if (off_w) *off_w = thr + offs;
@@ -998,9 +1051,17 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
         // multi-spin path: pack, step, unpack
         int64_t o_ms, o_w, o_j;
         ea_ws_layout(n_samples, m, total_sites, &o_ms, &o_w, &o_j);
+        // dispatch the largest sets first, so the heavy workgroups spread over the CUs
+        std::vector<int64_t> hdr(2 * (size_t)n_samples, 0);
+        std::copy(ms_off.begin(), ms_off.end(), hdr.begin());
+        int32_t* order = (int32_t*)(hdr.data() + n_samples);
+        for (int v = 0; v < 2 * n_samples; ++v) order[v] = v;
+        std::stable_sort(order, order + 2 * n_samples,
+                         [&](int32_t x, int32_t y) { return L_host[x >> 1] > L_host[y >> 1]; });
         int64_t* d_off = (int64_t*)((char*)workspace + o_ms);
-        CM_HIP_CHECK(upload_async(d_off, ms_off.data(), ms_off.size() * 8, st));
+        CM_HIP_CHECK(upload_async(d_off, hdr.data(), hdr.size() * 8, st));
         a.ms_off = d_off;
+        a.ms_order = (const int32_t*)(d_off + n_samples);
         a.msw = (uint32_t*)((char*)workspace + o_w);
         a.jneg = (uint8_t*)((char*)workspace + o_j);
         const dim3 pgrid((unsigned)std::min<int64_t>((maxN + 255) / 256, 256), (unsigned)n_samples);
@@ -1012,13 +1073,13 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
         for (int64_t t = 0; t < n_steps; ++t) {
             const uint64_t step = step0 + (uint64_t)t;
             if (ndim == 2) {
-                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(512), mlds, st, a);
-                else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(512), 0, st, a);
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
                 hipLaunchKernelGGL((ea_measure_ms<4>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
             } else {
-                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(512), mlds, st, a);
-                else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(512), 0, st, a);
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
                 hipLaunchKernelGGL((ea_measure_ms<6>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
             }
