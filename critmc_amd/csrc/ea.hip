@@ -672,28 +672,50 @@ __global__ __launch_bounds__(256) void ea_measure(EaArgs a, int64_t slot)
     if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
 }
 
-// Multi-spin measurement, one workgroup per (sample, temperature k): the
-// site products q_i of configurations perm_0[k] and perm_1[k] (bits of the
-// two sets' words) are staged in LDS, then each lane walks whole lattice rows
-// (x0 = 0 .. L-1 at fixed x1, x2).  Along a row it sums q, the link products
-// and the x-axis Fourier terms; the row sum goes to per-plane integer sums
-// M1[x1], M2[x2], from which the y and z Fourier terms follow at the end.
+// 32 x 32 bit transpose in registers: bit j of v[i] becomes bit i of v[j].
+__device__ __forceinline__ void transpose32(uint32_t (&v)[32])
+{
+    uint32_t msk = 0x0000FFFFu;
+#pragma unroll
+    for (int j = 16; j != 0; j >>= 1, msk ^= msk << j) {
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            if (k & j) continue;
+            const uint32_t t = ((v[k] >> j) ^ v[k + j]) & msk;
+            v[k + j] ^= t;
+            v[k] ^= t << j;
+        }
+    }
+}
+
+constexpr int EA_MS_ROWS = 256;  // lattice rows (L^(d-1)) a measurement workgroup holds
+
+// Multi-spin measurement, one workgroup per sample, all temperatures.
+// Phase 1 (a lane per lattice row x1, x2): the row's words of both sets are
+// transposed, so configuration c's spins along the row form one L-bit
+// pattern; temperature k's overlap pattern is then qp = ~(A_ca ^ B_cb) (bit x
+// set where q_i = +1), stored in LDS QP[row][k].
+// Phase 2 (eight lanes per temperature, each a band of rows): q = 2 popc - L
+// per row; link products from popcounts of qp against its rotation (x) and
+// the patterns of the next rows (y, z); the x Fourier sum 2 sum_{bit x set}
+// e^{i 2 pi x / L} from nibble tables (sum_x e^{i 2 pi x / L} = 0); the row
+// sums go to per-plane sums M1[k][x1], M2[k][x2] for the y and z terms.
 template <int Z>
 __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
 {
-    __shared__ int ired[2 * 16];
-    __shared__ double dred[2 * 16];
-    __shared__ double ctab[EA_LMAX], stab[EA_LMAX];
-    __shared__ int M1[EA_LMAX], M2[EA_LMAX];
-    extern __shared__ int8_t qlds[];
     constexpr int D = Z / 2;
+    __shared__ uint32_t QP[EA_MS_ROWS][32];
+    __shared__ double ctab[EA_LMAX], stab[EA_LMAX];
+    __shared__ double CT[8][16], ST[8][16];
+    __shared__ int M1[32][32], M2[32][32];
+    __shared__ int pca[32], pcb[32], kca[32], kcb[32];
     const int m = a.m;
-    const int s = blockIdx.x / m, k = blockIdx.x - (blockIdx.x / m) * m;
+    const int s = blockIdx.x;
     const int L = a.L[s];
     const uint32_t L32 = (uint32_t)L;
     const uint32_t N = L32 * (D >= 2 ? L32 : 1u) * (D >= 3 ? L32 : 1u);
-    const int ca = a.perm[((int64_t)s * 2 + 0) * m + k];
-    const int cb = a.perm[((int64_t)s * 2 + 1) * m + k];
+    const uint32_t rows = N / L32;
+    const uint32_t rowmask = L32 >= 32 ? ~0u : ((1u << L32) - 1u);
     const uint32_t* wa = a.msw + a.ms_off[s];
     const uint32_t* wb = wa + N;
     for (int x = threadIdx.x; x < L; x += blockDim.x) {
@@ -701,80 +723,114 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
         sincospi(2.0 * x / L, &sn, &cs);
         ctab[x] = cs;
         stab[x] = sn;
-        M1[x] = 0;
-        M2[x] = 0;
     }
-    for (uint32_t t = threadIdx.x; t < N; t += blockDim.x)
-        qlds[t] = (((wa[t] >> ca) ^ (wb[t] >> cb)) & 1u) ? (int8_t)-1 : (int8_t)1;
+    for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        M1[t >> 5][t & 31] = 0;
+        M2[t >> 5][t & 31] = 0;
+    }
+    if (threadIdx.x < 32) {
+        const int k = threadIdx.x;
+        pca[k] = k < m ? a.perm[((int64_t)s * 2 + 0) * m + k] : 0;
+        pcb[k] = k < m ? a.perm[((int64_t)s * 2 + 1) * m + k] : 0;
+        kca[k] = k < m ? a.kofc[((int64_t)s * 2 + 0) * m + k] : 0;
+        kcb[k] = k < m ? a.kofc[((int64_t)s * 2 + 1) * m + k] : 0;
+    }
     __syncthreads();
-    const uint32_t rows = N / L32;
+    for (int t = threadIdx.x; t < 8 * 16; t += blockDim.x) {
+        const int j = t >> 4, v = t & 15;
+        double c = 0.0, sn = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            const int x = 4 * j + b;
+            if (((v >> b) & 1) && x < L) {
+                c += ctab[x];
+                sn += stab[x];
+            }
+        }
+        CT[j][v] = c;
+        ST[j][v] = sn;
+    }
+    // phase 1
+    for (uint32_t row = threadIdx.x; row < rows; row += blockDim.x) {
+        uint32_t v[32];
+        // set 0: configuration c's row pattern is v[c]; place it at its temperature
+#pragma unroll
+        for (int x = 0; x < 32; ++x) v[x] = (uint32_t)x < L32 ? wa[row * L32 + x] : 0u;
+        transpose32(v);
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (c < m) QP[row][kca[c]] = v[c];
+        // set 1: the overlap pattern with the set-0 configuration at the same temperature
+#pragma unroll
+        for (int x = 0; x < 32; ++x) v[x] = (uint32_t)x < L32 ? wb[row * L32 + x] : 0u;
+        transpose32(v);
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (c < m) QP[row][kcb[c]] = ~(QP[row][kcb[c]] ^ v[c]) & rowmask;
+    }
+    __syncthreads();
+    // phase 2: lane = 8 k + j
+    const int k = threadIdx.x >> 3, j = threadIdx.x & 7;
     int q = 0, ql = 0;
     double re0 = 0.0, im0 = 0.0;
-    for (uint32_t row = threadIdx.x; row < rows; row += blockDim.x) {
-        const uint32_t x1 = D >= 2 ? row % L32 : 0u, x2 = D >= 3 ? row / L32 : 0u;
-        const uint32_t base = row * L32;
-        const uint32_t by = D >= 2 ? ((x1 + 1 == L32 ? 0u : x1 + 1) + L32 * x2) * L32 : 0u;
-        const uint32_t bz = D >= 3 ? (x1 + L32 * (x2 + 1 == L32 ? 0u : x2 + 1)) * L32 : 0u;
-        const int first = qlds[base];
-        int prev = first, rsum = 0;
-        for (uint32_t x0 = 0; x0 < L32; ++x0) {
-            const int qi = x0 == 0 ? first : prev;
-            const int qn = x0 + 1 == L32 ? first : qlds[base + x0 + 1];
-            rsum += qi;
-            ql += qi * qn;
-            if (D >= 2) ql += qi * qlds[by + x0];
-            if (D >= 3) ql += qi * qlds[bz + x0];
-            re0 += qi * ctab[x0];
-            im0 += qi * stab[x0];
-            prev = qn;
+    if (k < m) {
+        const int nib = (L + 3) >> 2;
+        for (uint32_t row = j; row < rows; row += 8) {
+            const uint32_t x1 = D >= 2 ? row % L32 : 0u, x2 = D >= 3 ? row / L32 : 0u;
+            const uint32_t qp = QP[row][k];
+            const int rs = 2 * __popc(qp) - L;
+            q += rs;
+            const uint32_t rot = ((qp >> 1) | (qp << (L32 - 1))) & rowmask;  // bit x: q at x + 1
+            ql += 2 * __popc(~(qp ^ rot) & rowmask) - L;
+            if (D >= 2) {
+                const uint32_t ry = (x1 + 1 == L32 ? 0u : x1 + 1) + L32 * x2;
+                ql += 2 * __popc(~(qp ^ QP[ry][k]) & rowmask) - L;
+                atomicAdd(&M1[k][x1], rs);
+            }
+            if (D >= 3) {
+                const uint32_t rz = x1 + L32 * (x2 + 1 == L32 ? 0u : x2 + 1);
+                ql += 2 * __popc(~(qp ^ QP[rz][k]) & rowmask) - L;
+                atomicAdd(&M2[k][x2], rs);
+            }
+            for (int t = 0; t < nib; ++t) {
+                const uint32_t v = (qp >> (4 * t)) & 15u;
+                re0 += CT[t][v];
+                im0 += ST[t][v];
+            }
         }
-        q += rsum;
-        if (D >= 2) atomicAdd(&M1[x1], rsum);
-        if (D >= 3) atomicAdd(&M2[x2], rsum);
     }
-    q = wave_sum(q);
-    ql = wave_sum(ql);
-    re0 = wave_sum_f64(re0);
-    im0 = wave_sum_f64(im0);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        ired[2 * w] = q;
-        ired[2 * w + 1] = ql;
-        dred[2 * w] = re0;
-        dred[2 * w + 1] = im0;
+    // reduce the eight lanes of each temperature (fixed order)
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        q += __shfl_xor(q, off, 64);
+        ql += __shfl_xor(ql, off, 64);
+        re0 += __shfl_xor(re0, off, 64);
+        im0 += __shfl_xor(im0, off, 64);
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int nw = (blockDim.x + 63) >> 6;
-    int Q = 0, QL = 0;
-    double RE[3] = {0, 0, 0}, IM[3] = {0, 0, 0};
-    for (int v = 0; v < nw; ++v) {
-        Q += ired[2 * v];
-        QL += ired[2 * v + 1];
-        RE[0] += dred[2 * v];
-        IM[0] += dred[2 * v + 1];
-    }
+    if (j != 0 || k >= m) return;
+    double RE[3] = {2.0 * re0, 0, 0}, IM[3] = {2.0 * im0, 0, 0};
     for (int v = 0; v < L; ++v) {
         if (D >= 2) {
-            RE[1] += M1[v] * ctab[v];
-            IM[1] += M1[v] * stab[v];
+            RE[1] += M1[k][v] * ctab[v];
+            IM[1] += M1[k][v] * stab[v];
         }
         if (D >= 3) {
-            RE[2] += M2[v] * ctab[v];
-            IM[2] += M2[v] * stab[v];
+            RE[2] += M2[k][v] * ctab[v];
+            IM[2] += M2[k][v] * stab[v];
         }
     }
     double qk2 = 0.0;
     for (int ax = 0; ax < D; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
     qk2 /= D;
+    const int ca = pca[k], cb = pcb[k];
     const double Ea = (double)a.energies[((int64_t)s * 2 + 0) * m + ca];
     const double Eb = (double)a.energies[((int64_t)s * 2 + 1) * m + cb];
     const int bin = 63 - __clzll((long long)(slot + 1));  // steps of this call, from 0
     if (bin < a.n_bins) {
         double* o = a.acc + (((int64_t)s * a.n_bins + bin) * m + k) * EA_NOBS;
-        const double qq = (double)Q / (double)N;
+        const double qq = (double)q / (double)N;
         o[0] += 0.5 * (Ea + Eb) / (double)N;
-        o[1] += (double)QL / (double)(D * N);
+        o[1] += (double)ql / (double)(D * N);
         o[2] += fabs(qq);
         o[3] += qq * qq;
         o[4] += qq * qq * qq * qq;
@@ -785,7 +841,7 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
         a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + slot] = (int32_t)Ea;
         a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + slot] = (int32_t)Eb;
     }
-    if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = Q;
+    if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = q;
 }
 
 // E = -sum_b J_b s_i s_j for every configuration (critmc/kernels/glass.py:33-38).
@@ -1047,7 +1103,7 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
     const size_t lds = all_lds ? (size_t)maxN : 0;
     const int sweep_blocks = n_samples * 2 * m;
     const int swap_threads = 2 * n_samples;
-    if (!jtype && m <= 32 && all_lds) {
+    if (!jtype && m <= 32 && all_lds && max_L <= 32 && maxN / max_L <= EA_MS_ROWS) {
         // multi-spin path: pack, step, unpack
         int64_t o_ms, o_w, o_j;
         ea_ws_layout(n_samples, m, total_sites, &o_ms, &o_w, &o_j);
@@ -1069,19 +1125,18 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
         else hipLaunchKernelGGL(ea_ms_pack<3>, pgrid, dim3(256), 0, st, a);
         const bool ms_lds = maxN * 5 <= 60 * 1024;
         const size_t mlds = ms_lds ? (size_t)maxN * 5 : 0;
-        const size_t qlds = (size_t)maxN;
         for (int64_t t = 0; t < n_steps; ++t) {
             const uint64_t step = step0 + (uint64_t)t;
             if (ndim == 2) {
                 if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
                 else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
-                hipLaunchKernelGGL((ea_measure_ms<4>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
+                hipLaunchKernelGGL((ea_measure_ms<4>), dim3(a.S), dim3(256), 0, st, a, t);
             } else {
                 if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
                 else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
-                hipLaunchKernelGGL((ea_measure_ms<6>), dim3(a.S * a.m), dim3(256), qlds, st, a, t);
+                hipLaunchKernelGGL((ea_measure_ms<6>), dim3(a.S), dim3(256), 0, st, a, t);
             }
         }
         hipLaunchKernelGGL(ea_ms_unpack, pgrid, dim3(256), 0, st, a);
