@@ -225,8 +225,8 @@ int cm_ea_energy(void *spins_dev, const void *J_dev, int32_t jtype, const int32_
  * [S][2], label_hist [S][2][m][2], swap_acc / swap_try [S][2][m-1].
  * betas_host / betas_dev: [S][m], hot -> cold.  step0: the global PT step of
  * the first step (keys the swap draws).  L_host mirrors L_dev.
- * Uniforms: with +-1 couplings, m <= 32 and L^d <= 32768 the sweep is
- * multi-spin coded (one bit per configuration): colour site k of set r draws
+ * Uniforms: with +-1 couplings, m <= 32 and L <= 32 (2D) or L <= 16 (3D) the
+ * sweep is multi-spin coded (one bit per configuration): colour site k of set r draws
  * its 32 bit planes from Philox blocks 8k .. 8k+7 (tag EA, stream r, the
  * set's sweep counter = that of its configuration 0), and configuration c
  * takes bit c of every plane, most significant first; otherwise configuration
