@@ -971,6 +971,20 @@ int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m, int64_t total_sites)
 
 double cm_det_exp(double x) { return det_exp(x); }
 
+// The multi-spin kernels take +-J batches of at most 32 temperatures whose
+// largest lattice keeps a set in LDS and a row in one word (L <= 32, at most
+// EA_MS_ROWS rows).
+static bool ea_use_ms(int32_t jtype, int32_t m, int32_t ndim, int32_t max_L)
+{
+    const int64_t maxN = (int64_t)max_L * max_L * (ndim == 3 ? max_L : 1);
+    return !jtype && m <= 32 && maxN <= EA_LDS_MAX && max_L <= 32 && maxN / max_L <= EA_MS_ROWS;
+}
+
+int cm_ea_multispin(int32_t jtype, int32_t m, int32_t ndim, int32_t max_L)
+{
+    return ea_use_ms(jtype, m, ndim, max_L) ? 1 : 0;
+}
+
 int cm_ea_init(void* spins, const int32_t* L_dev, const int64_t* spin_off_dev, int32_t n_samples,
                int32_t m, int32_t ndim, const uint32_t* keys_dev, void* stream)
 {
@@ -1103,7 +1117,7 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
     const size_t lds = all_lds ? (size_t)maxN : 0;
     const int sweep_blocks = n_samples * 2 * m;
     const int swap_threads = 2 * n_samples;
-    if (!jtype && m <= 32 && all_lds && max_L <= 32 && maxN / max_L <= EA_MS_ROWS) {
+    if (ea_use_ms(jtype, m, ndim, max_L)) {
         // multi-spin path: pack, step, unpack
         int64_t o_ms, o_w, o_j;
         ea_ws_layout(n_samples, m, total_sites, &o_ms, &o_w, &o_j);

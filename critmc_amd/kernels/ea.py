@@ -103,6 +103,12 @@ class EaBatch:
         self.step = 0
         self.recompute_energies()
 
+    @property
+    def multispin(self) -> bool:
+        """Whether ``run`` takes this batch multi-spin coded, which fixes its uniform
+        mapping (one bit per configuration, planes shared by a set)."""
+        return bool(lib().cm_ea_multispin(self.jtype, self.m, self.d, int(self.L.max())))
+
     def recompute_energies(self) -> None:
         check(lib().cm_ea_energy(ptr(self.spins), ptr(self.J), self.jtype, ptr(self.L_d),
                                  ptr(self.spin_off_d), ptr(self.j_off_d), self.S, self.m, self.d,

@@ -197,6 +197,12 @@ int64_t cm_ea_workspace_bytes(int32_t n_samples, int32_t m, int64_t total_sites)
 /* e^x as the tempering swap test evaluates it (fixed FP64 operation sequence). */
 double cm_det_exp(double x);
 
+/* 1 if cm_ea_pt_run runs a batch with these parameters multi-spin coded
+ * (+-1 couplings, m <= 32, largest extent max_L <= 32 in 2D or <= 16 in 3D),
+ * which fixes its uniform mapping; 0 otherwise.  No counterpart in the
+ * reference, whose kernels have one mapping. */
+int cm_ea_multispin(int32_t jtype, int32_t m, int32_t ndim, int32_t max_L);
+
 /* Random +-1 start of every configuration (Philox tag INIT, stream r*m + c,
  * key = the sample's).  Replaces the spin draw of ParallelTempering.__init__
  * (critmc/glass.py:148) for device-side batches. */

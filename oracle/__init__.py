@@ -280,8 +280,8 @@ _EXTRA_DECLS.update({
     "orc_ea_pt_run": ([_i8p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                        _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _u64p,
                        ctypes.c_void_p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f64p,
-                       ctypes.c_int, ctypes.c_void_p, _i32p, ctypes.c_uint64, ctypes.c_int64],
-                      ctypes.c_int),
+                       ctypes.c_int, ctypes.c_void_p, _i32p, ctypes.c_uint64, ctypes.c_int64,
+                       ctypes.c_int], ctypes.c_int),
 })
 
 
@@ -298,7 +298,7 @@ class EaReplay:
     """
 
     def __init__(self, spins, J, ndim: int, L: int, betas, key, code: int = 2,
-                 sweeps_per_step: int = 1):
+                 sweeps_per_step: int = 1, multispin: bool | None = None):
         self.spins = np.ascontiguousarray(spins, dtype=np.int8).copy()
         J = np.asarray(J, dtype=np.float64)
         self.jtype = 0 if np.all(np.abs(J) == 1.0) else 1
@@ -323,6 +323,13 @@ class EaReplay:
         self.kofc = self.perm.copy()
         self.labels = np.zeros(2 * m, np.int32)
         self.step = 0
+        # uniform mapping: the multi-spin one where the device takes a batch of
+        # this sample alone multi-spin coded (+-J, m <= 32, L <= 32 in 2D or
+        # L <= 16 in 3D); a mixed batch passes the batch's choice
+        if multispin is None:
+            multispin = (self.jtype == 0 and m <= 32
+                         and self.L <= (32 if self.d == 2 else 16))
+        self.multispin = bool(multispin)
 
     def run(self, n_steps: int, n_bins: int, record: bool = True) -> dict[str, np.ndarray]:
         m = self.m
@@ -339,7 +346,7 @@ class EaReplay:
             _ptr(self.sweeps, _u64p), self.energies.ctypes.data, _ptr(self.perm, _i32p),
             _ptr(self.kofc, _i32p), _ptr(self.labels, _i32p), _ptr(trips, _i32p), _ptr(hist, _i32p),
             _ptr(sacc, _i32p), _ptr(stry, _i32p), _ptr(acc, _f64p), int(n_bins), es.ctypes.data,
-            _ptr(qs, _i32p), int(self.step), int(n_steps))
+            _ptr(qs, _i32p), int(self.step), int(n_steps), int(self.multispin))
         if rc:
             raise ValueError(f"orc_ea_pt_run failed ({rc})")
         self.step += n_steps

@@ -21,10 +21,11 @@
  * temperature is measured.  Bin of the call's step t (from 0): floor(log2(t + 1));
  * random numbers are keyed by the global step step0 + t.
  *
- * Uniforms.  Gaussian couplings, or more than 32 temperatures: configuration
+ * Uniforms.  Per-configuration path (Gaussian couplings, more than 32
+ * temperatures, or lattices the multi-spin kernels do not take): configuration
  * c of set r, colour site k uses word k&3 of Philox block k>>2 at (its own
- * sweep counter, colour, tag EA, stream r*m + c).  +-J couplings with m <= 32
- * (the multi-spin path, one bit per configuration): the set's 32 bit planes
+ * sweep counter, colour, tag EA, stream r*m + c).  Multi-spin path
+ * (`multispin`, +-J couplings, m <= 32, one bit per configuration): the set's 32 bit planes
  * for colour site k are words 0..3 of Philox blocks 8k .. 8k+7 at (the set's
  * sweep counter = that of its configuration 0, colour, tag EA, stream r), and
  * configuration c's uniform is x = sum_t bit_c(plane_t) 2^(31-t); every
@@ -91,7 +92,7 @@ int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int
                   uint64_t *sweeps, void *energies_v, int32_t *perm, int32_t *kofc,
                   int32_t *labels, int32_t *trips, int32_t *label_hist, int32_t *swap_acc,
                   int32_t *swap_try, double *acc, int n_bins, void *e_series_v, int32_t *q_series,
-                  uint64_t step0, int64_t n_steps)
+                  uint64_t step0, int64_t n_steps, int multispin)
 {
     const int8_t *J = jtype ? NULL : (const int8_t *)Jv;
     const double *Jf = jtype ? (const double *)Jv : NULL;
@@ -101,6 +102,7 @@ int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int
     orc_cb g;
     if (orc_cb_init(&g, ndim, dims)) return -2;
     if (code != 0 && code != 2) return -3;
+    if (multispin && (jtype || m > 32)) return -4;
     const int z = 2 * ndim;
     const int64_t N = g.N;
     uint64_t *thr = (uint64_t *)malloc(sizeof(uint64_t) * m * 16);
@@ -124,7 +126,7 @@ int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int
             if (jtype)
                 for (int c = 0; c < 2 * m; ++c) E[c] = orc_ea_energy_f64(spins + (int64_t)c * N, Jf, ndim, L);
         }
-        if (!jtype && m <= 32) {
+        if (multispin) {
             /* multi-spin mapping: the set's planes are shared by its configurations */
             uint32_t planes[32];
             for (int r = 0; r < 2; ++r) {
@@ -164,7 +166,7 @@ int orc_ea_pt_run(int8_t *spins, const void *Jv, int jtype, int ndim, int L, int
                 for (int c = 0; c < m; ++c) sweeps[r * m + c] = sw;
             }
         }
-        for (int r = 0; r < 2 && (jtype || m > 32); ++r)
+        for (int r = 0; r < 2 && !multispin; ++r)
             for (int c = 0; c < m; ++c) {
                 int8_t *S = spins + ((int64_t)r * m + c) * N;
                 const uint64_t *T = thr + kofc[r * m + c] * 16;

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("ndim,Ls,code,disorder", [
     (3, [4, 6, 8, 4], 2, "bimodal"), (2, [4, 8, 6], 0, "bimodal"), (3, [8, 12, 16], 2, "bimodal"),
-    (3, [4, 6, 8, 4], 2, "gaussian"), (2, [4, 8, 6], 0, "gaussian"), (3, [8, 12], 0, "gaussian")])
+    (3, [4, 6, 8, 4], 2, "gaussian"), (2, [4, 8, 6], 0, "gaussian"), (3, [8, 12], 0, "gaussian"),
+    (3, [4, 18], 2, "bimodal")])  # L = 18 > 16: the per-configuration +-J path
 def test_batch_is_bit_exact_against_replay(ndim, Ls, code, disorder):
     m = 5
     models = [EdwardsAnderson(hypercubic((L,) * ndim), disorder=disorder, seed=7 + s)
@@ -35,10 +36,12 @@ def test_batch_is_bit_exact_against_replay(ndim, Ls, code, disorder):
     runs = batch.run(n_bins, record=True)
     runs2 = batch.run(3, record=True)  # a second call continues the streams
     eng = batch.engine
+    assert eng.multispin == (not fp and max(Ls) <= (32 if ndim == 2 else 16))
     for s, L in enumerate(Ls):
         N = L ** ndim
         rep = oracle.EaReplay(oracle.ea_init(m, N, keys[s]), models[s].device_couplings(), ndim, L,
-                              1.0 / np.sort(temps)[::-1], keys[s], code=code)
+                              1.0 / np.sort(temps)[::-1], keys[s], code=code,
+                              multispin=eng.multispin)
         ref = rep.run((1 << n_bins) - 1, n_bins)
         ref2 = rep.run(7, 3)
         assert np.array_equal(eng.configurations(s), rep.spins)
