@@ -688,6 +688,26 @@ __device__ __forceinline__ void transpose32(uint32_t (&v)[32])
     }
 }
 
+// Transpose of 16 words as two 16 x 16 bit matrices side by side: bit j of
+// v[i] (j < 16) becomes bit i of v[j], bit 16 + j becomes bit 16 + i of v[j].
+// With v[i] the words of a row of at most 16 sites, v[c] & 0xFFFF is then
+// configuration c's row pattern and v[c] >> 16 that of configuration c + 16
+// (the 32 x 32 transpose without its first stage, whose halves are zero).
+__device__ __forceinline__ void transpose16x2(uint32_t (&v)[16])
+{
+    uint32_t msk = 0x00FF00FFu;
+#pragma unroll
+    for (int j = 8; j != 0; j >>= 1, msk ^= msk << j) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k & j) continue;
+            const uint32_t t = ((v[k] >> j) ^ v[k + j]) & msk;
+            v[k + j] ^= t;
+            v[k] ^= t << j;
+        }
+    }
+}
+
 constexpr int EA_MS_ROWS = 256;  // lattice rows (L^(d-1)) a measurement workgroup holds
 
 // Multi-spin measurement, one workgroup per sample, all temperatures.
@@ -700,11 +720,11 @@ constexpr int EA_MS_ROWS = 256;  // lattice rows (L^(d-1)) a measurement workgro
 // the patterns of the next rows (y, z); the x Fourier sum 2 sum_{bit x set}
 // e^{i 2 pi x / L} from nibble tables (sum_x e^{i 2 pi x / L} = 0); the row
 // sums go to per-plane sums M1[k][x1], M2[k][x2] for the y and z terms.
-template <int Z>
+template <int Z, bool ROW16>
 __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
 {
     constexpr int D = Z / 2;
-    __shared__ uint32_t QP[EA_MS_ROWS][32];
+    __shared__ uint32_t QP[EA_MS_ROWS][33];  // padded: lanes on different rows hit different banks
     __shared__ double ctab[EA_LMAX], stab[EA_LMAX];
     __shared__ double CT[8][16], ST[8][16];
     __shared__ int M1[32][32], M2[32][32];
@@ -750,7 +770,25 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
         ST[j][v] = sn;
     }
     // phase 1
-    for (uint32_t row = threadIdx.x; row < rows; row += blockDim.x) {
+    for (uint32_t row = threadIdx.x; row < rows && ROW16; row += blockDim.x) {
+        uint32_t v[16];
+#pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = (uint32_t)x < L32 ? wa[row * L32 + x] : 0u;
+        transpose16x2(v);
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (c < m) QP[row][kca[c]] = c < 16 ? v[c] & 0xFFFFu : v[c - 16] >> 16;
+#pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = (uint32_t)x < L32 ? wb[row * L32 + x] : 0u;
+        transpose16x2(v);
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (c < m) {
+                const uint32_t pb = c < 16 ? v[c] & 0xFFFFu : v[c - 16] >> 16;
+                QP[row][kcb[c]] = ~(QP[row][kcb[c]] ^ pb) & rowmask;
+            }
+    }
+    for (uint32_t row = threadIdx.x; row < rows && !ROW16; row += blockDim.x) {
         uint32_t v[32];
         // set 0: configuration c's row pattern is v[c]; place it at its temperature
 #pragma unroll
@@ -1145,12 +1183,13 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
                 if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
                 else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
-                hipLaunchKernelGGL((ea_measure_ms<4>), dim3(a.S), dim3(256), 0, st, a, t);
+                if (max_L <= 16) hipLaunchKernelGGL((ea_measure_ms<4, true>), dim3(a.S), dim3(256), 0, st, a, t);
+                else hipLaunchKernelGGL((ea_measure_ms<4, false>), dim3(a.S), dim3(256), 0, st, a, t);
             } else {
                 if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
                 else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
-                hipLaunchKernelGGL((ea_measure_ms<6>), dim3(a.S), dim3(256), 0, st, a, t);
+                hipLaunchKernelGGL((ea_measure_ms<6, true>), dim3(a.S), dim3(256), 0, st, a, t);
             }
         }
         hipLaunchKernelGGL(ea_ms_unpack, pgrid, dim3(256), 0, st, a);
