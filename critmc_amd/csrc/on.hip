@@ -313,6 +313,113 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
     }
 }
 
+// Over-relaxation sweeps of XY spins on a square lattice, K half sweeps
+// (colour 0, 1, 0, 1, ...) fused in LDS.  A workgroup owns rows
+// [y0, y0 + T) of one replica: it loads colour 0 on rows y0-(K-1) .. y0+T+K-2
+// and colour 1 on rows y0-K .. y0+T+K-1, runs the half sweeps on shrinking row
+// ranges (half sweep h covers the rows still needed by the later ones), and
+// writes back its own rows.  Every site update does the same operations in
+// the same order as on_half_sweep, so the result is bit-identical to K
+// separate launches, with about a third of their HBM traffic.
+template <int K>
+__global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T)
+{
+    extern __shared__ float2 orl[];
+    const int M = a.g.M, L1 = a.g.L1;
+    const int tiles = (L1 + T - 1) / T;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int rep = (int)(lb / (uint32_t)tiles), tile = (int)(lb - (uint32_t)rep * tiles);
+    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
+    const int y0 = tile * T;
+    const int Te = min(T, L1 - y0);
+    const int R0 = Te + 2 * (K - 1), R1 = Te + 2 * K;
+    float2* P0 = orl;
+    float2* P1 = orl + (size_t)R0 * M;
+    float2* S = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
+    float2* G0 = S;
+    float2* G1 = S + a.g.half;
+    const FastDiv& dM = a.g.divM;
+    for (int t = threadIdx.x; t < R0 * M; t += blockDim.x) {
+        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
+        int y = (y0 - (K - 1) + r) % L1;
+        y = y < 0 ? y + L1 : y;
+        P0[t] = G0[(int64_t)y * M + mm];
+    }
+    for (int t = threadIdx.x; t < R1 * M; t += blockDim.x) {
+        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
+        int y = (y0 - K + r) % L1;
+        y = y < 0 ? y + L1 : y;
+        P1[t] = G1[(int64_t)y * M + mm];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int hs = 0; hs < K; ++hs) {
+        const int c = hs & 1;
+        const int ext = K - 1 - hs;  // rows beyond the tile still needed later
+        const int nrows = Te + 2 * ext;
+        float2* Pc = c ? P1 : P0;
+        const float2* Po = c ? P0 : P1;
+        const int offc = c ? K : K - 1, offo = c ? K - 1 : K;  // local row of y0 in Pc / Po
+        const bool norm = a.norm_out && hs == K - 1;
+        for (int t = threadIdx.x; t < nrows * M; t += blockDim.x) {
+            const int rr = (int)fdiv((uint32_t)t, dM), mm = t - rr * M;
+            const int y = y0 - ext + rr;            // L1 is even, so parity needs no wrap
+            const int p = (y + c) & 1;              // own site at x0 = 2 mm + p
+            const int lc = rr - ext + offc, lo = rr - ext + offo;
+            const int mp = p == 0 ? mm : (mm == M - 1 ? 0 : mm + 1);  // x0 + 1
+            const int mn = p == 0 ? (mm == 0 ? M - 1 : mm - 1) : mm;  // x0 - 1
+            const float2 nb[4] = {Po[lo * M + mp], Po[lo * M + mn], Po[(lo + 1) * M + mm],
+                                  Po[(lo - 1) * M + mm]};
+            double hd0 = 0.0, hd1 = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                hd0 += (double)nb[j].x;
+                hd1 += (double)nb[j].y;
+            }
+            const float h0 = (float)hd0, h1 = (float)hd1;
+            float2 v = Pc[lc * M + mm];
+            float hh = 0.0f, sh = 0.0f;
+            hh += h0 * h0;
+            sh += v.x * h0;
+            hh += h1 * h1;
+            sh += v.y * h1;
+            if (hh != 0.0f) {
+                const float f = 2.0f * sh / hh;
+                v.x = f * h0 - v.x;
+                v.y = f * h1 - v.y;
+            }
+            if (norm) {
+                double n2 = 0.0;
+                n2 += (double)v.x * (double)v.x;
+                n2 += (double)v.y * (double)v.y;
+                const double nrm = sqrt(n2);
+                v.x = (float)((double)v.x / nrm);
+                v.y = (float)((double)v.y / nrm);
+            }
+            Pc[lc * M + mm] = v;
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < Te * M; t += blockDim.x) {
+        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
+        G0[(int64_t)(y0 + r) * M + mm] = P0[(K - 1 + r) * M + mm];
+        G1[(int64_t)(y0 + r) * M + mm] = P1[(K + r) * M + mm];
+    }
+}
+
+constexpr int ON_OR_FUSE = 4;                // half sweeps per fused launch (two sweeps)
+constexpr int64_t ON_OR_LDS = 76 * 1024;     // LDS per workgroup: two per CU
+
+// Rows per tile for the fused over-relaxation (0: the lattice does not qualify).
+static int on_or_tile_rows(const CbGeo& g, int n)
+{
+    if (n != 2 || g.d != 2 || g.M < 2) return 0;
+    const int64_t row = (int64_t)g.M * 8;
+    const int64_t T = (ON_OR_LDS / row - (4 * ON_OR_FUSE - 2)) / 2;
+    if (T < 4) return 0;
+    return (int)std::min<int64_t>(T, g.L1);
+}
+
 // Renormalise every spin in FP64 (critmc/kernels/on.py:78-87).
 template <int NC>
 __global__ void on_normalize(float* __restrict__ spins, int64_t total_sites, int64_t N,
@@ -1053,6 +1160,12 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     const int rb = (n_rep + 255) / 256;
     if (any_cluster)
         hipLaunchKernelGGL(on_cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, ca.stats, 2 * n_rep);
+    const int or_rows = on_or_tile_rows(g, n);
+    if (or_rows > 0) {
+        const size_t lds = (size_t)(2 * or_rows + 4 * ON_OR_FUSE - 2) * g.M * 8;
+        CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_or_fused<ON_OR_FUSE>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
         if (any_cluster) hipLaunchKernelGGL(on_cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
@@ -1065,15 +1178,28 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                 sw += (uint64_t)schedule[2 * r + 1];
                 continue;
             }
-            for (int64_t rp = 0; rp < schedule[2 * r + 1]; ++rp) {
+            const int64_t count = schedule[2 * r + 1];
+            for (int64_t rp = 0; rp < count;) {
                 a.sweep_add = sw;
+                if (code == CODE_OVERRELAX && or_rows > 0 && count - rp >= 2) {
+                    // two over-relaxation sweeps in one launch, tiles of rows in LDS
+                    a.norm_out = fused && r == n_rows - 1 && rp + 2 == count;
+                    const int tiles = (g.L1 + or_rows - 1) / or_rows;
+                    const size_t lds = (size_t)(2 * or_rows + 4 * ON_OR_FUSE - 2) * g.M * 8;
+                    hipLaunchKernelGGL((on_or_fused<ON_OR_FUSE>), dim3((unsigned)(tiles * n_rep)),
+                                       dim3(1024), lds, st, a, or_rows);
+                    sw += 2;
+                    rp += 2;
+                    continue;
+                }
                 for (int c = 0; c < 2; ++c) {
                     a.colour = c;
-                    a.norm_out = fused && r == n_rows - 1 && rp == schedule[2 * r + 1] - 1 && c == 1;
+                    a.norm_out = fused && r == n_rows - 1 && rp == count - 1 && c == 1;
                     if (n == 2) launch_half_z<2>(code, a, st);
                     else launch_half_z<3>(code, a, st);
                 }
                 ++sw;
+                ++rp;
             }
         }
         const int64_t sites = (int64_t)n_rep * g.N;

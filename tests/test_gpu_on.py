@@ -153,3 +153,28 @@ def test_split_layout_round_trip(n, shape):
     want[:, par * (N // 2) + np.arange(N) // 2] = x.cpu().numpy()
     assert np.array_equal(sp.cpu().numpy(), want)
     assert torch.equal(K.to_sites(sp, shape, n), x)
+
+
+@pytest.mark.parametrize("shape,R", [((512, 40), 2), ((64, 64), 3), ((8, 6), 2)],
+                         ids=["tiles-of-12-rows", "one-tile", "halo-wraps"])
+@pytest.mark.parametrize("fused,split", [([[5, 2]], [[5, 1], [5, 1]]),
+                                         ([[0, 1], [5, 2]], [[0, 1], [5, 1], [5, 1]])],
+                         ids=["or", "metropolis+2or"])
+def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
+    """Two over-relaxation sweeps in one launch (row tiles in LDS with halos)
+    give exactly the spins and series of four separate half-sweep launches."""
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(sum(shape) + R)
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    N = int(np.prod(shape))
+    betas, steps = np.linspace(0.9, 1.2, R), np.linspace(0.5, 1.5, R)
+    out = []
+    for sched in (fused, split):
+        chains = Chains(keys, np.arange(R), None, dev)
+        s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
+        K.init_spins(s, shape, 2, chains)
+        series = K.run(s, shape, 2, betas, steps, chains, np.array(sched), 3)
+        out.append((s.cpu().numpy(), [t.cpu().numpy() for t in series]))
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
