@@ -17,12 +17,18 @@
 // sweep counters are [S][2][m] (configuration-indexed except perm, which maps
 // temperature -> configuration).
 //
-// One PT step is three launches: ea_sweep (one workgroup per configuration,
-// spins staged in LDS), ea_swap (one 64-lane workgroup per (sample, set), chain in LDS), ea_measure
-// (one workgroup per (sample, temperature)).
+// One PT step is three launches.  Per-configuration path (Gaussian
+// couplings, or batches ea_use_ms rejects): ea_sweep (one workgroup per
+// configuration, spins staged in LDS), ea_swap (one 64-lane workgroup per
+// (sample, set), chain in LDS), ea_measure (one workgroup per (sample,
+// temperature)).  Multi-spin path (+-J, m <= 32): the configurations are
+// packed one bit each into words for the call (ea_ms_pack / ea_ms_unpack);
+// ea_ms_sweep (one workgroup per (sample, set)), ea_swap, ea_measure_ms (one
+// workgroup per sample).
 //
-// Uniforms (key = the sample's): a configuration's site k of colour c uses
-// word k&3 of block k>>2 at (sweep, c, TAG_EA, stream r*m + c); the swap
+// Uniforms (key = the sample's), per-configuration path: a configuration's
+// site k of colour c uses word k&3 of block k>>2 at (sweep, c, TAG_EA,
+// stream r*m + c).  Multi-spin path: see ea_ms_pack.  The swap
 // between temperatures k and k+1 of set r uses word k&3 of block k>>2 at
 // (step, 0, TAG_PT, stream r).  The swap test compares u = x 2^-32 with
 // det_exp(x), a fixed sequence of IEEE operations (Cody-Waite reduction,

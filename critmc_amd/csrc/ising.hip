@@ -23,6 +23,11 @@
 // a word at once and stops as soon as every lane is decided -- usually after
 // one or two Philox blocks instead of eight.  The early stop does not change
 // any decision, so the result equals the full 32-bit comparison.
+//
+// Forms: plain (one chain, one launch per half sweep), batched (one launch,
+// replicas resident in LDS; undecided words leave their wave for an LDS queue)
+// and wide (one launch per half sweep over all replicas, state in HBM).  All
+// three draw the same uniforms for the same word, so they run the same chains.
 #include <algorithm>
 #include <cmath>
 #include <mutex>

@@ -17,9 +17,14 @@
 // from and to site order).  A half sweep then reads and writes its own plane
 // contiguously, and its neighbours are all in the other plane at nb >> 1:
 // full cache lines instead of every other 8 bytes.  A half sweep is one launch
-// over all replicas (grid = colour-site chunks x replicas); measurements are a
-// normalise launch, a partial-sum launch and an in-order finalise launch, so
-// every FP64 result is reproducible run to run.
+// over all replicas (grid = colour-site chunks x replicas).  Two consecutive
+// over-relaxation sweeps of XY on a square lattice run as one launch over row
+// tiles in LDS (on_or_fused), bit-identical to four half-sweep launches.
+// When the schedule ends with a local update, that last half sweep writes its
+// sites renormalised and on_measure_c0 renormalises colour 0 while summing
+// every bond once; otherwise a normalise launch precedes the partial-sum
+// launch.  Partial sums are finalised in order, so every FP64 result is
+// reproducible run to run.
 //
 // Heat bath (:186-261), FP64 from the exact FP64 field:
 //   XY         phi = atan2(h) + von Mises(kappa = beta |h|) by Best & Fisher's
