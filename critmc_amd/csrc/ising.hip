@@ -33,6 +33,7 @@
 #include <mutex>
 #include <vector>
 
+#include "cb_geom.hpp"
 #include "common.hpp"
 #include "philox.hpp"
 #include "ising_core.hpp"
@@ -246,23 +247,23 @@ __device__ void store_replica(void* __restrict__ spins, int rep, int L0, int L1,
 // neighbour words (same position in the other colour, shifted by one bit
 // with the carry from the adjacent word, rows above and below).
 __device__ __forceinline__ void batched_word(const uint32_t* __restrict__ Sc,
-                                             const uint32_t* __restrict__ So, int idx, int G, int M,
-                                             int L1, int c, uint32_t valid, uint32_t& own,
-                                             uint32_t (&nb)[4])
+                                             const uint32_t* __restrict__ So, int idx, int y, int w,
+                                             int G, int W, int M, int L1, int c, uint32_t valid,
+                                             uint32_t& own, uint32_t (&nb)[4])
 {
-    const int y = idx / G, w = idx - (idx / G) * G;
     const int p = (y + c) & 1;
-    const int yp = y + 1 == L1 ? 0 : y + 1;
-    const int ym = y == 0 ? L1 - 1 : y - 1;
+    const int row = idx - w;  // y * G
+    const int rp = y + 1 == L1 ? 0 : row + G;
+    const int rm = y == 0 ? W - G : row - G;
     own = Sc[idx];
     const uint32_t same = So[idx];
     uint32_t sh;
     if (M >= 32) {
         if (p == 0) {
-            const uint32_t prev = So[y * G + (w == 0 ? G - 1 : w - 1)];
+            const uint32_t prev = So[row + (w == 0 ? G - 1 : w - 1)];
             sh = (same << 1) | (prev >> 31);
         } else {
-            const uint32_t next = So[y * G + (w == G - 1 ? 0 : w + 1)];
+            const uint32_t next = So[row + (w == G - 1 ? 0 : w + 1)];
             sh = (same >> 1) | (next << 31);
         }
     } else if (p == 0) {
@@ -272,8 +273,8 @@ __device__ __forceinline__ void batched_word(const uint32_t* __restrict__ Sc,
     }
     nb[0] = same;
     nb[1] = sh;
-    nb[2] = So[yp * G + w];
-    nb[3] = So[ym * G + w];
+    nb[2] = So[rp + w];
+    nb[3] = So[rm + w];
 }
 
 // Philox blocks a word may draw before it leaves the wave's lockstep.  A wave
@@ -305,6 +306,8 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     int* qcount = (int*)(queue + 3 * ISING_QCAP);  // [2], alternating half sweeps
     const int nwaves = (blockDim.x + 63) >> 6;
     const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
+    const FastDiv dG = make_fastdiv((uint32_t)G);
+    const int dy = (int)blockDim.x % G == 0 ? (int)blockDim.x / G : 0;  // rows per stride, 0: divide
     if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
     uint32_t hs = 0;
 
@@ -326,9 +329,16 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                         uint32_t* Sc = lds + c * W;
                         const uint32_t* So = lds + (1 - c) * W;
                         int* qc = qcount + (hs & 1);
+                        // rows step by blockDim / G when G divides blockDim: no division per word
+                        int y = (int)fdiv(threadIdx.x, dG), w = (int)threadIdx.x - y * G;
                         for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+                            if (dy == 0) {
+                                y = (int)fdiv((uint32_t)idx, dG);
+                                w = idx - y * G;
+                            }
                             uint32_t own, nb[4];
-                            batched_word(Sc, So, idx, G, M, L1, c, valid, own, nb);
+                            batched_word(Sc, So, idx, y, w, G, W, M, L1, c, valid, own, nb);
+                            y += dy;
                             WordDecision<NCMAX> wd;
                             word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
                             word_draw<NCMAX>(wd, 0, ISING_DRAW_BUDGET, k0, k1, sid, sweep,
@@ -351,8 +361,9 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                         if (threadIdx.x == 0) qcount[(hs + 1) & 1] = 0;
                         for (int t = threadIdx.x; t < nq; t += blockDim.x) {
                             const int idx = (int)queue[3 * t];
+                            const int qy = (int)fdiv((uint32_t)idx, dG);
                             uint32_t own, nb[4];
-                            batched_word(Sc, So, idx, G, M, L1, c, valid, own, nb);
+                            batched_word(Sc, So, idx, qy, idx - qy * G, G, W, M, L1, c, valid, own, nb);
                             WordDecision<NCMAX> wd;
                             word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
                             wd.pending = queue[3 * t + 1];
@@ -444,7 +455,7 @@ __global__ __launch_bounds__(64) void ising2d_half_wide(
     const int idx = (int)(bx * blockDim.x + threadIdx.x);
     Delta d{0, 0, 0};
     if (idx < W) {
-        const int y = idx / G, w = idx - (idx / G) * G;
+        const int y = (int)fdiv((uint32_t)idx, make_fastdiv((uint32_t)G)), w = idx - y * G;
         const int p = (y + c) & 1;
         const int yp = y + 1 == L1 ? 0 : y + 1;
         const int ym = y == 0 ? L1 - 1 : y - 1;
