@@ -22,8 +22,9 @@
 // tiles in LDS (on_or_fused), bit-identical to four half-sweep launches.
 // When the schedule ends with a local update, that last half sweep writes its
 // sites renormalised and on_measure_c0 renormalises colour 0 while summing
-// every bond once; otherwise a normalise launch precedes the partial-sum
-// launch.  Partial sums are finalised in order, so every FP64 result is
+// every bond once (when it ends with two fused over-relaxation sweeps, the
+// fused launch does both); otherwise a normalise launch precedes the
+// partial-sum launch.  Partial sums are finalised in order, so every FP64 result is
 // reproducible run to run.
 //
 // Heat bath (:186-261), FP64 from the exact FP64 field:
@@ -104,15 +105,67 @@ __device__ __forceinline__ void store_vec(float* __restrict__ p, int64_t i, cons
     }
 }
 
+// Renormalise in FP64 (critmc/kernels/on.py:78-87), rounded to FP32.  The
+// squares of the FP32 components are exact in FP64.  Near |S| = 1 (every spin
+// after a sweep: |S|^2 - 1 is of order 1e-6), 1/|S| is the series
+// 1 - d/2 + 3d^2/8 - 5d^3/16 + 35d^4/128 in d = |S|^2 - 1, whose remainder is
+// below 2^-60 for |d| <= 2^-12; farther from 1, the FP64 square root and
+// division.  Every renormalisation on the device goes through this function,
+// so fused and separate launches store the same bits.
+template <int NC>
+__device__ __forceinline__ Vec<NC> unit_vec(Vec<NC> s)
+{
+    double n2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
+    const double d = n2 - 1.0;
+    if (fabs(d) <= 0x1p-12) {
+        const double inv = fma(d, fma(d, fma(d, fma(d, 35.0 / 128.0, -5.0 / 16.0), 3.0 / 8.0), -0.5), 1.0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] * inv);
+    } else {
+        const double nrm = sqrt(n2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+    }
+    return s;
+}
+
+__device__ __forceinline__ float2 unit_xy(float2 v)
+{
+    Vec<2> t{{v.x, v.y}};
+    t = unit_vec<2>(t);
+    return make_float2(t.v[0], t.v[1]);
+}
+
+// Box-Muller pair from two Philox words: radius sqrt(-2 ln u1), u1 = (xa+1)/2^32
+// in (0, 1], from v_log_f32 (log2) and v_sqrt_f32, each within about 1 ulp;
+// angle 2 pi xb / 2^32 split into the nearest quadrant q (top bits of xb) and
+// a remainder theta in [-pi/4, pi/4) taken exactly from the low 30 bits, then
+// Taylor polynomials for sin and cos (remainders below 2e-9 on that range)
+// and a rotation by q quadrants.  Largest error against the exact angle
+// 2 pi xb / 2^32: 1e-7 over 2^24 angles, all quadrant edges included.
 __device__ __forceinline__ void box_muller(uint32_t xa, uint32_t xb, float& g0, float& g1)
 {
-    const float u1 = ((float)xa + 1.0f) * 0x1p-32f;  // (0, 1]
-    const float u2 = (float)xb * 0x1p-32f;           // [0, 1)
-    const float r = sqrtf(-2.0f * logf(u1));
-    float sn, cs;
-    sincospif(2.0f * u2, &sn, &cs);
-    g0 = r * cs;
-    g1 = r * sn;
+    const float u1 = ((float)xa + 1.0f) * 0x1p-32f;
+    const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u1));
+    const uint32_t sh = xb + 0x20000000u;
+    const uint32_t q = sh >> 30;
+    const int32_t rem = (int32_t)(sh & 0x3FFFFFFFu) - 0x20000000;
+    const float th = (float)rem * (6.28318530717958647692f * 0x1p-32f);
+    const float t2 = th * th;
+    const float ps = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f),
+                                   -1.0f / 6.0f), 0.0f);
+    const float sn = fmaf(th, ps, th);
+    const float cs = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -1.0f / 3628800.0f, 1.0f / 40320.0f),
+                                                          -1.0f / 720.0f), 1.0f / 24.0f), -0.5f), 1.0f);
+    float c = (q & 1) ? -sn : cs, s = (q & 1) ? cs : sn;
+    if (q & 2) {
+        c = -c;
+        s = -s;
+    }
+    g0 = r * c;
+    g1 = r * s;
 }
 
 __device__ __forceinline__ double unit_open0(uint32_t x) { return ((double)x + 1.0) * 0x1p-32; }
@@ -273,20 +326,24 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
             }
             const float step = a.steps[rep];
             float tr[NC];
-            float nrm = 0.0f;
+            float n2 = 0.0f;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 tr[c] = s.v[c] + step * g[c];
-                nrm += tr[c] * tr[c];
+                n2 += tr[c] * tr[c];
             }
-            nrm = sqrtf(nrm);
+            // v_rsq_f32 (1 ulp) scales the proposal: its direction is exact, its
+            // length is 1 within 2e-7 and renormalised at every measurement
+            const float inv = __builtin_amdgcn_rsqf(n2);
             float de = 0.0f;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                tr[c] /= nrm;
+                tr[c] *= inv;
                 de -= (tr[c] - s.v[c]) * h[c];
             }
-            if (de <= 0.0f || (float)xacc * 0x1p-32f < expf(-a.betas[rep] * de)) {
+            // v_exp_f32: a decision can differ from the FP64 rule only where u lies
+            // within a few ulp of the acceptance probability
+            if (de <= 0.0f || (float)xacc * 0x1p-32f < __expf(-a.betas[rep] * de)) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s.v[c] = tr[c];
                 changed = true;
@@ -295,12 +352,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
         }
         if (a.norm_out) {
             // the measurement's renormalisation of this site, fused into its last update
-            double n2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
-            const double nrm = sqrt(n2);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+            s = unit_vec<NC>(s);
             store_vec<NC>(S, i, s);
         } else if (changed) {
             store_vec<NC>(S, i, s);
@@ -326,8 +378,16 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 // writes back its own rows.  Every site update does the same operations in
 // the same order as on_half_sweep, so the result is bit-identical to K
 // separate launches, with about a third of their HBM traffic.
+//
+// With `part` set (the measurement's last launch, a.norm_out), the workgroup
+// also measures: colour 0 is final on rows y0-1 .. y0+T after half sweep K-2,
+// so those rows are renormalised in LDS (the values on_measure_c0 would
+// store), and each colour-1 site of the own rows takes its 2d bonds and M of
+// itself and of its -e_0 neighbour (a bijection onto colour 0).  The tile's
+// FP64 partial sums go to part[rep][tile][7]; on_measure_final adds the
+// tiles in order.
 template <int K>
-__global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T)
+__global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __restrict__ part)
 {
     extern __shared__ float2 orl[];
     const int M = a.g.M, L1 = a.g.L1;
@@ -394,16 +454,58 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T)
                 v.y = f * h1 - v.y;
             }
             if (norm) {
-                double n2 = 0.0;
-                n2 += (double)v.x * (double)v.x;
-                n2 += (double)v.y * (double)v.y;
-                const double nrm = sqrt(n2);
-                v.x = (float)((double)v.x / nrm);
-                v.y = (float)((double)v.y / nrm);
+                v = unit_xy(v);
             }
             Pc[lc * M + mm] = v;
         }
         __syncthreads();
+    }
+    if (part) {
+        float2* Q = P0 + (K - 2) * M;  // colour 0 from row y0 - 1
+        for (int t = threadIdx.x; t < (Te + 2) * M; t += blockDim.x) {
+            Q[t] = unit_xy(Q[t]);
+        }
+        __syncthreads();
+        double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // E, M[2], c_a[2], s_a[2]
+        for (int t = threadIdx.x; t < Te * M; t += blockDim.x) {
+            const int rr = (int)fdiv((uint32_t)t, dM), mm = t - rr * M;
+            const int p = (y0 + rr + 1) & 1;  // colour-1 site at x0 = 2 mm + p
+            const int lc = K + rr, lo = K - 1 + rr;
+            const int mp = p == 0 ? mm : (mm == M - 1 ? 0 : mm + 1);
+            const int mn = p == 0 ? (mm == 0 ? M - 1 : mm - 1) : mm;
+            const float2 s = P1[lc * M + mm];
+            // (forward neighbour, backward neighbour) per axis
+            const float2 nf[2] = {P0[lo * M + mp], P0[(lo + 1) * M + mm]};
+            const float2 nbk[2] = {P0[lo * M + mn], P0[(lo - 1) * M + mm]};
+            acc[1] += (double)s.x + (double)nbk[0].x;
+            acc[2] += (double)s.y + (double)nbk[0].y;
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax) {
+#pragma unroll
+                for (int dir = 0; dir < 2; ++dir) {
+                    // bond (A, B) = (j, j + e_ax) forward, (j - e_ax, j) backward
+                    const float2 A = dir == 0 ? s : nbk[ax];
+                    const float2 B = dir == 0 ? nf[ax] : s;
+                    const double dot = (double)A.x * B.x + (double)A.y * B.y;
+                    acc[0] -= dot;
+                    acc[3 + ax] += dot;
+                    acc[5 + ax] += (double)A.x * B.y - (double)A.y * B.x;
+                }
+            }
+        }
+        __shared__ double red[7 * 16];
+        const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const double t = wave_sum_f64(acc[q]);
+            if ((threadIdx.x & 63) == 0) red[q * 16 + w] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x < 7) {
+            double t = 0.0;
+            for (int u = 0; u < nw; ++u) t += red[threadIdx.x * 16 + u];
+            part[((int64_t)rep * tiles + tile) * 7 + threadIdx.x] = t;
+        }
     }
     for (int t = threadIdx.x; t < Te * M; t += blockDim.x) {
         const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
@@ -433,13 +535,7 @@ __global__ void on_normalize(float* __restrict__ spins, int64_t total_sites, int
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total_sites;
          t += (int64_t)gridDim.x * blockDim.x) {
         if (n_meas_rep && meas >= n_meas_rep[t / N]) continue;
-        Vec<NC> s = load_vec<NC>(spins, t);
-        double n2 = 0.0;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
-        const double nrm = sqrt(n2);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+        const Vec<NC> s = unit_vec<NC>(load_vec<NC>(spins, t));
         store_vec<NC>(spins, t, s);
     }
 }
@@ -535,13 +631,7 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         int64_t nb[Z];
         (void)cb_site<Z>(g, k, 0, nb);
-        Vec<NC> s = load_vec<NC>(S, k);
-        double n2 = 0.0;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) n2 += (double)s.v[c] * (double)s.v[c];
-        const double nrm = sqrt(n2);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) s.v[c] = (float)((double)s.v[c] / nrm);
+        const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S, k));
         store_vec<NC>(S, k, s);
         const Vec<NC> xr = load_vec<NC>(S, P1 + (nb[0] >> 1));  // i + e_0
 #pragma unroll
@@ -1173,6 +1263,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
     }
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
+        int meas_tiles = 0;  // > 0: the fused over-relaxation launch measured
         if (any_cluster) hipLaunchKernelGGL(on_cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
         for (int r = 0; r < n_rows; ++r) {
             const int code = (int)schedule[2 * r];
@@ -1191,8 +1282,12 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                     a.norm_out = fused && r == n_rows - 1 && rp + 2 == count;
                     const int tiles = (g.L1 + or_rows - 1) / or_rows;
                     const size_t lds = (size_t)(2 * or_rows + 4 * ON_OR_FUSE - 2) * g.M * 8;
+                    // the measurement's last launch also measures, one partial per tile
+                    // (the workspace holds nchunk >= tiles partials per replica)
+                    double* tpart = a.norm_out && tiles <= nchunk ? part : nullptr;
+                    if (tpart) meas_tiles = tiles;
                     hipLaunchKernelGGL((on_or_fused<ON_OR_FUSE>), dim3((unsigned)(tiles * n_rep)),
-                                       dim3(1024), lds, st, a, or_rows);
+                                       dim3(1024), lds, st, a, or_rows, tpart);
                     sw += 2;
                     rp += 2;
                     continue;
@@ -1209,7 +1304,10 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         }
         const int64_t sites = (int64_t)n_rep * g.N;
         const int nb = (int)std::min<int64_t>((sites + 255) / 256, 8192);
-        if (fused) {
+        if (meas_tiles > 0) {
+            hipLaunchKernelGGL((on_measure_final<2, 2>), dim3((n_rep * 7 + 255) / 256), dim3(256), 0, st,
+                               part, meas_tiles, n_rep, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep);
+        } else if (fused) {
             if (n == 2) launch_measure_c0<2>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
             else launch_measure_c0<3>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         } else if (n == 2) {

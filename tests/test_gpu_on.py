@@ -162,7 +162,10 @@ def test_split_layout_round_trip(n, shape):
                          ids=["or", "metropolis+2or"])
 def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
     """Two over-relaxation sweeps in one launch (row tiles in LDS with halos)
-    give exactly the spins and series of four separate half-sweep launches."""
+    give exactly the spins and acceptance counts of four separate half-sweep
+    launches.  The fused launch that ends a measurement also measures, adding
+    the same FP64 bond terms per tile instead of per chunk of sites, so E, M
+    and the stiffness sums agree to FP64 rounding."""
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(sum(shape) + R)
     keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
@@ -176,5 +179,6 @@ def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
         series = K.run(s, shape, 2, betas, steps, chains, np.array(sched), 3)
         out.append((s.cpu().numpy(), [t.cpu().numpy() for t in series]))
     assert np.array_equal(out[0][0], out[1][0])
-    for a, b in zip(out[0][1], out[1][1]):
-        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1][0], out[1][1][0])  # accepted
+    for a, b in zip(out[0][1][1:], out[1][1][1:]):  # E, M, c_a, s_a
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * N)
