@@ -108,6 +108,38 @@ __host__ __device__ __forceinline__ uint32_t cb_site32(const CbGeo& g, uint32_t 
     return base + (uint32_t)x0;
 }
 
+// Offsets in the other colour's plane of the neighbours of colour-c site k,
+// in the same column order: nbo[j] = nb[j] >> 1 for cb_site's nb.  Site
+// (x0, r) sits at r M + (x0 >> 1) of its plane, so the +-x neighbours are in
+// row r at column m or m -+ 1 (by the parity of x0, with the wrap), and the
+// +-y / +-z neighbours at k +- M / k +- L1 M (with the wraps).  Needs
+// k < 2^31 and one FastDiv per axis beyond x, no multiplies.
+template <int Z>
+__host__ __device__ __forceinline__ void cb_other(const CbGeo& g, uint32_t k, int c, uint32_t (&nbo)[Z])
+{
+    const uint32_t M = (uint32_t)g.M;
+    const uint32_t r = fdiv(k, g.divM);
+    const uint32_t m = k - r * M;
+    uint32_t x1 = 0, x2 = 0;
+    if (Z >= 4) {
+        x2 = Z == 6 ? fdiv(r, g.divL1) : 0;
+        x1 = r - x2 * (uint32_t)g.L1;
+    }
+    const bool odd = ((x1 + x2 + (uint32_t)c) & 1) != 0;  // x0 = 2m + 1
+    nbo[0] = odd ? (m == M - 1 ? k - (M - 1) : k + 1) : k;
+    nbo[1] = odd ? k : (m == 0 ? k + (M - 1) : k - 1);
+    if (Z >= 4) {
+        const uint32_t wrap1 = ((uint32_t)g.L1 - 1) * M;
+        nbo[2] = x1 + 1 == (uint32_t)g.L1 ? k - wrap1 : k + M;
+        nbo[3] = x1 == 0 ? k + wrap1 : k - M;
+    }
+    if (Z == 6) {
+        const uint32_t plane = (uint32_t)g.L1 * M, wrap2 = ((uint32_t)g.L2 - 1) * plane;
+        nbo[4] = x2 + 1 == (uint32_t)g.L2 ? k - wrap2 : k + plane;
+        nbo[5] = x2 == 0 ? k + wrap2 : k - plane;
+    }
+}
+
 inline int cb_geo_from(int ndim, const int32_t* dims, CbGeo* g)
 {
     if (ndim < 1 || ndim > 3) return -1;

@@ -138,6 +138,29 @@ __device__ __forceinline__ float2 unit_xy(float2 v)
     return make_float2(t.v[0], t.v[1]);
 }
 
+// Over-relaxation S' = f h - S with f = 2 (S.h) / |h|^2, S unchanged when
+// |h|^2 = 0 (critmc/kernels/on.py:140-183).  FP32; 1/|h|^2 from v_rcp_f32
+// (1 ulp) for a normal |h|^2, the IEEE division below that.  Both the
+// per-half-sweep kernel and the fused LDS kernel use it, so they agree bit
+// for bit.
+template <int NC>
+__device__ __forceinline__ bool reflect(Vec<NC>& s, const float (&h)[NC])
+{
+    float hh = h[0] * h[0], sh = s.v[0] * h[0];
+#pragma unroll
+    for (int c = 1; c < NC; ++c) {
+        hh += h[c] * h[c];
+        sh += s.v[c] * h[c];
+    }
+    if (hh == 0.0f) return false;
+    float f;
+    if (hh >= 0x1p-126f) f = 2.0f * sh * __builtin_amdgcn_rcpf(hh);
+    else f = 2.0f * sh / hh;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s.v[c] = f * h[c] - s.v[c];
+    return true;
+}
+
 // Box-Muller pair from two Philox words: radius sqrt(-2 ln u1), u1 = (xa+1)/2^32
 // in (0, 1], from v_log_f32 (log2) and v_sqrt_f32, each within about 1 ulp;
 // angle 2 pi xb / 2^32 split into the nearest quadrant q (top bits of xb) and
@@ -226,21 +249,19 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
     int accepted = 0;
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
-        int64_t nb[Z];
-        (void)cb_site<Z>(a.g, k, a.colour, nb);
+        uint32_t nbo[Z];
+        cb_other<Z>(a.g, (uint32_t)k, a.colour, nbo);
         const int64_t i = (int64_t)a.colour * a.g.half + k;            // own plane
-        const int64_t other = (int64_t)(1 - a.colour) * a.g.half;     // neighbours' plane
+        const float* So = S + (int64_t)(1 - a.colour) * a.g.half * NC;  // neighbours' plane
         // The local field is summed in FP64: its Z FP32 terms are exact there, so
         // a nearly cancelling field keeps full relative precision (in FP32 the
         // sum's absolute error eps * Z would dominate when |h| -> 0).
         double hd[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) hd[c] = 0.0;
-#pragma unroll
         for (int j = 0; j < Z; ++j) {
-            const Vec<NC> t = load_vec<NC>(S, other + (nb[j] >> 1));
+            const Vec<NC> t = load_vec<NC>(So, nbo[j]);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) hd[c] += (double)t.v[c];
+            for (int c = 0; c < NC; ++c) hd[c] = j == 0 ? (double)t.v[c] : hd[c] + (double)t.v[c];
         }
         float h[NC];
 #pragma unroll
@@ -295,18 +316,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
             }
             changed = true;
         } else if (CODE == CODE_OVERRELAX) {
-            float hh = 0.0f, sh = 0.0f;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                hh += h[c] * h[c];
-                sh += s.v[c] * h[c];
-            }
-            if (hh != 0.0f) {
-                const float f = 2.0f * sh / hh;
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s.v[c] = f * h[c] - s.v[c];
-                changed = true;
-            }
+            changed = reflect<NC>(s, h);
         } else {
             const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
             const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
@@ -403,18 +413,25 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
     float2* S = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
     float2* G0 = S;
     float2* G1 = S + a.g.half;
-    const FastDiv& dM = a.g.divM;
-    for (int t = threadIdx.x; t < R0 * M; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
-        int y = (y0 - (K - 1) + r) % L1;
-        y = y < 0 ? y + L1 : y;
-        P0[t] = G0[(int64_t)y * M + mm];
-    }
-    for (int t = threadIdx.x; t < R1 * M; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
-        int y = (y0 - K + r) % L1;
-        y = y < 0 ? y + L1 : y;
-        P1[t] = G1[(int64_t)y * M + mm];
+    // Each thread keeps one column mm and visits rows tr0, tr0 + bq, ...: the
+    // workgroup covers bq whole rows per pass (threads beyond bq rows idle),
+    // so LDS offsets advance by a constant and no per-site division is left.
+    const int bq = (int)fdiv(blockDim.x, a.g.divM);
+    const int tr0 = (int)fdiv(threadIdx.x, a.g.divM);
+    const int mm = (int)threadIdx.x - tr0 * M;
+    const bool act = tr0 < bq;
+    const int stride = bq * M;
+    const int dp1 = mm == M - 1 ? -(M - 1) : 1;  // x0 + 1 in the other plane, x0 odd
+    const int dn0 = mm == 0 ? M - 1 : -1;        // x0 - 1 in the other plane, x0 even
+    // lattice row of LDS row r (the halo rows wrap; at most once when L1 >= K)
+    auto wrap = [&](int y) {
+        if (L1 >= K) return y < 0 ? y + L1 : (y >= L1 ? y - L1 : y);
+        const int q = y % L1;
+        return q < 0 ? q + L1 : q;
+    };
+    if (act) {
+        for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = G0[(int64_t)wrap(y0 - (K - 1) + r) * M + mm];
+        for (int r = tr0; r < R1; r += bq) P1[r * M + mm] = G1[(int64_t)wrap(y0 - K + r) * M + mm];
     }
     __syncthreads();
 #pragma unroll
@@ -426,71 +443,59 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
         const float2* Po = c ? P0 : P1;
         const int offc = c ? K : K - 1, offo = c ? K - 1 : K;  // local row of y0 in Pc / Po
         const bool norm = a.norm_out && hs == K - 1;
-        for (int t = threadIdx.x; t < nrows * M; t += blockDim.x) {
-            const int rr = (int)fdiv((uint32_t)t, dM), mm = t - rr * M;
-            const int y = y0 - ext + rr;            // L1 is even, so parity needs no wrap
-            const int p = (y + c) & 1;              // own site at x0 = 2 mm + p
-            const int lc = rr - ext + offc, lo = rr - ext + offo;
-            const int mp = p == 0 ? mm : (mm == M - 1 ? 0 : mm + 1);  // x0 + 1
-            const int mn = p == 0 ? (mm == 0 ? M - 1 : mm - 1) : mm;  // x0 - 1
-            const float2 nb[4] = {Po[lo * M + mp], Po[lo * M + mn], Po[(lo + 1) * M + mm],
-                                  Po[(lo - 1) * M + mm]};
-            double hd0 = 0.0, hd1 = 0.0;
+        if (act) {
+            int io = (tr0 - ext + offo) * M + mm;  // this site's row and column in Po
+            const int dc = (offc - offo) * M;      // ... and in Pc
+            const int pb = y0 - ext + c;           // L1 is even, so parity needs no wrap
+            for (int rr = tr0; rr < nrows; rr += bq, io += stride) {
+                const bool odd = ((rr + pb) & 1) != 0;  // own site at x0 = 2 mm + 1
+                const float2 nb[4] = {Po[io + (odd ? dp1 : 0)], Po[io + (odd ? 0 : dn0)], Po[io + M],
+                                      Po[io - M]};
+                double hd0 = (double)nb[0].x, hd1 = (double)nb[0].y;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                hd0 += (double)nb[j].x;
-                hd1 += (double)nb[j].y;
+                for (int j = 1; j < 4; ++j) {
+                    hd0 += (double)nb[j].x;
+                    hd1 += (double)nb[j].y;
+                }
+                const float h[2] = {(float)hd0, (float)hd1};
+                const float2 v0 = Pc[io + dc];
+                Vec<2> v{{v0.x, v0.y}};
+                (void)reflect<2>(v, h);
+                float2 out = make_float2(v.v[0], v.v[1]);
+                if (norm) out = unit_xy(out);
+                Pc[io + dc] = out;
             }
-            const float h0 = (float)hd0, h1 = (float)hd1;
-            float2 v = Pc[lc * M + mm];
-            float hh = 0.0f, sh = 0.0f;
-            hh += h0 * h0;
-            sh += v.x * h0;
-            hh += h1 * h1;
-            sh += v.y * h1;
-            if (hh != 0.0f) {
-                const float f = 2.0f * sh / hh;
-                v.x = f * h0 - v.x;
-                v.y = f * h1 - v.y;
-            }
-            if (norm) {
-                v = unit_xy(v);
-            }
-            Pc[lc * M + mm] = v;
         }
         __syncthreads();
     }
     if (part) {
         float2* Q = P0 + (K - 2) * M;  // colour 0 from row y0 - 1
-        for (int t = threadIdx.x; t < (Te + 2) * M; t += blockDim.x) {
-            Q[t] = unit_xy(Q[t]);
-        }
+        if (act)
+            for (int r = tr0; r < Te + 2; r += bq) Q[r * M + mm] = unit_xy(Q[r * M + mm]);
         __syncthreads();
         double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // E, M[2], c_a[2], s_a[2]
-        for (int t = threadIdx.x; t < Te * M; t += blockDim.x) {
-            const int rr = (int)fdiv((uint32_t)t, dM), mm = t - rr * M;
-            const int p = (y0 + rr + 1) & 1;  // colour-1 site at x0 = 2 mm + p
-            const int lc = K + rr, lo = K - 1 + rr;
-            const int mp = p == 0 ? mm : (mm == M - 1 ? 0 : mm + 1);
-            const int mn = p == 0 ? (mm == 0 ? M - 1 : mm - 1) : mm;
-            const float2 s = P1[lc * M + mm];
-            // (forward neighbour, backward neighbour) per axis
-            const float2 nf[2] = {P0[lo * M + mp], P0[(lo + 1) * M + mm]};
-            const float2 nbk[2] = {P0[lo * M + mn], P0[(lo - 1) * M + mm]};
-            acc[1] += (double)s.x + (double)nbk[0].x;
-            acc[2] += (double)s.y + (double)nbk[0].y;
+        if (act) {
+            int io = (K - 1 + tr0) * M + mm;  // P0 row and column of the site; P1 is one row on
+            for (int rr = tr0; rr < Te; rr += bq, io += stride) {
+                const bool odd = ((y0 + rr + 1) & 1) != 0;  // colour-1 site at x0 = 2 mm + 1
+                const float2 s = P1[io + M];
+                // forward (j + e_a) and backward (j - e_a) neighbours per axis
+                const float2 fw[2] = {P0[io + (odd ? dp1 : 0)], P0[io + M]};
+                const float2 bw[2] = {P0[io + (odd ? 0 : dn0)], P0[io - M]};
+                const double sx = s.x, sy = s.y;
+                acc[1] += sx + (double)bw[0].x;
+                acc[2] += sy + (double)bw[0].y;
+                double cs = 0.0;
 #pragma unroll
-            for (int ax = 0; ax < 2; ++ax) {
-#pragma unroll
-                for (int dir = 0; dir < 2; ++dir) {
-                    // bond (A, B) = (j, j + e_ax) forward, (j - e_ax, j) backward
-                    const float2 A = dir == 0 ? s : nbk[ax];
-                    const float2 B = dir == 0 ? nf[ax] : s;
-                    const double dot = (double)A.x * B.x + (double)A.y * B.y;
-                    acc[0] -= dot;
-                    acc[3 + ax] += dot;
-                    acc[5 + ax] += (double)A.x * B.y - (double)A.y * B.x;
+                for (int ax = 0; ax < 2; ++ax) {
+                    // bonds (j, j + e_a) and (j - e_a, j), oriented as the reference's
+                    const double fx = fw[ax].x, fy = fw[ax].y, bx = bw[ax].x, by = bw[ax].y;
+                    const double cax = sx * (fx + bx) + sy * (fy + by);
+                    acc[3 + ax] += cax;
+                    acc[5 + ax] += sx * (fy - by) - sy * (fx - bx);
+                    cs += cax;
                 }
+                acc[0] -= cs;
             }
         }
         __shared__ double red[7 * 16];
@@ -507,11 +512,11 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
             part[((int64_t)rep * tiles + tile) * 7 + threadIdx.x] = t;
         }
     }
-    for (int t = threadIdx.x; t < Te * M; t += blockDim.x) {
-        const int r = (int)fdiv((uint32_t)t, dM), mm = t - r * M;
-        G0[(int64_t)(y0 + r) * M + mm] = P0[(K - 1 + r) * M + mm];
-        G1[(int64_t)(y0 + r) * M + mm] = P1[(K + r) * M + mm];
-    }
+    if (act)
+        for (int r = tr0; r < Te; r += bq) {
+            G0[(int64_t)(y0 + r) * M + mm] = P0[(K - 1 + r) * M + mm];
+            G1[(int64_t)(y0 + r) * M + mm] = P1[(K + r) * M + mm];
+        }
 }
 
 constexpr int ON_OR_FUSE = 4;                // half sweeps per fused launch (two sweeps)
@@ -629,18 +634,18 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     const int64_t k0 = (int64_t)chunk * ON_CHUNK;
     const int64_t k1 = min(g.half, k0 + ON_CHUNK);
     for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        int64_t nb[Z];
-        (void)cb_site<Z>(g, k, 0, nb);
+        uint32_t nbo[Z];
+        cb_other<Z>(g, (uint32_t)k, 0, nbo);
         const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S, k));
         store_vec<NC>(S, k, s);
-        const Vec<NC> xr = load_vec<NC>(S, P1 + (nb[0] >> 1));  // i + e_0
+        const Vec<NC> xr = load_vec<NC>(S, P1 + nbo[0]);  // i + e_0
 #pragma unroll
         for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c] + (double)xr.v[c];
 #pragma unroll
         for (int ax = 0; ax < D; ++ax) {
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                const Vec<NC> t = load_vec<NC>(S, P1 + (nb[2 * ax + dir] >> 1));
+                const Vec<NC> t = load_vec<NC>(S, P1 + nbo[2 * ax + dir]);
                 // bond (a, b) = (i, i+e_ax) forward, (i-e_ax, i) backward
                 const Vec<NC>& A = dir == 0 ? s : t;
                 const Vec<NC>& B = dir == 0 ? t : s;
@@ -678,8 +683,18 @@ __global__ void on_measure_final(const double* __restrict__ part, int nchunk, in
     if (t >= R * K) return;
     const int rep = t / K, q = t - (t / K) * K;
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;
+    // in chunk order; the loads of eight chunks are issued before their adds
+    const double* pp = part + (int64_t)rep * nchunk * K + q;
     double s = 0.0;
-    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)rep * nchunk + c) * K + q];
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pp[(int64_t)(c + u) * K];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < nchunk; ++c) s += pp[(int64_t)c * K];
     const int64_t o = (int64_t)rep * n_meas + meas;
     if (q == 0) e_out[o] = s;
     else if (q < 1 + NC) m_out[o * NC + (q - 1)] = s;

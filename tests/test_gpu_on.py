@@ -116,7 +116,9 @@ def test_ring_matches_transfer_operator(n_comp, updates):
 def test_cluster_update_matches_replay(n, shape, sched):
     """Cluster decisions use FP64 projections of the FP32 spins and the fixed-operation
     exp, so from an identical state the clusters (aux: Wolff sizes, sum |C|^2, cluster
-    count) agree exactly and the reflected spins to FP32 rounding."""
+    count) agree exactly and the reflected spins to FP32 rounding.  A schedule that
+    starts with over-relaxation carries its conditioning (angle_error * |h| < 1e-5,
+    as in the one-sweep test) into the clusters, whose reflections keep angles."""
     dev = torch.device("cuda", 0)
     R = 2
     rng = np.random.default_rng(n + len(shape) + len(sched))
@@ -136,7 +138,17 @@ def test_cluster_update_matches_replay(n, shape, sched):
         ref = rep.run(sched, 1)
         assert np.array_equal(aux[r], ref["aux"]), (aux[r], ref["aux"])
         err = _angle_err(got[r], rep.s)
-        assert err.max() < 1e-5, err.max()
+        if sched[0][0] == 5:
+            from critmc_amd import hypercubic
+
+            pre = oracle.OnReplay(s0[r], shape, float(betas[r]), 1.0, keys[r], [5, 6][r])
+            pre.run([[5, 1]], 1)  # the state after the over-relaxation row
+            hmag = np.linalg.norm(pre.s[hypercubic(shape).nbr].sum(axis=1), axis=1)
+            bad = err > 1e-5
+            assert err.max() < 1e-3 and np.all(err[bad] * hmag[bad] < 1e-5), \
+                (bad.sum(), err.max(), (err * hmag)[bad].max() if bad.any() else 0.0)
+        else:
+            assert err.max() < 1e-5, err.max()
         assert float(e[r, 0]) == pytest.approx(ref["E"][0], rel=1e-6, abs=1e-5)
 
 
