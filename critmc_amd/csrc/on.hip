@@ -520,7 +520,10 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
 }
 
 constexpr int ON_OR_FUSE = 4;                // half sweeps per fused launch (two sweeps)
-constexpr int64_t ON_OR_LDS = 76 * 1024;     // LDS per workgroup: two per CU
+#ifndef CM_ON_OR_LDS_KB
+#define CM_ON_OR_LDS_KB 76
+#endif
+constexpr int64_t ON_OR_LDS = CM_ON_OR_LDS_KB * 1024;  // LDS per workgroup (76 KB: two per CU)
 
 // Rows per tile for the fused over-relaxation (0: the lattice does not qualify).
 static int on_or_tile_rows(const CbGeo& g, int n)
