@@ -132,9 +132,12 @@ __host__ __device__ __forceinline__ void word_draw(WordDecision<NCMAX>& w, uint3
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const uint32_t shift = 31u - (4u * q + (uint32_t)t);
+            // class k's threshold bit as an all-ones / all-zeros mask, uniform
+            // across the wave: tb takes one AND-OR per class
             uint32_t tb = 0;
 #pragma unroll
-            for (int k = 0; k < NCMAX; ++k) tb |= w.mask[k] & (0u - ((w.thr[k] >> shift) & 1u));
+            for (int k = 0; k < NCMAX; ++k)
+                tb |= w.mask[k] & (uint32_t)((int32_t)(w.thr[k] << (31u - shift)) >> 31);
             const uint32_t diff = w.pending & (planes[t] ^ tb);
             w.below |= diff & tb;
             w.pending &= ~diff;
