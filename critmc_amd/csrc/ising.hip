@@ -259,13 +259,11 @@ __device__ __forceinline__ void batched_word(const uint32_t* __restrict__ Sc,
     const uint32_t same = So[idx];
     uint32_t sh;
     if (M >= 32) {
-        if (p == 0) {
-            const uint32_t prev = So[row + (w == 0 ? G - 1 : w - 1)];
-            sh = (same << 1) | (prev >> 31);
-        } else {
-            const uint32_t next = So[row + (w == G - 1 ? 0 : w + 1)];
-            sh = (same >> 1) | (next << 31);
-        }
+        // the word before (p = 0) or after (p = 1) supplies the bit shifted in;
+        // lanes of both parities share a wave, so no branch
+        const int wn = p == 0 ? (w == 0 ? G - 1 : w - 1) : (w == G - 1 ? 0 : w + 1);
+        const uint32_t o = So[row + wn];
+        sh = p == 0 ? (same << 1) | (o >> 31) : (same >> 1) | (o << 31);
     } else if (p == 0) {
         sh = ((same << 1) | (same >> (M - 1))) & valid;
     } else {
@@ -322,6 +320,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
 
         for (int64_t meas = 0; meas < n_rep_meas; ++meas) {
             Delta d{0, 0, 0};
+            FlipCounts fc{0, 0, 0, 0, 0};
             for (int row = 0; row < sched.n; ++row) {
                 const int code = sched.code[row];
                 for (int rp = 0; rp < sched.count[row]; ++rp) {
@@ -354,7 +353,8 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                                 word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                                  (uint32_t)c, (uint32_t)idx);
                             }
-                            Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
+                            Sc[idx] = code == CODE_METROPOLIS ? word_apply_metropolis<NCMAX>(own, valid, wd, fc)
+                                                              : word_apply<4, NCMAX>(code, own, valid, wd, d);
                         }
                         __syncthreads();
                         const int nq = min(*qc, ISING_QCAP);
@@ -370,13 +370,15 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                             wd.below = queue[3 * t + 2];
                             word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                              (uint32_t)c, (uint32_t)idx);
-                            Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
+                            Sc[idx] = code == CODE_METROPOLIS ? word_apply_metropolis<NCMAX>(own, valid, wd, fc)
+                                                              : word_apply<4, NCMAX>(code, own, valid, wd, d);
                         }
                         __syncthreads();
                     }
                     ++sweep;
                 }
             }
+            fold_counts<4>(fc, d);
             const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
             if ((threadIdx.x & 63) == 0) {
                 red[3 * (threadIdx.x >> 6) + 0] = de;

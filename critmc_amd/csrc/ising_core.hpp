@@ -169,6 +169,37 @@ __host__ __device__ __forceinline__ uint32_t word_apply(int code, uint32_t own, 
     return own ^ X;
 }
 
+// Metropolis flips counted raw, per lane, and folded into a Delta once per
+// measurement (word_apply's deltas are linear in these counts): n_b =
+// popc(F & c_b) for the count bits, nf = popc(F), no = popc(F & own), so
+// de = 4 (n0 + 2 n1 + 4 n2) - 2 Z nf and dm = -2 (2 no - nf).
+struct FlipCounts {
+    int n0, n1, n2, nf, no;
+};
+
+template <int NCMAX>
+__host__ __device__ __forceinline__ uint32_t word_apply_metropolis(uint32_t own, uint32_t valid,
+                                                                   const WordDecision<NCMAX>& w,
+                                                                   FlipCounts& f)
+{
+    const uint32_t F = w.below & valid;
+    f.nf += popc(F);
+    f.n0 += popc(F & w.c0);
+    f.n1 += popc(F & w.c1);
+    f.n2 += popc(F & w.c2);
+    f.no += popc(F & own);
+    return own ^ F;
+}
+
+template <int Z>
+__host__ __device__ __forceinline__ void fold_counts(FlipCounts& f, Delta& d)
+{
+    d.de += 4 * (f.n0 + 2 * f.n1 + 4 * f.n2) - 2 * Z * f.nf;
+    d.dm -= 2 * (2 * f.no - f.nf);
+    d.acc += f.nf;
+    f = FlipCounts{0, 0, 0, 0, 0};
+}
+
 // One word of 32 same-colour sites; returns the new spins and accumulates
 // deltas.  "u < p" for the site at bit b is "x < T" with x the 32-bit uniform
 // whose bit 31-j is bit b of plane j; the comparison runs plane by plane.
