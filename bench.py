@@ -353,7 +353,12 @@ def main():
                    "record": "int32 E_b, M, accepted per replica per sweep"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic,
+                     "basis": f"{bytes_per_update:g} B per site update (each spin read and written "
+                              "once per sweep, the streaming bound) over the mean launch time; "
+                              "each replica stays in LDS for every sweep of a launch, so the HBM "
+                              "bytes actually moved are 'traffic' and VALU issue is the "
+                              "kernel's binding limit"},
         "plain_one_chain_rate": plain_rate,
         "vs_plain_one_chain": (value / plain_rate) if plain_rate else None,
         "wall_s": wall,
