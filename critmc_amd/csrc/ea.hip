@@ -736,7 +736,7 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
     __shared__ int M1[32][32], M2[32][32];
     __shared__ int pca[32], pcb[32], kca[32], kcb[32];
     const int m = a.m;
-    const int s = blockIdx.x;
+    const int s = a.ms_order[2 * blockIdx.x] >> 1;  // largest samples first
     const int L = a.L[s];
     const uint32_t L32 = (uint32_t)L;
     const uint32_t N = L32 * (D >= 2 ? L32 : 1u) * (D >= 3 ? L32 : 1u);
@@ -815,7 +815,7 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
     // phase 2: lane = 8 k + j
     const int k = threadIdx.x >> 3, j = threadIdx.x & 7;
     int q = 0, ql = 0;
-    double re0 = 0.0, im0 = 0.0;
+    double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;  // two chains: even and odd nibbles
     if (k < m) {
         const int nib = (L + 3) >> 2;
         for (uint32_t row = j; row < rows; row += 8) {
@@ -835,13 +835,20 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
                 ql += 2 * __popc(~(qp ^ QP[rz][k]) & rowmask) - L;
                 atomicAdd(&M2[k][x2], rs);
             }
-            for (int t = 0; t < nib; ++t) {
+            for (int t = 0; t < nib; t += 2) {
                 const uint32_t v = (qp >> (4 * t)) & 15u;
                 re0 += CT[t][v];
                 im0 += ST[t][v];
+                if (t + 1 < nib) {
+                    const uint32_t w = (qp >> (4 * t + 4)) & 15u;
+                    re1 += CT[t + 1][w];
+                    im1 += ST[t + 1][w];
+                }
             }
         }
     }
+    re0 += re1;
+    im0 += im1;
     // reduce the eight lanes of each temperature (fixed order)
 #pragma unroll
     for (int off = 1; off < 8; off <<= 1) {
