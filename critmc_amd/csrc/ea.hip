@@ -523,17 +523,47 @@ __global__ __launch_bounds__(64) void ea_swap(EaArgs a, uint64_t step)
         if (4 * b + 2 < EA_MMAX) sx[4 * b + 2] = rn.z;
         if (4 * b + 3 < EA_MMAX) sx[4 * b + 3] = rn.w;
     }
+    // m <= 32: the swaps are sequential, but the configuration at slot k when
+    // pair (k, k+1) is tried can only be the one that started at some slot
+    // j <= k (carried up by the swaps j .. k-1).  Every decision (j, k) is
+    // taken in parallel first, with the same operations as the sequential
+    // pass, which then only follows the chain: the same swaps bit for bit.
+    __shared__ uint32_t dmask[32];
+    const bool par = m <= 32;
+    if (par) {
+        if (threadIdx.x < 32) dmask[threadIdx.x] = 0u;
+        __syncthreads();
+        const int npair = (m - 1) * m / 2;  // (j, k), 0 <= j <= k < m - 1
+        for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+            int k = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+            while ((k + 1) * (k + 2) / 2 <= p) ++k;
+            while (k * (k + 1) / 2 > p) --k;
+            const int j = p - k * (k + 1) / 2;
+            const int c1 = sp[j], c2 = sp[k + 1];
+            const double x = (sb[k] - sb[k + 1]) * (sE[c1] - sE[c2]);
+            if (x >= 0.0 || (double)sx[k] * 0x1p-32 < det_exp(x)) atomicOr(&dmask[k], 1u << j);
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
+        int j = 0;  // slot where the configuration now at slot k started
         for (int k = 0; k < m - 1; ++k) {
             const int c1 = sp[k], c2 = sp[k + 1];
-            const double x = (sb[k] - sb[k + 1]) * (sE[c1] - sE[c2]);
-            if (x >= 0.0 || (double)sx[k] * 0x1p-32 < det_exp(x)) {
+            bool swap;
+            if (par) {
+                swap = ((dmask[k] >> j) & 1u) != 0u;
+            } else {
+                const double x = (sb[k] - sb[k + 1]) * (sE[c1] - sE[c2]);
+                swap = x >= 0.0 || (double)sx[k] * 0x1p-32 < det_exp(x);
+            }
+            if (swap) {
                 sp[k] = c2;
                 sp[k + 1] = c1;
                 sk[c2] = k;
                 sk[c1] = k + 1;
                 sacc_f[k] = 1;
+            } else {
+                j = k + 1;
             }
         }
         const int hot = sp[0], cold = sp[m - 1];
