@@ -46,8 +46,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "ising2d_traffic.json"))
-    ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "serve", "copy"), default="ising2d",
-                    help="ising2d: the headline batch; ea / xy / serve: the other configurations")
+    ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "potts", "serve", "copy"),
+                    default="ising2d",
+                    help="ising2d: the headline batch; ea / xy / serve: the other configurations; "
+                         "potts: the q = 10 heat bath of the serving mix at full size")
     ap.add_argument("--samples", type=int, default=512, help="ea: disorder samples")
     ap.add_argument("--jobs", type=int, default=10_000, help="serve: number of jobs")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 70%% of capacity)")
@@ -128,6 +130,46 @@ def _step_roofline(nbytes: float, seconds: float, basis: str) -> dict:
     achieved = nbytes / seconds / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": None, "basis": basis}
+
+
+def bench_potts(args) -> dict:
+    """128 replicas of 512^2 q = 10 Potts, beta within +-2 % of the transition point
+    ln(1 + sqrt 10), checkerboard heat bath, int32 E and q counts recorded per sweep."""
+    import torch
+
+    from critmc_amd.kernels import potts as K
+    from critmc_amd.kernels.chains import Chains
+    from critmc_amd.rng import seed_sequence
+
+    L, q = 512, 10
+    R = args.replicas if args.replicas != R_DEFAULT else 128
+    dev = torch.device("cuda", 0)
+    keys = seed_sequence(10).generate_state(2 * R, np.uint32).astype(np.uint64).reshape(R, 2)
+    ch = Chains(keys, np.arange(R), None, dev)
+    betas = np.log(1.0 + np.sqrt(q)) * (1.0 + np.random.default_rng(10).uniform(-0.02, 0.02, size=R))
+    s = torch.empty((R, L * L), dtype=torch.int8, device=dev)
+    K.init_states(s, (L, L), q, ch, "random")
+    e, cnt = K.measure(s, (L, L), q, R)
+    ws = K.Workspace(R, dev)
+    sched = np.array([[2, 1]])
+    K.run(s, (L, L), q, betas, ch, sched, max(args.warmup, 1), e, cnt, ws=ws)
+    torch.cuda.synchronize()
+    n = args.steps
+    t0 = time.perf_counter()
+    e_out, acc, counts = K.run(s, (L, L), q, betas, ch, sched, n, e, cnt, ws=ws)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    updates = R * L * L * n
+    e_chk, c_chk = K.measure(s, (L, L), q, R)
+    return {"metric": "site updates/ns, q=10 Potts 128 x 512^2 heat bath",
+            "value": round(updates / dt / 1e9, 3), "unit": "site updates/ns", "n_gpus": 1,
+            "steps": n, "warmup": args.warmup, "ms_per_step": dt * 1e3 / n,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8",
+            "data": "synthetic: random start and keys from seed 10", "config": {
+                "workload": "potts2d_q10_heatbath", "L": L, "replicas": R, "q": q,
+                "beta": "ln(1+sqrt 10) +- 2 %", "record": "int32 E, accepted, counts[q] per sweep"},
+            "roofline": _step_roofline(2.0 * updates, dt, "whole timed call; 2 B per site update"),
+            "bookkeeping_consistent": bool(torch.equal(e_chk, e) and torch.equal(c_chk, cnt))}
 
 
 def bench_xy(args) -> dict:
@@ -250,7 +292,8 @@ def main():
         sys.exit(2)
 
     if args.workload != "ising2d":
-        fn = {"ea": bench_ea, "xy": bench_xy, "serve": bench_serve, "copy": bench_copy}[args.workload]
+        fn = {"ea": bench_ea, "xy": bench_xy, "potts": bench_potts, "serve": bench_serve,
+              "copy": bench_copy}[args.workload]
         print(json.dumps(fn(args)))
         return
 
