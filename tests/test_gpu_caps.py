@@ -95,3 +95,40 @@ def test_potts_caps_match_replay(shape, grid):
             assert int(e_h[r]) == int(ref["E"][-1])
             assert np.array_equal(cnt_h[r], ref["counts"][-1])
         assert int(sweeps[r]) == int(sw0[r]) + 2 * c
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (512, 40)], ids=["one-tile", "four-tiles"])
+@pytest.mark.parametrize("sched", [[[0, 1], [5, 2]], [[0, 1]], [[5, 1], [0, 1]]],
+                         ids=["metropolis+2or", "metropolis", "or+metropolis"])
+def test_on_caps_match_alone(shape, sched):
+    """XY replicas with caps give what each replica gives alone with n_meas = its cap:
+    the same FP32 spins, acceptance counts and FP64 measurements, bit for bit (the
+    measurement partial sums are formed per replica in both cases)."""
+    from critmc_amd.kernels import on as KO
+
+    dev = torch.device("cuda", 0)
+    R, n_meas = len(CAPS), max(CAPS)
+    chains, keys, sids, sw0 = _chains(R, 13, dev)
+    N = int(np.prod(shape))
+    s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
+    KO.init_spins(s, shape, 2, chains)
+    s0 = s.clone()
+    betas = np.linspace(0.8, 1.3, R)
+    steps = np.linspace(0.5, 1.5, R)
+    sched = np.array(sched)
+    caps = torch.tensor(CAPS, dtype=torch.int64, device=dev)
+    out = KO.run(s, shape, 2, betas, steps, chains, sched, n_meas, n_meas_rep=caps)
+    out = [t.cpu().numpy() for t in out]
+    sweeps = chains.sweeps_d.cpu().numpy().view(np.uint64)
+    per_meas = int(sched[:, 1].sum())
+    for r, c in enumerate(CAPS):
+        assert int(sweeps[r]) == int(sw0[r]) + per_meas * c
+        if c == 0:
+            assert torch.equal(s[r], s0[r]), f"replica {r} moved"
+            continue
+        one = Chains(keys[r:r + 1], sids[r:r + 1], sw0[r:r + 1], dev)
+        s1 = s0[r:r + 1].clone()
+        alone = KO.run(s1, shape, 2, betas[r:r + 1], steps[r:r + 1], one, sched, c)
+        assert torch.equal(s[r], s1[0]), f"replica {r}"
+        for got, want in zip(out, alone):
+            assert np.array_equal(got[r, :c], want.cpu().numpy()[0]), f"replica {r}"
