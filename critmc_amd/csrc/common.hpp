@@ -89,6 +89,17 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
+// Sum over the wave of a per-lane value in [0, 2^B): one ballot and popcount
+// per bit instead of a shuffle tree (the result is wave-uniform).
+template <int B>
+__device__ __forceinline__ int wave_sum_bits(uint32_t v)
+{
+    int s = 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) s += __popcll(__ballot((v >> b) & 1u)) << b;
+    return s;
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v)
 {
 #pragma unroll

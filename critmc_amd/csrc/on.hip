@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
     }
     if (CODE == CODE_METROPOLIS) {  // one atomic per workgroup
         __shared__ int part[4];
-        const int t = wave_sum(accepted);
+        const int t = wave_sum_bits<1>((uint32_t)accepted);
         if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
         __syncthreads();
         if (threadIdx.x == 0) {

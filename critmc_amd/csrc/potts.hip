@@ -327,8 +327,10 @@ __global__ __launch_bounds__(256) void potts_half_wide(
         potts_quad<Z, QB>(S, g, u, colour, q, potts_pad<QB>(q), code, sweep,
                           ctr_z(sweep, (uint32_t)colour, TAG_POTTS), k0, k1, sid, &tl, de, acc,
                           counts);
-    de = wave_sum(de);
-    acc = wave_sum(acc);
+    // one quad per lane: de in [-4Z, 4Z], acc in [0, 4]; bitwise ballots sum them
+    const int lanes = __popcll(__ballot(1));
+    de = wave_sum_bits<6>((uint32_t)(de + 4 * Z)) - 4 * Z * lanes;
+    acc = wave_sum_bits<3>((uint32_t)acc);
     if ((threadIdx.x & 63) == 0) {
         red[2 * (threadIdx.x >> 6)] = de;
         red[2 * (threadIdx.x >> 6) + 1] = acc;
