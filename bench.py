@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--samples", type=int, default=512, help="ea: disorder samples")
     ap.add_argument("--jobs", type=int, default=10_000, help="serve: number of jobs")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 70%% of capacity)")
+    ap.add_argument("--slice-seconds", type=float, default=0.0,
+                    help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
     ap.add_argument("--capacity", type=float, default=SERVE_CAPACITY,
                     help="serve: sustained site updates/ns of the job mix, measured with all jobs "
                          "queued at once (--rate 1e9); arrivals default to 70%% of it")
@@ -237,7 +239,7 @@ def bench_serve(args) -> dict:
     if rate <= 0:  # Poisson arrivals at 70 % of the measured capacity
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
-    srv = Server(budget_bytes=200e9, slice_sweeps=1000)
+    srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds)
     for j in jobs:
         srv.submit(j)
     torch.cuda.synchronize()
@@ -253,7 +255,7 @@ def bench_serve(args) -> dict:
             "config": {"workload": "serve_continuous_batching", "jobs": len(done),
                        "arrival_rate_per_s": rate, "offered_site_updates_per_ns":
                        round(rate * work / len(jobs) / 1e9, 2), "budget_GB": 200,
-                       "slice_sweeps": 1000},
+                       "slice_sweeps": 1000, "slice_seconds": args.slice_seconds},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
             "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,

@@ -90,6 +90,7 @@ class _Pool:
         self.start_event = torch.cuda.Event(enable_timing=True)
         self.done_event = torch.cuda.Event(enable_timing=True)
         self.busy_s = 0.0                 # device time of this pool's slices
+        self.sec_per_sweep = 0.0          # running estimate of one slice sweep's device time
         self.site_updates = 0
         with torch.cuda.stream(self.stream):
             self._grow(8)
@@ -236,11 +237,18 @@ class Server:
     """Admits jobs against ``budget_bytes`` and advances them in ticks of ``slice_sweeps``."""
 
     def __init__(self, budget_bytes: float = 200e9, slice_sweeps: int = 500, n_streams: int = 4,
-                 max_active: int = 1 << 16):
+                 max_active: int = 1 << 16, slice_seconds: float = 0.0, min_slice: int = 16):
+        """``slice_sweeps`` caps the sweeps of one slice.  With ``slice_seconds`` > 0 a
+        pool's slice is instead sized from its measured device time per sweep to take
+        about that long (at least ``min_slice`` sweeps), so a large pool does not hold
+        its stream for long while smaller pools sharing it wait.  Slicing never
+        changes results: a job's chain depends only on its seed."""
         self.dev = device()
         self.budget = float(budget_bytes)
         self.used = 0.0
         self.slice = int(slice_sweeps)
+        self.slice_seconds = float(slice_seconds)
+        self.min_slice = max(1, int(min_slice))
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
         self.pools: dict[tuple, _Pool] = {}
         self.queue: list[tuple[float, int, Job]] = []
@@ -291,7 +299,10 @@ class Server:
 
     def _retire(self, pool: _Pool, now: float):
         caps, pool.inflight = pool.inflight, None
-        pool.busy_s += pool.start_event.elapsed_time(pool.done_event) / 1e3
+        dt = pool.start_event.elapsed_time(pool.done_event) / 1e3
+        pool.busy_s += dt
+        sps = dt / max(int(caps.max()), 1)
+        pool.sec_per_sweep = sps if pool.sec_per_sweep == 0.0 else 0.5 * (pool.sec_per_sweep + sps)
         for slot, job in enumerate(pool.jobs):
             if job is None or not caps[slot]:
                 continue
@@ -335,7 +346,7 @@ class Server:
                 for job in pool.pending:
                     pool.add(job)
                 pool.pending = []
-                if pool.active and pool.launch(self.slice) is not None:
+                if pool.active and pool.launch(self._slice_for(pool)) is not None:
                     self.ticks += 1
                     busy = True
             if progress > 0 and time.perf_counter() - last > progress:
@@ -350,6 +361,11 @@ class Server:
                 time.sleep(20e-6)
         torch.cuda.synchronize()
         return self.finished
+
+    def _slice_for(self, pool: _Pool) -> int:
+        if self.slice_seconds <= 0.0 or pool.sec_per_sweep <= 0.0:
+            return self.slice
+        return max(self.min_slice, min(self.slice, int(self.slice_seconds / pool.sec_per_sweep)))
 
     def pool_stats(self) -> dict[str, dict]:
         """Per (model, L) pool: device seconds of its slices (pools overlap on the

@@ -28,7 +28,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
         srv.run(realtime=False)
         alone.append(job.results())
     mix = poisson_jobs(24, rate=2000.0, seed=5, L_choices=(16, 32, 64), sweeps_range=(200, 2000))
-    srv = Server(budget_bytes=1e9, slice_sweeps=300)
+    srv = Server(budget_bytes=1e9, slice_sweeps=300, slice_seconds=0.002)  # adaptive slices too
     probes = [srv.submit(_clone(j, arrival=0.003 * (k + 1))) for k, j in enumerate(PROBES)]
     for j in mix:
         srv.submit(j)
