@@ -89,6 +89,23 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
+// Integer sum over the wave with DPP adds: inclusive sums within each row of
+// 16 lanes (row_shr 1, 2, 4, 8, lanes shifted in from outside the row read 0),
+// then row 0's and row 2's totals carried into rows 1 and 3 (row_bcast:15)
+// and the first half's total into the second half (row_bcast:31); lane 63
+// then holds the total, returned to every lane.  Six adds instead of six
+// shuffles through LDS.
+__device__ __forceinline__ int wave_sum_dpp(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Sum over the wave of a per-lane value in [0, 2^B): one ballot and popcount
 // per bit instead of a shuffle tree (the result is wave-uniform).
 template <int B>

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                 }
             }
             fold_counts<4>(fc, d);
-            const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+            const int de = wave_sum_dpp(d.de), dm = wave_sum_dpp(d.dm), ac = wave_sum_dpp(d.acc);
             if ((threadIdx.x & 63) == 0) {
                 red[3 * (threadIdx.x >> 6) + 0] = de;
                 red[3 * (threadIdx.x >> 6) + 1] = dm;
