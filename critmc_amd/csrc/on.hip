@@ -627,10 +627,15 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     constexpr int K = 1 + NC + 2 * D;
     constexpr int Z = 2 * D;
     __shared__ double red[K * 4];
-    const int rep = blockIdx.y, chunk = blockIdx.x;
+    // consecutive chunks of a replica on one XCD, so the rows a chunk reads
+    // beyond its own are in the same L2 as their neighbour chunk's
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x), chunk = (int)(lb - (uint32_t)rep * gridDim.x);
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;
-    float* S = spins + (int64_t)rep * g.N * NC;
-    const int64_t P1 = g.half;  // colour-1 plane
+    // the colour-0 plane is renormalised in place, the colour-1 plane only read:
+    // separate restrict pointers let the loads move ahead of the stores
+    float* __restrict__ S0 = spins + (int64_t)rep * g.N * NC;
+    const float* __restrict__ S1 = S0 + g.half * NC;
     double v[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = 0.0;
@@ -639,16 +644,16 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         uint32_t nbo[Z];
         cb_other<Z>(g, (uint32_t)k, 0, nbo);
-        const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S, k));
-        store_vec<NC>(S, k, s);
-        const Vec<NC> xr = load_vec<NC>(S, P1 + nbo[0]);  // i + e_0
+        const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S0, k));
+        store_vec<NC>(S0, k, s);
+        const Vec<NC> xr = load_vec<NC>(S1, nbo[0]);  // i + e_0
 #pragma unroll
         for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c] + (double)xr.v[c];
 #pragma unroll
         for (int ax = 0; ax < D; ++ax) {
 #pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
-                const Vec<NC> t = load_vec<NC>(S, P1 + nbo[2 * ax + dir]);
+                const Vec<NC> t = load_vec<NC>(S1, nbo[2 * ax + dir]);
                 // bond (a, b) = (i, i+e_ax) forward, (i-e_ax, i) backward
                 const Vec<NC>& A = dir == 0 ? s : t;
                 const Vec<NC>& B = dir == 0 ? t : s;
