@@ -239,6 +239,122 @@ struct OnArgs {
     int norm_out;  // write every site of this half sweep renormalised (the measurement's last)
 };
 
+// The update of one site of colour `colour` (index k within its colour), the
+// per-site work every local O(n) update shares: the per-half-sweep kernels
+// and the per-replica LDS kernel both call it, so they store the same bits.
+// `So` is the other colour's plane (HBM or LDS) and nbo the neighbours'
+// offsets in it; s is the site's spin, updated in place.  Returns whether s
+// changed; `accepted` is set to 1 when a Metropolis proposal is accepted.
+template <int NC, int Z, int CODE>
+__device__ __forceinline__ bool on_site(Vec<NC>& s, const float* __restrict__ So,
+                                        const uint32_t (&nbo)[Z], uint32_t k, int colour,
+                                        uint64_t sweep, uint32_t k0, uint32_t k1, uint32_t sid,
+                                        float beta, float step, int& accepted)
+{
+    // The local field is summed in FP64: its Z FP32 terms are exact there, so
+    // a nearly cancelling field keeps full relative precision (in FP32 the
+    // sum's absolute error eps * Z would dominate when |h| -> 0).
+    double hd[NC];
+#pragma unroll
+    for (int j = 0; j < Z; ++j) {
+        const Vec<NC> t = load_vec<NC>(So, nbo[j]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) hd[c] = j == 0 ? (double)t.v[c] : hd[c] + (double)t.v[c];
+    }
+    float h[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) h[c] = (float)hd[c];
+    bool changed = false;
+    if (CODE == CODE_HEATBATH) {
+        double hn2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) hn2 += hd[c] * hd[c];
+        const double hn = sqrt(hn2);
+        const double kappa = (double)beta * hn;
+        if constexpr (NC == 2) {
+            const double phi0 = hn > 0.0 ? atan2(hd[1], hd[0]) : 0.0;
+            const double phi = phi0 + von_mises(kappa, k, sweep, (uint32_t)colour, sid, k0, k1);
+            s.v[0] = (float)cos(phi);
+            s.v[1] = (float)sin(phi);
+        } else {
+            const u32x4 b = philox4x32_10(
+                u32x4{k, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_ONHB), sid}, k0, k1);
+            if (hn == 0.0) {
+                const double z = 2.0 * unit_open1(b.z) - 1.0;
+                const double az = 2.0 * M_PI * unit_open1(b.w);
+                const double w = sqrt(fmax(0.0, 1.0 - z * z));
+                s.v[0] = (float)(w * cos(az));
+                s.v[1] = (float)(w * sin(az));
+                s.v[2] = (float)z;
+            } else {
+                const double r = unit_open0(b.x);
+                double u = kappa < 1e-8 ? 2.0 * r - 1.0
+                                        : 1.0 + log(r + (1.0 - r) * exp(-2.0 * kappa)) / kappa;
+                u = fmin(1.0, fmax(-1.0, u));
+                const double hx = hd[0] / hn, hy = hd[1] / hn, hz = hd[2] / hn;
+                const double ax = fabs(hx) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
+                const double proj = ax * hx + ay * hy;
+                double e1x = ax - proj * hx, e1y = ay - proj * hy, e1z = -proj * hz;
+                const double nrm = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+                e1x /= nrm;
+                e1y /= nrm;
+                e1z /= nrm;
+                const double e2x = hy * e1z - hz * e1y, e2y = hz * e1x - hx * e1z,
+                             e2z = hx * e1y - hy * e1x;
+                const double phi = 2.0 * M_PI * unit_open1(b.y);
+                const double w = sqrt(fmax(0.0, 1.0 - u * u));
+                const double cp = cos(phi), sp = sin(phi);
+                s.v[0] = (float)(u * hx + w * (cp * e1x + sp * e2x));
+                s.v[1] = (float)(u * hy + w * (cp * e1y + sp * e2y));
+                s.v[2] = (float)(u * hz + w * (cp * e1z + sp * e2z));
+            }
+        }
+        changed = true;
+    } else if (CODE == CODE_OVERRELAX) {
+        changed = reflect<NC>(s, h);
+    } else {
+        const uint32_t zc = ctr_z(sweep, (uint32_t)colour, TAG_ON);
+        float g[4];
+        uint32_t xacc;
+        if constexpr (NC == 2) {
+            const u32x4 r = philox4x32_10(u32x4{k, (uint32_t)sweep, zc, sid}, k0, k1);
+            box_muller(r.x, r.y, g[0], g[1]);
+            xacc = r.z;
+        } else {
+            const u32x4 r = philox4x32_10(u32x4{2u * k, (uint32_t)sweep, zc, sid}, k0, k1);
+            const u32x4 r2 = philox4x32_10(u32x4{2u * k + 1u, (uint32_t)sweep, zc, sid}, k0, k1);
+            box_muller(r.x, r.y, g[0], g[1]);
+            box_muller(r.z, r.w, g[2], g[3]);
+            xacc = r2.x;
+        }
+        float tr[NC];
+        float n2 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            tr[c] = s.v[c] + step * g[c];
+            n2 += tr[c] * tr[c];
+        }
+        // v_rsq_f32 (1 ulp) scales the proposal: its direction is exact, its
+        // length is 1 within 2e-7 and renormalised at every measurement
+        const float inv = __builtin_amdgcn_rsqf(n2);
+        float de = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            tr[c] *= inv;
+            de -= (tr[c] - s.v[c]) * h[c];
+        }
+        // v_exp_f32: a decision can differ from the FP64 rule only where u lies
+        // within a few ulp of the acceptance probability
+        if (de <= 0.0f || (float)xacc * 0x1p-32f < __expf(-beta * de)) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s.v[c] = tr[c];
+            changed = true;
+            accepted = 1;
+        }
+    }
+    return changed;
+}
+
 template <int NC, int Z, int CODE>
 __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
 {
@@ -253,113 +369,11 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
         cb_other<Z>(a.g, (uint32_t)k, a.colour, nbo);
         const int64_t i = (int64_t)a.colour * a.g.half + k;            // own plane
         const float* So = S + (int64_t)(1 - a.colour) * a.g.half * NC;  // neighbours' plane
-        // The local field is summed in FP64: its Z FP32 terms are exact there, so
-        // a nearly cancelling field keeps full relative precision (in FP32 the
-        // sum's absolute error eps * Z would dominate when |h| -> 0).
-        double hd[NC];
-#pragma unroll
-        for (int j = 0; j < Z; ++j) {
-            const Vec<NC> t = load_vec<NC>(So, nbo[j]);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) hd[c] = j == 0 ? (double)t.v[c] : hd[c] + (double)t.v[c];
-        }
-        float h[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) h[c] = (float)hd[c];
         Vec<NC> s = load_vec<NC>(S, i);
-        bool changed = false;
-        if (CODE == CODE_HEATBATH) {
-            const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
-            const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
-            double hn2 = 0.0;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) hn2 += hd[c] * hd[c];
-            const double hn = sqrt(hn2);
-            const double kappa = (double)a.betas[rep] * hn;
-            if constexpr (NC == 2) {
-                const double phi0 = hn > 0.0 ? atan2(hd[1], hd[0]) : 0.0;
-                const double phi = phi0 + von_mises(kappa, (uint32_t)k, sweep, (uint32_t)a.colour, sid, k0, k1);
-                s.v[0] = (float)cos(phi);
-                s.v[1] = (float)sin(phi);
-            } else {
-                const u32x4 b = philox4x32_10(
-                    u32x4{(uint32_t)k, (uint32_t)sweep, ctr_z(sweep, (uint32_t)a.colour, TAG_ONHB), sid}, k0, k1);
-                if (hn == 0.0) {
-                    const double z = 2.0 * unit_open1(b.z) - 1.0;
-                    const double az = 2.0 * M_PI * unit_open1(b.w);
-                    const double w = sqrt(fmax(0.0, 1.0 - z * z));
-                    s.v[0] = (float)(w * cos(az));
-                    s.v[1] = (float)(w * sin(az));
-                    s.v[2] = (float)z;
-                } else {
-                    const double r = unit_open0(b.x);
-                    double u = kappa < 1e-8 ? 2.0 * r - 1.0
-                                            : 1.0 + log(r + (1.0 - r) * exp(-2.0 * kappa)) / kappa;
-                    u = fmin(1.0, fmax(-1.0, u));
-                    const double hx = hd[0] / hn, hy = hd[1] / hn, hz = hd[2] / hn;
-                    const double ax = fabs(hx) < 0.9 ? 1.0 : 0.0, ay = 1.0 - ax;
-                    const double proj = ax * hx + ay * hy;
-                    double e1x = ax - proj * hx, e1y = ay - proj * hy, e1z = -proj * hz;
-                    const double nrm = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
-                    e1x /= nrm;
-                    e1y /= nrm;
-                    e1z /= nrm;
-                    const double e2x = hy * e1z - hz * e1y, e2y = hz * e1x - hx * e1z,
-                                 e2z = hx * e1y - hy * e1x;
-                    const double phi = 2.0 * M_PI * unit_open1(b.y);
-                    const double w = sqrt(fmax(0.0, 1.0 - u * u));
-                    const double cp = cos(phi), sp = sin(phi);
-                    s.v[0] = (float)(u * hx + w * (cp * e1x + sp * e2x));
-                    s.v[1] = (float)(u * hy + w * (cp * e1y + sp * e2y));
-                    s.v[2] = (float)(u * hz + w * (cp * e1z + sp * e2z));
-                }
-            }
-            changed = true;
-        } else if (CODE == CODE_OVERRELAX) {
-            changed = reflect<NC>(s, h);
-        } else {
-            const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
-            const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
-            const uint32_t zc = ctr_z(sweep, (uint32_t)a.colour, TAG_ON);
-            float g[4];
-            uint32_t xacc;
-            if constexpr (NC == 2) {
-                const u32x4 r = philox4x32_10(u32x4{(uint32_t)k, (uint32_t)sweep, zc, sid}, k0, k1);
-                box_muller(r.x, r.y, g[0], g[1]);
-                xacc = r.z;
-            } else {
-                const u32x4 r = philox4x32_10(u32x4{(uint32_t)(2 * k), (uint32_t)sweep, zc, sid}, k0, k1);
-                const u32x4 r2 = philox4x32_10(u32x4{(uint32_t)(2 * k + 1), (uint32_t)sweep, zc, sid}, k0, k1);
-                box_muller(r.x, r.y, g[0], g[1]);
-                box_muller(r.z, r.w, g[2], g[3]);
-                xacc = r2.x;
-            }
-            const float step = a.steps[rep];
-            float tr[NC];
-            float n2 = 0.0f;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                tr[c] = s.v[c] + step * g[c];
-                n2 += tr[c] * tr[c];
-            }
-            // v_rsq_f32 (1 ulp) scales the proposal: its direction is exact, its
-            // length is 1 within 2e-7 and renormalised at every measurement
-            const float inv = __builtin_amdgcn_rsqf(n2);
-            float de = 0.0f;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                tr[c] *= inv;
-                de -= (tr[c] - s.v[c]) * h[c];
-            }
-            // v_exp_f32: a decision can differ from the FP64 rule only where u lies
-            // within a few ulp of the acceptance probability
-            if (de <= 0.0f || (float)xacc * 0x1p-32f < __expf(-a.betas[rep] * de)) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) s.v[c] = tr[c];
-                changed = true;
-                accepted = 1;
-            }
-        }
+        const bool changed = on_site<NC, Z, CODE>(s, So, nbo, (uint32_t)k, a.colour,
+                                                  a.sweeps[rep] + a.sweep_add, a.keys[2 * rep],
+                                                  a.keys[2 * rep + 1], a.sids[rep], a.betas[rep],
+                                                  a.steps[rep], accepted);
         if (a.norm_out) {
             // the measurement's renormalisation of this site, fused into its last update
             s = unit_vec<NC>(s);
@@ -612,6 +626,34 @@ __global__ __launch_bounds__(256) void on_measure_partial(const float* __restric
     }
 }
 
+// The measurement terms of colour-0 site i, already renormalised as s: M of
+// i and of i + e_0 (a bijection onto colour 1) and its 2d bonds, forward
+// (i, i + e_a) and backward (i - e_a, i), oriented as the reference's forward
+// bonds (every bond has exactly one colour-0 end).  v = (E, M[NC], c_a, s_a).
+template <int NC, int D>
+__device__ __forceinline__ void on_c0_bonds(const Vec<NC>& s, const float* __restrict__ S1,
+                                            const uint32_t (&nbo)[2 * D], double (&v)[1 + NC + 2 * D])
+{
+    const Vec<NC> xr = load_vec<NC>(S1, nbo[0]);  // i + e_0
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c] + (double)xr.v[c];
+#pragma unroll
+    for (int ax = 0; ax < D; ++ax) {
+#pragma unroll
+        for (int dir = 0; dir < 2; ++dir) {
+            const Vec<NC> t = load_vec<NC>(S1, nbo[2 * ax + dir]);
+            const Vec<NC>& A = dir == 0 ? s : t;
+            const Vec<NC>& B = dir == 0 ? t : s;
+            double dot = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dot += (double)A.v[c] * (double)B.v[c];
+            v[0] -= dot;
+            v[1 + NC + ax] += (double)A.v[0] * B.v[0] + (double)A.v[1] * B.v[1];
+            v[1 + NC + D + ax] += (double)A.v[0] * B.v[1] - (double)A.v[1] * B.v[0];
+        }
+    }
+}
+
 // Renormalise-and-measure over the colour-0 sites, for measurements whose
 // last half sweep (colour 1) already wrote its sites renormalised.  Every
 // bond joins a colour-0 site i to a colour-1 site, so i's lane takes all 2d
@@ -646,25 +688,7 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
         cb_other<Z>(g, (uint32_t)k, 0, nbo);
         const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S0, k));
         store_vec<NC>(S0, k, s);
-        const Vec<NC> xr = load_vec<NC>(S1, nbo[0]);  // i + e_0
-#pragma unroll
-        for (int c = 0; c < NC; ++c) v[1 + c] += (double)s.v[c] + (double)xr.v[c];
-#pragma unroll
-        for (int ax = 0; ax < D; ++ax) {
-#pragma unroll
-            for (int dir = 0; dir < 2; ++dir) {
-                const Vec<NC> t = load_vec<NC>(S1, nbo[2 * ax + dir]);
-                // bond (a, b) = (i, i+e_ax) forward, (i-e_ax, i) backward
-                const Vec<NC>& A = dir == 0 ? s : t;
-                const Vec<NC>& B = dir == 0 ? t : s;
-                double dot = 0.0;
-#pragma unroll
-                for (int c = 0; c < NC; ++c) dot += (double)A.v[c] * (double)B.v[c];
-                v[0] -= dot;
-                v[1 + NC + ax] += (double)A.v[0] * B.v[0] + (double)A.v[1] * B.v[1];
-                v[1 + NC + D + ax] += (double)A.v[0] * B.v[1] - (double)A.v[1] * B.v[0];
-            }
-        }
+        on_c0_bonds<NC, D>(s, S1, nbo, v);
     }
     const int w = threadIdx.x >> 6;
 #pragma unroll
@@ -677,6 +701,129 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
         double t = 0.0;
         for (int u = 0; u < (int)(blockDim.x >> 6); ++u) t += red[threadIdx.x * 4 + u];
         part[((int64_t)rep * nchunk + chunk) * K + threadIdx.x] = t;
+    }
+}
+
+// Per-replica form for lattices whose spins fit in LDS (N n 4 B <= ON_BATCHED_LDS):
+// one workgroup per replica runs every sweep and measurement of the call with
+// the replica in LDS, in the colour-split layout, and writes it back once.
+// Each site update is on_site, and each measurement repeats on_measure_c0's
+// per-site work, chunks and reductions and then on_measure_final's ordered
+// sum, so spins and series equal the per-half-sweep launches' bit for bit.
+// Local updates only (Metropolis, heat bath, over-relaxation); as there, the
+// last half sweep of a measurement stores its sites renormalised.  HB: the
+// schedule has heat-bath rows (a separate build, as the heat bath's registers
+// would otherwise limit every schedule's occupancy).
+constexpr int64_t ON_BATCHED_LDS = 60 * 1024;
+
+template <int NC, int D, bool HB>
+__global__ __launch_bounds__(256) void on_batched(OnArgs a, Sched sc, double* __restrict__ e_out,
+                                                  double* __restrict__ m_out,
+                                                  double* __restrict__ sc_out,
+                                                  double* __restrict__ ss_out)
+{
+    constexpr int Z = 2 * D, K = 1 + NC + 2 * D;
+    extern __shared__ float ls[];  // [2 colours][half][NC]
+    __shared__ double red[K * 4];
+    __shared__ int ired[4];
+    const CbGeo& g = a.g;
+    const int64_t half = g.half;
+    const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    float* L0 = ls;
+    float* L1 = ls + half * NC;
+    for (int rep = blockIdx.x; rep < a.R; rep += gridDim.x) {
+        const int64_t nmr = a.n_meas_rep ? min(a.n_meas, a.n_meas_rep[rep]) : a.n_meas;
+        if (nmr <= 0) continue;  // the whole block: replica capped
+        float* S = a.spins + (int64_t)rep * g.N * NC;
+        for (int64_t t = threadIdx.x; t < g.N * NC; t += blockDim.x) ls[t] = S[t];
+        __syncthreads();
+        const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+        const uint64_t sw0 = a.sweeps[rep];
+        const float beta = a.betas[rep], step = a.steps[rep];
+        uint64_t sw = 0;  // sweeps of this call before the current one
+        for (int64_t meas = 0; meas < nmr; ++meas) {
+            int acc = 0;
+            for (int r = 0; r < sc.n; ++r) {
+                const int code = sc.code[r];
+                for (int rp = 0; rp < sc.count[r]; ++rp, ++sw) {
+                    for (int c = 0; c < 2; ++c) {
+                        const bool norm = r == sc.n - 1 && rp == sc.count[r] - 1 && c == 1;
+                        float* Pc = c ? L1 : L0;
+                        const float* Po = c ? L0 : L1;
+                        for (int64_t k = threadIdx.x; k < half; k += blockDim.x) {
+                            uint32_t nbo[Z];
+                            cb_other<Z>(g, (uint32_t)k, c, nbo);
+                            Vec<NC> s = load_vec<NC>(Pc, k);
+                            int ac = 0;
+                            bool changed;
+                            if (code == CODE_METROPOLIS)
+                                changed = on_site<NC, Z, CODE_METROPOLIS>(s, Po, nbo, (uint32_t)k, c, sw0 + sw,
+                                                                          k0, k1, sid, beta, step, ac);
+                            else if (HB && code == CODE_HEATBATH)
+                                changed = on_site<NC, Z, CODE_HEATBATH>(s, Po, nbo, (uint32_t)k, c, sw0 + sw,
+                                                                        k0, k1, sid, beta, step, ac);
+                            else
+                                changed = on_site<NC, Z, CODE_OVERRELAX>(s, Po, nbo, (uint32_t)k, c, sw0 + sw,
+                                                                         k0, k1, sid, beta, step, ac);
+                            acc += ac;
+                            if (norm) {
+                                s = unit_vec<NC>(s);
+                                store_vec<NC>(Pc, k, s);
+                            } else if (changed) {
+                                store_vec<NC>(Pc, k, s);
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+            // the measurement: colour 0 renormalised in place, bonds summed per chunk
+            double total = 0.0;  // thread q < K: quantity q, chunk partials added in order
+            const int nchunk = (int)((half + ON_CHUNK - 1) / ON_CHUNK);
+            for (int ch = 0; ch < nchunk; ++ch) {
+                double v[K];
+#pragma unroll
+                for (int q = 0; q < K; ++q) v[q] = 0.0;
+                const int64_t c0 = (int64_t)ch * ON_CHUNK, c1 = min(half, c0 + ON_CHUNK);
+                for (int64_t k = c0 + threadIdx.x; k < c1; k += blockDim.x) {
+                    uint32_t nbo[Z];
+                    cb_other<Z>(g, (uint32_t)k, 0, nbo);
+                    const Vec<NC> s = unit_vec<NC>(load_vec<NC>(L0, k));
+                    store_vec<NC>(L0, k, s);
+                    on_c0_bonds<NC, D>(s, L1, nbo, v);
+                }
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const double t = wave_sum_f64(v[q]);
+                    if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
+                }
+                __syncthreads();
+                if (threadIdx.x < K) {
+                    double t = 0.0;
+                    for (int u = 0; u < nw; ++u) t += red[threadIdx.x * 4 + u];
+                    total += t;
+                }
+                __syncthreads();
+            }
+            const int ta = wave_sum(acc);
+            if ((threadIdx.x & 63) == 0) ired[w] = ta;
+            const int64_t o = (int64_t)rep * a.n_meas + meas;
+            if (threadIdx.x < K) {
+                const int q = threadIdx.x;
+                if (q == 0) e_out[o] = total;
+                else if (q < 1 + NC) m_out[o * NC + (q - 1)] = total;
+                else if (q < 1 + NC + D) sc_out[o * D + (q - 1 - NC)] = total;
+                else ss_out[o * D + (q - 1 - NC - D)] = total;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int tot = 0;
+                for (int u = 0; u < nw; ++u) tot += ired[u];
+                a.acc[o] = tot;
+            }
+        }
+        for (int64_t t = threadIdx.x; t < g.N * NC; t += blockDim.x) S[t] = ls[t];
+        __syncthreads();
     }
 }
 
@@ -1159,6 +1306,22 @@ static void launch_measure_c0(float* spins, const CbGeo& g, int R, double* part,
     }
 }
 
+template <bool HB>
+static void launch_batched_on(int n, int d, dim3 grid, size_t lds, hipStream_t st, const OnArgs& a,
+                              const Sched& sc, double* e_out, double* m_out, double* sc_out,
+                              double* ss_out)
+{
+    if (n == 2) {
+        if (d == 1) hipLaunchKernelGGL((on_batched<2, 1, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        else if (d == 2) hipLaunchKernelGGL((on_batched<2, 2, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        else hipLaunchKernelGGL((on_batched<2, 3, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+    } else {
+        if (d == 1) hipLaunchKernelGGL((on_batched<3, 1, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        else if (d == 2) hipLaunchKernelGGL((on_batched<3, 2, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        else hipLaunchKernelGGL((on_batched<3, 3, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+    }
+}
+
 }  // namespace cm
 
 using namespace cm;
@@ -1269,6 +1432,27 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                        dim3(256), 0, st, acc_out, total);
     OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, n_meas_rep,
              g, n_rep, n_meas, 0, 0, 0, 0};
+    if (!any_cluster && aux_out == nullptr && g.N * n * 4 <= ON_BATCHED_LDS) {
+        // the whole call in one launch, each replica in LDS
+        Sched sc{};
+        sc.n = n_rows;
+        for (int r = 0; r < n_rows; ++r) {
+            sc.code[r] = (int32_t)schedule[2 * r];
+            sc.count[r] = (int32_t)schedule[2 * r + 1];
+        }
+        const size_t lds = (size_t)g.N * n * 4;
+        const dim3 bgrid((unsigned)std::min(n_rep, 65535));
+        bool hb = false;
+        for (int r = 0; r < n_rows; ++r) hb |= schedule[2 * r] == CODE_HEATBATH;
+        if (hb) launch_batched_on<true>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        else launch_batched_on<false>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        int64_t per_meas = 0;
+        for (int r = 0; r < n_rows; ++r) per_meas += schedule[2 * r + 1];
+        hipLaunchKernelGGL(on_advance_sweeps, dim3((n_rep + 255) / 256), dim3(256), 0, st, sweeps,
+                           n_rep, (uint64_t)per_meas, n_meas, n_meas_rep);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    }
     // When the schedule ends with a local row, its last half sweep (colour 1)
     // writes renormalised spins and the measurement renormalises colour 0.
     const int64_t last_code = schedule[2 * (n_rows - 1)];

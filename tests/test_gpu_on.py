@@ -177,7 +177,8 @@ def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
     give exactly the spins and acceptance counts of four separate half-sweep
     launches.  The fused launch that ends a measurement also measures, adding
     the same FP64 bond terms per tile instead of per chunk of sites, so E, M
-    and the stiffness sums agree to FP64 rounding."""
+    and the stiffness sums agree to FP64 rounding.  (Recording aux keeps small
+    lattices on the per-half-sweep launches instead of the one-launch LDS form.)"""
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(sum(shape) + R)
     keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
@@ -188,9 +189,45 @@ def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
         chains = Chains(keys, np.arange(R), None, dev)
         s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
         K.init_spins(s, shape, 2, chains)
-        series = K.run(s, shape, 2, betas, steps, chains, np.array(sched), 3)
+        series = K.run(s, shape, 2, betas, steps, chains, np.array(sched), 3, with_aux=True)
         out.append((s.cpu().numpy(), [t.cpu().numpy() for t in series]))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1][0], out[1][1][0])  # accepted
     for a, b in zip(out[0][1][1:], out[1][1][1:]):  # E, M, c_a, s_a
         np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * N)
+
+
+@pytest.mark.parametrize("n,shape", [(2, (64, 32)), (2, (8, 6)), (2, (10,)), (3, (16, 16, 16)), (3, (6, 4)),
+                                     (2, (4, 4, 6))])
+@pytest.mark.parametrize("sched", [[[0, 1]], [[2, 1]], [[5, 1]], [[2, 2], [0, 1], [5, 1]], [[0, 1], [5, 2]]],
+                         ids=["metropolis", "heatbath", "overrelax", "mixed", "metropolis+2or"])
+def test_lds_form_matches_half_sweep_launches(n, shape, sched):
+    """Lattices whose spins fit in LDS run the whole call in one launch, one workgroup
+    per replica.  Against the per-half-sweep launches (taken when aux is recorded) the
+    spins and acceptance counts are the same bits, and so are E, M and the stiffness
+    sums, except after two fused over-relaxation sweeps on a square XY lattice, whose
+    launch sums its measurement per row tile (FP64 rounding)."""
+    dev = torch.device("cuda", 0)
+    R, n_meas = 3, 4
+    rng = np.random.default_rng(len(shape) * 10 + n + len(sched))
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    N = int(np.prod(shape))
+    betas, steps = np.array([0.7, 1.1, 1.6]), np.array([0.6, 1.0, 1.4])
+    outs = []
+    for with_aux in (False, True):
+        chains = Chains(keys, [7, 8, 9], None, dev)
+        s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
+        K.init_spins(s, shape, n, chains)
+        res = K.run(s, shape, n, betas, steps, chains, np.array(sched), n_meas, with_aux=with_aux)
+        outs.append((s.cpu().numpy(), [t.cpu().numpy() for t in res[:5]],
+                     chains.sweeps_d.cpu().numpy().view(np.uint64)))
+    (s_lds, r_lds, sw_lds), (s_ref, r_ref, sw_ref) = outs
+    assert np.array_equal(s_lds, s_ref)
+    assert np.array_equal(sw_lds, sw_ref)
+    assert np.array_equal(r_lds[0], r_ref[0])  # accepted
+    tiles = n == 2 and len(shape) == 2 and sched[-1] == [5, 2]
+    for a, b in zip(r_lds[1:], r_ref[1:]):
+        if tiles:
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * N)
+        else:
+            assert np.array_equal(a, b)
