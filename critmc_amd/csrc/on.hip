@@ -20,6 +20,9 @@
 // over all replicas (grid = colour-site chunks x replicas).  Two consecutive
 // over-relaxation sweeps of XY on a square lattice run as one launch over row
 // tiles in LDS (on_or_fused), bit-identical to four half-sweep launches.
+// Lattices whose spins fit in ON_BATCHED_LDS run a call of local updates as
+// one launch instead (on_batched: one workgroup per replica, spins in LDS,
+// the same per-site functions, so the same bits).
 // When the schedule ends with a local update, that last half sweep writes its
 // sites renormalised and on_measure_c0 renormalises colour 0 while summing
 // every bond once (when it ends with two fused over-relaxation sweeps, the
