@@ -289,8 +289,8 @@ R_DEFAULT = R
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; replicas shard across GPUs with no "
-                                   "traffic, so --gpus N > 1 is not measured here"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; replicas shard across GPUs "
+                                   "with no traffic between them, so --gpus N > 1 is not measured"}))
         sys.exit(2)
 
     if args.workload != "ising2d":
