@@ -473,6 +473,7 @@ static hipError_t launch_potts_q(int8_t* spins, const CbGeo& g, int q, int R, co
                         e_out, acc_out, counts_out, n_meas_rep, block, grid, st)
     if (q <= 4) return CM_POTTS_B(4);
     if (q <= 8) return CM_POTTS_B(8);
+    if (q == 10) return CM_POTTS_B(10);  // the serving mix's q: no padded states
     if (q <= 12) return CM_POTTS_B(12);
     if (q <= 16) return CM_POTTS_B(16);
     return CM_POTTS_B(0);
@@ -492,6 +493,7 @@ static void launch_wide(const dim3& wgrid, hipStream_t st, int8_t* spins, const 
                        counts_out)
     if (q <= 4) CM_POTTS_W(4);
     else if (q <= 8) CM_POTTS_W(8);
+    else if (q == 10) CM_POTTS_W(10);
     else if (q <= 12) CM_POTTS_W(12);
     else if (q <= 16) CM_POTTS_W(16);
     else CM_POTTS_W(0);
