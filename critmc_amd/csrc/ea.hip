@@ -848,8 +848,16 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
     double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;  // two chains: even and odd nibbles
     if (k < m) {
         const int nib = (L + 3) >> 2;
+        // row = x1 + L x2, carried forward by 8 rows per step (no division per row)
+        uint32_t x1 = 0, x2 = 0;
+        if (D >= 2) {
+            x1 = (uint32_t)j;
+            while (x1 >= L32) {
+                x1 -= L32;
+                ++x2;
+            }
+        }
         for (uint32_t row = j; row < rows; row += 8) {
-            const uint32_t x1 = D >= 2 ? row % L32 : 0u, x2 = D >= 3 ? row / L32 : 0u;
             const uint32_t qp = QP[row][k];
             const int rs = 2 * __popc(qp) - L;
             q += rs;
@@ -873,6 +881,13 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
                     const uint32_t w = (qp >> (4 * t + 4)) & 15u;
                     re1 += CT[t + 1][w];
                     im1 += ST[t + 1][w];
+                }
+            }
+            if (D >= 2) {
+                x1 += 8;
+                while (x1 >= L32) {
+                    x1 -= L32;
+                    ++x2;
                 }
             }
         }
