@@ -361,8 +361,13 @@ __device__ __forceinline__ void ms_draw(const uint32_t (*__restrict__ P)[8], uin
 // LDS: the set's words and neighbour-coupling bytes when they fit, and the
 // threshold planes P[t][class] (bit c = bit 31-t of configuration c's
 // threshold for that neighbour class) with the always-accept masks.
+#ifndef CM_EA_MS_THREADS
+#define CM_EA_MS_THREADS 256
+#endif
+constexpr int EA_MS_THREADS = CM_EA_MS_THREADS;  // lanes per (sample, set) workgroup
+
 template <int Z, bool LDS>
-__global__ __launch_bounds__(256) void ea_ms_sweep(EaArgs a)
+__global__ __launch_bounds__(EA_MS_THREADS) void ea_ms_sweep(EaArgs a)
 {
     extern __shared__ uint32_t lds32[];
     __shared__ __align__(16) uint32_t P[32][8];
@@ -1238,14 +1243,14 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
         for (int64_t t = 0; t < n_steps; ++t) {
             const uint64_t step = step0 + (uint64_t)t;
             if (ndim == 2) {
-                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
-                else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<4, true>), dim3(2 * n_samples), dim3(EA_MS_THREADS), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<4, false>), dim3(2 * n_samples), dim3(EA_MS_THREADS), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
                 if (max_L <= 16) hipLaunchKernelGGL((ea_measure_ms<4, true>), dim3(a.S), dim3(256), 0, st, a, t);
                 else hipLaunchKernelGGL((ea_measure_ms<4, false>), dim3(a.S), dim3(256), 0, st, a, t);
             } else {
-                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(256), mlds, st, a);
-                else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(256), 0, st, a);
+                if (ms_lds) hipLaunchKernelGGL((ea_ms_sweep<6, true>), dim3(2 * n_samples), dim3(EA_MS_THREADS), mlds, st, a);
+                else hipLaunchKernelGGL((ea_ms_sweep<6, false>), dim3(2 * n_samples), dim3(EA_MS_THREADS), 0, st, a);
                 hipLaunchKernelGGL((ea_swap<false>), dim3(swap_threads), dim3(64), 0, st, a, step);
                 hipLaunchKernelGGL((ea_measure_ms<6, true>), dim3(a.S), dim3(256), 0, st, a, t);
             }
