@@ -284,6 +284,10 @@ int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t
  * of the last Swendsen-Wang update).  cluster_ws_dev: null, or at least
  * cm_on_cluster_workspace_bytes for schedules with cluster rows (the embedded
  * Ising updates of critmc/kernels/on.py:264-345).
+ * A call of local updates only, with aux_out_dev null, on a lattice whose
+ * spins fit in 60 KB (N * n * 4 bytes) runs as one launch, one workgroup per
+ * replica with its spins in LDS; other calls launch per half sweep.  Both
+ * forms give the same spins and series.
  * Replaces critmc/kernels/on.py:348-418 (run). */
 int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,
               const float *betas_dev, const float *steps_dev, const uint32_t *keys_dev,
