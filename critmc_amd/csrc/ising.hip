@@ -306,6 +306,19 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
     const FastDiv dG = make_fastdiv((uint32_t)G);
     const int dy = (int)blockDim.x % G == 0 ? (int)blockDim.x / G : 0;  // rows per stride, 0: divide
+    // Row-parity split (rows of at least 32 words' worth of sites, whole waves
+    // and whole rows per half workgroup): the first half of the workgroup takes
+    // the even rows, the second half the odd rows.  A wave then holds rows of
+    // one parity, so which side supplies the shifted-in bit is uniform across
+    // it, and a lane keeps its word column for the whole half sweep.
+    const int hb = (int)blockDim.x >> 1;
+    const bool rowpar = M >= 32 && (hb & 63) == 0 && hb % G == 0;
+    const int par = (int)threadIdx.x >= hb ? 1 : 0;
+    const int u = (int)threadIdx.x - par * hb;
+    const int u_row = (int)fdiv((uint32_t)u, dG), u_w = u - u_row * G;
+    const int y0 = 2 * u_row + par, ystep = rowpar ? 2 * (hb / G) : 1, istep = ystep * G;
+    const int hoff0 = u_w == 0 ? G - 1 : -1;     // colour parity 0: the word before, wrapping
+    const int hoff1 = u_w == G - 1 ? 1 - G : 1;  // colour parity 1: the word after, wrapping
     if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
     uint32_t hs = 0;
 
@@ -328,16 +341,9 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                         uint32_t* Sc = lds + c * W;
                         const uint32_t* So = lds + (1 - c) * W;
                         int* qc = qcount + (hs & 1);
-                        // rows step by blockDim / G when G divides blockDim: no division per word
-                        int y = (int)fdiv(threadIdx.x, dG), w = (int)threadIdx.x - y * G;
-                        for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
-                            if (dy == 0) {
-                                y = (int)fdiv((uint32_t)idx, dG);
-                                w = idx - y * G;
-                            }
-                            uint32_t own, nb[4];
-                            batched_word(Sc, So, idx, y, w, G, W, M, L1, c, valid, own, nb);
-                            y += dy;
+                        // decide one word within the draw budget; still undecided, it
+                        // goes to the queue (or, with the queue full, finishes here)
+                        auto update = [&](int idx, uint32_t own, const uint32_t (&nb)[4]) {
                             WordDecision<NCMAX> wd;
                             word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
                             word_draw<NCMAX>(wd, 0, ISING_DRAW_BUDGET, k0, k1, sid, sweep,
@@ -348,13 +354,47 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                                     queue[3 * slot] = (uint32_t)idx;
                                     queue[3 * slot + 1] = wd.pending;
                                     queue[3 * slot + 2] = wd.below;
-                                    continue;
+                                    return;
                                 }
                                 word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                                  (uint32_t)c, (uint32_t)idx);
                             }
                             Sc[idx] = code == CODE_METROPOLIS ? word_apply_metropolis<NCMAX>(own, valid, wd, fc)
                                                               : word_apply<4, NCMAX>(code, own, valid, wd, d);
+                        };
+                        if (rowpar) {
+                            // even rows can wrap only downwards (row 0), odd rows only
+                            // upwards (row L1 - 1); the count is symmetric in the
+                            // neighbours, so their order does not matter
+                            const int wpar = __builtin_amdgcn_readfirstlane(par);
+                            const int p = (wpar + c) & 1;
+                            const int hoff = p ? hoff1 : hoff0;
+                            const int ywrap = wpar ? L1 - 1 : 0;
+                            const int off_in = wpar ? -G : G, off_out = -off_in;
+                            const int off_wrap = wpar ? G - W : W - G;
+                            for (int y = y0, idx = y0 * G + u_w; y < L1; y += ystep, idx += istep) {
+                                const uint32_t* q = So + idx;
+                                const uint32_t own = Sc[idx], same = q[0], o = q[hoff];
+                                uint32_t nb[4];
+                                nb[0] = same;
+                                nb[1] = p == 0 ? (same << 1) | (o >> 31) : (same >> 1) | (o << 31);
+                                nb[2] = q[off_in];
+                                nb[3] = q[y == ywrap ? off_wrap : off_out];
+                                update(idx, own, nb);
+                            }
+                        } else {
+                            // rows step by blockDim / G when G divides blockDim: no division per word
+                            int y = (int)fdiv(threadIdx.x, dG), w = (int)threadIdx.x - y * G;
+                            for (int idx = threadIdx.x; idx < W; idx += blockDim.x) {
+                                if (dy == 0) {
+                                    y = (int)fdiv((uint32_t)idx, dG);
+                                    w = idx - y * G;
+                                }
+                                uint32_t own, nb[4];
+                                batched_word(Sc, So, idx, y, w, G, W, M, L1, c, valid, own, nb);
+                                y += dy;
+                                update(idx, own, nb);
+                            }
                         }
                         __syncthreads();
                         const int nq = min(*qc, ISING_QCAP);

@@ -75,6 +75,10 @@ CASES = [
     ((64, 64), K.FORM_BATCHED, 0),
     ((128, 32), K.FORM_BATCHED, 0),
     ((256, 256), K.FORM_BATCHED, 0),
+    ((512, 34), K.FORM_BATCHED, 0),   # row-parity split, a ragged last stride
+    ((128, 128), K.FORM_BATCHED, 1),  # row-parity split, two words per row
+    ((64, 256), K.FORM_BATCHED, 0),   # row-parity split, one word per row
+    ((384, 64), K.FORM_BATCHED, 0),   # six words per row: no split
     ((64, 64), K.FORM_BATCHED, 1),
     ((256, 16), K.FORM_BATCHED, 1),
     ((6, 10), K.FORM_BATCHED, 0),
