@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "cb_geom.hpp"
@@ -336,16 +337,22 @@ __global__ __launch_bounds__(256) void ising2d_batched(
             FlipCounts fc{0, 0, 0, 0, 0};
             for (int row = 0; row < sched.n; ++row) {
                 const int code = sched.code[row];
+                ClassTab<NCMAX> ct;
+                decode_classes<NCMAX>(code == CODE_METROPOLIS ? tab : tab + 32, ct);
                 for (int rp = 0; rp < sched.count[row]; ++rp) {
                     for (int c = 0; c < 2; ++c, ++hs) {
                         uint32_t* Sc = lds + c * W;
                         const uint32_t* So = lds + (1 - c) * W;
                         int* qc = qcount + (hs & 1);
+                        // one half sweep; SEL (Metropolis in a field) as a template
+                        // argument keeps the spin selection out of the other rules
+                        auto half = [&](auto sel_tag) {
+                        constexpr bool SEL = decltype(sel_tag)::value;
                         // decide one word within the draw budget; still undecided, it
                         // goes to the queue (or, with the queue full, finishes here)
                         auto update = [&](int idx, uint32_t own, const uint32_t (&nb)[4]) {
                             WordDecision<NCMAX> wd;
-                            word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
+                            word_classify_decoded<4, NCMAX, SEL>(code, own, nb, valid, ct, wd);
                             word_draw<NCMAX>(wd, 0, ISING_DRAW_BUDGET, k0, k1, sid, sweep,
                                              (uint32_t)c, (uint32_t)idx);
                             if (wd.pending) {
@@ -405,7 +412,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                             uint32_t own, nb[4];
                             batched_word(Sc, So, idx, qy, idx - qy * G, G, W, M, L1, c, valid, own, nb);
                             WordDecision<NCMAX> wd;
-                            word_classify<4, NCMAX>(code, own, nb, valid, tab, wd);
+                            word_classify_decoded<4, NCMAX, SEL>(code, own, nb, valid, ct, wd);
                             wd.pending = queue[3 * t + 1];
                             wd.below = queue[3 * t + 2];
                             word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
@@ -414,6 +421,9 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                                                               : word_apply<4, NCMAX>(code, own, valid, wd, d);
                         }
                         __syncthreads();
+                        };
+                        if (ct.anysel) half(std::true_type{});
+                        else half(std::false_type{});
                     }
                     ++sweep;
                 }

@@ -116,6 +116,78 @@ __host__ __device__ __forceinline__ void word_classify(int code, uint32_t own, c
     w.below = valid & ~pending;
 }
 
+// One update rule's class table decoded once per half sweep: for class k,
+// (c_b ^ x_b[k]) is all-ones in the lanes whose count bit b equals bit b of the
+// class's count value, and (own ^ sa[k]) | sb[k] keeps the lanes of the
+// class's spin (all lanes for selector 0).  Unused classes get count value 7,
+// which a count of at most 6 neighbours never takes, so their masks are empty
+// without a branch.
+template <int NCMAX>
+struct ClassTab {
+    uint32_t x0[NCMAX], x1[NCMAX], x2[NCMAX], sa[NCMAX], sb[NCMAX], thr[NCMAX];
+    uint32_t anysel;  // nonzero when some class selects a spin
+};
+
+template <int NCMAX>
+__host__ __device__ __forceinline__ void decode_classes(const uint32_t* __restrict__ tab,
+                                                        ClassTab<NCMAX>& t)
+{
+    const uint32_t ncls = tab[0];
+    t.anysel = 0;
+#pragma unroll
+    for (int k = 0; k < NCMAX; ++k) {
+        uint32_t v = 7u, sel = 0u, thr = 0u;
+        if ((uint32_t)k < ncls) {
+            const uint32_t d = tab[1 + k];
+            v = d & 0xffu;
+            sel = d >> 8;
+            thr = tab[TAB_T + k];
+        }
+        t.x0[k] = (v & 1u) ? 0u : ~0u;
+        t.x1[k] = (v & 2u) ? 0u : ~0u;
+        t.x2[k] = (v & 4u) ? 0u : ~0u;
+        t.sa[k] = sel == 2u ? 0u : ~0u;
+        t.sb[k] = sel == 0u ? ~0u : 0u;
+        t.thr[k] = thr;
+        t.anysel |= sel;
+    }
+}
+
+// word_classify with the table already decoded (same masks for valid lanes;
+// a class mask may include invalid lanes, which are never pending, so no
+// decision and no count sees them).  SEL: some class selects a spin
+// (Metropolis in a field); without it the spin selection is skipped.
+template <int Z, int NCMAX, bool SEL>
+__host__ __device__ __forceinline__ void word_classify_decoded(int code, uint32_t own,
+                                                               const uint32_t (&nb)[Z],
+                                                               uint32_t valid,
+                                                               const ClassTab<NCMAX>& t,
+                                                               WordDecision<NCMAX>& w)
+{
+    if (code == CODE_METROPOLIS) {
+        uint32_t in[Z];
+#pragma unroll
+        for (int k = 0; k < Z; ++k) in[k] = ~(own ^ nb[k]);  // aligned neighbours
+        count_bits<Z>(in, w.c0, w.c1, w.c2);
+    } else {
+        count_bits<Z>(nb, w.c0, w.c1, w.c2);  // neighbours up
+    }
+#pragma unroll
+    for (int k = 0; k < NCMAX; ++k) {
+        w.mask[k] = (w.c0 ^ t.x0[k]) & (w.c1 ^ t.x1[k]) & (w.c2 ^ t.x2[k]);
+        w.thr[k] = t.thr[k];
+    }
+    if constexpr (SEL) {
+#pragma unroll
+        for (int k = 0; k < NCMAX; ++k) w.mask[k] &= (own ^ t.sa[k]) | t.sb[k];
+    }
+    uint32_t pending = 0;
+#pragma unroll
+    for (int k = 0; k < NCMAX; ++k) pending |= w.mask[k];
+    w.pending = pending & valid;
+    w.below = valid & ~pending;
+}
+
 // Continue the comparison with Philox blocks q_begin .. q_end-1 of the word's
 // group (block 8 group + q gives planes 4q .. 4q+3, most significant first);
 // stops early once no lane is pending, which cannot change a decision.
