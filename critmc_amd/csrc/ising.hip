@@ -339,20 +339,27 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                 const int code = sched.code[row];
                 ClassTab<NCMAX> ct;
                 decode_classes<NCMAX>(code == CODE_METROPOLIS ? tab : tab + 32, ct);
+                const bool met34 = code == CODE_METROPOLIS && table_is_met34(tab);
                 for (int rp = 0; rp < sched.count[row]; ++rp) {
                     for (int c = 0; c < 2; ++c, ++hs) {
                         uint32_t* Sc = lds + c * W;
                         const uint32_t* So = lds + (1 - c) * W;
                         int* qc = qcount + (hs & 1);
-                        // one half sweep; SEL (Metropolis in a field) as a template
-                        // argument keeps the spin selection out of the other rules
-                        auto half = [&](auto sel_tag) {
-                        constexpr bool SEL = decltype(sel_tag)::value;
+                        // one half sweep, the class rule as a template argument:
+                        // 0 any table, 1 Metropolis in a field (spin selection),
+                        // 2 zero-field Metropolis at z = 4 (classes 3 and 4)
+                        auto half = [&](auto mode_tag) {
+                        constexpr int MODE = decltype(mode_tag)::value;
+                        auto classify = [&](uint32_t own, const uint32_t (&nb)[4],
+                                            WordDecision<NCMAX>& wd) {
+                            if constexpr (MODE == 2) word_classify_met34<NCMAX>(own, nb, valid, ct, wd);
+                            else word_classify_decoded<4, NCMAX, MODE == 1>(code, own, nb, valid, ct, wd);
+                        };
                         // decide one word within the draw budget; still undecided, it
                         // goes to the queue (or, with the queue full, finishes here)
                         auto update = [&](int idx, uint32_t own, const uint32_t (&nb)[4]) {
                             WordDecision<NCMAX> wd;
-                            word_classify_decoded<4, NCMAX, SEL>(code, own, nb, valid, ct, wd);
+                            classify(own, nb, wd);
                             word_draw<NCMAX>(wd, 0, ISING_DRAW_BUDGET, k0, k1, sid, sweep,
                                              (uint32_t)c, (uint32_t)idx);
                             if (wd.pending) {
@@ -412,7 +419,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                             uint32_t own, nb[4];
                             batched_word(Sc, So, idx, qy, idx - qy * G, G, W, M, L1, c, valid, own, nb);
                             WordDecision<NCMAX> wd;
-                            word_classify_decoded<4, NCMAX, SEL>(code, own, nb, valid, ct, wd);
+                            classify(own, nb, wd);
                             wd.pending = queue[3 * t + 1];
                             wd.below = queue[3 * t + 2];
                             word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
@@ -422,8 +429,18 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                         }
                         __syncthreads();
                         };
-                        if (ct.anysel) half(std::true_type{});
-                        else half(std::false_type{});
+                        if (ct.anysel) {
+                            half(std::integral_constant<int, 1>{});
+                        } else {
+                            bool done = false;
+                            if constexpr (NCMAX == 2) {
+                                if (met34) {
+                                    half(std::integral_constant<int, 2>{});
+                                    done = true;
+                                }
+                            }
+                            if (!done) half(std::integral_constant<int, 0>{});
+                        }
                     }
                     ++sweep;
                 }

@@ -188,6 +188,36 @@ __host__ __device__ __forceinline__ void word_classify_decoded(int code, uint32_
     w.below = valid & ~pending;
 }
 
+// Metropolis at z = 4 without a field, whose classes are exactly aligned
+// counts 3 and 4 (every beta > 0): a count of at most 4 has c2 set only at 4,
+// and c0 = c1 = 1 only at 3, so the class masks are c0 & c1 and c2.
+template <int NCMAX>
+__host__ __device__ __forceinline__ void word_classify_met34(uint32_t own, const uint32_t (&nb)[4],
+                                                             uint32_t valid, const ClassTab<NCMAX>& t,
+                                                             WordDecision<NCMAX>& w)
+{
+    uint32_t in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) in[k] = ~(own ^ nb[k]);  // aligned neighbours
+    count_bits<4>(in, w.c0, w.c1, w.c2);
+#pragma unroll
+    for (int k = 0; k < NCMAX; ++k) {
+        w.mask[k] = k == 0 ? w.c0 & w.c1 : k == 1 ? w.c2 : 0u;
+        w.thr[k] = t.thr[k];
+    }
+    const uint32_t cls = w.mask[0] | w.mask[1];
+    w.pending = cls & valid;
+    w.below = valid & ~cls;
+}
+
+// True when a decoded Metropolis table is the z = 4, zero-field one that
+// word_classify_met34 handles: two classes, aligned counts 3 and 4, no spin
+// selection.
+__host__ __device__ __forceinline__ bool table_is_met34(const uint32_t* __restrict__ tab)
+{
+    return tab[0] == 2u && tab[1] == 3u && tab[2] == 4u;
+}
+
 // Continue the comparison with Philox blocks q_begin .. q_end-1 of the word's
 // group (block 8 group + q gives planes 4q .. 4q+3, most significant first);
 // stops early once no lane is pending, which cannot change a decision.
