@@ -288,7 +288,10 @@ __device__ __forceinline__ void batched_word(const uint32_t* __restrict__ Sc,
 constexpr uint32_t ISING_DRAW_BUDGET = CM_ISING_DRAW_BUDGET;
 constexpr int ISING_QCAP = 512;
 
-template <int LAYOUT, int NCMAX>
+// ZF: every row of the schedule is Metropolis and every replica's table is the
+// zero-field z = 4 one (classes 3 and 4), checked on the host; only that rule
+// is compiled, which keeps the kernel's register use down.
+template <int LAYOUT, int NCMAX, bool ZF>
 __global__ __launch_bounds__(256) void ising2d_batched(
     void* __restrict__ spins, int R, int L0, int L1, const uint32_t* __restrict__ tabs,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ sids,
@@ -373,8 +376,8 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                                 word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                                  (uint32_t)c, (uint32_t)idx);
                             }
-                            Sc[idx] = code == CODE_METROPOLIS ? word_apply_metropolis<NCMAX>(own, valid, wd, fc)
-                                                              : word_apply<4, NCMAX>(code, own, valid, wd, d);
+                            if (MODE == 2 || code == CODE_METROPOLIS) Sc[idx] = word_apply_metropolis<NCMAX>(own, valid, wd, fc);
+                            else Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
                         };
                         if (rowpar) {
                             // even rows can wrap only downwards (row 0), odd rows only
@@ -424,12 +427,14 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                             wd.below = queue[3 * t + 2];
                             word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                              (uint32_t)c, (uint32_t)idx);
-                            Sc[idx] = code == CODE_METROPOLIS ? word_apply_metropolis<NCMAX>(own, valid, wd, fc)
-                                                              : word_apply<4, NCMAX>(code, own, valid, wd, d);
+                            if (MODE == 2 || code == CODE_METROPOLIS) Sc[idx] = word_apply_metropolis<NCMAX>(own, valid, wd, fc);
+                            else Sc[idx] = word_apply<4, NCMAX>(code, own, valid, wd, d);
                         }
                         __syncthreads();
                         };
-                        if (ct.anysel) {
+                        if constexpr (ZF) {
+                            half(std::integral_constant<int, 2>{});
+                        } else if (ct.anysel) {
                             half(std::integral_constant<int, 1>{});
                         } else {
                             bool done = false;
@@ -805,7 +810,7 @@ static void launch_plain_nc(int nc, int8_t* s, const Geo& g, int code, int colou
     }
 }
 
-template <int LAYOUT, int NC>
+template <int LAYOUT, int NC, bool ZF = false>
 static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_t* tabs,
                                  const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
                                  const Sched& sc, int64_t n_meas, int32_t* e_run, int32_t* m_run,
@@ -814,11 +819,11 @@ static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_
 {
     const size_t lds = (size_t)2 * g.L1 * g.G * 4 + 3 * 16 * 4 + (3 * ISING_QCAP + 2) * 4;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)ising2d_batched<LAYOUT, NC>,
+        hipError_t e = hipFuncSetAttribute((const void*)ising2d_batched<LAYOUT, NC, ZF>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((ising2d_batched<LAYOUT, NC>), dim3(grid), dim3(block), lds, st, spins, R,
+    hipLaunchKernelGGL((ising2d_batched<LAYOUT, NC, ZF>), dim3(grid), dim3(block), lds, st, spins, R,
                        g.L0, g.L1, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out,
                        m_out, acc_out, n_meas_rep);
     return hipGetLastError();
@@ -830,8 +835,11 @@ static hipError_t launch_batched_nc(int nc, void* spins, int R, const Geo& g,
                                     const uint32_t* sids, uint64_t* sweeps, const Sched& sc,
                                     int64_t n_meas, int32_t* e_run, int32_t* m_run,
                                     int32_t* e_out, int32_t* m_out, int32_t* acc_out,
-                                    const int64_t* n_meas_rep, int block, int grid, hipStream_t st)
+                                    const int64_t* n_meas_rep, int block, int grid, hipStream_t st,
+                                    bool zf)
 {
+    if (nc == 2 && zf)
+        return launch_batched<LAYOUT, 2, true>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
     switch (nc) {
         case 2: return launch_batched<LAYOUT, 2>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
         case 4: return launch_batched<LAYOUT, 4>(spins, R, g, tabs, keys, sids, sweeps, sc, n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
@@ -1041,12 +1049,16 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         CM_REQUIRE(block % 64 == 0 && block <= 256, "block must be a multiple of 64, <= 256");
         if (grid <= 0) grid = n_rep;
         grid = std::min(grid, n_rep);
+        // zero-field Metropolis at z = 4 in every row and replica: the kernel
+        // compiled for that rule alone
+        bool zf = g.d == 2 && uses_met && !uses_hb;
+        for (int r = 0; zf && r < n_rep; ++r) zf = table_is_met34(&tabs[(size_t)r * TAB_WORDS]);
         hipError_t err = layout == 0
             ? launch_batched_nc<0>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
                                    n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block,
-                                   grid, st)
+                                   grid, st, zf)
             : launch_batched_nc<1>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
-                                   n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st);
+                                   n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block, grid, st, zf);
         CM_HIP_CHECK(err);
         return CM_OK;
     }
