@@ -310,8 +310,8 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
     const FastDiv dG = make_fastdiv((uint32_t)G);
     const int dy = (int)blockDim.x % G == 0 ? (int)blockDim.x / G : 0;  // rows per stride, 0: divide
-    // Row-parity split (rows of at least 32 words' worth of sites, whole waves
-    // and whole rows per half workgroup): the first half of the workgroup takes
+    // Row-parity split (rows of whole 32-site words, whole waves and whole
+    // rows per half workgroup): the first half of the workgroup takes
     // the even rows, the second half the odd rows.  A wave then holds rows of
     // one parity, so which side supplies the shifted-in bit is uniform across
     // it, and a lane keeps its word column for the whole half sweep.
