@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void ising2d_batched(
     const int G = (M + 31) >> 5;
     const int W = L1 * G;
     int* red = (int*)(lds + 2 * W);         // [16][3]
-    uint32_t* queue = lds + 2 * W + 48;     // [ISING_QCAP][3]: word, pending, below
+    uint32_t* queue = lds + 2 * W + 48;     // [3][ISING_QCAP]: words, pending, below
     int* qcount = (int*)(queue + 3 * ISING_QCAP);  // [2], alternating half sweeps
     const int nwaves = (blockDim.x + 63) >> 6;
     const uint32_t valid = M >= 32 ? ~0u : ((1u << M) - 1u);
@@ -368,9 +368,9 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                             if (wd.pending) {
                                 const int slot = atomicAdd(qc, 1);
                                 if (slot < ISING_QCAP) {
-                                    queue[3 * slot] = (uint32_t)idx;
-                                    queue[3 * slot + 1] = wd.pending;
-                                    queue[3 * slot + 2] = wd.below;
+                                    queue[slot] = (uint32_t)idx;
+                                    queue[ISING_QCAP + slot] = wd.pending;
+                                    queue[2 * ISING_QCAP + slot] = wd.below;
                                     return;
                                 }
                                 word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
@@ -417,14 +417,14 @@ __global__ __launch_bounds__(256) void ising2d_batched(
                         const int nq = min(*qc, ISING_QCAP);
                         if (threadIdx.x == 0) qcount[(hs + 1) & 1] = 0;
                         for (int t = threadIdx.x; t < nq; t += blockDim.x) {
-                            const int idx = (int)queue[3 * t];
+                            const int idx = (int)queue[t];
                             const int qy = (int)fdiv((uint32_t)idx, dG);
                             uint32_t own, nb[4];
                             batched_word(Sc, So, idx, qy, idx - qy * G, G, W, M, L1, c, valid, own, nb);
                             WordDecision<NCMAX> wd;
                             classify(own, nb, wd);
-                            wd.pending = queue[3 * t + 1];
-                            wd.below = queue[3 * t + 2];
+                            wd.pending = queue[ISING_QCAP + t];
+                            wd.below = queue[2 * ISING_QCAP + t];
                             word_draw<NCMAX>(wd, ISING_DRAW_BUDGET, 8, k0, k1, sid, sweep,
                                              (uint32_t)c, (uint32_t)idx);
                             if (MODE == 2 || code == CODE_METROPOLIS) Sc[idx] = word_apply_metropolis<NCMAX>(own, valid, wd, fc);
