@@ -30,6 +30,11 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 L, R, SEED = 256, 4096, 1234
+# VALU wave-instructions per site update of the zero-field Metropolis batched
+# kernel on this workload, from rocprofv3 SQ_INSTS_VALU (69,804,528,954 over
+# 4096 x 65536 x 2000 site updates); MI355X: 256 CUs at 2.4 GHz peak clock.
+VALU_PER_SITE_UPDATE = 0.130
+CU_COUNT, CLOCK_HZ = 256, 2.4e9
 
 
 def parse():
@@ -43,9 +48,11 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=R)
     ap.add_argument("--plain-sweeps", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="seconds of CPU work per baseline leg (typewriter, checkerboard)")
+    ap.add_argument("--warmup-seconds", type=float, default=1.0,
+                    help="minimum warm-up time, in launches of the timed shape")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "ising2d_traffic.json"))
     ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "potts", "serve", "copy"),
                     default="ising2d",
                     help="ising2d: the headline batch; ea / xy / serve: the other configurations; "
@@ -61,30 +68,92 @@ def parse():
     return ap.parse_args()
 
 
+def _host_cpu() -> dict:
+    """Core counts and model name of the host the baseline runs on."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": avail, "model": model}
+
+
+def _cpu_threads() -> int:
+    """Every core this process may use, within the OMP_NUM_THREADS share when one is set."""
+    host = _host_cpu()
+    n = host["affinity"] or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def _timed_chunks(step, seconds: float) -> tuple[int, float]:
+    """Run ``step(n)`` in chunks of about an eighth of ``seconds``; returns (units, elapsed)."""
+    step(2)  # threads and pages warm
+    t0 = time.perf_counter()
+    step(8)
+    chunk = max(2, int(8 * seconds / 8 / max(time.perf_counter() - t0, 1e-3)))
+    done = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        step(chunk)
+        done += chunk
+    return done, time.perf_counter() - t0
+
+
 def cpu_baseline(seconds: float) -> dict:
-    """Oracle typewriter Metropolis (the reference's sweep order) on host cores."""
+    """The oracle's C restatements on the host cores, one replica per thread.
+
+    Primary leg: the reference's typewriter Metropolis (kernels/ising.py:68-87).
+    Second leg: the checkerboard replay the GPU chains are bit-exact against.
+    Both on a bounded sample of the headline workload (256^2 replicas, beta in
+    [0.40, 0.48], hot start), with the rate flat in L and in sweeps.
+    """
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = _cpu_threads()
     reps = threads
     rng = np.random.default_rng(SEED)
     s = (2 * rng.integers(0, 2, size=(reps, L * L)) - 1).astype(np.int8)
     betas = rng.uniform(0.40, 0.48, size=reps)
     keys = rng.integers(0, 1 << 32, size=(reps, 2), dtype=np.uint64)
-    oracle.ising_batch_typewriter(s, (L, L), betas, 0.0, keys, 2, threads)  # threads and pages warm
-    t0 = time.perf_counter()
-    oracle.ising_batch_typewriter(s, (L, L), betas, 0.0, keys, 20, threads)
-    chunk = max(2, int(20 * seconds / 8 / max(time.perf_counter() - t0, 1e-3)))
-    sweeps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:  # chunks of about an eighth of the sample
-        oracle.ising_batch_typewriter(s, (L, L), betas, 0.0, keys, chunk, threads)
-        sweeps += chunk
-    dt = time.perf_counter() - t0
-    return {"value": reps * L * L * sweeps / dt / 1e9, "unit": "site updates/ns", "cores": threads,
-            "kind": "port",
-            "sample": f"{reps} replicas x {L}x{L} x {sweeps} typewriter Metropolis sweeps, "
-                      f"one replica per thread ({dt:.1f} s)"}
+
+    def typewriter(n):
+        oracle.ising_batch_typewriter(s, (L, L), betas, 0.0, keys, n, threads)
+
+    tw_sweeps, tw_dt = _timed_chunks(typewriter, seconds)
+
+    s2 = s.copy()
+    sids = np.arange(reps, dtype=np.uint32)
+    sweeps = np.zeros(reps, np.uint64)
+
+    def checkerboard(n):
+        oracle.ising_batch_checkerboard(s2, (L, L), betas, 0.0, keys, sids, sweeps, [[0, 1]], n,
+                                        threads)
+
+    cb_sweeps, cb_dt = _timed_chunks(checkerboard, seconds)
+    host = _host_cpu()
+    return {"value": reps * L * L * tw_sweeps / tw_dt / 1e9, "unit": "site updates/ns",
+            "cores": threads, "kind": "port",
+            "sample": f"{reps} replicas x {L}x{L} x {tw_sweeps} typewriter Metropolis sweeps "
+                      f"(kernels/ising.py:68-87 restated in C, -O2), one replica per thread "
+                      f"({tw_dt:.1f} s)",
+            "host": host,
+            "cores_note": "threads = the CPU share this process may use (affinity, capped by "
+                          "OMP_NUM_THREADS where the host sets it)",
+            "checkerboard_replay": {
+                "value": reps * L * L * cb_sweeps / cb_dt / 1e9, "unit": "site updates/ns",
+                "cores": threads,
+                "sample": f"{reps} replicas x {L}x{L} x {cb_sweeps} checkerboard Metropolis "
+                          f"sweeps of the bit-exactness replay, per-sweep series recorded "
+                          f"({cb_dt:.1f} s)"},
+            "published_single_core": {"value": 0.117, "unit": "site updates/ns",
+                                      "source": "reference numba Metropolis, i7-13700K, one core"}}
 
 
 def bench_ea(args) -> dict:
@@ -327,29 +396,40 @@ def main():
     n_launch = -(-args.steps // spl)
     out = tuple(torch.empty((Rn, spl), dtype=torch.int32, device=dev) for _ in range(3))
 
-    def sweep_block(n):
-        o = out if n == spl else None
-        return K.run(state, shape, betas, 0.0, chains, sched, n, e_run, m_run, layout=layout,
-                     form=K.FORM_BATCHED, block=args.block, ws=ws, out=o)
+    full_launch = K.BoundRun(state, shape, betas, 0.0, chains, sched, spl, e_run, m_run, out,
+                             layout=layout, form=K.FORM_BATCHED, block=args.block, ws=ws)
 
-    if args.warmup:
-        sweep_block(min(args.warmup, spl))
-    torch.cuda.synchronize()
+    def sweep_block(n):
+        if n == spl:
+            return full_launch()
+        return K.run(state, shape, betas, 0.0, chains, sched, n, e_run, m_run, layout=layout,
+                     form=K.FORM_BATCHED, block=args.block, ws=ws)
+
+    # Warm-up in launches of the timed shape: at least W sweeps and at least
+    # --warmup-seconds, so clocks, code objects, the class tables (built and
+    # uploaded once by the workspace) and the output buffers are all settled.
+    warm_sweeps = 0
+    t_w = time.perf_counter()
+    while warm_sweeps < args.warmup or time.perf_counter() - t_w < args.warmup_seconds:
+        sweep_block(spl)
+        warm_sweeps += spl
+        torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n_launch + 1)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_launch)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev[0].record(stream)
     done = 0
     for i in range(n_launch):
         k = min(spl, args.steps - done)
+        ev[2 * i].record(stream)
         sweep_block(k)
+        ev[2 * i + 1].record(stream)
         done += k
-        ev[i + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(n_launch)]
-    gpu_ms = sum(launch_ms)
+    launch_ms = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(n_launch)]
+    gpu_ms = ev[0].elapsed_time(ev[-1])
     updates = Rn * L * L * args.steps
     value = updates / (gpu_ms * 1e6)
 
@@ -362,11 +442,14 @@ def main():
     full = [t for t, i in zip(launch_ms, range(n_launch)) if (i + 1) * spl <= args.steps] or launch_ms
     mean_launch_s = float(np.mean(full)) / 1e3
     achieved = bytes_per_update * Rn * L * L * spl / mean_launch_s / 1e9
-    traffic = None
-    tpath = Path(args.traffic)
-    if tpath.exists():
-        t = json.loads(tpath.read_text())
-        traffic = t.get(f"{args.layout}_L{L}_R{Rn}_spl{spl}_bytes_per_launch")
+    # HBM bytes one launch moves: the state read and written once, the class
+    # tables and per-chain counters read, the int32 E_b / M / accepted series
+    # written (12 B per replica and sweep); calibrated against rocprofv3
+    # FETCH_SIZE + WRITE_SIZE of this kernel at 20, 200 and 10^4 sweeps
+    state_bytes = Rn * L * L * (1 if layout == K.LAYOUT_INT8 else 0.125)
+    traffic = int(2 * state_bytes + Rn * (256 + 16) + Rn * (12 * spl + 16))
+    rate_updates_per_s = Rn * L * L * spl / mean_launch_s
+    valu_issue = VALU_PER_SITE_UPDATE * rate_updates_per_s / (CU_COUNT * CLOCK_HZ)
 
     # plain one-chain-per-launch form of the same kernel on one replica
     plain_rate = None
@@ -390,6 +473,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_sweeps_run": warm_sweeps,
         "ms_per_step": gpu_ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -403,10 +487,19 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "basis": f"{bytes_per_update:g} B per site update (each spin read and written "
-                              "once per sweep, the streaming bound) over the mean launch time; "
-                              "each replica stays in LDS for every sweep of a launch, so the HBM "
-                              "bytes actually moved are 'traffic' and VALU issue is the "
-                              "kernel's binding limit"},
+                              "once per sweep: the streaming model) x site updates per launch / "
+                              "mean launch time. Each replica stays in LDS for every sweep of a "
+                              "launch, so the HBM bytes actually moved ('traffic', read and "
+                              "written once per launch plus series) are far below the model, "
+                              "and frac above 1 is the gain of keeping replicas on chip, not HBM "
+                              "bandwidth; the binding limit is VALU issue ('binding')"},
+        "binding": {"bound": "valu", "unit": "VALU wave-instructions per CU-cycle",
+                    "achieved": round(valu_issue, 3), "peak": 1.0,
+                    "frac": round(valu_issue, 3),
+                    "basis": f"{VALU_PER_SITE_UPDATE} VALU wave-instructions per site update "
+                             "(rocprofv3 SQ_INSTS_VALU of this kernel and workload) x site "
+                             f"updates/s / ({CU_COUNT} CUs x {CLOCK_HZ / 1e9:g} GHz); a CU "
+                             "issues at most one VALU wave-instruction per cycle"},
         "plain_one_chain_rate": plain_rate,
         "vs_plain_one_chain": (value / plain_rate) if plain_rate else None,
         "wall_s": wall,

@@ -44,9 +44,11 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_ising_convert": ([_vp, _vp, _c_i32, _i32p, _c_i32, _c_i32, _vp], _c_i32),
     "cm_ising_run": (
         [_vp, _c_i32, _c_i32, _i32p, _c_i32, _f64p, _c_f64, _vp, _vp, _vp, _u64p, _i64p, _c_i32,
-         _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _c_i32, _vp, _vp],
+         _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _c_i32, _vp, _i32p, _vp],
         _c_i32,
     ),
+    "cm_ising_prepare": (
+        [_c_i32, _i32p, _c_i32, _f64p, _c_f64, _i64p, _c_i32, _vp, _c_i64, _i32p, _vp], _c_i32),
     "cm_ising_cluster_workspace_bytes": ([_c_i32, _i32p, _c_i32], _c_i64),
     "cm_ising_cluster_run": (
         [_vp, _c_i32, _i32p, _c_i32, _f64p, _c_f64, _vp, _vp, _vp, _u64p, _i64p, _c_i32, _c_i64,

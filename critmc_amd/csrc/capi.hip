@@ -52,13 +52,15 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t st
         if (e != hipSuccess) return e;
         st.pending = false;
     }
-    if (st.cap < bytes) {
+    if (st.cap < bytes) {  // grown in powers of two from 64 KiB, kept for the process
         if (st.host) (void)hipHostFree(st.host);
         st.host = nullptr;
         st.cap = 0;
-        e = hipHostMalloc(&st.host, bytes, hipHostMallocDefault);
+        size_t cap = 64 * 1024;
+        while (cap < bytes) cap *= 2;
+        e = hipHostMalloc(&st.host, cap, hipHostMallocDefault);
         if (e != hipSuccess) return e;
-        st.cap = bytes;
+        st.cap = cap;
     }
     std::memcpy(st.host, src, bytes);
     e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, stream);

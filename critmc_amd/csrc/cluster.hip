@@ -48,9 +48,12 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
                  int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run, int32_t* e_out,
                  int32_t* m_out, int32_t* acc_out, void* workspace, int64_t workspace_bytes,
                  int32_t form, int32_t block, int32_t grid, const int64_t* n_meas_rep,
-                 void* stream);
+                 const int32_t* prepared, void* stream);
 int cm_ising_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
                      int32_t* e_run, int32_t* m_run, void* stream);
+int cm_ising_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
+                     double h, const int64_t* schedule, int32_t n_rows, void* workspace,
+                     int64_t workspace_bytes, int32_t* info, void* stream);
 int64_t cm_ising_workspace_bytes(int32_t n_rep);
 int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
                  const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
@@ -478,6 +481,19 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
     int32_t* row_e = (int32_t*)(ws + lay.rowe);
     int32_t* row_m = (int32_t*)(ws + lay.rowm);
     const int rb = (n_rep + 255) / 256;
+    // the local rows' class tables are built and uploaded once per call; the
+    // record of each update code says which class loop its launches need
+    int32_t prep[2][2] = {{0, 0}, {0, 0}};
+    for (int row = 0; row < n_rows; ++row) {
+        const int64_t code = schedule[2 * row];
+        if (code != CODE_METROPOLIS && code != CODE_HEATBATH) continue;
+        int32_t* rec = prep[code == CODE_HEATBATH ? 1 : 0];
+        if (rec[0] != 0) continue;
+        const int64_t one[2] = {code, 1};
+        int rc = cm_ising_prepare(ndim, dims, n_rep, betas, h, one, 1, ws + lay.tab,
+                                  lay.parent - lay.tab, rec, stream);
+        if (rc != CM_OK) return rc;
+    }
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
@@ -489,7 +505,7 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                 int rc = cm_ising_run(spins, 0, ndim, dims, n_rep, betas, h, keys, stream_ids,
                                       sweeps, sweeps_host, one, 1, 1, e_run, m_run, row_e, row_m,
                                       row_acc, ws + lay.tab, lay.parent - lay.tab, 0, 0, 0,
-                                      nullptr, stream);
+                                      nullptr, prep[code == CODE_HEATBATH ? 1 : 0], stream);
                 if (rc != CM_OK) return rc;
                 hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
             } else {

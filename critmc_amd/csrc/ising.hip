@@ -894,6 +894,50 @@ static void launch_wide_nc(int nc, const dim3& grid, hipStream_t st, void* spins
     }
 }
 
+// Schedule rows (code, count) of a checkerboard run, validated.
+static int parse_sched(const int64_t* schedule, int n_rows, Sched* sc)
+{
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    sc->n = n_rows;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
+                   "update code %lld has no checkerboard Ising kernel", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
+        sc->code[r] = (int32_t)code;
+        sc->count[r] = (int32_t)count;
+    }
+    return CM_OK;
+}
+
+// Class tables of every chain (tabs: n_rep x TAB_WORDS) and the record the
+// launch needs: info[0] = the largest class count a row of this schedule
+// reads, info[1] = 1 when every row is Metropolis and every table is the
+// zero-field z = 4 one (the kernel compiled for that rule alone applies).
+static int prepare_tables(int z, int n_rep, const double* betas, double h, const Sched& sc,
+                          uint32_t* tabs, int32_t* info)
+{
+    bool uses_met = false, uses_hb = false;
+    for (int r = 0; r < sc.n; ++r) {
+        uses_met |= sc.code[r] == CODE_METROPOLIS;
+        uses_hb |= sc.code[r] == CODE_HEATBATH;
+    }
+    int nc_max = 1;
+    bool met34 = z == 4 && uses_met && !uses_hb;
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        uint32_t* tab = tabs + (size_t)r * TAB_WORDS;
+        int a, b;
+        build_tables(betas[r], h, z, tab, &a, &b);
+        if (uses_met) nc_max = std::max(nc_max, a);
+        if (uses_hb) nc_max = std::max(nc_max, b);
+        met34 = met34 && table_is_met34(tab);
+    }
+    info[0] = nc_max;
+    info[1] = met34 ? 1 : 0;
+    return CM_OK;
+}
+
 }  // namespace cm
 
 using namespace cm;
@@ -953,53 +997,57 @@ int cm_ising_convert(void* spins_i8, void* spins_packed, int32_t ndim, const int
     return CM_OK;
 }
 
+int cm_ising_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
+                     double h, const int64_t* schedule, int32_t n_rows, void* workspace,
+                     int64_t workspace_bytes, int32_t* info, void* stream)
+{
+    Geo g;
+    if (int rc = geo_from(ndim, dims, &g)) return rc;
+    CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
+    CM_REQUIRE(workspace_bytes >= cm_ising_workspace_bytes(n_rep), "workspace too small");
+    CM_REQUIRE(std::isfinite(h), "field h must be finite");
+    Sched sc{};
+    if (int rc = parse_sched(schedule, n_rows, &sc)) return rc;
+    std::vector<uint32_t> tabs((size_t)n_rep * TAB_WORDS);
+    if (int rc = prepare_tables(2 * g.d, n_rep, betas, h, sc, tabs.data(), info)) return rc;
+    CM_HIP_CHECK(upload_async(workspace, tabs.data(), tabs.size() * 4, (hipStream_t)stream));
+    return CM_OK;
+}
+
 int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims, int32_t n_rep,
                  const double* betas, double h, const uint32_t* keys, const uint32_t* stream_ids,
                  uint64_t* sweeps, const uint64_t* sweeps_host, const int64_t* schedule,
                  int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run, int32_t* e_out,
                  int32_t* m_out, int32_t* acc_out, void* workspace, int64_t workspace_bytes,
                  int32_t form, int32_t block, int32_t grid, const int64_t* n_meas_rep,
-                 void* stream)
+                 const int32_t* prepared, void* stream)
 {
     Geo g;
     if (int rc = geo_from(ndim, dims, &g)) return rc;
     CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
     CM_REQUIRE(layout == 0 || layout == 1, "layout must be 0 (int8) or 1 (packed)");
     CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
-    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
     CM_REQUIRE(workspace_bytes >= cm_ising_workspace_bytes(n_rep), "workspace too small");
     CM_REQUIRE(std::isfinite(h), "field h must be finite");
     Sched sc{};
-    sc.n = n_rows;
-    for (int r = 0; r < n_rows; ++r) {
-        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
-        CM_REQUIRE(code == CODE_METROPOLIS || code == CODE_HEATBATH,
-                   "update code %lld has no checkerboard Ising kernel", (long long)code);
-        CM_REQUIRE(count >= 1 && count <= (1 << 20), "repetition count out of range");
-        sc.code[r] = (int32_t)code;
-        sc.count[r] = (int32_t)count;
-    }
+    if (int rc = parse_sched(schedule, n_rows, &sc)) return rc;
     if (n_meas == 0) return CM_OK;
     const int z = 2 * g.d;
-    std::vector<uint32_t> tabs((size_t)n_rep * TAB_WORDS);
-    bool uses_met = false, uses_hb = false;
-    for (int r = 0; r < n_rows; ++r) {
-        uses_met |= sc.code[r] == CODE_METROPOLIS;
-        uses_hb |= sc.code[r] == CODE_HEATBATH;
-    }
-    // the class loop is sized for the tables this schedule actually reads
-    int nc_max = 1;
-    for (int r = 0; r < n_rep; ++r) {
-        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
-        int a, b;
-        build_tables(betas[r], h, z, &tabs[(size_t)r * TAB_WORDS], &a, &b);
-        if (uses_met) nc_max = std::max(nc_max, a);
-        if (uses_hb) nc_max = std::max(nc_max, b);
-    }
-    const int nc = pick_nc(nc_max);
     hipStream_t st = (hipStream_t)stream;
     uint32_t* dtabs = (uint32_t*)workspace;
-    CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * 4, st));
+    int32_t info[2];
+    if (prepared != nullptr) {  // tables already in the workspace (cm_ising_prepare)
+        CM_REQUIRE(prepared[0] >= 1 && prepared[0] <= NC_LIMIT && (prepared[1] & ~1) == 0,
+                   "prepared table record is not one cm_ising_prepare wrote");
+        info[0] = prepared[0];
+        info[1] = prepared[1];
+    } else {
+        std::vector<uint32_t> tabs((size_t)n_rep * TAB_WORDS);
+        if (int rc = prepare_tables(z, n_rep, betas, h, sc, tabs.data(), info)) return rc;
+        CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * 4, st));
+    }
+    const int nc = pick_nc(info[0]);
+    const bool zf_tables = info[1] != 0;
 
     if (form == 0) {
         if (prefer_wide(g, layout, n_rep)) form = 3;
@@ -1051,8 +1099,7 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         grid = std::min(grid, n_rep);
         // zero-field Metropolis at z = 4 in every row and replica: the kernel
         // compiled for that rule alone
-        bool zf = g.d == 2 && uses_met && !uses_hb;
-        for (int r = 0; zf && r < n_rep; ++r) zf = table_is_met34(&tabs[(size_t)r * TAB_WORDS]);
+        const bool zf = g.d == 2 && zf_tables;
         hipError_t err = layout == 0
             ? launch_batched_nc<0>(nc, spins, n_rep, g, dtabs, keys, stream_ids, sweeps, sc,
                                    n_meas, e_run, m_run, e_out, m_out, acc_out, n_meas_rep, block,

@@ -85,12 +85,36 @@ def unpack(packed: torch.Tensor, shape, R: int) -> torch.Tensor:
 
 
 class Workspace:
-    """Device scratch owned by the caller (class tables, one 256-byte row per chain)."""
+    """Device scratch owned by the caller (class tables, one 256-byte row per chain).
+
+    The tables are built and uploaded by ``cm_ising_prepare`` when the betas,
+    field, lattice dimension, update codes or stream differ from the previous
+    call that used this workspace; otherwise ``cm_ising_run`` reuses them with
+    no host work.
+    """
 
     def __init__(self, R: int, dev: torch.device | None = None):
         nbytes = int(lib().cm_ising_workspace_bytes(R))
         self.nbytes = nbytes
         self.buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev or device())
+        self._key = None
+        self._info = np.zeros(2, dtype=np.int32)
+
+    def prepare(self, dims: np.ndarray, betas: np.ndarray, h: float, sched: np.ndarray):
+        """The ``prepared`` record for these tables, uploading them first if needed."""
+        st = stream()
+        key = (len(dims), betas.tobytes(), float(h), tuple(sorted(set(sched[:, 0].tolist()))),
+               st.value)
+        if key != self._key:
+            self._key = None
+            rc = lib().cm_ising_prepare(len(dims), host_ptr(dims, ctypes.c_int32), len(betas),
+                                        host_ptr(betas, ctypes.c_double), float(h),
+                                        host_ptr(sched, ctypes.c_int64), sched.shape[0],
+                                        ptr(self.buf), self.nbytes,
+                                        host_ptr(self._info, ctypes.c_int32), st)
+            check(rc, "cm_ising_prepare")
+            self._key = key
+        return host_ptr(self._info, ctypes.c_int32)
 
 
 def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: np.ndarray,
@@ -117,18 +141,68 @@ def run(spins: torch.Tensor, shape, betas, h: float, chains: Chains, schedule: n
         out = tuple(torch.empty((R, n_meas), dtype=torch.int32, device=dev) for _ in range(3))
     e_out, m_out, acc_out = out
     ws = ws or Workspace(R, dev)
+    prepared = ws.prepare(d, betas, h, sched) if n_meas > 0 else None
     rc = lib().cm_ising_run(
         ptr(spins), layout, len(d), host_ptr(d, ctypes.c_int32), R,
         host_ptr(betas, ctypes.c_double), float(h), ptr(chains.keys_d), ptr(chains.sids_d),
         ptr(chains.sweeps_d), host_ptr(chains.sweeps_h, ctypes.c_uint64),
         host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas), ptr(e_run), ptr(m_run),
         ptr(e_out), ptr(m_out), ptr(acc_out), ptr(ws.buf), ws.nbytes, form, block, grid,
-        ptr(n_meas_rep), stream(),
+        ptr(n_meas_rep), prepared, stream(),
     )
     check(rc, "cm_ising_run")
     if n_meas_rep is None:
         chains.advance(int(sched[:, 1].sum()) * int(n_meas))
     return e_out, m_out, acc_out
+
+
+class BoundRun:
+    """One ``cm_ising_run`` call with every argument marshalled once.
+
+    For loops that repeat the same call (benchmarks, fixed-shape slices):
+    calling the object enqueues the run on the stream that was current at
+    binding time and advances the chains' host counters, with no other host
+    work.  The tables are prepared at binding time; the betas, field and
+    schedule are fixed for the object's life.  Batched and wide forms only
+    (the plain form reads the host counters per launch).
+    """
+
+    def __init__(self, spins: torch.Tensor, shape, betas, h: float, chains: Chains,
+                 schedule: np.ndarray, n_meas: int, e_run: torch.Tensor, m_run: torch.Tensor,
+                 out: tuple[torch.Tensor, torch.Tensor, torch.Tensor], *,
+                 layout: int = LAYOUT_INT8, form: int = FORM_BATCHED, block: int = 0,
+                 grid: int = 0, ws: Workspace | None = None,
+                 n_meas_rep: torch.Tensor | None = None):
+        if form not in (FORM_BATCHED, FORM_WIDE):
+            raise ValueError("BoundRun needs the batched or the wide form")
+        R = chains.R
+        self._betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, np.float64), (R,)))
+        self._sched = np.ascontiguousarray(schedule, dtype=np.int64).reshape(-1, 2)
+        self._dims = _dims(shape)
+        if n_meas < 1:
+            raise ValueError("n_meas must be positive")
+        for t in out:
+            if t.shape != (R, n_meas) or t.dtype != torch.int32:
+                raise ValueError("out tensors must be int32 [R, n_meas]")
+        self.ws = ws or Workspace(R, spins.device)
+        self._keep = (spins, chains, e_run, m_run, out, n_meas_rep)
+        self._chains = chains
+        self._per_call = int(self._sched[:, 1].sum()) * int(n_meas) if n_meas_rep is None else 0
+        prepared = self.ws.prepare(self._dims, self._betas, h, self._sched)
+        self._fn = lib().cm_ising_run
+        self._args = (
+            ptr(spins), layout, len(self._dims), host_ptr(self._dims, ctypes.c_int32), R,
+            host_ptr(self._betas, ctypes.c_double), float(h), ptr(chains.keys_d),
+            ptr(chains.sids_d), ptr(chains.sweeps_d),
+            host_ptr(chains.sweeps_h, ctypes.c_uint64), host_ptr(self._sched, ctypes.c_int64),
+            self._sched.shape[0], int(n_meas), ptr(e_run), ptr(m_run), ptr(out[0]), ptr(out[1]),
+            ptr(out[2]), ptr(self.ws.buf), self.ws.nbytes, form, block, grid, ptr(n_meas_rep),
+            prepared, stream())
+
+    def __call__(self) -> None:
+        check(self._fn(*self._args), "cm_ising_run")
+        if self._per_call:
+            self._chains.advance(self._per_call)
 
 
 class ClusterWorkspace:

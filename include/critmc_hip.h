@@ -98,6 +98,11 @@ int cm_ising_convert(void *spins_i8_dev, void *spins_packed_dev, int32_t ndim,
  *   n_meas_rep_dev: null, or int64 [n_rep] caps (batched and wide forms): chain r performs
  *           min(n_meas, n_meas_rep[r]) measurements (continuous batching of
  *           jobs with different remaining work)
+ *   prepared: null (the tables are built from betas and h and uploaded on
+ *           `stream` by this call, as the reference builds them in every run,
+ *           kernels/ising.py:200), or the info[2] record cm_ising_prepare
+ *           wrote for these betas, h and update codes into this workspace
+ *           (no host work per call; betas is then not read)
  * The plain form synchronises `stream` before returning. */
 int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *dims,
                  int32_t n_rep, const double *betas, double h, const uint32_t *keys_dev,
@@ -105,7 +110,20 @@ int cm_ising_run(void *spins_dev, int32_t layout, int32_t ndim, const int32_t *d
                  const int64_t *schedule, int32_t n_rows, int64_t n_meas, int32_t *e_run_dev,
                  int32_t *m_run_dev, int32_t *e_out_dev, int32_t *m_out_dev, int32_t *acc_out_dev,
                  void *workspace_dev, int64_t workspace_bytes, int32_t form, int32_t block,
-                 int32_t grid, const int64_t *n_meas_rep_dev, void *stream);
+                 int32_t grid, const int64_t *n_meas_rep_dev, const int32_t *prepared,
+                 void *stream);
+
+/* Build the class tables of n_rep chains (critmc/kernels/ising.py:47-65,
+ * metropolis_table / heatbath_table, which the reference rebuilds at the top
+ * of every run, :200) and upload them on `stream` into workspace_dev, once for
+ * any number of later cm_ising_run calls with the same betas, h and update
+ * codes.  Writes info[0] = the largest acceptance-class count a row of this
+ * schedule reads, info[1] = 1 if every row is Metropolis and every table is the
+ * zero-field z = 4 one; pass info as cm_ising_run's `prepared`.  The workspace
+ * must not be written by anything else in between. */
+int cm_ising_prepare(int32_t ndim, const int32_t *dims, int32_t n_rep, const double *betas,
+                     double h, const int64_t *schedule, int32_t n_rows, void *workspace_dev,
+                     int64_t workspace_bytes, int32_t *info, void *stream);
 
 /* Bytes of caller-owned device scratch cm_ising_cluster_run needs (tables plus
  * parent, size, queue and mark arrays of n_rep x N int32). */

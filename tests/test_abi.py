@@ -47,7 +47,7 @@ def test_argument_errors_are_reported_without_a_gpu():
                         betas.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0.0, P(0), P(0),
                         P(0), sw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                         sched.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 1, 10, P(0), P(0),
-                        P(0), P(0), P(0), P(0), 1 << 20, 0, 0, 0, P(0), P(0))
+                        P(0), P(0), P(0), P(0), 1 << 20, 0, 0, 0, P(0), None, P(0))
     assert rc == CM_EINVAL
     assert b"even" in L.cm_last_error()
 
@@ -65,6 +65,6 @@ def test_library_refuses_cluster_codes():
                         betas.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0.0, P(0), P(0),
                         P(0), sw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                         sched.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 1, 10, P(0), P(0),
-                        P(0), P(0), P(0), P(0), 1 << 20, 0, 0, 0, P(0), P(0))
+                        P(0), P(0), P(0), P(0), 1 << 20, 0, 0, 0, P(0), None, P(0))
     assert rc == CM_EINVAL
     assert b"checkerboard" in L.cm_last_error()
