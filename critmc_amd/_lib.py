@@ -86,6 +86,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_mbar_workspace_bytes": ([_c_i32, _c_i32], _c_i64),
     "cm_mbar_objective": ([_vp, _vp, _c_i64, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp, _c_i64, _vp],
                           _c_i32),
+    "cm_mbar_objective_noalloc": ([_vp, _vp, _c_i64, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp],
+                                  _c_i32),
     "cm_mbar_log_z": ([_vp, _vp, _vp, _c_i64, _vp, _c_i32, _vp, _vp, _c_i64, _vp], _c_i32),
     "cm_wl_run": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp, _c_i32, _vp, _vp, _vp, _vp,
                    _vp, _vp, _vp, _vp, _vp, _c_i64, _vp], _c_i32),

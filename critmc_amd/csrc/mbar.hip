@@ -15,6 +15,7 @@
 // a second launch, so every result is reproducible run to run.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -199,6 +200,55 @@ __global__ void mbar_logz_final(const double* __restrict__ part, int nblocks, in
 
 }  // namespace cm
 
+// Scratch of cm_mbar_objective_noalloc: the per-workgroup partials of the
+// largest objective (K = 64 runs over 512 workgroups, 1 + K + K^2 doubles
+// each), 17,043,456 bytes on every GPU the library runs on, held until the
+// process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[(int64_t)cm::MBAR_GRID *
+                                                (1 + cm::MBAR_KMAX + cm::MBAR_KMAX * cm::MBAR_KMAX)];
+
+namespace cm {
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kScratchDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_on_device(void** out)
+{
+    int dev = 0;
+    CM_HIP_CHECK(hipGetDevice(&dev));
+    CM_REQUIRE(dev >= 0 && dev < kScratchDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[dev] == nullptr) {
+        void* p = nullptr;
+        CM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        CM_REQUIRE(p != nullptr, "no device address for the MBAR scratch");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return CM_OK;
+}
+
+// Both objective entry points: partials into `part`, then the ordered sums.
+int mbar_objective_launch(const double* u_energy, const double* u_logmult, int64_t U,
+                          const double* betas, const double* log_n, const double* f, int K,
+                          double* out, double* ld_out, double* part, hipStream_t st)
+{
+    const int64_t ntile = (U + MBAR_TILE - 1) / MBAR_TILE;
+    const int grid = (int)std::min<int64_t>(ntile, MBAR_GRID);
+    hipLaunchKernelGGL(mbar_partial, dim3(grid), dim3(256), 0, st, u_energy, u_logmult, U, betas,
+                       log_n, f, K, ld_out, part);
+    hipLaunchKernelGGL(mbar_final, dim3(1), dim3(256), 0, st, part, grid, K, log_n, f, out);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+}  // namespace
+}  // namespace cm
+
 using namespace cm;
 
 extern "C" {
@@ -218,15 +268,25 @@ int cm_mbar_objective(const double* u_energy, const double* u_logmult, int64_t U
     CM_REQUIRE(K >= 1 && K <= MBAR_KMAX, "MBAR takes 1..64 runs, got %d", K);
     CM_REQUIRE(U >= 1, "need at least one sample");
     CM_REQUIRE(workspace_bytes >= cm_mbar_workspace_bytes(K, 1), "workspace too small");
-    hipStream_t st = (hipStream_t)stream;
-    const int64_t ntile = (U + MBAR_TILE - 1) / MBAR_TILE;
-    const int grid = (int)std::min<int64_t>(ntile, MBAR_GRID);
-    double* part = (double*)workspace;
-    hipLaunchKernelGGL(mbar_partial, dim3(grid), dim3(256), 0, st, u_energy, u_logmult, U, betas,
-                       log_n, f, K, ld_out, part);
-    hipLaunchKernelGGL(mbar_final, dim3(1), dim3(256), 0, st, part, grid, K, log_n, f, out);
-    CM_HIP_CHECK(hipGetLastError());
-    return CM_OK;
+    return mbar_objective_launch(u_energy, u_logmult, U, betas, log_n, f, K, out, ld_out,
+                                 (double*)workspace, (hipStream_t)stream);
+}
+
+int cm_mbar_objective_noalloc(const double* u_energy, const double* u_logmult, int64_t U,
+                              const double* betas, const double* log_n, const double* f,
+                              int32_t K, double* out, double* ld_out, void* stream)
+{
+    CM_REQUIRE(K >= 1 && K <= MBAR_KMAX,
+               "MBAR without allocation takes 1..%d runs (its scratch is sized for that), got %d",
+               MBAR_KMAX, K);
+    CM_REQUIRE(U >= 1, "need at least one sample");
+    CM_REQUIRE(cm_mbar_workspace_bytes(K, 1) <= (int64_t)sizeof(scratch),
+               "objective needs %lld bytes of scratch, the static array holds %lld",
+               (long long)cm_mbar_workspace_bytes(K, 1), (long long)sizeof(scratch));
+    void* part = nullptr;
+    if (int rc = scratch_on_device(&part)) return rc;
+    return mbar_objective_launch(u_energy, u_logmult, U, betas, log_n, f, K, out, ld_out,
+                                 (double*)part, (hipStream_t)stream);
 }
 
 int cm_mbar_log_z(const double* u_energy, const double* u_logmult, const double* ld, int64_t U,

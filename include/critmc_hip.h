@@ -361,6 +361,22 @@ int cm_mbar_objective(const double *u_energy_dev, const double *u_logmult_dev, i
                       int32_t K, double *out_dev, double *ld_out_dev, void *workspace_dev,
                       int64_t workspace_bytes, void *stream);
 
+/* cm_mbar_objective without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight Newton loops in a serving process).  Same
+ * kernels, launch shape and results as cm_mbar_objective; the per-workgroup
+ * partials live in a static device array of 17,043,456 bytes per GPU (enough
+ * for the largest objective, K = 64), held until the process ends.  K above
+ * 64 is rejected with CM_EINVAL.  The array's device address is looked up on
+ * the first call on each device (make that call before any graph capture).
+ * Nothing in it needs clearing: each call writes every partial it reads.
+ * All calls on one GPU share the array, so calls that overlap on that GPU
+ * (on different streams, or from different threads) race; serialise them. */
+int cm_mbar_objective_noalloc(const double *u_energy_dev, const double *u_logmult_dev,
+                              int64_t U, const double *betas_dev, const double *log_n_dev,
+                              const double *f_dev, int32_t K, double *out_dev,
+                              double *ld_out_dev, void *stream);
+
 /* ln sum_u exp(m_u - beta E_u - ld_u) for each of n_targets betas (MultiHistogram.log_z,
  * critmc/reweight.py:313-320, before the energy-shift correction). */
 int cm_mbar_log_z(const double *u_energy_dev, const double *u_logmult_dev, const double *ld_dev,
