@@ -128,6 +128,15 @@ class _Pool:
         self.betas = np.resize(getattr(self, "betas", np.zeros(0)), cap)
         self.steps = np.resize(getattr(self, "steps", np.zeros(0)), cap)
         self.chains = None
+        # scratch kept for the pool's life (the Ising tables are rebuilt only
+        # when a slot's beta changes)
+        shape = (self.L, self.L)
+        if self.model == "ising":
+            self.ws = k_ising.Workspace(cap, self.dev)
+        elif self.model == "potts":
+            self.ws = k_potts.Workspace(cap, self.dev)
+        else:
+            self.ws = k_on.Workspace(shape, cap, 2, self.dev)
 
     def add(self, job: Job):
         with torch.cuda.stream(self.stream):
@@ -203,15 +212,15 @@ class _Pool:
             shape = (self.L, self.L)
             if self.model == "ising":
                 out = k_ising.run(st["spins"], shape, self.betas, 0.0, ch, sched, n, st["e"], st["m"],
-                                  n_meas_rep=caps_d)
+                                  ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": out[0], "M": out[1]}
             elif self.model == "potts":
                 e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
-                                           st["e"], st["counts"], n_meas_rep=caps_d)
+                                           st["e"], st["counts"], ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": e, "counts": counts}
             else:
                 _, e, m, _, _ = k_on.run(st["spins"], shape, 2, self.betas, self.steps, ch, sched, n,
-                                         n_meas_rep=caps_d)
+                                         ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": e, "M": m}
             for s, j in enumerate(self.jobs):
                 if j is not None and caps[s]:

@@ -1,22 +1,35 @@
 """GPU O(n) kernels (FP32) against the FP64 replay.
 
 One sweep from an identical state agrees to 1e-5 rad per spin, with stated,
-conditioning-bound exceptions:
+counted exceptions (measured at 512^2 from a hot start: at most 2 of 262,144
+spins for Metropolis and heat bath, 62 for over-relaxation; from a state
+thermalised at T = 0.89, at most 2):
 * Metropolis: a decision flips where the acceptance uniform lies within the
-  FP32 error of exp(-beta dE); a proposal S + step g of tiny length (|.| <
-  ~0.03, ~1e-4 of the proposals) turns the ~1e-7 rounding of the FP32
-  Gaussian into an angle above 1e-5.  Their number is bounded.
+  FP32 error of exp(-beta dE); a proposal S + step g of tiny length turns the
+  ~1e-7 rounding of the FP32 Gaussian into an angle above 1e-5.  At most
+  2e-4 N + 5 spins.
 * Heat bath (FP64 sampling from the FP64 field, stored FP32): the same
   conditioning as over-relaxation, angle_error * |h| < 1e-5, except where the
-  ~1e-7 field difference flips a rejection decision; those are bounded like
-  Metropolis flips.
-* Over-relaxation: the second colour reflects about fields built from
-  first-colour spins stored in FP32 (6e-8 rounding); the reflection axis is
-  then uncertain by ~2 delta_h / |h| with delta_h ~ 1e-7.  Every spin beyond
-  1e-5 rad must satisfy angle_error * |h| < 1e-5, and none may exceed 1e-3.  Measurements of the state the GPU wrote agree with FP64 sums of
-the same values to 1e-6 relative (the GPU rounds the renormalised spins to FP32
-before summing).  Statistics: rings of
-8 spins against the transfer operator through the reference's Simulation API.
+  ~1e-7 field difference flips a rejection decision.  At most 2e-4 N + 5.
+* Over-relaxation: the reflection axis is uncertain by ~2 delta_h / |h| with
+  delta_h ~ 1e-7 (FP32-stored neighbours), so every spin beyond 1e-5 rad
+  satisfies angle_error * |h| < 1e-5, and their number is at most the count of
+  sites whose field is below 0.05, plus 5.  None exceeds 1e-3.
+Measurements of the state the GPU wrote agree with FP64 sums of the same
+values to 1e-6 relative (the GPU rounds the renormalised spins to FP32 before
+summing).
+
+Several sweeps in a row: over-relaxation is deterministic and amplifies a
+perturbation of a spin's field by about 2/|h| per half sweep, so FP32 rounding
+grows through a measurement of metropolis + 2 x overrelax.  The fused 512^2
+launch is compared with the FP64 replay directly: from a thermalised state
+(the XY workload's regime, T = 0.89) at most 1e-3 N spins are beyond 1e-5 rad
+(measured 66-69 of 262,144) and none beyond 1e-3; from a hot start the count is
+bounded by five times that of two FP64 replays whose starts differ by 6e-8 rad
+rotations (the dynamics' own sensitivity; measured 8,330 against 2,819).  Its
+FP64 energy and helicity sums agree with the replay's to 1e-6 per site.
+Statistics: rings of 8 spins against the transfer operator through the
+reference's Simulation API.
 """
 
 import numpy as np
@@ -60,7 +73,8 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
         ref = rep.run([[code, 1]], 1)
         err = _angle_err(got[r], rep.s)
         bad = err > 1e-5
-        assert err.max() < 1e-3 or code != 5, err.max()
+        print(f"n={n} shape={shape} code={code} r={r}: {int(bad.sum())} of {N} spins "
+              f"({bad.mean():.2e}) beyond 1e-5 rad, max {err.max():.2e}")
         if code in (2, 5):
             from critmc_amd import hypercubic
 
@@ -68,13 +82,15 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
             hmag = np.linalg.norm(rep.s[lat.nbr].sum(axis=1), axis=1)
             cond = err * hmag < 1e-5
             if code == 5:
+                assert err.max() < 1e-3, err.max()
                 assert np.all(cond[bad]), (bad.sum(), err.max(), (err * hmag)[bad].max())
+                assert bad.sum() <= int((hmag < 0.05).sum()) + 5, (bad.sum(), (hmag < 0.05).sum())
             else:  # rejection decisions flipped by the FP32-stored neighbours
-                assert (bad & ~cond).sum() <= int(5e-4 * N) + 5, ((bad & ~cond).sum(), err.max())
+                assert bad.sum() <= int(2e-4 * N) + 5, (bad.sum(), err.max())
             limit = int(bad.sum())
         else:
             # decision flips plus ill-conditioned proposals
-            limit = int(5e-4 * N) + 5
+            limit = int(2e-4 * N) + 5
             assert bad.sum() <= limit, (bad.sum(), err.max())
         if code == 0:
             assert abs(int(acc[r, 0]) - int(ref["accepted"][0])) <= limit
@@ -92,6 +108,59 @@ def test_one_sweep_within_1e5_rad(n, shape, code):
         np.testing.assert_allclose(sc[r, 0].cpu().numpy(), scv, rtol=1e-6, atol=1e-5)
         np.testing.assert_allclose(ss[r, 0].cpu().numpy(), ssv, rtol=1e-6, atol=1e-5)
         assert np.allclose(np.linalg.norm(fin, axis=1), 1.0)
+
+
+def _rotate(s, eps, rng):
+    """Each planar spin turned by an angle uniform in [-eps, eps]."""
+    a = rng.uniform(-eps, eps, size=len(s))
+    c, sn = np.cos(a), np.sin(a)
+    return np.stack([c * s[:, 0] - sn * s[:, 1], sn * s[:, 0] + c * s[:, 1]], axis=1)
+
+
+@pytest.mark.parametrize("start", ["thermalised", "hot"])
+def test_fused_measurement_512_against_fp64_replay(start):
+    """One measurement of metropolis + 2 x overrelax on 512^2 XY, T = 0.89: the
+    fused over-relaxation launch (two sweeps and the measurement in LDS tiles)
+    against the FP64 replay from the identical state."""
+    dev = torch.device("cuda", 0)
+    shape, n, R = (512, 512), 2, 2
+    N = shape[0] * shape[1]
+    rng = np.random.default_rng(41)
+    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+    betas, steps = np.array([1 / 0.89, 1 / 0.91]), np.array([1.0, 1.6])
+    sched = np.array([[0, 1], [5, 2]])
+    chains = Chains(keys, [3, 4], None, dev)
+    s = torch.empty((R, N, n), dtype=torch.float32, device=dev)
+    K.init_spins(s, shape, n, chains)
+    if start == "thermalised":
+        K.run(s, shape, n, betas, steps, chains, sched, 100)
+    torch.cuda.synchronize()
+    sweep0 = chains.sweeps_h.copy()
+    s0 = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
+    acc, e, m, sc, ss = K.run(s, shape, n, betas, steps, chains, sched, 1)
+    got = K.to_sites(s, shape, n).cpu().numpy().astype(np.float64)
+    for r in range(R):
+        b, st = float(np.float32(betas[r])), float(np.float32(steps[r]))
+        rep = oracle.OnReplay(s0[r], shape, b, st, keys[r], [3, 4][r], sweep=int(sweep0[r]))
+        ref = rep.run(sched, 1)
+        err = _angle_err(got[r], rep.s)
+        bad = int((err > 1e-5).sum())
+        # the dynamics' own sensitivity: the same replay from a start turned by <= 6e-8 rad
+        twin = oracle.OnReplay(_rotate(s0[r], 6e-8, rng), shape, b, st, keys[r], [3, 4][r],
+                               sweep=int(sweep0[r]))
+        twin.run(sched, 1)
+        own = int((_angle_err(twin.s, rep.s) > 1e-5).sum())
+        print(f"{start} r={r}: {bad} of {N} spins ({bad / N:.2e}) beyond 1e-5 rad, max "
+              f"{err.max():.2e}; FP64 replay from a 6e-8 rad rotated start: {own}")
+        assert bad <= 5 * own + 50, (bad, own)
+        if start == "thermalised":
+            assert bad <= N // 1000 and err.max() < 1e-3, (bad, err.max())
+        else:
+            assert err.max() < 0.1, err.max()
+        assert abs(int(acc[r, 0]) - int(ref["accepted"][0])) <= 2
+        assert abs(float(e[r, 0]) - ref["E"][0]) <= 1e-6 * N
+        np.testing.assert_allclose(sc[r, 0].cpu().numpy(), ref["stiff_c"][0], rtol=0, atol=1e-6 * N)
+        np.testing.assert_allclose(ss[r, 0].cpu().numpy(), ref["stiff_s"][0], rtol=0, atol=1e-6 * N)
 
 
 @pytest.mark.parametrize("n_comp", [2, 3])

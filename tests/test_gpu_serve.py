@@ -42,6 +42,44 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     assert srv.ticks > 1
 
 
+# Large lattices: the pools that carry most of the serving work.  Ising and
+# Potts pools of a few 256^2 / 512^2 replicas take the wide forms (one launch
+# per half sweep over all replicas), 128^2 the per-replica forms; XY takes the
+# half-sweep launches.  Sweep counts differ from the background jobs', so the
+# per-slot caps stop each job at its own count inside shared slices.
+LARGE_PROBES = [Job(m, L, b, sw, seed=100 + k, arrival=0.004 * (1 + k % 3))
+                for k, (m, L, b, sw) in enumerate([
+                    ("ising", 128, 0.44, 333), ("ising", 256, 0.43, 517), ("ising", 512, 0.45, 261),
+                    ("potts", 128, 1.40, 311), ("potts", 256, 1.45, 205), ("potts", 512, 1.43, 133),
+                    ("xy", 128, 1.10, 219), ("xy", 256, 1.12, 151), ("xy", 512, 1.09, 97)])]
+
+
+def test_large_lattices_are_the_same_alone_and_joining_running_pools():
+    alone = []
+    for j in LARGE_PROBES:
+        srv = Server(budget_bytes=4e9, slice_sweeps=64)
+        job = srv.submit(_clone(j))
+        srv.run(realtime=False)
+        alone.append(job.results())
+    srv = Server(budget_bytes=8e9, slice_sweeps=64)
+    background = []
+    for k, j in enumerate(LARGE_PROBES):  # two running jobs per pool before any probe arrives
+        for b in range(2):
+            background.append(srv.submit(Job(j.model, j.L, j.beta * (1.01 + 0.01 * b),
+                                             20 * j.sweeps + 41 * b, seed=500 + 2 * k + b)))
+    probes = [srv.submit(_clone(j, arrival=j.arrival)) for j in LARGE_PROBES]
+    done = srv.run(realtime=True)
+    assert len(done) == len(background) + len(probes)
+    for res, job, bg in zip(alone, probes, zip(background[::2], background[1::2])):
+        # the pool was running when the probe joined it, and kept running after
+        assert min(b.admitted_at for b in bg) < job.admitted_at < max(b.finished_at for b in bg), \
+            (job.model, job.L)
+        got = job.results()
+        assert got.keys() == res.keys()
+        for k in res:
+            assert np.array_equal(got[k], res[k]), (job.model, job.L, k)
+
+
 def test_admission_respects_the_budget():
     jobs = [Job("ising", 64, 0.44, 3000, seed=s) for s in range(6)]
     budget = 2.5 * jobs[0].footprint()
