@@ -36,10 +36,20 @@ __all__ = ["EnergyLevels", "WangLandau", "WangLandauBatch", "WLResult", "Multica
            "MulticanonicalBatch", "MucaResult"]
 
 
+def _normalised(log_w: np.ndarray) -> np.ndarray:
+    """Weights proportional to exp(log_w), summing to one."""
+    w = np.exp(log_w - np.max(log_w))
+    return w / w.sum()
+
+
 @dataclass(frozen=True)
 class EnergyLevels:
-    """Level index <-> (bond) energy of a discrete model (critmc/flathist.py:36-93):
-    Ising ``idx = (E_b + B) / 4``, Potts ``idx = E + B``."""
+    """Level index <-> (bond) energy of a discrete zero-field model (critmc/flathist.py:36-93).
+
+    Ising bond energies run from -B to B in steps of 4 (even coordination),
+    Potts energies (minus the satisfied bonds) from -B to 0 in steps of 1, so
+    level k has energy ``k * step - B`` with B the number of bonds.
+    """
 
     kind: str
     n_sites: int
@@ -62,57 +72,69 @@ class EnergyLevels:
         raise TypeError("flat-histogram methods support Ising and Potts models")
 
     @property
+    def _step(self) -> int:
+        return 4 if self.kind == "ising" else 1
+
+    @property
     def size(self) -> int:
-        return self.n_bonds // 2 + 1 if self.kind == "ising" else self.n_bonds + 1
+        span = 2 * self.n_bonds if self.kind == "ising" else self.n_bonds
+        return span // self._step + 1
 
     @property
     def energies(self) -> np.ndarray:
-        k = np.arange(self.size)
-        return 4.0 * k - self.n_bonds if self.kind == "ising" else k - float(self.n_bonds)
+        return self._step * np.arange(self.size) - float(self.n_bonds)
 
-    def _level(self, energy: float) -> float:
-        return (energy + self.n_bonds) / 4 if self.kind == "ising" else energy + self.n_bonds
+    def _position(self, energy: float) -> float:
+        """Fractional level of an energy."""
+        return (energy + self.n_bonds) / self._step
 
     def index(self, energy: float) -> int:
-        return int(round(self._level(float(energy))))
+        return int(round(self._position(float(energy))))
 
     def window(self, e_min: float | None, e_max: float | None) -> tuple[int, int]:
         """Levels covering per-site energies ``[e_min, e_max]`` (``None`` = open end)."""
         n = self.n_sites
-        lo = 0 if e_min is None else max(0, int(np.ceil(self._level(e_min * n))))
-        hi = self.size - 1 if e_max is None else min(self.size - 1, int(np.floor(self._level(e_max * n))))
+        lo = 0 if e_min is None else int(np.ceil(self._position(e_min * n)))
+        hi = self.size - 1 if e_max is None else int(np.floor(self._position(e_max * n)))
+        lo, hi = max(lo, 0), min(hi, self.size - 1)
         if lo >= hi:
             raise ValueError("empty energy window")
         return lo, hi
 
     @property
     def log_total(self) -> float:
-        """ln of the number of configurations, the normalisation of g."""
+        """ln of the number of configurations, q^N: the normalisation of g."""
         return self.n_sites * np.log(self.q)
 
 
 def _walk_into_window(model: Model, state: np.ndarray, levels: EnergyLevels, lo: int, hi: int,
                       rng: np.random.Generator, max_steps: int = 10 ** 7) -> int:
-    """Greedy single-site walk until the level is inside ``[lo, hi]`` (critmc/flathist.py:100-129);
-    host-side and drawn from the walker's numpy generator, before its Philox key."""
+    """Bring ``state`` (in place) into levels ``[lo, hi]`` and return its level.
+
+    Greedy single-site moves (critmc/flathist.py:100-129): a random site and,
+    for Potts, a random other state are drawn from the walker's numpy
+    generator, and the move is made unless it leads away from the window.
+    Runs on the host, before the walker's Philox key is drawn.
+    """
     nbr = model.lattice.nbr
+    ising = levels.kind == "ising"
     idx = levels.index(model.energy(state))
-    n = model.n_sites
     for _ in range(max_steps):
         if lo <= idx <= hi:
             return idx
-        i = int(rng.integers(n))
-        if levels.kind == "ising":
-            new = idx + int(state[i] * state[nbr[i]].sum()) // 2
-            new_val = -state[i]
+        site = int(rng.integers(model.n_sites))
+        old, neigh = state[site], state[nbr[site]]
+        if ising:
+            new = -old
+            change = int(old * neigh.sum()) // 2  # bond energy +2 s_i h_i, in levels of 4
         else:
-            new_val = int(rng.integers(levels.q - 1))
-            new_val += new_val >= state[i]
-            neigh = state[nbr[i]]
-            new = idx + int(np.sum(neigh == state[i])) - int(np.sum(neigh == new_val))
-        if (idx < lo and new >= idx) or (idx > hi and new <= idx):
-            state[i] = new_val
-            idx = new
+            new = int(rng.integers(levels.q - 1))
+            new += new >= old
+            change = int(np.count_nonzero(neigh == old)) - int(np.count_nonzero(neigh == new))
+        towards = 1 if idx < lo else -1
+        if towards * change >= 0:
+            state[site] = new
+            idx += change
     raise RuntimeError("could not reach the energy window")
 
 
@@ -135,27 +157,28 @@ class WLResult:
         return self.levels.energies
 
     def normalized(self, ln_g: np.ndarray | None = None) -> np.ndarray:
-        """ln g normalised to the number of states (full window) or to 0 at the lowest level."""
-        ln_g = self.ln_g if ln_g is None else ln_g
-        out = np.full_like(ln_g, -np.inf)
-        v = self.visited
-        full = self.window == (0, self.levels.size - 1)
-        shift = self.levels.log_total - logsumexp(ln_g[v]) if full else -ln_g[v][0]
-        out[v] = ln_g[v] + shift
-        return out
+        """ln g on the visited levels (-inf elsewhere), fixed so that g sums to the
+        number of configurations when the walk covered every level, otherwise so
+        that the lowest visited level has ln g = 0."""
+        values = np.asarray(self.ln_g if ln_g is None else ln_g)
+        seen = self.visited
+        if self.window == (0, self.levels.size - 1):
+            offset = self.levels.log_total - logsumexp(values[seen])
+        else:
+            offset = -values[seen][0]
+        return np.where(seen, values + offset, -np.inf)
 
     def canonical(self, beta) -> dict[str, np.ndarray]:
-        """ln Z, energy and specific heat per site from ln g (restricted to the window)."""
+        """ln Z, energy and specific heat per site from ln g over the visited levels."""
         beta = np.atleast_1d(np.asarray(beta, dtype=np.float64))
-        v = self.visited
-        e = self.energies[v]
-        a = self.normalized()[v][None, :] - beta[:, None] * e[None, :]
-        lz = logsumexp(a, axis=1)
-        p = np.exp(a - lz[:, None])
-        mean = p @ e
-        var = p @ (e * e) - mean ** 2
+        seen = self.visited
+        e = self.energies[seen]
+        log_p = self.normalized()[seen] - np.outer(beta, e)
+        ln_z = logsumexp(log_p, axis=1)
+        p = np.exp(log_p - ln_z[:, None])
+        first, second = p @ e, p @ (e * e)
         n = self.levels.n_sites
-        return {"lnZ": lz, "u": mean / n, "c": beta ** 2 * var / n}
+        return {"lnZ": ln_z, "u": first / n, "c": beta * beta * (second - first * first) / n}
 
 
 class _Walkers:
@@ -314,9 +337,25 @@ class WangLandau:
         return self._batch.run(ln_f_final, **kw)[0]
 
 
+def _smoothed_visits(hist: np.ndarray, seen: np.ndarray, sigma: float) -> np.ndarray:
+    """Visit counts averaged over visited neighbouring levels with a Gaussian of
+    ``sigma`` levels (cut at 3 sigma); unvisited levels stay 0."""
+    reach = int(np.ceil(3 * sigma))
+    if reach < 1:
+        return hist
+    kernel = np.exp(-0.5 * (np.arange(-reach, reach + 1) / sigma) ** 2)
+    mass = np.convolve(seen.astype(np.float64), kernel, mode="same")
+    total = np.convolve(np.where(seen, hist, 0.0), kernel, mode="same")
+    return np.where(seen, total / np.maximum(mass, 1e-300), 0.0)
+
+
 @dataclass
 class MucaResult:
-    """Multicanonical series plus the weights that produced them (critmc/flathist.py:321-452)."""
+    """Multicanonical series and the ln g that weighted them (critmc/flathist.py:321-452).
+
+    A sample at level E was drawn with weight 1/g(E), so canonical averages at
+    beta weight it by exp(-beta E) g(E).
+    """
 
     levels: EnergyLevels
     ln_g: np.ndarray
@@ -330,107 +369,102 @@ class MucaResult:
         return self.levels.energies[self.level_series]
 
     def order_parameter_m2(self) -> np.ndarray:
-        n = self.levels.n_sites
+        """Squared order parameter per site of every sample: (M/N)^2 for Ising,
+        (q sum_k rho_k^2 - 1) / (q - 1), clipped at 0, for Potts."""
+        n, q = self.levels.n_sites, self.levels.q
         if self.levels.kind == "ising":
             return (self.order_series / n) ** 2
-        q = self.levels.q
-        return np.clip((q * self.order_series / n ** 2 - 1.0) / (q - 1.0), 0.0, None)
+        return np.maximum((q * self.order_series / (n * n) - 1.0) / (q - 1.0), 0.0)
 
     def log_weights(self, beta: float) -> np.ndarray:
         """ln[exp(-beta E) g(E)] of each sample (unnormalised)."""
-        return -beta * self.energy + self.ln_g[self.level_series]
+        return self.ln_g[self.level_series] - beta * self.energy
 
     def average(self, beta: float, obs: np.ndarray) -> float:
-        lw = self.log_weights(beta)
-        w = np.exp(lw - lw.max())
-        return float(np.sum(w * obs) / np.sum(w))
+        return float(np.dot(_normalised(self.log_weights(beta)), obs))
 
     def canonical(self, beta) -> dict[str, np.ndarray]:
+        """u, c per site and <m^2>, <m^4> at each beta."""
         beta = np.atleast_1d(np.asarray(beta, dtype=np.float64))
+        w = np.stack([_normalised(self.log_weights(b)) for b in beta])
+        e, m2 = self.energy, self.order_parameter_m2()
         n = self.levels.n_sites
-        e = self.energy
-        m2 = self.order_parameter_m2()
-        out = {k: np.empty_like(beta) for k in ("u", "c", "m2", "m4")}
-        for i, b in enumerate(beta):
-            lw = self.log_weights(b)
-            w = np.exp(lw - lw.max())
-            w /= w.sum()
-            mean = w @ e
-            out["u"][i] = mean / n
-            out["c"][i] = b * b * (w @ (e * e) - mean ** 2) / n
-            out["m2"][i] = w @ m2
-            out["m4"][i] = w @ (m2 * m2)
-        return out
+        first, second = w @ e, w @ (e * e)
+        return {"u": first / n, "c": beta * beta * (second - first * first) / n,
+                "m2": w @ m2, "m4": w @ (m2 * m2)}
 
     def energy_distribution(self, beta: float, smooth: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
-        """Canonical P_beta(e) over levels from the histogram of visits, optionally
-        smoothed by a Gaussian of width ``smooth`` (energy per site)."""
+        """Per-site level energies and the canonical P_beta over levels, from the visit
+        histogram; ``smooth`` > 0 first averages the visits over a Gaussian of that
+        width in energy per site, which steadies peak positions and valley depths."""
         hist = self.hist.astype(np.float64)
-        v = hist > 0
+        seen = hist > 0
         if smooth > 0:
-            sigma = smooth * self.levels.n_sites / (4.0 if self.levels.kind == "ising" else 1.0)
-            half = int(np.ceil(3 * sigma))
-            if half >= 1:
-                kern = np.exp(-0.5 * (np.arange(-half, half + 1) / sigma) ** 2)
-                num = np.convolve(np.where(v, hist, 0.0), kern, mode="same")
-                den = np.convolve(v.astype(float), kern, mode="same")
-                hist = np.where(v, num / np.maximum(den, 1e-300), 0.0)
-        a = np.full(hist.shape, -np.inf)
-        a[v] = np.log(hist[v]) + self.ln_g[v] - beta * self.levels.energies[v]
-        p = np.exp(a - logsumexp(a[v]))
+            per_level = self.levels.n_sites / (4.0 if self.levels.kind == "ising" else 1.0)
+            hist = _smoothed_visits(hist, seen, smooth * per_level)
+        log_p = np.full(hist.shape, -np.inf)
+        log_p[seen] = np.log(hist[seen]) + self.ln_g[seen] - beta * self.levels.energies[seen]
+        p = np.exp(log_p - logsumexp(log_p[seen]))
         return self.levels.energies / self.levels.n_sites, p
 
     def double_peak(self, beta: float, split: float | None = None,
                     smooth: float = 0.0) -> dict[str, float]:
-        """Peak positions / heights of a bimodal P_beta(e) separated at ``split``."""
+        """The largest P_beta(e) below and at or above ``split`` (default: the mean
+        energy), their positions, the lowest P between them and the weight below."""
         e, p = self.energy_distribution(beta, smooth)
-        keep = p > 0
-        e, p = e[keep], p[keep]
-        if split is None:
-            split = float(np.sum(e * p))
-        low, high = e < split, e >= split
-        i_lo = np.argmax(np.where(low, p, -1))
-        i_hi = np.argmax(np.where(high, p, -1))
-        between = (e > e[i_lo]) & (e < e[i_hi])
-        return {"e_ordered": float(e[i_lo]), "e_disordered": float(e[i_hi]),
-                "p_ordered": float(p[i_lo]), "p_disordered": float(p[i_hi]),
-                "p_min": float(p[between].min()) if between.any() else float("nan"),
-                "weight_ordered": float(p[low].sum())}
+        e, p = e[p > 0], p[p > 0]
+        cut = float(np.dot(e, p)) if split is None else split
+        below = e < cut
+        lo = int(np.argmax(np.where(below, p, -1.0)))
+        hi = int(np.argmax(np.where(below, -1.0, p)))
+        valley = p[(e > e[lo]) & (e < e[hi])]
+        return {"e_ordered": float(e[lo]), "e_disordered": float(e[hi]),
+                "p_ordered": float(p[lo]), "p_disordered": float(p[hi]),
+                "p_min": float(valley.min()) if valley.size else float("nan"),
+                "weight_ordered": float(p[below].sum())}
 
     def phase_means(self, beta: float, split: float) -> dict[str, float]:
-        """Mean energy per site of the samples below / above ``split`` at ``beta``."""
+        """Mean energy per site of the samples below and at or above ``split`` at beta,
+        and the weight of those below."""
         e = self.energy / self.levels.n_sites
-        lw = self.log_weights(beta)
-        w = np.exp(lw - lw.max())
-        low = e < split
-        w_lo, w_hi = w[low].sum(), w[~low].sum()
-        return {"e_ordered": float(w[low] @ e[low] / w_lo),
-                "e_disordered": float(w[~low] @ e[~low] / w_hi),
-                "weight_ordered": float(w_lo / (w_lo + w_hi))}
+        w = _normalised(self.log_weights(beta))
+        below = e < split
+        w_below, w_above = w[below].sum(), w[~below].sum()
+        return {"e_ordered": float(np.dot(w[below], e[below]) / w_below),
+                "e_disordered": float(np.dot(w[~below], e[~below]) / w_above),
+                "weight_ordered": float(w_below / (w_below + w_above))}
 
     def equal_height_beta(self, bracket, split: float | None = None, smooth: float = 0.0) -> float:
-        def gap(b):
-            dp = self.double_peak(b, split, smooth)
-            return np.log(dp["p_ordered"]) - np.log(dp["p_disordered"])
+        """Beta in ``bracket`` where both peaks of P_beta(e) are equally high."""
+        def log_ratio(b):
+            peaks = self.double_peak(b, split, smooth)
+            return np.log(peaks["p_ordered"] / peaks["p_disordered"])
 
-        return float(brentq(gap, *bracket, xtol=1e-10))
+        return float(brentq(log_ratio, *bracket, xtol=1e-10))
 
     def equal_weight_beta(self, bracket, split: float) -> float:
+        """Beta in ``bracket`` where both phases carry probability 1/2."""
         return float(brentq(lambda b: self.phase_means(b, split)["weight_ordered"] - 0.5,
                             *bracket, xtol=1e-10))
 
 
 def _muca_weights(levels: EnergyLevels, ln_g, window) -> tuple[np.ndarray, np.ndarray]:
-    """ln g with never-visited levels forbidden, and the weights -ln g (critmc/flathist.py:477-490)."""
+    """The ln g a walker uses and its weights -ln g (critmc/flathist.py:477-490).
+
+    From a :class:`WLResult`, levels its walk never visited get ln g = -inf.
+    A non-finite ln g inside the window becomes +inf (weight -inf: forbidden);
+    outside the window, -inf entries keep weight -inf as well.
+    """
     if isinstance(ln_g, WLResult):
         ln_g = np.where(ln_g.visited, ln_g.ln_g, -np.inf)
-    ln_g = np.asarray(ln_g, dtype=np.float64).copy()
-    if len(ln_g) != levels.size:
+    table = np.array(ln_g, dtype=np.float64)
+    if table.shape != (levels.size,):
         raise ValueError("ln_g does not match the model's energy levels")
     lo, hi = window
-    inside = ln_g[lo:hi + 1]
-    inside[~np.isfinite(inside)] = np.inf
-    return ln_g, np.where(np.isfinite(ln_g), -ln_g, -np.inf)
+    inside = np.zeros(levels.size, dtype=bool)
+    inside[lo:hi + 1] = True
+    table[inside & ~np.isfinite(table)] = np.inf
+    return table, np.where(np.isfinite(table), -table, -np.inf)
 
 
 class MulticanonicalBatch:

@@ -201,70 +201,93 @@ class ParallelTempering:
 def feedback_temperatures(temperatures: np.ndarray, label_hist: np.ndarray) -> np.ndarray:
     """One feedback-optimisation step of the ladder (critmc/glass.py:211-242).
 
-    Katzgraber, Trebst, Huse & Troyer, J. Stat. Mech. P03018 (2006): with f(T)
-    the fraction of configurations at T that last visited the hottest end, new
-    temperatures sit at equal quantiles of sum sqrt(df); end points are kept.
+    Katzgraber, Trebst, Huse & Troyer, J. Stat. Mech. P03018 (2006).  ``f(T)``
+    is the fraction of configurations at T whose last visited end was the
+    hottest one (``label_hist[k] = (visits from the hot end, from the cold
+    end)``); it is taken as 1/2 where T saw neither, and made non-decreasing
+    in T.  The optimal temperature density is sqrt(df/dT / dT), whose integral
+    over an interval of the current ladder is sqrt(df): the new temperatures
+    sit at equal steps of that cumulative weight.  The end points stay, and
+    the result is in ascending order.
     """
     temps = np.asarray(temperatures, dtype=np.float64)
     order = np.argsort(temps)
-    t = temps[order]
-    up = label_hist[order, 0].astype(float)
-    down = label_hist[order, 1].astype(float)
-    with np.errstate(invalid="ignore", divide="ignore"):
-        f = up / (up + down)
-    f = np.maximum.accumulate(np.nan_to_num(f, nan=0.5))
-    df = np.maximum(np.diff(f), 1e-6)
-    cum = np.concatenate([[0.0], np.cumsum(np.sqrt(df))])
-    cum /= cum[-1]
-    new = np.interp(np.linspace(0.0, 1.0, len(t)), cum, t)
-    new[0], new[-1] = t[0], t[-1]
+    ladder = temps[order]
+    hot, cold = (np.asarray(label_hist, dtype=np.float64)[order, c] for c in (0, 1))
+    visited = hot + cold
+    frac = np.divide(hot, visited, out=np.full_like(hot, 0.5), where=visited > 0)
+    frac = np.maximum.accumulate(frac)
+    cumulative = np.append(0.0, np.cumsum(np.sqrt(np.clip(np.diff(frac), 1e-6, None))))
+    profile = cumulative / cumulative[-1]
+    new = np.interp(np.linspace(0.0, 1.0, ladder.size), profile, ladder)
+    new[[0, -1]] = ladder[[0, -1]]
     return new
 
 
+def _estimates(mean: dict[str, np.ndarray], n_sites: int, L: int) -> dict[str, np.ndarray]:
+    """Binder ratio, susceptibility and xi_L / L from disorder means (any leading axes):
+    g = (3 - [q^4] / [q^2]^2) / 2, chi_SG = N [q^2],
+    xi_L = sqrt([q^2] / [|q(k_min)|^2] - 1) / (2 sin(k_min / 2)), k_min = 2 pi / L."""
+    q2, q4, qk2 = mean["q2"], mean["q4"], mean["qk2"]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ratio = np.clip(q2 / qk2 - 1.0, 0.0, None)
+        xi = np.sqrt(ratio) / (2.0 * np.sin(np.pi / L))
+        g = 0.5 * (3.0 - q4 / (q2 * q2))
+    return {"e": mean["e"], "ql": mean["ql"], "g": g, "chi_sg": n_sites * q2, "xi_L": xi / L}
+
+
 class DisorderAverage:
-    """Disorder averages over samples at common temperatures (critmc/glass.py:245-313)."""
+    """Disorder averages over samples run at common temperatures (critmc/glass.py:245-313).
+
+    ``samples[name]`` holds each sample's last-bin (equilibrium) thermal
+    average, shape ``(samples, temperatures)``.
+    """
 
     def __init__(self, runs: list[TemperingRun], L: int):
         if not runs:
             raise ValueError("no runs")
         self.runs = runs
         self.L = int(L)
-        self.temperatures = runs[0].temperatures
-        self.n_sites = runs[0].n_sites
-        self.z = runs[0].z
-        eq = [r.equilibrium() for r in runs]
-        self.samples = {k: np.array([e[k] for e in eq]) for k in eq[0]}
+        first = runs[0]
+        self.temperatures = first.temperatures
+        self.n_sites = first.n_sites
+        self.z = first.z
+        per_run = [r.equilibrium() for r in runs]
+        self.samples = {name: np.stack([eq[name] for eq in per_run]) for name in per_run[0]}
 
     @staticmethod
     def _observables(s, n_sites: int, L: int) -> dict[str, np.ndarray]:
-        q2, q4, qk2 = s["q2"].mean(axis=0), s["q4"].mean(axis=0), s["qk2"].mean(axis=0)
-        kmin = 2.0 * np.pi / L
-        with np.errstate(invalid="ignore"):
-            xi = np.sqrt(np.clip(q2 / qk2 - 1.0, 0.0, None)) / (2.0 * np.sin(kmin / 2.0))
-        return {"e": s["e"].mean(axis=0), "ql": s["ql"].mean(axis=0),
-                "g": 0.5 * (3.0 - q4 / q2 ** 2), "chi_sg": n_sites * q2, "xi_L": xi / L}
+        """Estimates from per-sample values ``s[name]`` (samples first)."""
+        return _estimates({name: v.mean(axis=0) for name, v in s.items()}, n_sites, L)
 
     def observables(self, n_boot: int = 500, seed: int = 0):
-        rng = make_rng(seed)
+        """``{name: (estimate, bootstrap error)}``; the bootstrap resamples disorder
+        realisations, all ``n_boot`` resamples drawn and estimated at once."""
         full = self._observables(self.samples, self.n_sites, self.L)
         n = len(self.runs)
-        boots = {k: [] for k in full}
-        for _ in range(n_boot):
-            idx = rng.integers(0, n, size=n)
-            b = self._observables({k: v[idx] for k, v in self.samples.items()}, self.n_sites, self.L)
-            for k in full:
-                boots[k].append(b[k])
-        return {k: (full[k], np.std(np.array(boots[k]), axis=0, ddof=1)) for k in full}
+        picks = make_rng(seed).integers(0, n, size=(n_boot, n))
+        resampled = _estimates({name: v[picks].mean(axis=1) for name, v in self.samples.items()},
+                               self.n_sites, self.L)
+        return {name: (full[name], resampled[name].std(axis=0, ddof=1)) for name in full}
+
+    def _bins(self, name: str) -> np.ndarray:
+        """Per-bin thermal means of every sample, shape ``(samples, bins, temperatures)``."""
+        return np.stack([r.bin_means()[name] for r in self.runs])
 
     def identity_residuals(self) -> np.ndarray:
-        e = np.array([r.bin_means()["e"] for r in self.runs])
-        ql = np.array([r.bin_means()["ql"] for r in self.runs])
-        res = e + (self.z / 2.0) * (1.0 - ql) / self.temperatures[None, None, :]
-        return np.transpose(res, (1, 0, 2))
+        """``e + (z/2)(1 - q_l)/T`` per bin and sample, shape ``(bins, samples, temperatures)``.
+
+        The Gaussian-disorder identity (Katzgraber, Palassini & Young, PRB 63,
+        184422 (2001)) makes its disorder average vanish in equilibrium.
+        """
+        residual = self._bins("e") + 0.5 * self.z * (1.0 - self._bins("ql")) / self.temperatures
+        return residual.swapaxes(0, 1)
 
     def equilibration(self) -> dict[str, np.ndarray]:
-        e = np.array([r.bin_means()["e"] for r in self.runs]).mean(axis=0)
-        ql = np.array([r.bin_means()["ql"] for r in self.runs]).mean(axis=0)
-        res = self.identity_residuals()
-        return {"e": e, "e_from_ql": -(self.z / 2.0) * (1.0 - ql) / self.temperatures[None, :],
-                "residual_error": res.std(axis=1, ddof=1) / np.sqrt(res.shape[1])}
+        """Per-bin disorder means of e and of -(z/2)(1 - q_l)/T, and the standard
+        error of their difference over samples."""
+        residual = self.identity_residuals()
+        n = residual.shape[1]
+        return {"e": self._bins("e").mean(axis=0),
+                "e_from_ql": -0.5 * self.z * (1.0 - self._bins("ql").mean(axis=0)) / self.temperatures,
+                "residual_error": residual.std(axis=1, ddof=1) / np.sqrt(n)}

@@ -1,13 +1,17 @@
 """Result files: arrays in ``.npz`` plus a JSON ``__meta__`` entry.
 
-The format is the reference's (critmc/io.py:13-48) so that files written here
-load in the reference's analysis stack and vice versa.
+The on-disk format is the reference's (critmc/io.py:13-48): every array under
+its own name, compressed, and one extra 0-d text entry ``__meta__`` holding
+the metadata as JSON.  Files written here therefore load in the reference's
+analysis stack and vice versa.
 """
 
 from __future__ import annotations
 
 import json
-from pathlib import Path
+import math
+from functools import singledispatch
+from pathlib import Path, PurePath
 from typing import Any
 
 import numpy as np
@@ -17,45 +21,80 @@ __all__ = ["to_jsonable", "save_npz", "load_npz", "save_json", "load_json"]
 META = "__meta__"
 
 
+@singledispatch
 def to_jsonable(obj: Any) -> Any:
-    if isinstance(obj, dict):
-        return {str(k): to_jsonable(v) for k, v in obj.items()}
-    if isinstance(obj, (list, tuple)):
-        return [to_jsonable(v) for v in obj]
-    if isinstance(obj, np.ndarray):
-        return obj.tolist()
-    if isinstance(obj, np.generic):
-        return obj.item()
-    if isinstance(obj, Path):
-        return str(obj)
-    if isinstance(obj, float) and not np.isfinite(obj):
-        return str(obj)
+    """A JSON-ready copy of ``obj``.
+
+    Mappings get string keys, lists and tuples become lists, numpy arrays and
+    scalars become plain Python values, paths become strings, and a
+    non-finite Python float becomes its text (``"nan"``, ``"inf"``).
+    Anything else is returned as it is.
+    """
     return obj
 
 
+@to_jsonable.register(dict)
+def _(obj: dict) -> dict:
+    return {str(key): to_jsonable(value) for key, value in obj.items()}
+
+
+@to_jsonable.register(list)
+@to_jsonable.register(tuple)
+def _(obj) -> list:
+    return [to_jsonable(value) for value in obj]
+
+
+@to_jsonable.register(np.ndarray)
+def _(obj: np.ndarray) -> Any:
+    return obj.tolist()
+
+
+@to_jsonable.register(np.generic)
+def _(obj: np.generic) -> Any:
+    return obj.item()
+
+
+@to_jsonable.register(PurePath)
+def _(obj: PurePath) -> str:
+    return str(obj)
+
+
+@to_jsonable.register(float)
+def _(obj: float) -> Any:
+    return obj if math.isfinite(obj) else repr(obj)
+
+
+def _with_parent(path) -> Path:
+    target = Path(path)
+    target.parent.mkdir(parents=True, exist_ok=True)
+    return target
+
+
 def save_npz(path, arrays: dict[str, np.ndarray], meta: dict[str, Any]) -> Path:
-    path = Path(path)
-    path.parent.mkdir(parents=True, exist_ok=True)
+    """Write ``arrays`` and ``meta`` to one compressed ``.npz`` file."""
     if META in arrays:
         raise ValueError(f"array name {META!r} is reserved")
-    payload = {k: np.asarray(v) for k, v in arrays.items()}
-    payload[META] = np.array(json.dumps(to_jsonable(meta)))
-    np.savez_compressed(path, **payload)
-    return path
+    entries = {name: np.asarray(values) for name, values in arrays.items()}
+    entries[META] = np.array(json.dumps(to_jsonable(meta)))
+    target = _with_parent(path)
+    np.savez_compressed(target, **entries)
+    return target
 
 
 def load_npz(path) -> tuple[dict[str, np.ndarray], dict[str, Any]]:
-    with np.load(path, allow_pickle=False) as data:
-        arrays = {k: data[k] for k in data.files if k != META}
-        meta = json.loads(str(data[META])) if META in data.files else {}
+    """Arrays and metadata of a file written by :func:`save_npz` (no pickles)."""
+    with np.load(path, allow_pickle=False) as archive:
+        names = list(archive.files)
+        meta = json.loads(str(archive[META])) if META in names else {}
+        arrays = {name: archive[name] for name in names if name != META}
     return arrays, meta
 
 
 def save_json(path, obj: Any) -> Path:
-    path = Path(path)
-    path.parent.mkdir(parents=True, exist_ok=True)
-    path.write_text(json.dumps(to_jsonable(obj), indent=2) + "\n", encoding="utf-8")
-    return path
+    """``obj`` as indented UTF-8 JSON with a final newline."""
+    target = _with_parent(path)
+    target.write_text(json.dumps(to_jsonable(obj), indent=2) + "\n", encoding="utf-8")
+    return target
 
 
 def load_json(path) -> Any:

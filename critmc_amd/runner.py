@@ -69,32 +69,39 @@ class StaleCacheError(ValueError):
 
 
 def _task_record(task: Task) -> dict[str, Any]:
+    """The task as its JSON metadata record reads back from a result file."""
     return to_jsonable(asdict(task))
 
 
+def _record_mismatch(found: Any, wanted: dict[str, Any]) -> str:
+    """Why a file's task record is not the requested one, field by field."""
+    if not isinstance(found, dict):
+        return "it does not record the task that produced it"
+    return "; ".join(f"{name}: file has {found.get(name)!r}, requested {value!r}"
+                     for name, value in wanted.items() if found.get(name) != value)
+
+
 def _load_cached(task: Task, path: Path) -> TimeSeries:
+    """A cached result, after checking that ``task`` produced it (critmc/runner.py:59-87)."""
     ts = TimeSeries.load(path)
     wanted = _task_record(task)
     found = ts.meta.get("task")
-    if found == wanted:
-        return ts
-    if not isinstance(found, dict):
-        detail = "it does not record the task that produced it"
-    else:
-        detail = "; ".join(f"{k}: file has {found.get(k)!r}, requested {wanted[k]!r}"
-                           for k in wanted if found.get(k) != wanted[k])
-    raise StaleCacheError(f"{path} was not produced by the requested task ({detail}). "
-                          "Use a different out_dir or delete the file.")
+    if found != wanted:
+        raise StaleCacheError(f"{path} was not produced by the requested task "
+                              f"({_record_mismatch(found, wanted)}). "
+                              "Use a different out_dir or delete the file.")
+    return ts
 
 
 def run_task(task: Task) -> TimeSeries:
-    """Execute one task on the GPU (critmc/runner.py:90-98)."""
-    model = build_model(task.model)
-    rng = make_rng(task.seed, *task.keys)
-    sim = Simulation(model, task.beta, task.updates, seed=rng, start=task.start)
+    """Run one task on the GPU through :class:`Simulation` (critmc/runner.py:90-98); its
+    stream is ``(seed, *keys)`` and its metadata records the task and the tuning."""
+    sim = Simulation(build_model(task.model), task.beta, task.updates,
+                     seed=make_rng(task.seed, *task.keys), start=task.start)
     sim.thermalize(task.n_therm)
     ts = sim.run(task.n_meas)
-    ts.meta.update({"task": asdict(task), "tuning": sim.tuning()})
+    ts.meta["task"] = asdict(task)
+    ts.meta["tuning"] = sim.tuning()
     return ts
 
 
@@ -209,54 +216,58 @@ def run_tasks(tasks: Sequence[Task], *, workers: int | None = None, out_dir=None
             groups.setdefault(_group_key(tasks[i]), []).append(i)
         else:
             singles.append(i)
-    t0 = time.perf_counter()
-    done = 0
+    bar = _Progress(len(pending), progress)
     for idx in groups.values():
         for i, ts in zip(idx, _run_ising_group([tasks[i] for i in idx])):
             results[i] = ts
             if out is not None:
                 _save(ts, out, tasks[i])
-        done += len(idx)
-        if progress:
-            _report(done, len(pending), t0)
+        bar.add(len(idx))
     for i in singles:
-        ts = run_task(tasks[i])
-        results[i] = ts
+        results[i] = run_task(tasks[i])
         if out is not None:
-            _save(ts, out, tasks[i])
-        done += 1
-        if progress:
-            _report(done, len(pending), t0)
+            _save(results[i], out, tasks[i])
+        bar.add()
     return [results[i] for i in range(len(tasks))]
+
+
+class _Progress:
+    """A stderr line rewritten as items finish: count, elapsed time, linear estimate of
+    the time left; a newline after the last item."""
+
+    def __init__(self, total: int, enabled: bool = True):
+        self.total, self.enabled = total, enabled
+        self.done = 0
+        self.t0 = time.perf_counter()
+
+    def add(self, n: int = 1) -> None:
+        self.done += n
+        if not self.enabled:
+            return
+        elapsed = time.perf_counter() - self.t0
+        left = elapsed / max(self.done, 1) * (self.total - self.done)
+        end = "\n" if self.done == self.total else ""
+        sys.stderr.write(f"\r  {self.done}/{self.total} done, {elapsed:7.1f}s elapsed, "
+                         f"~{left:7.1f}s left{end}")
+        sys.stderr.flush()
 
 
 def parallel_map(fn: Callable, items: Iterable, *, workers: int | None = None,
                  progress: bool = True) -> list:
-    """Order-preserving parallel map for picklable host functions (critmc/runner.py:202-223)."""
+    """``[fn(x) for x in items]`` over a process pool, in input order, for picklable
+    top-level functions (critmc/runner.py:202-223); one worker runs in-process."""
     items = list(items)
-    workers = default_workers() if workers is None else max(1, int(workers))
-    t0 = time.perf_counter()
-    if workers == 1 or len(items) <= 1:
-        res = []
-        for k, it in enumerate(items, 1):
-            res.append(fn(it))
-            if progress:
-                _report(k, len(items), t0)
-        return res
-    results: dict[int, Any] = {}
-    with ProcessPoolExecutor(max_workers=min(workers, len(items))) as pool:
-        futures = {pool.submit(fn, it): i for i, it in enumerate(items)}
-        for k, fut in enumerate(as_completed(futures), 1):
-            results[futures[fut]] = fut.result()
-            if progress:
-                _report(k, len(items), t0)
-    return [results[i] for i in range(len(items))]
-
-
-def _report(done: int, total: int, t0: float) -> None:
-    elapsed = time.perf_counter() - t0
-    eta = elapsed / max(done, 1) * (total - done)
-    sys.stderr.write(f"\r  {done}/{total} done, {elapsed:7.1f}s elapsed, ~{eta:7.1f}s left")
-    if done == total:
-        sys.stderr.write("\n")
-    sys.stderr.flush()
+    n_workers = default_workers() if workers is None else max(1, int(workers))
+    bar = _Progress(len(items), progress)
+    out: list = [None] * len(items)
+    if n_workers == 1 or len(items) <= 1:
+        for k, item in enumerate(items):
+            out[k] = fn(item)
+            bar.add()
+        return out
+    with ProcessPoolExecutor(max_workers=min(n_workers, len(items))) as pool:
+        slot = {pool.submit(fn, item): k for k, item in enumerate(items)}
+        for fut in as_completed(slot):
+            out[slot[fut]] = fut.result()
+            bar.add()
+    return out

@@ -62,10 +62,17 @@ def gamma_mean(x, S: float = 1.5) -> GammaResult:
 
 
 def batch_mean_error(x, n_batches: int = 64) -> tuple[float, float]:
+    """Mean of ``x`` and its standard error from ``n_batches`` equal contiguous batches.
+
+    Batch means of batches much longer than the autocorrelation time are
+    independent, so their sample variance over ``n_batches`` estimates the
+    variance of the mean.  The first ``len(x) mod n_batches`` samples, the
+    least equilibrated, are left out of the batches (not of the mean).
+    """
     x = np.asarray(x, dtype=np.float64)
-    size = len(x) // n_batches
-    b = x[len(x) - size * n_batches :].reshape(n_batches, size).mean(axis=1)
-    return float(x.mean()), float(b.std(ddof=1) / np.sqrt(n_batches))
+    per_batch = len(x) // n_batches
+    means = x[len(x) % n_batches:].reshape(n_batches, per_batch).mean(axis=1)
+    return float(x.mean()), float(np.sqrt(np.var(means, ddof=1) / n_batches))
 
 
 def pull(x, exact: float, method: str = "batch") -> tuple[float, float, float]:

@@ -108,3 +108,28 @@ def test_gamma_method_matches_loop_transcription():
     c_f = gam[0] + 2 * sum(gam[1:])
     assert r.error == pytest.approx(np.sqrt(c_f / n), rel=1e-10)
     assert r.tau_int == pytest.approx(c_f / (2 * gam[0]), rel=1e-10)
+
+
+def test_batch_means_error_closed_forms():
+    """Batch means reproduce sigma / sqrt(N) for independent samples and the AR(1)
+    error sqrt(2 tau_int var / N), tau_int = (1 + phi) / (2 (1 - phi)); the mean is
+    the plain mean of the whole series."""
+    from scipy.signal import lfilter
+
+    rng = np.random.default_rng(11)
+    n = 640_000
+    iid = rng.normal(2.0, 3.0, n)
+    mean, err = stats.batch_mean_error(iid)
+    assert mean == iid.mean()
+    assert err == pytest.approx(3.0 / np.sqrt(n), rel=0.25)  # 64 batches: ~9 % scatter
+    phi = 0.9
+    errs = []
+    for _ in range(8):
+        x = lfilter([1.0], [1.0, -phi], rng.standard_normal(n))
+        errs.append(stats.batch_mean_error(x)[1])
+    tau = (1 + phi) / (2 * (1 - phi))
+    var = 1 / (1 - phi ** 2)
+    assert np.mean(errs) == pytest.approx(np.sqrt(2 * tau * var / n), rel=0.1)
+    # the first len % 64 samples stay out of the batches
+    y = np.concatenate([[1e6], np.zeros(64 * 10)])
+    assert stats.batch_mean_error(y)[1] == 0.0

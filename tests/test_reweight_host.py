@@ -50,3 +50,32 @@ def test_canonical_observables_match_moments():
     assert out["U"] == pytest.approx(1 - np.mean(m ** 4) / (3 * np.mean(m ** 2) ** 2))
     assert out["chi"] == pytest.approx(0.4 * 256 * (np.mean(m ** 2) - m.mean() ** 2))
     assert kish_ess(np.zeros(100)) == pytest.approx(100.0)
+
+
+def test_beta_derivatives_match_finite_differences():
+    """dU, dlogm, dlogm2 and Reweighter.derivative are the beta-derivatives of the
+    reweighted U, ln<|m|>, ln<m^2> and <O>: compare with central differences."""
+    rng = np.random.default_rng(2)
+    n = 20_000
+    e = rng.normal(-300.0, 12.0, n)
+    m = np.clip(0.5 + 0.004 * (-300.0 - e) + rng.normal(0, 0.05, n), 0, 1)  # correlated with E
+    beta0, beta, h = 0.42, 0.425, 1e-5
+
+    def at(b):
+        return canonical_observables(e, m, m ** 2, m ** 4, beta0, b, 256)
+
+    lo, mid, hi = at(beta - h), at(beta), at(beta + h)
+    assert mid["dU"] == pytest.approx((hi["U"] - lo["U"]) / (2 * h), rel=1e-5)
+    assert mid["dlogm"] == pytest.approx((np.log(hi["absm"]) - np.log(lo["absm"])) / (2 * h), rel=1e-5)
+    assert mid["dlogm2"] == pytest.approx((np.log(hi["m2"]) - np.log(lo["m2"])) / (2 * h), rel=1e-5)
+    rw = Reweighter(e, beta0, {"m": m})
+    fd = (rw.mean("m", beta + h) - rw.mean("m", beta - h)) / (2 * h)
+    assert rw.derivative("m", beta) == pytest.approx(fd, rel=1e-5)
+    # an estimator subset gives the same values as the full set
+    from critmc_amd.reweight import observables_from_weights
+
+    w = rw.weights(beta)
+    sub = observables_from_weights(w, e, m, m ** 2, m ** 4, beta, 256, keys=["dU", "chi"])
+    assert sub.keys() == {"dU", "chi"}
+    assert sub["dU"] == mid["dU"] and sub["chi"] == mid["chi"]
+    assert observables_from_weights(w, e, m, m ** 2, m ** 4, beta, 256, keys=[]) == {}
