@@ -40,6 +40,7 @@
 #include "cluster_core.hpp"
 #include "common.hpp"
 #include "philox.hpp"
+#include "seq_core.hpp"
 
 extern "C" {
 int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims, int32_t n_rep,
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void sw_union(ClArgs a)
 {
     const int rep = blockIdx.y;
     const ClGeo& g = a.g;
-    const uint32_t N = g.N, nb = (uint32_t)g.d * N;
+    const uint32_t N = g.N, nb = (uint32_t)cl_axes(g) * N;
     const int8_t* S = a.spins + (int64_t)rep * N;
     int32_t* P = a.parent + (int64_t)rep * N;
     const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
@@ -121,11 +122,13 @@ __global__ __launch_bounds__(256) void sw_union(ClArgs a)
             const uint32_t b = 4 * blk + t;
             if (b >= nb) break;
             if ((uint64_t)w[t] >= T) continue;
-            const int ax = (int)(b >= N) + (int)(b >= 2 * N);
-            const uint32_t i = b - (uint32_t)ax * N;
-            uint32_t x[3];
-            cl_coords(g, i, x);
-            const uint32_t j = cl_nbr(g, i, x, ax, true);
+            int ax = 0;
+            uint32_t i = b;
+            while (i >= N) {  // at most three steps: z <= 8
+                i -= N;
+                ++ax;
+            }
+            const uint32_t j = cl_nbr_col(g, i, 2 * ax);
             if (S[i] != S[j]) continue;
             cl_unite(P, i, j);
         }
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
     int32_t* Mk = a.mark + (int64_t)rep * N;
     const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
     const uint64_t T = a.t_add[rep];
-    const int z = 2 * g.d;
+    const int z = 2 * cl_axes(g);
     long long total = 0;
     for (int f = 0; f < count; ++f) {
         const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
@@ -247,10 +250,11 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
             for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
                 const uint32_t k = (uint32_t)Q[q];
                 uint32_t x[3];
-                cl_coords(g, k, x);
+                if (!g.nbr) cl_coords(g, k, x);
                 for (int c = 0; c < z; ++c) {
                     const int ax = c >> 1;
-                    const uint32_t j = cl_nbr(g, k, x, ax, (c & 1) == 0);
+                    const uint32_t j = g.nbr ? (uint32_t)g.nbr[(int64_t)k * g.zt + c]
+                                             : cl_nbr(g, k, x, ax, (c & 1) == 0);
                     if (S[j] != s0) continue;
                     const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
                     const u32x4 r = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
@@ -406,6 +410,8 @@ inline void cl_args(ClArgs& a, char* ws, const ClLayout& lay, void* spins, int32
     a.g.N = (uint32_t)N;
     a.g.div0 = make_fastdiv(a.g.L[0]);
     a.g.div1 = make_fastdiv(a.g.L[1]);
+    a.g.nbr = nullptr;
+    a.g.zt = 2 * ndim;
     a.R = n_rep;
     a.q = q;
 }
@@ -417,7 +423,7 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
 {
     const int64_t N = a.g.N;
     const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
-    const unsigned bond_chunks = (unsigned)std::min<int64_t>((a.g.d * N / 4 + 256) / 256, 1024);
+    const unsigned bond_chunks = (unsigned)std::min<int64_t>((cl_axes(a.g) * N / 4 + 256) / 256, 1024);
     if (code == CODE_SWENDSEN_WANG) {
         for (int64_t k = 0; k < count; ++k) {
             a.sweep_add = (uint64_t)k;
@@ -435,6 +441,155 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
     }
     hipLaunchKernelGGL(cl_advance, dim3((a.R + 255) / 256), dim3(256), 0, st, sweeps, a.R,
                        (uint64_t)count);
+}
+
+// Per-chain tables of the sequential kernels (seq_core.hpp) and the cluster
+// bond thresholds, from the reference's FP64 formulas: Ising
+// critmc/kernels/ising.py:47-65 and p_add = 1 - e^{-2 beta} (:201); Potts
+// e^{-beta d}, e^{beta n} and p_add = 1 - e^{-beta} (kernels/potts.py:194-200).
+inline void seq_tables(int q, int z, int n_rep, const double* betas, double h,
+                       std::vector<uint64_t>& tab, std::vector<uint64_t>& tadd)
+{
+    tab.assign((size_t)n_rep * SEQ_TAB, 0);
+    tadd.resize(n_rep);
+    for (int r = 0; r < n_rep; ++r) {
+        uint64_t* t = &tab[(size_t)r * SEQ_TAB];
+        const double b = betas[r];
+        if (q == 0) {
+            for (int al = -z; al <= z; ++al) {
+                for (int up = 0; up < 2; ++up) {
+                    const double de = 2.0 * al + 2.0 * h * (2 * up - 1);
+                    t[2 * (al + z) + up] = cl_threshold(de <= 0.0 ? 1.0 : std::exp(-b * de));
+                }
+                t[SEQ_HB + al + z] = cl_threshold(1.0 / (1.0 + std::exp(-2.0 * b * (al + h))));
+            }
+            tadd[r] = cl_threshold(1.0 - std::exp(-2.0 * b));
+        } else {
+            double* w = (double*)(t + SEQ_W);
+            for (int d = 0; d <= z; ++d) {
+                t[d] = cl_threshold(std::exp(-b * d));
+                w[d] = std::exp(b * d);
+            }
+            tadd[r] = cl_threshold(1.0 - std::exp(-b));
+        }
+    }
+}
+
+// Schedules of any lattice given by its neighbour table: local rows run as
+// sequential chains (seq_run), cluster rows through the same Wolff /
+// Swendsen-Wang kernels as the hypercubic path with neighbours from the table.
+inline int table_run(int q, void* spins, int64_t N, int32_t z, const int32_t* nbr, int32_t n_rep,
+                     const double* betas, double h, const uint32_t* keys, const uint32_t* sids,
+                     uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
+                     int32_t* e_run, int32_t* mc_run, int32_t* e_out, int32_t* mc_out,
+                     int64_t* aux_out, void* workspace, int64_t workspace_bytes, hipStream_t st)
+{
+    CM_REQUIRE(N >= 2 && N * z < ((int64_t)1 << 31), "lattice of %lld sites out of range",
+               (long long)N);
+    CM_REQUIRE(z >= 2 && z <= SEQ_ZMAX && z % 2 == 0,
+               "neighbour table needs an even number of columns in 2..%d, got %d", SEQ_ZMAX, z);
+    CM_REQUIRE(nbr != nullptr, "the neighbour table is required");
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(n_meas >= 0, "n_meas must not be negative");
+    CM_REQUIRE(n_rows >= 1 && n_rows <= 8, "schedule must have 1..8 rows");
+    CM_REQUIRE(std::isfinite(h), "field h must be finite");
+    bool any_cluster = false, any_sw = false;
+    SeqRows all{};
+    all.n = n_rows;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r], count = schedule[2 * r + 1];
+        CM_REQUIRE(code >= CODE_METROPOLIS && code <= CODE_SWENDSEN_WANG,
+                   "update code %lld has no GPU kernel for this model", (long long)code);
+        CM_REQUIRE(count >= 1 && count <= (1 << 24), "repetition count out of range");
+        any_cluster |= code == CODE_WOLFF || code == CODE_SWENDSEN_WANG;
+        any_sw |= code == CODE_SWENDSEN_WANG;
+        all.code[r] = (int32_t)code;
+        all.count[r] = (int32_t)count;
+    }
+    CM_REQUIRE(!any_cluster || h == 0.0, "cluster updates require zero field h");
+    const ClLayout lay = cl_layout(N, n_rep, (int64_t)n_rep * SEQ_TAB * 8, q > 0 ? q : 1);
+    CM_REQUIRE(workspace_bytes >= lay.total, "workspace too small");
+    for (int r = 0; r < n_rep; ++r) CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+    if (n_meas == 0) return CM_OK;
+    char* ws = (char*)workspace;
+    std::vector<uint64_t> tab, tadd;
+    seq_tables(q, z, n_rep, betas, h, tab, tadd);
+    CM_HIP_CHECK(upload_async(ws + lay.tab, tab.data(), tab.size() * 8, st));
+    CM_HIP_CHECK(upload_async(ws + lay.tadd, tadd.data(), tadd.size() * 8, st));
+    SeqArgs sa{};
+    sa.spins = (int8_t*)spins;
+    sa.nbr = nbr;
+    sa.N = (uint32_t)N;
+    sa.z = z;
+    sa.R = n_rep;
+    sa.q = q;
+    sa.tab = (const uint64_t*)(ws + lay.tab);
+    sa.keys = keys;
+    sa.sids = sids;
+    sa.sweeps = sweeps;
+    sa.e_run = e_run;
+    sa.m_run = q ? nullptr : mc_run;
+    sa.counts = q ? mc_run : nullptr;
+    const dim3 lanes((unsigned)((n_rep + 63) / 64));
+    if (!any_cluster) {  // the whole call in one launch
+        sa.acc = nullptr;
+        hipLaunchKernelGGL(seq_run, lanes, dim3(64), 0, st, sa, all, n_meas, e_out, mc_out, aux_out);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    }
+    ClArgs ca{};
+    ca.spins = (int8_t*)spins;
+    ca.parent = (int32_t*)(ws + lay.parent);
+    ca.size = (int32_t*)(ws + lay.size);
+    ca.queue = (int32_t*)(ws + lay.queue);
+    ca.mark = (int32_t*)(ws + lay.mark);
+    ca.stats = (int64_t*)(ws + lay.stats);
+    ca.wsize = (int64_t*)(ws + lay.wsize);
+    ca.t_add = (const uint64_t*)(ws + lay.tadd);
+    ca.keys = keys;
+    ca.sids = sids;
+    ca.sweeps = sweeps;
+    ca.g.d = z / 2;
+    ca.g.L[0] = ca.g.L[1] = ca.g.L[2] = 1;
+    ca.g.N = (uint32_t)N;
+    ca.g.div0 = make_fastdiv(1);
+    ca.g.div1 = make_fastdiv(1);
+    ca.g.nbr = nbr;
+    ca.g.zt = z;
+    ca.R = n_rep;
+    ca.q = q;
+    int64_t* acc = (int64_t*)(ws + lay.acc);
+    sa.acc = acc;
+    const int rb = (n_rep + 255) / 256;
+    hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, ca.stats, 2 * n_rep);
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
+        for (int r = 0; r < n_rows; ++r) {
+            const int code = all.code[r];
+            if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
+                cl_cluster_row(ca, code, all.count[r], sweeps, st);
+                hipLaunchKernelGGL(seq_measure, dim3(n_rep), dim3(256), 0, st, sa);
+            } else {
+                SeqRows one{};
+                one.n = 1;
+                one.code[0] = code;
+                one.count[0] = all.count[r];
+                hipLaunchKernelGGL(seq_run, lanes, dim3(64), 0, st, sa, one, (int64_t)1,
+                                   (int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr);
+            }
+        }
+        if (q)
+            hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, st, n_rep, q, n_meas, meas,
+                               e_run, mc_run, e_out, mc_out, acc, ca.wsize, ca.stats,
+                               any_sw ? 1 : 0, aux_out);
+        else
+            hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas, e_run,
+                               mc_run, e_out, mc_out, acc, ca.wsize, ca.stats, any_sw ? 1 : 0,
+                               aux_out);
+    }
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
 }
 
 }  // namespace cm
@@ -582,6 +737,57 @@ int cm_potts_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                            e_run, counts_run, e_out, counts_out, acc, a.wsize, a.stats,
                            any_sw ? 1 : 0, aux_out);
     }
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int64_t cm_table_workspace_bytes(int64_t N, int32_t n_rep, int32_t q)
+{
+    return cl_layout(N, n_rep, (int64_t)n_rep * SEQ_TAB * 8, q > 0 ? q : 1).total;
+}
+
+int cm_ising_table_run(void* spins, int64_t N, int32_t z, const int32_t* nbr, int32_t n_rep,
+                       const double* betas, double h, const uint32_t* keys,
+                       const uint32_t* stream_ids, uint64_t* sweeps, const int64_t* schedule,
+                       int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* m_run,
+                       int32_t* e_out, int32_t* m_out, int64_t* aux_out, void* workspace,
+                       int64_t workspace_bytes, void* stream)
+{
+    return table_run(0, spins, N, z, nbr, n_rep, betas, h, keys, stream_ids, sweeps, schedule,
+                     n_rows, n_meas, e_run, m_run, e_out, m_out, aux_out, workspace,
+                     workspace_bytes, (hipStream_t)stream);
+}
+
+int cm_potts_table_run(void* spins, int64_t N, int32_t z, const int32_t* nbr, int32_t n_rep,
+                       int32_t q, const double* betas, const uint32_t* keys,
+                       const uint32_t* stream_ids, uint64_t* sweeps, const int64_t* schedule,
+                       int32_t n_rows, int64_t n_meas, int32_t* e_run, int32_t* counts_run,
+                       int32_t* e_out, int32_t* counts_out, int64_t* aux_out, void* workspace,
+                       int64_t workspace_bytes, void* stream)
+{
+    CM_REQUIRE(q >= 2 && q <= 127, "q must be in [2, 127], got %d", q);
+    return table_run(q, spins, N, z, nbr, n_rep, betas, 0.0, keys, stream_ids, sweeps, schedule,
+                     n_rows, n_meas, e_run, counts_run, e_out, counts_out, aux_out, workspace,
+                     workspace_bytes, (hipStream_t)stream);
+}
+
+int cm_table_measure(const void* spins, int64_t N, int32_t z, const int32_t* nbr, int32_t n_rep,
+                     int32_t q, int32_t* e_run, int32_t* mc_run, void* stream)
+{
+    CM_REQUIRE(N >= 2 && z >= 2 && z <= SEQ_ZMAX && z % 2 == 0 && n_rep >= 1 && n_rep <= 65535,
+               "bad lattice or replica count");
+    CM_REQUIRE(q == 0 || (q >= 2 && q <= 127), "q must be 0 (Ising) or in [2, 127]");
+    SeqArgs sa{};
+    sa.spins = (int8_t*)spins;
+    sa.nbr = nbr;
+    sa.N = (uint32_t)N;
+    sa.z = z;
+    sa.R = n_rep;
+    sa.q = q;
+    sa.e_run = e_run;
+    sa.m_run = q ? nullptr : mc_run;
+    sa.counts = q ? mc_run : nullptr;
+    hipLaunchKernelGGL(seq_measure, dim3(n_rep), dim3(256), 0, (hipStream_t)stream, sa);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }

@@ -17,7 +17,17 @@ struct ClGeo {
     uint32_t L[3];
     uint32_t N;
     FastDiv div0, div1;
+    // null: periodic hypercubic neighbours from L; else a neighbour table
+    // [N][zt] in pair form (column 2a = +e_a, 2a+1 = -e_a), any lattice
+    const int32_t* nbr;
+    int zt;
 };
+
+// Bond axes: bond b = a N + i joins i and its +e_a neighbour, a < cl_axes.
+__host__ __device__ __forceinline__ int cl_axes(const ClGeo& g) { return g.nbr ? g.zt / 2 : g.d; }
+
+// Neighbour column c (2a = +e_a, 2a+1 = -e_a) of site i.
+__device__ __forceinline__ uint32_t cl_nbr_col(const ClGeo& g, uint32_t i, int c);
 
 // Coordinates of site i (x fastest).
 __device__ __forceinline__ void cl_coords(const ClGeo& g, uint32_t i, uint32_t (&x)[3])
@@ -45,6 +55,14 @@ __device__ __forceinline__ uint32_t cl_nbr(const ClGeo& g, uint32_t i, const uin
     const uint32_t st = cl_stride(g, a), L = g.L[a];
     if (fwd) return x[a] + 1 == L ? i - (L - 1) * st : i + st;
     return x[a] == 0 ? i + (L - 1) * st : i - st;
+}
+
+__device__ __forceinline__ uint32_t cl_nbr_col(const ClGeo& g, uint32_t i, int c)
+{
+    if (g.nbr) return (uint32_t)g.nbr[(int64_t)i * g.zt + c];
+    uint32_t x[3];
+    cl_coords(g, i, x);
+    return cl_nbr(g, i, x, c >> 1, (c & 1) == 0);
 }
 
 __device__ __forceinline__ uint32_t ld_relaxed(const int32_t* p)

@@ -31,9 +31,9 @@ AUX_SW_C2 = 2
 AUX_SW_NCLUSTERS = 3
 N_AUX = 4
 
-# codes every checkerboard GPU kernel implements, per model kind
+# codes the GPU kernels implement, per model kind
 GPU_UPDATES = {
-    "ising": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
-    "potts": ("metropolis", "heatbath", "wolff", "sw", "swendsen-wang"),
+    "ising": ("metropolis", "metropolis-random", "heatbath", "wolff", "sw", "swendsen-wang"),
+    "potts": ("metropolis", "metropolis-random", "heatbath", "wolff", "sw", "swendsen-wang"),
     "on": ("metropolis", "heatbath", "overrelax", "wolff", "sw", "swendsen-wang"),
 }

@@ -7,8 +7,9 @@ axis-major with ``bond_axis`` (:136-138), and a two-colouring ``sublattice``
 when every extent is even (:140-141).  Sites are numbered with the first
 coordinate running fastest, ``i = x0 + L0 (x1 + L1 x2 ...)`` (:118-128).
 
-The GPU kernels never read ``nbr``: they derive neighbours from ``shape``.
-The tables are kept because host code and tests use them.
+The checkerboard kernels derive neighbours from ``shape``; lattices they cannot
+sweep (odd extents, triangular) run on the neighbour-table kernels, which read
+``nbr`` from the device.
 """
 
 from __future__ import annotations
@@ -18,7 +19,8 @@ from math import prod
 
 import numpy as np
 
-__all__ = ["Lattice", "hypercubic", "chain", "square", "cubic", "triangular", "make_lattice"]
+__all__ = ["Lattice", "hypercubic", "chain", "square", "cubic", "triangular", "make_lattice",
+           "checkerboard_ok", "require_checkerboard"]
 
 
 @dataclass(frozen=True, eq=False)
@@ -166,6 +168,12 @@ def make_lattice(name: str, L: int) -> Lattice:
         return _BUILDERS[name](L)
     except KeyError:
         raise ValueError(f"unknown lattice {name!r}; choose from {sorted(_BUILDERS)}") from None
+
+
+def checkerboard_ok(lattice: Lattice) -> bool:
+    """Whether the checkerboard kernels can sweep this lattice: hypercubic, d <= 3,
+    every extent even."""
+    return lattice.hypercubic and lattice.dim <= 3 and lattice.bipartite
 
 
 def require_checkerboard(lattice: Lattice) -> None:

@@ -26,9 +26,10 @@ from .device import device
 from .kernels import ising as k_ising
 from .kernels import on as k_on
 from .kernels import potts as k_potts
+from .kernels import table as k_table
 from .kernels.chains import Chains
-from .kernels.common import CLUSTER_CODES, GPU_UPDATES, N_AUX, UPDATE_CODES
-from .lattice import Lattice, make_lattice, require_checkerboard
+from .kernels.common import CLUSTER_CODES, GPU_UPDATES, METROPOLIS_RANDOM, N_AUX, UPDATE_CODES
+from .lattice import Lattice, checkerboard_ok, make_lattice, require_checkerboard
 
 __all__ = ["Model", "Ising", "Potts", "ON", "XY", "Heisenberg", "build_model", "derive",
            "DeviceState"]
@@ -91,7 +92,15 @@ class Model(ABC):
                 f"{type(self).__name__} does not support updates {bad}; "
                 f"available: {list(self.updates)}"
             )
+        self._check_lattice()
+
+    def _check_lattice(self) -> None:
         require_checkerboard(self.lattice)
+
+    def sweep_order(self, schedule) -> str:
+        """``"checkerboard"``, or ``"sequential"`` where the schedule runs in the
+        reference's own site orders (typewriter / random sites)."""
+        return "checkerboard"
 
     def spec(self) -> dict[str, Any]:
         return {"model": self.kind, "lattice": self.lattice.name, "L": self.lattice.L}
@@ -104,11 +113,34 @@ class Model(ABC):
         return f"{type(self).__name__}({self.lattice!r})"
 
 
+class _AnyLattice:
+    """Discrete models run on any pair-form lattice: the checkerboard kernels
+    where they apply, the neighbour-table kernels otherwise (and for
+    ``metropolis-random``, whose sites are drawn at random)."""
+
+    def _check_lattice(self) -> None:
+        if not checkerboard_ok(self.lattice):
+            k_table.check_lattice(self.lattice)
+
+    def _sequential(self, schedule) -> bool:
+        codes = np.asarray(schedule)[:, 0]
+        return not checkerboard_ok(self.lattice) or bool((codes == METROPOLIS_RANDOM).any())
+
+    def sweep_order(self, schedule) -> str:
+        return "sequential" if self._sequential(schedule) else "checkerboard"
+
+    def _table_ws(self, ds: DeviceState, q: int):
+        if "table_ws" not in ds.extra:
+            ds.extra["table_ws"] = k_table.Workspace(self.n_sites, ds.spins.shape[0], q,
+                                                     ds.spins.device)
+        return k_table.neighbours(self.lattice, ds.spins.device), ds.extra["table_ws"]
+
+
 def _ising_bond_energy(s: np.ndarray, bonds: np.ndarray) -> int:
     return -int(np.sum(s[bonds[:, 0]].astype(np.int64) * s[bonds[:, 1]]))
 
 
-class Ising(Model):
+class Ising(_AnyLattice, Model):
     """Ising model with optional uniform field ``h`` (critmc/models.py:95-162)."""
 
     kind = "ising"
@@ -158,7 +190,13 @@ class Ising(Model):
         return k_ising.Workspace(1)
 
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
-        if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
+        if self._sequential(schedule):
+            nbr, tws = self._table_ws(ds, 0)
+            e_out, m_out, aux_d = k_table.ising_run(ds.spins, nbr, [float(beta)], self.h, chains,
+                                                    schedule, n_meas, ds.extra["e"],
+                                                    ds.extra["m"], ws=tws)
+            aux = aux_d[0].cpu().numpy()
+        elif np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
             if "cluster_ws" not in ds.extra:
                 ds.extra["cluster_ws"] = k_ising.ClusterWorkspace(self.lattice.shape, 1)
             e_out, m_out, aux_d = k_ising.run_cluster(
@@ -186,7 +224,7 @@ def _potts_bond_energy(s: np.ndarray, bonds: np.ndarray) -> int:
     return -int(np.sum(s[bonds[:, 0]] == s[bonds[:, 1]]))
 
 
-class Potts(Model):
+class Potts(_AnyLattice, Model):
     """q-state Potts model (critmc/models.py:165-222)."""
 
     kind = "potts"
@@ -228,7 +266,13 @@ class Potts(Model):
         return k_potts.Workspace(1)
 
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
-        if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
+        if self._sequential(schedule):
+            nbr, tws = self._table_ws(ds, self.q)
+            e_out, counts, aux_d = k_table.potts_run(ds.spins, nbr, self.q, [float(beta)], chains,
+                                                     schedule, n_meas, ds.extra["e"],
+                                                     ds.extra["counts"], ws=tws)
+            aux = aux_d[0].cpu().numpy()
+        elif np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
             if "cluster_ws" not in ds.extra:
                 ds.extra["cluster_ws"] = k_potts.ClusterWorkspace(self.lattice.shape, 1, self.q)
             e_out, counts, aux_d = k_potts.run_cluster(

@@ -33,6 +33,7 @@ from .io import to_jsonable
 from .kernels import ising as k_ising
 from .kernels.chains import Chains
 from .kernels.common import N_AUX, UPDATE_CODES
+from .lattice import checkerboard_ok
 from .models import Ising, build_model
 from .rng import chain_key, make_rng
 from .sim import Simulation, TimeSeries, parse_updates
@@ -106,13 +107,15 @@ def run_task(task: Task) -> TimeSeries:
 
 
 def _batchable(task: Task) -> bool:
+    """Ising tasks with checkerboard rows only, on a lattice those kernels sweep."""
     if task.model.get("model") != "ising":
         return False
     try:
         rows = parse_updates(task.updates)
+        lattice = build_model(task.model).lattice
     except ValueError:
         return False
-    return all(name in ("metropolis", "heatbath") for name, _ in rows)
+    return checkerboard_ok(lattice) and all(name in ("metropolis", "heatbath") for name, _ in rows)
 
 
 def _group_key(task: Task):

@@ -9,9 +9,12 @@ The interface is the reference's (critmc/sim.py:48-297):
 >>> ts["e"].mean(), ts["absm"].mean()
 
 The chain's state lives on the GPU between calls; ``sim.state`` copies it
-back.  Sweeps are checkerboard (colour 0 then colour 1) and draw
-counter-based Philox numbers keyed by the chain, so a run split into chunks
-of any size reproduces the same series (critmc tests/test_sim_chunks.py).
+back.  On bipartite chain / square / cubic lattices sweeps are checkerboard
+(colour 0 then colour 1); on other lattices, and for schedules with
+``metropolis-random``, the chain runs in the reference's own site orders
+(typewriter sweeps, random sites).  Every draw is a counter-based Philox
+number keyed by the chain, so a run split into chunks of any size reproduces
+the same series (critmc tests/test_sim_chunks.py).
 
 Schedules are ``+``-separated update names with optional ``k*`` repetition
 counts.  A ``wolff`` term without a count is retuned during thermalisation
@@ -287,5 +290,5 @@ class Simulation:
             "wall_time": wall,
             "site_updates_per_s": sweeps * n * n_meas / wall if wall > 0 else None,
             "backend": "hip",
-            "sweep_order": "checkerboard",
+            "sweep_order": self.model.sweep_order(self.schedule),
         }

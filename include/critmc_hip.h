@@ -345,6 +345,49 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
                 int64_t n_meas, int64_t moves_per_meas, int32_t *idx_out_dev, int64_t *ord_out_dev,
                 void *stream);
 
+/* ---- Chains on any lattice (neighbour table) ----------------------------- */
+
+/* Bytes of device scratch for n_rep chains of n_sites sites (q = 0: Ising,
+ * else the number of Potts states). */
+int64_t cm_table_workspace_bytes(int64_t n_sites, int32_t n_rep, int32_t q);
+
+/* Any Ising schedule on a lattice given by its neighbour table nbr_dev
+ * int32 [n_sites][z] in pair form (column 2a = +e_a, 2a+1 = -e_a; z even, <= 8):
+ * odd extents, the triangular lattice (critmc/lattice.py:173-213), or any
+ * lattice with `metropolis-random` rows.  Replaces critmc/kernels/ising.py:
+ * 173-237 (run) with the reference's own site orders: metropolis (0) and
+ * heatbath (2) visit sites 0..N-1 in turn, metropolis-random (1) makes N
+ * attempts at random sites (:68-107), one lane per chain; wolff (3) and sw (4)
+ * (:110-170) take neighbours and bonds (a N + i joins i and nbr[i][2a]) from
+ * the table.  Records E_b, M (int32 [n_rep][n_meas]) and aux_out int64
+ * [n_rep][n_meas][4] as cm_ising_cluster_run; e_run_dev / m_run_dev and
+ * sweeps_dev (one value per repetition of any row) advance.  Cluster rows need
+ * h == 0. */
+int cm_ising_table_run(void *spins_dev, int64_t n_sites, int32_t z, const int32_t *nbr_dev,
+                       int32_t n_rep, const double *betas, double h, const uint32_t *keys_dev,
+                       const uint32_t *stream_ids_dev, uint64_t *sweeps_dev,
+                       const int64_t *schedule, int32_t n_rows, int64_t n_meas,
+                       int32_t *e_run_dev, int32_t *m_run_dev, int32_t *e_out_dev,
+                       int32_t *m_out_dev, int64_t *aux_out_dev, void *workspace_dev,
+                       int64_t workspace_bytes, void *stream);
+
+/* The q-state Potts counterpart (critmc/kernels/potts.py:44-235): records E,
+ * counts (int32 [n_rep][n_meas][q]) and aux_out; counts_run_dev [n_rep][q]. */
+int cm_potts_table_run(void *spins_dev, int64_t n_sites, int32_t z, const int32_t *nbr_dev,
+                       int32_t n_rep, int32_t q, const double *betas, const uint32_t *keys_dev,
+                       const uint32_t *stream_ids_dev, uint64_t *sweeps_dev,
+                       const int64_t *schedule, int32_t n_rows, int64_t n_meas,
+                       int32_t *e_run_dev, int32_t *counts_run_dev, int32_t *e_out_dev,
+                       int32_t *counts_out_dev, int64_t *aux_out_dev, void *workspace_dev,
+                       int64_t workspace_bytes, void *stream);
+
+/* Exact energy and magnetisation (q = 0: Ising E_b, M into mc_run_dev [n_rep])
+ * or occupation counts (q >= 2: mc_run_dev [n_rep][q]) from the neighbour table:
+ * critmc/kernels/ising.py:31-44, kernels/potts.py:27-41. */
+int cm_table_measure(const void *spins_dev, int64_t n_sites, int32_t z, const int32_t *nbr_dev,
+                     int32_t n_rep, int32_t q, int32_t *e_run_dev, int32_t *mc_run_dev,
+                     void *stream);
+
 /* ---- Multi-histogram reweighting (critmc/reweight.py:159-345) ------------ */
 
 /* Bytes of device scratch for K runs and n_targets target betas. */

@@ -456,3 +456,59 @@ class FlatWalker:
                            self.stream, _ptr(self.moves, _u64p), int(n_meas), int(moves_per_meas),
                            _ptr(idx_out, _i32p), _ptr(ord_out, _i64p))
         return idx_out, ord_out
+
+
+_EXTRA_DECLS.update({
+    "orc_table_run": ([_i8p, ctypes.c_int64, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double,
+                       ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p,
+                       _i64p, ctypes.c_int, ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p],
+                      ctypes.c_int),
+})
+
+
+class TableReplay:
+    """One Ising (``q`` = 0) or Potts chain on a lattice given by its neighbour table,
+    in the reference's own site orders (replay_table.c); mutates ``s``."""
+
+    def __init__(self, s, nbr, q: int, beta: float, h: float, key, stream: int = 0,
+                 sweep: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.int8).copy()
+        self.nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        self.N, self.z = self.nbr.shape
+        self.q, self.beta, self.h = int(q), float(beta), float(h)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.sweep = np.array([sweep], np.uint64)
+        e = self._energy()
+        self.e = np.array([e], np.int64)
+        if self.q == 0:
+            self.mc = np.array([int(self.s.astype(np.int64).sum())], np.int64)
+        else:
+            self.mc = np.bincount(self.s, minlength=self.q).astype(np.int64)
+
+    def _energy(self) -> int:
+        s = self.s.astype(np.int64)
+        fwd = self.nbr[:, 0::2]
+        if self.q == 0:
+            return int(-(s[:, None] * s[fwd]).sum())
+        return int(-(s[:, None] == s[fwd]).sum())
+
+    def run(self, schedule, n_meas: int) -> dict[str, np.ndarray]:
+        sched = np.ascontiguousarray(schedule, np.int64).reshape(-1, 2)
+        nmc = 1 if self.q == 0 else self.q
+        e_out = np.zeros(n_meas, np.int64)
+        mc_out = np.zeros((n_meas, nmc), np.int64)
+        aux = np.zeros((n_meas, 4), np.int64)
+        rc = lib().orc_table_run(
+            _ptr(self.s, _i8p), self.N, self.z, _ptr(self.nbr, _i32p), self.q, self.beta, self.h,
+            self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p), _ptr(sched, _i64p),
+            sched.shape[0], int(n_meas), _ptr(self.e, _i64p), _ptr(self.mc, _i64p),
+            _ptr(e_out, _i64p), _ptr(mc_out, _i64p), _ptr(aux, _i64p))
+        if rc:
+            raise ValueError(f"orc_table_run failed ({rc})")
+        out = {"E": e_out, "aux": aux}
+        if self.q == 0:
+            out["M"] = mc_out[:, 0]
+        else:
+            out["counts"] = mc_out
+        return out

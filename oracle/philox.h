@@ -55,6 +55,7 @@ enum {
     ORC_TAG_ONHB = 8,
     ORC_TAG_CLUSTER = 9,
     ORC_TAG_FLAT = 10,
+    ORC_TAG_SEQ = 11,
 };
 
 static inline void orc_draw_block(uint32_t k0, uint32_t k1, uint32_t stream, uint64_t sweep,

@@ -30,20 +30,41 @@ def test_lattice_contract():
 
 
 def test_model_validation():
+    from critmc_amd import ON, Potts, XY
+
     m = Ising(square(4))
-    m.check_updates(["metropolis", "heatbath", "wolff", "sw"])
-    with pytest.raises(ValueError, match="checkerboard GPU kernel"):
-        m.check_updates(["metropolis-random"])
+    m.check_updates(["metropolis", "metropolis-random", "heatbath", "wolff", "sw"])
     with pytest.raises(ValueError):
         m.check_updates(["overrelax"])
+    # odd extents and the triangular lattice run in the reference's site orders
+    for lat in (square(5), square(3), triangular(4), triangular(3)):
+        for model in (Ising(lat), Potts(lat, 3)):
+            model.check_updates(["metropolis", "metropolis-random", "heatbath", "wolff", "sw"])
+            assert model.sweep_order([[0, 1]]) == "sequential"
+    assert Ising(square(4)).sweep_order([[0, 1], [3, 1]]) == "checkerboard"
+    assert Ising(square(4)).sweep_order([[1, 1]]) == "sequential"
+    # the O(n) kernels stay checkerboard-only
     with pytest.raises(ValueError, match="bipartite"):
-        Ising(square(5)).check_updates(["metropolis"])
-    with pytest.raises(ValueError):
-        Ising(triangular(4)).check_updates(["metropolis"])
+        XY(square(5)).check_updates(["metropolis"])
+    with pytest.raises(ValueError, match="checkerboard GPU kernel"):
+        ON(square(4), 3).check_updates(["metropolis-random"])
     with pytest.raises(ValueError, match="zero field"):
         Ising(square(4), h=0.1).check_updates(["wolff"])
     with pytest.raises(ValueError):
         Ising(square(4), h=float("inf"))
+
+
+def test_neighbour_tables_must_be_in_pair_form():
+    import dataclasses
+
+    from critmc_amd.kernels.table import check_lattice
+
+    check_lattice(triangular(3))
+    check_lattice(square(5, 3))
+    lat = triangular(3)
+    bad = dataclasses.replace(lat, nbr=lat.nbr[:, [2, 1, 0, 3, 4, 5]])  # +x and +y swapped
+    with pytest.raises(ValueError, match="pair form"):
+        check_lattice(bad)
 
 
 def test_initial_state_matches_reference_generator_use():
