@@ -80,6 +80,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
          _vp, _c_i64, _c_i32, _vp],
         _c_i32,
     ),
+    "cm_set_graph_min_meas": ([_c_i64], _c_i64),
     "cm_table_workspace_bytes": ([_c_i64, _c_i32, _c_i32], _c_i64),
     "cm_ising_table_run": (
         [_vp, _c_i64, _c_i32, _vp, _c_i32, _f64p, _c_f64, _vp, _vp, _vp, _i64p, _c_i32, _c_i64,

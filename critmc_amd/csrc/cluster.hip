@@ -68,6 +68,20 @@ int64_t cm_potts_workspace_bytes(int32_t n_rep);
 }
 
 namespace cm {
+// potts.hip: the tables of cm_potts_run, and its launches without the upload
+int potts_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
+                  void* workspace, int64_t workspace_bytes, void* stream);
+int potts_rows(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+               const uint32_t* keys, const uint32_t* stream_ids, uint64_t* sweeps,
+               const int64_t* schedule, int32_t n_rows, int64_t n_meas, int32_t* e_run,
+               int32_t* counts_run, int32_t* e_out, int32_t* acc_out, int32_t* counts_out,
+               void* workspace, int64_t workspace_bytes, int32_t block, int32_t grid,
+               const int64_t* n_meas_rep, void* stream);
+// ising.hip: whether cm_ising_run's automatic form only launches kernels
+bool ising_rows_capturable(int32_t ndim, const int32_t* dims, int32_t n_rep);
+}  // namespace cm
+
+namespace cm {
 
 struct ClArgs {
     int8_t* spins;          // [R][N] int8
@@ -283,14 +297,14 @@ __global__ void cl_advance(uint64_t* sweeps, int R, uint64_t n)
 }
 
 // aux[rep][meas][4] = (accepted, wolff size, sum |C|^2, clusters); e/m record.
-__global__ void cl_record(int R, int64_t n_meas, int64_t meas, const int32_t* e_run,
+__global__ void cl_record(int R, int64_t n_meas, const int64_t* meas_d, const int32_t* e_run,
                           const int32_t* m_run, int32_t* e_out, int32_t* m_out,
                           const int64_t* acc, const int64_t* wsize, const int64_t* stats,
                           int have_sw, int64_t* aux)
 {
     const int rep = blockIdx.x * blockDim.x + threadIdx.x;
     if (rep >= R) return;
-    const int64_t o = (int64_t)rep * n_meas + meas;
+    const int64_t o = (int64_t)rep * n_meas + *meas_d;
     e_out[o] = e_run[rep];
     m_out[o] = m_run[rep];
     aux[4 * o + 0] = acc[rep];
@@ -311,6 +325,18 @@ __global__ void cl_zero64(int64_t* p, int n)
     if (t < n) p[t] = 0;
 }
 
+// The measurement index lives on the device, so that every measurement
+// launches identical kernels (and can be replayed from one captured graph).
+__global__ void cl_set64(int64_t* p, int64_t v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+__global__ void cl_next(int64_t* p)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p += 1;
+}
+
 inline uint64_t cl_threshold(double p)
 {
     if (!(p < 1.0)) return 1ull << 32;
@@ -318,14 +344,14 @@ inline uint64_t cl_threshold(double p)
     return (uint64_t)std::ceil(p * 4294967296.0);
 }
 
-__global__ void cl_record_potts(int R, int q, int64_t n_meas, int64_t meas, const int32_t* e_run,
-                                const int32_t* counts_run, int32_t* e_out, int32_t* counts_out,
-                                const int64_t* acc, const int64_t* wsize, const int64_t* stats,
-                                int have_sw, int64_t* aux)
+__global__ void cl_record_potts(int R, int q, int64_t n_meas, const int64_t* meas_d,
+                                const int32_t* e_run, const int32_t* counts_run, int32_t* e_out,
+                                int32_t* counts_out, const int64_t* acc, const int64_t* wsize,
+                                const int64_t* stats, int have_sw, int64_t* aux)
 {
     const int rep = blockIdx.x * blockDim.x + threadIdx.x;
     if (rep >= R) return;
-    const int64_t o = (int64_t)rep * n_meas + meas;
+    const int64_t o = (int64_t)rep * n_meas + *meas_d;
     e_out[o] = e_run[rep];
     for (int k = 0; k < q; ++k) counts_out[o * q + k] = counts_run[(int64_t)rep * q + k];
     aux[4 * o + 0] = acc[rep];
@@ -335,7 +361,7 @@ __global__ void cl_record_potts(int R, int q, int64_t n_meas, int64_t meas, cons
 }
 
 struct ClLayout {
-    int64_t tab, parent, size, queue, mark, stats, wsize, acc, tadd, rowacc, rowe, rowm, total;
+    int64_t tab, parent, size, queue, mark, stats, wsize, acc, tadd, rowacc, rowe, rowm, meas, total;
 };
 
 // Workspace: the local kernels' tables (tab_bytes), then the cluster arrays;
@@ -357,6 +383,7 @@ inline ClLayout cl_layout(int64_t N, int32_t n_rep, int64_t tab_bytes, int row_m
     l.rowacc = o; o += up(4 * (int64_t)n_rep);
     l.rowe = o;   o += up(4 * (int64_t)n_rep);
     l.rowm = o;   o += up(4 * (int64_t)n_rep * row_m_words);
+    l.meas = o;   o += up(8);
     l.total = o;
     return l;
 }
@@ -441,6 +468,91 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
     }
     hipLaunchKernelGGL(cl_advance, dim3((a.R + 255) / 256), dim3(256), 0, st, sweeps, a.R,
                        (uint64_t)count);
+}
+
+// ---------------------------------------------------------------------------
+// Measurement loops as graphs.  A measurement of a cluster schedule is a dozen
+// small launches (fills, the Swendsen-Wang passes or the Wolff search, local
+// sweeps, a recount, the record), so on small lattices the loop is bound by
+// launch cost.  When every launch of a measurement is a plain kernel launch
+// with the same arguments each time (the measurement index and the sweep
+// counters live on the device), one measurement is captured into a hipGraph on
+// a private stream and the graph is replayed n times on the caller's stream.
+
+// Measurements from which a loop is replayed as a graph (0: never).  Off by
+// default: on MI355X a replayed measurement took 0.89-1.02x the eager time
+// (SW and Wolff on 16^2-64^2, 1-64 replicas), because each node still costs a
+// kernel dispatch on the device; the host launch cost it removes is hidden.
+int64_t g_graph_min_meas = 0;
+
+namespace {
+struct Retired {
+    hipGraphExec_t exec;
+    hipEvent_t done;  // recorded after the graph's last replay
+};
+thread_local std::vector<Retired> g_retired;
+thread_local hipStream_t g_capture[64] = {};
+
+void release_finished_graphs()
+{
+    size_t keep = 0;
+    for (Retired& r : g_retired) {
+        if (hipEventQuery(r.done) == hipSuccess) {
+            (void)hipGraphExecDestroy(r.exec);
+            (void)hipEventDestroy(r.done);
+        } else {
+            g_retired[keep++] = r;
+        }
+    }
+    g_retired.resize(keep);
+}
+}  // namespace
+
+// body(stream, eager) enqueues one measurement.  Runs it n times on st: eagerly
+// (eager = true), or, when `capturable` and n reaches g_graph_min_meas,
+// captured once (eager = false) and replayed n times.  *graphed says which.
+template <class F>
+int repeat_measurements(int64_t n, bool capturable, hipStream_t st, bool* graphed, F&& body)
+{
+    *graphed = false;
+    release_finished_graphs();
+    if (!capturable || g_graph_min_meas <= 0 || n < g_graph_min_meas) {
+        for (int64_t i = 0; i < n; ++i)
+            if (int rc = body(st, true)) return rc;
+        return CM_OK;
+    }
+    int dev = 0;
+    CM_HIP_CHECK(hipGetDevice(&dev));
+    CM_REQUIRE(dev >= 0 && dev < 64, "device %d out of range", dev);
+    hipStream_t& cap = g_capture[dev];
+    if (cap == nullptr) CM_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    CM_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int rc = body(cap, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ended = hipStreamEndCapture(cap, &graph);
+    if (rc != CM_OK) {
+        if (ended == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    CM_HIP_CHECK(ended);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t made = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CM_HIP_CHECK(made);
+    hipError_t e = hipSuccess;
+    for (int64_t i = 0; i < n && e == hipSuccess; ++i) e = hipGraphLaunch(exec, st);
+    hipEvent_t done = nullptr;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(done, st);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        if (done) (void)hipEventDestroy(done);
+        CM_HIP_CHECK(e);
+    }
+    g_retired.push_back({exec, done});
+    *graphed = true;
+    return CM_OK;
 }
 
 // Per-chain tables of the sequential kernels (seq_core.hpp) and the cluster
@@ -559,37 +671,42 @@ inline int table_run(int q, void* spins, int64_t N, int32_t z, const int32_t* nb
     ca.R = n_rep;
     ca.q = q;
     int64_t* acc = (int64_t*)(ws + lay.acc);
+    int64_t* meas_d = (int64_t*)(ws + lay.meas);
     sa.acc = acc;
     const int rb = (n_rep + 255) / 256;
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, ca.stats, 2 * n_rep);
-    for (int64_t meas = 0; meas < n_meas; ++meas) {
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
+    hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
+    auto measurement = [&](hipStream_t s, bool) -> int {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, ca.wsize, n_rep);
         for (int r = 0; r < n_rows; ++r) {
             const int code = all.code[r];
             if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
-                cl_cluster_row(ca, code, all.count[r], sweeps, st);
-                hipLaunchKernelGGL(seq_measure, dim3(n_rep), dim3(256), 0, st, sa);
+                cl_cluster_row(ca, code, all.count[r], sweeps, s);
+                hipLaunchKernelGGL(seq_measure, dim3(n_rep), dim3(256), 0, s, sa);
             } else {
                 SeqRows one{};
                 one.n = 1;
                 one.code[0] = code;
                 one.count[0] = all.count[r];
-                hipLaunchKernelGGL(seq_run, lanes, dim3(64), 0, st, sa, one, (int64_t)1,
+                hipLaunchKernelGGL(seq_run, lanes, dim3(64), 0, s, sa, one, (int64_t)1,
                                    (int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr);
             }
         }
         if (q)
-            hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, st, n_rep, q, n_meas, meas,
-                               e_run, mc_run, e_out, mc_out, acc, ca.wsize, ca.stats,
-                               any_sw ? 1 : 0, aux_out);
+            hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, s, n_rep, q, n_meas,
+                               (const int64_t*)meas_d, e_run, mc_run, e_out, mc_out, acc,
+                               ca.wsize, ca.stats, any_sw ? 1 : 0, aux_out);
         else
-            hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas, e_run,
-                               mc_run, e_out, mc_out, acc, ca.wsize, ca.stats, any_sw ? 1 : 0,
-                               aux_out);
-    }
-    CM_HIP_CHECK(hipGetLastError());
-    return CM_OK;
+            hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, s, n_rep, n_meas,
+                               (const int64_t*)meas_d, e_run, mc_run, e_out, mc_out, acc,
+                               ca.wsize, ca.stats, any_sw ? 1 : 0, aux_out);
+        hipLaunchKernelGGL(cl_next, dim3(1), dim3(64), 0, s, meas_d);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    };
+    bool graphed;
+    return repeat_measurements(n_meas, true, st, &graphed, measurement);
 }
 
 }  // namespace cm
@@ -649,10 +766,19 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                                   lay.parent - lay.tab, rec, stream);
         if (rc != CM_OK) return rc;
     }
+    int64_t* meas_d = (int64_t*)(ws + lay.meas);
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
-    for (int64_t meas = 0; meas < n_meas; ++meas) {
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, a.wsize, n_rep);
+    hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
+    int64_t per_meas = 0;
+    bool any_local = false;
+    for (int row = 0; row < n_rows; ++row) {
+        per_meas += schedule[2 * row + 1];
+        any_local |= schedule[2 * row] == CODE_METROPOLIS || schedule[2 * row] == CODE_HEATBATH;
+    }
+    // eager: the host counters advance row by row (the plain form reads them)
+    auto measurement = [&](hipStream_t s, bool eager) -> int {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, a.wsize, n_rep);
         for (int row = 0; row < n_rows; ++row) {
             const int64_t code = schedule[2 * row], count = schedule[2 * row + 1];
             if (code == CODE_METROPOLIS || code == CODE_HEATBATH) {
@@ -660,20 +786,29 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
                 int rc = cm_ising_run(spins, 0, ndim, dims, n_rep, betas, h, keys, stream_ids,
                                       sweeps, sweeps_host, one, 1, 1, e_run, m_run, row_e, row_m,
                                       row_acc, ws + lay.tab, lay.parent - lay.tab, 0, 0, 0,
-                                      nullptr, prep[code == CODE_HEATBATH ? 1 : 0], stream);
+                                      nullptr, prep[code == CODE_HEATBATH ? 1 : 0], s);
                 if (rc != CM_OK) return rc;
-                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
+                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, s, n_rep, row_acc, acc);
             } else {
-                cl_cluster_row(a, code, count, sweeps, st);
-                int rc = cm_ising_measure(spins, ndim, dims, n_rep, e_run, m_run, stream);
+                cl_cluster_row(a, code, count, sweeps, s);
+                int rc = cm_ising_measure(spins, ndim, dims, n_rep, e_run, m_run, s);
                 if (rc != CM_OK) return rc;
             }
-            for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
+            if (eager)
+                for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)count;
         }
-        hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas, e_run,
-                           m_run, e_out, m_out, acc, a.wsize, a.stats, any_sw ? 1 : 0, aux_out);
-    }
-    CM_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(cl_record, dim3(rb), dim3(256), 0, s, n_rep, n_meas,
+                           (const int64_t*)meas_d, e_run, m_run, e_out, m_out, acc, a.wsize,
+                           a.stats, any_sw ? 1 : 0, aux_out);
+        hipLaunchKernelGGL(cl_next, dim3(1), dim3(64), 0, s, meas_d);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    };
+    bool graphed;
+    const bool capturable = !any_local || ising_rows_capturable(ndim, dims, n_rep);
+    if (int rc = repeat_measurements(n_meas, capturable, st, &graphed, measurement)) return rc;
+    if (graphed)
+        for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)(per_meas * n_meas);
     return CM_OK;
 }
 
@@ -714,31 +849,44 @@ int cm_potts_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
     int32_t* row_e = (int32_t*)(ws + lay.rowe);
     int32_t* row_c = (int32_t*)(ws + lay.rowm);
     const int rb = (n_rep + 255) / 256;
+    int64_t* meas_d = (int64_t*)(ws + lay.meas);
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
-    for (int64_t meas = 0; meas < n_meas; ++meas) {
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, acc, n_rep);
-        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, st, a.wsize, n_rep);
+    hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
+    // the local rows' tables are uploaded once per call
+    for (int row = 0; row < n_rows; ++row) {
+        if (schedule[2 * row] != CODE_METROPOLIS && schedule[2 * row] != CODE_HEATBATH) continue;
+        if (int rc = potts_prepare(ndim, dims, n_rep, betas, ws + lay.tab, lay.parent - lay.tab,
+                                   stream))
+            return rc;
+        break;
+    }
+    auto measurement = [&](hipStream_t s, bool) -> int {
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, acc, n_rep);
+        hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, a.wsize, n_rep);
         for (int row = 0; row < n_rows; ++row) {
             const int64_t code = schedule[2 * row], count = schedule[2 * row + 1];
             if (code == CODE_METROPOLIS || code == CODE_HEATBATH) {
                 const int64_t one[2] = {code, count};
-                int rc = cm_potts_run(spins, ndim, dims, n_rep, q, betas, keys, stream_ids, sweeps,
-                                      one, 1, 1, e_run, counts_run, row_e, row_acc, row_c,
-                                      ws + lay.tab, lay.parent - lay.tab, 0, 0, nullptr, stream);
+                int rc = potts_rows(spins, ndim, dims, n_rep, q, keys, stream_ids, sweeps, one, 1,
+                                    1, e_run, counts_run, row_e, row_acc, row_c, ws + lay.tab,
+                                    lay.parent - lay.tab, 0, 0, nullptr, s);
                 if (rc != CM_OK) return rc;
-                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, st, n_rep, row_acc, acc);
+                hipLaunchKernelGGL(cl_add_acc, dim3(rb), dim3(256), 0, s, n_rep, row_acc, acc);
             } else {
-                cl_cluster_row(a, code, count, sweeps, st);
-                int rc = cm_potts_measure(spins, ndim, dims, n_rep, q, e_run, counts_run, stream);
+                cl_cluster_row(a, code, count, sweeps, s);
+                int rc = cm_potts_measure(spins, ndim, dims, n_rep, q, e_run, counts_run, s);
                 if (rc != CM_OK) return rc;
             }
         }
-        hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, st, n_rep, q, n_meas, meas,
-                           e_run, counts_run, e_out, counts_out, acc, a.wsize, a.stats,
-                           any_sw ? 1 : 0, aux_out);
-    }
-    CM_HIP_CHECK(hipGetLastError());
-    return CM_OK;
+        hipLaunchKernelGGL(cl_record_potts, dim3(rb), dim3(256), 0, s, n_rep, q, n_meas,
+                           (const int64_t*)meas_d, e_run, counts_run, e_out, counts_out, acc,
+                           a.wsize, a.stats, any_sw ? 1 : 0, aux_out);
+        hipLaunchKernelGGL(cl_next, dim3(1), dim3(64), 0, s, meas_d);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
+    };
+    bool graphed;
+    return repeat_measurements(n_meas, true, st, &graphed, measurement);
 }
 
 int64_t cm_table_workspace_bytes(int64_t N, int32_t n_rep, int32_t q)
@@ -790,6 +938,13 @@ int cm_table_measure(const void* spins, int64_t N, int32_t z, const int32_t* nbr
     hipLaunchKernelGGL(seq_measure, dim3(n_rep), dim3(256), 0, (hipStream_t)stream, sa);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
+}
+
+int64_t cm_set_graph_min_meas(int64_t n)
+{
+    const int64_t old = g_graph_min_meas;
+    g_graph_min_meas = n;
+    return old;
 }
 
 }  // extern "C"

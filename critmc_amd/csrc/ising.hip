@@ -938,6 +938,16 @@ static int prepare_tables(int z, int n_rep, const double* betas, double h, const
     return CM_OK;
 }
 
+// Whether cm_ising_run with form 0 on int8 spins only launches kernels (the
+// batched or wide form) rather than the plain form, which reads the host sweep
+// counters and synchronises the stream: then its launches can be captured.
+bool ising_rows_capturable(int32_t ndim, const int32_t* dims, int32_t n_rep)
+{
+    Geo g;
+    if (geo_from(ndim, dims, &g) != CM_OK) return false;
+    return prefer_wide(g, 0, n_rep) || batched_ok(g);
+}
+
 }  // namespace cm
 
 using namespace cm;

@@ -504,50 +504,39 @@ static void launch_wide(const dim3& wgrid, hipStream_t st, int8_t* spins, const 
 
 using namespace cm;
 
-extern "C" {
+extern "C" int64_t cm_potts_workspace_bytes(int32_t n_rep);
 
-int64_t cm_potts_workspace_bytes(int32_t n_rep) { return (int64_t)n_rep * sizeof(PottsTab); }
+namespace cm {
 
-int cm_potts_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
-                  const uint32_t* keys, const uint32_t* stream_ids, int32_t start, void* stream)
+// Tables of n_rep chains (Metropolis thresholds ceil(e^{-beta d} 2^32), heat-bath
+// weights e^{beta n}; critmc/kernels/potts.py:194-200) uploaded into the workspace.
+int potts_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
+                  void* workspace, int64_t workspace_bytes, void* stream)
 {
     CbGeo g;
     CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
-    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
-    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
-    CM_REQUIRE(start == 0 || start == 1, "start must be 0 (random) or 1 (ordered)");
-    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((g.N / 4 + 255) / 256, 1), 1024);
-    hipLaunchKernelGGL(potts_init_kernel, dim3((unsigned)blocks, n_rep), dim3(256), 0,
-                       (hipStream_t)stream, (int8_t*)spins, g.N, n_rep, q, keys, stream_ids,
-                       start);
-    CM_HIP_CHECK(hipGetLastError());
+    CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
+    CM_REQUIRE(workspace_bytes >= cm_potts_workspace_bytes(n_rep), "workspace too small");
+    const int z = 2 * g.d;
+    std::vector<PottsTab> tabs(n_rep);
+    for (int r = 0; r < n_rep; ++r) {
+        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
+        for (int d = 0; d < 8; ++d) tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
+        for (int d = 0; d < 16; ++d) tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
+    }
+    CM_HIP_CHECK(upload_async(workspace, tabs.data(), tabs.size() * sizeof(PottsTab),
+                              (hipStream_t)stream));
     return CM_OK;
 }
 
-int cm_potts_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
-                     int32_t q, int32_t* e_run, int32_t* counts_run, void* stream)
-{
-    CbGeo g;
-    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
-    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
-    hipStream_t st = (hipStream_t)stream;
-    const int8_t* s = (const int8_t*)spins;
-    if (g.d == 1)
-        hipLaunchKernelGGL(potts_measure_kernel<2>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
-    else if (g.d == 2)
-        hipLaunchKernelGGL(potts_measure_kernel<4>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
-    else
-        hipLaunchKernelGGL(potts_measure_kernel<6>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
-    CM_HIP_CHECK(hipGetLastError());
-    return CM_OK;
-}
-
-int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
-                 const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
-                 uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
-                 int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
-                 int32_t* counts_out, void* workspace, int64_t workspace_bytes, int32_t block,
-                 int32_t grid, const int64_t* n_meas_rep, void* stream)
+// The sweeps of cm_potts_run with the tables already in the workspace: kernel
+// launches only (n_meas = 0 validates the arguments and launches nothing).
+int potts_rows(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+               const uint32_t* keys, const uint32_t* stream_ids, uint64_t* sweeps,
+               const int64_t* schedule, int32_t n_rows, int64_t n_meas, int32_t* e_run,
+               int32_t* counts_run, int32_t* e_out, int32_t* acc_out, int32_t* counts_out,
+               void* workspace, int64_t workspace_bytes, int32_t block, int32_t grid,
+               const int64_t* n_meas_rep, void* stream)
 {
     CbGeo g;
     CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
@@ -569,15 +558,8 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
     }
     if (n_meas == 0) return CM_OK;
     const int z = 2 * g.d;
-    std::vector<PottsTab> tabs(n_rep);
-    for (int r = 0; r < n_rep; ++r) {
-        CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
-        for (int d = 0; d < 8; ++d) tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
-        for (int d = 0; d < 16; ++d) tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
-    }
     hipStream_t st = (hipStream_t)stream;
     PottsTab* dtabs = (PottsTab*)workspace;
-    CM_HIP_CHECK(upload_async(dtabs, tabs.data(), tabs.size() * sizeof(PottsTab), st));
     if (block <= 0) {  // one lane per site quad of a colour, in whole waves, at most 256
         const int64_t quads = (g.half + 3) / 4;
         block = (int)std::min<int64_t>(256, std::max<int64_t>(64, (quads + 63) / 64 * 64));
@@ -634,6 +616,65 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
                               counts_run, e_out, acc_out, counts_out, n_meas_rep, block, grid, st);
     CM_HIP_CHECK(err);
     return CM_OK;
+}
+
+}  // namespace cm
+
+extern "C" {
+
+int64_t cm_potts_workspace_bytes(int32_t n_rep) { return (int64_t)n_rep * sizeof(PottsTab); }
+
+int cm_potts_init(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                  const uint32_t* keys, const uint32_t* stream_ids, int32_t start, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
+    CM_REQUIRE(n_rep >= 1 && n_rep <= 65535, "n_rep out of range");
+    CM_REQUIRE(start == 0 || start == 1, "start must be 0 (random) or 1 (ordered)");
+    const int64_t blocks = std::min<int64_t>(std::max<int64_t>((g.N / 4 + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(potts_init_kernel, dim3((unsigned)blocks, n_rep), dim3(256), 0,
+                       (hipStream_t)stream, (int8_t*)spins, g.N, n_rep, q, keys, stream_ids,
+                       start);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_potts_measure(const void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep,
+                     int32_t q, int32_t* e_run, int32_t* counts_run, void* stream)
+{
+    CbGeo g;
+    CM_REQUIRE(cb_geo_from(ndim, dims, &g) == 0, "checkerboard sweeps need 1..3 even extents >= 2");
+    CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
+    hipStream_t st = (hipStream_t)stream;
+    const int8_t* s = (const int8_t*)spins;
+    if (g.d == 1)
+        hipLaunchKernelGGL(potts_measure_kernel<2>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    else if (g.d == 2)
+        hipLaunchKernelGGL(potts_measure_kernel<4>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    else
+        hipLaunchKernelGGL(potts_measure_kernel<6>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
+
+int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int32_t q,
+                 const double* betas, const uint32_t* keys, const uint32_t* stream_ids,
+                 uint64_t* sweeps, const int64_t* schedule, int32_t n_rows, int64_t n_meas,
+                 int32_t* e_run, int32_t* counts_run, int32_t* e_out, int32_t* acc_out,
+                 int32_t* counts_out, void* workspace, int64_t workspace_bytes, int32_t block,
+                 int32_t grid, const int64_t* n_meas_rep, void* stream)
+{
+    if (int rc = potts_rows(spins, ndim, dims, n_rep, q, keys, stream_ids, sweeps, schedule,
+                            n_rows, 0, e_run, counts_run, e_out, acc_out, counts_out, workspace,
+                            workspace_bytes, block, grid, n_meas_rep, stream))
+        return rc;  // arguments checked before anything is uploaded
+    if (n_meas == 0) return CM_OK;
+    if (int rc = potts_prepare(ndim, dims, n_rep, betas, workspace, workspace_bytes, stream))
+        return rc;
+    return potts_rows(spins, ndim, dims, n_rep, q, keys, stream_ids, sweeps, schedule, n_rows,
+                      n_meas, e_run, counts_run, e_out, acc_out, counts_out, workspace,
+                      workspace_bytes, block, grid, n_meas_rep, stream);
 }
 
 }  // extern "C"

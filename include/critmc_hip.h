@@ -345,6 +345,14 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
                 int64_t n_meas, int64_t moves_per_meas, int32_t *idx_out_dev, int64_t *ord_out_dev,
                 void *stream);
 
+/* Schedules with cluster rows run measurement by measurement; from n such
+ * measurements in one call on (n <= 0, the default: never), one measurement's
+ * launches are captured into a hipGraph and the graph is replayed n times.
+ * Applies to cm_ising_cluster_run, cm_potts_cluster_run and the table runs;
+ * results are the same either way.  Returns the previous value.  No reference
+ * counterpart (the reference loops in compiled code, kernels/ising.py:203-236). */
+int64_t cm_set_graph_min_meas(int64_t n);
+
 /* ---- Chains on any lattice (neighbour table) ----------------------------- */
 
 /* Bytes of device scratch for n_rep chains of n_sites sites (q = 0: Ising,
