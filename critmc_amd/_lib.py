@@ -80,7 +80,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
          _vp, _c_i64, _c_i32, _vp],
         _c_i32,
     ),
-    "cm_set_graph_min_meas": ([_c_i64], _c_i64),
+    "cm_set_option": ([ctypes.c_char_p, _c_i64, _i64p], _c_i32),
     "cm_table_workspace_bytes": ([_c_i64, _c_i32, _c_i32], _c_i64),
     "cm_ising_table_run": (
         [_vp, _c_i64, _c_i32, _vp, _c_i32, _f64p, _c_f64, _vp, _vp, _vp, _i64p, _c_i32, _c_i64,
@@ -145,3 +145,10 @@ def check(rc: int, what: str = "") -> None:
     if rc == CM_EUNSUPPORTED:
         raise NotImplementedError(text)
     raise RuntimeError(text)
+
+
+def set_option(name: str, value: int) -> int:
+    """Set a library tuning option (see ``cm_set_option``); returns the old value."""
+    old = ctypes.c_int64(0)
+    check(lib().cm_set_option(name.encode(), int(value), ctypes.byref(old)), "cm_set_option")
+    return int(old.value)

@@ -34,6 +34,7 @@
 // cm_potts_measure), as the reference recounts after Swendsen-Wang.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "cb_geom.hpp"
@@ -260,6 +261,8 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
         const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
         while (true) {
             const uint32_t lo = s_head, hi = s_tail;
+            // every lane has this level's bounds before any lane appends to the queue
+            __syncthreads();
             if (lo == hi) break;
             for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
                 const uint32_t k = (uint32_t)Q[q];
@@ -335,6 +338,278 @@ __global__ void cl_set64(int64_t* p, int64_t v)
 __global__ void cl_next(int64_t* p)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Schedules made only of cluster rows on lattices of at most CL_LDS_SITES
+// sites: one workgroup per replica runs every update and measurement of the
+// call with the spins, parent / size (Swendsen-Wang) and queue / mark (Wolff)
+// arrays in LDS, in one launch.  It draws the same Philox words as the kernels
+// above and builds the same components (a component does not depend on the
+// order its bonds are joined in, and its root is its smallest site either
+// way), so the chains are identical to the multi-launch path.
+constexpr int CL_LDS_SITES = 8192;
+constexpr int CL_LDS_THREADS = 256;
+int64_t g_cluster_lds_max_sites = CL_LDS_SITES;  // 0: always the multi-launch path
+
+__device__ __forceinline__ uint32_t lds_ld(const int32_t* p)
+{
+    return (uint32_t)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_st(int32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, (int32_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t lds_find(int32_t* P, uint32_t x)
+{
+    while (true) {
+        const uint32_t p = lds_ld(P + x);
+        if (p == x) return x;
+        const uint32_t gp = lds_ld(P + p);
+        if (gp != p) lds_st(P + x, gp);
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void lds_unite(int32_t* P, uint32_t i, uint32_t j)
+{
+    while (true) {
+        i = lds_find(P, i);
+        j = lds_find(P, j);
+        if (i == j) return;
+        if (i > j) {
+            const uint32_t t = i;
+            i = j;
+            j = t;
+        }
+        if (atomicCAS(P + j, (int32_t)j, (int32_t)i) == (int32_t)j) return;
+    }
+}
+
+// Sum of v over the workgroup, returned to every lane (uses red[0..3]).
+__device__ __forceinline__ long long cl_block_sum(long long v, long long* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    return t;
+}
+
+__global__ __launch_bounds__(CL_LDS_THREADS) void cl_lds_run(ClArgs a, SeqRows rows, int64_t n_meas,
+                                                             uint64_t* sweeps, int32_t* e_run,
+                                                             int32_t* mc_run, int32_t* e_out,
+                                                             int32_t* mc_out, int64_t* aux_out)
+{
+    extern __shared__ __align__(16) unsigned char cl_smem[];
+    __shared__ long long red[CL_LDS_THREADS / 64];
+    __shared__ uint32_t s_head, s_tail;
+    __shared__ int s_a, s_b;
+    __shared__ int32_t s_cnt[128];
+    const ClGeo g = a.g;
+    const uint32_t N = g.N;
+    const int axes = cl_axes(g), z = 2 * axes, q = a.q;
+    int32_t* A = (int32_t*)cl_smem;   // parent (SW) or queue (Wolff)
+    int32_t* B = A + N;               // size (SW) or mark (Wolff)
+    int8_t* S = (int8_t*)(B + N);     // spins
+    const int rep = blockIdx.x;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    int8_t* Sg = a.spins + (int64_t)rep * N;
+    for (uint32_t i = tid; i < N; i += bs) S[i] = Sg[i];
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    uint64_t sweep = a.sweeps[rep];
+    long long e = 0;
+    __syncthreads();
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        long long wsize = 0, c2 = 0, ncl = 0;
+        for (int r = 0; r < rows.n; ++r) {
+            if (rows.code[r] == CODE_WOLFF) {
+                for (uint32_t i = tid; i < N; i += bs) B[i] = -1;
+                __syncthreads();
+            }
+            for (int f = 0; f < rows.count[r]; ++f, ++sweep) {
+                if (rows.code[r] == CODE_SWENDSEN_WANG) {
+                    for (uint32_t i = tid; i < N; i += bs) {
+                        A[i] = (int32_t)i;
+                        B[i] = 0;
+                    }
+                    __syncthreads();
+                    const uint32_t nb = (uint32_t)axes * N;
+                    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+                    for (uint32_t blk = tid; 4 * blk < nb; blk += bs) {
+                        const u32x4 w4 = philox4x32_10(u32x4{blk, (uint32_t)sweep, zc, sid}, k0, k1);
+                        const uint32_t w[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const uint32_t b = 4 * blk + t;
+                            if (b >= nb) break;
+                            if ((uint64_t)w[t] >= T) continue;
+                            int ax = 0;
+                            uint32_t i = b;
+                            while (i >= N) {
+                                i -= N;
+                                ++ax;
+                            }
+                            const uint32_t j = cl_nbr_col(g, i, 2 * ax);
+                            if (S[i] != S[j]) continue;
+                            lds_unite(A, i, j);
+                        }
+                    }
+                    __syncthreads();
+                    for (uint32_t i = tid; i < N; i += bs) {
+                        uint32_t x = i;
+                        while (true) {  // read-only find: every entry written here is a root
+                            const uint32_t p = lds_ld(A + x);
+                            if (p == x) break;
+                            x = p;
+                        }
+                        lds_st(A + i, x);
+                        atomicAdd(B + x, 1);
+                    }
+                    __syncthreads();
+                    long long tc2 = 0, tn = 0;
+                    for (uint32_t i = tid; i < N; i += bs) {
+                        const uint32_t root = (uint32_t)A[i];
+                        if (root == i) {
+                            tc2 += (long long)B[i] * B[i];
+                            tn += 1;
+                        }
+                        const uint32_t x = cl_word(k0, k1, sid, sweep, 1, root);
+                        if (q == 0) {
+                            if ((x >> 31) == 0) S[i] = (int8_t)(-S[i]);
+                        } else {
+                            S[i] = (int8_t)(((uint64_t)x * (uint32_t)q) >> 32);
+                        }
+                    }
+                    c2 = cl_block_sum(tc2, red);
+                    ncl = cl_block_sum(tn, red);
+                    __syncthreads();
+                } else {  // one Wolff cluster
+                    if (tid == 0) {
+                        const u32x4 w4 = philox4x32_10(
+                            u32x4{0u, (uint32_t)sweep, ctr_z(sweep, 2, TAG_CLUSTER), sid}, k0, k1);
+                        const uint32_t seed = (uint32_t)(((uint64_t)w4.x * N) >> 32);
+                        const int av = S[seed];
+                        int bv;
+                        if (q == 0) {
+                            bv = -av;
+                        } else {
+                            bv = (int)(((uint64_t)w4.y * (uint32_t)(q - 1)) >> 32);
+                            if (bv >= av) ++bv;
+                        }
+                        S[seed] = (int8_t)bv;
+                        B[seed] = f;
+                        A[0] = (int32_t)seed;
+                        s_a = av;
+                        s_b = bv;
+                        s_head = 0;
+                        s_tail = 1;
+                    }
+                    __syncthreads();
+                    const int sa0 = s_a, sb1 = s_b;
+                    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+                    while (true) {
+                        const uint32_t lo = s_head, hi = s_tail;
+                        // every lane has this level's bounds before any lane appends
+                        __syncthreads();
+                        if (lo == hi) break;
+                        for (uint32_t qi = lo + tid; qi < hi; qi += bs) {
+                            const uint32_t k = (uint32_t)A[qi];
+                            for (int c = 0; c < z; ++c) {
+                                const uint32_t j = cl_nbr_col(g, k, c);
+                                if (S[j] != sa0) continue;
+                                const uint32_t b = (uint32_t)(c >> 1) * N + ((c & 1) ? j : k);
+                                const u32x4 w4 = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+                                const uint32_t wi = b & 3;
+                                const uint32_t u = wi == 0 ? w4.x : (wi == 1 ? w4.y : (wi == 2 ? w4.z : w4.w));
+                                if ((uint64_t)u >= T) continue;
+                                if (atomicExch(B + j, f) == f) continue;
+                                S[j] = (int8_t)sb1;
+                                A[atomicAdd(&s_tail, 1u)] = (int32_t)j;
+                            }
+                        }
+                        __syncthreads();
+                        if (tid == 0) s_head = hi;
+                        __syncthreads();
+                    }
+                    wsize += s_tail;
+                    __syncthreads();
+                }
+            }
+        }
+        // recount the energy and magnetisation / occupation counts
+        if (q)
+            for (int k = tid; k < q; k += bs) s_cnt[k] = 0;
+        __syncthreads();
+        long long te = 0, tm = 0;
+        for (uint32_t i = tid; i < N; i += bs) {
+            const int si = S[i];
+            for (int ax = 0; ax < axes; ++ax) {
+                const int sj = S[cl_nbr_col(g, i, 2 * ax)];
+                te -= q ? (long long)(si == sj) : (long long)(si * sj);
+            }
+            if (q) atomicAdd(s_cnt + si, 1);
+            else tm += si;
+        }
+        e = cl_block_sum(te, red);
+        const long long m = q ? 0 : cl_block_sum(tm, red);
+        __syncthreads();
+        const int64_t o = (int64_t)rep * n_meas + meas;
+        if (tid == 0) {
+            e_out[o] = (int32_t)e;
+            if (!q) mc_out[o] = (int32_t)m;
+            aux_out[4 * o + 0] = 0;
+            aux_out[4 * o + 1] = wsize;
+            aux_out[4 * o + 2] = c2;
+            aux_out[4 * o + 3] = ncl;
+            if (!q) mc_run[rep] = (int32_t)m;
+        }
+        if (q)
+            for (int k = tid; k < q; k += bs) {
+                mc_out[o * q + k] = s_cnt[k];
+                mc_run[(int64_t)rep * q + k] = s_cnt[k];
+            }
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < N; i += bs) Sg[i] = S[i];
+    if (tid == 0) {
+        sweeps[rep] = sweep;
+        e_run[rep] = (int32_t)e;
+    }
+}
+
+// Launch cl_lds_run when the schedule has only cluster rows and the lattice
+// fits; returns false (nothing launched) otherwise.
+inline bool cl_lds_try(const ClArgs& a, const int64_t* schedule, int n_rows, int64_t n_meas,
+                       uint64_t* sweeps, int32_t* e_run, int32_t* mc_run, int32_t* e_out,
+                       int32_t* mc_out, int64_t* aux_out, hipStream_t st, hipError_t* err)
+{
+    if ((int64_t)a.g.N > std::min<int64_t>(g_cluster_lds_max_sites, CL_LDS_SITES)) return false;
+    SeqRows rows{};
+    rows.n = n_rows;
+    for (int r = 0; r < n_rows; ++r) {
+        const int64_t code = schedule[2 * r];
+        if (code != CODE_WOLFF && code != CODE_SWENDSEN_WANG) return false;
+        rows.code[r] = (int32_t)code;
+        rows.count[r] = (int32_t)schedule[2 * r + 1];
+    }
+    const size_t lds = (size_t)a.g.N * 9 + 16;
+    *err = hipSuccess;
+    if (lds > 64 * 1024)
+        *err = hipFuncSetAttribute((const void*)cl_lds_run,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (*err == hipSuccess) {
+        hipLaunchKernelGGL(cl_lds_run, dim3(a.R), dim3(CL_LDS_THREADS), lds, st, a, rows, n_meas,
+                           sweeps, e_run, mc_run, e_out, mc_out, aux_out);
+        *err = hipGetLastError();
+    }
+    return true;
 }
 
 inline uint64_t cl_threshold(double p)
@@ -670,6 +945,12 @@ inline int table_run(int q, void* spins, int64_t N, int32_t z, const int32_t* nb
     ca.g.zt = z;
     ca.R = n_rep;
     ca.q = q;
+    hipError_t lds_err;
+    if (cl_lds_try(ca, schedule, n_rows, n_meas, sweeps, e_run, mc_run, e_out, mc_out, aux_out,
+                   st, &lds_err)) {
+        CM_HIP_CHECK(lds_err);
+        return CM_OK;
+    }
     int64_t* acc = (int64_t*)(ws + lay.acc);
     int64_t* meas_d = (int64_t*)(ws + lay.meas);
     sa.acc = acc;
@@ -767,14 +1048,21 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
         if (rc != CM_OK) return rc;
     }
     int64_t* meas_d = (int64_t*)(ws + lay.meas);
-    hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
-    hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
     int64_t per_meas = 0;
     bool any_local = false;
     for (int row = 0; row < n_rows; ++row) {
         per_meas += schedule[2 * row + 1];
         any_local |= schedule[2 * row] == CODE_METROPOLIS || schedule[2 * row] == CODE_HEATBATH;
     }
+    hipError_t lds_err;
+    if (cl_lds_try(a, schedule, n_rows, n_meas, sweeps, e_run, m_run, e_out, m_out, aux_out, st,
+                   &lds_err)) {
+        CM_HIP_CHECK(lds_err);
+        for (int r = 0; r < n_rep; ++r) sweeps_host[r] += (uint64_t)(per_meas * n_meas);
+        return CM_OK;
+    }
+    hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
+    hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
     // eager: the host counters advance row by row (the plain form reads them)
     auto measurement = [&](hipStream_t s, bool eager) -> int {
         hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, acc, n_rep);
@@ -850,6 +1138,12 @@ int cm_potts_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
     int32_t* row_c = (int32_t*)(ws + lay.rowm);
     const int rb = (n_rep + 255) / 256;
     int64_t* meas_d = (int64_t*)(ws + lay.meas);
+    hipError_t lds_err;
+    if (cl_lds_try(a, schedule, n_rows, n_meas, sweeps, e_run, counts_run, e_out, counts_out,
+                   aux_out, st, &lds_err)) {
+        CM_HIP_CHECK(lds_err);
+        return CM_OK;
+    }
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
     hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
     // the local rows' tables are uploaded once per call
@@ -940,11 +1234,16 @@ int cm_table_measure(const void* spins, int64_t N, int32_t z, const int32_t* nbr
     return CM_OK;
 }
 
-int64_t cm_set_graph_min_meas(int64_t n)
+int cm_set_option(const char* name, int64_t value, int64_t* previous)
 {
-    const int64_t old = g_graph_min_meas;
-    g_graph_min_meas = n;
-    return old;
+    CM_REQUIRE(name != nullptr, "option name missing");
+    int64_t* slot = nullptr;
+    if (std::strcmp(name, "graph_min_meas") == 0) slot = &g_graph_min_meas;
+    else if (std::strcmp(name, "cluster_lds_max_sites") == 0) slot = &g_cluster_lds_max_sites;
+    CM_REQUIRE(slot != nullptr, "unknown option %s", name);
+    if (previous) *previous = *slot;
+    *slot = value;
+    return CM_OK;
 }
 
 }  // extern "C"

@@ -1088,6 +1088,8 @@ __global__ __launch_bounds__(256) void on_wolff(OnClArgs a, int count)
         const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
         while (true) {
             const uint32_t lo = s_head, hi = s_tail;
+            // every lane has this level's bounds before any lane appends to the queue
+            __syncthreads();
             if (lo == hi) break;
             for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
                 const uint32_t k = (uint32_t)Q[q];

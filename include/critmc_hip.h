@@ -345,13 +345,17 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
                 int64_t n_meas, int64_t moves_per_meas, int32_t *idx_out_dev, int64_t *ord_out_dev,
                 void *stream);
 
-/* Schedules with cluster rows run measurement by measurement; from n such
- * measurements in one call on (n <= 0, the default: never), one measurement's
- * launches are captured into a hipGraph and the graph is replayed n times.
- * Applies to cm_ising_cluster_run, cm_potts_cluster_run and the table runs;
- * results are the same either way.  Returns the previous value.  No reference
- * counterpart (the reference loops in compiled code, kernels/ising.py:203-236). */
-int64_t cm_set_graph_min_meas(int64_t n);
+/* Library tuning options (no reference counterpart; results never depend on them).
+ *   "graph_min_meas": schedules with cluster rows that run measurement by
+ *       measurement capture one measurement's launches into a hipGraph and
+ *       replay it when a call has at least this many measurements (<= 0, the
+ *       default: never; measured no faster on MI355X).
+ *   "cluster_lds_max_sites": schedules made only of Wolff / Swendsen-Wang rows
+ *       on lattices of at most this many sites (and at most 8192) run every
+ *       update of the call in one launch, each replica in one workgroup's LDS
+ *       (default 8192; 0: always the multi-launch kernels).
+ * Sets `name` to value; *previous (if not null) receives the old value. */
+int cm_set_option(const char *name, int64_t value, int64_t *previous);
 
 /* ---- Chains on any lattice (neighbour table) ----------------------------- */
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from critmc_amd._lib import lib
+from critmc_amd._lib import set_option
 from critmc_amd import square, triangular
 from critmc_amd.kernels import ising as KI
 from critmc_amd.kernels import potts as KP
@@ -17,12 +17,13 @@ from critmc_amd.kernels.chains import Chains
 pytestmark = pytest.mark.gpu
 
 
-def _with_threshold(n, fn):
-    old = lib().cm_set_graph_min_meas(n)
+def _with_options(fn, **opts):
+    old = {k: set_option(k, v) for k, v in opts.items()}
     try:
         return fn()
     finally:
-        lib().cm_set_graph_min_meas(old)
+        for k, v in old.items():
+            set_option(k, v)
 
 
 def _ising_calls(shape, sched, q=0):
@@ -104,10 +105,36 @@ CASES = {
 
 @pytest.mark.parametrize("case", list(CASES))
 def test_graph_replays_equal_eager_launches(case):
+    """Multi-launch path (no LDS form): graph replays against eager launches."""
     run = CASES[case]
     print(f"graph vs eager: {case}", flush=True)
-    eager = _with_threshold(0, run)
-    graph = _with_threshold(1, run)
+    eager = _with_options(run, graph_min_meas=0, cluster_lds_max_sites=0)
+    graph = _with_options(run, graph_min_meas=1, cluster_lds_max_sites=0)
     for k, (a, b) in enumerate(zip(eager, graph)):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), (case, k)
+
+
+LDS_CASES = {
+    "ising-sw": _ising_calls((16, 16), [[4, 2]]),
+    "ising-wolff-sw": _ising_calls((64, 64), [[3, 5], [4, 1]]),
+    "ising-wolff-3d": _ising_calls((8, 8, 8), [[3, 3]]),
+    "ising-sw-chain": _ising_calls((40,), [[4, 1]]),
+    "potts-sw": _ising_calls((16, 16), [[4, 1]], q=3),
+    "potts-wolff+sw": _ising_calls((24, 16), [[3, 4], [4, 1]], q=10),
+    "table-ising": _table_calls(triangular(9, 7), [[4, 1], [3, 2]], 0),
+    "table-potts": _table_calls(square(7, 5), [[3, 1], [4, 1]], 4),
+}
+
+
+@pytest.mark.parametrize("case", list(LDS_CASES))
+def test_one_launch_lds_form_equals_multi_launch_kernels(case):
+    """Cluster-only schedules: every update of the call in one launch with the replica
+    in LDS, against the multi-launch kernels."""
+    run = LDS_CASES[case]
+    print(f"LDS form vs multi-launch: {case}", flush=True)
+    multi = _with_options(run, cluster_lds_max_sites=0)
+    lds = _with_options(run, cluster_lds_max_sites=8192)
+    for k, (a, b) in enumerate(zip(multi, lds)):
         for x, y in zip(a, b):
             assert np.array_equal(x, y), (case, k)
