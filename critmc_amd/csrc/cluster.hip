@@ -473,18 +473,24 @@ __global__ __launch_bounds__(CL_LDS_THREADS) void cl_lds_run(ClArgs a, SeqRows r
                         atomicAdd(B + x, 1);
                     }
                     __syncthreads();
+                    // each cluster's new state is drawn once, at its root (whose size
+                    // entry is read first), then every site takes it from there
                     long long tc2 = 0, tn = 0;
                     for (uint32_t i = tid; i < N; i += bs) {
-                        const uint32_t root = (uint32_t)A[i];
-                        if (root == i) {
-                            tc2 += (long long)B[i] * B[i];
-                            tn += 1;
-                        }
-                        const uint32_t x = cl_word(k0, k1, sid, sweep, 1, root);
+                        if ((uint32_t)A[i] != i) continue;
+                        tc2 += (long long)B[i] * B[i];
+                        tn += 1;
+                        const uint32_t x = cl_word(k0, k1, sid, sweep, 1, i);
+                        B[i] = q == 0 ? (int32_t)((x >> 31) == 0)
+                                      : (int32_t)(((uint64_t)x * (uint32_t)q) >> 32);
+                    }
+                    __syncthreads();
+                    for (uint32_t i = tid; i < N; i += bs) {
+                        const int32_t d = B[A[i]];
                         if (q == 0) {
-                            if ((x >> 31) == 0) S[i] = (int8_t)(-S[i]);
+                            if (d) S[i] = (int8_t)(-S[i]);
                         } else {
-                            S[i] = (int8_t)(((uint64_t)x * (uint32_t)q) >> 32);
+                            S[i] = (int8_t)d;
                         }
                     }
                     c2 = cl_block_sum(tc2, red);
@@ -604,7 +610,7 @@ inline bool cl_lds_try(const ClArgs& a, const int64_t* schedule, int n_rows, int
     if (lds > 64 * 1024)
         *err = hipFuncSetAttribute((const void*)cl_lds_run,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (*err == hipSuccess) {
+    if (*err == hipSuccess) {  // 256 lanes: 64 or 128 measured slower for SW on 16^2 and 32^2
         hipLaunchKernelGGL(cl_lds_run, dim3(a.R), dim3(CL_LDS_THREADS), lds, st, a, rows, n_meas,
                            sweeps, e_run, mc_run, e_out, mc_out, aux_out);
         *err = hipGetLastError();
