@@ -766,6 +766,9 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
 // kernel dispatch on the device; the host launch cost it removes is hidden.
 int64_t g_graph_min_meas = 0;
 
+// defined with the tempering kernels
+extern int64_t g_ea_fused, g_ea_order;
+
 namespace {
 struct Retired {
     hipGraphExec_t exec;
@@ -1246,6 +1249,8 @@ int cm_set_option(const char* name, int64_t value, int64_t* previous)
     int64_t* slot = nullptr;
     if (std::strcmp(name, "graph_min_meas") == 0) slot = &g_graph_min_meas;
     else if (std::strcmp(name, "cluster_lds_max_sites") == 0) slot = &g_cluster_lds_max_sites;
+    else if (std::strcmp(name, "ea_fused") == 0) slot = &g_ea_fused;
+    else if (std::strcmp(name, "ea_order") == 0) slot = &g_ea_order;
     CM_REQUIRE(slot != nullptr, "unknown option %s", name);
     if (previous) *previous = *slot;
     *slot = value;

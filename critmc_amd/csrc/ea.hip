@@ -43,6 +43,7 @@
 // the start of every bin after the first (critmc/kernels/glass.py:116-122).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "cb_geom.hpp"
@@ -52,6 +53,11 @@
 #include "philox.hpp"
 
 namespace cm {
+
+// Multi-spin batches run as one fused launch per call (1) or three launches
+// per PT step (0); both give the same chains.
+int64_t g_ea_fused = 1;
+int64_t g_ea_order = 0;
 
 constexpr int EA_NOBS = 7;
 constexpr int EA_TAB = 16;
@@ -87,6 +93,7 @@ struct EaArgs {
     uint8_t* jneg;          // [sum_s N_s]: bit j set where the coupling to neighbour j is -1
     const int64_t* ms_off;  // [S] word offsets (the jneg offset is ms_off[s] / 2)
     const int32_t* ms_order;  // [2 S] (sample, set) = (v >> 1, v & 1) per workgroup, largest first
+    const int32_t* samples;   // [S] fused path: the sample of each workgroup
     int S, m, d, code, sweeps_per_step, n_bins;
     int64_t n_series;
 };
@@ -945,6 +952,492 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
     if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + slot] = q;
 }
 
+// ---------------------------------------------------------------------------
+// Fused multi-spin tempering: one workgroup per sample runs every PT step of
+// the call in one launch.  Lanes 0-255 sweep set 0 and lanes 256-511 set 1;
+// both sets' words and the neighbour-coupling bytes stay in LDS for the whole
+// call, and the energy recount, the swap chains, the threshold planes and the
+// measurement run between barriers of the same workgroup, so samples of
+// different sizes never wait for each other.  Every operation that decides a
+// spin, a swap or an integer observable is the one of the three-launch step
+// above, in the same order, so chains and series are the same bit for bit;
+// only the FP64 Fourier sums of a measurement are added in another order
+// (sixteen lanes per temperature).
+//
+// Dynamic LDS: words [2][N], coupling bytes [N] (padded to 16 B), then one
+// region used by the sweep queues [2][3 QCAP] during sweeps and by the
+// measurement (overlap patterns QP[32][rowsP], row sums M1[32][32] and
+// M2[32][16]) afterwards.
+constexpr int EAF_THREADS = 512;
+constexpr int EAF_SET = 256;  // lanes per set
+constexpr int64_t EAF_UNION = 2 * 3 * EA_MS_QCAP * 4;
+
+// Overlap patterns of one configuration are EAF_QP16 (16-bit patterns, rows
+// <= 256) or EAF_QP32 (32-bit patterns, rows <= 32) entries apart: fixed
+// strides, so pattern addresses are immediate offsets, and sixteen lanes of
+// each of four temperatures read 16-bit patterns from disjoint LDS banks
+// (272 / 2 = 136 = 8 mod 32 banks per configuration).
+constexpr uint32_t EAF_QP16 = 272;
+constexpr uint32_t EAF_QP32 = 80;
+
+__host__ __device__ inline int64_t eaf_union_need(bool row16)
+{
+    return 32 * (row16 ? 2 * (int64_t)EAF_QP16 : 4 * (int64_t)EAF_QP32) + 4 * (32 * 32 + 32 * 16);
+}
+
+// P[r][t][cl] bit c = bit 31-t of the threshold of configuration c's
+// temperature for neighbour class cl (0 where that class always accepts),
+// ALW[r][cl] bit c = the class always accepts; lane = (r, t, cl).
+__device__ __forceinline__ void eaf_planes(int tid, int m, const int (*kofc)[32], const uint32_t (*TT)[8],
+                                           const uint32_t* TA, uint32_t (*P)[32][8], uint32_t (*ALW)[8])
+{
+    const int r = tid >> 8, t = (tid >> 3) & 31, cl = tid & 7;
+    uint32_t p = 0, al = 0;
+    for (int c = 0; c < m; ++c) {
+        const int k = kofc[r][c];
+        p |= ((TT[k][cl] >> (31 - t)) & 1u) << c;
+        al |= ((TA[k] >> cl) & 1u) << c;
+    }
+    P[r][t][cl] = p;
+    if (t == 0) ALW[r][cl] = al;
+}
+
+// Row `row` of a set: word x of the row in v[x] (x < L, zero above).
+template <int NW>
+__device__ __forceinline__ void eaf_row_words(const uint32_t* __restrict__ W, uint32_t row, uint32_t L,
+                                              uint32_t (&v)[NW])
+{
+    const uint32_t* src = W + row * L;
+    if ((L & 3u) == 0u) {
+#pragma unroll
+        for (int x = 0; x < NW; x += 4) {
+            uint4 u = (uint32_t)x < L ? *(const uint4*)(src + x) : uint4{0u, 0u, 0u, 0u};
+            v[x] = u.x;
+            v[x + 1] = u.y;
+            v[x + 2] = u.z;
+            v[x + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < NW; x += 2) {
+            uint2 u = (uint32_t)x < L ? *(const uint2*)(src + x) : uint2{0u, 0u};
+            v[x] = u.x;
+            v[x + 1] = u.y;
+        }
+    }
+}
+
+// At most 128 VGPRs, so two workgroups (16 waves) share a CU; what the
+// register allocator then spills lies outside the sweep loops.
+template <int Z, bool ROW16>
+__global__ __launch_bounds__(EAF_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
+{
+    constexpr int D = Z / 2;
+    using QT = typename std::conditional<ROW16, uint16_t, uint32_t>::type;
+    extern __shared__ __align__(16) uint32_t dyn[];
+    __shared__ __align__(16) uint32_t P[2][32][8];
+    __shared__ uint32_t ALW[2][8];
+    __shared__ uint32_t TT[32][8];  // threshold of temperature k, class cl (0 if it always accepts)
+    __shared__ uint32_t TA[32];     // bit cl: temperature k always accepts class cl
+    __shared__ int ucnt[2][32];
+    __shared__ int qcount[2][2];
+    __shared__ int sperm[2][32], skofc[2][32], slab[2][32], sE[2][32], sflag[2][32];
+    __shared__ uint32_t sx[2][32], dmask[2][32];
+    __shared__ __align__(16) uint32_t sigma[8];
+    __shared__ double sbeta[32];
+    __shared__ double ctab[32], stab[32];
+    __shared__ double CT[8][16], ST[8][16];
+    __shared__ double sacc[32][EA_NOBS];
+    __shared__ int shist[2][32][2], sswap[2][32][2];  // label histogram; swap accepts and tries
+    __shared__ int strips[2];
+
+    const int m = a.m;
+    const int s = a.samples[blockIdx.x];
+    const int L = a.L[s];
+    const CbGeo g = ea_geo(D, L);
+    const uint32_t N = (uint32_t)g.N;
+    const uint32_t L32 = (uint32_t)L;
+    const uint32_t rows = N / L32;  // <= 256 (<= 32 for 32-bit patterns): one row per lane
+    constexpr uint32_t rowsP = ROW16 ? EAF_QP16 : EAF_QP32;
+    const uint32_t rowmask = L32 >= 32 ? ~0u : ((1u << L32) - 1u);
+    const int tid = threadIdx.x, r = tid >> 8, tl = tid & (EAF_SET - 1);
+    uint32_t* const W0 = dyn;
+    uint32_t* const W1 = dyn + N;
+    uint32_t* const W = r ? W1 : W0;
+    uint8_t* const JN = (uint8_t*)(dyn + 2 * N);
+    uint32_t* const U = dyn + 2 * N + (N + 15) / 16 * 4;
+    uint32_t* const queue = U + r * 3 * EA_MS_QCAP;
+    QT* const QP = (QT*)U;
+    int* const M1 = (int*)(U + 32 * rowsP * (uint32_t)sizeof(QT) / 4);
+    int* const M2 = M1 + 32 * 32;
+
+    uint32_t* const gw = a.msw + a.ms_off[s];
+    const uint8_t* const gj = a.jneg + a.ms_off[s] / 2;
+    for (uint32_t t = tid; t < 2 * N; t += EAF_THREADS) dyn[t] = gw[t];
+    for (uint32_t t = tid; t < N; t += EAF_THREADS) JN[t] = gj[t];
+    const int64_t cfg0 = ((int64_t)s * 2 + r) * m;
+    if (tl < m) {
+        sperm[r][tl] = a.perm[cfg0 + tl];
+        skofc[r][tl] = a.kofc[cfg0 + tl];
+        slab[r][tl] = a.labels[cfg0 + tl];
+        sE[r][tl] = a.energies[cfg0 + tl];
+        shist[r][tl][0] = shist[r][tl][1] = 0;
+        sswap[r][tl][0] = sswap[r][tl][1] = 0;
+    }
+    if (tl < 32) ucnt[r][tl] = 0;
+    if (tl < 2) qcount[r][tl] = 0;
+    if (tid < 8) sigma[tid] = 0u;
+    if (tl == 0) strips[r] = 0;
+    if (tid < m) {
+        sbeta[tid] = a.betas[(int64_t)s * m + tid];
+        const uint64_t* T = a.thr + ((int64_t)s * m + tid) * EA_TAB;
+        uint32_t al = 0;
+        for (int cl = 0; cl < 8; ++cl) {
+            const uint64_t tv = cl <= Z ? T[2 * cl] : 0;
+            TT[tid][cl] = tv < (1ull << 32) ? (uint32_t)tv : 0u;
+            al |= (tv >= (1ull << 32) ? 1u : 0u) << cl;
+        }
+        TA[tid] = al;
+    }
+    if (tid < L) {
+        double sn, cs;
+        sincospi(2.0 * tid / L, &sn, &cs);
+        ctab[tid] = cs;
+        stab[tid] = sn;
+    }
+    const int64_t acc0 = (int64_t)s * a.n_bins * m * EA_NOBS;
+    for (int e = tid; e < m * EA_NOBS; e += EAF_THREADS) sacc[e / EA_NOBS][e % EA_NOBS] = a.acc[acc0 + e];
+    uint64_t sweep = a.sweeps[cfg0];
+    const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
+    __syncthreads();
+    for (int t = tid; t < 8 * 16; t += EAF_THREADS) {
+        const int j = t >> 4, v = t & 15;
+        double c = 0.0, sn = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            const int x = 4 * j + b;
+            if (((v >> b) & 1) && x < L) {
+                c += ctab[x];
+                sn += stab[x];
+            }
+        }
+        CT[j][v] = c;
+        ST[j][v] = sn;
+    }
+    eaf_planes(tid, m, skofc, TT, TA, P, ALW);
+    __syncthreads();
+
+    const uint32_t lanes = m >= 32 ? ~0u : ((1u << m) - 1u);
+    const uint32_t half = (uint32_t)g.half;
+    const int npair = (m - 1) * m / 2;
+    const FastDiv dl = make_fastdiv(L32);
+    const int mk = tid >> 4, mj = tid & 15;  // measurement lane: temperature, row band
+    int cur_bin = 0;
+    uint32_t hs = 0;
+    for (int64_t t = 0; t < n_steps; ++t) {
+        const uint64_t step = step0 + (uint64_t)t;
+        // lane-derived values are recomputed every step from an opaque copy of
+        // the lane index, so the compiler keeps them out of registers across steps
+        int tid_ = tid;
+        asm volatile("" : "+v"(tid_));
+        const int tid = tid_, r = tid >> 8, tl = tid & (EAF_SET - 1), mk = tid >> 4, mj = tid & 15;
+        uint32_t* const W = r ? W1 : W0;
+        uint32_t* const queue = U + r * 3 * EA_MS_QCAP;
+        // sweeps of both sets
+        uint32_t alw[8];
+#pragma unroll
+        for (int cl = 0; cl < 8; ++cl) alw[cl] = ALW[r][cl];
+        for (int sp = 0; sp < a.sweeps_per_step; ++sp) {
+            for (int col = 0; col < 2; ++col, ++hs) {
+                const uint32_t zc = ctr_z(sweep, (uint32_t)col, TAG_EA);
+                int* qc = &qcount[r][hs & 1];
+                for (uint32_t kk = tl; kk < half; kk += EAF_SET) {
+                    uint32_t i, own, c0, c1, c2;
+                    ms_classify<Z>(g, W, JN, kk, col, a.code, i, own, c0, c1, c2);
+                    uint32_t below = ms_select(alw, c0, c1, c2) & lanes;
+                    uint32_t pending = lanes & ~below;
+                    ms_draw(P[r], c0, c1, c2, pending, below, 0, EA_MS_BUDGET, kk, sweep, zc, r, k0, k1);
+                    if (pending) {
+                        const int slot = atomicAdd(qc, 1);
+                        if (slot < EA_MS_QCAP) {
+                            queue[3 * slot] = kk;
+                            queue[3 * slot + 1] = pending;
+                            queue[3 * slot + 2] = below;
+                            continue;
+                        }
+                        ms_draw(P[r], c0, c1, c2, pending, below, EA_MS_BUDGET, 8, kk, sweep, zc, r, k0, k1);
+                    }
+                    W[i] = a.code == CODE_METROPOLIS ? own ^ below : below;
+                }
+                __syncthreads();
+                const int nq = min(*qc, EA_MS_QCAP);
+                if (tl == 0) qcount[r][(hs + 1) & 1] = 0;
+                for (int e = tl; e < nq; e += EAF_SET) {
+                    const uint32_t kk = queue[3 * e];
+                    uint32_t i, own, c0, c1, c2;
+                    ms_classify<Z>(g, W, JN, kk, col, a.code, i, own, c0, c1, c2);
+                    uint32_t pending = queue[3 * e + 1], below = queue[3 * e + 2];
+                    ms_draw(P[r], c0, c1, c2, pending, below, EA_MS_BUDGET, 8, kk, sweep, zc, r, k0, k1);
+                    W[i] = a.code == CODE_METROPOLIS ? own ^ below : below;
+                }
+                __syncthreads();
+            }
+            ++sweep;
+        }
+        // energies recounted from the bonds (bit-sliced counters, ballots)
+        {
+            uint32_t cnt[EA_MS_CNT];
+#pragma unroll
+            for (int p = 0; p < EA_MS_CNT; ++p) cnt[p] = 0;
+            for (uint32_t i = tl; i < N; i += EAF_SET) {
+                int x[3];
+                ea_coords<D>(i, L32, dl, x);
+                const uint32_t si = W[i], jn = JN[i];
+#pragma unroll
+                for (int ax = 0; ax < D; ++ax) {
+                    const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L32 : L32 * L32);
+                    const uint32_t j = (uint32_t)x[ax] + 1 == L32 ? i - (L32 - 1) * stride : i + stride;
+                    uint32_t carry = si ^ W[j] ^ (0u - ((jn >> (2 * ax)) & 1u));
+                    for (int p = 0; p < EA_MS_CNT && carry; ++p) {
+                        const uint32_t tt = cnt[p] & carry;
+                        cnt[p] ^= carry;
+                        carry = tt;
+                    }
+                }
+            }
+            const uint32_t most = (uint32_t)D * ((N + EAF_SET - 1) / EAF_SET);
+            const int planes = min(EA_MS_CNT, 32 - __clz(most));
+            for (int c = 0; c < m; ++c) {
+                int u = 0;
+                for (int p = 0; p < planes; ++p) u += __popcll(__ballot((cnt[p] >> c) & 1u)) << p;
+                if ((tid & 63) == 0 && u) atomicAdd(&ucnt[r][c], u);
+            }
+        }
+        if (tl < 32) dmask[r][tl] = 0u;
+        if (4 * tl < m - 1) {
+            const uint32_t zpt = ctr_z(step, 0, TAG_PT);
+            const u32x4 rn = philox4x32_10(u32x4{(uint32_t)tl, (uint32_t)step, zpt, (uint32_t)r}, k0, k1);
+            sx[r][4 * tl] = rn.x;
+            sx[r][4 * tl + 1] = rn.y;
+            sx[r][4 * tl + 2] = rn.z;
+            sx[r][4 * tl + 3] = rn.w;
+        }
+        __syncthreads();
+        if (tl < m) {
+            sE[r][tl] = 2 * ucnt[r][tl] - D * (int)N;
+            ucnt[r][tl] = 0;
+        }
+        __syncthreads();
+        // every swap decision (j, k) in parallel, then the chain (as in ea_swap)
+        for (int p = tl; p < npair; p += EAF_SET) {
+            int k = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+            while ((k + 1) * (k + 2) / 2 <= p) ++k;
+            while (k * (k + 1) / 2 > p) --k;
+            const int j = p - k * (k + 1) / 2;
+            const int c1 = sperm[r][j], c2 = sperm[r][k + 1];
+            const double x = (sbeta[k] - sbeta[k + 1]) * ((double)sE[r][c1] - (double)sE[r][c2]);
+            if (x >= 0.0 || (double)sx[r][k] * 0x1p-32 < det_exp(x)) atomicOr(&dmask[r][k], 1u << j);
+        }
+        __syncthreads();
+        if (tl == 0) {
+            int j = 0;
+            for (int k = 0; k < m - 1; ++k) {
+                const int c1 = sperm[r][k], c2 = sperm[r][k + 1];
+                const bool swap = ((dmask[r][k] >> j) & 1u) != 0u;
+                if (swap) {
+                    sperm[r][k] = c2;
+                    sperm[r][k + 1] = c1;
+                    skofc[r][c2] = k;
+                    skofc[r][c1] = k + 1;
+                } else {
+                    j = k + 1;
+                }
+                sflag[r][k] = swap ? 1 : 0;
+            }
+            const int hot = sperm[r][0], cold = sperm[r][m - 1];
+            int trips = 0;
+            if (slab[r][hot] == 2) ++trips;
+            slab[r][hot] = 1;
+            if (slab[r][cold] == 1) ++trips;
+            slab[r][cold] = 2;
+            strips[r] += trips;
+        }
+        __syncthreads();
+        if (tl < m) {
+            if (tl < m - 1) {
+                sswap[r][tl][0] += sflag[r][tl];
+                sswap[r][tl][1] += 1;
+            }
+            const int l = slab[r][sperm[r][tl]];
+            if (l == 1) shist[r][tl][0] += 1;
+            else if (l == 2) shist[r][tl][1] += 1;
+            // byte c of sigma: the set-0 configuration at the temperature of set-1 configuration c
+            if (r == 1) atomicOr(&sigma[tl >> 2], (uint32_t)sperm[0][skofc[1][tl]] << (8 * (tl & 3)));
+        }
+        // thresholds of the new temperatures
+        eaf_planes(tid, m, skofc, TT, TA, P, ALW);
+        for (int e = tid; e < 32 * 32 + 32 * 16; e += EAF_THREADS) M1[e] = 0;
+        __syncthreads();
+        // overlap patterns (as in ea_measure_ms): row pattern of set-0 configuration c
+        // at QP[c], then XORed with the set-1 configuration at the same temperature
+        if ((uint32_t)tid < rows) {
+            const uint32_t row = tid;
+            uint32_t sg[8];
+            *(uint4*)sg = *(const uint4*)sigma;
+            *(uint4*)(sg + 4) = *(const uint4*)(sigma + 4);
+            if constexpr (ROW16) {
+                uint32_t v[16];
+                eaf_row_words<16>(W0, row, L32, v);
+                transpose16x2(v);
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c < m) QP[c * rowsP + row] = (QT)(c < 16 ? v[c] & 0xFFFFu : v[c - 16] >> 16);
+                eaf_row_words<16>(W1, row, L32, v);
+                transpose16x2(v);
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c < m) {
+                        const uint32_t pb = c < 16 ? v[c] & 0xFFFFu : v[c - 16] >> 16;
+                        QT* q = QP + ((sg[c >> 2] >> (8 * (c & 3))) & 0xFFu) * rowsP + row;
+                        *q = (QT)(~((uint32_t)*q ^ pb) & rowmask);
+                        if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                    }
+            } else {
+                uint32_t v[32];
+                eaf_row_words<32>(W0, row, L32, v);
+                transpose32(v);
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c < m) QP[c * rowsP + row] = (QT)v[c];
+                eaf_row_words<32>(W1, row, L32, v);
+                transpose32(v);
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c < m) {
+                        QT* q = QP + ((sg[c >> 2] >> (8 * (c & 3))) & 0xFFu) * rowsP + row;
+                        *q = (QT)(~((uint32_t)*q ^ v[c]) & rowmask);
+                        if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                    }
+            }
+        }
+        __syncthreads();
+        if (tid < 8) sigma[tid] = 0u;
+        // per temperature: q, link overlap and Fourier sums over its rows
+        int q = 0, ql = 0;
+        double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;
+        if (mk < m) {
+            const int nib = (L + 3) >> 2;
+            const QT* qk = QP + sperm[0][mk] * rowsP;
+            uint32_t x1 = (uint32_t)mj, x2 = 0;
+            while (x1 >= L32) {
+                x1 -= L32;
+                ++x2;
+            }
+            for (uint32_t row = mj; row < rows; row += 16) {
+                const uint32_t qp = qk[row];
+                const int rs = 2 * __popc(qp) - L;
+                q += rs;
+                const uint32_t rot = ((qp >> 1) | (qp << (L32 - 1))) & rowmask;
+                ql += 2 * __popc(~(qp ^ rot) & rowmask) - L;
+                const uint32_t ry = (x1 + 1 == L32 ? 0u : x1 + 1) + L32 * x2;
+                ql += 2 * __popc(~(qp ^ (uint32_t)qk[ry]) & rowmask) - L;
+                atomicAdd(&M1[mk * 32 + x1], rs);
+                if (D >= 3) {
+                    const uint32_t rz = x1 + L32 * (x2 + 1 == L32 ? 0u : x2 + 1);
+                    ql += 2 * __popc(~(qp ^ (uint32_t)qk[rz]) & rowmask) - L;
+                    atomicAdd(&M2[mk * 16 + x2], rs);
+                }
+                for (int n = 0; n < nib; n += 2) {
+                    const uint32_t v = (qp >> (4 * n)) & 15u;
+                    re0 += CT[n][v];
+                    im0 += ST[n][v];
+                    if (n + 1 < nib) {
+                        const uint32_t w = (qp >> (4 * n + 4)) & 15u;
+                        re1 += CT[n + 1][w];
+                        im1 += ST[n + 1][w];
+                    }
+                }
+                x1 += 16;
+                while (x1 >= L32) {
+                    x1 -= L32;
+                    ++x2;
+                }
+            }
+        }
+        re0 += re1;
+        im0 += im1;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            q += __shfl_xor(q, off, 64);
+            ql += __shfl_xor(ql, off, 64);
+            re0 += __shfl_xor(re0, off, 64);
+            im0 += __shfl_xor(im0, off, 64);
+        }
+        __syncthreads();
+        if (mj == 0 && mk < m) {
+            const int k = mk;
+            double RE[3] = {2.0 * re0, 0, 0}, IM[3] = {2.0 * im0, 0, 0};
+            for (int v = 0; v < L; ++v) {
+                RE[1] += M1[k * 32 + v] * ctab[v];
+                IM[1] += M1[k * 32 + v] * stab[v];
+                if (D >= 3) {
+                    RE[2] += M2[k * 16 + v] * ctab[v];
+                    IM[2] += M2[k * 16 + v] * stab[v];
+                }
+            }
+            double qk2 = 0.0;
+            for (int ax = 0; ax < D; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
+            qk2 /= D;
+            const double Ea = (double)sE[0][sperm[0][k]];
+            const double Eb = (double)sE[1][sperm[1][k]];
+            const int bin = 63 - __clzll((long long)(t + 1));
+            if (bin != cur_bin) {  // the finished bin goes out, the next one comes in
+                if (cur_bin < a.n_bins)
+                    for (int o = 0; o < EA_NOBS; ++o)
+                        a.acc[acc0 + ((int64_t)cur_bin * m + k) * EA_NOBS + o] = sacc[k][o];
+                if (bin < a.n_bins)
+                    for (int o = 0; o < EA_NOBS; ++o)
+                        sacc[k][o] = a.acc[acc0 + ((int64_t)bin * m + k) * EA_NOBS + o];
+                cur_bin = bin;
+            }
+            if (bin < a.n_bins) {
+                double* o = sacc[k];
+                const double qq = (double)q / (double)N;
+                o[0] += 0.5 * (Ea + Eb) / (double)N;
+                o[1] += (double)ql / (double)(D * N);
+                o[2] += fabs(qq);
+                o[3] += qq * qq;
+                o[4] += qq * qq * qq * qq;
+                o[5] += qk2;
+                o[6] += 1.0;
+            }
+            if (a.e_series) {
+                a.e_series[(((int64_t)s * 2 + 0) * m + k) * a.n_series + t] = (int32_t)Ea;
+                a.e_series[(((int64_t)s * 2 + 1) * m + k) * a.n_series + t] = (int32_t)Eb;
+            }
+            if (a.q_series) a.q_series[((int64_t)s * m + k) * a.n_series + t] = q;
+        }
+        __syncthreads();  // the next sweep's queues reuse the measurement's LDS
+    }
+    for (uint32_t t = tid; t < 2 * N; t += EAF_THREADS) gw[t] = dyn[t];
+    if (tl < m) {
+        a.energies[cfg0 + tl] = sE[r][tl];
+        a.sweeps[cfg0 + tl] = sweep;
+        a.perm[cfg0 + tl] = sperm[r][tl];
+        a.kofc[cfg0 + tl] = skofc[r][tl];
+        a.labels[cfg0 + tl] = slab[r][tl];
+        a.label_hist[(cfg0 + tl) * 2] += shist[r][tl][0];
+        a.label_hist[(cfg0 + tl) * 2 + 1] += shist[r][tl][1];
+        if (tl < m - 1) {
+            a.swap_acc[((int64_t)s * 2 + r) * (m - 1) + tl] += sswap[r][tl][0];
+            a.swap_try[((int64_t)s * 2 + r) * (m - 1) + tl] += sswap[r][tl][1];
+        }
+    }
+    if (tl == 0 && strips[r]) a.trips[s * 2 + r] += strips[r];
+    if (mj == 0 && mk < m && cur_bin < a.n_bins)
+        for (int o = 0; o < EA_NOBS; ++o) a.acc[acc0 + ((int64_t)cur_bin * m + mk) * EA_NOBS + o] = sacc[mk][o];
+}
+
 // E = -sum_b J_b s_i s_j for every configuration (critmc/kernels/glass.py:33-38).
 template <int Z, bool G>
 __global__ __launch_bounds__(256) void ea_energy(EaArgs a)
@@ -1226,18 +1719,50 @@ int cm_ea_pt_run(void* spins, const void* J, int32_t jtype, const int32_t* L_dev
         std::vector<int64_t> hdr(2 * (size_t)n_samples, 0);
         std::copy(ms_off.begin(), ms_off.end(), hdr.begin());
         int32_t* order = (int32_t*)(hdr.data() + n_samples);
-        for (int v = 0; v < 2 * n_samples; ++v) order[v] = v;
-        std::stable_sort(order, order + 2 * n_samples,
-                         [&](int32_t x, int32_t y) { return L_host[x >> 1] > L_host[y >> 1]; });
+        const bool fused = g_ea_fused != 0;
+        if (fused) {
+            // one workgroup per sample, largest first: a CU that finishes a sample
+            // takes the largest one still waiting (longest-first list scheduling).
+            // Order 1: the larger half largest first, then the smaller half
+            // smallest first, pairing workgroups b and b + ceil(S/2).
+            std::vector<int32_t> by(n_samples);
+            for (int v = 0; v < n_samples; ++v) by[v] = v;
+            std::stable_sort(by.begin(), by.end(), [&](int32_t x, int32_t y) { return L_host[x] > L_host[y]; });
+            const int H = (n_samples + 1) / 2;
+            for (int b = 0; b < n_samples; ++b)
+                order[b] = g_ea_order == 1 && b >= H ? by[n_samples - 1 - (b - H)] : by[b];
+        } else {
+            for (int v = 0; v < 2 * n_samples; ++v) order[v] = v;
+            std::stable_sort(order, order + 2 * n_samples,
+                             [&](int32_t x, int32_t y) { return L_host[x >> 1] > L_host[y >> 1]; });
+        }
         int64_t* d_off = (int64_t*)((char*)workspace + o_ms);
         CM_HIP_CHECK(upload_async(d_off, hdr.data(), hdr.size() * 8, st));
         a.ms_off = d_off;
         a.ms_order = (const int32_t*)(d_off + n_samples);
+        a.samples = a.ms_order;
         a.msw = (uint32_t*)((char*)workspace + o_w);
         a.jneg = (uint8_t*)((char*)workspace + o_j);
         const dim3 pgrid((unsigned)std::min<int64_t>((maxN + 255) / 256, 256), (unsigned)n_samples);
         if (ndim == 2) hipLaunchKernelGGL(ea_ms_pack<2>, pgrid, dim3(256), 0, st, a);
         else hipLaunchKernelGGL(ea_ms_pack<3>, pgrid, dim3(256), 0, st, a);
+        if (fused) {
+            const bool row16 = max_L <= 16;
+            const size_t dyn = (size_t)(8 * maxN + (maxN + 15) / 16 * 16 +
+                                        std::max<int64_t>(EAF_UNION, eaf_union_need(row16)));
+            const void* fn = ndim == 2 ? (row16 ? (const void*)ea_ms_fused<4, true> : (const void*)ea_ms_fused<4, false>)
+                                       : (const void*)ea_ms_fused<6, true>;
+            CM_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+            if (ndim == 2 && row16)
+                hipLaunchKernelGGL((ea_ms_fused<4, true>), dim3(n_samples), dim3(EAF_THREADS), dyn, st, a, step0, n_steps);
+            else if (ndim == 2)
+                hipLaunchKernelGGL((ea_ms_fused<4, false>), dim3(n_samples), dim3(EAF_THREADS), dyn, st, a, step0, n_steps);
+            else
+                hipLaunchKernelGGL((ea_ms_fused<6, true>), dim3(n_samples), dim3(EAF_THREADS), dyn, st, a, step0, n_steps);
+            hipLaunchKernelGGL(ea_ms_unpack, pgrid, dim3(256), 0, st, a);
+            CM_HIP_CHECK(hipGetLastError());
+            return CM_OK;
+        }
         const bool ms_lds = maxN * 5 <= 60 * 1024;
         const size_t mlds = ms_lds ? (size_t)maxN * 5 : 0;
         for (int64_t t = 0; t < n_steps; ++t) {

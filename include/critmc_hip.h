@@ -354,6 +354,12 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
  *       on lattices of at most this many sites (and at most 8192) run every
  *       update of the call in one launch, each replica in one workgroup's LDS
  *       (default 8192; 0: always the multi-launch kernels).
+ *   "ea_fused": multi-spin EA tempering (+-J, m <= 32) runs every PT step of
+ *       a call in one launch, one workgroup per sample (1, the default), or
+ *       three launches per step (0).
+ *   "ea_order": dispatch order of the fused launch's samples: largest first
+ *       (0, the default) or the larger half largest first and the smaller
+ *       half smallest first (1).
  * Sets `name` to value; *previous (if not null) receives the old value. */
 int cm_set_option(const char *name, int64_t value, int64_t *previous);
 

@@ -15,12 +15,22 @@ import pytest
 
 import oracle
 from critmc_amd import hypercubic, square
+from critmc_amd._lib import set_option
 from critmc_amd.glass import EdwardsAnderson, ParallelTempering, TemperingBatch, geometric_temperatures
 
 pytestmark = pytest.mark.gpu
 
 
-def _check_against_replay(ndim, Ls, m, code, disorder, n_bins=5, spot=None):
+@pytest.fixture
+def ea_launch_form():
+    """Yields a setter for the multi-spin launch form (1: one fused launch per
+    call, 0: three launches per step); the default is restored afterwards."""
+    old = set_option("ea_fused", 1)
+    yield lambda form: set_option("ea_fused", form)
+    set_option("ea_fused", old)
+
+
+def _check_against_replay(ndim, Ls, m, code, disorder, n_bins=5, spot=None, sweeps_per_step=1):
     """Run a batch on the device and compare the samples in ``spot`` (default: all)
     with the CPU replay from the same start: two calls, the second continuing."""
     models = [EdwardsAnderson(hypercubic((L,) * ndim), disorder=disorder, seed=7 + s)
@@ -28,7 +38,8 @@ def _check_against_replay(ndim, Ls, m, code, disorder, n_bins=5, spot=None):
     fp = disorder == "gaussian"
     temps = geometric_temperatures(0.8, 2.0, m)
     keys = [(1000 + s, 77) for s in range(len(Ls))]
-    batch = TemperingBatch(models, temps, keys=keys, update="heatbath" if code == 2 else "metropolis")
+    batch = TemperingBatch(models, temps, keys=keys, update="heatbath" if code == 2 else "metropolis",
+                           sweeps_per_step=sweeps_per_step)
     runs = batch.run(n_bins, record=True)
     runs2 = batch.run(3, record=True)  # a second call continues the streams
     eng = batch.engine
@@ -38,7 +49,7 @@ def _check_against_replay(ndim, Ls, m, code, disorder, n_bins=5, spot=None):
         N = L ** ndim
         rep = oracle.EaReplay(oracle.ea_init(m, N, keys[s]), models[s].device_couplings(), ndim, L,
                               1.0 / np.sort(temps)[::-1], keys[s], code=code,
-                              multispin=eng.multispin)
+                              sweeps_per_step=sweeps_per_step, multispin=eng.multispin)
         ref = rep.run((1 << n_bins) - 1, n_bins)
         ref2 = rep.run(7, 3)
         assert np.array_equal(eng.configurations(s), rep.spins), f"sample {s}: spins"
@@ -83,6 +94,20 @@ def test_batch_is_bit_exact_against_replay(ndim, Ls, code, disorder):
 ])
 def test_full_ladder_is_bit_exact_against_replay(ndim, Ls, m, code, disorder):
     _check_against_replay(ndim, Ls, m, code, disorder, n_bins=4)
+
+
+@pytest.mark.parametrize("form", [1, 0], ids=["fused", "three-launch"])
+@pytest.mark.parametrize("ndim,Ls,m,code,sps", [
+    (3, [16, 8, 12, 8, 16], 32, 2, 1),
+    (3, [6, 4, 10], 7, 0, 2),            # two sweeps per PT step
+    (2, [20, 32, 8, 18], 32, 2, 1),      # rows of 17-32 sites: 32-bit overlap patterns
+    (2, [12, 16, 4], 9, 0, 3),
+])
+def test_multispin_launch_forms_are_bit_exact(ea_launch_form, form, ndim, Ls, m, code, sps):
+    """The fused launch (every PT step of a call in one workgroup per sample) and
+    the three-launch step give the replay's chains and series bit for bit."""
+    ea_launch_form(form)
+    _check_against_replay(ndim, Ls, m, code, "bimodal", n_bins=4, sweeps_per_step=sps)
 
 
 def _recount(spins, J, L):
