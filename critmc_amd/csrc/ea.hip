@@ -971,6 +971,7 @@ __global__ __launch_bounds__(256) void ea_measure_ms(EaArgs a, int64_t slot)
 constexpr int EAF_THREADS = 512;
 constexpr int EAF_SET = 256;  // lanes per set
 constexpr int64_t EAF_UNION = 2 * 3 * EA_MS_QCAP * 4;
+constexpr int EAF_CNT = 8;  // counter planes of the energy recount: a lane counts at most 3 * 16 bonds
 
 // Overlap patterns of one configuration are EAF_QP16 (16-bit patterns, rows
 // <= 256) or EAF_QP32 (32-bit patterns, rows <= 32) entries apart: fixed
@@ -1042,12 +1043,12 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
     __shared__ uint32_t TA[32];     // bit cl: temperature k always accepts class cl
     __shared__ int ucnt[2][32];
     __shared__ int qcount[2][2];
-    __shared__ int sperm[2][32], skofc[2][32], slab[2][32], sE[2][32], sflag[2][32];
-    __shared__ uint32_t sx[2][32], dmask[2][32];
+    __shared__ int sperm[2][32], skofc[2][32], slab[2][32], sE[2][32];
+    __shared__ uint32_t sx[2][32];
+    __shared__ __align__(16) uint32_t dmask[2][32];
     __shared__ __align__(16) uint32_t sigma[8];
     __shared__ double sbeta[32];
     __shared__ double ctab[32], stab[32];
-    __shared__ double CT[8][16], ST[8][16];
     __shared__ double sacc[32][EA_NOBS];
     __shared__ int shist[2][32][2], sswap[2][32][2];  // label histogram; swap accepts and tries
     __shared__ int strips[2];
@@ -1111,19 +1112,6 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
     uint64_t sweep = a.sweeps[cfg0];
     const uint32_t k0 = a.keys[2 * s], k1 = a.keys[2 * s + 1];
     __syncthreads();
-    for (int t = tid; t < 8 * 16; t += EAF_THREADS) {
-        const int j = t >> 4, v = t & 15;
-        double c = 0.0, sn = 0.0;
-        for (int b = 0; b < 4; ++b) {
-            const int x = 4 * j + b;
-            if (((v >> b) & 1) && x < L) {
-                c += ctab[x];
-                sn += stab[x];
-            }
-        }
-        CT[j][v] = c;
-        ST[j][v] = sn;
-    }
     eaf_planes(tid, m, skofc, TT, TA, P, ALW);
     __syncthreads();
 
@@ -1184,32 +1172,52 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
             }
             ++sweep;
         }
-        // energies recounted from the bonds (bit-sliced counters, ballots)
+        // energies recounted from the bonds: per lane a bit-sliced binary counter
+        // of unsatisfied bonds for every configuration, fed by carry-save adders
+        // (the D bonds of a site first, then the running planes), then ballots
         {
-            uint32_t cnt[EA_MS_CNT];
+            uint32_t cnt[EAF_CNT];
 #pragma unroll
-            for (int p = 0; p < EA_MS_CNT; ++p) cnt[p] = 0;
+            for (int p = 0; p < EAF_CNT; ++p) cnt[p] = 0;
             for (uint32_t i = tl; i < N; i += EAF_SET) {
                 int x[3];
                 ea_coords<D>(i, L32, dl, x);
                 const uint32_t si = W[i], jn = JN[i];
+                uint32_t b[D];
 #pragma unroll
                 for (int ax = 0; ax < D; ++ax) {
                     const uint32_t stride = ax == 0 ? 1u : (ax == 1 ? L32 : L32 * L32);
                     const uint32_t j = (uint32_t)x[ax] + 1 == L32 ? i - (L32 - 1) * stride : i + stride;
-                    uint32_t carry = si ^ W[j] ^ (0u - ((jn >> (2 * ax)) & 1u));
-                    for (int p = 0; p < EA_MS_CNT && carry; ++p) {
-                        const uint32_t tt = cnt[p] & carry;
-                        cnt[p] ^= carry;
-                        carry = tt;
-                    }
+                    b[ax] = si ^ W[j] ^ (0u - ((jn >> (2 * ax)) & 1u));  // J s_i s_j = -1
+                }
+                uint32_t s1, c2;  // the site's count: s1 + 2 c2
+                if constexpr (D == 3) {
+                    const uint32_t x01 = b[0] ^ b[1];
+                    s1 = x01 ^ b[2];
+                    c2 = (b[0] & b[1]) | (x01 & b[2]);
+                } else {
+                    s1 = b[0] ^ b[1];
+                    c2 = b[0] & b[1];
+                }
+                const uint32_t k1 = cnt[0] & s1;  // weight 2
+                cnt[0] ^= s1;
+                const uint32_t t2 = cnt[1] ^ c2;
+                uint32_t carry = (cnt[1] & c2) | (t2 & k1);  // weight 4
+                cnt[1] = t2 ^ k1;
+#pragma unroll
+                for (int p = 2; p < EAF_CNT; ++p) {
+                    const uint32_t tt = cnt[p] & carry;
+                    cnt[p] ^= carry;
+                    carry = tt;
                 }
             }
             const uint32_t most = (uint32_t)D * ((N + EAF_SET - 1) / EAF_SET);
-            const int planes = min(EA_MS_CNT, 32 - __clz(most));
+            const int planes = 32 - __clz(most);
             for (int c = 0; c < m; ++c) {
                 int u = 0;
-                for (int p = 0; p < planes; ++p) u += __popcll(__ballot((cnt[p] >> c) & 1u)) << p;
+#pragma unroll
+                for (int p = 0; p < EAF_CNT; ++p)
+                    if (p < planes) u += __popcll(__ballot((cnt[p] >> c) & 1u)) << p;
                 if ((tid & 63) == 0 && u) atomicAdd(&ucnt[r][c], u);
             }
         }
@@ -1239,22 +1247,37 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
             if (x >= 0.0 || (double)sx[r][k] * 0x1p-32 < det_exp(x)) atomicOr(&dmask[r][k], 1u << j);
         }
         __syncthreads();
-        if (tl == 0) {
+        // The chain (as in ea_swap) in registers: flag k = swap (k, k+1) taken, the
+        // decision of the configuration that started at slot j, carried to k by
+        // the swaps j .. k-1.  A run of taken swaps a .. b moves the configuration
+        // of slot a to b + 1 and those of a+1 .. b+1 down by one, so each slot's
+        // new configuration follows from the flags.
+        uint32_t flags = 0;
+        {
+            uint32_t dm[32];
+#pragma unroll
+            for (int k = 0; k < 32; k += 4) *(uint4*)(dm + k) = *(const uint4*)&dmask[r][k];
             int j = 0;
-            for (int k = 0; k < m - 1; ++k) {
-                const int c1 = sperm[r][k], c2 = sperm[r][k + 1];
-                const bool swap = ((dmask[r][k] >> j) & 1u) != 0u;
-                if (swap) {
-                    sperm[r][k] = c2;
-                    sperm[r][k + 1] = c1;
-                    skofc[r][c2] = k;
-                    skofc[r][c1] = k + 1;
-                } else {
-                    j = k + 1;
+#pragma unroll
+            for (int k = 0; k < 31; ++k) {
+                if (k < m - 1) {
+                    const uint32_t f = (dm[k] >> j) & 1u;
+                    flags |= f << k;
+                    j = f ? j : k + 1;
                 }
-                sflag[r][k] = swap ? 1 : 0;
             }
-            const int hot = sperm[r][0], cold = sperm[r][m - 1];
+        }
+        auto slot_config = [&](int k) {
+            if ((flags >> k) & 1u) return sperm[r][k + 1];
+            if (k >= 1 && ((flags >> (k - 1)) & 1u)) {
+                const uint32_t zeros = ~flags & ((1u << (k - 1)) - 1u);
+                return sperm[r][zeros ? 32 - __clz(zeros) : 0];
+            }
+            return sperm[r][k];
+        };
+        const int newc = tl < m ? slot_config(tl) : 0;
+        if (tl == 0) {
+            const int hot = newc, cold = slot_config(m - 1);
             int trips = 0;
             if (slab[r][hot] == 2) ++trips;
             slab[r][hot] = 1;
@@ -1264,11 +1287,16 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
         }
         __syncthreads();
         if (tl < m) {
+            sperm[r][tl] = newc;
+            skofc[r][newc] = tl;
+        }
+        __syncthreads();
+        if (tl < m) {
             if (tl < m - 1) {
-                sswap[r][tl][0] += sflag[r][tl];
+                sswap[r][tl][0] += (int)((flags >> tl) & 1u);
                 sswap[r][tl][1] += 1;
             }
-            const int l = slab[r][sperm[r][tl]];
+            const int l = slab[r][newc];
             if (l == 1) shist[r][tl][0] += 1;
             else if (l == 2) shist[r][tl][1] += 1;
             // byte c of sigma: the set-0 configuration at the temperature of set-1 configuration c
@@ -1322,12 +1350,17 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
         }
         __syncthreads();
         if (tid < 8) sigma[tid] = 0u;
-        // per temperature: q, link overlap and Fourier sums over its rows
+        // per temperature, sixteen lanes with a band of rows each: q and the link
+        // overlap from popcounts, row sums into M1 (per x1) and M2 (per x2), and
+        // per-column counts of q = +1 in bytes (column 4n + b in byte b of cx[n])
+        constexpr int NIB = ROW16 ? 4 : 8;
         int q = 0, ql = 0;
-        double re0 = 0.0, im0 = 0.0, re1 = 0.0, im1 = 0.0;
+        double re0 = 0.0, im0 = 0.0;
         if (mk < m) {
-            const int nib = (L + 3) >> 2;
             const QT* qk = QP + sperm[0][mk] * rowsP;
+            uint32_t cx[NIB];
+#pragma unroll
+            for (int n = 0; n < NIB; ++n) cx[n] = 0;
             uint32_t x1 = (uint32_t)mj, x2 = 0;
             while (x1 >= L32) {
                 x1 -= L32;
@@ -1347,25 +1380,29 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
                     ql += 2 * __popc(~(qp ^ (uint32_t)qk[rz]) & rowmask) - L;
                     atomicAdd(&M2[mk * 16 + x2], rs);
                 }
-                for (int n = 0; n < nib; n += 2) {
-                    const uint32_t v = (qp >> (4 * n)) & 15u;
-                    re0 += CT[n][v];
-                    im0 += ST[n][v];
-                    if (n + 1 < nib) {
-                        const uint32_t w = (qp >> (4 * n + 4)) & 15u;
-                        re1 += CT[n + 1][w];
-                        im1 += ST[n + 1][w];
-                    }
-                }
+                // bit b of nibble n to byte b: the four shifted copies of the
+                // nibble 7 bits apart never overlap, and bit b of copy b is bit 8 b
+#pragma unroll
+                for (int n = 0; n < NIB; ++n) cx[n] += __umul24((qp >> (4 * n)) & 15u, 0x204081u) & 0x01010101u;
                 x1 += 16;
                 while (x1 >= L32) {
                     x1 -= L32;
                     ++x2;
                 }
             }
+            // this lane's share of sum_x n_x e^(i 2 pi x / L), n_x = sites of column x with q = +1
+#pragma unroll
+            for (int n = 0; n < NIB; ++n)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int x = 4 * n + b;
+                    if (x < L) {
+                        const double c = (double)((cx[n] >> (8 * b)) & 0xFFu);
+                        re0 += c * ctab[x];
+                        im0 += c * stab[x];
+                    }
+                }
         }
-        re0 += re1;
-        im0 += im1;
 #pragma unroll
         for (int off = 1; off < 16; off <<= 1) {
             q += __shfl_xor(q, off, 64);
@@ -1374,17 +1411,29 @@ void ea_ms_fused(EaArgs a, uint64_t step0, int64_t n_steps)
             im0 += __shfl_xor(im0, off, 64);
         }
         __syncthreads();
-        if (mj == 0 && mk < m) {
-            const int k = mk;
-            double RE[3] = {2.0 * re0, 0, 0}, IM[3] = {2.0 * im0, 0, 0};
-            for (int v = 0; v < L; ++v) {
-                RE[1] += M1[k * 32 + v] * ctab[v];
-                IM[1] += M1[k * 32 + v] * stab[v];
+        // y and z Fourier terms from the row sums, lane mj taking x1 / x2 = mj, mj + 16
+        double re1 = 0.0, im1 = 0.0, re2 = 0.0, im2 = 0.0;
+        if (mk < m)
+            for (int v = mj; v < L; v += 16) {
+                re1 += M1[mk * 32 + v] * ctab[v];
+                im1 += M1[mk * 32 + v] * stab[v];
                 if (D >= 3) {
-                    RE[2] += M2[k * 16 + v] * ctab[v];
-                    IM[2] += M2[k * 16 + v] * stab[v];
+                    re2 += M2[mk * 16 + v] * ctab[v];
+                    im2 += M2[mk * 16 + v] * stab[v];
                 }
             }
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            re1 += __shfl_xor(re1, off, 64);
+            im1 += __shfl_xor(im1, off, 64);
+            if (D >= 3) {
+                re2 += __shfl_xor(re2, off, 64);
+                im2 += __shfl_xor(im2, off, 64);
+            }
+        }
+        if (mj == 0 && mk < m) {
+            const int k = mk;
+            const double RE[3] = {2.0 * re0, re1, re2}, IM[3] = {2.0 * im0, im1, im2};
             double qk2 = 0.0;
             for (int ax = 0; ax < D; ++ax) qk2 += (RE[ax] * RE[ax] + IM[ax] * IM[ax]) / ((double)N * (double)N);
             qk2 /= D;
