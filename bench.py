@@ -35,6 +35,10 @@ L, R, SEED = 256, 4096, 1234
 # 4096 x 65536 x 2000 site updates); MI355X: 256 CUs at 2.4 GHz peak clock.
 VALU_PER_SITE_UPDATE = 0.130
 CU_COUNT, CLOCK_HZ = 256, 2.4e9
+# The same for the fused EA tempering launch on its workload (512 samples, 2 x 32
+# temperatures, heat bath + swaps + measurement every sweep): SQ_INSTS_VALU
+# 14,498,224,517 over 500 PT steps of 72,724,480 site updates each.
+EA_VALU_PER_SITE_UPDATE = 0.399
 
 
 def parse():
@@ -192,8 +196,22 @@ def bench_ea(args) -> dict:
                 "L_counts": {int(L): int((Ls == L).sum()) for L in (8, 12, 16)},
                 "temperatures": 32, "sets": 2, "record": "int32 E [S][2][32][steps], q [S][32][steps]"},
             "series_bytes": int(out["E"].numel() * 4 + q.numel() * 4),
-            "roofline": _step_roofline(2.0 * updates, dt, "whole PT step (sweep + swap + measure "
-                                       "launches); 2 B per site update")}
+            "roofline": _step_roofline(2.0 * updates, dt, "whole call: one fused launch runs every PT "
+                                       "step (sweeps, swaps, measurements) of all samples; 2 B per site "
+                                       "update"),
+            "binding": _valu_binding(EA_VALU_PER_SITE_UPDATE, updates / dt,
+                                     "the fused tempering launch (sweeps, energy recount, swaps and "
+                                     "measurement)")}
+
+
+def _valu_binding(per_update: float, updates_per_s: float, what: str) -> dict:
+    """VALU issue rate against a CU's one wave-instruction per cycle."""
+    issue = per_update * updates_per_s / (CU_COUNT * CLOCK_HZ)
+    return {"bound": "valu", "unit": "VALU wave-instructions per CU-cycle", "achieved": round(issue, 3),
+            "peak": 1.0, "frac": round(issue, 3),
+            "basis": f"{per_update} VALU wave-instructions per site update (rocprofv3 SQ_INSTS_VALU "
+                     f"of {what} on this workload) x site updates/s / ({CU_COUNT} CUs x "
+                     f"{CLOCK_HZ / 1e9:g} GHz)"}
 
 
 def _step_roofline(nbytes: float, seconds: float, basis: str) -> dict:
