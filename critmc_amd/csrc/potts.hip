@@ -32,9 +32,10 @@
 
 namespace cm {
 
-struct PottsTab {
-    uint64_t T[8];  // Metropolis: accept iff x < T[delta] (delta >= 1); T = ceil(exp(-beta d) 2^32)
-    double w[16];   // heat bath: exp(beta n) for n <= z; 0 elsewhere (w[15]: padding states)
+struct alignas(16) PottsTab {
+    uint64_t T[8];      // Metropolis: accept iff x < T[delta] (delta >= 1); T = ceil(exp(-beta d) 2^32)
+    double w[16];       // heat bath: exp(beta n) for n <= z; 0 elsewhere (w[15]: padding states)
+    double w2[64][2];   // w2[a + 8 b] = {w[a], w[b]}: two consecutive 3-bit histogram fields at once
 };
 
 constexpr int POTTS_QMAX = 127;
@@ -163,8 +164,139 @@ __device__ __forceinline__ uint64_t potts_pad(int q)
 // Copy one replica's table into LDS.
 __device__ __forceinline__ void stage_tab(PottsTab* __restrict__ dst, const PottsTab* __restrict__ src)
 {
-    if (threadIdx.x < sizeof(PottsTab) / 4)
-        ((uint32_t*)dst)[threadIdx.x] = ((const uint32_t*)src)[threadIdx.x];
+    for (uint32_t t = threadIdx.x; t < sizeof(PottsTab) / 16; t += blockDim.x)
+        ((uint4*)dst)[t] = ((const uint4*)src)[t];
+}
+
+#ifndef POTTS_HB_SITES
+#define POTTS_HB_SITES 2
+#endif
+
+// Heat-bath choices of NS sites with 3-bit histograms h: the weights of fields
+// 2i and 2i+1 come in one 16-byte read, total = w[n_0] + w[n_1] + ... in
+// order k = 0 .. QB-1 keeping every prefix sum, r = u * total, and the new
+// state is the number of k <= QB-2 with !(r < cum_k) (potts_update's rule).
+template <int QB, int NS>
+__device__ __forceinline__ void potts_hb_decide(const double2* __restrict__ w2, const uint32_t* h,
+                                                const uint32_t* x, int* nw)
+{
+    static_assert(QB % 2 == 0, "fields are read in pairs");
+    double cum[NS][QB], total[NS];
+#pragma unroll
+    for (int k = 0; k < QB; k += 2)
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const double2 wp = w2[(h[j] >> (3 * k)) & 63u];
+            total[j] = k == 0 ? wp.x : __dadd_rn(total[j], wp.x);  // 0 + w[n_0]: exact
+            cum[j][k] = total[j];
+            total[j] = __dadd_rn(total[j], wp.y);
+            cum[j][k + 1] = total[j];
+        }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const double rr = __dmul_rn((double)x[j] * 0x1p-32, total[j]);
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < QB - 1; ++k) n += !(rr < cum[j][k]);
+        nw[j] = n;
+    }
+}
+
+// Square lattices with L0 % 8 == 0 and q <= QB <= 10: the quad's four colour
+// sites x0 = x8 + 2j + p (j = 0..3) of row x1 lie in the aligned 8-byte span
+// [x8, x8 + 8) of their row, so the quad reads three 8-byte words (its row and
+// the rows above and below) and the one x-neighbour just outside the span.
+// Its row is realigned into a window whose byte 2j+1 is site j, 2j its -x and
+// 2j+2 its +x neighbour.  Bytes are scaled to 3 s (states <= 9: no carry
+// between bytes), and the histogram has 3-bit fields, h = pad + sum_j 8^{s_j}
+// (fields q .. QB-1 set to 7, whose weight w[7] is 0).  The decisions, and
+// the FP64 sums behind them, are those of potts_update.
+template <int QB>
+__device__ __forceinline__ void potts_quad2d(int8_t* __restrict__ S, const CbGeo& g, uint32_t u,
+                                             int c, int q, uint32_t pad3, int code, uint64_t sweep,
+                                             uint32_t zc, uint32_t k0, uint32_t k1, uint32_t sid,
+                                             const PottsTab* __restrict__ tl, int& de, int& acc,
+                                             int* __restrict__ counts, const double2* __restrict__ w2)
+{
+    static_assert(QB > 0 && QB <= 10, "3-bit histogram fields for at most 10 states");
+    uint32_t xa[4], xb[4];
+    if (code == CODE_METROPOLIS) {
+        const u32x4 r0 = philox4x32_10(u32x4{2 * u, (uint32_t)sweep, zc, sid}, k0, k1);
+        const u32x4 r1 = philox4x32_10(u32x4{2 * u + 1, (uint32_t)sweep, zc, sid}, k0, k1);
+        xa[0] = r0.x; xb[0] = r0.y; xa[1] = r0.z; xb[1] = r0.w;
+        xa[2] = r1.x; xb[2] = r1.y; xa[3] = r1.z; xb[3] = r1.w;
+    } else {
+        const u32x4 r0 = philox4x32_10(u32x4{u, (uint32_t)sweep, zc, sid}, k0, k1);
+        xa[0] = r0.x; xa[1] = r0.y; xa[2] = r0.z; xa[3] = r0.w;
+        xb[0] = xb[1] = xb[2] = xb[3] = 0;
+    }
+    const uint32_t M = (uint32_t)g.M, L0 = (uint32_t)g.L0, L1 = (uint32_t)g.L1;
+    const uint32_t k4 = 4u * u;
+    const uint32_t r = fdiv(k4, g.divM);  // the row, x1
+    const uint32_t x8 = 2u * (k4 - r * M);
+    const uint32_t p = (r + (uint32_t)c) & 1u;
+    const uint32_t row = r * L0;
+    const uint32_t rup = (r + 1 == L1 ? 0u : r + 1) * L0, rdn = (r == 0 ? L1 - 1 : r - 1) * L0;
+    const uint2* const S2 = (const uint2*)S;  // 8-byte words: 32-bit indices off one base
+    const uint2 own = S2[(row + x8) >> 3];
+    const uint2 up = S2[(rup + x8) >> 3];
+    const uint2 dn = S2[(rdn + x8) >> 3];
+    const uint32_t xe = p ? (x8 + 8 == L0 ? 0u : x8 + 8) : (x8 == 0 ? L0 - 1 : x8 - 1);
+    const uint32_t ex = (uint32_t)(uint8_t)S[row + xe];
+    const uint32_t A = p ? own.x : (own.x << 8) | ex;
+    const uint32_t B = p ? own.y : (own.y << 8) | (own.x >> 24);
+    const uint32_t C = p ? ex : own.y >> 24;
+    const uint32_t A3 = A + (A << 1), B3 = B + (B << 1), C3 = C + (C << 1);
+    const uint32_t U3[2] = {up.x + (up.x << 1), up.y + (up.y << 1)};
+    const uint32_t D3[2] = {dn.x + (dn.x << 1), dn.y + (dn.y << 1)};
+    const uint32_t sh = 8u * p;
+    uint32_t old[4], h[4];
+    int c_old[4], nw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t wo = j < 2 ? A : B, wo3 = j < 2 ? A3 : B3;
+        const uint32_t bo = 8u * (uint32_t)((2 * j + 1) & 3);
+        old[j] = (wo >> bo) & 0xFFu;
+        const uint32_t old3 = (wo3 >> bo) & 0xFFu;
+        const uint32_t mx3 = (wo3 >> (bo - 8)) & 0xFFu;
+        const uint32_t px3 = j == 3 ? C3 : ((j == 1 ? B3 : wo3) >> ((bo + 8) & 31)) & 0xFFu;
+        const uint32_t yb = 16u * (uint32_t)(j & 1) + sh;
+        const uint32_t uy3 = __builtin_amdgcn_ubfe(U3[j >> 1], yb, 8);
+        const uint32_t dy3 = __builtin_amdgcn_ubfe(D3[j >> 1], yb, 8);
+        h[j] = pad3 + (1u << px3) + (1u << mx3) + (1u << uy3) + (1u << dy3);
+        c_old[j] = (int)((h[j] >> old3) & 7u);
+    }
+    if (code == CODE_METROPOLIS) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int n = (int)(((uint64_t)xa[j] * (uint32_t)(q - 1)) >> 32);
+            if (n >= (int)old[j]) ++n;
+            const int delta = c_old[j] - (int)((h[j] >> (3 * n)) & 7u);
+            nw[j] = delta <= 0 || (uint64_t)xb[j] < tl->T[delta] ? n : (int)old[j];
+        }
+    } else {
+        // The sites' FP64 chains side by side (POTTS_HB_SITES at a time), so
+        // their dependent additions and table reads overlap.
+#pragma unroll
+        for (int j0 = 0; j0 < 4; j0 += POTTS_HB_SITES)
+            potts_hb_decide<QB, POTTS_HB_SITES>(w2, h + j0, xa + j0, nw + j0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (nw[j] == (int)old[j]) continue;
+        de += c_old[j] - (int)((h[j] >> (3 * nw[j])) & 7u);
+        ++acc;
+        S[row + x8 + 2 * j + p] = (int8_t)nw[j];
+        atomicAdd(&counts[old[j]], -1);
+        atomicAdd(&counts[nw[j]], 1);
+    }
+}
+
+__device__ __forceinline__ uint32_t potts_pad3(int q, int qb)
+{
+    uint32_t pad = 0;
+    for (int k = q; k < qb; ++k) pad |= 7u << (3 * k);
+    return pad;
 }
 
 // The quad of colour-c sites 4u .. 4u+3 (those below `half`): Metropolis draws
@@ -177,6 +309,13 @@ __device__ __forceinline__ void potts_quad(int8_t* __restrict__ S, const CbGeo& 
                                            const PottsTab* __restrict__ tl, int& de, int& acc,
                                            int* __restrict__ counts)
 {
+    if constexpr (Z == 4 && QB > 0 && QB <= 10) {
+        if ((g.L0 & 7) == 0) {
+            potts_quad2d<QB>(S, g, u, c, q, potts_pad3(q, QB), code, sweep, zc, k0, k1, sid, tl, de,
+                             acc, counts, (const double2*)tl->w2);
+            return;
+        }
+    }
     uint32_t xa[4], xb[4];
     if (code == CODE_METROPOLIS) {
         const u32x4 r0 = philox4x32_10(u32x4{2 * u, (uint32_t)sweep, zc, sid}, k0, k1);
@@ -305,7 +444,7 @@ __global__ __launch_bounds__(256) void potts_half_wide(
     const PottsTab* __restrict__ tabs, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
     int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
-    int32_t* __restrict__ acc_out, int32_t* __restrict__ dcounts_out)
+    int32_t* __restrict__ acc_out, int32_t* __restrict__ dcounts_out, int qpl)
 {
     __shared__ PottsTab tl;
     __shared__ int counts[128];
@@ -321,16 +460,21 @@ __global__ __launch_bounds__(256) void potts_half_wide(
     const uint64_t sweep = sweeps[rep] + sweep_add;
     const uint32_t k0 = keys[2 * rep], k1 = keys[2 * rep + 1], sid = sids[rep];
     const uint32_t nquads = (uint32_t)((g.half + 3) >> 2);
-    const uint32_t u = bx * blockDim.x + threadIdx.x;
+    const uint32_t zc = ctr_z(sweep, (uint32_t)colour, TAG_POTTS);
+    const uint64_t pad = potts_pad<QB>(q);
     int de = 0, acc = 0;
-    if (u < nquads)
-        potts_quad<Z, QB>(S, g, u, colour, q, potts_pad<QB>(q), code, sweep,
-                          ctr_z(sweep, (uint32_t)colour, TAG_POTTS), k0, k1, sid, &tl, de, acc,
-                          counts);
-    // one quad per lane: de in [-4Z, 4Z], acc in [0, 4]; bitwise ballots sum them
+    // qpl quads per lane; for each, the wave's lanes take consecutive quads
+    for (int j = 0; j < qpl; ++j) {
+        const uint32_t u = (bx * (uint32_t)qpl + (uint32_t)j) * blockDim.x + threadIdx.x;
+        if (u < nquads)
+            potts_quad<Z, QB>(S, g, u, colour, q, pad, code, sweep, zc, k0, k1, sid, &tl, de, acc,
+                              counts);
+    }
+    // de in [-4Z qpl, 4Z qpl], acc in [0, 4 qpl] (qpl <= 8): bitwise ballots sum them
     const int lanes = __popcll(__ballot(1));
-    de = wave_sum_bits<6>((uint32_t)(de + 4 * Z)) - 4 * Z * lanes;
-    acc = wave_sum_bits<3>((uint32_t)acc);
+    const int off = 4 * Z * qpl;
+    de = wave_sum_bits<10>((uint32_t)(de + off)) - off * lanes;
+    acc = wave_sum_bits<6>((uint32_t)acc);
     if ((threadIdx.x & 63) == 0) {
         red[2 * (threadIdx.x >> 6)] = de;
         red[2 * (threadIdx.x >> 6) + 1] = acc;
@@ -485,12 +629,12 @@ static void launch_wide(const dim3& wgrid, hipStream_t st, int8_t* spins, const 
                         int code, int c, uint64_t sw, const PottsTab* tabs, const uint32_t* keys,
                         const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
                         const int64_t* n_meas_rep, int32_t* e_out, int32_t* acc_out,
-                        int32_t* counts_out)
+                        int32_t* counts_out, int qpl)
 {
 #define CM_POTTS_W(QB) \
     hipLaunchKernelGGL((potts_half_wide<Z, QB>), wgrid, dim3(256), 0, st, spins, g, q, code, c, \
                        sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, \
-                       counts_out)
+                       counts_out, qpl)
     if (q <= 4) CM_POTTS_W(4);
     else if (q <= 8) CM_POTTS_W(8);
     else if (q == 10) CM_POTTS_W(10);
@@ -523,6 +667,11 @@ int potts_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double
         CM_REQUIRE(std::isfinite(betas[r]), "beta must be finite");
         for (int d = 0; d < 8; ++d) tabs[r].T[d] = d <= z ? potts_threshold(std::exp(-betas[r] * d)) : 0;
         for (int d = 0; d < 16; ++d) tabs[r].w[d] = d <= z ? std::exp(betas[r] * d) : 0.0;
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 8; ++b) {
+                tabs[r].w2[a + 8 * b][0] = tabs[r].w[a];
+                tabs[r].w2[a + 8 * b][1] = tabs[r].w[b];
+            }
     }
     CM_HIP_CHECK(upload_async(workspace, tabs.data(), tabs.size() * sizeof(PottsTab),
                               (hipStream_t)stream));
@@ -578,7 +727,9 @@ int potts_rows(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, in
         hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, acc_out, total);
         hipLaunchKernelGGL(potts_zero, dim3(zb), dim3(256), 0, st, counts_out, total * q);
         const int64_t nquads = (g.half + 3) / 4;
-        const dim3 wgrid((unsigned)((nquads + 255) / 256), n_rep);
+        // quads per lane: up to 4 while the launch keeps >= 2048 waves (8 per CU)
+        const int qpl = (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)n_rep * nquads / (64 * 2048)));
+        const dim3 wgrid((unsigned)((nquads + 256 * qpl - 1) / (256 * qpl)), n_rep);
         uint64_t sw = 0;
         int64_t per_meas = 0;
         for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];
@@ -587,11 +738,11 @@ int potts_rows(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, in
                 for (int rp = 0; rp < sc.count[r]; ++rp) {
                     for (int c = 0; c < 2; ++c) {
                         if (z == 2)
-                            launch_wide<2>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<2>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out, qpl);
                         else if (z == 4)
-                            launch_wide<4>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<4>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out, qpl);
                         else
-                            launch_wide<6>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out);
+                            launch_wide<6>(wgrid, st, (int8_t*)spins, g, q, sc.code[r], c, sw, dtabs, keys, stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out, acc_out, counts_out, qpl);
                     }
                     ++sw;
                 }

@@ -20,7 +20,10 @@ def _chains(R, seed, dev):
 
 
 @pytest.mark.parametrize("shape,q", [((4, 4), 3), ((32, 32), 10), ((16, 6), 2), ((10,), 4),
-                                     ((4, 4, 6), 5), ((512, 512), 10), ((6, 4), 127)])
+                                     ((4, 4, 6), 5), ((512, 512), 10), ((6, 4), 127),
+                                     # rows of 8k sites with q <= 10 take the 8-byte-span quads:
+                                     # two rows (+y = -y), one quad per row, q = 7 of 8 fields
+                                     ((8, 2), 8), ((8, 8), 10), ((24, 10), 7)])
 @pytest.mark.parametrize("sched", [[[0, 1]], [[2, 1]], [[0, 1], [2, 2]]])
 @pytest.mark.parametrize("grid", [0, 2, -1], ids=["auto", "per-replica", "wide"])
 def test_potts_run_is_bit_exact_against_replay(shape, q, sched, grid):
