@@ -89,6 +89,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
         [_vp, _c_i64, _c_i32, _vp, _c_i32, _c_i32, _f64p, _vp, _vp, _vp, _i64p, _c_i32, _c_i64,
          _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp], _c_i32),
     "cm_table_measure": ([_vp, _c_i64, _c_i32, _vp, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
+    "cm_on_table_workspace_bytes": ([_c_i64, _c_i32], _c_i64),
+    "cm_on_table_run": (
+        [_vp, _c_i64, _c_i32, _vp, _c_i32, _c_i32, _f64p, _f64p, _vp, _vp, _vp, _i64p, _c_i32,
+         _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp], _c_i32),
     "cm_on_workspace_bytes": ([_c_i32, _i32p, _c_i32, _c_i32], _c_i64),
     "cm_on_init": ([_vp, _c_i32, _i32p, _c_i32, _c_i32, _vp, _vp, _vp], _c_i32),
     "cm_on_convert": ([_vp, _vp, _c_i32, _i32p, _c_i32, _c_i32, _c_i32, _vp], _c_i32),

@@ -69,6 +69,93 @@ int64_t cm_potts_workspace_bytes(int32_t n_rep);
 }
 
 namespace cm {
+
+// `rows` applied n_meas times to every chain, one lane per chain.  With
+// e_out != null each measurement records E (and M or counts, and aux) as the
+// schedule interpreter of the reference does; otherwise accepted moves are
+// added to a.acc.  Each repetition of a row takes one sweep-counter value.
+__global__ __launch_bounds__(64) void seq_run(SeqArgs a, SeqRows rows, int64_t n_meas,
+                                              int32_t* e_out, int32_t* mc_out, int64_t* aux_out)
+{
+    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rep >= a.R) return;
+    int8_t* s = a.spins + (int64_t)rep * a.N;
+    const uint64_t* tab = a.tab + (int64_t)rep * SEQ_TAB;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    int32_t* cnt = a.q ? a.counts + (int64_t)rep * a.q : nullptr;
+    uint64_t sweep = a.sweeps[rep];
+    int e = a.e_run[rep];
+    int m = a.q ? 0 : a.m_run[rep];
+    int64_t total = 0;
+    for (int64_t meas = 0; meas < n_meas; ++meas) {
+        int64_t accepted = 0;
+        for (int r = 0; r < rows.n; ++r)
+            for (int k = 0; k < rows.count[r]; ++k, ++sweep)
+                accepted += a.q ? seq_potts_sweep(a, s, tab, cnt, rows.code[r], sweep, k0, k1, sid, e)
+                                : seq_ising_sweep(a, s, tab, rows.code[r], sweep, k0, k1, sid, e, m);
+        total += accepted;
+        if (e_out) {
+            const int64_t o = (int64_t)rep * n_meas + meas;
+            e_out[o] = e;
+            if (a.q) {
+                for (int c = 0; c < a.q; ++c) mc_out[o * a.q + c] = cnt[c];
+            } else {
+                mc_out[o] = m;
+            }
+            aux_out[4 * o + 0] = accepted;
+            aux_out[4 * o + 1] = 0;
+            aux_out[4 * o + 2] = 0;
+            aux_out[4 * o + 3] = 0;
+        }
+    }
+    a.sweeps[rep] = sweep;
+    a.e_run[rep] = e;
+    if (!a.q) a.m_run[rep] = m;
+    if (a.acc) a.acc[rep] += total;
+}
+
+// Exact E (and M or counts) of every chain from the table: one workgroup per chain.
+__global__ __launch_bounds__(256) void seq_measure(SeqArgs a)
+{
+    __shared__ long long red[2][4];
+    const int rep = blockIdx.x;
+    const int8_t* s = a.spins + (int64_t)rep * a.N;
+    int32_t* cnt = a.q ? a.counts + (int64_t)rep * a.q : nullptr;
+    if (a.q)
+        for (int k = threadIdx.x; k < a.q; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    long long e = 0, m = 0;
+    for (uint32_t i = threadIdx.x; i < a.N; i += blockDim.x) {
+        const int si = s[i];
+        const int32_t* nb = a.nbr + (int64_t)i * a.z;
+        for (int ax = 0; ax < a.z / 2; ++ax) {
+            const int sj = s[nb[2 * ax]];
+            e -= a.q ? (long long)(si == sj) : (long long)(si * sj);
+        }
+        if (a.q) atomicAdd(cnt + si, 1);
+        else m += si;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off, 64);
+        m += __shfl_xor(m, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = e;
+        red[1][threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long te = 0, tm = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            te += red[0][w];
+            tm += red[1][w];
+        }
+        a.e_run[rep] = (int32_t)te;
+        if (!a.q) a.m_run[rep] = (int32_t)tm;
+    }
+}
+
 // potts.hip: the tables of cm_potts_run, and its launches without the upload
 int potts_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
                   void* workspace, int64_t workspace_bytes, void* stream);

@@ -35,5 +35,6 @@ N_AUX = 4
 GPU_UPDATES = {
     "ising": ("metropolis", "metropolis-random", "heatbath", "wolff", "sw", "swendsen-wang"),
     "potts": ("metropolis", "metropolis-random", "heatbath", "wolff", "sw", "swendsen-wang"),
-    "on": ("metropolis", "heatbath", "overrelax", "wolff", "sw", "swendsen-wang"),
+    "on": ("metropolis", "metropolis-random", "heatbath", "overrelax", "wolff", "sw",
+           "swendsen-wang"),
 }

@@ -1,4 +1,5 @@
-"""Chains on any lattice: host side of ``cm_ising_table_run`` / ``cm_potts_table_run``.
+"""Chains on any lattice: host side of ``cm_ising_table_run`` / ``cm_potts_table_run``
+and ``cm_on_table_run``.
 
 For lattices the checkerboard kernels cannot sweep (odd extents, the
 triangular lattice, or a schedule with ``metropolis-random``), the schedule
@@ -122,3 +123,43 @@ def measure(spins: torch.Tensor, nbr: torch.Tensor, R: int, q: int = 0):
     check(lib().cm_table_measure(ptr(spins), N, z, ptr(nbr), R, int(q), ptr(e), ptr(mc), stream()),
           "cm_table_measure")
     return e, (mc[:, 0] if q == 0 else mc)
+
+
+class OnWorkspace:
+    """Device scratch of the O(n) neighbour-table kernel: per-chain β and step,
+    union-find / queue arrays and projections for cluster rows."""
+
+    def __init__(self, n_sites: int, R: int, dev: torch.device | None = None):
+        self.nbytes = int(lib().cm_on_table_workspace_bytes(int(n_sites), int(R)))
+        self.buf = torch.empty(max(self.nbytes, 256), dtype=torch.uint8, device=dev or device())
+
+
+def on_run(spins: torch.Tensor, nbr: torch.Tensor, n: int, betas, steps, chains: Chains, schedule,
+           n_meas: int, *, ws: OnWorkspace | None = None):
+    """Apply ``schedule`` ``n_meas`` times to FP64 O(n) spins ``[R, N, n]`` (site order),
+    one lane per chain, in the reference's own site orders (critmc/kernels/on.py:112-261,
+    348-418).  Returns device FP64 ``E [R, n_meas]``, ``M [R, n_meas, n]``, stiffness sums
+    ``c, s [R, n_meas, z/2]`` and ``aux`` int64 ``[R, n_meas, 4]``."""
+    R = chains.R
+    N, z = nbr.shape
+    if spins.dtype != torch.float64 or tuple(spins.shape) != (R, N, n):
+        raise ValueError(f"spins must be float64 [{R}, {N}, {n}]")
+    betas = np.ascontiguousarray(np.broadcast_to(np.asarray(betas, np.float64), (R,)))
+    steps = np.ascontiguousarray(np.broadcast_to(np.asarray(steps, np.float64), (R,)))
+    sched = _schedule(schedule)
+    dv = spins.device
+    A = z // 2
+    e_out = torch.empty((R, n_meas), dtype=torch.float64, device=dv)
+    m_out = torch.empty((R, n_meas, n), dtype=torch.float64, device=dv)
+    c_out = torch.empty((R, n_meas, A), dtype=torch.float64, device=dv)
+    s_out = torch.empty((R, n_meas, A), dtype=torch.float64, device=dv)
+    aux = torch.empty((R, n_meas, 4), dtype=torch.int64, device=dv)
+    ws = ws or OnWorkspace(N, R, dv)
+    rc = lib().cm_on_table_run(
+        ptr(spins), N, z, ptr(nbr), R, int(n), host_ptr(betas, ctypes.c_double),
+        host_ptr(steps, ctypes.c_double), ptr(chains.keys_d), ptr(chains.sids_d),
+        ptr(chains.sweeps_d), host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas),
+        ptr(e_out), ptr(m_out), ptr(c_out), ptr(s_out), ptr(aux), ptr(ws.buf), ws.nbytes, stream())
+    check(rc, "cm_on_table_run")
+    chains.advance(int(sched[:, 1].sum()) * int(n_meas))
+    return e_out, m_out, c_out, s_out, aux

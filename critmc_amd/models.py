@@ -295,8 +295,12 @@ class Potts(_AnyLattice, Model):
         return {**super().spec(), "q": self.q}
 
 
-class ON(Model):
-    """Classical O(n) model with unit spins (critmc/models.py:225-299); GPU path n = 2, 3."""
+class ON(_AnyLattice, Model):
+    """Classical O(n) model with unit spins (critmc/models.py:225-299); GPU path n = 2, 3.
+
+    On bipartite chain / square / cubic lattices the spins live on the device in
+    FP32, colour-split; elsewhere they are FP64 in site order and every update
+    runs in the reference's own site orders (``cm_on_table_run``)."""
 
     kind = "on"
     updates = GPU_UPDATES["on"]
@@ -329,17 +333,41 @@ class ON(Model):
         return {"step": 1.0}
 
     def to_device(self, state, running):
+        if not checkerboard_ok(self.lattice):
+            s = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float64)).to(device())
+            return DeviceState(s.reshape(1, self.n_sites, self.n))
         s = torch.from_numpy(np.ascontiguousarray(state, dtype=np.float32)).to(device())
         return DeviceState(k_on.to_split(s.reshape(1, self.n_sites, self.n), self.lattice.shape, self.n))
 
     def from_device(self, ds):
+        if not checkerboard_ok(self.lattice):
+            return ds.spins.reshape(self.n_sites, self.n).cpu().numpy().copy()
         sites = k_on.to_sites(ds.spins, self.lattice.shape, self.n)
         return sites.reshape(self.n_sites, self.n).cpu().numpy().astype(np.float64)
 
     def workspace(self):
+        if not checkerboard_ok(self.lattice):
+            return None
         return k_on.Workspace(self.lattice.shape, 1, self.n)
 
+    def _advance_sequential(self, ds, running, beta, chains, schedule, n_meas):
+        dev = ds.spins.device
+        nbr = k_table.neighbours(self.lattice, dev)
+        if "on_table_ws" not in ds.extra:
+            ds.extra["on_table_ws"] = k_table.OnWorkspace(self.n_sites, 1, dev)
+        split = checkerboard_ok(self.lattice)
+        # on a checkerboard lattice (metropolis-random rows), an FP64 site-order copy for the call
+        s64 = (k_on.to_sites(ds.spins, self.lattice.shape, self.n).double() if split else ds.spins)
+        e, m, sc, ss, aux = k_table.on_run(s64, nbr, self.n, [float(beta)], [float(running["step"])],
+                                           chains, schedule, n_meas, ws=ds.extra["on_table_ws"])
+        if split:
+            ds.spins.copy_(k_on.to_split(s64.float(), self.lattice.shape, self.n))
+        return {"E": e[0].cpu().numpy(), "M": m[0].cpu().numpy(), "stiff_c": sc[0].cpu().numpy(),
+                "stiff_s": ss[0].cpu().numpy(), "aux": aux[0].cpu().numpy()}
+
     def advance(self, ds, running, beta, chains, schedule, n_meas, ws):
+        if self._sequential(schedule):
+            return self._advance_sequential(ds, running, beta, chains, schedule, n_meas)
         cws = None
         if np.isin(np.asarray(schedule)[:, 0], CLUSTER_CODES).any():
             if "cluster_ws" not in ds.extra:

@@ -406,6 +406,24 @@ int cm_table_measure(const void *spins_dev, int64_t n_sites, int32_t z, const in
                      int32_t n_rep, int32_t q, int32_t *e_run_dev, int32_t *mc_run_dev,
                      void *stream);
 
+/* O(n) chains (n = 2, 3) on a neighbour-table lattice, one lane per chain, in
+ * FP64: replaces k_on.run (critmc/kernels/on.py:348-418) on lattices without a
+ * checkerboard and for metropolis-random (:112-137), with heat bath
+ * (:186-261), over-relaxation (:140-183) and Wolff / Swendsen-Wang rows
+ * (:264-345).  spins_dev: FP64 [n_rep][n_sites][n] in site order; betas,
+ * steps: host [n_rep].  Per measurement every spin is renormalised and
+ * e_out_dev [n_rep][n_meas], m_out_dev [..][n], c_out_dev / s_out_dev
+ * [..][z/2] (stiffness sums per bond axis) and aux_out_dev [..][4] (accepted,
+ * Wolff sizes, sum |C|^2, clusters) are written.  Uniforms: see seq_on.hip. */
+int64_t cm_on_table_workspace_bytes(int64_t n_sites, int32_t n_rep);
+int cm_on_table_run(void *spins_dev, int64_t n_sites, int32_t z, const int32_t *nbr_dev,
+                    int32_t n_rep, int32_t n, const double *betas, const double *steps,
+                    const uint32_t *keys_dev, const uint32_t *stream_ids_dev,
+                    uint64_t *sweeps_dev, const int64_t *schedule, int32_t n_rows,
+                    int64_t n_meas, double *e_out_dev, double *m_out_dev, double *c_out_dev,
+                    double *s_out_dev, int64_t *aux_out_dev, void *workspace_dev,
+                    int64_t workspace_bytes, void *stream);
+
 /* ---- Multi-histogram reweighting (critmc/reweight.py:159-345) ------------ */
 
 /* Bytes of device scratch for K runs and n_targets target betas. */

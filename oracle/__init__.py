@@ -512,3 +512,43 @@ class TableReplay:
         else:
             out["counts"] = mc_out
         return out
+
+
+_EXTRA_DECLS.update({
+    "orc_seq_on_run": ([_f64p, ctypes.c_int64, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double,
+                        ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p,
+                        _i64p, ctypes.c_int, ctypes.c_int64, _f64p, _f64p, _f64p, _f64p, _i64p],
+                       ctypes.c_int),
+})
+
+
+class SeqOnReplay:
+    """One O(n) chain (n = 2, 3) in FP64 on a lattice given by its neighbour table,
+    in the reference's own site orders (replay_seq_on.c); mutates ``s`` ([N, n])."""
+
+    def __init__(self, s, nbr, n: int, beta: float, step: float, key, stream: int = 0,
+                 sweep: int = 0):
+        self.s = np.ascontiguousarray(s, dtype=np.float64).reshape(-1, int(n)).copy()
+        self.nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        self.N, self.z = self.nbr.shape
+        self.n, self.beta, self.step = int(n), float(beta), float(step)
+        self.key = (int(key[0]), int(key[1]))
+        self.stream = int(stream)
+        self.sweep = np.array([sweep], np.uint64)
+
+    def run(self, schedule, n_meas: int) -> dict[str, np.ndarray]:
+        sched = np.ascontiguousarray(schedule, np.int64).reshape(-1, 2)
+        A = self.z // 2
+        e = np.zeros(n_meas)
+        m = np.zeros((n_meas, self.n))
+        sc = np.zeros((n_meas, A))
+        ss = np.zeros((n_meas, A))
+        aux = np.zeros((n_meas, 4), np.int64)
+        rc = lib().orc_seq_on_run(
+            _ptr(self.s, _f64p), self.N, self.z, _ptr(self.nbr, _i32p), self.n, self.beta,
+            self.step, self.key[0], self.key[1], self.stream, _ptr(self.sweep, _u64p),
+            _ptr(sched, _i64p), sched.shape[0], int(n_meas), _ptr(e, _f64p), _ptr(m, _f64p),
+            _ptr(sc, _f64p), _ptr(ss, _f64p), _ptr(aux, _i64p))
+        if rc:
+            raise ValueError(f"orc_seq_on_run failed ({rc})")
+        return {"E": e, "M": m, "stiff_c": sc, "stiff_s": ss, "aux": aux}

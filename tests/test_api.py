@@ -41,13 +41,16 @@ def test_model_validation():
         for model in (Ising(lat), Potts(lat, 3)):
             model.check_updates(["metropolis", "metropolis-random", "heatbath", "wolff", "sw"])
             assert model.sweep_order([[0, 1]]) == "sequential"
+        for model in (XY(lat), ON(lat, 3)):
+            model.check_updates(["metropolis", "metropolis-random", "heatbath", "overrelax",
+                                 "wolff", "sw"])
+            assert model.sweep_order([[5, 1]]) == "sequential"
     assert Ising(square(4)).sweep_order([[0, 1], [3, 1]]) == "checkerboard"
     assert Ising(square(4)).sweep_order([[1, 1]]) == "sequential"
-    # the O(n) kernels stay checkerboard-only
-    with pytest.raises(ValueError, match="bipartite"):
-        XY(square(5)).check_updates(["metropolis"])
-    with pytest.raises(ValueError, match="checkerboard GPU kernel"):
-        ON(square(4), 3).check_updates(["metropolis-random"])
+    assert XY(square(4)).sweep_order([[0, 1], [5, 2]]) == "checkerboard"
+    assert ON(square(4), 3).sweep_order([[1, 1]]) == "sequential"
+    with pytest.raises(ValueError, match="n = 2 and 3"):
+        ON(square(4), 4)
     with pytest.raises(ValueError, match="zero field"):
         Ising(square(4), h=0.1).check_updates(["wolff"])
     with pytest.raises(ValueError):
