@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--warmup-seconds", type=float, default=1.0,
                     help="minimum warm-up time, in launches of the timed shape")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="headline run only: skip the short runs of the EA, XY and Potts "
+                         "configurations that the default line carries under 'secondary'")
     ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "potts", "serve", "copy"),
                     default="ising2d",
                     help="ising2d: the headline batch; ea / xy / serve: the other configurations; "
@@ -66,6 +69,9 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 70%% of capacity)")
     ap.add_argument("--slice-seconds", type=float, default=0.0,
                     help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
+    ap.add_argument("--no-resident", action="store_true",
+                    help="serve: leave every pool's launch form to the kernels' automatic choice "
+                         "instead of one launch per slice wherever the lattice fits on chip")
     ap.add_argument("--capacity", type=float, default=SERVE_CAPACITY,
                     help="serve: sustained site updates/ns of the job mix, measured with all jobs "
                          "queued at once (--rate 1e9); arrivals default to 70%% of it")
@@ -326,7 +332,8 @@ def bench_serve(args) -> dict:
     if rate <= 0:  # Poisson arrivals at 70 % of the measured capacity
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
-    srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds)
+    srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds,
+                 resident=not args.no_resident)
     for j in jobs:
         srv.submit(j)
     torch.cuda.synchronize()
@@ -342,7 +349,8 @@ def bench_serve(args) -> dict:
             "config": {"workload": "serve_continuous_batching", "jobs": len(done),
                        "arrival_rate_per_s": rate, "offered_site_updates_per_ns":
                        round(rate * work / len(jobs) / 1e9, 2), "budget_GB": 200,
-                       "slice_sweeps": 1000, "slice_seconds": args.slice_seconds},
+                       "slice_sweeps": 1000, "slice_seconds": args.slice_seconds,
+                       "resident_forms": not args.no_resident},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
             "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,
@@ -371,6 +379,27 @@ def bench_copy(args) -> dict:
 
 
 R_DEFAULT = R
+
+
+def secondary_runs(args) -> dict:
+    """Short timed runs of the other full-size configurations, in the same process
+    as the headline: EA tempering (512 samples x 2 x 32 T, 2000 PT steps), XY
+    (256 x 512^2, 100 measurements of metropolis + 2 x overrelax) and q = 10
+    Potts heat bath (128 x 512^2, 2000 sweeps).  Each is the configuration of
+    its own --workload line at fewer steps; the per-step cost is flat in the
+    step count, so the rates compare with those lines."""
+    import torch
+
+    out = {}
+    for name, fn, steps, warm in (("ea", bench_ea, 2000, 20), ("xy", bench_xy, 100, 10),
+                                  ("potts", bench_potts, 2000, 20)):
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.replicas = steps, warm, R_DEFAULT
+        rec = fn(sub)
+        keep = ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config", "roofline")
+        out[name] = {k: rec[k] for k in keep if k in rec}
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -483,6 +512,7 @@ def main():
         plain_rate = L * L * args.plain_sweeps / ((time.perf_counter() - t1) * 1e9)
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    secondary = None if args.no_secondary else secondary_runs(args)
 
     line = {
         "metric": "site updates/ns, 2D Ising L=256 x4096 reps vs 1-chain GPU; p50/p99 job latency",
@@ -523,6 +553,7 @@ def main():
         "wall_s": wall,
         "bookkeeping_consistent": consistent,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(line))
 

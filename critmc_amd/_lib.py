@@ -69,6 +69,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
          _vp, _vp, _vp, _vp, _vp, _c_i64, _c_i32, _c_i32, _vp, _vp],
         _c_i32,
     ),
+    "cm_potts_prepare": ([_c_i32, _i32p, _c_i32, _f64p, _vp, _c_i64, _vp], _c_i32),
     "cm_ea_workspace_bytes": ([_c_i32, _c_i32, _c_i64], _c_i64),
     "cm_ea_multispin": ([_c_i32, _c_i32, _c_i32, _c_i32], _c_i32),
     "cm_det_exp": ([_c_f64], _c_f64),

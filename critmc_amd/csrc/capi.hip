@@ -1,6 +1,8 @@
 // Library-wide C entry points: version, error text, raw Philox draws.
+#include <cstddef>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "philox.hpp"
@@ -23,7 +25,9 @@ namespace {
 // Pinned staging for small host->device uploads.  A ring of slots per device,
 // each guarded by the event recorded after its copy, so an upload on one
 // stream waits only for the copy that last used the same slot, not for work
-// queued on other streams.
+// queued on other streams.  The ring grows (up to kMaxSlots) instead of
+// waiting when its oldest slot's copy is still queued behind other work, so a
+// caller that keeps several streams busy is not held to their pace.
 struct Staging {
     void* host = nullptr;
     size_t cap = 0;
@@ -31,9 +35,41 @@ struct Staging {
     bool pending = false;
 };
 constexpr int kMaxDevices = 64;
-constexpr int kSlots = 16;
-thread_local Staging g_staging[kMaxDevices][kSlots];
-thread_local unsigned g_next[kMaxDevices];
+constexpr size_t kMaxSlots = 256;
+thread_local std::vector<Staging> g_ring[kMaxDevices];
+thread_local size_t g_next[kMaxDevices];  // the oldest slot in use order
+
+// A slot whose previous copy has completed: the oldest if it has, else a new
+// slot placed just before it (it becomes the newest), else the oldest after
+// waiting for its copy.
+hipError_t take_slot(int dev, Staging** out)
+{
+    std::vector<Staging>& ring = g_ring[dev];
+    size_t& next = g_next[dev];
+    if (!ring.empty()) {
+        Staging& old = ring[next % ring.size()];
+        hipError_t q = hipSuccess;
+        if (old.pending) {
+            q = hipEventQuery(old.done);
+            if (q != hipSuccess && q != hipErrorNotReady) return q;
+        }
+        if (!old.pending || q == hipSuccess || ring.size() >= kMaxSlots) {
+            if (old.pending && q != hipSuccess) {
+                const hipError_t e = hipEventSynchronize(old.done);
+                if (e != hipSuccess) return e;
+            }
+            old.pending = false;
+            *out = &old;
+            next = (next % ring.size()) + 1;
+            return hipSuccess;
+        }
+    }
+    const size_t pos = ring.empty() ? 0 : next % ring.size();
+    ring.insert(ring.begin() + (std::ptrdiff_t)pos, Staging{});
+    next = pos + 1;
+    *out = &ring[pos];
+    return hipSuccess;
+}
 }  // namespace
 
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t stream)
@@ -42,15 +78,13 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t st
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
-    Staging& st = g_staging[dev][g_next[dev]++ % kSlots];
+    Staging* slot = nullptr;
+    e = take_slot(dev, &slot);
+    if (e != hipSuccess) return e;
+    Staging& st = *slot;
     if (st.done == nullptr) {
         e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
         if (e != hipSuccess) return e;
-    }
-    if (st.pending) {
-        e = hipEventSynchronize(st.done);
-        if (e != hipSuccess) return e;
-        st.pending = false;
     }
     if (st.cap < bytes) {  // grown in powers of two from 64 KiB, kept for the process
         if (st.host) (void)hipHostFree(st.host);

@@ -716,19 +716,27 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
 // Local updates only (Metropolis, heat bath, over-relaxation); as there, the
 // last half sweep of a measurement stores its sites renormalised.  HB: the
 // schedule has heat-bath rows (a separate build, as the heat bath's registers
-// would otherwise limit every schedule's occupancy).
-constexpr int64_t ON_BATCHED_LDS = 60 * 1024;
+// would otherwise limit every schedule's occupancy).  BLK lanes per workgroup:
+// 256, or 1024 for replicas large enough to give each lane several sites
+// (local updates without heat bath); the measurement then runs in groups of
+// 256 lanes, group j taking chunk j of every NG = BLK / 256 consecutive
+// chunks exactly as one on_measure_c0 workgroup takes it, and the chunk
+// partials are added in chunk order, so the bits do not depend on BLK.
+// Up to 158 KB of spins (XY 128^2 at 128 KB): one workgroup per CU.
+constexpr int64_t ON_BATCHED_LDS = 158 * 1024;
 
-template <int NC, int D, bool HB>
-__global__ __launch_bounds__(256) void on_batched(OnArgs a, Sched sc, double* __restrict__ e_out,
+template <int NC, int D, bool HB, int BLK>
+__global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __restrict__ e_out,
                                                   double* __restrict__ m_out,
                                                   double* __restrict__ sc_out,
                                                   double* __restrict__ ss_out)
 {
     constexpr int Z = 2 * D, K = 1 + NC + 2 * D;
+    constexpr int NG = BLK / 256;  // measurement groups of 256 lanes
+    static_assert(BLK % 256 == 0, "the measurement takes chunks in groups of 256 lanes");
     extern __shared__ float ls[];  // [2 colours][half][NC]
-    __shared__ double red[K * 4];
-    __shared__ int ired[4];
+    __shared__ double red[K * NG * 4];
+    __shared__ int ired[BLK / 64];
     const CbGeo& g = a.g;
     const int64_t half = g.half;
     const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
@@ -783,12 +791,13 @@ __global__ __launch_bounds__(256) void on_batched(OnArgs a, Sched sc, double* __
             // the measurement: colour 0 renormalised in place, bonds summed per chunk
             double total = 0.0;  // thread q < K: quantity q, chunk partials added in order
             const int nchunk = (int)((half + ON_CHUNK - 1) / ON_CHUNK);
-            for (int ch = 0; ch < nchunk; ++ch) {
+            const int grp = (int)(threadIdx.x >> 8), gt = (int)(threadIdx.x & 255), gw = (gt >> 6);
+            for (int cb = 0; cb < nchunk; cb += NG) {
                 double v[K];
 #pragma unroll
                 for (int q = 0; q < K; ++q) v[q] = 0.0;
-                const int64_t c0 = (int64_t)ch * ON_CHUNK, c1 = min(half, c0 + ON_CHUNK);
-                for (int64_t k = c0 + threadIdx.x; k < c1; k += blockDim.x) {
+                const int64_t c0 = (int64_t)(cb + grp) * ON_CHUNK, c1 = min(half, c0 + ON_CHUNK);
+                for (int64_t k = c0 + gt; k < c1; k += 256) {
                     uint32_t nbo[Z];
                     cb_other<Z>(g, (uint32_t)k, 0, nbo);
                     const Vec<NC> s = unit_vec<NC>(load_vec<NC>(L0, k));
@@ -798,13 +807,15 @@ __global__ __launch_bounds__(256) void on_batched(OnArgs a, Sched sc, double* __
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
                     const double t = wave_sum_f64(v[q]);
-                    if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
+                    if ((threadIdx.x & 63) == 0) red[(q * NG + grp) * 4 + gw] = t;
                 }
                 __syncthreads();
                 if (threadIdx.x < K) {
-                    double t = 0.0;
-                    for (int u = 0; u < nw; ++u) t += red[threadIdx.x * 4 + u];
-                    total += t;
+                    for (int j = 0; j < NG && cb + j < nchunk; ++j) {
+                        double t = 0.0;
+                        for (int u = 0; u < 4; ++u) t += red[((int)threadIdx.x * NG + j) * 4 + u];
+                        total += t;
+                    }
                 }
                 __syncthreads();
             }
@@ -1311,20 +1322,46 @@ static void launch_measure_c0(float* spins, const CbGeo& g, int R, double* part,
     }
 }
 
+template <int NC, int D, bool HB, int BLK>
+static hipError_t launch_batched_one(dim3 grid, size_t lds, hipStream_t st, const OnArgs& a,
+                                     const Sched& sc, double* e_out, double* m_out, double* sc_out,
+                                     double* ss_out)
+{
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void*)on_batched<NC, D, HB, BLK>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((on_batched<NC, D, HB, BLK>), grid, dim3(BLK), lds, st, a, sc, e_out, m_out,
+                       sc_out, ss_out);
+    return hipGetLastError();
+}
+
+// 1024 lanes once a colour has 4096 sites (four or more per lane), local
+// updates without heat bath (whose registers need the 256-lane build).
+template <int NC, int D, bool HB>
+static hipError_t launch_batched_blk(dim3 grid, size_t lds, hipStream_t st, const OnArgs& a,
+                                     const Sched& sc, double* e_out, double* m_out, double* sc_out,
+                                     double* ss_out)
+{
+    if (!HB && a.g.half >= 4096)
+        return launch_batched_one<NC, D, HB, 1024>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+    return launch_batched_one<NC, D, HB, 256>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+}
+
 template <bool HB>
-static void launch_batched_on(int n, int d, dim3 grid, size_t lds, hipStream_t st, const OnArgs& a,
-                              const Sched& sc, double* e_out, double* m_out, double* sc_out,
-                              double* ss_out)
+static hipError_t launch_batched_on(int n, int d, dim3 grid, size_t lds, hipStream_t st,
+                                    const OnArgs& a, const Sched& sc, double* e_out, double* m_out,
+                                    double* sc_out, double* ss_out)
 {
     if (n == 2) {
-        if (d == 1) hipLaunchKernelGGL((on_batched<2, 1, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-        else if (d == 2) hipLaunchKernelGGL((on_batched<2, 2, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-        else hipLaunchKernelGGL((on_batched<2, 3, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-    } else {
-        if (d == 1) hipLaunchKernelGGL((on_batched<3, 1, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-        else if (d == 2) hipLaunchKernelGGL((on_batched<3, 2, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-        else hipLaunchKernelGGL((on_batched<3, 3, HB>), grid, dim3(256), lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        if (d == 1) return launch_batched_blk<2, 1, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        if (d == 2) return launch_batched_blk<2, 2, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        return launch_batched_blk<2, 3, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
     }
+    if (d == 1) return launch_batched_blk<3, 1, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+    if (d == 2) return launch_batched_blk<3, 2, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+    return launch_batched_blk<3, 3, HB>(grid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
 }
 
 }  // namespace cm
@@ -1449,8 +1486,8 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         const dim3 bgrid((unsigned)std::min(n_rep, 65535));
         bool hb = false;
         for (int r = 0; r < n_rows; ++r) hb |= schedule[2 * r] == CODE_HEATBATH;
-        if (hb) launch_batched_on<true>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
-        else launch_batched_on<false>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out);
+        if (hb) CM_HIP_CHECK(launch_batched_on<true>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out));
+        else CM_HIP_CHECK(launch_batched_on<false>(n, g.d, bgrid, lds, st, a, sc, e_out, m_out, sc_out, ss_out));
         int64_t per_meas = 0;
         for (int r = 0; r < n_rows; ++r) per_meas += schedule[2 * r + 1];
         hipLaunchKernelGGL(on_advance_sweeps, dim3((n_rep + 255) / 256), dim3(256), 0, st, sweeps,

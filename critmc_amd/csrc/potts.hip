@@ -821,11 +821,19 @@ int cm_potts_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, 
                             workspace_bytes, block, grid, n_meas_rep, stream))
         return rc;  // arguments checked before anything is uploaded
     if (n_meas == 0) return CM_OK;
-    if (int rc = potts_prepare(ndim, dims, n_rep, betas, workspace, workspace_bytes, stream))
-        return rc;
+    if (betas != nullptr)  // null: cm_potts_prepare already put these chains' tables in the workspace
+        if (int rc = potts_prepare(ndim, dims, n_rep, betas, workspace, workspace_bytes, stream))
+            return rc;
     return potts_rows(spins, ndim, dims, n_rep, q, keys, stream_ids, sweeps, schedule, n_rows,
                       n_meas, e_run, counts_run, e_out, acc_out, counts_out, workspace,
                       workspace_bytes, block, grid, n_meas_rep, stream);
+}
+
+int cm_potts_prepare(int32_t ndim, const int32_t* dims, int32_t n_rep, const double* betas,
+                     void* workspace, int64_t workspace_bytes, void* stream)
+{
+    CM_REQUIRE(betas != nullptr, "betas must not be null");
+    return potts_prepare(ndim, dims, n_rep, betas, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

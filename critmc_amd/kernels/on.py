@@ -64,10 +64,22 @@ def init_spins(spins: torch.Tensor, shape, n: int, chains: Chains) -> None:
                            ptr(chains.keys_d), ptr(chains.sids_d), stream()), "cm_on_init")
 
 
+def _per_chain_f32(x, R: int, dv: torch.device) -> torch.Tensor:
+    """A per-chain FP32 device vector: a device tensor as given, or host values
+    copied from pinned memory without waiting for the stream."""
+    if isinstance(x, torch.Tensor) and x.device == dv:
+        if x.dtype != torch.float32 or x.numel() != R or not x.is_contiguous():
+            raise ValueError("per-chain device values must be contiguous float32 of length R")
+        return x
+    h = torch.from_numpy(np.broadcast_to(np.asarray(x, np.float32), (R,)).copy())
+    return h.pin_memory().to(dv, non_blocking=True)
+
+
 def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedule, n_meas: int,
         *, ws: Workspace | None = None, n_meas_rep: torch.Tensor | None = None,
         cluster_ws: ClusterWorkspace | None = None, with_aux: bool = False):
-    """Returns device tensors (accepted int32 [R, n_meas], E f64 [R, n_meas],
+    """``betas`` and ``steps``: per-chain values (host, or float32 device tensors of
+    length R).  Returns device tensors (accepted int32 [R, n_meas], E f64 [R, n_meas],
     M f64 [R, n_meas, n], stiff_c / stiff_s f64 [R, n_meas, d]), plus ``aux``
     int64 [R, n_meas, 4] (accepted, Wolff sizes, sum |C|^2, clusters) when
     ``with_aux``."""
@@ -80,8 +92,8 @@ def run(spins: torch.Tensor, shape, n: int, betas, steps, chains: Chains, schedu
         cluster_ws = ClusterWorkspace(shape, R, spins.device)
     dv = spins.device
     d = _dims(shape)
-    b = torch.as_tensor(np.broadcast_to(np.asarray(betas, np.float32), (R,)).copy(), device=dv)
-    st = torch.as_tensor(np.broadcast_to(np.asarray(steps, np.float32), (R,)).copy(), device=dv)
+    b = _per_chain_f32(betas, R, dv)
+    st = _per_chain_f32(steps, R, dv)
     acc = torch.empty((R, n_meas), dtype=torch.int32, device=dv)
     e = torch.empty((R, n_meas), dtype=torch.float64, device=dv)
     m = torch.empty((R, n_meas, n), dtype=torch.float64, device=dv)

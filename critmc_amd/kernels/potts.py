@@ -24,9 +24,25 @@ def _dims(shape) -> np.ndarray:
 
 
 class Workspace:
+    """Device scratch owned by the caller: the chains' tables (thresholds and
+    heat-bath weights), uploaded by ``cm_potts_prepare`` when the betas, the
+    lattice dimension or the stream differ from the previous call that used
+    this workspace, and otherwise reused with no host work."""
+
     def __init__(self, R: int, dev: torch.device | None = None):
         self.nbytes = int(lib().cm_potts_workspace_bytes(R))
         self.buf = torch.empty(max(self.nbytes, 256), dtype=torch.uint8, device=dev or device())
+        self._key = None
+
+    def prepare(self, dims: np.ndarray, betas: np.ndarray) -> None:
+        st = stream()
+        key = (len(dims), betas.tobytes(), st.value)
+        if key != self._key:
+            self._key = None
+            check(lib().cm_potts_prepare(len(dims), host_ptr(dims, ctypes.c_int32), len(betas),
+                                         host_ptr(betas, ctypes.c_double), ptr(self.buf), self.nbytes,
+                                         st), "cm_potts_prepare")
+            self._key = key
 
 
 def init_states(spins: torch.Tensor, shape, q: int, chains: Chains, start: str = "random") -> None:
@@ -63,10 +79,16 @@ def run(spins: torch.Tensor, shape, q: int, betas, chains: Chains, schedule, n_m
     e_out = torch.empty((R, n_meas), dtype=torch.int32, device=dev)
     acc = torch.empty((R, n_meas), dtype=torch.int32, device=dev)
     counts = torch.empty((R, n_meas, q), dtype=torch.int32, device=dev)
-    ws = ws or Workspace(R, dev)
+    if ws is None:  # one-shot scratch: the run uploads the tables itself
+        ws = Workspace(R, dev)
+        tabs = host_ptr(betas, ctypes.c_double)
+    else:  # a caller's workspace keeps its tables across calls with the same betas
+        if n_meas > 0:
+            ws.prepare(d, betas)
+        tabs = None
     rc = lib().cm_potts_run(
         ptr(spins), len(d), host_ptr(d, ctypes.c_int32), R, int(q),
-        host_ptr(betas, ctypes.c_double), ptr(chains.keys_d), ptr(chains.sids_d),
+        tabs, ptr(chains.keys_d), ptr(chains.sids_d),
         ptr(chains.sweeps_d), host_ptr(sched, ctypes.c_int64), sched.shape[0], int(n_meas),
         ptr(e_run), ptr(counts_run), ptr(e_out), ptr(acc), ptr(counts), ptr(ws.buf), ws.nbytes,
         block, grid, ptr(n_meas_rep), stream())

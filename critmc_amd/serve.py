@@ -29,7 +29,6 @@ from .device import device
 from .kernels import ising as k_ising
 from .kernels import on as k_on
 from .kernels import potts as k_potts
-from .kernels.chains import Chains
 from .rng import chain_key, make_rng
 
 MODELS = ("ising", "potts", "xy")
@@ -76,12 +75,28 @@ class Job:
         return {k: v.cpu().numpy() for k, v in self.series.items()}
 
 
+# Largest replica, in states, that the per-replica Potts form keeps in LDS.
+POTTS_LDS_STATES = 96 * 1024
+
+
+def _ising_resident_ok(L: int) -> bool:
+    """Whether an L x L Ising replica fits the batched form (its packed words in LDS)."""
+    M = L // 2
+    return (M <= 32 or M % 32 == 0) and 2 * L * (-(-M // 32)) * 4 <= 96 * 1024
+
+
 class _Pool:
     """Resident state of all slots of one (model, L) group."""
 
-    def __init__(self, model: str, L: int, q: int, dev, stream):
+    def __init__(self, model: str, L: int, q: int, dev, stream, resident: bool = True):
         self.model, self.L, self.q = model, L, q
         self.dev, self.stream = dev, stream
+        # One launch per slice (each replica a workgroup holding its lattice on
+        # chip for every sweep of the slice) wherever the lattice fits, instead
+        # of the launch per half sweep the automatic choice makes for few
+        # replicas of a large lattice: a serving stream is many small pools
+        # sharing the GPU, so launches per slice, not CUs per pool, set the pace.
+        self.resident = resident
         self.cap = 0
         self.jobs: list[Job | None] = []
         self.free: list[int] = []
@@ -124,9 +139,26 @@ class _Pool:
         self.state = new
         self.free.extend(range(self.cap, cap))
         self.jobs.extend([None] * (cap - self.cap))
+        old = self.cap
         self.cap = cap
         self.betas = np.resize(getattr(self, "betas", np.zeros(0)), cap)
         self.steps = np.resize(getattr(self, "steps", np.zeros(0)), cap)
+        # Host values reach the device from pinned buffers with copies that do
+        # not wait for the stream (a blocking copy would wait for every slice
+        # queued on the stream, including other pools').  A buffer is written
+        # again only after the slice that followed its copy has completed.
+        keys_h = torch.zeros((cap, 2), dtype=torch.int32).pin_memory()
+        if old:
+            keys_h[:old].copy_(self.keys_h)
+        self.keys_h = keys_h
+        self.caps_h = torch.zeros(cap, dtype=torch.int64).pin_memory()
+        self.caps_d = torch.zeros(cap, dtype=torch.int64, device=self.dev)
+        if self.model == "xy":
+            self.onpar_h = torch.zeros((2, cap), dtype=torch.float32).pin_memory()
+            self.onpar_d = torch.zeros((2, cap), dtype=torch.float32, device=self.dev)
+            self.onpar_h[0].copy_(torch.from_numpy(self.betas.astype(np.float32)))
+            self.onpar_h[1].copy_(torch.from_numpy(self.steps.astype(np.float32)))
+            self.onpar_dirty = True
         self.chains = None
         # scratch kept for the pool's life (the Ising tables are rebuilt only
         # when a slot's beta changes)
@@ -150,12 +182,18 @@ class _Pool:
         self.jobs[slot] = job
         self.betas[slot] = job.beta
         self.steps[slot] = job.step
-        key = np.array([chain_key(make_rng(job.seed))], dtype=np.uint64)
-        one = Chains(key, [0], [0], self.dev)
+        if self.model == "xy":
+            self.onpar_h[0, slot] = float(np.float32(job.beta))
+            self.onpar_h[1, slot] = float(np.float32(job.step))
+            self.onpar_dirty = True
+        key = np.array(chain_key(make_rng(job.seed)), dtype=np.uint64).astype(np.uint32)
+        self.keys_h[slot] = torch.from_numpy(key.view(np.int32))
         st = self.state
-        st["keys"][slot] = one.keys_d[0]
+        st["keys"][slot].copy_(self.keys_h[slot], non_blocking=True)
         st["sids"][slot] = 0
         st["sweeps"][slot] = 0
+        one = _DeviceChains(st["keys"][slot: slot + 1], st["sids"][slot: slot + 1],
+                            st["sweeps"][slot: slot + 1], 1)
         shape = (self.L, self.L)
         if self.model == "ising":
             s = st["spins"][slot: slot + 1]
@@ -205,22 +243,31 @@ class _Pool:
             return None
         st = self.state
         with torch.cuda.stream(self.stream):
-            caps_d = torch.from_numpy(caps).to(self.dev, non_blocking=False)
+            self.caps_h.numpy()[:] = caps
+            caps_d = self.caps_d
+            caps_d.copy_(self.caps_h, non_blocking=True)
+            if self.model == "xy" and self.onpar_dirty:
+                self.onpar_d.copy_(self.onpar_h, non_blocking=True)
+                self.onpar_dirty = False
             self.start_event.record(self.stream)
             ch = _DeviceChains(st["keys"], st["sids"], st["sweeps"], self.cap)
             sched = np.array(SCHEDULE[self.model])
             shape = (self.L, self.L)
             if self.model == "ising":
+                form = (k_ising.FORM_BATCHED if self.resident and _ising_resident_ok(self.L)
+                        else k_ising.FORM_AUTO)
                 out = k_ising.run(st["spins"], shape, self.betas, 0.0, ch, sched, n, st["e"], st["m"],
-                                  ws=self.ws, n_meas_rep=caps_d)
+                                  form=form, ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": out[0], "M": out[1]}
             elif self.model == "potts":
+                grid = self.cap if self.resident and self.L * self.L <= POTTS_LDS_STATES else 0
                 e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
-                                           st["e"], st["counts"], ws=self.ws, n_meas_rep=caps_d)
+                                           st["e"], st["counts"], grid=grid, ws=self.ws,
+                                           n_meas_rep=caps_d)
                 rec = {"E": e, "counts": counts}
             else:
-                _, e, m, _, _ = k_on.run(st["spins"], shape, 2, self.betas, self.steps, ch, sched, n,
-                                         ws=self.ws, n_meas_rep=caps_d)
+                _, e, m, _, _ = k_on.run(st["spins"], shape, 2, self.onpar_d[0], self.onpar_d[1], ch,
+                                         sched, n, ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": e, "M": m}
             for s, j in enumerate(self.jobs):
                 if j is not None and caps[s]:
@@ -246,12 +293,16 @@ class Server:
     """Admits jobs against ``budget_bytes`` and advances them in ticks of ``slice_sweeps``."""
 
     def __init__(self, budget_bytes: float = 200e9, slice_sweeps: int = 500, n_streams: int = 4,
-                 max_active: int = 1 << 16, slice_seconds: float = 0.0, min_slice: int = 16):
+                 max_active: int = 1 << 16, slice_seconds: float = 0.0, min_slice: int = 16,
+                 resident: bool = True):
         """``slice_sweeps`` caps the sweeps of one slice.  With ``slice_seconds`` > 0 a
         pool's slice is instead sized from its measured device time per sweep to take
         about that long (at least ``min_slice`` sweeps), so a large pool does not hold
         its stream for long while smaller pools sharing it wait.  Slicing never
-        changes results: a job's chain depends only on its seed."""
+        changes results: a job's chain depends only on its seed.  ``resident`` runs
+        each Ising and Potts pool whose lattice fits on chip in its one-launch
+        per-slice form; False leaves the form to the kernels' automatic choice
+        (the same chains either way)."""
         self.dev = device()
         self.budget = float(budget_bytes)
         self.used = 0.0
@@ -264,6 +315,7 @@ class Server:
         self.waiting: list[Job] = []
         self.finished: list[Job] = []
         self.max_active = max_active
+        self.resident = bool(resident)
         self.n_active = 0
         self._ids = 0
         self.site_updates = 0
@@ -288,7 +340,8 @@ class Server:
         key = (job.model, job.L, job.q if job.model == "potts" else 0)
         if key not in self.pools:
             self.pools[key] = _Pool(job.model, job.L, job.q, self.dev,
-                                    self.streams[len(self.pools) % len(self.streams)])
+                                    self.streams[len(self.pools) % len(self.streams)],
+                                    resident=self.resident)
         return self.pools[key]
 
     def _admit(self, now: float):

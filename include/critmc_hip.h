@@ -170,7 +170,8 @@ int cm_potts_measure(const void *spins_dev, int32_t ndim, const int32_t *dims, i
  * e_run_dev [n_rep] and counts_run_dev [n_rep][q] are the running totals (in/out);
  * grid 0 picks the form, grid > 0 runs one workgroup per replica (states in
  * LDS up to 96 KB), grid < 0 the wide form (one launch per half sweep over all
- * replicas); every form gives the same chains.
+ * replicas); every form gives the same chains.  betas null: the tables are
+ * already in the workspace (cm_potts_prepare) and no host work is done.
  * Other arguments as cm_ising_run (batched form). */
 int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t q,
                  const double *betas, const uint32_t *keys_dev, const uint32_t *stream_ids_dev,
@@ -179,6 +180,15 @@ int cm_potts_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_r
                  int32_t *acc_out_dev, int32_t *counts_out_dev, void *workspace_dev,
                  int64_t workspace_bytes, int32_t block, int32_t grid,
                  const int64_t *n_meas_rep_dev, void *stream);
+
+/* Build the tables of n_rep Potts chains (Metropolis thresholds and heat-bath
+ * weights, critmc/kernels/potts.py:194-200, which the reference rebuilds at
+ * the top of every run) and upload them on `stream` into workspace_dev, once
+ * for any number of later cm_potts_run calls with betas = null on the same
+ * lattice dimension.  The workspace must not be written by anything else in
+ * between. */
+int cm_potts_prepare(int32_t ndim, const int32_t *dims, int32_t n_rep, const double *betas,
+                     void *workspace_dev, int64_t workspace_bytes, void *stream);
 
 /* Bytes of device scratch cm_potts_cluster_run needs. */
 int64_t cm_potts_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep,
@@ -303,8 +313,9 @@ int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t
  * cm_on_cluster_workspace_bytes for schedules with cluster rows (the embedded
  * Ising updates of critmc/kernels/on.py:264-345).
  * A call of local updates only, with aux_out_dev null, on a lattice whose
- * spins fit in 60 KB (N * n * 4 bytes) runs as one launch, one workgroup per
- * replica with its spins in LDS; other calls launch per half sweep.  Both
+ * spins fit in 158 KB (N * n * 4 bytes; XY up to 128^2) runs as one launch,
+ * one workgroup per replica with its spins in LDS; other calls launch per half
+ * sweep.  Both
  * forms give the same spins and series.
  * Replaces critmc/kernels/on.py:348-418 (run). */
 int cm_on_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n,

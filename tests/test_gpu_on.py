@@ -267,7 +267,7 @@ def test_fused_overrelaxation_is_bit_identical(shape, R, fused, split):
 
 
 @pytest.mark.parametrize("n,shape", [(2, (64, 32)), (2, (8, 6)), (2, (10,)), (3, (16, 16, 16)), (3, (6, 4)),
-                                     (2, (4, 4, 6))])
+                                     (2, (4, 4, 6)), (2, (128, 128)), (2, (96, 96)), (3, (24, 24, 16))])
 @pytest.mark.parametrize("sched", [[[0, 1]], [[2, 1]], [[5, 1]], [[2, 2], [0, 1], [5, 1]], [[0, 1], [5, 2]]],
                          ids=["metropolis", "heatbath", "overrelax", "mixed", "metropolis+2or"])
 def test_lds_form_matches_half_sweep_launches(n, shape, sched):
