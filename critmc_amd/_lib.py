@@ -36,6 +36,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "cm_last_error": ([], ctypes.c_char_p),
     "cm_philox_fill": ([_vp, _c_i64, _c_u32, _c_u32, _c_u32, _c_u32, _c_u32, _c_u32, _vp], _c_i32),
     "cm_philox_host": ([_u32p, _u32p, _u32p], None),
+    "cm_copy_spans": ([_vp, _c_i32, _vp], _c_i32),
     "cm_device_info": ([_i32p, _i64p, _i32p, _i32p], _c_i32),
     "cm_ising_workspace_bytes": ([_c_i32], _c_i64),
     "cm_ising_tables": ([_c_f64, _c_f64, _c_i32, _u32p], _c_i32),

@@ -118,11 +118,34 @@ __global__ void philox_fill_kernel(uint32_t* __restrict__ out, int64_t n_blocks,
     }
 }
 
+// One workgroup per span: copies spans[3 s + 2] bytes (a multiple of 4) from
+// spans[3 s + 1] to spans[3 s], both 4-byte aligned, in 4-byte words.
+__global__ __launch_bounds__(256) void copy_spans_kernel(const uint64_t* __restrict__ spans, int n)
+{
+    for (int sp = blockIdx.x; sp < n; sp += gridDim.x) {
+        uint32_t* dst = (uint32_t*)spans[3 * sp];
+        const uint32_t* src = (const uint32_t*)spans[3 * sp + 1];
+        const uint64_t words = spans[3 * sp + 2] >> 2;
+        for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) dst[w] = src[w];
+    }
+}
+
 }  // namespace cm
 
 using namespace cm;
 
 extern "C" {
+
+int cm_copy_spans(const void* spans, int32_t n, void* stream)
+{
+    CM_REQUIRE(n >= 0, "n must not be negative");
+    if (n == 0) return CM_OK;
+    const int blocks = std::min(n, 65535);
+    hipLaunchKernelGGL(copy_spans_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const uint64_t*)spans, n);
+    CM_HIP_CHECK(hipGetLastError());
+    return CM_OK;
+}
 
 int32_t cm_version(void) { return 1; }
 

@@ -24,10 +24,12 @@
 // one launch instead (on_batched: one workgroup per replica, spins in LDS,
 // the same per-site functions, so the same bits).
 // When the schedule ends with a local update, that last half sweep writes its
-// sites renormalised and on_measure_c0 renormalises colour 0 while summing
-// every bond once (when it ends with two fused over-relaxation sweeps, the
-// fused launch does both); otherwise a normalise launch precedes the
-// partial-sum launch.  Partial sums are finalised in order, so every FP64 result is
+// sites renormalised and colour 0 is renormalised while every bond is summed
+// once: by the next measurement's first colour-0 half sweep, which then
+// updates those sites (OnDefer), or by on_measure_c0 for a call's last
+// measurement (when it ends with two fused over-relaxation sweeps, the fused
+// launch does both); otherwise a normalise launch precedes the partial-sum
+// launch.  Partial sums are finalised in order, so every FP64 result is
 // reproducible run to run.
 //
 // Heat bath (:186-261), FP64 from the exact FP64 field:
@@ -75,7 +77,8 @@
 
 namespace cm {
 
-constexpr int ON_CHUNK = 4096;  // sites per measurement partial
+constexpr int ON_CHUNK = 4096;  // sites per partial of on_measure_partial (site order)
+constexpr int ON_BLOCK = 256;   // colour-0 sites per partial of every colour-0 measurement
 
 template <int NC>
 struct Vec {
@@ -358,14 +361,101 @@ __device__ __forceinline__ bool on_site(Vec<NC>& s, const float* __restrict__ So
     return changed;
 }
 
-template <int NC, int Z, int CODE>
-__global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
+template <int NC, int D>
+__device__ __forceinline__ void on_c0_bonds(const Vec<NC>& s, const float* __restrict__ S1,
+                                            const uint32_t (&nbo)[2 * D], double (&v)[1 + NC + 2 * D]);
+
+// The ordered sum of one quantity of one replica's colour-0 block partials
+// (thread t of R K: replica t / K, quantity t % K), written to its output
+// series.  The loads of eight partials are issued before their adds.
+template <int NC, int D>
+__device__ __forceinline__ void on_final_sum(const double* __restrict__ part, int nb, int R,
+                                             int64_t n_meas, int64_t meas, double* __restrict__ e_out,
+                                             double* __restrict__ m_out, double* __restrict__ sc_out,
+                                             double* __restrict__ ss_out,
+                                             const int64_t* __restrict__ n_meas_rep, int t)
 {
+    constexpr int K = 1 + NC + 2 * D;
+    if (t >= R * K) return;
+    const int rep = t / K, q = t - (t / K) * K;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;
+    const double* pp = part + (int64_t)rep * nb * K + q;
+    double s = 0.0;
+    int c = 0;
+    for (; c + 8 <= nb; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pp[(int64_t)(c + u) * K];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; c < nb; ++c) s += pp[(int64_t)c * K];
+    const int64_t o = (int64_t)rep * n_meas + meas;
+    if (q == 0) e_out[o] = s;
+    else if (q < 1 + NC) m_out[o * NC + (q - 1)] = s;
+    else if (q < 1 + NC + D) sc_out[o * D + (q - 1 - NC)] = s;
+    else ss_out[o * D + (q - 1 - NC - D)] = s;
+}
+
+// A 256-lane block's partial of K quantities: each wave's sum, then the four
+// waves added in order (thread q < K writes quantity q to dst[q]).
+template <int K>
+__device__ __forceinline__ void on_block_partial(const double (&v)[K], double* __restrict__ red,
+                                                 double* __restrict__ dst)
+{
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        const double t = wave_sum_f64(v[q]);
+        if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double t = 0.0;
+        for (int u = 0; u < 4; ++u) t += red[threadIdx.x * 4 + u];
+        dst[threadIdx.x] = t;
+    }
+}
+
+// A measurement carried by the next sweep's launches (local schedules whose
+// measurement ends with a colour-1 half sweep): the first colour-0 half sweep
+// of measurement m renormalises its sites and forms measurement m-1's block
+// partials before updating them (the reference renormalises at the
+// measurement and starts the next sweep from those spins), and the colour-1
+// half sweep that follows adds them up.
+struct OnDefer {
+    double* part;         // colour-0 launch: partials of measurement `meas` (null: none)
+    int64_t meas;
+    const double* fpart;  // ordered sums of measurement `fmeas` from these partials (null: none)
+    int64_t fmeas;
+    int nb;               // block partials per replica
+    double* e_out;
+    double* m_out;
+    double* sc_out;
+    double* ss_out;
+};
+
+template <int NC, int Z, int CODE, bool DEFER>
+__global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnDefer df)
+{
+    constexpr int D = Z / 2, K = 1 + NC + Z;
     const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int rep = (int)(lb / gridDim.x);
-    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
-    const int64_t k = (int64_t)(lb - (uint32_t)rep * gridDim.x) * blockDim.x + threadIdx.x;
+    const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
+    if constexpr (DEFER) {
+        if (df.fpart)  // one thread per (replica, quantity), before any replica's early exit
+            on_final_sum<NC, D>(df.fpart, df.nb, a.R, a.n_meas, df.fmeas, df.e_out, df.m_out,
+                                df.sc_out, df.ss_out, a.n_meas_rep, (int)(lb * blockDim.x + threadIdx.x));
+    }
+    const bool upd = !(a.n_meas_rep && a.meas >= a.n_meas_rep[rep]);
+    bool msr = false;
+    if constexpr (DEFER) msr = df.part && !(a.n_meas_rep && df.meas >= a.n_meas_rep[rep]);
+    if (!upd && !msr) return;  // whole block: replica done
+    const int64_t k = (int64_t)bx * blockDim.x + threadIdx.x;
     int accepted = 0;
+    double v[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = 0.0;
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
         uint32_t nbo[Z];
@@ -373,11 +463,18 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
         const int64_t i = (int64_t)a.colour * a.g.half + k;            // own plane
         const float* So = S + (int64_t)(1 - a.colour) * a.g.half * NC;  // neighbours' plane
         Vec<NC> s = load_vec<NC>(S, i);
-        const bool changed = on_site<NC, Z, CODE>(s, So, nbo, (uint32_t)k, a.colour,
-                                                  a.sweeps[rep] + a.sweep_add, a.keys[2 * rep],
-                                                  a.keys[2 * rep + 1], a.sids[rep], a.betas[rep],
-                                                  a.steps[rep], accepted);
-        if (a.norm_out) {
+        bool changed = false;
+        if (DEFER && msr) {  // colour 0: the previous measurement's renormalisation and bonds
+            s = unit_vec<NC>(s);
+            on_c0_bonds<NC, D>(s, So, nbo, v);
+            changed = true;
+        }
+        if (upd)
+            changed |= on_site<NC, Z, CODE>(s, So, nbo, (uint32_t)k, a.colour,
+                                            a.sweeps[rep] + a.sweep_add, a.keys[2 * rep],
+                                            a.keys[2 * rep + 1], a.sids[rep], a.betas[rep],
+                                            a.steps[rep], accepted);
+        if (upd && a.norm_out) {
             // the measurement's renormalisation of this site, fused into its last update
             s = unit_vec<NC>(s);
             store_vec<NC>(S, i, s);
@@ -385,6 +482,13 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a)
             store_vec<NC>(S, i, s);
         }
     }
+    if constexpr (DEFER) {
+        if (msr) {
+            __shared__ double mred[K * 4];
+            on_block_partial<K>(v, mred, df.part + ((int64_t)rep * df.nb + bx) * K);
+        }
+    }
+    if (!upd) return;
     if (CODE == CODE_METROPOLIS) {  // one atomic per workgroup
         __shared__ int part[4];
         const int t = wave_sum_bits<1>((uint32_t)accepted);
@@ -662,20 +766,24 @@ __device__ __forceinline__ void on_c0_bonds(const Vec<NC>& s, const float* __res
 // bond joins a colour-0 site i to a colour-1 site, so i's lane takes all 2d
 // bonds of i (forward (i, i+e_a) and backward (i-e_a, i), oriented as the
 // reference's forward bonds), renormalises i in place (no other lane reads
-// it) and adds M of i and of i+e_0 (a bijection onto colour 1).
+// it) and adds M of i and of i+e_0 (a bijection onto colour 1).  One lane per
+// site, one partial per block of ON_BLOCK sites (on_block_partial), and
+// on_measure_final adds the blocks in order: the colour-0 half sweep that
+// carries a measurement and the per-replica LDS form form the same partials,
+// so every path gives the same bits.
 template <int NC, int D>
 __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, CbGeo g,
-                                                     double* __restrict__ part, int nchunk,
+                                                     double* __restrict__ part, int nb,
                                                      const int64_t* __restrict__ n_meas_rep,
                                                      int64_t meas)
 {
     constexpr int K = 1 + NC + 2 * D;
     constexpr int Z = 2 * D;
     __shared__ double red[K * 4];
-    // consecutive chunks of a replica on one XCD, so the rows a chunk reads
-    // beyond its own are in the same L2 as their neighbour chunk's
+    // consecutive blocks of a replica on one XCD, so the rows a block reads
+    // beyond its own are in the same L2 as their neighbour block's
     const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int rep = (int)(lb / gridDim.x), chunk = (int)(lb - (uint32_t)rep * gridDim.x);
+    const int rep = (int)(lb / gridDim.x), blk = (int)(lb - (uint32_t)rep * gridDim.x);
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;
     // the colour-0 plane is renormalised in place, the colour-1 plane only read:
     // separate restrict pointers let the loads move ahead of the stores
@@ -684,34 +792,22 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     double v[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = 0.0;
-    const int64_t k0 = (int64_t)chunk * ON_CHUNK;
-    const int64_t k1 = min(g.half, k0 + ON_CHUNK);
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    const int64_t k = (int64_t)blk * ON_BLOCK + threadIdx.x;
+    if (k < g.half) {
         uint32_t nbo[Z];
         cb_other<Z>(g, (uint32_t)k, 0, nbo);
         const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S0, k));
         store_vec<NC>(S0, k, s);
         on_c0_bonds<NC, D>(s, S1, nbo, v);
     }
-    const int w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const double t = wave_sum_f64(v[q]);
-        if ((threadIdx.x & 63) == 0) red[q * 4 + w] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < K) {
-        double t = 0.0;
-        for (int u = 0; u < (int)(blockDim.x >> 6); ++u) t += red[threadIdx.x * 4 + u];
-        part[((int64_t)rep * nchunk + chunk) * K + threadIdx.x] = t;
-    }
+    on_block_partial<K>(v, red, part + ((int64_t)rep * nb + blk) * K);
 }
 
 // Per-replica form for lattices whose spins fit in LDS (N n 4 B <= ON_BATCHED_LDS):
 // one workgroup per replica runs every sweep and measurement of the call with
 // the replica in LDS, in the colour-split layout, and writes it back once.
 // Each site update is on_site, and each measurement repeats on_measure_c0's
-// per-site work, chunks and reductions and then on_measure_final's ordered
+// per-site work, blocks and reductions and then on_measure_final's ordered
 // sum, so spins and series equal the per-half-sweep launches' bit for bit.
 // Local updates only (Metropolis, heat bath, over-relaxation); as there, the
 // last half sweep of a measurement stores its sites renormalised.  HB: the
@@ -719,9 +815,9 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
 // would otherwise limit every schedule's occupancy).  BLK lanes per workgroup:
 // 256, or 1024 for replicas large enough to give each lane several sites
 // (local updates without heat bath); the measurement then runs in groups of
-// 256 lanes, group j taking chunk j of every NG = BLK / 256 consecutive
-// chunks exactly as one on_measure_c0 workgroup takes it, and the chunk
-// partials are added in chunk order, so the bits do not depend on BLK.
+// 256 lanes, group j taking block j of every NG = BLK / 256 consecutive
+// blocks exactly as one on_measure_c0 workgroup takes it, and the block
+// partials are added in block order, so the bits do not depend on BLK.
 // Up to 158 KB of spins (XY 128^2 at 128 KB): one workgroup per CU.
 constexpr int64_t ON_BATCHED_LDS = 158 * 1024;
 
@@ -788,16 +884,17 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
                     }
                 }
             }
-            // the measurement: colour 0 renormalised in place, bonds summed per chunk
-            double total = 0.0;  // thread q < K: quantity q, chunk partials added in order
-            const int nchunk = (int)((half + ON_CHUNK - 1) / ON_CHUNK);
+            // the measurement: colour 0 renormalised in place, bonds summed per
+            // block of ON_BLOCK sites (one per lane of a group), as on_measure_c0
+            double total = 0.0;  // thread q < K: quantity q, block partials added in order
+            const int nblk = (int)((half + ON_BLOCK - 1) / ON_BLOCK);
             const int grp = (int)(threadIdx.x >> 8), gt = (int)(threadIdx.x & 255), gw = (gt >> 6);
-            for (int cb = 0; cb < nchunk; cb += NG) {
+            for (int cb = 0; cb < nblk; cb += NG) {
                 double v[K];
 #pragma unroll
                 for (int q = 0; q < K; ++q) v[q] = 0.0;
-                const int64_t c0 = (int64_t)(cb + grp) * ON_CHUNK, c1 = min(half, c0 + ON_CHUNK);
-                for (int64_t k = c0 + gt; k < c1; k += 256) {
+                const int64_t k = (int64_t)(cb + grp) * ON_BLOCK + gt;
+                if (cb + grp < nblk && k < half) {
                     uint32_t nbo[Z];
                     cb_other<Z>(g, (uint32_t)k, 0, nbo);
                     const Vec<NC> s = unit_vec<NC>(load_vec<NC>(L0, k));
@@ -811,7 +908,7 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
                 }
                 __syncthreads();
                 if (threadIdx.x < K) {
-                    for (int j = 0; j < NG && cb + j < nchunk; ++j) {
+                    for (int j = 0; j < NG && cb + j < nblk; ++j) {
                         double t = 0.0;
                         for (int u = 0; u < 4; ++u) t += red[((int)threadIdx.x * NG + j) * 4 + u];
                         total += t;
@@ -842,33 +939,13 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
 }
 
 template <int NC, int D>
-__global__ void on_measure_final(const double* __restrict__ part, int nchunk, int R,
+__global__ void on_measure_final(const double* __restrict__ part, int nb, int R,
                                  int64_t n_meas, int64_t meas, double* __restrict__ e_out,
                                  double* __restrict__ m_out, double* __restrict__ sc_out,
                                  double* __restrict__ ss_out, const int64_t* __restrict__ n_meas_rep)
 {
-    constexpr int K = 1 + NC + 2 * D;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= R * K) return;
-    const int rep = t / K, q = t - (t / K) * K;
-    if (n_meas_rep && meas >= n_meas_rep[rep]) return;
-    // in chunk order; the loads of eight chunks are issued before their adds
-    const double* pp = part + (int64_t)rep * nchunk * K + q;
-    double s = 0.0;
-    int c = 0;
-    for (; c + 8 <= nchunk; c += 8) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = pp[(int64_t)(c + u) * K];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
-    }
-    for (; c < nchunk; ++c) s += pp[(int64_t)c * K];
-    const int64_t o = (int64_t)rep * n_meas + meas;
-    if (q == 0) e_out[o] = s;
-    else if (q < 1 + NC) m_out[o * NC + (q - 1)] = s;
-    else if (q < 1 + NC + D) sc_out[o * D + (q - 1 - NC)] = s;
-    else ss_out[o * D + (q - 1 - NC - D)] = s;
+    on_final_sum<NC, D>(part, nb, R, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep,
+                        (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 __global__ void fill_i32_on(int32_t* __restrict__ p, int64_t n)
@@ -1265,24 +1342,34 @@ __global__ void on_init_kernel(float* __restrict__ spins, ClGeo g, int NC,
     }
 }
 
-template <int NC, int Z>
-static void launch_half(int code, const OnArgs& a, hipStream_t st)
+// A half sweep; with df.part or df.fpart set it also carries the previous
+// measurement (OnDefer).  Blocks of ON_BLOCK lanes, one colour site each, so a
+// colour-0 launch forms the same block partials as on_measure_c0.
+template <int NC, int Z, bool DEFER>
+static void launch_half(int code, const OnArgs& a, const OnDefer& df, hipStream_t st)
 {
-    const dim3 grid((unsigned)((a.g.half + 255) / 256), a.R);
+    const dim3 grid((unsigned)((a.g.half + ON_BLOCK - 1) / ON_BLOCK), a.R);
     if (code == CODE_OVERRELAX)
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
     else if (code == CODE_HEATBATH)
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_HEATBATH>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_HEATBATH, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
     else
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
 }
 
 template <int NC>
-static void launch_half_z(int code, const OnArgs& a, hipStream_t st)
+static void launch_half_z(int code, const OnArgs& a, hipStream_t st, const OnDefer* df = nullptr)
 {
-    if (a.g.d == 1) launch_half<NC, 2>(code, a, st);
-    else if (a.g.d == 2) launch_half<NC, 4>(code, a, st);
-    else launch_half<NC, 6>(code, a, st);
+    const OnDefer none{};
+    if (df && (df->part || df->fpart)) {
+        if (a.g.d == 1) launch_half<NC, 2, true>(code, a, *df, st);
+        else if (a.g.d == 2) launch_half<NC, 4, true>(code, a, *df, st);
+        else launch_half<NC, 6, true>(code, a, *df, st);
+        return;
+    }
+    if (a.g.d == 1) launch_half<NC, 2, false>(code, a, none, st);
+    else if (a.g.d == 2) launch_half<NC, 4, false>(code, a, none, st);
+    else launch_half<NC, 6, false>(code, a, none, st);
 }
 
 template <int NC>
@@ -1308,7 +1395,7 @@ static void launch_measure_c0(float* spins, const CbGeo& g, int R, double* part,
                               int64_t meas, double* e_out, double* m_out, double* sc_out,
                               double* ss_out, const int64_t* nr, hipStream_t st)
 {
-    const int nc0 = (int)((g.half + ON_CHUNK - 1) / ON_CHUNK);
+    const int nc0 = (int)((g.half + ON_BLOCK - 1) / ON_BLOCK);
     const dim3 grid(nc0, R);
     if (g.d == 1) {
         hipLaunchKernelGGL((on_measure_c0<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
@@ -1375,7 +1462,8 @@ int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep, 
     int64_t N = 1;
     for (int a = 0; a < ndim; ++a) N *= dims[a];
     const int64_t nchunk = (N + ON_CHUNK - 1) / ON_CHUNK;
-    return (int64_t)n_rep * nchunk * (1 + n + 2 * ndim) * 8 + (int64_t)n_rep * 8;
+    const int64_t nblk = (N / 2 + ON_BLOCK - 1) / ON_BLOCK;  // colour-0 block partials
+    return (int64_t)n_rep * std::max(nchunk, nblk) * (1 + n + 2 * ndim) * 8 + (int64_t)n_rep * 8;
 }
 
 int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep)
@@ -1510,9 +1598,23 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_or_fused<ON_OR_FUSE>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
+    // Deferred measurements (OnDefer): when a measurement ends with a plain
+    // colour-1 half sweep and the next one starts with a plain colour-0 half
+    // sweep, that colour-0 launch renormalises and measures the previous
+    // state and the colour-1 launch after it adds the block partials; only
+    // the call's last measurement takes the separate measure launches.
+    const int64_t first_code = schedule[0], first_count = schedule[1];
+    const int64_t last_count = schedule[2 * (n_rows - 1) + 1];
+    const bool first_plain = !(first_code == CODE_OVERRELAX && or_rows > 0 && first_count >= 2);
+    const bool last_plain = !(last_code == CODE_OVERRELAX && or_rows > 0 && last_count % 2 == 0);
+    const bool defer = fused && !any_cluster && first_plain && last_plain;
+    const int nb0 = (int)((g.half + ON_BLOCK - 1) / ON_BLOCK);
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
         int meas_tiles = 0;  // > 0: the fused over-relaxation launch measured
+        OnDefer df{};         // the previous measurement, carried by this one's first sweep
+        if (defer && meas > 0)
+            df = OnDefer{part, meas - 1, part, meas - 1, nb0, e_out, m_out, sc_out, ss_out};
         if (any_cluster) hipLaunchKernelGGL(on_cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
         for (int r = 0; r < n_rows; ++r) {
             const int code = (int)schedule[2 * r];
@@ -1544,8 +1646,16 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                 for (int c = 0; c < 2; ++c) {
                     a.colour = c;
                     a.norm_out = fused && r == n_rows - 1 && rp == count - 1 && c == 1;
-                    if (n == 2) launch_half_z<2>(code, a, st);
-                    else launch_half_z<3>(code, a, st);
+                    // the measurement's first sweep: colour 0 forms the previous
+                    // measurement's partials, colour 1 adds them up
+                    OnDefer dc{};
+                    if (r == 0 && rp == 0 && df.part) {
+                        dc = df;
+                        if (c == 0) dc.fpart = nullptr;
+                        else dc.part = nullptr;
+                    }
+                    if (n == 2) launch_half_z<2>(code, a, st, &dc);
+                    else launch_half_z<3>(code, a, st, &dc);
                 }
                 ++sw;
                 ++rp;
@@ -1556,6 +1666,8 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         if (meas_tiles > 0) {
             hipLaunchKernelGGL((on_measure_final<2, 2>), dim3((n_rep * 7 + 255) / 256), dim3(256), 0, st,
                                part, meas_tiles, n_rep, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep);
+        } else if (defer && meas + 1 < n_meas) {
+            // measured by the next measurement's first sweep
         } else if (fused) {
             if (n == 2) launch_measure_c0<2>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
             else launch_measure_c0<3>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);

@@ -25,7 +25,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .device import device
+from ._lib import check, lib
+from .device import device, ptr, stream
 from .kernels import ising as k_ising
 from .kernels import on as k_on
 from .kernels import potts as k_potts
@@ -75,8 +76,11 @@ class Job:
         return {k: v.cpu().numpy() for k, v in self.series.items()}
 
 
-# Largest replica, in states, that the per-replica Potts form keeps in LDS.
-POTTS_LDS_STATES = 96 * 1024
+# Largest Potts replica, in states, that a serving pool runs in the per-replica
+# form (one workgroup per replica, the lattice in LDS): up to 256^2, whose
+# sweep on one CU takes about as long as the two launches per sweep of the wide
+# form cost the device in a busy stream.
+POTTS_RESIDENT_STATES = 256 * 256
 
 
 def _ising_resident_ok(L: int) -> bool:
@@ -92,16 +96,18 @@ class _Pool:
         self.model, self.L, self.q = model, L, q
         self.dev, self.stream = dev, stream
         # One launch per slice (each replica a workgroup holding its lattice on
-        # chip for every sweep of the slice) wherever the lattice fits, instead
-        # of the launch per half sweep the automatic choice makes for few
-        # replicas of a large lattice: a serving stream is many small pools
-        # sharing the GPU, so launches per slice, not CUs per pool, set the pace.
+        # chip for every sweep of the slice) instead of the launch per half
+        # sweep the automatic choice makes for few replicas of a large lattice:
+        # a serving stream is many small pools sharing the GPU, and the host's
+        # kernel launches, not CUs per pool, set its pace.  Ising: wherever the
+        # lattice fits; Potts: up to 256^2 (see POTTS_RESIDENT_STATES).
         self.resident = resident
         self.cap = 0
         self.jobs: list[Job | None] = []
         self.free: list[int] = []
         self.pending: list[Job] = []      # admitted, joining at the next slice boundary
         self.inflight = None              # caps of the slice on the device
+        self.enqueue = None               # future of a slice being enqueued by a launch thread
         self.start_event = torch.cuda.Event(enable_timing=True)
         self.done_event = torch.cuda.Event(enable_timing=True)
         self.busy_s = 0.0                 # device time of this pool's slices
@@ -153,6 +159,9 @@ class _Pool:
         self.keys_h = keys_h
         self.caps_h = torch.zeros(cap, dtype=torch.int64).pin_memory()
         self.caps_d = torch.zeros(cap, dtype=torch.int64, device=self.dev)
+        # (dst, src, bytes) of every job's slice of the time series, one copy launch per slice
+        self.spans_h = torch.zeros((2 * cap, 3), dtype=torch.int64).pin_memory()
+        self.spans_d = torch.zeros((2 * cap, 3), dtype=torch.int64, device=self.dev)
         if self.model == "xy":
             self.onpar_h = torch.zeros((2, cap), dtype=torch.float32).pin_memory()
             self.onpar_d = torch.zeros((2, cap), dtype=torch.float32, device=self.dev)
@@ -190,8 +199,10 @@ class _Pool:
         self.keys_h[slot] = torch.from_numpy(key.view(np.int32))
         st = self.state
         st["keys"][slot].copy_(self.keys_h[slot], non_blocking=True)
-        st["sids"][slot] = 0
-        st["sweeps"][slot] = 0
+        # fill kernels: assigning a Python number to a device element would
+        # copy it from pageable host memory and wait for the stream
+        st["sids"][slot: slot + 1].zero_()
+        st["sweeps"][slot: slot + 1].zero_()
         one = _DeviceChains(st["keys"][slot: slot + 1], st["sids"][slot: slot + 1],
                             st["sweeps"][slot: slot + 1], 1)
         shape = (self.L, self.L)
@@ -260,7 +271,7 @@ class _Pool:
                                   form=form, ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": out[0], "M": out[1]}
             elif self.model == "potts":
-                grid = self.cap if self.resident and self.L * self.L <= POTTS_LDS_STATES else 0
+                grid = self.cap if self.resident and self.L * self.L <= POTTS_RESIDENT_STATES else 0
                 e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
                                            st["e"], st["counts"], grid=grid, ws=self.ws,
                                            n_meas_rep=caps_d)
@@ -269,10 +280,20 @@ class _Pool:
                 _, e, m, _, _ = k_on.run(st["spins"], shape, 2, self.onpar_d[0], self.onpar_d[1], ch,
                                          sched, n, ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": e, "M": m}
+            spans = self.spans_h.numpy()
+            ns = 0
             for s, j in enumerate(self.jobs):
                 if j is not None and caps[s]:
                     for k, v in rec.items():
-                        j.series[k][j.done: j.done + caps[s]].copy_(v[s, : caps[s]])
+                        dst = j.series[k]
+                        per = dst[0].numel() * dst.element_size()  # bytes of one measurement
+                        spans[ns] = (dst.data_ptr() + j.done * per,
+                                     v.data_ptr() + s * v.stride(0) * v.element_size(),
+                                     int(caps[s]) * per)
+                        ns += 1
+            if ns:
+                self.spans_d[:ns].copy_(self.spans_h[:ns], non_blocking=True)
+                check(lib().cm_copy_spans(ptr(self.spans_d), ns, stream()), "cm_copy_spans")
             self.done_event.record(self.stream)
         self.inflight = caps
         return caps
@@ -294,7 +315,7 @@ class Server:
 
     def __init__(self, budget_bytes: float = 200e9, slice_sweeps: int = 500, n_streams: int = 4,
                  max_active: int = 1 << 16, slice_seconds: float = 0.0, min_slice: int = 16,
-                 resident: bool = True):
+                 resident: bool = True, launch_threads: int = 0):
         """``slice_sweeps`` caps the sweeps of one slice.  With ``slice_seconds`` > 0 a
         pool's slice is instead sized from its measured device time per sweep to take
         about that long (at least ``min_slice`` sweeps), so a large pool does not hold
@@ -302,7 +323,10 @@ class Server:
         changes results: a job's chain depends only on its seed.  ``resident`` runs
         each Ising and Potts pool whose lattice fits on chip in its one-launch
         per-slice form; False leaves the form to the kernels' automatic choice
-        (the same chains either way)."""
+        (the same chains either way).  ``launch_threads`` > 0 enqueues the pools'
+        slices from that many host threads (the library releases the interpreter
+        while it launches), so a pool whose slice is thousands of launches does
+        not hold up the others; 0 enqueues them from the calling thread."""
         self.dev = device()
         self.budget = float(budget_bytes)
         self.used = 0.0
@@ -316,6 +340,13 @@ class Server:
         self.finished: list[Job] = []
         self.max_active = max_active
         self.resident = bool(resident)
+        self.launch_threads = int(launch_threads)
+        self._launcher = None
+        if self.launch_threads > 0:
+            from concurrent.futures import ThreadPoolExecutor
+
+            self._launcher = ThreadPoolExecutor(max_workers=self.launch_threads,
+                                                thread_name_prefix="critmc-serve")
         self.n_active = 0
         self._ids = 0
         self.site_updates = 0
@@ -400,6 +431,13 @@ class Server:
             self._admit(clock())
             busy = False
             for pool in self.pools.values():
+                if pool.enqueue is not None:  # its slice is still being enqueued
+                    if not pool.enqueue.done():
+                        busy = True
+                        continue
+                    caps, pool.enqueue = pool.enqueue.result(), None
+                    if caps is not None:
+                        self.ticks += 1
                 if pool.inflight is not None:
                     if not pool.done_event.query():
                         busy = True
@@ -408,7 +446,12 @@ class Server:
                 for job in pool.pending:
                     pool.add(job)
                 pool.pending = []
-                if pool.active and pool.launch(self._slice_for(pool)) is not None:
+                if not pool.active:
+                    continue
+                if self._launcher is not None:
+                    pool.enqueue = self._launcher.submit(pool.launch, self._slice_for(pool))
+                    busy = True
+                elif pool.launch(self._slice_for(pool)) is not None:
                     self.ticks += 1
                     busy = True
             if progress > 0 and time.perf_counter() - last > progress:

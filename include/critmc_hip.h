@@ -37,6 +37,14 @@ int32_t cm_version(void);
 /* Text of the last error raised on the calling thread. */
 const char *cm_last_error(void);
 
+/* Copy n device-to-device spans in one launch: spans_dev holds n records
+ * (dst, src, bytes) of uint64, with dst and src 4-byte aligned and bytes a
+ * multiple of 4, and the spans must not overlap.  No counterpart in the
+ * reference (whose runner copies one chain's numpy arrays at a time); the
+ * serving loop hands every job of a pool its slice of the time series with it
+ * instead of one copy per job and series. */
+int cm_copy_spans(const void *spans_dev, int32_t n, void *stream);
+
 /* Device properties used by the host-side tuner. */
 int cm_device_info(int32_t *cu_count, int64_t *lds_per_block, int32_t *arch_major,
                    int32_t *arch_minor);
