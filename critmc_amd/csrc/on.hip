@@ -518,22 +518,25 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnDefer df)
 // FP64 partial sums go to part[rep][tile][7]; on_measure_final adds the
 // tiles in order.
 template <int K>
-__global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __restrict__ part)
+__global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, const float* __restrict__ src, int T,
+                                                    double* __restrict__ part)
 {
     extern __shared__ float2 orl[];
     const int M = a.g.M, L1 = a.g.L1;
     const int tiles = (L1 + T - 1) / T;
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const int rep = (int)(lb / (uint32_t)tiles), tile = (int)(lb - (uint32_t)rep * tiles);
-    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
     const int y0 = tile * T;
     const int Te = min(T, L1 - y0);
     const int R0 = Te + 2 * (K - 1), R1 = Te + 2 * K;
     float2* P0 = orl;
     float2* P1 = orl + (size_t)R0 * M;
-    float2* S = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
-    float2* G0 = S;
-    float2* G1 = S + a.g.half;
+    // out of place: the tile reads its rows and halos from `src` and writes its
+    // own rows to a.spins, so no tile can read a row another has already updated
+    const float2* I0 = (const float2*)(src + (int64_t)rep * a.g.N * 2);
+    const float2* I1 = I0 + a.g.half;
+    float2* G0 = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
+    float2* G1 = G0 + a.g.half;
     // Each thread keeps one column mm and visits rows tr0, tr0 + bq, ...: the
     // workgroup covers bq whole rows per pass (threads beyond bq rows idle),
     // so LDS offsets advance by a constant and no per-site division is left.
@@ -544,6 +547,15 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
     const int stride = bq * M;
     const int dp1 = mm == M - 1 ? -(M - 1) : 1;  // x0 + 1 in the other plane, x0 odd
     const int dn0 = mm == 0 ? M - 1 : -1;        // x0 - 1 in the other plane, x0 even
+    if (a.n_meas_rep && a.n_meas_rep[rep] <= 0) return;  // no measurement this call: state stays put
+    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) {  // whole block: replica done, rows carried over
+        if (act)
+            for (int r = tr0; r < Te; r += bq) {
+                G0[(int64_t)(y0 + r) * M + mm] = I0[(int64_t)(y0 + r) * M + mm];
+                G1[(int64_t)(y0 + r) * M + mm] = I1[(int64_t)(y0 + r) * M + mm];
+            }
+        return;
+    }
     // lattice row of LDS row r (the halo rows wrap; at most once when L1 >= K)
     auto wrap = [&](int y) {
         if (L1 >= K) return y < 0 ? y + L1 : (y >= L1 ? y - L1 : y);
@@ -551,8 +563,8 @@ __global__ __launch_bounds__(1024) void on_or_fused(OnArgs a, int T, double* __r
         return q < 0 ? q + L1 : q;
     };
     if (act) {
-        for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = G0[(int64_t)wrap(y0 - (K - 1) + r) * M + mm];
-        for (int r = tr0; r < R1; r += bq) P1[r * M + mm] = G1[(int64_t)wrap(y0 - K + r) * M + mm];
+        for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = I0[(int64_t)wrap(y0 - (K - 1) + r) * M + mm];
+        for (int r = tr0; r < R1; r += bq) P1[r * M + mm] = I1[(int64_t)wrap(y0 - K + r) * M + mm];
     }
     __syncthreads();
 #pragma unroll
@@ -654,6 +666,256 @@ static int on_or_tile_rows(const CbGeo& g, int n)
     const int64_t T = (ON_OR_LDS / row - (4 * ON_OR_FUSE - 2)) / 2;
     if (T < 4) return 0;
     return (int)std::min<int64_t>(T, g.L1);
+}
+
+// One sweep of XY spins on a square lattice in one launch (colour 0, then
+// colour 1), over row tiles in LDS, for any local update code.  A 1024-lane
+// workgroup owns rows [y0, y0 + Te) of one replica: it holds colour 0 on rows
+// y0-1 .. y0+Te and colour 1 on rows y0-2 .. y0+Te+1, updates colour 0 on
+// every row it holds (the two halo rows repeat the neighbouring tiles'
+// updates: the same inputs and the same Philox block give the same result)
+// and colour 1 on its own rows.  Every update is on_site, so the spins equal
+// two half-sweep launches bit for bit.
+//
+// With mt.part set it first carries the previous measurement, as the
+// colour-0 half sweep does (OnDefer): colour 0 renormalised on every row it
+// holds, then one partial per block of 256 own colour-0 sites, one site per
+// lane exactly as on_measure_c0 forms it (T M is a multiple of 256, so blocks
+// do not straddle tiles), and the replica's last workgroup to finish (a ticket
+// per replica) adds the blocks in order.  The results equal on_measure_c0's
+// and on_measure_final's bit for bit.
+struct OnTile {
+    double* part;  // partials of measurement `meas` (null: no measurement in this launch)
+    int64_t meas;
+    int nb;        // block partials per replica
+    int* ticket;   // [R] workgroups done with the measurement; zero between launches
+    double* e_out;
+    double* m_out;
+    double* sc_out;
+    double* ss_out;
+};
+
+template <int CODE>
+__global__ __launch_bounds__(1024) void on_sweep_tile(OnArgs a, const float* __restrict__ src, int T,
+                                                      OnTile mt)
+{
+    constexpr int K = 1 + 2 + 4;  // E, M[2], c_a[2], s_a[2]
+    extern __shared__ float2 stl[];
+    __shared__ double tred[K * 16];
+    __shared__ int ired[16];
+    __shared__ int last;
+    const int M = a.g.M, L1 = a.g.L1;
+    const int tiles = (L1 + T - 1) / T;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int rep = (int)(lb / (uint32_t)tiles), tile = (int)(lb - (uint32_t)rep * tiles);
+    const bool upd = !(a.n_meas_rep && a.meas >= a.n_meas_rep[rep]);
+    const bool msr = mt.part && !(a.n_meas_rep && mt.meas >= a.n_meas_rep[rep]);
+    const int y0 = tile * T;
+    const int Te = min(T, L1 - y0);
+    const int R0 = Te + 2, R1 = Te + 4;
+    float2* P0 = stl;                    // colour 0, LDS row r = lattice row y0 - 1 + r
+    float2* P1 = stl + (size_t)R0 * M;   // colour 1, LDS row r = lattice row y0 - 2 + r
+    // out of place: rows and halos from `src`, own rows to a.spins, so no tile
+    // reads a row another tile has already updated
+    const float2* I0 = (const float2*)(src + (int64_t)rep * a.g.N * 2);
+    const float2* I1 = I0 + a.g.half;
+    float2* G0 = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
+    float2* G1 = G0 + a.g.half;
+    // each thread keeps one column mm and visits rows tr0, tr0 + bq, ...
+    const int bq = (int)fdiv(blockDim.x, a.g.divM);
+    const int tr0 = (int)fdiv(threadIdx.x, a.g.divM);
+    const int mm = (int)threadIdx.x - tr0 * M;
+    const bool act = tr0 < bq;
+    const int dp1 = mm == M - 1 ? -(M - 1) : 1;  // x0 + 1 in the other plane, x0 odd
+    const int dn0 = mm == 0 ? M - 1 : -1;        // x0 - 1 in the other plane, x0 even
+    if (a.n_meas_rep && a.n_meas_rep[rep] <= 0) return;  // no measurement this call: state stays put
+    if (!upd && !msr) {  // whole block: replica done, rows carried over
+        if (act)
+            for (int r = tr0; r < Te; r += bq) {
+                G0[(int64_t)(y0 + r) * M + mm] = I0[(int64_t)(y0 + r) * M + mm];
+                G1[(int64_t)(y0 + r) * M + mm] = I1[(int64_t)(y0 + r) * M + mm];
+            }
+        return;
+    }
+    auto wrap = [&](int y) {
+        const int q = y % L1;
+        return q < 0 ? q + L1 : q;
+    };
+    if (act) {
+        for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = I0[(int64_t)wrap(y0 - 1 + r) * M + mm];
+        for (int r = tr0; r < R1; r += bq) P1[r * M + mm] = I1[(int64_t)wrap(y0 - 2 + r) * M + mm];
+    }
+    __syncthreads();
+    if (msr) {
+        if (act)
+            for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = unit_xy(P0[r * M + mm]);
+        __syncthreads();
+        const int own = Te * M;
+        const int blk0 = (int)((int64_t)y0 * M / ON_BLOCK);
+        const int w = threadIdx.x >> 6;
+        for (int base = 0; base < own; base += (int)blockDim.x) {
+            double v[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) v[q] = 0.0;
+            const int j = base + (int)threadIdx.x;  // own colour-0 site, row-major in the tile
+            if (j < own) {
+                const int rr = (int)fdiv((uint32_t)j, a.g.divM);
+                const int cm = j - rr * M;
+                const bool odd = ((y0 + rr) & 1) != 0;  // colour 0: x0 = 2 cm + 1 on odd rows
+                const int io = (rr + 2) * M + cm;       // the same lattice row and column in P1
+                const int dpj = cm == M - 1 ? -(M - 1) : 1, dnj = cm == 0 ? M - 1 : -1;
+                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dpj : 0)), (uint32_t)(io + (odd ? 0 : dnj)),
+                                         (uint32_t)(io + M), (uint32_t)(io - M)};
+                const float2 sv = P0[(rr + 1) * M + cm];
+                const Vec<2> sp{{sv.x, sv.y}};
+                on_c0_bonds<2, 2>(sp, (const float*)P1, nbo, v);
+            }
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const double t = wave_sum_f64(v[q]);
+                if ((threadIdx.x & 63) == 0) tred[q * 16 + w] = t;
+            }
+            __syncthreads();
+            // each four consecutive waves are one block of ON_BLOCK sites
+            const int g = (int)(threadIdx.x >> 8), q = (int)(threadIdx.x & 255);
+            if (q < K && base + g * ON_BLOCK < own) {
+                double t = 0.0;
+                for (int u = 0; u < 4; ++u) t += tred[q * 16 + g * 4 + u];
+                mt.part[((int64_t)rep * mt.nb + blk0 + base / ON_BLOCK + g) * K + q] = t;
+            }
+            __syncthreads();
+        }
+    }
+    int accepted = 0;
+    if (upd) {
+        const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+        const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+        const float beta = a.betas[rep], step = a.steps[rep];
+        if (act)  // colour 0 on every row held
+            for (int r = tr0; r < R0; r += bq) {
+                const int y = wrap(y0 - 1 + r);
+                const bool odd = (y & 1) != 0;
+                const int io = (r + 1) * M + mm;  // the same lattice row in P1
+                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dp1 : 0)), (uint32_t)(io + (odd ? 0 : dn0)),
+                                         (uint32_t)(io + M), (uint32_t)(io - M)};
+                const float2 sv = P0[r * M + mm];
+                Vec<2> sp{{sv.x, sv.y}};
+                int ac = 0;
+                if (on_site<2, 4, CODE>(sp, (const float*)P1, nbo, (uint32_t)(y * M + mm), 0, sweep, k0, k1,
+                                        sid, beta, step, ac))
+                    P0[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
+                if (r >= 1 && r <= Te) accepted += ac;
+            }
+        __syncthreads();
+        if (act)  // colour 1 on the own rows
+            for (int r = 2 + tr0; r < Te + 2; r += bq) {
+                const int y = y0 - 2 + r;
+                const bool odd = ((y + 1) & 1) != 0;
+                const int io = (r - 1) * M + mm;  // the same lattice row in P0
+                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dp1 : 0)), (uint32_t)(io + (odd ? 0 : dn0)),
+                                         (uint32_t)(io + M), (uint32_t)(io - M)};
+                const float2 sv = P1[r * M + mm];
+                Vec<2> sp{{sv.x, sv.y}};
+                int ac = 0;
+                const bool changed = on_site<2, 4, CODE>(sp, (const float*)P0, nbo, (uint32_t)(y * M + mm), 1,
+                                                         sweep, k0, k1, sid, beta, step, ac);
+                accepted += ac;
+                if (a.norm_out) {
+                    // the measurement's renormalisation, fused into the site's last update
+                    sp = unit_vec<2>(sp);
+                    P1[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
+                } else if (changed) {
+                    P1[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
+                }
+            }
+        __syncthreads();
+    }
+    if (act) {  // every own row of both colours (colour 1 unchanged when only measuring)
+        for (int r = 1 + tr0; r <= Te; r += bq) G0[(int64_t)(y0 + r - 1) * M + mm] = P0[r * M + mm];
+        for (int r = 2 + tr0; r < Te + 2; r += bq) G1[(int64_t)(y0 + r - 2) * M + mm] = P1[r * M + mm];
+    }
+    if (CODE == CODE_METROPOLIS && upd) {  // one atomic per workgroup
+        const int t = wave_sum(accepted);
+        if ((threadIdx.x & 63) == 0) ired[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int u = 0; u < (int)(blockDim.x >> 6); ++u) tot += ired[u];
+            if (tot) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], tot);
+        }
+    }
+    if (msr) {
+        // the replica's last workgroup adds the block partials in order
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) last = atomicAdd(&mt.ticket[rep], 1) == tiles - 1;
+        __syncthreads();
+        if (last) {
+            __threadfence();
+            if ((int)threadIdx.x < K)
+                on_final_sum<2, 2>(mt.part, mt.nb, a.R, a.n_meas, mt.meas, mt.e_out, mt.m_out, mt.sc_out,
+                                   mt.ss_out, a.n_meas_rep, rep * K + (int)threadIdx.x);
+            if (threadIdx.x == 0) mt.ticket[rep] = 0;
+        }
+    }
+}
+
+// The spins of every replica that made a measurement in this call, from the
+// buffer the last row-tile launch wrote back to the caller's (replicas with no
+// measurement never left it).
+__global__ void on_copy_back(float* __restrict__ home, const float* __restrict__ cur, int R,
+                             int64_t per_rep, const int64_t* __restrict__ n_meas_rep)
+{
+    const int rep = blockIdx.y;
+    if (rep >= R || (n_meas_rep && n_meas_rep[rep] <= 0)) return;
+    const float4* src = (const float4*)(cur + (int64_t)rep * per_rep);
+    float4* dst = (float4*)(home + (int64_t)rep * per_rep);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_rep / 4;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// LDS per workgroup of the one-launch sweep (76 KB: two workgroups per CU).
+constexpr int64_t ON_SWEEP_LDS = 76 * 1024;
+// Option "on_sweep_tile": the largest launch, in colour sites (replicas x
+// sites per colour), that runs a sweep as one row-tile launch instead of two
+// half-sweep launches (0: never, the default).  Measured on MI355X the tiles
+// lose: a few replicas give too few workgroups to fill the CUs, and at 256 x
+// 512^2 the Metropolis sweep ran at 181 against 218 site updates/ns.
+int64_t g_on_sweep_tile = 0;
+
+// Rows per tile of the one-launch sweep (0: the lattice does not qualify):
+// XY on a square lattice with rows of at most 1024 colour sites, T M a
+// multiple of ON_BLOCK (whole measurement blocks per tile) unless one tile
+// holds the lattice.
+static int on_sweep_tile_rows(const CbGeo& g, int n, int n_rep)
+{
+    if ((int64_t)n_rep * g.half > g_on_sweep_tile || n != 2 || g.d != 2 || g.M < 2 || g.M > 1024) return 0;
+    const int64_t tmax = (ON_SWEEP_LDS / ((int64_t)g.M * 8) - 6) / 2;
+    if (tmax >= g.L1) return g.L1;
+    int step = ON_BLOCK;  // rows per step: ON_BLOCK / gcd(M, ON_BLOCK)
+    for (int m = g.M; step > 1 && (m & 1) == 0; m >>= 1) step >>= 1;
+    const int64_t T = tmax / step * step;
+    return T >= 4 ? (int)T : 0;
+}
+
+// The workspace of cm_on_run: FP64 measurement partials (chunk partials of
+// on_measure_partial or colour-0 block partials, whichever are more), one
+// int32 ticket per replica (on_sweep_tile), and for XY on square lattices a
+// second spin buffer that the row-tile launches write into (they run out of
+// place, alternating between the caller's buffer and this one).
+struct OnWs {
+    int64_t tickets, alt, total;
+};
+static OnWs on_ws_layout(int64_t N, int n_rep, int n, int d)
+{
+    const int64_t nchunk = (N + ON_CHUNK - 1) / ON_CHUNK;
+    const int64_t nblk = (N / 2 + ON_BLOCK - 1) / ON_BLOCK;
+    OnWs w;
+    w.tickets = (int64_t)n_rep * std::max(nchunk, nblk) * (1 + n + 2 * d) * 8;
+    w.alt = (w.tickets + (int64_t)n_rep * 4 + 255) / 256 * 256;
+    w.total = w.alt + (n == 2 && d == 2 ? (int64_t)n_rep * N * n * 4 : 0);
+    return w;
 }
 
 // Renormalise every spin in FP64 (critmc/kernels/on.py:78-87).
@@ -1461,9 +1723,7 @@ int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep, 
 {
     int64_t N = 1;
     for (int a = 0; a < ndim; ++a) N *= dims[a];
-    const int64_t nchunk = (N + ON_CHUNK - 1) / ON_CHUNK;
-    const int64_t nblk = (N / 2 + ON_BLOCK - 1) / ON_BLOCK;  // colour-0 block partials
-    return (int64_t)n_rep * std::max(nchunk, nblk) * (1 + n + 2 * ndim) * 8 + (int64_t)n_rep * 8;
+    return on_ws_layout(N, n_rep, n, ndim).total;
 }
 
 int64_t cm_on_cluster_workspace_bytes(int32_t ndim, const int32_t* dims, int32_t n_rep)
@@ -1598,6 +1858,23 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_or_fused<ON_OR_FUSE>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
+    // one launch per Metropolis or over-relaxation sweep (on_sweep_tile)
+    const int st_rows = any_cluster ? 0 : on_sweep_tile_rows(g, n, n_rep);
+    const size_t st_lds = (size_t)(2 * st_rows + 6) * g.M * 8;
+    const OnWs wl = on_ws_layout(g.N, n_rep, n, g.d);
+    int* tickets = (int*)((char*)workspace + wl.tickets);
+    // the row-tile launches write out of place: `cur` holds the spins between launches
+    float* const home = (float*)spins;
+    float* const alt = n == 2 && g.d == 2 ? (float*)((char*)workspace + wl.alt) : nullptr;
+    float* cur = home;
+    auto other = [&](const float* p) { return p == home ? alt : home; };
+    if (st_rows > 0) {
+        CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_sweep_tile<CODE_METROPOLIS>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)st_lds));
+        CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_sweep_tile<CODE_OVERRELAX>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)st_lds));
+        hipLaunchKernelGGL(fill_i32_on, dim3((n_rep + 255) / 256), dim3(256), 0, st, tickets, (int64_t)n_rep);
+    }
     // Deferred measurements (OnDefer): when a measurement ends with a plain
     // colour-1 half sweep and the next one starts with a plain colour-0 half
     // sweep, that colour-0 launch renormalises and measures the previous
@@ -1620,6 +1897,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
             const int code = (int)schedule[2 * r];
             if (code == CODE_WOLFF || code == CODE_SWENDSEN_WANG) {
                 ca.sweep_add = sw;
+                ca.spins = cur;
                 if (n == 2) on_cluster_row<2>(ca, code, schedule[2 * r + 1], st);
                 else on_cluster_row<3>(ca, code, schedule[2 * r + 1], st);
                 sw += (uint64_t)schedule[2 * r + 1];
@@ -1637,10 +1915,32 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                     // (the workspace holds nchunk >= tiles partials per replica)
                     double* tpart = a.norm_out && tiles <= nchunk ? part : nullptr;
                     if (tpart) meas_tiles = tiles;
+                    a.spins = other(cur);
                     hipLaunchKernelGGL((on_or_fused<ON_OR_FUSE>), dim3((unsigned)(tiles * n_rep)),
-                                       dim3(1024), lds, st, a, or_rows, tpart);
+                                       dim3(1024), lds, st, a, (const float*)cur, or_rows, tpart);
+                    cur = a.spins;
                     sw += 2;
                     rp += 2;
+                    continue;
+                }
+                if (st_rows > 0 && (code == CODE_METROPOLIS || code == CODE_OVERRELAX)) {
+                    // the whole sweep in one launch; the measurement's first sweep
+                    // also carries the previous measurement
+                    a.norm_out = fused && r == n_rows - 1 && rp == count - 1;
+                    OnTile mt{};
+                    if (r == 0 && rp == 0 && df.part)
+                        mt = OnTile{part, meas - 1, nb0, tickets, e_out, m_out, sc_out, ss_out};
+                    const int tiles = (g.L1 + st_rows - 1) / st_rows;
+                    a.spins = other(cur);
+                    if (code == CODE_METROPOLIS)
+                        hipLaunchKernelGGL(on_sweep_tile<CODE_METROPOLIS>, dim3((unsigned)(tiles * n_rep)),
+                                           dim3(1024), st_lds, st, a, (const float*)cur, st_rows, mt);
+                    else
+                        hipLaunchKernelGGL(on_sweep_tile<CODE_OVERRELAX>, dim3((unsigned)(tiles * n_rep)),
+                                           dim3(1024), st_lds, st, a, (const float*)cur, st_rows, mt);
+                    cur = a.spins;
+                    ++sw;
+                    ++rp;
                     continue;
                 }
                 for (int c = 0; c < 2; ++c) {
@@ -1669,19 +1969,23 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         } else if (defer && meas + 1 < n_meas) {
             // measured by the next measurement's first sweep
         } else if (fused) {
-            if (n == 2) launch_measure_c0<2>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
-            else launch_measure_c0<3>((float*)spins, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+            if (n == 2) launch_measure_c0<2>(cur, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+            else launch_measure_c0<3>(cur, g, n_rep, part, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         } else if (n == 2) {
-            hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
-            launch_measure<2>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+            hipLaunchKernelGGL(on_normalize<2>, dim3(nb), dim3(256), 0, st, cur, sites, g.N, n_meas_rep, meas);
+            launch_measure<2>((const float*)cur, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         } else {
-            hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, (float*)spins, sites, g.N, n_meas_rep, meas);
-            launch_measure<3>((const float*)spins, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
+            hipLaunchKernelGGL(on_normalize<3>, dim3(nb), dim3(256), 0, st, cur, sites, g.N, n_meas_rep, meas);
+            launch_measure<3>((const float*)cur, g, n_rep, part, nchunk, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep, st);
         }
         if (aux_out)
             hipLaunchKernelGGL(on_cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas,
                                acc_out, ca.wsize, ca.stats, any_sw ? 1 : 0, aux_out);
     }
+    if (cur != home)  // the last row-tile launch wrote the second buffer
+        hipLaunchKernelGGL(on_copy_back, dim3((unsigned)std::min<int64_t>((g.N * n / 4 + 255) / 256, 256),
+                                              (unsigned)n_rep),
+                           dim3(256), 0, st, home, (const float*)cur, n_rep, g.N * n, n_meas_rep);
     int64_t per_meas = 0;
     for (int r = 0; r < n_rows; ++r) per_meas += schedule[2 * r + 1];
     hipLaunchKernelGGL(on_advance_sweeps, dim3((n_rep + 255) / 256), dim3(256), 0, st, sweeps,

@@ -77,10 +77,10 @@ class Job:
 
 
 # Largest Potts replica, in states, that a serving pool runs in the per-replica
-# form (one workgroup per replica, the lattice in LDS): up to 256^2, whose
-# sweep on one CU takes about as long as the two launches per sweep of the wide
-# form cost the device in a busy stream.
-POTTS_RESIDENT_STATES = 256 * 256
+# form (one workgroup per replica, the lattice in LDS): each replica is one
+# workgroup, so its sweep takes longer as L grows; measured on the 10^4-job
+# stream, 256^2 pools did better with the launch per half sweep.
+POTTS_RESIDENT_STATES = 128 * 128
 
 
 def _ising_resident_ok(L: int) -> bool:
@@ -100,7 +100,7 @@ class _Pool:
         # sweep the automatic choice makes for few replicas of a large lattice:
         # a serving stream is many small pools sharing the GPU, and the host's
         # kernel launches, not CUs per pool, set its pace.  Ising: wherever the
-        # lattice fits; Potts: up to 256^2 (see POTTS_RESIDENT_STATES).
+        # lattice fits; Potts: up to 128^2 (see POTTS_RESIDENT_STATES).
         self.resident = resident
         self.cap = 0
         self.jobs: list[Job | None] = []
