@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: arrivals per second (0: 70%% of capacity)")
     ap.add_argument("--slice-seconds", type=float, default=0.0,
                     help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
+    ap.add_argument("--launch-threads", type=int, default=4,
+                    help="serve: host threads that enqueue the pools' slices (0: the serving thread)")
     ap.add_argument("--no-resident", action="store_true",
                     help="serve: leave every pool's launch form to the kernels' automatic choice "
                          "instead of one launch per slice wherever the lattice fits on chip")
@@ -333,7 +335,7 @@ def bench_serve(args) -> dict:
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
     srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds,
-                 resident=not args.no_resident)
+                 resident=not args.no_resident, launch_threads=args.launch_threads)
     for j in jobs:
         srv.submit(j)
     torch.cuda.synchronize()
@@ -350,7 +352,8 @@ def bench_serve(args) -> dict:
                        "arrival_rate_per_s": rate, "offered_site_updates_per_ns":
                        round(rate * work / len(jobs) / 1e9, 2), "budget_GB": 200,
                        "slice_sweeps": 1000, "slice_seconds": args.slice_seconds,
-                       "resident_forms": not args.no_resident},
+                       "resident_forms": not args.no_resident,
+                       "launch_threads": args.launch_threads},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
             "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,
