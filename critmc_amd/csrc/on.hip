@@ -1080,8 +1080,28 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
 // 256 lanes, group j taking block j of every NG = BLK / 256 consecutive
 // blocks exactly as one on_measure_c0 workgroup takes it, and the block
 // partials are added in block order, so the bits do not depend on BLK.
-// Up to 158 KB of spins (XY 128^2 at 128 KB): one workgroup per CU.
+// Up to 60 KB of spins any number of replicas take this form (two or more
+// workgroups per CU); up to 158 KB (XY 128^2 at 128 KB, one workgroup per CU)
+// only as many replicas as the GPU has CUs.  More such replicas would hold
+// every CU for several rounds of a whole call and starve work on other streams
+// (the all-queued serving stream lost about 10 % that way); they take the
+// half-sweep launches instead.
+constexpr int64_t ON_BATCHED_LDS_ANY = 60 * 1024;
 constexpr int64_t ON_BATCHED_LDS = 158 * 1024;
+
+// CUs of the current device (cached per device).
+static int on_cu_count()
+{
+    static int cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (cached[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            cached[dev] = v;
+    }
+    return cached[dev];
+}
 
 template <int NC, int D, bool HB, int BLK>
 __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __restrict__ e_out,
@@ -1822,7 +1842,9 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                        dim3(256), 0, st, acc_out, total);
     OnArgs a{(float*)spins, betas_dev, steps_dev, keys, stream_ids, sweeps, acc_out, n_meas_rep,
              g, n_rep, n_meas, 0, 0, 0, 0};
-    if (!any_cluster && aux_out == nullptr && g.N * n * 4 <= ON_BATCHED_LDS) {
+    const int64_t rep_bytes = g.N * n * 4;
+    if (!any_cluster && aux_out == nullptr &&
+        (rep_bytes <= ON_BATCHED_LDS_ANY || (rep_bytes <= ON_BATCHED_LDS && n_rep <= on_cu_count()))) {
         // the whole call in one launch, each replica in LDS
         Sched sc{};
         sc.n = n_rows;

@@ -81,6 +81,9 @@ class Job:
 # workgroup, so its sweep takes longer as L grows; measured on the 10^4-job
 # stream, 256^2 pools did better with the launch per half sweep.
 POTTS_RESIDENT_STATES = 128 * 128
+# Pools of more jobs than this leave the form to the kernels' automatic choice,
+# which is tuned for throughput over many replicas.
+RESIDENT_MAX_JOBS = 256
 
 
 def _ising_resident_ok(L: int) -> bool:
@@ -265,13 +268,15 @@ class _Pool:
             sched = np.array(SCHEDULE[self.model])
             shape = (self.L, self.L)
             if self.model == "ising":
-                form = (k_ising.FORM_BATCHED if self.resident and _ising_resident_ok(self.L)
+                small = self.active <= RESIDENT_MAX_JOBS
+                form = (k_ising.FORM_BATCHED if self.resident and small and _ising_resident_ok(self.L)
                         else k_ising.FORM_AUTO)
                 out = k_ising.run(st["spins"], shape, self.betas, 0.0, ch, sched, n, st["e"], st["m"],
                                   form=form, ws=self.ws, n_meas_rep=caps_d)
                 rec = {"E": out[0], "M": out[1]}
             elif self.model == "potts":
-                grid = self.cap if self.resident and self.L * self.L <= POTTS_RESIDENT_STATES else 0
+                grid = (self.cap if self.resident and self.active <= RESIDENT_MAX_JOBS
+                        and self.L * self.L <= POTTS_RESIDENT_STATES else 0)
                 e, _, counts = k_potts.run(st["spins"], shape, self.q, self.betas, ch, sched, n,
                                            st["e"], st["counts"], grid=grid, ws=self.ws,
                                            n_meas_rep=caps_d)
