@@ -318,7 +318,7 @@ def bench_xy(args) -> dict:
 # Site updates/ns of the 10^4-job mix on one MI355X with every job queued at
 # once (bench.py --workload serve --rate 1e9); the default arrival rate is 70 %
 # of it.
-SERVE_CAPACITY = 138.1
+SERVE_CAPACITY = 184.5
 
 
 def bench_serve(args) -> dict:

@@ -855,7 +855,6 @@ int64_t g_graph_min_meas = 0;
 
 // defined with the tempering kernels
 extern int64_t g_ea_fused, g_ea_order;
-extern int64_t g_on_sweep_tile;
 
 namespace {
 struct Retired {
@@ -1339,7 +1338,6 @@ int cm_set_option(const char* name, int64_t value, int64_t* previous)
     else if (std::strcmp(name, "cluster_lds_max_sites") == 0) slot = &g_cluster_lds_max_sites;
     else if (std::strcmp(name, "ea_fused") == 0) slot = &g_ea_fused;
     else if (std::strcmp(name, "ea_order") == 0) slot = &g_ea_order;
-    else if (std::strcmp(name, "on_sweep_tile") == 0) slot = &g_on_sweep_tile;
     CM_REQUIRE(slot != nullptr, "unknown option %s", name);
     if (previous) *previous = *slot;
     *slot = value;

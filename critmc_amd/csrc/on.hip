@@ -26,7 +26,7 @@
 // When the schedule ends with a local update, that last half sweep writes its
 // sites renormalised and colour 0 is renormalised while every bond is summed
 // once: by the next measurement's first colour-0 half sweep, which then
-// updates those sites (OnDefer), or by on_measure_c0 for a call's last
+// updates those sites (on_c0_sweep_measure), or by on_measure_c0 for a call's last
 // measurement (when it ends with two fused over-relaxation sweeps, the fused
 // launch does both); otherwise a normalise launch precedes the partial-sum
 // launch.  Partial sums are finalised in order, so every FP64 result is
@@ -77,8 +77,7 @@
 
 namespace cm {
 
-constexpr int ON_CHUNK = 4096;  // sites per partial of on_measure_partial (site order)
-constexpr int ON_BLOCK = 256;   // colour-0 sites per partial of every colour-0 measurement
+constexpr int ON_CHUNK = 4096;  // sites per measurement partial
 
 template <int NC>
 struct Vec {
@@ -365,31 +364,46 @@ template <int NC, int D>
 __device__ __forceinline__ void on_c0_bonds(const Vec<NC>& s, const float* __restrict__ S1,
                                             const uint32_t (&nbo)[2 * D], double (&v)[1 + NC + 2 * D]);
 
-// The ordered sum of one quantity of one replica's colour-0 block partials
-// (thread t of R K: replica t / K, quantity t % K), written to its output
-// series.  The loads of eight partials are issued before their adds.
+// The ordered sum of one quantity of one replica's colour-0 block partials,
+// taken by one wave (pair = replica * K + quantity, wave-uniform) and written
+// to its output series: the lanes load 512 partials at a time in parallel and
+// the wave adds them in block order from lane broadcasts, so the sum is the
+// plain left-to-right one without a chain of dependent loads.
 template <int NC, int D>
-__device__ __forceinline__ void on_final_sum(const double* __restrict__ part, int nb, int R,
-                                             int64_t n_meas, int64_t meas, double* __restrict__ e_out,
-                                             double* __restrict__ m_out, double* __restrict__ sc_out,
-                                             double* __restrict__ ss_out,
-                                             const int64_t* __restrict__ n_meas_rep, int t)
+__device__ __forceinline__ void on_final_pair(const double* __restrict__ part, int nb, int R,
+                                              int64_t n_meas, int64_t meas, double* __restrict__ e_out,
+                                              double* __restrict__ m_out, double* __restrict__ sc_out,
+                                              double* __restrict__ ss_out,
+                                              const int64_t* __restrict__ n_meas_rep, int pair)
 {
     constexpr int K = 1 + NC + 2 * D;
-    if (t >= R * K) return;
-    const int rep = t / K, q = t - (t / K) * K;
+    if (pair >= R * K) return;
+    const int rep = pair / K, q = pair - (pair / K) * K;
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;
+    const int lane = (int)(threadIdx.x & 63);
     const double* pp = part + (int64_t)rep * nb * K + q;
     double s = 0.0;
-    int c = 0;
-    for (; c + 8 <= nb; c += 8) {
+    for (int c0 = 0; c0 < nb; c0 += 512) {
         double v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = pp[(int64_t)(c + u) * K];
+        for (int j = 0; j < 8; ++j) {
+            const int c = c0 + 64 * j + lane;
+            v[j] = c < nb ? pp[(int64_t)c * K] : 0.0;
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int j = 0; j < 8; ++j) {
+            const int lim = nb - c0 - 64 * j;  // partials of this group of 64 (wave-uniform)
+#pragma unroll
+            for (int l = 0; l < 64; ++l) {
+                if (l >= lim) break;
+                const uint64_t bits = (uint64_t)__double_as_longlong(v[j]);
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, l);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), l);
+                s += __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+            }
+        }
     }
-    for (; c < nb; ++c) s += pp[(int64_t)c * K];
+    if (lane != 0) return;
     const int64_t o = (int64_t)rep * n_meas + meas;
     if (q == 0) e_out[o] = s;
     else if (q < 1 + NC) m_out[o * NC + (q - 1)] = s;
@@ -417,45 +431,38 @@ __device__ __forceinline__ void on_block_partial(const double (&v)[K], double* _
     }
 }
 
-// A measurement carried by the next sweep's launches (local schedules whose
-// measurement ends with a colour-1 half sweep): the first colour-0 half sweep
-// of measurement m renormalises its sites and forms measurement m-1's block
-// partials before updating them (the reference renormalises at the
-// measurement and starts the next sweep from those spins), and the colour-1
-// half sweep that follows adds them up.
-struct OnDefer {
-    double* part;         // colour-0 launch: partials of measurement `meas` (null: none)
+// The ordered sums of a measurement whose chunk partials an earlier launch
+// formed (on_c0_sweep_measure), taken by the next half-sweep launch in an
+// extra row of workgroups: one wave per (replica, quantity).
+struct OnFinal {
+    const double* part;  // null: none
     int64_t meas;
-    const double* fpart;  // ordered sums of measurement `fmeas` from these partials (null: none)
-    int64_t fmeas;
-    int nb;               // block partials per replica
+    int nchunk;          // partials per replica
     double* e_out;
     double* m_out;
     double* sc_out;
     double* ss_out;
 };
 
-template <int NC, int Z, int CODE, bool DEFER>
-__global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnDefer df)
+template <int NC, int Z, int CODE, bool FIN>
+__global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnFinal fin)
 {
     constexpr int D = Z / 2, K = 1 + NC + Z;
     const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int rep = (int)(lb / gridDim.x);
     const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
-    if constexpr (DEFER) {
-        if (df.fpart)  // one thread per (replica, quantity), before any replica's early exit
-            on_final_sum<NC, D>(df.fpart, df.nb, a.R, a.n_meas, df.fmeas, df.e_out, df.m_out,
-                                df.sc_out, df.ss_out, a.n_meas_rep, (int)(lb * blockDim.x + threadIdx.x));
+    if constexpr (FIN) {
+        if (rep == a.R) {  // the extra row of workgroups
+            const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+            for (int pr = (int)(bx * (blockDim.x >> 6) + (threadIdx.x >> 6)); pr < a.R * K; pr += waves)
+                on_final_pair<NC, D>(fin.part, fin.nchunk, a.R, a.n_meas, fin.meas, fin.e_out, fin.m_out,
+                                     fin.sc_out, fin.ss_out, a.n_meas_rep, pr);
+            return;
+        }
     }
-    const bool upd = !(a.n_meas_rep && a.meas >= a.n_meas_rep[rep]);
-    bool msr = false;
-    if constexpr (DEFER) msr = df.part && !(a.n_meas_rep && df.meas >= a.n_meas_rep[rep]);
-    if (!upd && !msr) return;  // whole block: replica done
+    if (a.n_meas_rep && a.meas >= a.n_meas_rep[rep]) return;  // whole block: replica done
     const int64_t k = (int64_t)bx * blockDim.x + threadIdx.x;
     int accepted = 0;
-    double v[K];
-#pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = 0.0;
     if (k < a.g.half) {
         float* S = a.spins + (int64_t)rep * a.g.N * NC;
         uint32_t nbo[Z];
@@ -463,18 +470,11 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnDefer df)
         const int64_t i = (int64_t)a.colour * a.g.half + k;            // own plane
         const float* So = S + (int64_t)(1 - a.colour) * a.g.half * NC;  // neighbours' plane
         Vec<NC> s = load_vec<NC>(S, i);
-        bool changed = false;
-        if (DEFER && msr) {  // colour 0: the previous measurement's renormalisation and bonds
-            s = unit_vec<NC>(s);
-            on_c0_bonds<NC, D>(s, So, nbo, v);
-            changed = true;
-        }
-        if (upd)
-            changed |= on_site<NC, Z, CODE>(s, So, nbo, (uint32_t)k, a.colour,
-                                            a.sweeps[rep] + a.sweep_add, a.keys[2 * rep],
-                                            a.keys[2 * rep + 1], a.sids[rep], a.betas[rep],
-                                            a.steps[rep], accepted);
-        if (upd && a.norm_out) {
+        const bool changed = on_site<NC, Z, CODE>(s, So, nbo, (uint32_t)k, a.colour,
+                                                  a.sweeps[rep] + a.sweep_add, a.keys[2 * rep],
+                                                  a.keys[2 * rep + 1], a.sids[rep], a.betas[rep],
+                                                  a.steps[rep], accepted);
+        if (a.norm_out) {
             // the measurement's renormalisation of this site, fused into its last update
             s = unit_vec<NC>(s);
             store_vec<NC>(S, i, s);
@@ -482,13 +482,6 @@ __global__ __launch_bounds__(256) void on_half_sweep(OnArgs a, OnDefer df)
             store_vec<NC>(S, i, s);
         }
     }
-    if constexpr (DEFER) {
-        if (msr) {
-            __shared__ double mred[K * 4];
-            on_block_partial<K>(v, mred, df.part + ((int64_t)rep * df.nb + bx) * K);
-        }
-    }
-    if (!upd) return;
     if (CODE == CODE_METROPOLIS) {  // one atomic per workgroup
         __shared__ int part[4];
         const int t = wave_sum_bits<1>((uint32_t)accepted);
@@ -668,198 +661,6 @@ static int on_or_tile_rows(const CbGeo& g, int n)
     return (int)std::min<int64_t>(T, g.L1);
 }
 
-// One sweep of XY spins on a square lattice in one launch (colour 0, then
-// colour 1), over row tiles in LDS, for any local update code.  A 1024-lane
-// workgroup owns rows [y0, y0 + Te) of one replica: it holds colour 0 on rows
-// y0-1 .. y0+Te and colour 1 on rows y0-2 .. y0+Te+1, updates colour 0 on
-// every row it holds (the two halo rows repeat the neighbouring tiles'
-// updates: the same inputs and the same Philox block give the same result)
-// and colour 1 on its own rows.  Every update is on_site, so the spins equal
-// two half-sweep launches bit for bit.
-//
-// With mt.part set it first carries the previous measurement, as the
-// colour-0 half sweep does (OnDefer): colour 0 renormalised on every row it
-// holds, then one partial per block of 256 own colour-0 sites, one site per
-// lane exactly as on_measure_c0 forms it (T M is a multiple of 256, so blocks
-// do not straddle tiles), and the replica's last workgroup to finish (a ticket
-// per replica) adds the blocks in order.  The results equal on_measure_c0's
-// and on_measure_final's bit for bit.
-struct OnTile {
-    double* part;  // partials of measurement `meas` (null: no measurement in this launch)
-    int64_t meas;
-    int nb;        // block partials per replica
-    int* ticket;   // [R] workgroups done with the measurement; zero between launches
-    double* e_out;
-    double* m_out;
-    double* sc_out;
-    double* ss_out;
-};
-
-template <int CODE>
-__global__ __launch_bounds__(1024) void on_sweep_tile(OnArgs a, const float* __restrict__ src, int T,
-                                                      OnTile mt)
-{
-    constexpr int K = 1 + 2 + 4;  // E, M[2], c_a[2], s_a[2]
-    extern __shared__ float2 stl[];
-    __shared__ double tred[K * 16];
-    __shared__ int ired[16];
-    __shared__ int last;
-    const int M = a.g.M, L1 = a.g.L1;
-    const int tiles = (L1 + T - 1) / T;
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const int rep = (int)(lb / (uint32_t)tiles), tile = (int)(lb - (uint32_t)rep * tiles);
-    const bool upd = !(a.n_meas_rep && a.meas >= a.n_meas_rep[rep]);
-    const bool msr = mt.part && !(a.n_meas_rep && mt.meas >= a.n_meas_rep[rep]);
-    const int y0 = tile * T;
-    const int Te = min(T, L1 - y0);
-    const int R0 = Te + 2, R1 = Te + 4;
-    float2* P0 = stl;                    // colour 0, LDS row r = lattice row y0 - 1 + r
-    float2* P1 = stl + (size_t)R0 * M;   // colour 1, LDS row r = lattice row y0 - 2 + r
-    // out of place: rows and halos from `src`, own rows to a.spins, so no tile
-    // reads a row another tile has already updated
-    const float2* I0 = (const float2*)(src + (int64_t)rep * a.g.N * 2);
-    const float2* I1 = I0 + a.g.half;
-    float2* G0 = (float2*)(a.spins + (int64_t)rep * a.g.N * 2);
-    float2* G1 = G0 + a.g.half;
-    // each thread keeps one column mm and visits rows tr0, tr0 + bq, ...
-    const int bq = (int)fdiv(blockDim.x, a.g.divM);
-    const int tr0 = (int)fdiv(threadIdx.x, a.g.divM);
-    const int mm = (int)threadIdx.x - tr0 * M;
-    const bool act = tr0 < bq;
-    const int dp1 = mm == M - 1 ? -(M - 1) : 1;  // x0 + 1 in the other plane, x0 odd
-    const int dn0 = mm == 0 ? M - 1 : -1;        // x0 - 1 in the other plane, x0 even
-    if (a.n_meas_rep && a.n_meas_rep[rep] <= 0) return;  // no measurement this call: state stays put
-    if (!upd && !msr) {  // whole block: replica done, rows carried over
-        if (act)
-            for (int r = tr0; r < Te; r += bq) {
-                G0[(int64_t)(y0 + r) * M + mm] = I0[(int64_t)(y0 + r) * M + mm];
-                G1[(int64_t)(y0 + r) * M + mm] = I1[(int64_t)(y0 + r) * M + mm];
-            }
-        return;
-    }
-    auto wrap = [&](int y) {
-        const int q = y % L1;
-        return q < 0 ? q + L1 : q;
-    };
-    if (act) {
-        for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = I0[(int64_t)wrap(y0 - 1 + r) * M + mm];
-        for (int r = tr0; r < R1; r += bq) P1[r * M + mm] = I1[(int64_t)wrap(y0 - 2 + r) * M + mm];
-    }
-    __syncthreads();
-    if (msr) {
-        if (act)
-            for (int r = tr0; r < R0; r += bq) P0[r * M + mm] = unit_xy(P0[r * M + mm]);
-        __syncthreads();
-        const int own = Te * M;
-        const int blk0 = (int)((int64_t)y0 * M / ON_BLOCK);
-        const int w = threadIdx.x >> 6;
-        for (int base = 0; base < own; base += (int)blockDim.x) {
-            double v[K];
-#pragma unroll
-            for (int q = 0; q < K; ++q) v[q] = 0.0;
-            const int j = base + (int)threadIdx.x;  // own colour-0 site, row-major in the tile
-            if (j < own) {
-                const int rr = (int)fdiv((uint32_t)j, a.g.divM);
-                const int cm = j - rr * M;
-                const bool odd = ((y0 + rr) & 1) != 0;  // colour 0: x0 = 2 cm + 1 on odd rows
-                const int io = (rr + 2) * M + cm;       // the same lattice row and column in P1
-                const int dpj = cm == M - 1 ? -(M - 1) : 1, dnj = cm == 0 ? M - 1 : -1;
-                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dpj : 0)), (uint32_t)(io + (odd ? 0 : dnj)),
-                                         (uint32_t)(io + M), (uint32_t)(io - M)};
-                const float2 sv = P0[(rr + 1) * M + cm];
-                const Vec<2> sp{{sv.x, sv.y}};
-                on_c0_bonds<2, 2>(sp, (const float*)P1, nbo, v);
-            }
-#pragma unroll
-            for (int q = 0; q < K; ++q) {
-                const double t = wave_sum_f64(v[q]);
-                if ((threadIdx.x & 63) == 0) tred[q * 16 + w] = t;
-            }
-            __syncthreads();
-            // each four consecutive waves are one block of ON_BLOCK sites
-            const int g = (int)(threadIdx.x >> 8), q = (int)(threadIdx.x & 255);
-            if (q < K && base + g * ON_BLOCK < own) {
-                double t = 0.0;
-                for (int u = 0; u < 4; ++u) t += tred[q * 16 + g * 4 + u];
-                mt.part[((int64_t)rep * mt.nb + blk0 + base / ON_BLOCK + g) * K + q] = t;
-            }
-            __syncthreads();
-        }
-    }
-    int accepted = 0;
-    if (upd) {
-        const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
-        const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
-        const float beta = a.betas[rep], step = a.steps[rep];
-        if (act)  // colour 0 on every row held
-            for (int r = tr0; r < R0; r += bq) {
-                const int y = wrap(y0 - 1 + r);
-                const bool odd = (y & 1) != 0;
-                const int io = (r + 1) * M + mm;  // the same lattice row in P1
-                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dp1 : 0)), (uint32_t)(io + (odd ? 0 : dn0)),
-                                         (uint32_t)(io + M), (uint32_t)(io - M)};
-                const float2 sv = P0[r * M + mm];
-                Vec<2> sp{{sv.x, sv.y}};
-                int ac = 0;
-                if (on_site<2, 4, CODE>(sp, (const float*)P1, nbo, (uint32_t)(y * M + mm), 0, sweep, k0, k1,
-                                        sid, beta, step, ac))
-                    P0[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
-                if (r >= 1 && r <= Te) accepted += ac;
-            }
-        __syncthreads();
-        if (act)  // colour 1 on the own rows
-            for (int r = 2 + tr0; r < Te + 2; r += bq) {
-                const int y = y0 - 2 + r;
-                const bool odd = ((y + 1) & 1) != 0;
-                const int io = (r - 1) * M + mm;  // the same lattice row in P0
-                const uint32_t nbo[4] = {(uint32_t)(io + (odd ? dp1 : 0)), (uint32_t)(io + (odd ? 0 : dn0)),
-                                         (uint32_t)(io + M), (uint32_t)(io - M)};
-                const float2 sv = P1[r * M + mm];
-                Vec<2> sp{{sv.x, sv.y}};
-                int ac = 0;
-                const bool changed = on_site<2, 4, CODE>(sp, (const float*)P0, nbo, (uint32_t)(y * M + mm), 1,
-                                                         sweep, k0, k1, sid, beta, step, ac);
-                accepted += ac;
-                if (a.norm_out) {
-                    // the measurement's renormalisation, fused into the site's last update
-                    sp = unit_vec<2>(sp);
-                    P1[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
-                } else if (changed) {
-                    P1[r * M + mm] = make_float2(sp.v[0], sp.v[1]);
-                }
-            }
-        __syncthreads();
-    }
-    if (act) {  // every own row of both colours (colour 1 unchanged when only measuring)
-        for (int r = 1 + tr0; r <= Te; r += bq) G0[(int64_t)(y0 + r - 1) * M + mm] = P0[r * M + mm];
-        for (int r = 2 + tr0; r < Te + 2; r += bq) G1[(int64_t)(y0 + r - 2) * M + mm] = P1[r * M + mm];
-    }
-    if (CODE == CODE_METROPOLIS && upd) {  // one atomic per workgroup
-        const int t = wave_sum(accepted);
-        if ((threadIdx.x & 63) == 0) ired[threadIdx.x >> 6] = t;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int u = 0; u < (int)(blockDim.x >> 6); ++u) tot += ired[u];
-            if (tot) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], tot);
-        }
-    }
-    if (msr) {
-        // the replica's last workgroup adds the block partials in order
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) last = atomicAdd(&mt.ticket[rep], 1) == tiles - 1;
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            if ((int)threadIdx.x < K)
-                on_final_sum<2, 2>(mt.part, mt.nb, a.R, a.n_meas, mt.meas, mt.e_out, mt.m_out, mt.sc_out,
-                                   mt.ss_out, a.n_meas_rep, rep * K + (int)threadIdx.x);
-            if (threadIdx.x == 0) mt.ticket[rep] = 0;
-        }
-    }
-}
-
 // The spins of every replica that made a measurement in this call, from the
 // buffer the last row-tile launch wrote back to the caller's (replicas with no
 // measurement never left it).
@@ -875,45 +676,18 @@ __global__ void on_copy_back(float* __restrict__ home, const float* __restrict__
         dst[i] = src[i];
 }
 
-// LDS per workgroup of the one-launch sweep (76 KB: two workgroups per CU).
-constexpr int64_t ON_SWEEP_LDS = 76 * 1024;
-// Option "on_sweep_tile": the largest launch, in colour sites (replicas x
-// sites per colour), that runs a sweep as one row-tile launch instead of two
-// half-sweep launches (0: never, the default).  Measured on MI355X the tiles
-// lose: a few replicas give too few workgroups to fill the CUs, and at 256 x
-// 512^2 the Metropolis sweep ran at 181 against 218 site updates/ns.
-int64_t g_on_sweep_tile = 0;
-
-// Rows per tile of the one-launch sweep (0: the lattice does not qualify):
-// XY on a square lattice with rows of at most 1024 colour sites, T M a
-// multiple of ON_BLOCK (whole measurement blocks per tile) unless one tile
-// holds the lattice.
-static int on_sweep_tile_rows(const CbGeo& g, int n, int n_rep)
-{
-    if ((int64_t)n_rep * g.half > g_on_sweep_tile || n != 2 || g.d != 2 || g.M < 2 || g.M > 1024) return 0;
-    const int64_t tmax = (ON_SWEEP_LDS / ((int64_t)g.M * 8) - 6) / 2;
-    if (tmax >= g.L1) return g.L1;
-    int step = ON_BLOCK;  // rows per step: ON_BLOCK / gcd(M, ON_BLOCK)
-    for (int m = g.M; step > 1 && (m & 1) == 0; m >>= 1) step >>= 1;
-    const int64_t T = tmax / step * step;
-    return T >= 4 ? (int)T : 0;
-}
-
-// The workspace of cm_on_run: FP64 measurement partials (chunk partials of
-// on_measure_partial or colour-0 block partials, whichever are more), one
-// int32 ticket per replica (on_sweep_tile), and for XY on square lattices a
-// second spin buffer that the row-tile launches write into (they run out of
-// place, alternating between the caller's buffer and this one).
+// The workspace of cm_on_run: FP64 measurement partials (ON_CHUNK-site chunk
+// partials), then for XY on square lattices a second spin buffer that the
+// fused over-relaxation launch writes into (it runs out of place, alternating
+// between the caller's buffer and this one).
 struct OnWs {
-    int64_t tickets, alt, total;
+    int64_t alt, total;
 };
 static OnWs on_ws_layout(int64_t N, int n_rep, int n, int d)
 {
     const int64_t nchunk = (N + ON_CHUNK - 1) / ON_CHUNK;
-    const int64_t nblk = (N / 2 + ON_BLOCK - 1) / ON_BLOCK;
     OnWs w;
-    w.tickets = (int64_t)n_rep * std::max(nchunk, nblk) * (1 + n + 2 * d) * 8;
-    w.alt = (w.tickets + (int64_t)n_rep * 4 + 255) / 256 * 256;
+    w.alt = ((int64_t)n_rep * nchunk * (1 + n + 2 * d) * 8 + 255) / 256 * 256;
     w.total = w.alt + (n == 2 && d == 2 ? (int64_t)n_rep * N * n * 4 : 0);
     return w;
 }
@@ -1028,24 +802,25 @@ __device__ __forceinline__ void on_c0_bonds(const Vec<NC>& s, const float* __res
 // bond joins a colour-0 site i to a colour-1 site, so i's lane takes all 2d
 // bonds of i (forward (i, i+e_a) and backward (i-e_a, i), oriented as the
 // reference's forward bonds), renormalises i in place (no other lane reads
-// it) and adds M of i and of i+e_0 (a bijection onto colour 1).  One lane per
-// site, one partial per block of ON_BLOCK sites (on_block_partial), and
-// on_measure_final adds the blocks in order: the colour-0 half sweep that
-// carries a measurement and the per-replica LDS form form the same partials,
-// so every path gives the same bits.
+// it) and adds M of i and of i+e_0 (a bijection onto colour 1).  One
+// workgroup per chunk of ON_CHUNK colour-0 sites, lane t taking sites
+// t, t + 256, ... in turn, and on_measure_final adds the chunks in order: the
+// colour-0 half sweep that carries a measurement (on_c0_sweep_measure) and
+// the per-replica LDS form form the same partials, so every path gives the
+// same bits.
 template <int NC, int D>
 __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, CbGeo g,
-                                                     double* __restrict__ part, int nb,
+                                                     double* __restrict__ part, int nchunk,
                                                      const int64_t* __restrict__ n_meas_rep,
                                                      int64_t meas)
 {
     constexpr int K = 1 + NC + 2 * D;
     constexpr int Z = 2 * D;
     __shared__ double red[K * 4];
-    // consecutive blocks of a replica on one XCD, so the rows a block reads
-    // beyond its own are in the same L2 as their neighbour block's
+    // consecutive chunks of a replica on one XCD, so the rows a chunk reads
+    // beyond its own are in the same L2 as their neighbour chunk's
     const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int rep = (int)(lb / gridDim.x), blk = (int)(lb - (uint32_t)rep * gridDim.x);
+    const int rep = (int)(lb / gridDim.x), chunk = (int)(lb - (uint32_t)rep * gridDim.x);
     if (n_meas_rep && meas >= n_meas_rep[rep]) return;
     // the colour-0 plane is renormalised in place, the colour-1 plane only read:
     // separate restrict pointers let the loads move ahead of the stores
@@ -1054,22 +829,85 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
     double v[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) v[q] = 0.0;
-    const int64_t k = (int64_t)blk * ON_BLOCK + threadIdx.x;
-    if (k < g.half) {
+    const int64_t k0 = (int64_t)chunk * ON_CHUNK;
+    const int64_t k1 = min(g.half, k0 + ON_CHUNK);
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
         uint32_t nbo[Z];
         cb_other<Z>(g, (uint32_t)k, 0, nbo);
         const Vec<NC> s = unit_vec<NC>(load_vec<NC>(S0, k));
         store_vec<NC>(S0, k, s);
         on_c0_bonds<NC, D>(s, S1, nbo, v);
     }
-    on_block_partial<K>(v, red, part + ((int64_t)rep * nb + blk) * K);
+    on_block_partial<K>(v, red, part + ((int64_t)rep * nchunk + chunk) * K);
+}
+
+// The colour-0 half sweep that carries the previous measurement (a
+// measurement ending with a colour-1 half sweep, the next starting with this
+// one): per chunk of ON_CHUNK sites, lane t takes sites t, t + 256, ... as
+// on_measure_c0 does, renormalises each and adds its bond and magnetisation
+// terms to the chunk partial of measurement `pmeas` (the same terms in the
+// same order, so the same bits), then updates it with on_site.  The reference
+// renormalises at the measurement and starts the next sweep from those spins
+// (critmc/kernels/on.py:404-413), so the chain is the same.  A replica whose
+// cap ends before `a.meas` is only measured; one capped before `pmeas` is
+// left alone.
+template <int NC, int Z, int CODE>
+__global__ __launch_bounds__(256) void on_c0_sweep_measure(OnArgs a, double* __restrict__ part,
+                                                           int nchunk, int64_t pmeas)
+{
+    constexpr int D = Z / 2, K = 1 + NC + Z;
+    __shared__ double red[K * 4];
+    __shared__ int ired[4];
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x), chunk = (int)(lb - (uint32_t)rep * gridDim.x);
+    const bool upd = !(a.n_meas_rep && a.meas >= a.n_meas_rep[rep]);
+    const bool msr = !(a.n_meas_rep && pmeas >= a.n_meas_rep[rep]);
+    if (!upd && !msr) return;  // whole block
+    float* __restrict__ S0 = a.spins + (int64_t)rep * a.g.N * NC;
+    const float* __restrict__ S1 = S0 + a.g.half * NC;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const float beta = a.betas[rep], step = a.steps[rep];
+    double v[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = 0.0;
+    int accepted = 0;
+    const int64_t c0 = (int64_t)chunk * ON_CHUNK;
+    const int64_t c1 = min(a.g.half, c0 + ON_CHUNK);
+    for (int64_t k = c0 + threadIdx.x; k < c1; k += blockDim.x) {
+        uint32_t nbo[Z];
+        cb_other<Z>(a.g, (uint32_t)k, 0, nbo);
+        Vec<NC> s = load_vec<NC>(S0, k);
+        bool changed = false;
+        if (msr) {
+            s = unit_vec<NC>(s);
+            on_c0_bonds<NC, D>(s, S1, nbo, v);
+            changed = true;
+        }
+        if (upd) {
+            int ac = 0;  // on_site sets it to 1 on an accepted proposal
+            changed |= on_site<NC, Z, CODE>(s, S1, nbo, (uint32_t)k, 0, sweep, k0, k1, sid, beta, step, ac);
+            accepted += ac;
+        }
+        if (changed) store_vec<NC>(S0, k, s);
+    }
+    if (msr) on_block_partial<K>(v, red, part + ((int64_t)rep * nchunk + chunk) * K);
+    if (CODE == CODE_METROPOLIS && upd) {  // one atomic per workgroup
+        const int t = wave_sum(accepted);
+        if ((threadIdx.x & 63) == 0) ired[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = ired[0] + ired[1] + ired[2] + ired[3];
+            if (tot) atomicAdd(&a.acc[(int64_t)rep * a.n_meas + a.meas], tot);
+        }
+    }
 }
 
 // Per-replica form for lattices whose spins fit in LDS (N n 4 B <= ON_BATCHED_LDS):
 // one workgroup per replica runs every sweep and measurement of the call with
 // the replica in LDS, in the colour-split layout, and writes it back once.
 // Each site update is on_site, and each measurement repeats on_measure_c0's
-// per-site work, blocks and reductions and then on_measure_final's ordered
+// per-site work, chunks and reductions and then on_measure_final's ordered
 // sum, so spins and series equal the per-half-sweep launches' bit for bit.
 // Local updates only (Metropolis, heat bath, over-relaxation); as there, the
 // last half sweep of a measurement stores its sites renormalised.  HB: the
@@ -1077,9 +915,9 @@ __global__ __launch_bounds__(256) void on_measure_c0(float* __restrict__ spins, 
 // would otherwise limit every schedule's occupancy).  BLK lanes per workgroup:
 // 256, or 1024 for replicas large enough to give each lane several sites
 // (local updates without heat bath); the measurement then runs in groups of
-// 256 lanes, group j taking block j of every NG = BLK / 256 consecutive
-// blocks exactly as one on_measure_c0 workgroup takes it, and the block
-// partials are added in block order, so the bits do not depend on BLK.
+// 256 lanes, group j taking chunk j of every NG = BLK / 256 consecutive
+// chunks exactly as one on_measure_c0 workgroup takes it, and the chunk
+// partials are added in chunk order, so the bits do not depend on BLK.
 // Up to 60 KB of spins any number of replicas take this form (two or more
 // workgroups per CU); up to 158 KB (XY 128^2 at 128 KB, one workgroup per CU)
 // only as many replicas as the GPU has CUs.  More such replicas would hold
@@ -1167,16 +1005,16 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
                 }
             }
             // the measurement: colour 0 renormalised in place, bonds summed per
-            // block of ON_BLOCK sites (one per lane of a group), as on_measure_c0
-            double total = 0.0;  // thread q < K: quantity q, block partials added in order
-            const int nblk = (int)((half + ON_BLOCK - 1) / ON_BLOCK);
+            // chunk of ON_CHUNK sites by a group of 256 lanes, as on_measure_c0
+            double total = 0.0;  // thread q < K: quantity q, chunk partials added in order
+            const int nchunk = (int)((half + ON_CHUNK - 1) / ON_CHUNK);
             const int grp = (int)(threadIdx.x >> 8), gt = (int)(threadIdx.x & 255), gw = (gt >> 6);
-            for (int cb = 0; cb < nblk; cb += NG) {
+            for (int cb = 0; cb < nchunk; cb += NG) {
                 double v[K];
 #pragma unroll
                 for (int q = 0; q < K; ++q) v[q] = 0.0;
-                const int64_t k = (int64_t)(cb + grp) * ON_BLOCK + gt;
-                if (cb + grp < nblk && k < half) {
+                const int64_t c0 = (int64_t)(cb + grp) * ON_CHUNK, c1 = min(half, c0 + ON_CHUNK);
+                for (int64_t k = c0 + gt; k < c1; k += 256) {
                     uint32_t nbo[Z];
                     cb_other<Z>(g, (uint32_t)k, 0, nbo);
                     const Vec<NC> s = unit_vec<NC>(load_vec<NC>(L0, k));
@@ -1190,7 +1028,7 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
                 }
                 __syncthreads();
                 if (threadIdx.x < K) {
-                    for (int j = 0; j < NG && cb + j < nblk; ++j) {
+                    for (int j = 0; j < NG && cb + j < nchunk; ++j) {
                         double t = 0.0;
                         for (int u = 0; u < 4; ++u) t += red[((int)threadIdx.x * NG + j) * 4 + u];
                         total += t;
@@ -1220,14 +1058,16 @@ __global__ __launch_bounds__(BLK) void on_batched(OnArgs a, Sched sc, double* __
     }
 }
 
+// One wave per (replica, quantity): launch with (R K + 3) / 4 blocks of 256.
 template <int NC, int D>
-__global__ void on_measure_final(const double* __restrict__ part, int nb, int R,
-                                 int64_t n_meas, int64_t meas, double* __restrict__ e_out,
-                                 double* __restrict__ m_out, double* __restrict__ sc_out,
-                                 double* __restrict__ ss_out, const int64_t* __restrict__ n_meas_rep)
+__global__ __launch_bounds__(256) void on_measure_final(const double* __restrict__ part, int nb, int R,
+                                                        int64_t n_meas, int64_t meas, double* __restrict__ e_out,
+                                                        double* __restrict__ m_out, double* __restrict__ sc_out,
+                                                        double* __restrict__ ss_out,
+                                                        const int64_t* __restrict__ n_meas_rep)
 {
-    on_final_sum<NC, D>(part, nb, R, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep,
-                        (int)(blockIdx.x * blockDim.x + threadIdx.x));
+    on_final_pair<NC, D>(part, nb, R, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep,
+                         (int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
 }
 
 __global__ void fill_i32_on(int32_t* __restrict__ p, int64_t n)
@@ -1624,34 +1464,55 @@ __global__ void on_init_kernel(float* __restrict__ spins, ClGeo g, int NC,
     }
 }
 
-// A half sweep; with df.part or df.fpart set it also carries the previous
-// measurement (OnDefer).  Blocks of ON_BLOCK lanes, one colour site each, so a
-// colour-0 launch forms the same block partials as on_measure_c0.
-template <int NC, int Z, bool DEFER>
-static void launch_half(int code, const OnArgs& a, const OnDefer& df, hipStream_t st)
+// A half sweep over all replicas, one colour site per lane; with fin.part set
+// one more row of workgroups takes that earlier measurement's ordered sums.
+template <int NC, int Z, bool FIN>
+static void launch_half(int code, const OnArgs& a, const OnFinal& fin, hipStream_t st)
 {
-    const dim3 grid((unsigned)((a.g.half + ON_BLOCK - 1) / ON_BLOCK), a.R);
+    const dim3 grid((unsigned)((a.g.half + 255) / 256), a.R + (FIN ? 1 : 0));
     if (code == CODE_OVERRELAX)
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_OVERRELAX, FIN>), grid, dim3(256), 0, st, a, fin);
     else if (code == CODE_HEATBATH)
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_HEATBATH, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_HEATBATH, FIN>), grid, dim3(256), 0, st, a, fin);
     else
-        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS, DEFER>), grid, dim3(ON_BLOCK), 0, st, a, df);
+        hipLaunchKernelGGL((on_half_sweep<NC, Z, CODE_METROPOLIS, FIN>), grid, dim3(256), 0, st, a, fin);
 }
 
 template <int NC>
-static void launch_half_z(int code, const OnArgs& a, hipStream_t st, const OnDefer* df = nullptr)
+static void launch_half_z(int code, const OnArgs& a, hipStream_t st, const OnFinal& fin = OnFinal{})
 {
-    const OnDefer none{};
-    if (df && (df->part || df->fpart)) {
-        if (a.g.d == 1) launch_half<NC, 2, true>(code, a, *df, st);
-        else if (a.g.d == 2) launch_half<NC, 4, true>(code, a, *df, st);
-        else launch_half<NC, 6, true>(code, a, *df, st);
+    if (fin.part) {
+        if (a.g.d == 1) launch_half<NC, 2, true>(code, a, fin, st);
+        else if (a.g.d == 2) launch_half<NC, 4, true>(code, a, fin, st);
+        else launch_half<NC, 6, true>(code, a, fin, st);
         return;
     }
-    if (a.g.d == 1) launch_half<NC, 2, false>(code, a, none, st);
-    else if (a.g.d == 2) launch_half<NC, 4, false>(code, a, none, st);
-    else launch_half<NC, 6, false>(code, a, none, st);
+    if (a.g.d == 1) launch_half<NC, 2, false>(code, a, fin, st);
+    else if (a.g.d == 2) launch_half<NC, 4, false>(code, a, fin, st);
+    else launch_half<NC, 6, false>(code, a, fin, st);
+}
+
+// The colour-0 half sweep that carries measurement `pmeas` (on_c0_sweep_measure).
+template <int NC, int Z>
+static void launch_c0_measure_z(int code, const OnArgs& a, double* part, int nchunk, int64_t pmeas,
+                                hipStream_t st)
+{
+    const dim3 grid((unsigned)nchunk, a.R);
+    if (code == CODE_OVERRELAX)
+        hipLaunchKernelGGL((on_c0_sweep_measure<NC, Z, CODE_OVERRELAX>), grid, dim3(256), 0, st, a, part, nchunk, pmeas);
+    else if (code == CODE_HEATBATH)
+        hipLaunchKernelGGL((on_c0_sweep_measure<NC, Z, CODE_HEATBATH>), grid, dim3(256), 0, st, a, part, nchunk, pmeas);
+    else
+        hipLaunchKernelGGL((on_c0_sweep_measure<NC, Z, CODE_METROPOLIS>), grid, dim3(256), 0, st, a, part, nchunk, pmeas);
+}
+
+template <int NC>
+static void launch_c0_measure(int code, const OnArgs& a, double* part, int nchunk, int64_t pmeas,
+                              hipStream_t st)
+{
+    if (a.g.d == 1) launch_c0_measure_z<NC, 2>(code, a, part, nchunk, pmeas, st);
+    else if (a.g.d == 2) launch_c0_measure_z<NC, 4>(code, a, part, nchunk, pmeas, st);
+    else launch_c0_measure_z<NC, 6>(code, a, part, nchunk, pmeas, st);
 }
 
 template <int NC>
@@ -1662,13 +1523,13 @@ static void launch_measure(const float* spins, const CbGeo& g, int R, double* pa
     const dim3 grid(nchunk, R);
     if (g.d == 1) {
         hipLaunchKernelGGL((on_measure_partial<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 3) / 4), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else if (g.d == 2) {
         hipLaunchKernelGGL((on_measure_partial<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 3) / 4), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else {
         hipLaunchKernelGGL((on_measure_partial<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nchunk, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 3) / 4), dim3(256), 0, st, part, nchunk, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     }
 }
 
@@ -1677,17 +1538,17 @@ static void launch_measure_c0(float* spins, const CbGeo& g, int R, double* part,
                               int64_t meas, double* e_out, double* m_out, double* sc_out,
                               double* ss_out, const int64_t* nr, hipStream_t st)
 {
-    const int nc0 = (int)((g.half + ON_BLOCK - 1) / ON_BLOCK);
+    const int nc0 = (int)((g.half + ON_CHUNK - 1) / ON_CHUNK);
     const dim3 grid(nc0, R);
     if (g.d == 1) {
         hipLaunchKernelGGL((on_measure_c0<NC, 1>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 1>), dim3((R * (1 + NC + 2) + 3) / 4), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else if (g.d == 2) {
         hipLaunchKernelGGL((on_measure_c0<NC, 2>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 2>), dim3((R * (1 + NC + 4) + 3) / 4), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     } else {
         hipLaunchKernelGGL((on_measure_c0<NC, 3>), grid, dim3(256), 0, st, spins, g, part, nc0, nr, meas);
-        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 255) / 256), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
+        hipLaunchKernelGGL((on_measure_final<NC, 3>), dim3((R * (1 + NC + 6) + 3) / 4), dim3(256), 0, st, part, nc0, R, n_meas, meas, e_out, m_out, sc_out, ss_out, nr);
     }
 }
 
@@ -1880,40 +1741,28 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_or_fused<ON_OR_FUSE>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    // one launch per Metropolis or over-relaxation sweep (on_sweep_tile)
-    const int st_rows = any_cluster ? 0 : on_sweep_tile_rows(g, n, n_rep);
-    const size_t st_lds = (size_t)(2 * st_rows + 6) * g.M * 8;
     const OnWs wl = on_ws_layout(g.N, n_rep, n, g.d);
-    int* tickets = (int*)((char*)workspace + wl.tickets);
-    // the row-tile launches write out of place: `cur` holds the spins between launches
+    // the fused over-relaxation launch writes out of place: `cur` holds the spins between launches
     float* const home = (float*)spins;
     float* const alt = n == 2 && g.d == 2 ? (float*)((char*)workspace + wl.alt) : nullptr;
     float* cur = home;
     auto other = [&](const float* p) { return p == home ? alt : home; };
-    if (st_rows > 0) {
-        CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_sweep_tile<CODE_METROPOLIS>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)st_lds));
-        CM_HIP_CHECK(hipFuncSetAttribute((const void*)on_sweep_tile<CODE_OVERRELAX>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)st_lds));
-        hipLaunchKernelGGL(fill_i32_on, dim3((n_rep + 255) / 256), dim3(256), 0, st, tickets, (int64_t)n_rep);
-    }
-    // Deferred measurements (OnDefer): when a measurement ends with a plain
-    // colour-1 half sweep and the next one starts with a plain colour-0 half
-    // sweep, that colour-0 launch renormalises and measures the previous
-    // state and the colour-1 launch after it adds the block partials; only
-    // the call's last measurement takes the separate measure launches.
+    // Measurements carried by the next sweep: when a measurement ends with a
+    // plain colour-1 half sweep and the next one starts with a plain colour-0
+    // half sweep, that colour-0 launch (on_c0_sweep_measure) renormalises and
+    // measures the previous state and the colour-1 launch after it adds the
+    // chunk partials; only the call's last measurement takes the separate
+    // measure launches.
     const int64_t first_code = schedule[0], first_count = schedule[1];
     const int64_t last_count = schedule[2 * (n_rows - 1) + 1];
     const bool first_plain = !(first_code == CODE_OVERRELAX && or_rows > 0 && first_count >= 2);
     const bool last_plain = !(last_code == CODE_OVERRELAX && or_rows > 0 && last_count % 2 == 0);
     const bool defer = fused && !any_cluster && first_plain && last_plain;
-    const int nb0 = (int)((g.half + ON_BLOCK - 1) / ON_BLOCK);
+    const int nc0 = (int)((g.half + ON_CHUNK - 1) / ON_CHUNK);
     for (int64_t meas = 0; meas < n_meas; ++meas) {
         a.meas = meas;
         int meas_tiles = 0;  // > 0: the fused over-relaxation launch measured
-        OnDefer df{};         // the previous measurement, carried by this one's first sweep
-        if (defer && meas > 0)
-            df = OnDefer{part, meas - 1, part, meas - 1, nb0, e_out, m_out, sc_out, ss_out};
+        const bool carry = defer && meas > 0;  // this measurement's first sweep carries the previous one
         if (any_cluster) hipLaunchKernelGGL(on_cl_zero64, dim3(rb), dim3(256), 0, st, ca.wsize, n_rep);
         for (int r = 0; r < n_rows; ++r) {
             const int code = (int)schedule[2 * r];
@@ -1945,39 +1794,20 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
                     rp += 2;
                     continue;
                 }
-                if (st_rows > 0 && (code == CODE_METROPOLIS || code == CODE_OVERRELAX)) {
-                    // the whole sweep in one launch; the measurement's first sweep
-                    // also carries the previous measurement
-                    a.norm_out = fused && r == n_rows - 1 && rp == count - 1;
-                    OnTile mt{};
-                    if (r == 0 && rp == 0 && df.part)
-                        mt = OnTile{part, meas - 1, nb0, tickets, e_out, m_out, sc_out, ss_out};
-                    const int tiles = (g.L1 + st_rows - 1) / st_rows;
-                    a.spins = other(cur);
-                    if (code == CODE_METROPOLIS)
-                        hipLaunchKernelGGL(on_sweep_tile<CODE_METROPOLIS>, dim3((unsigned)(tiles * n_rep)),
-                                           dim3(1024), st_lds, st, a, (const float*)cur, st_rows, mt);
-                    else
-                        hipLaunchKernelGGL(on_sweep_tile<CODE_OVERRELAX>, dim3((unsigned)(tiles * n_rep)),
-                                           dim3(1024), st_lds, st, a, (const float*)cur, st_rows, mt);
-                    cur = a.spins;
-                    ++sw;
-                    ++rp;
-                    continue;
-                }
                 for (int c = 0; c < 2; ++c) {
                     a.colour = c;
                     a.norm_out = fused && r == n_rows - 1 && rp == count - 1 && c == 1;
-                    // the measurement's first sweep: colour 0 forms the previous
-                    // measurement's partials, colour 1 adds them up
-                    OnDefer dc{};
-                    if (r == 0 && rp == 0 && df.part) {
-                        dc = df;
-                        if (c == 0) dc.fpart = nullptr;
-                        else dc.part = nullptr;
+                    const bool first = carry && r == 0 && rp == 0;
+                    if (first && c == 0) {  // measures the previous state, then updates colour 0
+                        if (n == 2) launch_c0_measure<2>(code, a, part, nc0, meas - 1, st);
+                        else launch_c0_measure<3>(code, a, part, nc0, meas - 1, st);
+                        continue;
                     }
-                    if (n == 2) launch_half_z<2>(code, a, st, &dc);
-                    else launch_half_z<3>(code, a, st, &dc);
+                    OnFinal fin{};
+                    if (first)  // colour 1 adds the previous measurement's partials up
+                        fin = OnFinal{part, meas - 1, nc0, e_out, m_out, sc_out, ss_out};
+                    if (n == 2) launch_half_z<2>(code, a, st, fin);
+                    else launch_half_z<3>(code, a, st, fin);
                 }
                 ++sw;
                 ++rp;
@@ -1986,7 +1816,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
         const int64_t sites = (int64_t)n_rep * g.N;
         const int nb = (int)std::min<int64_t>((sites + 255) / 256, 8192);
         if (meas_tiles > 0) {
-            hipLaunchKernelGGL((on_measure_final<2, 2>), dim3((n_rep * 7 + 255) / 256), dim3(256), 0, st,
+            hipLaunchKernelGGL((on_measure_final<2, 2>), dim3((n_rep * 7 + 3) / 4), dim3(256), 0, st,
                                part, meas_tiles, n_rep, n_meas, meas, e_out, m_out, sc_out, ss_out, n_meas_rep);
         } else if (defer && meas + 1 < n_meas) {
             // measured by the next measurement's first sweep
@@ -2004,7 +1834,7 @@ int cm_on_run(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, int
             hipLaunchKernelGGL(on_cl_record, dim3(rb), dim3(256), 0, st, n_rep, n_meas, meas,
                                acc_out, ca.wsize, ca.stats, any_sw ? 1 : 0, aux_out);
     }
-    if (cur != home)  // the last row-tile launch wrote the second buffer
+    if (cur != home)  // the last fused over-relaxation launch wrote the second buffer
         hipLaunchKernelGGL(on_copy_back, dim3((unsigned)std::min<int64_t>((g.N * n / 4 + 255) / 256, 256),
                                               (unsigned)n_rep),
                            dim3(256), 0, st, home, (const float*)cur, n_rep, g.N * n, n_meas_rep);

@@ -379,12 +379,6 @@ int cm_muca_run(void *spins_dev, int32_t ndim, const int32_t *dims, int32_t n_re
  *   "ea_order": dispatch order of the fused launch's samples: largest first
  *       (0, the default) or the larger half largest first and the smaller
  *       half smallest first (1).
- *   "on_sweep_tile": XY Metropolis and over-relaxation sweeps on square
- *       lattices too large for the per-replica LDS form run one launch per
- *       sweep over row tiles in LDS when the launch has at most this many
- *       colour sites (replicas x sites per colour), two half-sweep launches
- *       otherwise (0, the default: always, as measured faster on MI355X); the
- *       same bits either way.
  * Sets `name` to value; *previous (if not null) receives the old value. */
 int cm_set_option(const char *name, int64_t value, int64_t *previous);
 

@@ -302,86 +302,31 @@ def test_lds_form_matches_half_sweep_launches(n, shape, sched):
             assert np.array_equal(a, b)
 
 
-def _set_option(name: str, value: int) -> int:
-    import ctypes
-
-    from critmc_amd._lib import lib
-
-    prev = ctypes.c_int64(0)
-    assert lib().cm_set_option(name.encode(), value, ctypes.byref(prev)) == 0
-    return int(prev.value)
-
-
-@pytest.mark.parametrize("shape", [(256, 256), (192, 166), (512, 64), (640, 34)])
-@pytest.mark.parametrize("sched", [[[0, 1]], [[5, 1]], [[0, 1], [5, 1]], [[2, 1], [0, 1]], [[0, 2], [5, 3]]],
-                         ids=["metropolis", "overrelax", "met+or", "heatbath+met", "2met+3or"])
-def test_sweep_tile_matches_half_sweep_launches(shape, sched):
-    """XY sweeps on square lattices too large for the per-replica LDS form run one
-    launch per sweep over row tiles (option on_sweep_tile), each measurement's first
-    sweep carrying the previous measurement.  Against two half-sweep launches per
-    sweep the spins, acceptance counts and FP64 measurements are the same bits,
-    with per-chain caps that stop replicas at different measurements."""
-    dev = torch.device("cuda", 0)
-    R, n_meas = 4, 5
-    N = shape[0] * shape[1]
-    rng = np.random.default_rng(shape[1] + len(sched))
-    keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
-    betas, steps = np.array([0.8, 1.12, 1.4, 2.0]), np.array([0.7, 1.0, 1.3, 0.4])
-    caps = torch.tensor([5, 3, 0, 1], dtype=torch.int64, device=dev)
-    outs = []
-    prev = _set_option("on_sweep_tile", 1 << 40)
-    try:
-        for tile in (1 << 40, 0):
-            _set_option("on_sweep_tile", tile)
-            chains = Chains(keys, [5, 6, 7, 8], None, dev)
-            s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
-            K.init_spins(s, shape, 2, chains)
-            res = K.run(s, shape, 2, betas, steps, chains, np.array(sched), n_meas, n_meas_rep=caps)
-            outs.append((s.cpu().numpy(), [t.cpu().numpy() for t in res],
-                         chains.sweeps_d.cpu().numpy().view(np.uint64)))
-    finally:
-        _set_option("on_sweep_tile", prev)
-    (s_t, r_t, sw_t), (s_h, r_h, sw_h) = outs
-    assert np.array_equal(s_t, s_h)
-    assert np.array_equal(sw_t, sw_h)
-    c = caps.cpu().numpy()
-    for a, b in zip(r_t, r_h):  # accepted, E, M, c_a, s_a: the measurements each replica made
-        for r in range(R):
-            assert np.array_equal(a[r, : c[r]], b[r, : c[r]])
-
-
-@pytest.mark.parametrize("sched", [[[5, 1]], [[0, 1]], [[5, 2]]], ids=["overrelax", "metropolis", "or-pair"])
-def test_row_tiles_at_scale_match_half_sweep_launches(sched):
-    """Row-tile launches (one sweep, or two fused over-relaxation sweeps) with more
-    tiles than the GPU holds at once: each tile reads the state before the launch
-    and writes its own rows to the other buffer, so tiles that start after others
-    finish still see the old rows.  96 replicas of 256^2 (768 tiles) against
-    half-sweep launches: identical spins and acceptance counts, measurements equal
-    (the fused pair's per-tile sums to FP64 rounding)."""
+@pytest.mark.parametrize("fused,split", [([[5, 2]], [[5, 1], [5, 1]]),
+                                         ([[0, 1], [5, 2]], [[0, 1], [5, 1], [5, 1]])],
+                         ids=["or-pair", "metropolis+or-pair"])
+def test_or_tiles_at_scale_match_half_sweep_launches(fused, split):
+    """The fused over-relaxation launch with more row tiles than the GPU holds at
+    once (96 replicas of 256^2: 864 tiles): each tile reads the state before the
+    launch and writes its own rows to the other buffer, so tiles that start after
+    others finish still see the old rows.  Against half-sweep launches: identical
+    spins and acceptance counts, measurements equal to FP64 rounding (the fused
+    launch sums per tile)."""
     dev = torch.device("cuda", 0)
     shape, R, n_meas = (256, 256), 96, 3
     N = shape[0] * shape[1]
     rng = np.random.default_rng(77)
     keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
     betas, steps = rng.uniform(0.8, 1.6, size=R), rng.uniform(0.5, 1.5, size=R)
-    split = [[5, 1], [5, 1]] if sched == [[5, 2]] else sched
     outs = []
-    prev = _set_option("on_sweep_tile", 1 << 40)
-    try:
-        for tile, sc in ((1 << 40, sched), (0, split)):
-            _set_option("on_sweep_tile", tile)
-            chains = Chains(keys, np.arange(R), None, dev)
-            s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
-            K.init_spins(s, shape, 2, chains)
-            res = K.run(s, shape, 2, betas, steps, chains, np.array(sc), n_meas)
-            outs.append((s.cpu().numpy(), [t.cpu().numpy() for t in res]))
-    finally:
-        _set_option("on_sweep_tile", prev)
-    (s_t, r_t), (s_h, r_h) = outs
-    assert np.array_equal(s_t, s_h)
-    assert np.array_equal(r_t[0], r_h[0])
-    for a, b in zip(r_t[1:], r_h[1:]):
-        if sched == [[5, 2]]:
-            np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * N)
-        else:
-            assert np.array_equal(a, b)
+    for sc in (fused, split):
+        chains = Chains(keys, np.arange(R), None, dev)
+        s = torch.empty((R, N, 2), dtype=torch.float32, device=dev)
+        K.init_spins(s, shape, 2, chains)
+        res = K.run(s, shape, 2, betas, steps, chains, np.array(sc), n_meas)
+        outs.append((s.cpu().numpy(), [t.cpu().numpy() for t in res]))
+    (s_f, r_f), (s_s, r_s) = outs
+    assert np.array_equal(s_f, s_s)
+    assert np.array_equal(r_f[0], r_s[0])
+    for a, b in zip(r_f[1:], r_s[1:]):
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * N)
