@@ -387,9 +387,10 @@ R_DEFAULT = R
 def secondary_runs(args) -> dict:
     """Short timed runs of the other full-size configurations, in the same process
     as the headline: EA tempering (512 samples x 2 x 32 T, 2000 PT steps), XY
-    (256 x 512^2, 100 measurements of metropolis + 2 x overrelax) and q = 10
-    Potts heat bath (128 x 512^2, 2000 sweeps).  Each is the configuration of
-    its own --workload line at fewer steps; the per-step cost is flat in the
+    (256 x 512^2, 100 measurements of metropolis + 2 x overrelax), q = 10
+    Potts heat bath (128 x 512^2, 2000 sweeps) and the serving stream's first
+    10^3 jobs at 70 % of its capacity.  Each is the configuration of its own
+    --workload line at fewer steps or jobs; the per-step cost is flat in the
     step count, so the rates compare with those lines."""
     import torch
 
@@ -402,6 +403,13 @@ def secondary_runs(args) -> dict:
         keep = ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config", "roofline")
         out[name] = {k: rec[k] for k in keep if k in rec}
         torch.cuda.empty_cache()
+    # the serving stream's first 10^3 jobs (same draws as the 10^4-job stream) at
+    # 70 % of the measured capacity: p50 / p99 latency in the same record
+    sub = argparse.Namespace(**vars(args))
+    sub.jobs, sub.rate = 1000, 0.0
+    rec = bench_serve(sub)
+    out["serve"] = {k: rec[k] for k in ("metric", "value", "unit", "latency_p50_s", "latency_p99_s",
+                                        "wall_s", "config")}
     return out
 
 

@@ -286,7 +286,9 @@ int cm_ea_pt_run(void *spins_dev, const void *J_dev, int32_t jtype, const int32_
 
 /* ---- O(n) vector model, n = 2 (XY) and 3 (critmc/kernels/on.py) ---------- */
 
-/* Bytes of caller-owned device scratch cm_on_run needs (FP64 partial sums). */
+/* Bytes of caller-owned device scratch cm_on_run needs: FP64 partial sums,
+ * and for XY on square lattices a second spin buffer of n_rep * N * 2 floats
+ * (the fused over-relaxation launch writes out of place). */
 int64_t cm_on_workspace_bytes(int32_t ndim, const int32_t *dims, int32_t n_rep, int32_t n);
 
 /* O(n) spins live on the device in the colour-split layout
