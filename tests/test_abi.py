@@ -68,3 +68,12 @@ def test_library_refuses_cluster_codes():
                         P(0), P(0), P(0), P(0), 1 << 20, 0, 0, 0, P(0), None, P(0))
     assert rc == CM_EINVAL
     assert b"checkerboard" in L.cm_last_error()
+
+
+def test_copy_spans_checks_its_count_without_a_gpu():
+    from critmc_amd._lib import CM_EINVAL, CM_OK, lib
+
+    L = lib()
+    assert L.cm_copy_spans(ctypes.c_void_p(0), -1, None) == CM_EINVAL
+    assert b"negative" in L.cm_last_error()
+    assert L.cm_copy_spans(ctypes.c_void_p(0), 0, None) == CM_OK  # nothing to copy, no launch

@@ -67,3 +67,37 @@ def test_potts_simulation_api_matches_replay():
     assert np.array_equal(ts["E"], ref["E"]) and np.array_equal(ts["counts"], ref["counts"])
     assert sim.running["energy"] == int(ref["E"][-1])
     assert np.array_equal(sim.running["counts"], np.bincount(sim.state, minlength=3))
+
+
+@pytest.mark.parametrize("grid", [2, -1], ids=["per-replica", "wide"])
+def test_caller_workspace_tables_follow_the_betas(grid):
+    """A caller-owned workspace keeps its tables while the betas stay the same
+    and is prepared again when they change; each call continues the chains
+    exactly as the replay does with that call's betas."""
+    dev = torch.device("cuda", 0)
+    shape, q, R = (16, 16), 10, 2
+    chains, keys, sids, sw0 = _chains(R, 77, dev)
+    n = int(np.prod(shape))
+    s = torch.empty((R, n), dtype=torch.int8, device=dev)
+    P.init_states(s, shape, q, chains, "random")
+    state = s.cpu().numpy()
+    e, c = P.measure(s, shape, q, R)
+    ws = P.Workspace(R, dev)
+    sched = np.array([[2, 1], [0, 1]])
+    rng = np.random.default_rng(5)
+    beta_a, beta_b = rng.uniform(0.8, 1.6, size=R), rng.uniform(0.8, 1.6, size=R)
+    sweep = sw0.astype(np.int64).copy()
+    for betas in (beta_a, beta_b, beta_b, beta_a):
+        n_meas = 3
+        e_out, acc, counts = P.run(s, shape, q, betas, chains, sched, n_meas, e, c, grid=grid, ws=ws)
+        got = s.cpu().numpy()
+        e_out, acc, counts = (t.cpu().numpy() for t in (e_out, acc, counts))
+        for r in range(R):
+            rep = oracle.PottsReplay(state[r], shape, q, betas[r], keys[r], int(sids[r]), int(sweep[r]))
+            ref = rep.run(sched.tolist(), n_meas)
+            assert np.array_equal(got[r], rep.s), f"states differ (replica {r})"
+            assert np.array_equal(e_out[r], ref["E"])
+            assert np.array_equal(acc[r], ref["accepted"])
+            assert np.array_equal(counts[r], ref["counts"])
+        state = got
+        sweep += n_meas * int(sched[:, 1].sum())

@@ -20,15 +20,30 @@ def _clone(j, arrival=0.0):
     return Job(j.model, j.L, j.beta, j.sweeps, j.seed, q=j.q, step=j.step, arrival=arrival)
 
 
-def test_each_result_is_the_same_alone_and_in_the_mix():
-    alone = []
-    for j in PROBES:
-        srv = Server(budget_bytes=1e9, slice_sweeps=256)
+def _alone(jobs, **kw):
+    out = []
+    for j in jobs:
+        srv = Server(**kw)
         job = srv.submit(_clone(j))
         srv.run(realtime=False)
-        alone.append(job.results())
+        out.append(job.results())
+    return out
+
+
+@pytest.fixture(scope="module")
+def probes_alone():
+    return _alone(PROBES, budget_bytes=1e9, slice_sweeps=256)
+
+
+# resident: small pools in their one-launch-per-slice forms (the default) or in
+# the kernels' automatic choice; launch_threads: slices enqueued from host
+# threads (the serving bench uses 4) or from the serving loop itself
+@pytest.mark.parametrize("resident,threads", [(True, 0), (False, 0), (True, 4)])
+def test_each_result_is_the_same_alone_and_in_the_mix(probes_alone, resident, threads):
+    alone = probes_alone
     mix = poisson_jobs(24, rate=2000.0, seed=5, L_choices=(16, 32, 64), sweeps_range=(200, 2000))
-    srv = Server(budget_bytes=1e9, slice_sweeps=300, slice_seconds=0.002)  # adaptive slices too
+    srv = Server(budget_bytes=1e9, slice_sweeps=300, slice_seconds=0.002,  # adaptive slices too
+                 resident=resident, launch_threads=threads)
     probes = [srv.submit(_clone(j, arrival=0.003 * (k + 1))) for k, j in enumerate(PROBES)]
     for j in mix:
         srv.submit(j)
@@ -42,11 +57,14 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     assert srv.ticks > 1
 
 
-# Large lattices: the pools that carry most of the serving work.  Ising and
-# Potts pools of a few 256^2 / 512^2 replicas take the wide forms (one launch
-# per half sweep over all replicas), 128^2 the per-replica forms; XY takes the
-# half-sweep launches.  Sweep counts differ from the background jobs', so the
-# per-slot caps stop each job at its own count inside shared slices.
+# Large lattices: the pools that carry most of the serving work.  With the
+# resident forms, Ising pools run one launch per slice (each replica on chip)
+# at every L here and Potts pools at 128^2; Potts 256^2 / 512^2 take the wide
+# form (one launch per half sweep over all replicas), and so does every Ising
+# and Potts pool without the resident forms.  XY takes the O(n) LDS form at
+# 128^2 and the half-sweep launches above it.  Sweep counts differ from the
+# background jobs', so the per-slot caps stop each job at its own count
+# inside shared slices.
 LARGE_PROBES = [Job(m, L, b, sw, seed=100 + k, arrival=0.004 * (1 + k % 3))
                 for k, (m, L, b, sw) in enumerate([
                     ("ising", 128, 0.44, 333), ("ising", 256, 0.43, 517), ("ising", 512, 0.45, 261),
@@ -54,14 +72,16 @@ LARGE_PROBES = [Job(m, L, b, sw, seed=100 + k, arrival=0.004 * (1 + k % 3))
                     ("xy", 128, 1.10, 219), ("xy", 256, 1.12, 151), ("xy", 512, 1.09, 97)])]
 
 
-def test_large_lattices_are_the_same_alone_and_joining_running_pools():
-    alone = []
-    for j in LARGE_PROBES:
-        srv = Server(budget_bytes=4e9, slice_sweeps=64)
-        job = srv.submit(_clone(j))
-        srv.run(realtime=False)
-        alone.append(job.results())
-    srv = Server(budget_bytes=8e9, slice_sweeps=64)
+@pytest.fixture(scope="module")
+def large_alone():
+    return _alone(LARGE_PROBES, budget_bytes=4e9, slice_sweeps=64)
+
+
+@pytest.mark.parametrize("resident,threads", [(True, 0), (False, 4)])
+def test_large_lattices_are_the_same_alone_and_joining_running_pools(large_alone, resident,
+                                                                     threads):
+    alone = large_alone
+    srv = Server(budget_bytes=8e9, slice_sweeps=64, resident=resident, launch_threads=threads)
     background = []
     for k, j in enumerate(LARGE_PROBES):  # two running jobs per pool before any probe arrives
         for b in range(2):
@@ -104,3 +124,28 @@ def test_latency_report():
     assert np.all(lat > 0)
     assert np.percentile(lat, 99) >= np.percentile(lat, 50)
     assert srv.site_updates == sum(j.sweeps * j.sites for j in done)
+
+
+def test_copy_spans_copies_every_span():
+    """cm_copy_spans, the one launch per slice that moves each job's part of
+    the pool's series into the job's buffers: spans of 0, 1, a few and many
+    4-byte words, at unrelated offsets of two buffers."""
+    import torch
+
+    from critmc_amd._lib import check, lib
+    from critmc_amd.device import ptr, stream
+
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(3)
+    src = torch.randint(-(1 << 31), (1 << 31) - 1, (9000,), dtype=torch.int32, generator=g)
+    dst0 = torch.randint(-(1 << 31), (1 << 31) - 1, (9000,), dtype=torch.int32, generator=g)
+    src_d, dst_d = src.to(dev), dst0.to(dev)
+    spans = [(0, 0, 0), (5, 17, 1), (100, 3, 7), (300, 1000, 256), (1000, 2000, 257),
+             (3000, 4000, 5000), (8999, 8999, 1), (20, 40, 33)]
+    table = torch.tensor([[dst_d.data_ptr() + 4 * d, src_d.data_ptr() + 4 * s, 4 * n]
+                          for d, s, n in spans], dtype=torch.int64).to(dev)
+    check(lib().cm_copy_spans(ptr(table), len(spans), stream()), "cm_copy_spans")
+    want = dst0.clone()
+    for d, s, n in spans:
+        want[d: d + n] = src[s: s + n]
+    assert torch.equal(dst_d.cpu(), want)
