@@ -71,6 +71,9 @@ def parse():
                     help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
     ap.add_argument("--launch-threads", type=int, default=4,
                     help="serve: host threads that enqueue the pools' slices (0: the serving thread)")
+    ap.add_argument("--fixed-streams", action="store_true",
+                    help="serve: keep each pool on the stream it was created with instead of "
+                         "queueing each slice on the least loaded stream")
     ap.add_argument("--no-resident", action="store_true",
                     help="serve: leave every pool's launch form to the kernels' automatic choice "
                          "instead of one launch per slice wherever the lattice fits on chip")
@@ -318,7 +321,7 @@ def bench_xy(args) -> dict:
 # Site updates/ns of the 10^4-job mix on one MI355X with every job queued at
 # once (bench.py --workload serve --rate 1e9); the default arrival rate is 70 %
 # of it.
-SERVE_CAPACITY = 184.5
+SERVE_CAPACITY = 198.4
 
 
 def bench_serve(args) -> dict:
@@ -335,7 +338,8 @@ def bench_serve(args) -> dict:
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
     srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds,
-                 resident=not args.no_resident, launch_threads=args.launch_threads)
+                 resident=not args.no_resident, launch_threads=args.launch_threads,
+                 balance_streams=not args.fixed_streams)
     for j in jobs:
         srv.submit(j)
     torch.cuda.synchronize()
@@ -353,7 +357,8 @@ def bench_serve(args) -> dict:
                        round(rate * work / len(jobs) / 1e9, 2), "budget_GB": 200,
                        "slice_sweeps": 1000, "slice_seconds": args.slice_seconds,
                        "resident_forms": not args.no_resident,
-                       "launch_threads": args.launch_threads},
+                       "launch_threads": args.launch_threads,
+                       "streams": "fixed per pool" if args.fixed_streams else "least loaded per slice"},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
             "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,

@@ -4,9 +4,9 @@ Plays the role of critmc/runner.py:154-199 (``run_tasks``) for a stream of
 independent chains: jobs ``(model, L, beta, sweeps, seed)`` arrive over time,
 are admitted against an HBM byte budget, and advance together.  Jobs of the
 same (model, L) share a resident slot pool; every tick advances every active
-slot by up to ``slice_sweeps`` sweeps in one launch per pool (each pool on its
-own HIP stream, so pools overlap on the device), with per-slot caps so a job
-stops exactly at its sweep count.  A finished job frees its slot and budget at
+slot by up to ``slice_sweeps`` sweeps in one launch per pool (each slice on
+the least loaded of a few HIP streams, so pools overlap on the device), with
+per-slot caps so a job stops exactly at its sweep count.  A finished job frees its slot and budget at
 once, and queued jobs take them on the next tick.
 
 A job's chain depends only on its seed (Philox key, stream 0, its own sweep
@@ -116,8 +116,21 @@ class _Pool:
         self.busy_s = 0.0                 # device time of this pool's slices
         self.sec_per_sweep = 0.0          # running estimate of one slice sweep's device time
         self.site_updates = 0
+        self.ready = torch.cuda.Event()   # recorded after the pool's last queued work
+        self.stream_idx = 0
+        self.est_s = 0.0                  # estimated device time of the slice in flight
         with torch.cuda.stream(self.stream):
             self._grow(8)
+            self.ready.record(self.stream)
+
+    def use_stream(self, stream, idx: int):
+        """Queue the pool's next work on ``stream``.  A pool has one slice in
+        flight at most, so any stream will do; the new stream first waits for
+        the pool's last work on the old one (on the device, not the host)."""
+        if stream is not self.stream:
+            stream.wait_event(self.ready)
+            self.stream = stream
+        self.stream_idx = idx
 
     # -- storage --------------------------------------------------------------
     def _alloc(self, cap: int):
@@ -143,8 +156,12 @@ class _Pool:
     def _grow(self, cap: int):
         new = self._alloc(cap)
         if self.cap:
+            cur = torch.cuda.current_stream()
             for k, v in self.state.items():
                 new[k][: self.cap].copy_(v)
+                # v was allocated on the stream the pool used then: keep its
+                # memory from reuse there until this copy has run
+                v.record_stream(cur)
         self.state = new
         self.free.extend(range(self.cap, cap))
         self.jobs.extend([None] * (cap - self.cap))
@@ -185,6 +202,7 @@ class _Pool:
     def add(self, job: Job):
         with torch.cuda.stream(self.stream):
             self._add(job)
+            self.ready.record(self.stream)
 
     def _add(self, job: Job):
         if not self.free:
@@ -300,6 +318,7 @@ class _Pool:
                 self.spans_d[:ns].copy_(self.spans_h[:ns], non_blocking=True)
                 check(lib().cm_copy_spans(ptr(self.spans_d), ns, stream()), "cm_copy_spans")
             self.done_event.record(self.stream)
+            self.ready.record(self.stream)
         self.inflight = caps
         return caps
 
@@ -320,7 +339,7 @@ class Server:
 
     def __init__(self, budget_bytes: float = 200e9, slice_sweeps: int = 500, n_streams: int = 4,
                  max_active: int = 1 << 16, slice_seconds: float = 0.0, min_slice: int = 16,
-                 resident: bool = True, launch_threads: int = 0):
+                 resident: bool = True, launch_threads: int = 0, balance_streams: bool = True):
         """``slice_sweeps`` caps the sweeps of one slice.  With ``slice_seconds`` > 0 a
         pool's slice is instead sized from its measured device time per sweep to take
         about that long (at least ``min_slice`` sweeps), so a large pool does not hold
@@ -331,7 +350,12 @@ class Server:
         (the same chains either way).  ``launch_threads`` > 0 enqueues the pools'
         slices from that many host threads (the library releases the interpreter
         while it launches), so a pool whose slice is thousands of launches does
-        not hold up the others; 0 enqueues them from the calling thread."""
+        not hold up the others; 0 enqueues them from the calling thread.
+        ``balance_streams`` queues each slice on the stream with the least
+        estimated device time still queued (a pool's last slices' time per
+        sweep times the slice's sweeps), so a long job's many slices do not
+        each wait behind the pools that happen to share a fixed stream; False
+        keeps every pool on the stream it was given when it was created."""
         self.dev = device()
         self.budget = float(budget_bytes)
         self.used = 0.0
@@ -345,6 +369,8 @@ class Server:
         self.finished: list[Job] = []
         self.max_active = max_active
         self.resident = bool(resident)
+        self.balance_streams = bool(balance_streams)
+        self._stream_est = [0.0] * len(self.streams)  # estimated device seconds queued per stream
         self.launch_threads = int(launch_threads)
         self._launcher = None
         if self.launch_threads > 0:
@@ -375,9 +401,10 @@ class Server:
     def _pool(self, job: Job) -> _Pool:
         key = (job.model, job.L, job.q if job.model == "potts" else 0)
         if key not in self.pools:
-            self.pools[key] = _Pool(job.model, job.L, job.q, self.dev,
-                                    self.streams[len(self.pools) % len(self.streams)],
-                                    resident=self.resident)
+            idx = len(self.pools) % len(self.streams)
+            pool = _Pool(job.model, job.L, job.q, self.dev, self.streams[idx], resident=self.resident)
+            pool.stream_idx = idx
+            self.pools[key] = pool
         return self.pools[key]
 
     def _admit(self, now: float):
@@ -395,7 +422,14 @@ class Server:
                 still.append(job)
         self.waiting = still
 
+    def _queued(self, pool: _Pool, est_s: float):
+        """Book (est_s > 0) or release (est_s = 0) the pool's slice on its stream."""
+        i = pool.stream_idx
+        self._stream_est[i] = max(0.0, self._stream_est[i] - pool.est_s) + est_s
+        pool.est_s = est_s
+
     def _retire(self, pool: _Pool, now: float):
+        self._queued(pool, 0.0)
         caps, pool.inflight = pool.inflight, None
         dt = pool.start_event.elapsed_time(pool.done_event) / 1e3
         pool.busy_s += dt
@@ -443,22 +477,33 @@ class Server:
                     caps, pool.enqueue = pool.enqueue.result(), None
                     if caps is not None:
                         self.ticks += 1
+                    else:
+                        self._queued(pool, 0.0)
                 if pool.inflight is not None:
                     if not pool.done_event.query():
                         busy = True
                         continue
                     self._retire(pool, stamp())
+                if not (pool.pending or pool.active):
+                    continue
+                if self.balance_streams:
+                    i = min(range(len(self.streams)), key=self._stream_est.__getitem__)
+                    pool.use_stream(self.streams[i], i)
                 for job in pool.pending:
                     pool.add(job)
                 pool.pending = []
                 if not pool.active:
                     continue
+                n = self._slice_for(pool)
+                self._queued(pool, pool.sec_per_sweep * n if pool.sec_per_sweep > 0.0 else 1e-3)
                 if self._launcher is not None:
-                    pool.enqueue = self._launcher.submit(pool.launch, self._slice_for(pool))
+                    pool.enqueue = self._launcher.submit(pool.launch, n)
                     busy = True
-                elif pool.launch(self._slice_for(pool)) is not None:
+                elif pool.launch(n) is not None:
                     self.ticks += 1
                     busy = True
+                else:
+                    self._queued(pool, 0.0)
             if progress > 0 and time.perf_counter() - last > progress:
                 last = time.perf_counter()
                 print(f"serve: t={last - t0:.1f}s finished={len(self.finished)} active={self.n_active} "

@@ -37,13 +37,16 @@ def probes_alone():
 
 # resident: small pools in their one-launch-per-slice forms (the default) or in
 # the kernels' automatic choice; launch_threads: slices enqueued from host
-# threads (the serving bench uses 4) or from the serving loop itself
-@pytest.mark.parametrize("resident,threads", [(True, 0), (False, 0), (True, 4)])
-def test_each_result_is_the_same_alone_and_in_the_mix(probes_alone, resident, threads):
+# threads (the serving bench uses 4) or from the serving loop itself;
+# balance: each slice on the least loaded stream (the default), or every pool
+# on the stream it was created with
+@pytest.mark.parametrize("resident,threads,balance", [(True, 0, True), (False, 0, True),
+                                                      (True, 4, True), (True, 0, False)])
+def test_each_result_is_the_same_alone_and_in_the_mix(probes_alone, resident, threads, balance):
     alone = probes_alone
     mix = poisson_jobs(24, rate=2000.0, seed=5, L_choices=(16, 32, 64), sweeps_range=(200, 2000))
     srv = Server(budget_bytes=1e9, slice_sweeps=300, slice_seconds=0.002,  # adaptive slices too
-                 resident=resident, launch_threads=threads)
+                 resident=resident, launch_threads=threads, balance_streams=balance)
     probes = [srv.submit(_clone(j, arrival=0.003 * (k + 1))) for k, j in enumerate(PROBES)]
     for j in mix:
         srv.submit(j)
