@@ -71,6 +71,9 @@ def parse():
                     help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
     ap.add_argument("--launch-threads", type=int, default=4,
                     help="serve: host threads that enqueue the pools' slices (0: the serving thread)")
+    ap.add_argument("--streams", type=int, default=8,
+                    help="serve: HIP streams the pools' slices are spread over (and hardware "
+                         "queues asked of HIP, up to 32)")
     ap.add_argument("--fixed-streams", action="store_true",
                     help="serve: keep each pool on the stream it was created with instead of "
                          "queueing each slice on the least loaded stream")
@@ -321,7 +324,7 @@ def bench_xy(args) -> dict:
 # Site updates/ns of the 10^4-job mix on one MI355X with every job queued at
 # once (bench.py --workload serve --rate 1e9); the default arrival rate is 70 %
 # of it.
-SERVE_CAPACITY = 198.4
+SERVE_CAPACITY = 210.8
 
 
 def bench_serve(args) -> dict:
@@ -338,6 +341,7 @@ def bench_serve(args) -> dict:
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
     srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds,
+                 n_streams=args.streams,
                  resident=not args.no_resident, launch_threads=args.launch_threads,
                  balance_streams=not args.fixed_streams)
     for j in jobs:
@@ -358,7 +362,9 @@ def bench_serve(args) -> dict:
                        "slice_sweeps": 1000, "slice_seconds": args.slice_seconds,
                        "resident_forms": not args.no_resident,
                        "launch_threads": args.launch_threads,
-                       "streams": "fixed per pool" if args.fixed_streams else "least loaded per slice"},
+                       "streams": args.streams,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)"),
+                       "stream_choice": "fixed per pool" if args.fixed_streams else "least booked per slice"},
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall,
             "peak_budget_used_GB": srv.peak_used / 1e9, "ticks": srv.ticks,
@@ -420,6 +426,17 @@ def secondary_runs(args) -> dict:
 
 def main():
     args = parse()
+    # The serving runs (--workload serve, and the default run's serving probe)
+    # spread slices over --streams HIP streams.  HIP gives a process 4
+    # hardware queues unless GPU_MAX_HW_QUEUES says otherwise, and streams
+    # beyond them share queues, so ask for one per stream before HIP starts
+    # (at most 32; a larger value already set is kept).  The other workloads
+    # use one stream.
+    if args.workload == "serve" or (args.workload == "ising2d" and not args.no_secondary):
+        want = min(32, max(4, args.streams))
+        have = os.environ.get("GPU_MAX_HW_QUEUES", "")
+        if not have.isdigit() or int(have) < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU; replicas shard across GPUs "
                                    "with no traffic between them, so --gpus N > 1 is not measured"}))

@@ -355,7 +355,9 @@ class Server:
         estimated device time still queued (a pool's last slices' time per
         sweep times the slice's sweeps), so a long job's many slices do not
         each wait behind the pools that happen to share a fixed stream; False
-        keeps every pool on the stream it was given when it was created."""
+        keeps every pool on the stream it was given when it was created.
+        Streams beyond the process's hardware queues share them (HIP gives 4
+        unless the environment sets ``GPU_MAX_HW_QUEUES`` before HIP starts)."""
         self.dev = device()
         self.budget = float(budget_bytes)
         self.used = 0.0
