@@ -71,6 +71,8 @@ def parse():
                     help="serve: target device time of one pool slice (0: fixed 1000-sweep slices)")
     ap.add_argument("--launch-threads", type=int, default=4,
                     help="serve: host threads that enqueue the pools' slices (0: the serving thread)")
+    ap.add_argument("--slice-sweeps", type=int, default=1000,
+                    help="serve: sweeps per slice (the most a pool advances between admissions)")
     ap.add_argument("--streams", type=int, default=8,
                     help="serve: HIP streams the pools' slices are spread over (and hardware "
                          "queues asked of HIP, up to 32)")
@@ -340,7 +342,7 @@ def bench_serve(args) -> dict:
     if rate <= 0:  # Poisson arrivals at 70 % of the measured capacity
         rate = 0.7 * args.capacity * 1e9 / (work / len(jobs))
     jobs = poisson_jobs(args.jobs, rate=rate, seed=2024)
-    srv = Server(budget_bytes=200e9, slice_sweeps=1000, slice_seconds=args.slice_seconds,
+    srv = Server(budget_bytes=200e9, slice_sweeps=args.slice_sweeps, slice_seconds=args.slice_seconds,
                  n_streams=args.streams,
                  resident=not args.no_resident, launch_threads=args.launch_threads,
                  balance_streams=not args.fixed_streams)
@@ -359,7 +361,7 @@ def bench_serve(args) -> dict:
             "config": {"workload": "serve_continuous_batching", "jobs": len(done),
                        "arrival_rate_per_s": rate, "offered_site_updates_per_ns":
                        round(rate * work / len(jobs) / 1e9, 2), "budget_GB": 200,
-                       "slice_sweeps": 1000, "slice_seconds": args.slice_seconds,
+                       "slice_sweeps": args.slice_sweeps, "slice_seconds": args.slice_seconds,
                        "resident_forms": not args.no_resident,
                        "launch_threads": args.launch_threads,
                        "streams": args.streams,
