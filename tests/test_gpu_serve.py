@@ -152,3 +152,25 @@ def test_copy_spans_copies_every_span():
     for d, s, n in spans:
         want[d: d + n] = src[s: s + n]
     assert torch.equal(dst_d.cpu(), want)
+
+
+def test_pools_that_grow_while_changing_streams_keep_every_result():
+    """Twenty jobs of one pool arriving over time: the pool grows from 8 to 16
+    to 32 slots (its state copied to new buffers) while its slices move between
+    eight streams; every job's series equals the job run alone."""
+    jobs = [Job(m, 16, b, 150 + 37 * k, seed=900 + k, arrival=0.0005 * k)
+            for k in range(20) for m, b in [(("ising", "potts", "xy")[k % 3],
+                                             (0.44, 1.42, 1.1)[k % 3])]]
+    alone = _alone(jobs, budget_bytes=1e9, slice_sweeps=64)
+    srv = Server(budget_bytes=1e9, slice_sweeps=64, n_streams=8)
+    probes = [srv.submit(_clone(j, arrival=j.arrival)) for j in jobs]
+    # three pools of about seven jobs each would not grow past 8 slots: twenty
+    # more Ising jobs, all there at the start, take the Ising pool to 32
+    extra = [srv.submit(Job("ising", 16, 0.43, 400 + 20 * k, seed=950 + k)) for k in range(20)]
+    done = srv.run(realtime=True)
+    assert len(done) == len(jobs) + len(extra)
+    assert srv.pools[("ising", 16, 0)].cap >= 32
+    for res, job in zip(alone, probes):
+        got = job.results()
+        for k in res:
+            assert np.array_equal(got[k], res[k]), (job.model, job.seed, k)
