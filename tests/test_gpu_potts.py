@@ -87,9 +87,19 @@ def test_caller_workspace_tables_follow_the_betas(grid):
     rng = np.random.default_rng(5)
     beta_a, beta_b = rng.uniform(0.8, 1.6, size=R), rng.uniform(0.8, 1.6, size=R)
     sweep = sw0.astype(np.int64).copy()
-    for betas in (beta_a, beta_b, beta_b, beta_a):
+    # calls alternate between two streams, each after the previous call's work
+    # (the chains' state is shared); a call on another stream than the last
+    # upload's prepares the tables again on its own stream
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for call, betas in enumerate((beta_a, beta_b, beta_b, beta_a, beta_a)):
         n_meas = 3
-        e_out, acc, counts = P.run(s, shape, q, betas, chains, sched, n_meas, e, c, grid=grid, ws=ws)
+        st = streams[call % 2]
+        st.wait_stream(streams[1 - call % 2])
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):
+            e_out, acc, counts = P.run(s, shape, q, betas, chains, sched, n_meas, e, c, grid=grid,
+                                       ws=ws)
+        torch.cuda.current_stream().wait_stream(st)
         got = s.cpu().numpy()
         e_out, acc, counts = (t.cpu().numpy() for t in (e_out, acc, counts))
         for r in range(R):
