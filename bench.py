@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--no-secondary", action="store_true",
                     help="headline run only: skip the short runs of the EA, XY and Potts "
                          "configurations that the default line carries under 'secondary'")
-    ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "potts", "serve", "copy"),
+    ap.add_argument("--workload", choices=("ising2d", "ea", "xy", "potts", "serve", "copy", "matrix"),
                     default="ising2d",
                     help="ising2d: the headline batch; ea / xy / serve: the other configurations; "
                          "potts: the q = 10 heat bath of the serving mix at full size")
@@ -397,6 +397,148 @@ def bench_copy(args) -> dict:
 R_DEFAULT = R
 
 
+def bench_matrix(args) -> dict:
+    """Every checkerboard and sequential kernel family at one standard size each,
+    besides the configurations with their own --workload line: a warm-up call, then
+    one timed call bracketed by synchronisation.  Each entry gives site updates/ns
+    and the streaming model's HBM fraction (the state read and written once per
+    site update: 2 B int8, 0.25 B bit-packed, 16 B FP32 XY, 24 B FP32 Heisenberg,
+    16 B FP64 site-order XY)."""
+    import torch
+
+    from critmc_amd import hypercubic, triangular
+    from critmc_amd.kernels import ising as KI
+    from critmc_amd.kernels import on as KO
+    from critmc_amd.kernels import potts as KP
+    from critmc_amd.kernels import table as KT
+    from critmc_amd.kernels.chains import Chains
+    from critmc_amd.kernels.ea import EaBatch
+    from critmc_amd.glass import geometric_temperatures
+
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(31)
+
+    def chains(R):
+        keys = rng.integers(0, 1 << 32, size=(R, 2), dtype=np.uint64)
+        return Chains(keys, np.arange(R), None, dev)
+
+    def timed(call):
+        call()  # warm-up: code objects, tables, workspaces
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        call()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    out = []
+
+    def add(name, config, updates, dt, bytes_per_update):
+        rate = updates / dt / 1e9
+        out.append({"kernel": name, "config": config, "value": round(rate, 3),
+                    "unit": "site updates/ns", "seconds": round(dt, 4),
+                    "hbm_frac_streaming": round(rate * bytes_per_update / 8000.0, 4)})
+
+    # Ising, checkerboard: heat bath, 32-spin packed, in a field, wide form, 3D
+    for name, shape, R, sched, h, layout, n in (
+            ("ising2d_heatbath_int8", (256, 256), 4096, [[2, 1]], 0.0, KI.LAYOUT_INT8, 200),
+            ("ising2d_metropolis_packed", (256, 256), 4096, [[0, 1]], 0.0, KI.LAYOUT_PACKED, 200),
+            ("ising2d_metropolis_field_h0.1", (256, 256), 4096, [[0, 1]], 0.1, KI.LAYOUT_INT8, 200),
+            ("ising2d_metropolis_wide_L1024", (1024, 1024), 16, [[0, 1]], 0.0, KI.LAYOUT_INT8, 200),
+            ("ising3d_metropolis_int8_L64", (64, 64, 64), 256, [[0, 1]], 0.0, KI.LAYOUT_INT8, 100)):
+        N = int(np.prod(shape))
+        ch = chains(R)
+        sp = torch.empty((R, N), dtype=torch.int8, device=dev)
+        KI.init_spins(sp, shape, ch, "random")
+        e, m = KI.measure(sp, shape, R)
+        st = KI.pack(sp, shape, R) if layout == KI.LAYOUT_PACKED else sp
+        ws = KI.Workspace(R, dev)
+        betas = rng.uniform(0.40, 0.48, size=R) if len(shape) == 2 else np.full(R, 0.2217)
+        dt = timed(lambda: KI.run(st, shape, betas, h, ch, np.array(sched), n, e, m, layout=layout,
+                                  ws=ws))
+        add(name, {"shape": list(shape), "replicas": R, "sweeps": n, "h": h}, R * N * n, dt,
+            0.25 if layout == KI.LAYOUT_PACKED else 2.0)
+        del sp, st
+        torch.cuda.empty_cache()
+
+    # Potts: q = 10 Metropolis (wide form), q = 3 heat bath (per-replica LDS form)
+    for name, L, R, q, sched, n in (("potts2d_q10_metropolis", 512, 128, 10, [[0, 1]], 500),
+                                    ("potts2d_q3_heatbath_L128", 128, 1024, 3, [[2, 1]], 500)):
+        shape = (L, L)
+        ch = chains(R)
+        sp = torch.empty((R, L * L), dtype=torch.int8, device=dev)
+        KP.init_states(sp, shape, q, ch, "random")
+        e, c = KP.measure(sp, shape, q, R)
+        ws = KP.Workspace(R, dev)
+        betas = np.full(R, np.log(1.0 + np.sqrt(q)))
+        dt = timed(lambda: KP.run(sp, shape, q, betas, ch, np.array(sched), n, e, c, ws=ws))
+        add(name, {"shape": list(shape), "replicas": R, "q": q, "sweeps": n}, R * L * L * n, dt, 2.0)
+        del sp
+        torch.cuda.empty_cache()
+
+    # O(n), checkerboard FP32: XY Metropolis alone, XY heat bath (von Mises),
+    # Heisenberg metropolis + 2 x overrelax on a cubic lattice
+    for name, shape, R, nc, sched, n, sweeps, bpu in (
+            ("xy2d_metropolis", (512, 512), 256, 2, [[0, 1]], 100, 1, 16.0),
+            ("xy2d_heatbath", (512, 512), 256, 2, [[2, 1]], 50, 1, 16.0),
+            ("heisenberg3d_metropolis_2overrelax_L64", (64, 64, 64), 64, 3, [[0, 1], [5, 2]], 20, 3,
+             24.0)):
+        N = int(np.prod(shape))
+        ch = chains(R)
+        sp = torch.empty((R, N, nc), dtype=torch.float32, device=dev)
+        KO.init_spins(sp, shape, nc, ch)
+        ws = KO.Workspace(shape, R, nc, dev)
+        betas = np.full(R, 1.0 / 0.89 if nc == 2 else 1.0 / 1.443)
+        steps = np.full(R, 1.0)
+        dt = timed(lambda: KO.run(sp, shape, nc, betas, steps, ch, np.array(sched), n, ws=ws))
+        add(name, {"shape": list(shape), "replicas": R, "n": nc, "measurements": n,
+                   "sweeps_per_measurement": sweeps}, R * N * n * sweeps, dt, bpu)
+        del sp
+        torch.cuda.empty_cache()
+
+    # EA tempering, FP64 path: Gaussian couplings, 64 samples of 8^3, 2 x 16 temperatures
+    S, L, m_t, n = 64, 8, 16, 200
+    Nn = L ** 3
+    coup = [rng.standard_normal(3 * Nn) for _ in range(S)]
+    temps = geometric_temperatures(0.2, 2.0, m_t)
+    eb = EaBatch(np.full(S, L), 3, coup, 1.0 / np.asarray(temps), rng.integers(0, 1 << 32, size=(S, 2),
+                                                                           dtype=np.uint64), dev=dev)
+    dt = timed(lambda: eb.run(n, int(np.log2(n)) + 1, update="heatbath"))
+    add("ea3d_gaussian_pt_heatbath", {"samples": S, "L": L, "temperatures": m_t, "sets": 2,
+                                      "steps": n}, S * 2 * m_t * Nn * n, dt, 2.0)
+    del eb
+    torch.cuda.empty_cache()
+
+    # sequential chains on a lattice without a checkerboard: metropolis-random Ising
+    # on the triangular 64 x 64 torus, one lane per chain
+    lat = triangular(64)
+    nbr = KT.neighbours(lat, dev)
+    R, n = 8192, 5
+    ch = chains(R)
+    sp = (2 * torch.randint(0, 2, (R, lat.n_sites), device=dev) - 1).to(torch.int8)
+    e, m = KT.measure(sp, nbr, R)
+    tws = KT.Workspace(lat.n_sites, R, 0, dev)
+    dt = timed(lambda: KT.ising_run(sp, nbr, np.full(R, 0.27), 0.0, ch, np.array([[1, 1]]), n, e, m,
+                                    ws=tws))
+    add("ising_triangular_metropolis_random", {"lattice": "triangular 64x64", "chains": R,
+                                               "sweeps": n}, R * lat.n_sites * n, dt, 2.0)
+    del sp
+    torch.cuda.empty_cache()
+
+    # Swendsen-Wang (the wide cluster path): 64 replicas of 256^2 at beta_c
+    shape, R, n = (256, 256), 64, 20
+    ch = chains(R)
+    sp = torch.empty((R, 256 * 256), dtype=torch.int8, device=dev)
+    KI.init_spins(sp, shape, ch, "random")
+    e, m = KI.measure(sp, shape, R)
+    cws = KI.ClusterWorkspace(shape, R, dev)
+    dt = timed(lambda: KI.run_cluster(sp, shape, np.full(R, np.log(1 + np.sqrt(2)) / 2), 0.0, ch,
+                                      np.array([[4, 1]]), n, e, m, ws=cws))
+    add("ising2d_swendsen_wang", {"shape": [256, 256], "replicas": R, "updates": n},
+        R * 256 * 256 * n, dt, 2.0)
+    return {"metric": "site updates/ns per kernel family", "unit": "site updates/ns", "n_gpus": 1,
+            "data": "synthetic: random starts, keys from seed 31", "entries": out}
+
+
 def secondary_runs(args) -> dict:
     """Short timed runs of the other full-size configurations, in the same process
     as the headline: EA tempering (512 samples x 2 x 32 T, 2000 PT steps), XY
@@ -446,7 +588,7 @@ def main():
 
     if args.workload != "ising2d":
         fn = {"ea": bench_ea, "xy": bench_xy, "potts": bench_potts, "serve": bench_serve,
-              "copy": bench_copy}[args.workload]
+              "copy": bench_copy, "matrix": bench_matrix}[args.workload]
         print(json.dumps(fn(args)))
         return
 

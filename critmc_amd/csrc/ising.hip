@@ -26,8 +26,9 @@
 //
 // Forms: plain (one chain, one launch per half sweep), batched (one launch,
 // replicas resident in LDS; undecided words leave their wave for an LDS queue)
-// and wide (one launch per half sweep over all replicas, state in HBM).  All
-// three draw the same uniforms for the same word, so they run the same chains.
+// and wide (one launch per half sweep over all replicas, state in HBM: 2D
+// rows of 64-site segments, or any lattice through the plain form's words).
+// All draw the same uniforms for the same word, so they run the same chains.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -49,6 +50,64 @@ struct Geo {
     int64_t rows;
 };
 
+// One word of the half sweep: word t = r G + w of colour `colour` (row r of
+// the lattice's rows, 32-site group w), its spins gathered from the int8
+// lattice s, updated and scattered back.  The group index t selects the
+// word's Philox blocks, so every form that calls this draws the same
+// uniforms.
+template <int Z, int NCMAX>
+__device__ __forceinline__ void plain_word(int8_t* __restrict__ s, const Geo& g, int code,
+                                           int colour, uint64_t sweep, uint32_t k0, uint32_t k1,
+                                           uint32_t sid, const uint32_t* __restrict__ tabs,
+                                           int64_t t, Delta& d)
+{
+    const int64_t r = t / g.G;
+    const int w = (int)(t - r * g.G);
+    int x1 = 0, x2 = 0;
+    if (Z >= 4) {
+        x1 = (int)(r % g.L1);
+        x2 = (int)(r / g.L1);
+    }
+    const int p = (x1 + x2 + colour) & 1;
+    const int64_t base = r * g.L0;
+    int64_t rows_nb[4] = {0, 0, 0, 0};
+    if (Z >= 4) {
+        const int64_t y0 = r - x1;
+        rows_nb[0] = (y0 + (x1 + 1 == g.L1 ? 0 : x1 + 1)) * g.L0;
+        rows_nb[1] = (y0 + (x1 == 0 ? g.L1 - 1 : x1 - 1)) * g.L0;
+    }
+    if (Z == 6) {
+        const int64_t plane = (int64_t)g.L1;
+        const int64_t z0 = r - (int64_t)x2 * plane;
+        rows_nb[2] = (z0 + (int64_t)(x2 + 1 == g.L2 ? 0 : x2 + 1) * plane) * g.L0;
+        rows_nb[3] = (z0 + (int64_t)(x2 == 0 ? g.L2 - 1 : x2 - 1) * plane) * g.L0;
+    }
+    const int nbits = min(32, g.M - 32 * w);
+    uint32_t own = 0;
+    uint32_t nb[Z];
+#pragma unroll
+    for (int k = 0; k < Z; ++k) nb[k] = 0;
+    for (int b = 0; b < nbits; ++b) {
+        const int x0 = 2 * (32 * w + b) + p;
+        const int xp = x0 + 1 == g.L0 ? 0 : x0 + 1;
+        const int xm = x0 == 0 ? g.L0 - 1 : x0 - 1;
+        own |= (uint32_t)(s[base + x0] > 0) << b;
+        nb[0] |= (uint32_t)(s[base + xp] > 0) << b;
+        nb[1] |= (uint32_t)(s[base + xm] > 0) << b;
+#pragma unroll
+        for (int k = 2; k < Z; ++k) nb[k] |= (uint32_t)(s[rows_nb[k - 2] + x0] > 0) << b;
+    }
+    const uint32_t valid = nbits == 32 ? ~0u : ((1u << nbits) - 1u);
+    const uint32_t nw = update_word<Z, NCMAX>(code, own, nb, valid, tabs, k0, k1, sid, sweep,
+                                              (uint32_t)colour, (uint32_t)t, d);
+    uint32_t changed = (nw ^ own) & valid;
+    while (changed) {
+        const int b = __ffs(changed) - 1;
+        changed &= changed - 1;
+        s[base + 2 * (32 * w + b) + p] = ((nw >> b) & 1u) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
 template <int Z, int NCMAX>
 __global__ __launch_bounds__(256) void ising_plain_half(int8_t* __restrict__ s, Geo g, int code,
                                                         int colour, uint64_t sweep, uint32_t k0,
@@ -60,58 +119,44 @@ __global__ __launch_bounds__(256) void ising_plain_half(int8_t* __restrict__ s, 
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     Delta d{0, 0, 0};
-    if (t < g.rows * g.G) {
-        const int64_t r = t / g.G;
-        const int w = (int)(t - r * g.G);
-        int x1 = 0, x2 = 0;
-        if (Z >= 4) {
-            x1 = (int)(r % g.L1);
-            x2 = (int)(r / g.L1);
-        }
-        const int p = (x1 + x2 + colour) & 1;
-        const int64_t base = r * g.L0;
-        int64_t rows_nb[4] = {0, 0, 0, 0};
-        if (Z >= 4) {
-            const int64_t y0 = r - x1;
-            rows_nb[0] = (y0 + (x1 + 1 == g.L1 ? 0 : x1 + 1)) * g.L0;
-            rows_nb[1] = (y0 + (x1 == 0 ? g.L1 - 1 : x1 - 1)) * g.L0;
-        }
-        if (Z == 6) {
-            const int64_t plane = (int64_t)g.L1;
-            const int64_t z0 = r - (int64_t)x2 * plane;
-            rows_nb[2] = (z0 + (int64_t)(x2 + 1 == g.L2 ? 0 : x2 + 1) * plane) * g.L0;
-            rows_nb[3] = (z0 + (int64_t)(x2 == 0 ? g.L2 - 1 : x2 - 1) * plane) * g.L0;
-        }
-        const int nbits = min(32, g.M - 32 * w);
-        uint32_t own = 0;
-        uint32_t nb[Z];
-#pragma unroll
-        for (int k = 0; k < Z; ++k) nb[k] = 0;
-        for (int b = 0; b < nbits; ++b) {
-            const int x0 = 2 * (32 * w + b) + p;
-            const int xp = x0 + 1 == g.L0 ? 0 : x0 + 1;
-            const int xm = x0 == 0 ? g.L0 - 1 : x0 - 1;
-            own |= (uint32_t)(s[base + x0] > 0) << b;
-            nb[0] |= (uint32_t)(s[base + xp] > 0) << b;
-            nb[1] |= (uint32_t)(s[base + xm] > 0) << b;
-#pragma unroll
-            for (int k = 2; k < Z; ++k) nb[k] |= (uint32_t)(s[rows_nb[k - 2] + x0] > 0) << b;
-        }
-        const uint32_t valid = nbits == 32 ? ~0u : ((1u << nbits) - 1u);
-        const uint32_t nw = update_word<Z, NCMAX>(code, own, nb, valid, tabs, k0, k1, sid, sweep,
-                                                  (uint32_t)colour, (uint32_t)t, d);
-        uint32_t changed = (nw ^ own) & valid;
-        while (changed) {
-            const int b = __ffs(changed) - 1;
-            changed &= changed - 1;
-            s[base + 2 * (32 * w + b) + p] = ((nw >> b) & 1u) ? (int8_t)1 : (int8_t)-1;
-        }
-    }
+    if (t < g.rows * g.G) plain_word<Z, NCMAX>(s, g, code, colour, sweep, k0, k1, sid, tabs, t, d);
     const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
     if ((threadIdx.x & 63) == 0 && (de | dm | ac)) {
         atomicAdd(de_out, de);
         atomicAdd(dm_out, dm);
         atomicAdd(acc_out, ac);
+    }
+}
+
+// Wide form on any lattice the plain kernel sweeps (chains, 2D with any even
+// L0, cubic): one launch per half sweep over (word chunks x replicas), each
+// replica's keys, stream id and sweep counter read on the device, its
+// deltas added to its own measurement slot.  The words and uniforms are the
+// plain form's, so both give the same chains.
+template <int Z, int NCMAX>
+__global__ __launch_bounds__(256) void ising_any_half(
+    int8_t* __restrict__ spins, Geo g, int code, int colour, uint64_t sweep_add,
+    const uint32_t* __restrict__ tabs, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
+    int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
+    int32_t* __restrict__ dm_out, int32_t* __restrict__ acc_out)
+{
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x);
+    const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;  // whole workgroup
+    const int64_t t = (int64_t)bx * blockDim.x + threadIdx.x;
+    Delta d{0, 0, 0};
+    if (t < g.rows * g.G)
+        plain_word<Z, NCMAX>(spins + (int64_t)rep * g.rows * g.L0, g, code, colour,
+                             sweeps[rep] + sweep_add, keys[2 * rep], keys[2 * rep + 1], sids[rep],
+                             tabs + (int64_t)TAB_WORDS * rep, t, d);
+    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+    if ((threadIdx.x & 63) == 0 && (de | dm | ac)) {
+        const int64_t o = (int64_t)rep * n_meas + meas;
+        atomicAdd(&de_out[o], de);
+        atomicAdd(&dm_out[o], dm);
+        atomicAdd(&acc_out[o], ac);
     }
 }
 
@@ -599,6 +644,83 @@ __global__ __launch_bounds__(64) void ising2d_half_wide(
     }
 }
 
+// Wide form on cubic lattices with L0 % 64 == 0 (int8): as ising2d_half_wide,
+// a lane takes the 32 colour-c sites of one 64-site row segment, read as four
+// 16-byte words, plus the segments of the rows at +-y and +-z and the one
+// x-neighbour outside its segment.  Word idx = r G + w (row r = y + L1 z) is
+// the plain form's word index, so the uniforms, and the chains, are the same;
+// the neighbour order differs, which the symmetric count does not see.
+template <int NC>
+__global__ __launch_bounds__(64) void ising3d_half_wide(
+    int8_t* __restrict__ spins, int L0, int L1, int L2, int code, int c, uint64_t sweep_add,
+    const uint32_t* __restrict__ tabs, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
+    int64_t meas, const int64_t* __restrict__ n_meas_rep, int32_t* __restrict__ de_out,
+    int32_t* __restrict__ dm_out, int32_t* __restrict__ acc_out)
+{
+    const uint32_t lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int rep = (int)(lb / gridDim.x);
+    const uint32_t bx = lb - (uint32_t)rep * gridDim.x;
+    if (n_meas_rep && meas >= n_meas_rep[rep]) return;  // whole workgroup
+    const int G = L0 >> 6;
+    const int rows = L1 * L2;
+    const int W = rows * G;
+    const int idx = (int)(bx * blockDim.x + threadIdx.x);
+    Delta d{0, 0, 0};
+    if (idx < W) {
+        const int r = (int)fdiv((uint32_t)idx, make_fastdiv((uint32_t)G)), w = idx - r * G;
+        const int z = (int)fdiv((uint32_t)r, make_fastdiv((uint32_t)L1)), y = r - z * L1;
+        const int p = (y + z + c) & 1;
+        const int yp = y + 1 == L1 ? 0 : y + 1, ym = y == 0 ? L1 - 1 : y - 1;
+        const int zp = z + 1 == L2 ? 0 : z + 1, zm = z == 0 ? L2 - 1 : z - 1;
+        int8_t* S = spins + (int64_t)rep * L0 * rows;
+        int8_t* row8 = S + (int64_t)r * L0 + 64 * w;
+        uint32_t ev, od;
+        load_segment(row8, ev, od);
+        const uint32_t own = p == 0 ? ev : od;
+        const uint32_t same = p == 0 ? od : ev;
+        uint32_t sh;
+        if (p == 0) {
+            const int8_t b = S[(int64_t)r * L0 + (w == 0 ? L0 - 1 : 64 * w - 1)];
+            sh = (same << 1) | (b > 0 ? 1u : 0u);
+        } else {
+            const int8_t b = S[(int64_t)r * L0 + (w == G - 1 ? 0 : 64 * w + 64)];
+            sh = (same >> 1) | (b > 0 ? 0x80000000u : 0u);
+        }
+        uint32_t nb[6] = {same, sh, 0u, 0u, 0u, 0u};
+        const int nrow[4] = {z * L1 + yp, z * L1 + ym, zp * L1 + y, zm * L1 + y};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            load_segment(S + (int64_t)nrow[k] * L0 + 64 * w, ev, od);
+            nb[2 + k] = p == 0 ? ev : od;
+        }
+        const uint64_t sweep = sweeps[rep] + sweep_add;
+        const uint32_t nw = update_word<6, NC>(code, own, nb, ~0u, tabs + (int64_t)TAB_WORDS * rep,
+                                               keys[2 * rep], keys[2 * rep + 1], sids[rep], sweep,
+                                               (uint32_t)c, (uint32_t)idx, d);
+        if (nw != own) {
+            const uint32_t nev = p == 0 ? nw : same, nod = p == 0 ? same : nw;
+            uint4* v = (uint4*)row8;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint4 q;
+                q.x = pack_pair_u32(nev, nod, 8 * k + 0);
+                q.y = pack_pair_u32(nev, nod, 8 * k + 2);
+                q.z = pack_pair_u32(nev, nod, 8 * k + 4);
+                q.w = pack_pair_u32(nev, nod, 8 * k + 6);
+                v[k] = q;
+            }
+        }
+    }
+    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+    if (threadIdx.x == 0 && (de | dm | ac)) {
+        const int64_t o = (int64_t)rep * n_meas + meas;
+        atomicAdd(&de_out[o], de);
+        atomicAdd(&dm_out[o], dm);
+        atomicAdd(&acc_out[o], ac);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Initial states, exact E_b / M, layout conversion.
 
@@ -810,6 +932,56 @@ static void launch_plain_nc(int nc, int8_t* s, const Geo& g, int code, int colou
     }
 }
 
+template <int Z, int NC>
+static void launch_any(const dim3& grid, hipStream_t st, int8_t* s, const Geo& g, int code,
+                       int colour, uint64_t sw, const uint32_t* tabs, const uint32_t* keys,
+                       const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
+                       const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out, int32_t* acc_out)
+{
+    hipLaunchKernelGGL((ising_any_half<Z, NC>), grid, dim3(256), 0, st, s, g, code, colour, sw,
+                       tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out);
+}
+
+template <int NC>
+static void launch_3d(const dim3& grid, hipStream_t st, int8_t* s, const Geo& g, int code,
+                      int colour, uint64_t sw, const uint32_t* tabs, const uint32_t* keys,
+                      const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
+                      const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out, int32_t* acc_out)
+{
+    hipLaunchKernelGGL((ising3d_half_wide<NC>), grid, dim3(64), 0, st, s, g.L0, g.L1, g.L2, code,
+                       colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                       m_out, acc_out);
+}
+
+static void launch_3d_nc(int nc, const dim3& grid, hipStream_t st, int8_t* s, const Geo& g,
+                         int code, int colour, uint64_t sw, const uint32_t* tabs,
+                         const uint32_t* keys, const uint32_t* sids, const uint64_t* sweeps,
+                         int64_t n_meas, int64_t meas, const int64_t* n_meas_rep, int32_t* e_out,
+                         int32_t* m_out, int32_t* acc_out)
+{
+    switch (nc) {
+        case 2: launch_3d<2>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 4: launch_3d<4>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 7: launch_3d<7>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        default: launch_3d<NC_LIMIT>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+    }
+}
+
+template <int Z>
+static void launch_any_nc(int nc, const dim3& grid, hipStream_t st, int8_t* s, const Geo& g,
+                          int code, int colour, uint64_t sw, const uint32_t* tabs,
+                          const uint32_t* keys, const uint32_t* sids, const uint64_t* sweeps,
+                          int64_t n_meas, int64_t meas, const int64_t* n_meas_rep, int32_t* e_out,
+                          int32_t* m_out, int32_t* acc_out)
+{
+    switch (nc) {
+        case 2: launch_any<Z, 2>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 4: launch_any<Z, 4>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        case 7: launch_any<Z, 7>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+        default: launch_any<Z, NC_LIMIT>(grid, st, s, g, code, colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out, m_out, acc_out); break;
+    }
+}
+
 template <int LAYOUT, int NC, bool ZF = false>
 static hipError_t launch_batched(void* spins, int R, const Geo& g, const uint32_t* tabs,
                                  const uint32_t* keys, const uint32_t* sids, uint64_t* sweeps,
@@ -938,14 +1110,15 @@ static int prepare_tables(int z, int n_rep, const double* betas, double h, const
     return CM_OK;
 }
 
-// Whether cm_ising_run with form 0 on int8 spins only launches kernels (the
-// batched or wide form) rather than the plain form, which reads the host sweep
-// counters and synchronises the stream: then its launches can be captured.
+// Whether cm_ising_run with form 0 on int8 spins only launches kernels, so
+// that its launches can be captured: form 0 there is always the batched or a
+// wide form, never the plain form (which reads the host sweep counters and
+// synchronises the stream), so every valid lattice qualifies.
 bool ising_rows_capturable(int32_t ndim, const int32_t* dims, int32_t n_rep)
 {
+    (void)n_rep;  // on int8 spins form 0 is always the batched or a wide form
     Geo g;
-    if (geo_from(ndim, dims, &g) != CM_OK) return false;
-    return prefer_wide(g, 0, n_rep) || batched_ok(g);
+    return geo_from(ndim, dims, &g) == CM_OK;
 }
 
 }  // namespace cm
@@ -1060,11 +1233,63 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
     const bool zf_tables = info[1] != 0;
 
     if (form == 0) {
+        // int8 lattices that neither 2D form takes (chains, cubic, 2D rows
+        // that are not whole 64-site segments) go wide on the plain form's
+        // words: every replica in each launch, no host synchronisation
         if (prefer_wide(g, layout, n_rep)) form = 3;
-        else form = batched_ok(g) ? 2 : 1;
+        else if (batched_ok(g)) form = 2;
+        else form = layout == 0 ? 3 : 1;
+    }
+    if (form == 3 && !wide_ok(g)) {
+        CM_REQUIRE(layout == 0, "the wide form off 2D lattices with L0 a multiple of 64 works "
+                                "on int8 spins");
+        CM_REQUIRE(n_rep <= 65535, "n_rep out of range");
+        const int64_t total = (int64_t)n_rep * n_meas;
+        const int fb = (int)std::min<int64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, e_out, total, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, m_out, total, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, acc_out, total, 0);
+        const int64_t words = g.rows * g.G;
+        // cubic lattices of 64-site row segments: whole segments per lane
+        const bool seg3 = g.d == 3 && g.L0 % 64 == 0;
+        const int wpb = seg3 ? 64 : 256;  // lanes per workgroup
+        CM_REQUIRE((words + wpb - 1) / wpb * n_rep < (1ll << 32),
+                   "too many words x replicas per launch");
+        const dim3 agrid((unsigned)((words + wpb - 1) / wpb), (unsigned)n_rep);
+        int8_t* s = (int8_t*)spins;
+        uint64_t sw = 0;
+        int64_t per_meas = 0;
+        for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];
+        for (int64_t meas = 0; meas < n_meas; ++meas)
+            for (int r = 0; r < sc.n; ++r)
+                for (int rp = 0; rp < sc.count[r]; ++rp) {
+                    for (int c = 0; c < 2; ++c) {
+                        if (seg3)
+                            launch_3d_nc(nc, agrid, st, s, g, sc.code[r], c, sw, dtabs, keys,
+                                         stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                                         m_out, acc_out);
+                        else if (z == 2)
+                            launch_any_nc<2>(nc, agrid, st, s, g, sc.code[r], c, sw, dtabs, keys,
+                                             stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                                             m_out, acc_out);
+                        else if (z == 4)
+                            launch_any_nc<4>(nc, agrid, st, s, g, sc.code[r], c, sw, dtabs, keys,
+                                             stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                                             m_out, acc_out);
+                        else
+                            launch_any_nc<6>(nc, agrid, st, s, g, sc.code[r], c, sw, dtabs, keys,
+                                             stream_ids, sweeps, n_meas, meas, n_meas_rep, e_out,
+                                             m_out, acc_out);
+                    }
+                    ++sw;
+                }
+        hipLaunchKernelGGL(prefix_series, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out,
+                           m_out, n_meas, e_run, m_run, n_rep, n_meas_rep, sweeps,
+                           (uint64_t)per_meas);
+        CM_HIP_CHECK(hipGetLastError());
+        return CM_OK;
     }
     if (form == 3) {
-        CM_REQUIRE(wide_ok(g), "wide form needs a 2D lattice with L0 a multiple of 64");
         CM_REQUIRE(n_rep <= 65535, "n_rep out of range");
         const int64_t total = (int64_t)n_rep * n_meas;
         const int fb = (int)std::min<int64_t>((total + 255) / 256, 4096);

@@ -99,9 +99,11 @@ int cm_ising_convert(void *spins_i8_dev, void *spins_packed_dev, int32_t ndim,
  *   form    0 = auto, 1 = plain (one chain, one launch per half sweep),
  *           2 = batched (one launch, LDS-resident bit-packed replicas; 2D with
  *           L0 <= 64 or L0 % 64 == 0), 3 = wide (one launch per half sweep
- *           over all replicas, one lane per 32-site word; 2D with L0 % 64 == 0;
- *           auto picks it for few replicas of >= 65536 sites, where one
- *           workgroup per replica would leave CUs idle)
+ *           over all replicas, one lane per 32-site word: 2D with L0 % 64 == 0
+ *           in either layout, any other lattice with int8 spins through the
+ *           plain form's words; auto picks it for few replicas of >= 65536
+ *           sites, where one workgroup per replica would leave CUs idle, and
+ *           for every int8 lattice the batched form does not take)
  *   block, grid: launch shape of the batched form (0 = default)
  *   n_meas_rep_dev: null, or int64 [n_rep] caps (batched and wide forms): chain r performs
  *           min(n_meas, n_meas_rep[r]) measurements (continuous batching of

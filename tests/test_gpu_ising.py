@@ -88,6 +88,16 @@ CASES = [
     ((8,), K.FORM_PLAIN, 0),
     ((4, 6, 8), K.FORM_PLAIN, 0),
     ((34, 2, 2), K.FORM_PLAIN, 0),
+    # the wide form off 2D rows of 64-site segments: the plain form's words,
+    # every replica in each launch (auto takes it on these int8 lattices)
+    ((8,), K.FORM_WIDE, 0),
+    ((100, 6), K.FORM_WIDE, 0),
+    ((6, 10), K.FORM_WIDE, 0),
+    ((4, 6, 8), K.FORM_WIDE, 0),
+    ((34, 2, 2), K.FORM_WIDE, 0),
+    ((16, 16, 16), K.FORM_AUTO, 0),
+    ((64, 4, 6), K.FORM_WIDE, 0),     # cubic, 64-site row segments: whole segments per lane
+    ((128, 6, 4), K.FORM_AUTO, 0),    # two segments per row
     ((64, 64), K.FORM_WIDE, 0),
     ((128, 32), K.FORM_WIDE, 0),
     ((64, 64), K.FORM_WIDE, 1),
