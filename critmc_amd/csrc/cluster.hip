@@ -238,17 +238,38 @@ __global__ __launch_bounds__(256) void sw_union(ClArgs a)
 }
 
 // parent[i] = root(i); size[root] += 1.  Only lane i writes parent[i], and it
-// writes a root, so the flip pass reads every site's root.
+// writes a root, so the flip pass reads every site's root.  Near the critical
+// point one cluster holds a large share of the sites, and one atomic add per
+// site on its root serialises; so up to three times the wave takes the root of
+// its first remaining lane and adds, in one atomic, the number of lanes that
+// share it.  The sizes are integer sums, so they do not depend on the order.
 __global__ __launch_bounds__(256) void sw_flatten(ClArgs a)
 {
     const int rep = blockIdx.y;
     const uint32_t N = a.g.N;
     int32_t* P = a.parent + (int64_t)rep * N;
     int32_t* Z = a.size + (int64_t)rep * N;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const uint32_t r = cl_find_ro(P, i);
-        st_relaxed(P + i, r);
-        atomicAdd(Z + r, 1);
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < N; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;  // the wave stays together: i0 is uniform
+        const bool site = i < N;
+        uint32_t r = 0;
+        if (site) {
+            r = cl_find_ro(P, i);
+            st_relaxed(P + i, r);
+        }
+        bool left = site;
+#pragma unroll
+        for (int pass = 0; pass < 3; ++pass) {
+            const uint64_t rem = __ballot(left);
+            if (rem == 0) break;
+            const int first = __ffsll((long long)rem) - 1;
+            const uint32_t r0 = (uint32_t)__shfl((int)r, first, 64);
+            const bool mine = left && r == r0;
+            const uint64_t grp = __ballot(mine);
+            if (mine && (int)(threadIdx.x & 63) == first) atomicAdd(Z + r0, __popcll(grp));
+            left = left && !mine;
+        }
+        if (left) atomicAdd(Z + r, 1);
     }
 }
 
