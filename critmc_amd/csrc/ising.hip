@@ -557,8 +557,41 @@ __device__ __forceinline__ void load_segment(const int8_t* __restrict__ seg, uin
     }
 }
 
+constexpr int WIDE_LANES = 256;  // lanes per workgroup of the wide Ising kernels
+
+// The workgroup's energy, magnetisation and acceptance deltas added to
+// measurement slot o with one atomic each: the wave sums are combined in LDS
+// first, so a launch issues a quarter of the atomics one per wave would.
+__device__ __forceinline__ void wide_deltas(const Delta& d, int32_t* __restrict__ de_out,
+                                            int32_t* __restrict__ dm_out,
+                                            int32_t* __restrict__ acc_out, int64_t o)
+{
+    __shared__ int red[3][WIDE_LANES / 64];
+    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = de;
+        red[1][w] = dm;
+        red[2][w] = ac;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t0 = 0, t1 = 0, t2 = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            t0 += red[0][k];
+            t1 += red[1][k];
+            t2 += red[2][k];
+        }
+        if (t0 | t1 | t2) {
+            atomicAdd(&de_out[o], t0);
+            atomicAdd(&dm_out[o], t1);
+            atomicAdd(&acc_out[o], t2);
+        }
+    }
+}
+
 template <int LAYOUT, int NC>
-__global__ __launch_bounds__(64) void ising2d_half_wide(
+__global__ __launch_bounds__(WIDE_LANES) void ising2d_half_wide(
     void* __restrict__ spins, int L0, int L1, int code, int c, uint64_t sweep_add,
     const uint32_t* __restrict__ tabs, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
@@ -635,13 +668,7 @@ __global__ __launch_bounds__(64) void ising2d_half_wide(
             }
         }
     }
-    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
-    if (threadIdx.x == 0 && (de | dm | ac)) {
-        const int64_t o = (int64_t)rep * n_meas + meas;
-        atomicAdd(&de_out[o], de);
-        atomicAdd(&dm_out[o], dm);
-        atomicAdd(&acc_out[o], ac);
-    }
+    wide_deltas(d, de_out, dm_out, acc_out, (int64_t)rep * n_meas + meas);
 }
 
 // Wide form on cubic lattices with L0 % 64 == 0 (int8): as ising2d_half_wide,
@@ -651,7 +678,7 @@ __global__ __launch_bounds__(64) void ising2d_half_wide(
 // the plain form's word index, so the uniforms, and the chains, are the same;
 // the neighbour order differs, which the symmetric count does not see.
 template <int NC>
-__global__ __launch_bounds__(64) void ising3d_half_wide(
+__global__ __launch_bounds__(WIDE_LANES) void ising3d_half_wide(
     int8_t* __restrict__ spins, int L0, int L1, int L2, int code, int c, uint64_t sweep_add,
     const uint32_t* __restrict__ tabs, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ sids, const uint64_t* __restrict__ sweeps, int64_t n_meas,
@@ -712,13 +739,7 @@ __global__ __launch_bounds__(64) void ising3d_half_wide(
             }
         }
     }
-    const int de = wave_sum(d.de), dm = wave_sum(d.dm), ac = wave_sum(d.acc);
-    if (threadIdx.x == 0 && (de | dm | ac)) {
-        const int64_t o = (int64_t)rep * n_meas + meas;
-        atomicAdd(&de_out[o], de);
-        atomicAdd(&dm_out[o], dm);
-        atomicAdd(&acc_out[o], ac);
-    }
+    wide_deltas(d, de_out, dm_out, acc_out, (int64_t)rep * n_meas + meas);
 }
 
 // ---------------------------------------------------------------------------
@@ -948,7 +969,7 @@ static void launch_3d(const dim3& grid, hipStream_t st, int8_t* s, const Geo& g,
                       const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
                       const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out, int32_t* acc_out)
 {
-    hipLaunchKernelGGL((ising3d_half_wide<NC>), grid, dim3(64), 0, st, s, g.L0, g.L1, g.L2, code,
+    hipLaunchKernelGGL((ising3d_half_wide<NC>), grid, dim3(WIDE_LANES), 0, st, s, g.L0, g.L1, g.L2, code,
                        colour, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out,
                        m_out, acc_out);
 }
@@ -1046,7 +1067,7 @@ static void launch_wide(const dim3& grid, hipStream_t st, void* spins, const Geo
                         const uint32_t* sids, const uint64_t* sweeps, int64_t n_meas, int64_t meas,
                         const int64_t* n_meas_rep, int32_t* e_out, int32_t* m_out, int32_t* acc_out)
 {
-    hipLaunchKernelGGL((ising2d_half_wide<LAYOUT, NC>), grid, dim3(64), 0, st, spins, g.L0, g.L1,
+    hipLaunchKernelGGL((ising2d_half_wide<LAYOUT, NC>), grid, dim3(WIDE_LANES), 0, st, spins, g.L0, g.L1,
                        code, c, sw, tabs, keys, sids, sweeps, n_meas, meas, n_meas_rep, e_out,
                        m_out, acc_out);
 }
@@ -1252,7 +1273,7 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         const int64_t words = g.rows * g.G;
         // cubic lattices of 64-site row segments: whole segments per lane
         const bool seg3 = g.d == 3 && g.L0 % 64 == 0;
-        const int wpb = seg3 ? 64 : 256;  // lanes per workgroup
+        const int wpb = seg3 ? WIDE_LANES : 256;  // lanes per workgroup
         CM_REQUIRE((words + wpb - 1) / wpb * n_rep < (1ll << 32),
                    "too many words x replicas per launch");
         const dim3 agrid((unsigned)((words + wpb - 1) / wpb), (unsigned)n_rep);
@@ -1297,7 +1318,7 @@ int cm_ising_run(void* spins, int32_t layout, int32_t ndim, const int32_t* dims,
         hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, m_out, total, 0);
         hipLaunchKernelGGL(fill_i32, dim3(fb), dim3(256), 0, st, acc_out, total, 0);
         const int64_t words = (int64_t)g.L1 * (g.L0 / 64);
-        const dim3 wgrid((unsigned)((words + 63) / 64), (unsigned)n_rep);
+        const dim3 wgrid((unsigned)((words + WIDE_LANES - 1) / WIDE_LANES), (unsigned)n_rep);
         uint64_t sw = 0;
         int64_t per_meas = 0;
         for (int r = 0; r < sc.n; ++r) per_meas += sc.count[r];
