@@ -1180,7 +1180,7 @@ int cm_ising_cluster_run(void* spins, int32_t ndim, const int32_t* dims, int32_t
     }
     hipLaunchKernelGGL(cl_zero64, dim3((2 * n_rep + 255) / 256), dim3(256), 0, st, a.stats, 2 * n_rep);
     hipLaunchKernelGGL(cl_set64, dim3(1), dim3(64), 0, st, meas_d, (int64_t)0);
-    // eager: the host counters advance row by row (the plain form reads them)
+    // eager: the host counters advance row by row, mirroring the device ones
     auto measurement = [&](hipStream_t s, bool eager) -> int {
         hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, acc, n_rep);
         hipLaunchKernelGGL(cl_zero64, dim3(rb), dim3(256), 0, s, a.wsize, n_rep);
