@@ -444,7 +444,8 @@ def bench_matrix(args) -> dict:
             ("ising2d_metropolis_packed", (256, 256), 4096, [[0, 1]], 0.0, KI.LAYOUT_PACKED, 200),
             ("ising2d_metropolis_field_h0.1", (256, 256), 4096, [[0, 1]], 0.1, KI.LAYOUT_INT8, 200),
             ("ising2d_metropolis_wide_L1024", (1024, 1024), 16, [[0, 1]], 0.0, KI.LAYOUT_INT8, 200),
-            ("ising3d_metropolis_int8_L64", (64, 64, 64), 256, [[0, 1]], 0.0, KI.LAYOUT_INT8, 100)):
+            ("ising3d_metropolis_int8_L64", (64, 64, 64), 256, [[0, 1]], 0.0, KI.LAYOUT_INT8, 100),
+            ("ising2d_metropolis_L100", (100, 100), 1024, [[0, 1]], 0.0, KI.LAYOUT_INT8, 100)):
         N = int(np.prod(shape))
         ch = chains(R)
         sp = torch.empty((R, N), dtype=torch.int8, device=dev)
@@ -461,17 +462,19 @@ def bench_matrix(args) -> dict:
         torch.cuda.empty_cache()
 
     # Potts: q = 10 Metropolis (wide form), q = 3 heat bath (per-replica LDS form)
-    for name, L, R, q, sched, n in (("potts2d_q10_metropolis", 512, 128, 10, [[0, 1]], 500),
-                                    ("potts2d_q3_heatbath_L128", 128, 1024, 3, [[2, 1]], 500)):
-        shape = (L, L)
+    for name, shape, R, q, sched, n in (
+            ("potts2d_q10_metropolis", (512, 512), 128, 10, [[0, 1]], 500),
+            ("potts2d_q3_heatbath_L128", (128, 128), 1024, 3, [[2, 1]], 500),
+            ("potts3d_q3_heatbath_L64", (64, 64, 64), 64, 3, [[2, 1]], 100)):
+        L = int(np.prod(shape))
         ch = chains(R)
-        sp = torch.empty((R, L * L), dtype=torch.int8, device=dev)
+        sp = torch.empty((R, L), dtype=torch.int8, device=dev)
         KP.init_states(sp, shape, q, ch, "random")
         e, c = KP.measure(sp, shape, q, R)
         ws = KP.Workspace(R, dev)
-        betas = np.full(R, np.log(1.0 + np.sqrt(q)))
+        betas = np.full(R, np.log(1.0 + np.sqrt(q)) if len(shape) == 2 else 0.55)
         dt = timed(lambda: KP.run(sp, shape, q, betas, ch, np.array(sched), n, e, c, ws=ws))
-        add(name, {"shape": list(shape), "replicas": R, "q": q, "sweeps": n}, R * L * L * n, dt, 2.0)
+        add(name, {"shape": list(shape), "replicas": R, "q": q, "sweeps": n}, R * L * n, dt, 2.0)
         del sp
         torch.cuda.empty_cache()
 
