@@ -538,6 +538,33 @@ def bench_matrix(args) -> dict:
                                       np.array([[4, 1]]), n, e, m, ws=cws))
     add("ising2d_swendsen_wang", {"shape": [256, 256], "replicas": R, "updates": n},
         R * 256 * 256 * n, dt, 2.0)
+
+    # Wolff (single clusters, a level-synchronous search per replica): 64 x 256^2
+    # at beta_c; the rate counts the sites of the clusters flipped
+    aux_box = {}
+
+    def wolff():
+        aux_box["aux"] = KI.run_cluster(sp, shape, np.full(R, np.log(1 + np.sqrt(2)) / 2), 0.0, ch,
+                                        np.array([[3, 1]]), n, e, m, ws=cws)[2]
+
+    dt = timed(wolff)
+    flipped = int(aux_box["aux"][:, :, 1].sum().item())
+    add("ising2d_wolff", {"shape": [256, 256], "replicas": R, "updates": n,
+                          "mean_cluster": round(flipped / (R * n), 1)}, flipped, dt, 2.0)
+    del sp
+    torch.cuda.empty_cache()
+
+    # Potts q = 3 Swendsen-Wang: 64 x 256^2 at the transition point
+    q = 3
+    ch = chains(R)
+    sp = torch.empty((R, 256 * 256), dtype=torch.int8, device=dev)
+    KP.init_states(sp, shape, q, ch, "random")
+    e, c = KP.measure(sp, shape, q, R)
+    pws = KP.ClusterWorkspace(shape, R, q, dev)
+    dt = timed(lambda: KP.run_cluster(sp, shape, q, np.full(R, np.log(1 + np.sqrt(q))), ch,
+                                      np.array([[4, 1]]), n, e, c, ws=pws))
+    add("potts2d_q3_swendsen_wang", {"shape": [256, 256], "replicas": R, "q": q, "updates": n},
+        R * 256 * 256 * n, dt, 2.0)
     return {"metric": "site updates/ns per kernel family", "unit": "site updates/ns", "n_gpus": 1,
             "data": "synthetic: random starts, keys from seed 31", "entries": out}
 
