@@ -573,8 +573,8 @@ def secondary_runs(args) -> dict:
     """Short timed runs of the other full-size configurations, in the same process
     as the headline: EA tempering (512 samples x 2 x 32 T, 2000 PT steps), XY
     (256 x 512^2, 100 measurements of metropolis + 2 x overrelax), q = 10
-    Potts heat bath (128 x 512^2, 2000 sweeps) and the serving stream's first
-    10^3 jobs at 70 % of its capacity.  Each is the configuration of its own
+    Potts heat bath (128 x 512^2, 2000 sweeps), the serving stream's first
+    10^3 jobs at 70 % of its capacity, and the --workload matrix cases.  Each is the configuration of its own
     --workload line at fewer steps or jobs; the per-step cost is flat in the
     step count, so the rates compare with those lines."""
     import torch
@@ -595,6 +595,11 @@ def secondary_runs(args) -> dict:
     rec = bench_serve(sub)
     out["serve"] = {k: rec[k] for k in ("metric", "value", "unit", "latency_p50_s", "latency_p99_s",
                                         "wall_s", "config")}
+    torch.cuda.empty_cache()
+    # every other kernel family at one size each (--workload matrix)
+    mat = bench_matrix(argparse.Namespace(**vars(args)))
+    out["matrix"] = {"unit": mat["unit"],
+                     "entries": {e["kernel"]: e["value"] for e in mat["entries"]}}
     return out
 
 
