@@ -199,13 +199,21 @@ __device__ __forceinline__ void box_muller(uint32_t xa, uint32_t xb, float& g0, 
 __device__ __forceinline__ double unit_open0(uint32_t x) { return ((double)x + 1.0) * 0x1p-32; }
 __device__ __forceinline__ double unit_open1(uint32_t x) { return (double)x * 0x1p-32; }
 
-// Best & Fisher (1979) von Mises deviate on (-pi, pi] (critmc/kernels/on.py:186-204).
-__device__ double von_mises(double kappa, uint32_t blk, uint64_t sweep, uint32_t colour,
-                            uint32_t sid, uint32_t k0, uint32_t k1)
+// Best & Fisher (1979) von Mises deviate theta (critmc/kernels/on.py:186-204),
+// given as its cosine and sine: the rejection loop ends with cos(theta) = f,
+// and the deviate is +-acos(f), so sin(theta) = +-sqrt(1 - f^2) with the
+// loop's sign.  The caller rotates (ct, st) by the field's direction instead
+// of forming atan2(h) + theta and taking its cosine and sine; the decisions,
+// and the uniforms each attempt draws, are those of the FP64 rule.  Returns
+// false for kappa < 1e-8, where the deviate is uniform: `angle` then holds it.
+__device__ bool von_mises_cs(double kappa, uint32_t blk, uint64_t sweep, uint32_t colour,
+                             uint32_t sid, uint32_t k0, uint32_t k1, double& ct, double& st,
+                             double& angle)
 {
     if (kappa < 1e-8) {
         const u32x4 b = philox4x32_10(u32x4{blk, (uint32_t)sweep, ctr_z(sweep, colour, TAG_ONHB), sid}, k0, k1);
-        return M_PI * (2.0 * unit_open1(b.x) - 1.0);
+        angle = M_PI * (2.0 * unit_open1(b.x) - 1.0);
+        return false;
     }
     const double root = sqrt(1.0 + 4.0 * kappa * kappa);
     const double tau = 1.0 + root;
@@ -223,8 +231,10 @@ __device__ double von_mises(double kappa, uint32_t blk, uint64_t sweep, uint32_t
         sign = b.z >> 31;
         if (c * (2.0 - c) - u2 > 0.0 || log(c / u2) + 1.0 - c >= 0.0) break;
     }
-    const double theta = acos(fmin(1.0, fmax(-1.0, f)));
-    return sign == 0 ? theta : -theta;
+    ct = fmin(1.0, fmax(-1.0, f));
+    const double sn = sqrt(fmax(0.0, 1.0 - ct * ct));  // sin(acos(ct)) >= 0
+    st = sign == 0 ? sn : -sn;
+    return true;
 }
 
 struct OnArgs {
@@ -277,10 +287,19 @@ __device__ __forceinline__ bool on_site(Vec<NC>& s, const float* __restrict__ So
         const double hn = sqrt(hn2);
         const double kappa = (double)beta * hn;
         if constexpr (NC == 2) {
-            const double phi0 = hn > 0.0 ? atan2(hd[1], hd[0]) : 0.0;
-            const double phi = phi0 + von_mises(kappa, k, sweep, (uint32_t)colour, sid, k0, k1);
-            s.v[0] = (float)cos(phi);
-            s.v[1] = (float)sin(phi);
+            double ct, st, theta;
+            if (von_mises_cs(kappa, k, sweep, (uint32_t)colour, sid, k0, k1, ct, st, theta)) {
+                // new angle phi0 + theta with phi0 the field's direction h / |h|
+                // (kappa >= 1e-8 implies |h| > 0)
+                const double inv = 1.0 / hn;
+                const double cx = hd[0] * inv, sy = hd[1] * inv;
+                s.v[0] = (float)(cx * ct - sy * st);
+                s.v[1] = (float)(sy * ct + cx * st);
+            } else {
+                const double phi = (hn > 0.0 ? atan2(hd[1], hd[0]) : 0.0) + theta;
+                s.v[0] = (float)cos(phi);
+                s.v[1] = (float)sin(phi);
+            }
         } else {
             const u32x4 b = philox4x32_10(
                 u32x4{k, (uint32_t)sweep, ctr_z(sweep, (uint32_t)colour, TAG_ONHB), sid}, k0, k1);
