@@ -401,6 +401,94 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
     if (threadIdx.x == 0) a.wsize[rep] += total;
 }
 
+// wolff_run with the replica's spins and a membership bitmask in LDS (lattices
+// of at most CL_WOLFF_LDS_SITES sites): the spin tests, claims and flips of
+// the search are LDS operations, and only the queue stays in HBM.  The marks
+// are cleared per cluster, so a claim is "bit j newly set".  The search, and
+// with it every cluster, is that of wolff_run.
+constexpr int64_t CL_WOLFF_LDS_SITES = 65536;
+
+__global__ __launch_bounds__(256) void wolff_run_lds(ClArgs a, int count)
+{
+    extern __shared__ uint32_t wl[];
+    __shared__ uint32_t s_head, s_tail;
+    __shared__ int s_a, s_b;
+    const int rep = blockIdx.x;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N;
+    int8_t* Sg = a.spins + (int64_t)rep * N;
+    int8_t* S = (int8_t*)wl;
+    const uint32_t nmw = (N + 31) / 32;
+    uint32_t* Mb = wl + (N + 3) / 4;
+    int32_t* Q = a.queue + (int64_t)rep * N;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const int z = 2 * cl_axes(g);
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) S[i] = Sg[i];
+    long long total = 0;
+    for (int f = 0; f < count; ++f) {
+        for (uint32_t w = threadIdx.x; w < nmw; w += blockDim.x) Mb[w] = 0u;
+        __syncthreads();
+        const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
+        if (threadIdx.x == 0) {
+            const u32x4 r = philox4x32_10(
+                u32x4{0u, (uint32_t)sweep, ctr_z(sweep, 2, TAG_CLUSTER), sid}, k0, k1);
+            const uint32_t seed = (uint32_t)(((uint64_t)r.x * N) >> 32);
+            const int av = S[seed];
+            int bv;
+            if (a.q == 0) {
+                bv = -av;
+            } else {
+                bv = (int)(((uint64_t)r.y * (uint32_t)(a.q - 1)) >> 32);
+                if (bv >= av) ++bv;
+            }
+            S[seed] = (int8_t)bv;
+            Mb[seed >> 5] |= 1u << (seed & 31u);
+            Q[0] = (int32_t)seed;
+            s_a = av;
+            s_b = bv;
+            s_head = 0;
+            s_tail = 1;
+        }
+        __syncthreads();
+        const int s0 = s_a, s1 = s_b;
+        const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+        while (true) {
+            const uint32_t lo = s_head, hi = s_tail;
+            // every lane has this level's bounds before any lane appends to the queue
+            __syncthreads();
+            if (lo == hi) break;
+            for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+                const uint32_t k = (uint32_t)Q[q];
+                uint32_t x[3];
+                if (!g.nbr) cl_coords(g, k, x);
+                for (int c = 0; c < z; ++c) {
+                    const int ax = c >> 1;
+                    const uint32_t j = g.nbr ? (uint32_t)g.nbr[(int64_t)k * g.zt + c]
+                                             : cl_nbr(g, k, x, ax, (c & 1) == 0);
+                    if (S[j] != s0) continue;
+                    const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
+                    const u32x4 r = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+                    const uint32_t w = b & 3;
+                    const uint32_t u = w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
+                    if ((uint64_t)u >= T) continue;
+                    const uint32_t bit = 1u << (j & 31u);
+                    if (atomicOr(&Mb[j >> 5], bit) & bit) continue;  // another lane added it
+                    S[j] = (int8_t)s1;
+                    Q[atomicAdd(&s_tail, 1u)] = (int32_t)j;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_head = hi;
+            __syncthreads();
+        }
+        total += s_tail;
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) Sg[i] = S[i];
+    if (threadIdx.x == 0) a.wsize[rep] += total;
+}
+
 __global__ void cl_advance(uint64_t* sweeps, int R, uint64_t n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -850,10 +938,20 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
         }
     } else {
         a.sweep_add = 0;
-        const int64_t nm = N * a.R;
-        hipLaunchKernelGGL(cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
-                           dim3(256), 0, st, a.mark, nm, -1);
-        hipLaunchKernelGGL(wolff_run, dim3(a.R), dim3(256), 0, st, a, (int)count);
+        if (N <= CL_WOLFF_LDS_SITES) {
+            const size_t lds = (size_t)(N + 3) / 4 * 4 + (size_t)(N + 31) / 32 * 4;
+            // a launch beyond the attribute fails, and the callers' launch
+            // check reports it
+            if (lds > 64 * 1024)
+                (void)hipFuncSetAttribute((const void*)wolff_run_lds,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(wolff_run_lds, dim3(a.R), dim3(256), lds, st, a, (int)count);
+        } else {
+            const int64_t nm = N * a.R;
+            hipLaunchKernelGGL(cl_fill, dim3((unsigned)std::min<int64_t>((nm + 255) / 256, 8192)),
+                               dim3(256), 0, st, a.mark, nm, -1);
+            hipLaunchKernelGGL(wolff_run, dim3(a.R), dim3(256), 0, st, a, (int)count);
+        }
     }
     hipLaunchKernelGGL(cl_advance, dim3((a.R + 255) / 256), dim3(256), 0, st, sweeps, a.R,
                        (uint64_t)count);
