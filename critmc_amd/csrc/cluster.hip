@@ -372,12 +372,16 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
             // every lane has this level's bounds before any lane appends to the queue
             __syncthreads();
             if (lo == hi) break;
-            for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
-                const uint32_t k = (uint32_t)Q[q];
-                uint32_t x[3];
-                if (!g.nbr) cl_coords(g, k, x);
-                for (int c = 0; c < z; ++c) {
+            // one (site, neighbour) pair per lane: a level's Philox draws
+            // run side by side rather than z after each other in one lane
+            const uint32_t npair = (hi - lo) * (uint32_t)z;
+            for (uint32_t t = threadIdx.x; t < npair; t += blockDim.x) {
+                const uint32_t k = (uint32_t)Q[lo + t / (uint32_t)z];
+                const int c = (int)(t % (uint32_t)z);
+                {
                     const int ax = c >> 1;
+                    uint32_t x[3];
+                    if (!g.nbr) cl_coords(g, k, x);
                     const uint32_t j = g.nbr ? (uint32_t)g.nbr[(int64_t)k * g.zt + c]
                                              : cl_nbr(g, k, x, ax, (c & 1) == 0);
                     if (S[j] != s0) continue;
@@ -458,12 +462,16 @@ __global__ __launch_bounds__(256) void wolff_run_lds(ClArgs a, int count)
             // every lane has this level's bounds before any lane appends to the queue
             __syncthreads();
             if (lo == hi) break;
-            for (uint32_t q = lo + threadIdx.x; q < hi; q += blockDim.x) {
-                const uint32_t k = (uint32_t)Q[q];
-                uint32_t x[3];
-                if (!g.nbr) cl_coords(g, k, x);
-                for (int c = 0; c < z; ++c) {
+            // one (site, neighbour) pair per lane: a level's Philox draws
+            // run side by side rather than z after each other in one lane
+            const uint32_t npair = (hi - lo) * (uint32_t)z;
+            for (uint32_t t = threadIdx.x; t < npair; t += blockDim.x) {
+                const uint32_t k = (uint32_t)Q[lo + t / (uint32_t)z];
+                const int c = (int)(t % (uint32_t)z);
+                {
                     const int ax = c >> 1;
+                    uint32_t x[3];
+                    if (!g.nbr) cl_coords(g, k, x);
                     const uint32_t j = g.nbr ? (uint32_t)g.nbr[(int64_t)k * g.zt + c]
                                              : cl_nbr(g, k, x, ax, (c & 1) == 0);
                     if (S[j] != s0) continue;
@@ -486,6 +494,102 @@ __global__ __launch_bounds__(256) void wolff_run_lds(ClArgs a, int count)
         __syncthreads();
     }
     for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) Sg[i] = S[i];
+    if (threadIdx.x == 0) a.wsize[rep] += total;
+}
+
+// wolff_run for Ising lattices of at most CL_WOLFF_LDS_SITES sites with
+// everything in LDS: the spins as bits (bit i of word i / 32 set for +1) and
+// the queue as 16-bit site indices, 8 KB + 128 KB at 256^2.  A claim moves a
+// site's bit away from the cluster's spin with one atomic AND or OR, whose old
+// word says whether this lane made the change, so no membership mask is kept.
+// Each level appends through its own counter of three that rotate, so one
+// barrier per level separates the level's appends from the next level's
+// reads: the counter for level L + 1 is cleared during level L, and it was
+// last read at the start of level L - 1.  The search, and with it every
+// cluster, is that of wolff_run.
+__global__ __launch_bounds__(256) void wolff_ising_bits(ClArgs a, int count)
+{
+    extern __shared__ uint32_t wl[];
+    __shared__ uint32_t s_cnt[3];
+    __shared__ uint32_t s_b0;
+    const int rep = blockIdx.x;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N;
+    const uint32_t nw = (N + 31) / 32;
+    int8_t* Sg = a.spins + (int64_t)rep * N;
+    uint32_t* Sb = wl;
+    uint16_t* Q = (uint16_t*)(wl + nw);
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const int z = 2 * cl_axes(g);
+    const uint32_t lane = threadIdx.x & 63u;
+    // 64 sites per wave-step: the ballot of "spin is +1" is two bit words
+    for (uint32_t base = (threadIdx.x >> 6) * 64u; base < N; base += blockDim.x) {
+        const uint32_t i = base + lane;
+        const uint64_t m = __ballot(i < N && Sg[i] > 0);
+        if (lane == 0) {
+            Sb[base >> 5] = (uint32_t)m;
+            if ((base >> 5) + 1 < nw) Sb[(base >> 5) + 1] = (uint32_t)(m >> 32);
+        }
+    }
+    long long total = 0;
+    for (int f = 0; f < count; ++f) {
+        const uint64_t sweep = a.sweeps[rep] + a.sweep_add + (uint64_t)f;
+        __syncthreads();  // the previous cluster's last counter reads are done
+        if (threadIdx.x == 0) {
+            const u32x4 r = philox4x32_10(
+                u32x4{0u, (uint32_t)sweep, ctr_z(sweep, 2, TAG_CLUSTER), sid}, k0, k1);
+            const uint32_t seed = (uint32_t)(((uint64_t)r.x * N) >> 32);
+            const uint32_t bit = 1u << (seed & 31u);
+            s_b0 = (Sb[seed >> 5] & bit) ? 1u : 0u;
+            Sb[seed >> 5] ^= bit;
+            Q[0] = (uint16_t)seed;
+            s_cnt[0] = 0;
+        }
+        __syncthreads();
+        const uint32_t b0 = s_b0;
+        const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+        uint32_t lo = 0, hi = 1;
+        for (uint32_t lev = 0; lo < hi; ++lev) {
+            const uint32_t cur = lev % 3u;
+            if (threadIdx.x == 0) s_cnt[(lev + 1u) % 3u] = 0;
+            // one (site, neighbour) pair per lane: a level's Philox draws
+            // run side by side rather than z after each other in one lane
+            const uint32_t npair = (hi - lo) * (uint32_t)z;
+            for (uint32_t t = threadIdx.x; t < npair; t += blockDim.x) {
+                const uint32_t k = Q[lo + t / (uint32_t)z];
+                const int c = (int)(t % (uint32_t)z);
+                {
+                    const int ax = c >> 1;
+                    uint32_t x[3];
+                    if (!g.nbr) cl_coords(g, k, x);
+                    const uint32_t j = g.nbr ? (uint32_t)g.nbr[(int64_t)k * g.zt + c]
+                                             : cl_nbr(g, k, x, ax, (c & 1) == 0);
+                    const uint32_t sh = j & 31u;
+                    const uint32_t word = __hip_atomic_load(Sb + (j >> 5), __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (((word >> sh) & 1u) != b0) continue;
+                    const uint32_t b = (uint32_t)ax * N + ((c & 1) ? j : k);
+                    const u32x4 r = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+                    const uint32_t w = b & 3;
+                    const uint32_t u = w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
+                    if ((uint64_t)u >= T) continue;
+                    const uint32_t bit = 1u << sh;
+                    const uint32_t old = b0 ? atomicAnd(Sb + (j >> 5), ~bit)
+                                            : atomicOr(Sb + (j >> 5), bit);
+                    if (((old >> sh) & 1u) != b0) continue;  // another lane added it
+                    Q[hi + atomicAdd(&s_cnt[cur], 1u)] = (uint16_t)j;
+                }
+            }
+            __syncthreads();
+            lo = hi;
+            hi += s_cnt[cur];
+        }
+        total += hi;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < N; i += blockDim.x)
+        Sg[i] = ((Sb[i >> 5] >> (i & 31u)) & 1u) ? (int8_t)1 : (int8_t)-1;
     if (threadIdx.x == 0) a.wsize[rep] += total;
 }
 
@@ -721,9 +825,12 @@ __global__ __launch_bounds__(CL_LDS_THREADS) void cl_lds_run(ClArgs a, SeqRows r
                         // every lane has this level's bounds before any lane appends
                         __syncthreads();
                         if (lo == hi) break;
-                        for (uint32_t qi = lo + tid; qi < hi; qi += bs) {
-                            const uint32_t k = (uint32_t)A[qi];
-                            for (int c = 0; c < z; ++c) {
+                        // one (site, neighbour) pair per lane
+                        const uint32_t npair = (hi - lo) * (uint32_t)z;
+                        for (uint32_t t = (uint32_t)tid; t < npair; t += (uint32_t)bs) {
+                            const uint32_t k = (uint32_t)A[lo + t / (uint32_t)z];
+                            const int c = (int)(t % (uint32_t)z);
+                            {
                                 const uint32_t j = cl_nbr_col(g, k, c);
                                 if (S[j] != sa0) continue;
                                 const uint32_t b = (uint32_t)(c >> 1) * N + ((c & 1) ? j : k);
@@ -938,7 +1045,13 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
         }
     } else {
         a.sweep_add = 0;
-        if (N <= CL_WOLFF_LDS_SITES) {
+        if (a.q == 0 && N <= CL_WOLFF_LDS_SITES) {
+            const size_t lds = (size_t)(N + 31) / 32 * 4 + (size_t)(N + 1) / 2 * 4;
+            if (lds > 64 * 1024)
+                (void)hipFuncSetAttribute((const void*)wolff_ising_bits,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL(wolff_ising_bits, dim3(a.R), dim3(256), lds, st, a, (int)count);
+        } else if (N <= CL_WOLFF_LDS_SITES) {
             const size_t lds = (size_t)(N + 3) / 4 * 4 + (size_t)(N + 31) / 32 * 4;
             // a launch beyond the attribute fails, and the callers' launch
             // check reports it
