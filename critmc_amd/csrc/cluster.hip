@@ -688,6 +688,138 @@ __device__ __forceinline__ void lds_unite(int32_t* P, uint32_t i, uint32_t j)
     }
 }
 
+// Swendsen-Wang bonds of square lattices whose sides are multiples of
+// SW_TILE, in two steps that replace sw_union.  sw_tile joins the bonds inside
+// one SW_TILE x SW_TILE tile with a union-find over the tile in LDS and writes
+// each site's parent as the global index of its tile component's smallest
+// site; sw_edges then unites, in HBM, only the bonds that cross a tile edge
+// (x or y = SW_TILE - 1 mod SW_TILE, the periodic wrap included).  A bond is
+// open on the same Philox word, spins and threshold as in sw_union, local
+// order is global order inside a tile, and both steps hook a larger root below
+// a smaller one, so the components and their smallest-site roots are those of
+// sw_union.
+constexpr uint32_t SW_TILE = 64;
+
+__host__ __device__ inline bool sw_tiled(const ClGeo& g)
+{
+    return !g.nbr && g.d == 2 && g.L[0] % SW_TILE == 0 && g.L[1] % SW_TILE == 0;
+}
+
+__global__ __launch_bounds__(256) void sw_tile(ClArgs a)
+{
+    __shared__ int8_t s_sp[SW_TILE * SW_TILE];
+    __shared__ int32_t s_par[SW_TILE * SW_TILE];
+    __shared__ int32_t s_sz[SW_TILE * SW_TILE];
+    const int rep = blockIdx.y;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N, L0 = g.L[0];
+    const uint32_t ntx = L0 / SW_TILE;
+    const uint32_t x0 = (blockIdx.x % ntx) * SW_TILE, y0 = (blockIdx.x / ntx) * SW_TILE;
+    const int8_t* S = a.spins + (int64_t)rep * N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+    constexpr uint32_t NT = SW_TILE * SW_TILE;
+    for (uint32_t l = threadIdx.x; l < NT; l += blockDim.x) {
+        s_sp[l] = S[(y0 + l / SW_TILE) * L0 + x0 + l % SW_TILE];
+        s_par[l] = (int32_t)l;
+        s_sz[l] = 0;
+    }
+    __syncthreads();
+    // groups of four bonds of one axis at sites lx0 .. lx0 + 3 of a tile row:
+    // a + b = N * ax + i with i = 0 mod 4 (N and L0 are multiples of 4), so
+    // the four are the words of one Philox block, as in sw_union
+    constexpr uint32_t GPR = SW_TILE / 4;
+    for (uint32_t gi = threadIdx.x; gi < 2 * SW_TILE * GPR; gi += blockDim.x) {
+        const uint32_t ax = gi / (SW_TILE * GPR);
+        const uint32_t rem = gi % (SW_TILE * GPR);
+        const uint32_t ly = rem / GPR, lx0 = (rem % GPR) * 4;
+        if (ax == 1 && ly == SW_TILE - 1) continue;  // crosses the tile edge
+        const uint32_t b0 = ax * N + (y0 + ly) * L0 + x0 + lx0;
+        const u32x4 r = philox4x32_10(u32x4{b0 >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+        const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t lx = lx0 + (uint32_t)t;
+            if (ax == 0 && lx == SW_TILE - 1) continue;  // crosses the tile edge
+            if ((uint64_t)w[t] >= T) continue;
+            const uint32_t li = ly * SW_TILE + lx;
+            const uint32_t lj = ax == 0 ? li + 1 : li + SW_TILE;
+            if (s_sp[li] != s_sp[lj]) continue;
+            lds_unite(s_par, li, lj);
+        }
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < NT; l += blockDim.x) {
+        const uint32_t r = lds_find(s_par, l);
+        P[(y0 + l / SW_TILE) * L0 + x0 + l % SW_TILE] =
+            (int32_t)((y0 + r / SW_TILE) * L0 + x0 + r % SW_TILE);
+        atomicAdd(&s_sz[r], 1);
+    }
+    __syncthreads();
+    // each tile component's size at its smallest site (size[] is zero elsewhere)
+    for (uint32_t l = threadIdx.x; l < NT; l += blockDim.x)
+        if (s_sz[l]) a.size[(int64_t)rep * N + (y0 + l / SW_TILE) * L0 + x0 + l % SW_TILE] = s_sz[l];
+}
+
+// The bonds that cross a tile edge: L1 * L0 / SW_TILE of axis 0, then
+// L0 * L1 / SW_TILE of axis 1.
+__global__ __launch_bounds__(256) void sw_edges(ClArgs a)
+{
+    const int rep = blockIdx.y;
+    const ClGeo& g = a.g;
+    const uint32_t N = g.N, L0 = g.L[0], L1 = g.L[1];
+    const uint32_t ntx = L0 / SW_TILE, nty = L1 / SW_TILE;
+    const uint32_t E0 = L1 * ntx, E = E0 + L0 * nty;
+    const int8_t* S = a.spins + (int64_t)rep * N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    const uint64_t sweep = a.sweeps[rep] + a.sweep_add;
+    const uint32_t k0 = a.keys[2 * rep], k1 = a.keys[2 * rep + 1], sid = a.sids[rep];
+    const uint64_t T = a.t_add[rep];
+    const uint32_t zc = ctr_z(sweep, 0, TAG_CLUSTER);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+        uint32_t ax, i;
+        if (e < E0) {
+            ax = 0;
+            i = (e / ntx) * L0 + (e % ntx) * SW_TILE + SW_TILE - 1;
+        } else {
+            const uint32_t e1 = e - E0;
+            ax = 1;
+            i = ((e1 / L0) * SW_TILE + SW_TILE - 1) * L0 + e1 % L0;
+        }
+        const uint32_t b = ax * N + i;
+        const u32x4 r = philox4x32_10(u32x4{b >> 2, (uint32_t)sweep, zc, sid}, k0, k1);
+        const uint32_t w = b & 3;
+        const uint32_t u = w == 0 ? r.x : (w == 1 ? r.y : (w == 2 ? r.z : r.w));
+        if ((uint64_t)u >= T) continue;
+        const uint32_t j = cl_nbr_col(g, i, 2 * (int)ax);
+        if (S[i] != S[j]) continue;
+        cl_unite(P, i, j);
+    }
+}
+
+// sw_flatten after sw_tile / sw_edges: size[] holds each tile component's
+// size at its smallest site, so a site that is such a component's root but
+// not its cluster's adds that size to the cluster's root, and every other
+// site only stores its root.  The root's own entry is read by no other lane.
+__global__ __launch_bounds__(256) void sw_flatten_tiled(ClArgs a)
+{
+    const int rep = blockIdx.y;
+    const uint32_t N = a.g.N;
+    int32_t* P = a.parent + (int64_t)rep * N;
+    int32_t* Z = a.size + (int64_t)rep * N;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const uint32_t r = cl_find_ro(P, i);
+        st_relaxed(P + i, r);
+        if (r != i) {
+            const int32_t z = Z[i];
+            if (z) atomicAdd(Z + r, z);
+        }
+    }
+}
+
 // Sum of v over the workgroup, returned to every lane (uses red[0..3]).
 __device__ __forceinline__ long long cl_block_sum(long long v, long long* red)
 {
@@ -1036,11 +1168,24 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
     const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
     const unsigned bond_chunks = (unsigned)std::min<int64_t>((cl_axes(a.g) * N / 4 + 256) / 256, 1024);
     if (code == CODE_SWENDSEN_WANG) {
+        const bool tiled = sw_tiled(a.g);
+        const unsigned n_tiles = tiled ? (a.g.L[0] / SW_TILE) * (a.g.L[1] / SW_TILE) : 0;
+        const unsigned edge_chunks =
+            tiled ? (unsigned)std::min<int64_t>(
+                        ((int64_t)a.g.L[1] * (a.g.L[0] / SW_TILE) +
+                         (int64_t)a.g.L[0] * (a.g.L[1] / SW_TILE) + 255) / 256, 1024)
+                  : 0;
         for (int64_t k = 0; k < count; ++k) {
             a.sweep_add = (uint64_t)k;
             hipLaunchKernelGGL(sw_init, dim3(site_chunks, a.R), dim3(256), 0, st, a);
-            hipLaunchKernelGGL(sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
-            hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            if (tiled) {
+                hipLaunchKernelGGL(sw_tile, dim3(n_tiles, a.R), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(sw_edges, dim3(edge_chunks, a.R), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(sw_flatten_tiled, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            } else {
+                hipLaunchKernelGGL(sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            }
             hipLaunchKernelGGL(sw_flip, dim3(site_chunks, a.R), dim3(256), 0, st, a);
         }
     } else {
