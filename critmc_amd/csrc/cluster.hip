@@ -1169,6 +1169,7 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
     const unsigned bond_chunks = (unsigned)std::min<int64_t>((cl_axes(a.g) * N / 4 + 256) / 256, 1024);
     if (code == CODE_SWENDSEN_WANG) {
         const bool tiled = sw_tiled(a.g);
+        const unsigned flip_chunks = (unsigned)std::min<int64_t>((N + 4095) / 4096, 1024);
         const unsigned n_tiles = tiled ? (a.g.L[0] / SW_TILE) * (a.g.L[1] / SW_TILE) : 0;
         const unsigned edge_chunks =
             tiled ? (unsigned)std::min<int64_t>(
@@ -1186,7 +1187,9 @@ inline void cl_cluster_row(ClArgs a, int64_t code, int64_t count, uint64_t* swee
                 hipLaunchKernelGGL(sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
                 hipLaunchKernelGGL(sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
             }
-            hipLaunchKernelGGL(sw_flip, dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            // 16 sites per lane: each workgroup ends with one 64-bit atomic
+            // pair on its replica's statistics, and these serialise
+            hipLaunchKernelGGL(sw_flip, dim3(flip_chunks, a.R), dim3(256), 0, st, a);
         }
     } else {
         a.sweep_add = 0;

@@ -766,27 +766,48 @@ __global__ void ising_init_kernel(int8_t* __restrict__ s, int64_t N, int R,
     }
 }
 
+// E_b and M of each replica.  Workgroup b takes `chunk` sites of replica
+// b / nchunk and adds its integer sums atomically onto sums zeroed by
+// ising_measure_zero, so they do not depend on the order.  IX is uint32_t for
+// replicas of fewer than 2^31 sites, whose coordinates then cost 32-bit
+// divisions.
+__global__ void ising_measure_zero(int32_t* e_run, int32_t* m_run, int n_rep)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_rep) {
+        e_run[t] = 0;
+        m_run[t] = 0;
+    }
+}
+
+template <typename IX>
 __global__ __launch_bounds__(256) void ising_measure_kernel(const int8_t* __restrict__ s, Geo g,
-                                                            int64_t N, int32_t* __restrict__ e_run,
+                                                            int64_t N, int64_t chunk,
+                                                            int64_t nchunk,
+                                                            int32_t* __restrict__ e_run,
                                                             int32_t* __restrict__ m_run)
 {
-    __shared__ int red[2 * 16];
-    const int rep = blockIdx.x;
-    const int8_t* sr = s + (int64_t)rep * N;
+    __shared__ int red[2 * 4];
+    const int64_t rep = blockIdx.x / nchunk, part = blockIdx.x % nchunk;
+    const int8_t* sr = s + rep * N;
+    const IX L0 = (IX)g.L0, L1 = (IX)g.L1, L2 = (IX)g.L2;
+    const IX st1 = L0, st2 = L0 * L1;
+    const IX i1 = (IX)std::min<int64_t>(N, (part + 1) * chunk);
     int e = 0, m = 0;
-    int64_t st1 = g.L0, st2 = (int64_t)g.L0 * g.L1;
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+    for (IX i = (IX)(part * chunk) + threadIdx.x; i < i1; i += blockDim.x) {
         const int si = sr[i];
         m += si;
-        const int x0 = (int)(i % g.L0);
-        int nbsum = sr[i - x0 + (x0 + 1 == g.L0 ? 0 : x0 + 1)];
+        const IX q0 = i / L0;
+        const IX x0 = i - q0 * L0;
+        int nbsum = sr[x0 + 1 == L0 ? i - x0 : i + 1];
         if (g.d >= 2) {
-            const int x1 = (int)((i / st1) % g.L1);
-            nbsum += sr[i + (x1 + 1 == g.L1 ? -(int64_t)(g.L1 - 1) * st1 : st1)];
-        }
-        if (g.d >= 3) {
-            const int x2 = (int)((i / st2) % g.L2);
-            nbsum += sr[i + (x2 + 1 == g.L2 ? -(int64_t)(g.L2 - 1) * st2 : st2)];
+            const IX q1 = q0 / L1;
+            const IX x1 = q0 - q1 * L1;
+            nbsum += sr[x1 + 1 == L1 ? i - x1 * st1 : i + st1];
+            if (g.d >= 3) {
+                const IX x2 = q1 % L2;
+                nbsum += sr[x2 + 1 == L2 ? i - x2 * st2 : i + st2];
+            }
         }
         e -= si * nbsum;
     }
@@ -803,8 +824,8 @@ __global__ __launch_bounds__(256) void ising_measure_kernel(const int8_t* __rest
             se += red[2 * k];
             sm += red[2 * k + 1];
         }
-        e_run[rep] = se;
-        m_run[rep] = sm;
+        if (se) atomicAdd(e_run + rep, se);
+        if (sm) atomicAdd(m_run + rep, sm);
     }
 }
 
@@ -1180,8 +1201,21 @@ int cm_ising_measure(const void* spins, int32_t ndim, const int32_t* dims, int32
     Geo g;
     if (int rc = geo_from(ndim, dims, &g)) return rc;
     CM_REQUIRE(n_rep >= 1, "n_rep must be positive");
-    hipLaunchKernelGGL(ising_measure_kernel, dim3(n_rep), dim3(256), 0, (hipStream_t)stream,
-                       (const int8_t*)spins, g, g.rows * g.L0, e_run, m_run);
+    const int64_t N = g.rows * g.L0;
+    int64_t chunk = 4096;
+    while ((int64_t)n_rep * ((N + chunk - 1) / chunk) > (int64_t)1 << 22) chunk *= 2;
+    const int64_t nchunk = (N + chunk - 1) / chunk;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(ising_measure_zero, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_run,
+                       m_run, n_rep);
+    if (N < ((int64_t)1 << 31))
+        hipLaunchKernelGGL(ising_measure_kernel<uint32_t>, dim3((unsigned)(n_rep * nchunk)),
+                           dim3(256), 0, st, (const int8_t*)spins, g, N, chunk, nchunk, e_run,
+                           m_run);
+    else
+        hipLaunchKernelGGL(ising_measure_kernel<uint64_t>, dim3((unsigned)(n_rep * nchunk)),
+                           dim3(256), 0, st, (const int8_t*)spins, g, N, chunk, nchunk, e_run,
+                           m_run);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }

@@ -542,20 +542,35 @@ __global__ void potts_init_kernel(int8_t* __restrict__ s, int64_t N, int R, int 
 }
 
 // E = -#satisfied bonds and counts[q] per replica (critmc/kernels/potts.py:27-41).
+// E and the state counts of each replica.  Workgroup b takes `chunk` of the
+// replica b / nchunk's sites (colour 0, then colour 1) and adds its integer
+// sums atomically onto sums zeroed by potts_measure_zero, so they do not
+// depend on the order.
+__global__ void potts_measure_zero(int32_t* e_run, int32_t* counts_run, int n_rep, int q)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_rep) e_run[t] = 0;
+    if (t < (int64_t)n_rep * q) counts_run[t] = 0;
+}
+
 template <int Z>
 __global__ __launch_bounds__(256) void potts_measure_kernel(const int8_t* __restrict__ s, CbGeo g,
-                                                            int q, int32_t* __restrict__ e_run,
+                                                            int q, int64_t chunk, int64_t nchunk,
+                                                            int32_t* __restrict__ e_run,
                                                             int32_t* __restrict__ counts_run)
 {
     __shared__ int counts[128];
     __shared__ int red[16];
-    const int rep = blockIdx.x;
-    const int8_t* S = s + (int64_t)rep * g.N;
+    const int64_t rep = blockIdx.x / nchunk, part = blockIdx.x % nchunk;
+    const int8_t* S = s + rep * g.N;
     for (int k = threadIdx.x; k < q; k += blockDim.x) counts[k] = 0;
     __syncthreads();
     int e = 0;
-    for (int c = 0; c < 2; ++c)
-        for (int64_t k = threadIdx.x; k < g.half; k += blockDim.x) {
+    const int64_t t1 = std::min<int64_t>(2 * g.half, (part + 1) * chunk);
+    for (int64_t t = part * chunk + threadIdx.x; t < t1; t += blockDim.x) {
+        const int c = t >= g.half ? 1 : 0;
+        const int64_t k = c ? t - g.half : t;
+        {
             int64_t nbi[Z];
             const int64_t i = cb_site<Z>(g, k, c, nbi);
             const int v = S[i];
@@ -564,15 +579,17 @@ __global__ __launch_bounds__(256) void potts_measure_kernel(const int8_t* __rest
 #pragma unroll
             for (int j = 0; j < Z; j += 2) e -= S[nbi[j]] == v;
         }
+    }
     e = wave_sum(e);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-        e_run[rep] = t;
+        if (t) atomicAdd(e_run + rep, t);
     }
-    for (int k = threadIdx.x; k < q; k += blockDim.x) counts_run[(int64_t)rep * q + k] = counts[k];
+    for (int k = threadIdx.x; k < q; k += blockDim.x)
+        if (counts[k]) atomicAdd(counts_run + rep * q + k, counts[k]);
 }
 
 static uint64_t potts_threshold(double p)
@@ -799,12 +816,21 @@ int cm_potts_measure(const void* spins, int32_t ndim, const int32_t* dims, int32
     CM_REQUIRE(q >= 2 && q <= POTTS_QMAX, "q must be in [2, 127], got %d", q);
     hipStream_t st = (hipStream_t)stream;
     const int8_t* s = (const int8_t*)spins;
+    int64_t chunk = 4096;
+    while ((int64_t)n_rep * ((g.N + chunk - 1) / chunk) > (int64_t)1 << 22) chunk *= 2;
+    const int64_t nchunk = (g.N + chunk - 1) / chunk;
+    const dim3 grid((unsigned)(n_rep * nchunk));
+    hipLaunchKernelGGL(potts_measure_zero, dim3((unsigned)(((int64_t)n_rep * q + 255) / 256)),
+                       dim3(256), 0, st, e_run, counts_run, n_rep, q);
     if (g.d == 1)
-        hipLaunchKernelGGL(potts_measure_kernel<2>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+        hipLaunchKernelGGL(potts_measure_kernel<2>, grid, dim3(256), 0, st, s, g, q, chunk, nchunk,
+                           e_run, counts_run);
     else if (g.d == 2)
-        hipLaunchKernelGGL(potts_measure_kernel<4>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+        hipLaunchKernelGGL(potts_measure_kernel<4>, grid, dim3(256), 0, st, s, g, q, chunk, nchunk,
+                           e_run, counts_run);
     else
-        hipLaunchKernelGGL(potts_measure_kernel<6>, dim3(n_rep), dim3(256), 0, st, s, g, q, e_run, counts_run);
+        hipLaunchKernelGGL(potts_measure_kernel<6>, grid, dim3(256), 0, st, s, g, q, chunk, nchunk,
+                           e_run, counts_run);
     CM_HIP_CHECK(hipGetLastError());
     return CM_OK;
 }
