@@ -501,28 +501,38 @@ __global__ void potts_zero(int32_t* __restrict__ p, int64_t n)
         p[i] = 0;
 }
 
-// Deltas -> running E and counts per replica (one thread per replica).
+// Deltas -> running E and counts: one thread per (replica, column), column q
+// being E and column c < q the count of state c, so each running sum stays in
+// a register and the replica's columns run side by side.
 __global__ void potts_prefix(int32_t* __restrict__ e_out, int32_t* __restrict__ counts_out,
                              int64_t n_meas, int32_t* __restrict__ e_run,
                              int32_t* __restrict__ counts_run, int q, int R,
                              const int64_t* __restrict__ n_meas_rep, uint64_t* __restrict__ sweeps,
                              uint64_t per_meas)
 {
-    const int rep = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rep >= R) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)R * (q + 1)) return;
+    const int rep = (int)(t / (q + 1));
+    const int c = (int)(t % (q + 1));
     const int64_t n = n_meas_rep ? min(n_meas, n_meas_rep[rep]) : n_meas;
-    int32_t e = e_run[rep];
-    for (int64_t k = 0; k < n; ++k) {
-        const int64_t o = (int64_t)rep * n_meas + k;
-        e += e_out[o];
-        e_out[o] = e;
-        for (int c = 0; c < q; ++c) {
-            counts_run[(int64_t)rep * q + c] += counts_out[o * q + c];
-            counts_out[o * q + c] = counts_run[(int64_t)rep * q + c];
+    if (c == q) {
+        int32_t e = e_run[rep];
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t o = (int64_t)rep * n_meas + k;
+            e += e_out[o];
+            e_out[o] = e;
         }
+        e_run[rep] = e;
+        sweeps[rep] += per_meas * (uint64_t)n;
+    } else {
+        int32_t v = counts_run[(int64_t)rep * q + c];
+        for (int64_t k = 0; k < n; ++k) {
+            const int64_t o = ((int64_t)rep * n_meas + k) * q + c;
+            v += counts_out[o];
+            counts_out[o] = v;
+        }
+        counts_run[(int64_t)rep * q + c] = v;
     }
-    e_run[rep] = e;
-    sweeps[rep] += per_meas * (uint64_t)n;
 }
 
 __global__ void potts_init_kernel(int8_t* __restrict__ s, int64_t N, int R, int q,
@@ -763,7 +773,8 @@ int potts_rows(void* spins, int32_t ndim, const int32_t* dims, int32_t n_rep, in
                     }
                     ++sw;
                 }
-        hipLaunchKernelGGL(potts_prefix, dim3((n_rep + 255) / 256), dim3(256), 0, st, e_out,
+        hipLaunchKernelGGL(potts_prefix, dim3((unsigned)(((int64_t)n_rep * (q + 1) + 255) / 256)),
+                           dim3(256), 0, st, e_out,
                            counts_out, n_meas, e_run, counts_run, q, n_rep, n_meas_rep, sweeps,
                            (uint64_t)per_meas);
         CM_HIP_CHECK(hipGetLastError());
