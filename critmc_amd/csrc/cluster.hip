@@ -405,9 +405,10 @@ __global__ __launch_bounds__(256) void wolff_run(ClArgs a, int count)
     if (threadIdx.x == 0) a.wsize[rep] += total;
 }
 
-// wolff_run with the replica's spins and a membership bitmask in LDS (lattices
-// of at most CL_WOLFF_LDS_SITES sites): the spin tests, claims and flips of
-// the search are LDS operations, and only the queue stays in HBM.  The marks
+// wolff_run with the replica's spins and a membership bitmask in LDS (Potts
+// lattices of at most CL_WOLFF_LDS_SITES sites; Ising ones take
+// wolff_ising_bits): the spin tests, claims and flips of the search are LDS
+// operations, and only the queue stays in HBM.  The marks
 // are cleared per cluster, so a claim is "bit j newly set".  The search, and
 // with it every cluster, is that of wolff_run.
 constexpr int64_t CL_WOLFF_LDS_SITES = 65536;
