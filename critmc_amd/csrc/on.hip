@@ -1230,10 +1230,30 @@ __global__ __launch_bounds__(256) void on_sw_flatten(OnClArgs a)
     const uint32_t N = a.g.N;
     int32_t* P = a.parent + (int64_t)rep * N;
     int32_t* Z = a.size + (int64_t)rep * N;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const uint32_t r = cl_find_ro(P, i);
-        __hip_atomic_store(P + i, (int32_t)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        atomicAdd(Z + r, 1);
+    // near the critical point one cluster holds many of the sites: up to three
+    // times, the lanes of a wave that share its first remaining lane's root
+    // add their count in one atomic (integer sums, so the order is free)
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < N; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;  // i0 is uniform, so the wave stays together
+        const bool site = i < N;
+        uint32_t r = 0;
+        if (site) {
+            r = cl_find_ro(P, i);
+            __hip_atomic_store(P + i, (int32_t)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        bool left = site;
+#pragma unroll
+        for (int pass = 0; pass < 3; ++pass) {
+            const uint64_t rem = __ballot(left);
+            if (rem == 0) break;
+            const int first = __ffsll((long long)rem) - 1;
+            const uint32_t r0 = (uint32_t)__shfl((int)r, first, 64);
+            const bool mine = left && r == r0;
+            const uint64_t grp = __ballot(mine);
+            if (mine && (int)(threadIdx.x & 63) == first) atomicAdd(Z + r0, __popcll(grp));
+            left = left && !mine;
+        }
+        if (left) atomicAdd(Z + r, 1);
     }
 }
 
@@ -1436,6 +1456,7 @@ static void on_cluster_row(OnClArgs a, int64_t code, int64_t count, hipStream_t 
     const int64_t N = a.g.N;
     const unsigned site_chunks = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
     const unsigned bond_chunks = (unsigned)std::min<int64_t>((a.g.d * N / 4 + 256) / 256, 1024);
+    const unsigned flip_chunks = (unsigned)std::min<int64_t>((N + 4095) / 4096, 1024);
     if (code == CODE_SWENDSEN_WANG) {
         const uint64_t base = a.sweep_add;
         for (int64_t k = 0; k < count; ++k) {
@@ -1443,7 +1464,8 @@ static void on_cluster_row(OnClArgs a, int64_t code, int64_t count, hipStream_t 
             hipLaunchKernelGGL((on_sw_prep<NC>), dim3(site_chunks, a.R), dim3(256), 0, st, a);
             hipLaunchKernelGGL(on_sw_union, dim3(bond_chunks, a.R), dim3(256), 0, st, a);
             hipLaunchKernelGGL(on_sw_flatten, dim3(site_chunks, a.R), dim3(256), 0, st, a);
-            hipLaunchKernelGGL((on_sw_flip<NC>), dim3(site_chunks, a.R), dim3(256), 0, st, a);
+            // 16 sites per lane: fewer workgroup-end atomics on the statistics
+            hipLaunchKernelGGL((on_sw_flip<NC>), dim3(flip_chunks, a.R), dim3(256), 0, st, a);
         }
     } else {
         const int64_t nm = N * a.R;
